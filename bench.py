@@ -1,0 +1,271 @@
+#!/usr/bin/env python3
+"""AMG-PCG time-to-1e-8 on one MI355X: FP32-cycle vs FP64, V-cycle HBM GB/s.
+
+Workload (default): 3-D 27-point Laplacian on a 256^3 grid (16,777,216 rows,
+449,455,096 nonzeros), Ruge-Stuben setup (classical strength 0.25, PMIS,
+extended interpolation with filter 0.3), weighted Jacobi w = 2/3, b ~ N(0,1)
+(seed 2).  One step = one complete solve from x = 0 to a true FP64 relative
+residual ||b - Ax|| / ||b|| <= 1e-8 with flexible PCG in FP64 and the
+V-cycle in FP32 (the headline `value`, seconds, lower is better); the same
+solve with the V-cycle in FP64 is timed beside it.  `--config c2` runs the
+128^3 grid with b = A x_true, x_true ~ U(-1,1) (seed 1).
+
+Also reported: live HIP-event timing of the dominant kernel (roofline) and
+the oracle's single-core CPU time for one V-cycle + one SpMV on the same
+hierarchy, scaled to the solve's iteration count (cpu_baseline).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "AMG-PCG time-to-1e-8 residual (s), FP32-cycle vs FP64; V-cycle HBM GB/s"
+PEAK_HBM_GBS = 8000.0
+CONFIGS = {
+    "c2": dict(m=128, rhs="xtrue_uniform", seed=1),
+    "c3": dict(m=256, rhs="normal", seed=2),
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def vcycle_bytes(ml, sv):
+    """Algorithmic HBM bytes of one V(1,1) cycle from x = 0 with the value
+    width sv (8 FP64, 4 FP32), int32 indices, per level:
+      pre-smooth from zero  x = w b / d           3 sv n
+      residual  r = b - A x                        nnz (sv+4) + 4(n+1) + 3 sv n
+      restrict  b_c = R r                          nnzP (sv+4) + 4(nc+1) + sv nf + sv nc
+      interpolate x += P e                         nnzP (sv+4) + 4(nf+1) + sv nc + 2 sv nf
+      post-smooth Jacobi                           nnz (sv+4) + 4(n+1) + 3 sv n
+    plus the coarse FP64 LU solve 8 nc^2 + 16 nc."""
+    n, nnzA, nnzP = ml.level_sizes()
+    total = 0.0
+    L = len(n)
+    for l in range(L - 1):
+        nf, nc = int(n[l]), int(n[l + 1])
+        a = nnzA[l] * (sv + 4) + 4 * (nf + 1)
+        p = nnzP[l] * (sv + 4)
+        total += 3 * sv * nf
+        total += a + 3 * sv * nf
+        total += p + 4 * (nc + 1) + sv * nf + sv * nc
+        total += p + 4 * (nf + 1) + sv * nc + 2 * sv * nf
+        total += a + 3 * sv * nf
+    nc = int(n[-1])
+    total += 8 * nc * nc + 16 * nc
+    return total
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--m", type=int, default=0, help="override the grid edge")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-cycle comparison solves")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import raptor_amd as ra
+    from raptor_amd import _lib, synth
+
+    cfg = dict(CONFIGS[args.config])
+    m = args.m or cfg["m"]
+    ctx = ra.Context(0)
+    t = time.time()
+    rp, col, val = ra.gallery_laplace27(m)
+    n = rp.size - 1
+    log(f"27-point Laplacian {m}^3: n={n} nnz={col.size} generated in {time.time() - t:.1f}s")
+
+    t = time.time()
+    ml = ra.ParRugeStubenSolver(ctx, 0.25, ra.PMIS, ra.Extended, ra.Classical, ra.Jacobi)
+    ml.relax_weight = 2.0 / 3.0
+    ml.setup(rp, col, val)
+    setup_s = time.time() - t
+    tphase = ml.setup_times()
+    nl, nnzA, nnzP = ml.level_sizes()
+    log(f"setup {setup_s:.1f}s (strength {tphase[0]:.1f} split {tphase[1]:.1f} interp {tphase[2]:.1f} "
+        f"galerkin {tphase[3]:.1f} other {tphase[4]:.1f}); levels {list(map(int, nl))}")
+    ml.enable_fp32()
+    A = ra.ParCSRMatrix.from_csr(ctx, rp, col, val)
+
+    if cfg["rhs"] == "normal":
+        b_h = synth.normal(cfg["seed"], n)
+    else:
+        xt = synth.uniform_range(cfg["seed"], n, -1.0, 1.0)
+        xd = ra.ParVector.from_numpy(ctx, xt)
+        bd = ra.ParVector(ctx, n)
+        A.mult(xd, bd)
+        b_h = bd.numpy()
+        del xd, bd
+    b = ra.ParVector.from_numpy(ctx, b_h)
+    x = ra.ParVector(ctx, n)
+
+    def solve(prec):
+        x.set_const_value(0.0)
+        hist, it, rel = ra.FPCG(A, ml, x, b, 1e-8, 500, prec)
+        return it, rel
+
+    for _ in range(args.warmup):
+        solve(ra.PREC_FP32)
+        if not args.no_fp64:
+            solve(ra.PREC_FP64)
+    log("warmup done")
+
+    L = _lib.lib()
+    fine_nnz = int(nnzA[0])
+    check = _lib.check
+    # ---- timed: FP32-cycle solves (headline), dominant kernels timed live
+    check(L.ramd_prof_enable(ctx.h, fine_nnz))
+    ctx.sync()
+    t0 = time.perf_counter()
+    its32 = []
+    for _ in range(args.steps):
+        it, rel32 = solve(ra.PREC_FP32)
+        its32.append(it)
+    ctx.sync()
+    t32 = (time.perf_counter() - t0) / args.steps
+    prof = {}
+    import ctypes as C
+    names = {0: "spmv", 1: "append", 2: "residual", 3: "jacobi"}
+    for op in range(4):
+        for prec in (ra.PREC_FP64, ra.PREC_FP32):
+            nl_, ms, by = C.c_int64(), C.c_double(), C.c_double()
+            check(L.ramd_prof_read(ctx.h, op, prec, C.byref(nl_), C.byref(ms), C.byref(by)))
+            if nl_.value:
+                prof[f"{names[op]}_{'fp32' if prec else 'fp64'}"] = dict(
+                    launches=nl_.value, ms=ms.value, bytes=by.value)
+    check(L.ramd_prof_enable(ctx.h, -1))
+    log(f"FP32-cycle solve: {t32:.4f}s, iterations {its32}, relres {rel32:.3e}")
+
+    # ---- FP64-cycle solves (same protocol)
+    t64 = None
+    its64 = []
+    rel64 = None
+    if not args.no_fp64:
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            it, rel64 = solve(ra.PREC_FP64)
+            its64.append(it)
+        ctx.sync()
+        t64 = (time.perf_counter() - t0) / args.steps
+        log(f"FP64-cycle solve: {t64:.4f}s, iterations {its64}, relres {rel64:.3e}")
+
+    # ---- V-cycle throughput (FP32 and FP64), 10 cycles each
+    vc = {}
+    zb = ra.ParVector.from_numpy(ctx, b_h)
+    x32 = ra.ParVector(ctx, n, np.float32)
+    b32 = ra.ParVector.from_numpy(ctx, b_h.astype(np.float32))
+    for prec_name, sv in (("fp32", 4), ("fp64", 8)):
+        cyc = 10
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(cyc):
+            if prec_name == "fp64":
+                ml.cycle(x, zb, x_is_zero=True)
+            else:
+                check(L.ramd_vcycle_fp32(ml.h, x32.ptr, b32.ptr))
+        ctx.sync()
+        dt = (time.perf_counter() - t0) / cyc
+        by = vcycle_bytes(ml, sv)
+        vc[prec_name] = dict(bytes=by, s=round(dt, 6), gbs=round(by / dt / 1e9, 1))
+        log(f"V-cycle {prec_name}: {dt * 1e3:.3f} ms, {by / dt / 1e9:.0f} GB/s algorithmic")
+
+    # ---- roofline: dominant kernel (most total event time)
+    dom_name, dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    per_launch_bytes = dom["bytes"] / dom["launches"]
+    per_launch_s = dom["ms"] / 1e3 / dom["launches"]
+    achieved = per_launch_bytes / per_launch_s / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                    frac=round(achieved / PEAK_HBM_GBS, 4), traffic=None, kernel=f"stream_kernel:{dom_name}",
+                    bytes_per_launch=per_launch_bytes, us_per_launch=round(per_launch_s * 1e6, 2))
+    log(f"dominant kernel {dom_name}: {achieved:.0f} GB/s ({achieved / PEAK_HBM_GBS:.1%} of 8 TB/s)")
+
+    # ---- CPU baseline: the oracle (plain C, one core) on the same hierarchy
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(ml, rp, col, val, b_h, max(its64 or its32))
+        except Exception as e:  # the checker is optional on machines without it
+            log(f"cpu baseline skipped: {e}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(t32, 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t32 * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-cycle/f64-outer",
+        "data": "synthetic",
+        "config": dict(workload=f"27pt-laplacian-{m}^3 RS(PMIS, extended, theta=0.25) Jacobi(2/3) FPCG to 1e-8",
+                       n=n, nnz=int(col.size), rhs=cfg["rhs"], seed=cfg["seed"], levels=list(map(int, nl))),
+        "fp64_cycle_s": round(t64, 5) if t64 else None,
+        "fp32_speedup_vs_fp64": round(t64 / t32, 3) if t64 else None,
+        "iterations_fp32": its32,
+        "iterations_fp64": its64,
+        "relres_fp32": rel32,
+        "relres_fp64": rel64,
+        "setup_s": round(setup_s, 2),
+        "vcycle": vc,
+        "kernels": prof,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def cpu_baseline(ml, rp, col, val, b_h, iters):
+    """Oracle (oracle/oracle.c, one core) timing of one V-cycle plus one SpMV
+    on the product's own hierarchy, scaled to a full solve of `iters`
+    iterations (+2 for the initial cycles)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import checkers as K
+    As, Ps = [], []
+    for l in range(ml.num_levels):
+        Al, Pl = ml.level(l)
+        As.append(K.Csr(*Al.to_csr(), Al.n_cols))
+        if Pl is not None:
+            Ps.append(K.Csr(*Pl.to_csr(), Pl.n_cols))
+    H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=2.0 / 3.0)
+    A = K.Csr(rp, col, val)
+    reps, tc, ts = 0, 0.0, 0.0
+    t_start = time.perf_counter()
+    while reps < 3 and (reps == 0 or time.perf_counter() - t_start < 20.0):
+        t = time.perf_counter()
+        H.cycle(np.zeros(A.n_rows), b_h)
+        tc += time.perf_counter() - t
+        t = time.perf_counter()
+        K.spmv(A, b_h)
+        ts += time.perf_counter() - t
+        reps += 1
+    per_iter = (tc + ts) / reps
+    est = per_iter * (iters + 2)
+    log(f"cpu baseline: {per_iter:.2f}s per iteration (1 core) -> {est:.1f}s per solve")
+    return dict(value=round(est, 2), unit="s", cores=1, kind="port",
+                sample=f"{reps} x (one FP64 V-cycle + one SpMV) of the oracle on the full hierarchy, "
+                       f"{per_iter:.3f}s each, scaled by {iters}+2 iterations")
+
+
+if __name__ == "__main__":
+    main()

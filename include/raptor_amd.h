@@ -1,0 +1,275 @@
+/*
+ * raptor_amd.h -- C ABI of the MI355X (gfx950) AMG solve path.
+ *
+ * One context owns one GPU and one HIP stream.  Matrices are CSR with int32
+ * row pointers and columns and FP64 values, the layout of
+ * raptor::CSRMatrix (core/matrix.hpp:392-394 of the reference); at one rank
+ * a raptor::ParCSRMatrix is exactly its on_proc block with identity column
+ * maps (core/par_matrix.cpp:111-157), so the same arrays describe both.
+ *
+ * Conventions
+ *   - Every function returns RAMD_OK (0) or a negative RAMD_E* code; the
+ *     text of the last error of the calling thread is in ramd_last_error().
+ *   - Pointers named *_d are device pointers (from ramd_malloc or any HIP
+ *     allocation on the context's device); all others are host pointers.
+ *   - Calls on one context are serialised on its stream.  Calls that return
+ *     host data synchronise; the rest are stream-ordered.  Handles are not
+ *     thread-safe.
+ *   - The caller keeps ownership of every array it passes; handles returned
+ *     through out-parameters are owned by the caller and freed with the
+ *     matching *_destroy / ramd_free call.
+ *   - No entry point falls back to a CPU path: without a usable gfx950
+ *     device every device entry point fails with RAMD_E_DEVICE.
+ */
+#ifndef RAPTOR_AMD_H
+#define RAPTOR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* status codes                                                         */
+/* ------------------------------------------------------------------ */
+#define RAMD_OK               0
+#define RAMD_E_ARG          (-1)  /* invalid argument / shape mismatch          */
+#define RAMD_E_DEVICE       (-2)  /* HIP runtime error or no usable device      */
+#define RAMD_E_ALLOC        (-3)  /* device or host allocation failed           */
+#define RAMD_E_INDEFINITE   (-4)  /* p.Ap < 0 in CG/PCG (reference: exit(-1))   */
+#define RAMD_E_NOT_CONVERGED (-5) /* iteration limit reached (info, x is valid) */
+#define RAMD_E_STATE        (-6)  /* handle in the wrong state (e.g. no setup)  */
+#define RAMD_E_NUMERIC      (-7)  /* NaN/Inf detected or singular coarse matrix */
+
+typedef struct ramd_ctx_s*  ramd_ctx;
+typedef struct ramd_csr_s*  ramd_csr;
+typedef struct ramd_hier_s* ramd_hier;
+
+/* Text of the calling thread's last error ("" when none). */
+int ramd_last_error(char* buf, size_t len);
+
+/* ------------------------------------------------------------------ */
+/* context and device memory                                            */
+/* ------------------------------------------------------------------ */
+/* Opens device `device` and creates the context's stream.  No reference
+ * counterpart: the reference's equivalent state is MPI_COMM_WORLD
+ * (core/mpi_types.hpp:52-131). */
+int ramd_init(int device, ramd_ctx* out);
+int ramd_finalize(ramd_ctx ctx);
+int ramd_sync(ramd_ctx ctx);
+int ramd_malloc(ramd_ctx ctx, size_t bytes, void** out_d);
+int ramd_free(ramd_ctx ctx, void* p_d);
+int ramd_memcpy_h2d(ramd_ctx ctx, void* dst_d, const void* src, size_t bytes);
+int ramd_memcpy_d2h(ramd_ctx ctx, void* dst, const void* src_d, size_t bytes);
+int ramd_memcpy_d2d(ramd_ctx ctx, void* dst_d, const void* src_d, size_t bytes);
+int ramd_memset_zero(ramd_ctx ctx, void* p_d, size_t bytes);
+
+/* ------------------------------------------------------------------ */
+/* device-resident CSR (ParCSRMatrix at one rank)                       */
+/* ------------------------------------------------------------------ */
+/* Copies host CSR arrays to the device.  Replaces constructing a
+ * ParCSRMatrix and its finalize() (core/par_matrix.hpp:481-611,
+ * core/par_matrix.cpp:111-157); the rows are stored as given. */
+int ramd_csr_create(ramd_ctx ctx, int32_t n_rows, int32_t n_cols, int64_t nnz,
+                    const int32_t* rowptr, const int32_t* col, const double* val,
+                    ramd_csr* out);
+int ramd_csr_info(ramd_csr A, int32_t* n_rows, int32_t* n_cols, int64_t* nnz);
+/* Copies the device arrays back (any output pointer may be NULL). */
+int ramd_csr_download(ramd_csr A, int32_t* rowptr, int32_t* col, double* val);
+int ramd_csr_destroy(ramd_csr A);
+
+/* ------------------------------------------------------------------ */
+/* SpMV family (FP64; each row summed left to right in stored order, as  */
+/* util/linalg/spmv.cpp:52-163 does, so results are bit-identical)       */
+/* ------------------------------------------------------------------ */
+/* y = A x.        ParCSRMatrix::mult, util/linalg/par_spmv.cpp:25-59,292-295 */
+int ramd_spmv(ramd_csr A, const double* x_d, double* y_d);
+/* y += A x.       ParMatrix::mult_append, util/linalg/par_spmv.cpp:91-125 */
+int ramd_spmv_append(ramd_csr A, const double* x_d, double* y_d);
+/* r = b - A x.    ParMatrix::residual, util/linalg/par_spmv.cpp:215-252 */
+int ramd_residual(ramd_csr A, const double* x_d, const double* b_d, double* r_d);
+/* y = A^T x (y has n_cols entries).  ParMatrix::mult_T,
+ * util/linalg/par_spmv.cpp:157-187.  Computed as a gather over a device
+ * transpose built on first use, summing in ascending row order like the
+ * reference's scatter. */
+int ramd_spmv_t(ramd_csr A, const double* x_d, double* y_d);
+/* `sweeps` sweeps of weighted Jacobi, x <- (1-w) x + w (b - sum_{j!=i} a_ij x_j)/a_ii.
+ * The formula of util/linalg/relax.cpp:10-43, which the reference's
+ * parallel jacobi (util/linalg/par_relax.cpp:196-220) intends; tmp_d is n
+ * doubles of workspace, as in the reference's signature. */
+int ramd_jacobi(ramd_csr A, double* x_d, const double* b_d, double* tmp_d,
+                int sweeps, double omega);
+
+/* ------------------------------------------------------------------ */
+/* vectors (core/par_vector.cpp)                                         */
+/* ------------------------------------------------------------------ */
+/* ParVector::inner_product, core/par_vector.cpp:101-121 (deterministic:
+ * fixed partition and fixed combination order). */
+int ramd_dot(ramd_ctx ctx, int64_t n, const double* x_d, const double* y_d, double* out);
+/* ParVector::norm(2), core/par_vector.cpp:88-98 (entries |v|<=1e-16 skipped). */
+int ramd_norm2(ramd_ctx ctx, int64_t n, const double* x_d, double* out);
+/* y += alpha x.   ParVector::axpy, core/par_vector.cpp:21-27 */
+int ramd_axpy(ramd_ctx ctx, int64_t n, double alpha, const double* x_d, double* y_d);
+
+/* ------------------------------------------------------------------ */
+/* kernel timing (replaces the reference's track_times buckets,          */
+/* multilevel/par_multilevel.hpp:133-211, with device-side events)       */
+/* ------------------------------------------------------------------ */
+/* From now on, every row-block kernel (SpMV, append, residual, Jacobi) on a
+ * matrix with at least min_nnz stored entries is bracketed by hipEvents on
+ * the context's stream; min_nnz < 0 turns timing off.  Resets the totals. */
+int ramd_prof_enable(ramd_ctx ctx, int64_t min_nnz);
+/* Totals for op (0 SpMV, 1 append, 2 residual, 3 Jacobi) and precision
+ * (RAMD_PREC_FP64 / RAMD_PREC_FP32): launches, summed kernel milliseconds
+ * and summed algorithmic bytes (matrix once, x once per column, vectors
+ * once).  Synchronises the stream. */
+int ramd_prof_read(ramd_ctx ctx, int op, int prec, int64_t* launches, double* ms, double* bytes);
+
+/* ------------------------------------------------------------------ */
+/* Galerkin product (two-phase SpGEMM on the device)                     */
+/* ------------------------------------------------------------------ */
+/* C = A B.  ParCSRMatrix::mult(ParCSRMatrix*), util/linalg/par_matmult.cpp:
+ * 73-107 -> spgemm_helper util/linalg/matmult.cpp:71-137 + add_append
+ * util/linalg/add.cpp:39-78.  Entries with |c_ij| <= 1e-16 are dropped,
+ * every entry is accumulated in the reference's product order, and the
+ * rows come out sorted by column. */
+int ramd_spgemm(ramd_csr A, ramd_csr B, ramd_csr* C);
+/* C = P^T B.  ParCSRMatrix::mult_T(ParCSRMatrix*),
+ * util/linalg/par_matmult.cpp:141-196,387-570 -> spgemm_T_helper
+ * util/linalg/matmult.cpp:139-204.  Same drop rule and row order. */
+int ramd_spgemm_tn(ramd_csr P, ramd_csr B, ramd_csr* C);
+/* Ac = P^T (A P) with the diagonal moved to the front of each row: the two
+ * calls at ruge_stuben/par_ruge_stuben_solver.hpp:139-143 (and
+ * aggregation/par_smoothed_aggregation_solver.hpp:99-101). */
+int ramd_galerkin(ramd_csr A, ramd_csr P, ramd_csr* Ac);
+
+/* ------------------------------------------------------------------ */
+/* host hierarchy setup (bit-identical to the reference's setup)         */
+/* ------------------------------------------------------------------ */
+/* coarsen_t, interp_t, strength_t, relax_t, agg_t of core/types.hpp:158-164 */
+enum { RAMD_RS = 0, RAMD_CLJP = 1, RAMD_FALGOUT = 2, RAMD_PMIS = 3, RAMD_HMIS = 4 };
+enum { RAMD_DIRECT = 0, RAMD_MOD_CLASSICAL = 1, RAMD_EXTENDED = 2 };
+enum { RAMD_CLASSICAL = 0, RAMD_SYMMETRIC = 1 };
+enum { RAMD_JACOBI = 0 };
+enum { RAMD_PREC_FP64 = 0, RAMD_PREC_FP32 = 1 };
+enum { RAMD_SOLVER_RS = 0, RAMD_SOLVER_SA = 1 };
+
+/* Options of ParMultilevel / ParRugeStubenSolver /
+ * ParSmoothedAggregationSolver (multilevel/par_multilevel.hpp:79-100,
+ * ruge_stuben/par_ruge_stuben_solver.hpp:15-25,
+ * aggregation/par_smoothed_aggregation_solver.hpp:17-32). */
+typedef struct {
+    int solver;             /* RAMD_SOLVER_RS or RAMD_SOLVER_SA                 */
+    double strong_threshold;/* RS default 0.25, SA 0.0                          */
+    int coarsen_type;       /* RS only, RAMD_RS..RAMD_HMIS                      */
+    int interp_type;        /* RS only, RAMD_DIRECT..RAMD_EXTENDED              */
+    int strength_type;      /* RAMD_CLASSICAL or RAMD_SYMMETRIC                 */
+    double interp_filter;   /* extended interpolation filter (default 0.3)      */
+    double prolong_weight;  /* SA Jacobi prolongation weight (default 4/3)      */
+    int num_smooth_sweeps;  /* default 1                                        */
+    double relax_weight;    /* default 1.0 (configs use 2/3)                    */
+    int max_coarse;         /* default 50                                       */
+    int max_levels;         /* default 25                                       */
+    const double* weights;  /* optional CLJP/PMIS random weights (n entries);
+                               NULL = the reference's srand(2448422)+rand()
+                               stream (multilevel/par_multilevel.hpp:215-225)  */
+} ramd_amg_opts;
+
+/* Fills opts with the reference's defaults for `solver`. */
+int ramd_amg_opts_default(int solver, ramd_amg_opts* opts);
+
+/* ParMultilevel::setup (multilevel/par_multilevel.hpp:126-212): builds
+ * strength, splitting/aggregation and interpolation on the host exactly as
+ * the reference does, forms every Galerkin product on the device, factors
+ * the coarsest matrix and keeps every level device-resident.  A is the
+ * caller's fine matrix (host arrays, copied). */
+int ramd_hier_setup(ramd_ctx ctx, int32_t n, int64_t nnz, const int32_t* rowptr,
+                    const int32_t* col, const double* val, const ramd_amg_opts* opts,
+                    ramd_hier* out);
+int ramd_hier_destroy(ramd_hier h);
+/* num_levels and per-level sizes (arrays of length >= num_levels, may be NULL). */
+int ramd_hier_info(ramd_hier h, int* num_levels, int32_t* n_rows, int64_t* nnz_A,
+                   int64_t* nnz_P);
+/* Borrowed handles of level l's A (diagonal first) and P (NULL at the
+ * coarsest level); valid until ramd_hier_destroy. */
+int ramd_hier_level(ramd_hier h, int level, ramd_csr* A, ramd_csr* P);
+/* C/F states (1 = C, 0 = F, -2 = no neighbours) or SA aggregates of level l. */
+int ramd_hier_states(ramd_hier h, int level, int32_t* states);
+/* Host-seconds spent per setup phase: [strength, coarsen, interp, galerkin, other]. */
+int ramd_hier_setup_times(ramd_hier h, double* times5);
+/* Builds the FP32 copies of every level used by RAMD_PREC_FP32 cycles. */
+int ramd_hier_enable_fp32(ramd_hier h);
+
+/* ------------------------------------------------------------------ */
+/* solve path                                                          */
+/* ------------------------------------------------------------------ */
+/* ParMultilevel::cycle(x, b, 0), multilevel/par_multilevel.hpp:341-457:
+ * one V(nu,nu) cycle on the device.  x_is_zero != 0 promises x == 0 on
+ * entry (as in every PCG call), which lets the first pre-smoothing sweep
+ * skip its matrix read without changing a bit of the result. */
+int ramd_vcycle(ramd_hier h, double* x_d, const double* b_d, int x_is_zero);
+/* The same cycle from x = 0 on FP32 copies of every level (values and
+ * vectors), the coarsest solve kept in FP64: the preconditioner of
+ * ramd_fpcg(..., RAMD_PREC_FP32).  No reference counterpart (the reference
+ * is FP64 only). */
+int ramd_vcycle_fp32(ramd_hier h, float* x_d, const float* b_d);
+
+/* ParMultilevel::solve, multilevel/par_multilevel.hpp:459-538: stationary
+ * V-cycles until ||b-Ax||/||b|| <= tol or max_iter cycles; residuals has
+ * max_iter+1 entries (may be NULL).  Returns the iteration count in *iters. */
+int ramd_amg_solve(ramd_hier h, double* x_d, const double* b_d, double tol,
+                   int max_iter, double* residuals, int32_t* iters);
+
+/* PCG(A, ml, x, b, res, tol, max_iter), krylov/par_cg.cpp:121-263, with the
+ * reference's rules: an extra V-cycle on b sets the scale (tol *= sqrt(b.z0)),
+ * res[0] = sqrt(r0.z0) and later res[k] = (r_k.z_k)/(b.z0), stop when
+ * r.z < tol, true residual and p = z restart every 8th iteration.  A is the
+ * caller's matrix (the one PCG multiplies by; typically the sorted fine
+ * matrix).  res must hold max_res entries; *n_res gets the count written.
+ * Returns RAMD_E_INDEFINITE where the reference calls exit(-1). */
+int ramd_pcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double tol,
+             int max_iter, double* res, int32_t max_res, int32_t* n_res, int32_t* iters);
+
+/* Mixed-precision solve: flexible PCG in FP64 (true residual b-Ax, inner
+ * products and the coarse LU in FP64) preconditioned by one V-cycle in the
+ * precision `cycle_prec`, iterated until ||b-Ax||_2/||b||_2 <= rtol (true,
+ * FP64) or max_iter.  No reference counterpart (the reference is FP64 only);
+ * it is the precision form of PCG above.  relres_hist (may be NULL) receives
+ * the true relative residual after each iteration. */
+int ramd_fpcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double rtol,
+              int max_iter, int cycle_prec, double* relres_hist, int32_t* iters,
+              double* final_relres);
+
+/* ------------------------------------------------------------------ */
+/* host setup pieces, exposed for parity tests                          */
+/* ------------------------------------------------------------------ */
+/* Results are malloc'd by the library and released with ramd_host_free. */
+void ramd_host_free(void* p);
+/* strength(Classical|Symmetric, theta): par_strength.cpp:8-547. */
+int ramd_host_strength(int32_t n, const int32_t* rowptr, const int32_t* col,
+                       const double* val, int strength_type, double theta,
+                       int32_t** s_rowptr, int32_t** s_col, int64_t* s_nnz);
+/* C/F splitting of S (split_* ruge_stuben/par_cf_splitting.cpp:7-112) with
+ * optional weights (NULL = the reference's rand() stream). */
+int ramd_host_split(int32_t n, const int32_t* s_rowptr, const int32_t* s_col,
+                    int coarsen_type, const double* weights, int32_t* states);
+/* Interpolation P (ruge_stuben/par_interpolation.cpp): direct :1453-1754,
+ * mod_classical :991-1451, extended :280-989 (+ filter_interp :175-278). */
+int ramd_host_interp(int32_t n, const int32_t* rowptr, const int32_t* col, const double* val,
+                     const int32_t* s_rowptr, const int32_t* s_col, const int32_t* states,
+                     int interp_type, double filter, int32_t* n_coarse,
+                     int32_t** p_rowptr, int32_t** p_col, double** p_val, int64_t* p_nnz);
+/* Stencil-grid generators (gallery/par_stencil.hpp:16-236 with
+ * gallery/laplacian27pt.hpp:27-39 / gallery/diffusion.hpp:58-85); kind 0 =
+ * 27-point Laplacian on m^3, kind 1 = rotated anisotropic 9-point diffusion
+ * (eps, theta) on m^2, kind 2 = 7-point finite-volume variable-coefficient
+ * diffusion on m^3 with log-normal cell coefficients (seed). */
+int ramd_host_gallery(int kind, int32_t m, double eps, double theta, uint64_t seed,
+                      int32_t* n, int32_t** rowptr, int32_t** col, double** val, int64_t* nnz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RAPTOR_AMD_H */

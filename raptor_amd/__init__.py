@@ -1,0 +1,389 @@
+"""raptor_amd -- MI355X (gfx950) AMG solve path behind RAPtor's interfaces.
+
+Python mirror of the reference's one-rank API, over the C ABI of
+libraptor_amd.so (include/raptor_amd.h):
+
+    ParCSRMatrix      device CSR; mult / mult_append / residual / mult_T and
+                      the Galerkin products mult(P) / mult_T(AP)
+                      (core/par_matrix.hpp, util/linalg/par_spmv.cpp,
+                      util/linalg/par_matmult.cpp)
+    ParVector         device FP64 vector (core/par_vector.hpp)
+    ParRugeStubenSolver
+                      ParMultilevel with RS setup: setup / cycle / solve
+                      (multilevel/par_multilevel.hpp,
+                      ruge_stuben/par_ruge_stuben_solver.hpp)
+    PCG               krylov/par_cg.cpp:121-263
+    strength / split / interpolation / gallery
+                      host setup pieces (par_strength.cpp,
+                      ruge_stuben/par_cf_splitting.cpp,
+                      ruge_stuben/par_interpolation.cpp, gallery/)
+
+Every compute call runs the HIP library; nothing falls back to a CPU path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (CLJP, HMIS, PMIS, RS, Classical, Direct, Extended, Falgout, Jacobi, ModClassical,
+                   PREC_FP32, PREC_FP64, RamdError, Symmetric, check, lib)
+
+__all__ = [
+    "Context", "ParVector", "ParCSRMatrix", "ParRugeStubenSolver", "PCG", "strength", "split",
+    "interpolation", "gallery_laplace27", "gallery_aniso2d", "gallery_fv7", "RamdError",
+    "RS", "CLJP", "Falgout", "PMIS", "HMIS", "Direct", "ModClassical", "Extended", "Classical",
+    "Symmetric", "Jacobi", "PREC_FP64", "PREC_FP32",
+]
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One GPU and its stream (ramd_init)."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        check(lib().ramd_init(device, C.byref(self.h)))
+
+    def sync(self):
+        check(lib().ramd_sync(self.h))
+
+    def close(self):
+        if self.h:
+            lib().ramd_finalize(self.h)
+            self.h = C.c_void_p()
+
+
+class ParVector:
+    """Device FP64 vector of length n (core/par_vector.hpp:44-176)."""
+
+    def __init__(self, ctx, n, dtype=np.float64):
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        self.ptr = C.c_void_p()
+        check(lib().ramd_malloc(ctx.h, max(1, self.n) * self.dtype.itemsize, C.byref(self.ptr)))
+
+    @classmethod
+    def from_numpy(cls, ctx, a):
+        a = np.ascontiguousarray(a)
+        v = cls(ctx, a.size, a.dtype)
+        v.copy_from(a)
+        return v
+
+    def copy_from(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        check(lib().ramd_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        check(lib().ramd_memcpy_d2h(self.ctx.h, out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def set_const_value(self, v):
+        if v == 0:
+            check(lib().ramd_memset_zero(self.ctx.h, self.ptr, self.n * self.dtype.itemsize))
+        else:
+            self.copy_from(np.full(self.n, v, dtype=self.dtype))
+
+    def copy(self, other):
+        check(lib().ramd_memcpy_d2d(self.ctx.h, self.ptr, other.ptr, self.n * self.dtype.itemsize))
+
+    def inner_product(self, other):
+        out = C.c_double()
+        check(lib().ramd_dot(self.ctx.h, self.n, self.ptr, other.ptr, C.byref(out)))
+        return out.value
+
+    def norm(self, p=2):
+        assert p == 2
+        out = C.c_double()
+        check(lib().ramd_norm2(self.ctx.h, self.n, self.ptr, C.byref(out)))
+        return out.value
+
+    def axpy(self, x, alpha):
+        check(lib().ramd_axpy(self.ctx.h, self.n, float(alpha), x.ptr, self.ptr))
+
+    def free(self):
+        if self.ptr:
+            lib().ramd_free(self.ctx.h, self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class ParCSRMatrix:
+    """Device CSR matrix: a ParCSRMatrix at one rank (on_proc block only)."""
+
+    def __init__(self, ctx, handle, owned=True):
+        self.ctx, self.h, self.owned = ctx, handle, owned
+        nr, nc, nnz = C.c_int32(), C.c_int32(), C.c_int64()
+        check(lib().ramd_csr_info(self.h, C.byref(nr), C.byref(nc), C.byref(nnz)))
+        self.n_rows, self.n_cols, self.nnz = nr.value, nc.value, nnz.value
+
+    @classmethod
+    def from_csr(cls, ctx, rowptr, col, val, n_cols=None):
+        rowptr, col, val = _i32(rowptr), _i32(col), _f64(val)
+        n = rowptr.size - 1
+        n_cols = n if n_cols is None else int(n_cols)
+        h = C.c_void_p()
+        check(lib().ramd_csr_create(ctx.h, n, n_cols, col.size, _ptr(rowptr, C.c_int32), _ptr(col, C.c_int32),
+                                    _ptr(val, C.c_double), C.byref(h)))
+        return cls(ctx, h)
+
+    def to_csr(self):
+        rp = np.empty(self.n_rows + 1, np.int32)
+        col = np.empty(max(1, self.nnz), np.int32)
+        val = np.empty(max(1, self.nnz), np.float64)
+        check(lib().ramd_csr_download(self.h, _ptr(rp, C.c_int32), _ptr(col, C.c_int32), _ptr(val, C.c_double)))
+        return rp, col[:self.nnz], val[:self.nnz]
+
+    # --- SpMV family (util/linalg/par_spmv.cpp)
+    def mult(self, x, b=None):
+        if isinstance(x, ParCSRMatrix):
+            h = C.c_void_p()
+            check(lib().ramd_spgemm(self.h, x.h, C.byref(h)))
+            return ParCSRMatrix(self.ctx, h)
+        check(lib().ramd_spmv(self.h, x.ptr, b.ptr))
+
+    def mult_append(self, x, b):
+        check(lib().ramd_spmv_append(self.h, x.ptr, b.ptr))
+
+    def residual(self, x, b, r):
+        check(lib().ramd_residual(self.h, x.ptr, b.ptr, r.ptr))
+
+    def mult_T(self, x, b=None):
+        """mult_T(vector, out): b = A^T x.   mult_T(P): returns P^T * self."""
+        if isinstance(x, ParCSRMatrix):
+            h = C.c_void_p()
+            check(lib().ramd_spgemm_tn(x.h, self.h, C.byref(h)))
+            return ParCSRMatrix(self.ctx, h)
+        check(lib().ramd_spmv_t(self.h, x.ptr, b.ptr))
+
+    def galerkin(self, P):
+        h = C.c_void_p()
+        check(lib().ramd_galerkin(self.h, P.h, C.byref(h)))
+        return ParCSRMatrix(self.ctx, h)
+
+    def jacobi(self, x, b, tmp, num_sweeps=1, omega=1.0):
+        check(lib().ramd_jacobi(self.h, x.ptr, b.ptr, tmp.ptr, num_sweeps, float(omega)))
+
+    def free(self):
+        if self.h and self.owned:
+            lib().ramd_csr_destroy(self.h)
+        self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def default_opts(solver=_lib.SOLVER_RS):
+    o = _lib.AmgOpts()
+    check(lib().ramd_amg_opts_default(solver, C.byref(o)))
+    return o
+
+
+class ParRugeStubenSolver:
+    """ParRugeStubenSolver(strong_threshold, coarsen, interp, strength, relax)
+    (ruge_stuben/par_ruge_stuben_solver.hpp:15-25) with the public fields of
+    ParMultilevel (multilevel/par_multilevel.hpp:626-658)."""
+
+    def __init__(self, ctx, strong_threshold=0.0, coarsen_type=RS, interp_type=Direct,
+                 strength_type=Classical, relax_type=Jacobi):
+        if relax_type != Jacobi:
+            raise ValueError("the device V-cycle smooths with weighted Jacobi only")
+        self.ctx = ctx
+        self.strong_threshold = strong_threshold
+        self.coarsen_type = coarsen_type
+        self.interp_type = interp_type
+        self.strength_type = strength_type
+        self.relax_type = relax_type
+        o = default_opts()
+        self.num_smooth_sweeps = o.num_smooth_sweeps
+        self.relax_weight = o.relax_weight
+        self.max_coarse = o.max_coarse
+        self.max_levels = o.max_levels
+        self.interp_filter = o.interp_filter
+        self.solve_tol = 1e-7
+        self.max_iterations = 100
+        self.weights = None
+        self.h = None
+
+    def setup(self, rowptr, col, val):
+        """ParMultilevel::setup(Af): A is given as host CSR arrays."""
+        rowptr, col, val = _i32(rowptr), _i32(col), _f64(val)
+        o = default_opts()
+        o.strong_threshold = self.strong_threshold
+        o.coarsen_type = self.coarsen_type
+        o.interp_type = self.interp_type
+        o.strength_type = self.strength_type
+        o.interp_filter = self.interp_filter
+        o.num_smooth_sweeps = self.num_smooth_sweeps
+        o.relax_weight = self.relax_weight
+        o.max_coarse = self.max_coarse
+        o.max_levels = self.max_levels
+        w = None
+        if self.weights is not None:
+            w = _f64(self.weights)
+            o.weights = _ptr(w, C.c_double)
+        h = C.c_void_p()
+        check(lib().ramd_hier_setup(self.ctx.h, rowptr.size - 1, col.size, _ptr(rowptr, C.c_int32),
+                                    _ptr(col, C.c_int32), _ptr(val, C.c_double), C.byref(o), C.byref(h)))
+        self.h = h
+        nl = C.c_int()
+        check(lib().ramd_hier_info(self.h, C.byref(nl), None, None, None))
+        self.num_levels = nl.value
+        return self
+
+    def level_sizes(self):
+        L = self.num_levels
+        n = np.zeros(L, np.int32)
+        na = np.zeros(L, np.int64)
+        npp = np.zeros(L, np.int64)
+        check(lib().ramd_hier_info(self.h, None, _ptr(n, C.c_int32), _ptr(na, C.c_int64), _ptr(npp, C.c_int64)))
+        return n, na, npp
+
+    def level(self, l):
+        a, p = C.c_void_p(), C.c_void_p()
+        check(lib().ramd_hier_level(self.h, l, C.byref(a), C.byref(p)))
+        A = ParCSRMatrix(self.ctx, a, owned=False)
+        P = ParCSRMatrix(self.ctx, p, owned=False) if p.value else None
+        return A, P
+
+    def states(self, l):
+        n, _, _ = self.level_sizes()
+        s = np.empty(int(n[l]), np.int32)
+        check(lib().ramd_hier_states(self.h, l, _ptr(s, C.c_int32)))
+        return s
+
+    def setup_times(self):
+        t = np.zeros(5)
+        check(lib().ramd_hier_setup_times(self.h, _ptr(t, C.c_double)))
+        return t
+
+    def enable_fp32(self):
+        check(lib().ramd_hier_enable_fp32(self.h))
+
+    def cycle(self, x, b, x_is_zero=False):
+        check(lib().ramd_vcycle(self.h, x.ptr, b.ptr, 1 if x_is_zero else 0))
+
+    def solve(self, x, b):
+        res = np.zeros(self.max_iterations + 1)
+        it = C.c_int32()
+        check(lib().ramd_amg_solve(self.h, x.ptr, b.ptr, self.solve_tol, self.max_iterations,
+                                   _ptr(res, C.c_double), C.byref(it)))
+        self.residuals = res[:it.value + 1]
+        return it.value
+
+    def free(self):
+        if self.h:
+            lib().ramd_hier_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def PCG(A, ml, x, b, tol=1e-5, max_iter=-1, max_res=10000):
+    """PCG(A, ml, x, b, res, tol, max_iter), krylov/par_cg.cpp:121-263.
+    Returns (residual history, iteration count)."""
+    res = np.zeros(max_res)
+    nres, it = C.c_int32(), C.c_int32()
+    check(lib().ramd_pcg(ml.h, A.h, x.ptr, b.ptr, float(tol), int(max_iter), _ptr(res, C.c_double), max_res,
+                         C.byref(nres), C.byref(it)))
+    return res[:min(nres.value, max_res)], it.value
+
+
+def FPCG(A, ml, x, b, rtol=1e-8, max_iter=200, cycle_prec=PREC_FP64):
+    """Flexible PCG to a true FP64 relative residual; returns (history, iters, final relres)."""
+    hist = np.zeros(max_iter + 1)
+    it, fin = C.c_int32(), C.c_double()
+    check(lib().ramd_fpcg(ml.h, A.h, x.ptr, b.ptr, float(rtol), int(max_iter), int(cycle_prec),
+                          _ptr(hist, C.c_double), C.byref(it), C.byref(fin)))
+    return hist[:it.value + 1], it.value, fin.value
+
+
+# ---------------------------------------------------------------------------
+# host setup pieces
+# ---------------------------------------------------------------------------
+def _take(ptr, n, dtype):
+    if n == 0:
+        lib().ramd_host_free(C.cast(ptr, C.c_void_p))
+        return np.zeros(0, dtype)
+    a = np.ctypeslib.as_array(ptr, shape=(n,)).copy().astype(dtype)
+    lib().ramd_host_free(C.cast(ptr, C.c_void_p))
+    return a
+
+
+def strength(rowptr, col, val, strength_type=Classical, theta=0.0):
+    rowptr, col, val = _i32(rowptr), _i32(col), _f64(val)
+    n = rowptr.size - 1
+    srp, scol, snnz = C.POINTER(C.c_int32)(), C.POINTER(C.c_int32)(), C.c_int64()
+    check(lib().ramd_host_strength(n, _ptr(rowptr, C.c_int32), _ptr(col, C.c_int32), _ptr(val, C.c_double),
+                                   strength_type, float(theta), C.byref(srp), C.byref(scol), C.byref(snnz)))
+    return _take(srp, n + 1, np.int32), _take(scol, snnz.value, np.int32)
+
+
+def split(s_rowptr, s_col, coarsen_type, weights=None):
+    s_rowptr, s_col = _i32(s_rowptr), _i32(s_col)
+    n = s_rowptr.size - 1
+    states = np.empty(n, np.int32)
+    w = None if weights is None else _f64(weights)
+    check(lib().ramd_host_split(n, _ptr(s_rowptr, C.c_int32), _ptr(s_col, C.c_int32), coarsen_type,
+                                None if w is None else _ptr(w, C.c_double), _ptr(states, C.c_int32)))
+    return states
+
+
+def interpolation(rowptr, col, val, s_rowptr, s_col, states, interp_type, filter_threshold=0.0):
+    rowptr, col, val = _i32(rowptr), _i32(col), _f64(val)
+    s_rowptr, s_col, states = _i32(s_rowptr), _i32(s_col), _i32(states)
+    n = rowptr.size - 1
+    nc = C.c_int32()
+    prp, pcol, pval, pnnz = C.POINTER(C.c_int32)(), C.POINTER(C.c_int32)(), C.POINTER(C.c_double)(), C.c_int64()
+    check(lib().ramd_host_interp(n, _ptr(rowptr, C.c_int32), _ptr(col, C.c_int32), _ptr(val, C.c_double),
+                                 _ptr(s_rowptr, C.c_int32), _ptr(s_col, C.c_int32), _ptr(states, C.c_int32),
+                                 interp_type, float(filter_threshold), C.byref(nc), C.byref(prp), C.byref(pcol),
+                                 C.byref(pval), C.byref(pnnz)))
+    return (_take(prp, n + 1, np.int32), _take(pcol, pnnz.value, np.int32), _take(pval, pnnz.value, np.float64),
+            nc.value)
+
+
+def _gallery(kind, m, eps=1.0, theta=0.0, seed=0):
+    n, nnz = C.c_int32(), C.c_int64()
+    rp, col, val = C.POINTER(C.c_int32)(), C.POINTER(C.c_int32)(), C.POINTER(C.c_double)()
+    check(lib().ramd_host_gallery(kind, int(m), float(eps), float(theta), int(seed), C.byref(n), C.byref(rp),
+                                  C.byref(col), C.byref(val), C.byref(nnz)))
+    return (_take(rp, n.value + 1, np.int32), _take(col, nnz.value, np.int32), _take(val, nnz.value, np.float64))
+
+
+def gallery_laplace27(m):
+    """par_stencil_grid(laplace_stencil_27pt(), {m,m,m}, 3)."""
+    return _gallery(0, m)
+
+
+def gallery_aniso2d(m, eps=0.001, theta=np.pi / 8):
+    """par_stencil_grid(diffusion_stencil_2d(eps, theta), {m,m}, 2)."""
+    return _gallery(1, m, eps, theta)
+
+
+def gallery_fv7(m, seed=3):
+    """7-point finite-volume diffusion, log-normal cell coefficients (seeded)."""
+    return _gallery(2, m, seed=seed)

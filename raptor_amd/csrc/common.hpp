@@ -1,0 +1,122 @@
+// Shared definitions of libraptor_amd: error plumbing, the context and the
+// device-resident CSR handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "raptor_amd.h"
+#include "status.hpp"
+
+namespace ramd {
+
+#define RAMD_HIP(call)                                                                    \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            ::ramd::fail(RAMD_E_DEVICE, "%s failed: %s (%s:%d)", #call,                   \
+                         hipGetErrorString(e_), __FILE__, __LINE__);                      \
+    } while (0)
+
+// Device allocation that reports RAMD_E_ALLOC with the size.
+void* dmalloc(size_t bytes);
+void dfree(void* p);
+
+template <class T>
+T* dalloc(size_t n)
+{
+    return static_cast<T*>(dmalloc(n * sizeof(T)));
+}
+
+// Launch geometry of the row-block ("CSR-stream") kernels.
+constexpr int kBlockThreads = 256;   // one workgroup = 4 wavefronts of 64
+constexpr int kStreamCap = 4096;     // products staged in LDS per workgroup
+constexpr int kMaxRowsPerBlock = 256;
+
+// Row blocks for the LDS-staged kernels: consecutive rows whose nonzeros fit
+// in kStreamCap products and number at most kMaxRowsPerBlock; a row longer
+// than kStreamCap forms a block of its own.
+std::vector<int> make_row_blocks(const int* rowptr_host, int n_rows, int cap);
+
+// Keeps XCD-local row ranges together: workgroup ids are dealt round-robin
+// over the 8 XCDs, so remap id -> (id % 8) chunk so each XCD sweeps one
+// contiguous slice of rows (one L2 per XCD keeps its x halo).
+__device__ __forceinline__ int xcd_remap(int bid, int nb)
+{
+    int q = nb >> 3, r = nb & 7, x = bid & 7, k = bid >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
+}  // namespace ramd
+
+// Live kernel timing: hipEvent pairs around the row-block kernels of large
+// matrices, accumulated per (operation, precision).
+struct ProfSlot {
+    int64_t launches = 0;
+    double bytes = 0;  // algorithmic bytes, summed over launches
+    double ms = 0;     // resolved event time
+};
+struct ProfPending {
+    hipEvent_t beg, end;
+    int slot;
+};
+
+struct ramd_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t prof_min_nnz = -1;  // < 0: off
+    ProfSlot prof[8];
+    std::vector<ProfPending> prof_pending;
+    std::vector<hipEvent_t> prof_pool;
+    // scratch for reductions: partial sums and a completion counter
+    double* red_partials = nullptr;
+    unsigned* red_counter = nullptr;
+    double* red_result = nullptr;  // device scalars (16 slots)
+    double* pinned = nullptr;      // pinned host staging for scalars (16 slots)
+    void set_current() const;
+};
+
+struct ramd_csr_s {
+    ramd_ctx ctx = nullptr;
+    int32_t n_rows = 0, n_cols = 0;
+    int64_t nnz = 0;
+    int* rowptr = nullptr;   // device, n_rows+1
+    int* col = nullptr;      // device, nnz
+    double* val = nullptr;   // device, nnz
+    // row blocks of the LDS-staged kernels (device, n_blocks+1)
+    int* blocks = nullptr;
+    int n_blocks = 0;
+    // lazily built transpose (gather form of A^T x), same drop-free pattern
+    ramd_csr_s* trans = nullptr;
+    // FP32 copy of the values (mixed-precision cycle)
+    float* val32 = nullptr;
+    // host copy of rowptr (kept: used to build blocks and by setup)
+    std::vector<int> h_rowptr;
+    bool borrowed = false;   // owned by a hierarchy; ramd_csr_destroy refuses it
+};
+
+namespace ramd {
+// Creates a handle around device arrays the library already owns.
+ramd_csr_s* csr_wrap(ramd_ctx ctx, int n_rows, int n_cols, int64_t nnz, int* rowptr_d, int* col_d,
+                     double* val_d, const int* rowptr_host);
+void csr_free(ramd_csr_s* A);
+// Builds (once) A->trans, the CSR of A^T with rows in ascending source-row order.
+ramd_csr_s* csr_transpose(ramd_csr_s* A);
+void csr_make_fp32(ramd_csr_s* A);
+
+// Kernel launchers (stream-ordered), FP64 and FP32.
+enum StreamOp { OP_SPMV = 0, OP_APPEND = 1, OP_RESIDUAL = 2, OP_JACOBI = 3 };
+void launch_stream_op(const ramd_csr_s* A, int op, const double* x, const double* b, double* y,
+                      double omega, hipStream_t s);
+void launch_stream_op32(const ramd_csr_s* A, int op, const float* x, const float* b, float* y,
+                        float omega, hipStream_t s);
+
+// Deterministic device reductions; result left in *out_d (device scalar).
+void launch_dot(ramd_ctx ctx, int64_t n, const double* x, const double* y, double* out_d);
+void launch_norm2sq(ramd_ctx ctx, int64_t n, const double* x, double* out_d);
+double fetch_scalar(ramd_ctx ctx, const double* d);
+}  // namespace ramd

@@ -1,0 +1,360 @@
+// Context, device memory, device CSR handles and the C entry points that
+// manage them.
+#include <cstdarg>
+#include <cstring>
+#include <new>
+
+#include "common.hpp"
+
+namespace ramd {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+void fail(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    throw Failure{code};
+}
+
+void* dmalloc(size_t bytes)
+{
+    void* p = nullptr;
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(RAMD_E_ALLOC, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return p;
+}
+
+void dfree(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+std::vector<int> make_row_blocks(const int* rp, int n_rows, int cap)
+{
+    std::vector<int> b;
+    b.reserve(n_rows / 64 + 2);
+    b.push_back(0);
+    int r = 0;
+    while (r < n_rows) {
+        int start = r;
+        if (rp[r + 1] - rp[r] > cap) {
+            r++;
+        } else {
+            int64_t acc = 0;
+            while (r < n_rows && r - start < kMaxRowsPerBlock && acc + (rp[r + 1] - rp[r]) <= cap) {
+                acc += rp[r + 1] - rp[r];
+                r++;
+            }
+        }
+        b.push_back(r);
+    }
+    return b;
+}
+
+ramd_csr_s* csr_wrap(ramd_ctx ctx, int n_rows, int n_cols, int64_t nnz, int* rowptr_d, int* col_d,
+                     double* val_d, const int* rowptr_host)
+{
+    ramd_csr_s* A = new ramd_csr_s();
+    A->ctx = ctx;
+    A->n_rows = n_rows;
+    A->n_cols = n_cols;
+    A->nnz = nnz;
+    A->rowptr = rowptr_d;
+    A->col = col_d;
+    A->val = val_d;
+    if (rowptr_host) {
+        A->h_rowptr.assign(rowptr_host, rowptr_host + n_rows + 1);
+    } else {
+        A->h_rowptr.resize((size_t)n_rows + 1);
+        RAMD_HIP(hipMemcpyAsync(A->h_rowptr.data(), rowptr_d, sizeof(int) * (n_rows + 1),
+                                hipMemcpyDeviceToHost, ctx->stream));
+        RAMD_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    std::vector<int> blocks = make_row_blocks(A->h_rowptr.data(), n_rows, kStreamCap);
+    A->n_blocks = (int)blocks.size() - 1;
+    A->blocks = dalloc<int>(blocks.size());
+    RAMD_HIP(hipMemcpyAsync(A->blocks, blocks.data(), sizeof(int) * blocks.size(),
+                            hipMemcpyHostToDevice, ctx->stream));
+    RAMD_HIP(hipStreamSynchronize(ctx->stream));
+    return A;
+}
+
+void csr_free(ramd_csr_s* A)
+{
+    if (!A) return;
+    if (A->trans) csr_free(A->trans);
+    dfree(A->rowptr);
+    dfree(A->col);
+    dfree(A->val);
+    dfree(A->blocks);
+    dfree(A->val32);
+    delete A;
+}
+
+// Counting-sort transpose: entries of column c are emitted in ascending
+// source-row order, the order CSR_to_CSC (core/matrix.cpp:491-538) produces
+// and the order CSR_append_T (util/linalg/spmv.cpp:150-163) adds in.
+ramd_csr_s* csr_transpose(ramd_csr_s* A)
+{
+    if (A->trans) return A->trans;
+    ramd_ctx ctx = A->ctx;
+    const int n = A->n_rows, m = A->n_cols;
+    const int64_t nnz = A->nnz;
+    std::vector<int> col((size_t)nnz);
+    std::vector<double> val((size_t)nnz);
+    if (nnz) {
+        RAMD_HIP(hipMemcpyAsync(col.data(), A->col, sizeof(int) * nnz, hipMemcpyDeviceToHost, ctx->stream));
+        RAMD_HIP(hipMemcpyAsync(val.data(), A->val, sizeof(double) * nnz, hipMemcpyDeviceToHost, ctx->stream));
+        RAMD_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    std::vector<int> tp((size_t)m + 1, 0), trow((size_t)nnz);
+    std::vector<double> tval((size_t)nnz);
+    for (int64_t j = 0; j < nnz; j++) tp[col[j] + 1]++;
+    for (int c = 0; c < m; c++) tp[c + 1] += tp[c];
+    std::vector<int> pos(tp.begin(), tp.end() - 1);
+    const int* rp = A->h_rowptr.data();
+    for (int i = 0; i < n; i++)
+        for (int j = rp[i]; j < rp[i + 1]; j++) {
+            int d = pos[col[j]]++;
+            trow[d] = i;
+            tval[d] = val[j];
+        }
+    int* d_rp = dalloc<int>((size_t)m + 1);
+    int* d_col = dalloc<int>((size_t)nnz);
+    double* d_val = dalloc<double>((size_t)nnz);
+    RAMD_HIP(hipMemcpyAsync(d_rp, tp.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, ctx->stream));
+    if (nnz) {
+        RAMD_HIP(hipMemcpyAsync(d_col, trow.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, ctx->stream));
+        RAMD_HIP(hipMemcpyAsync(d_val, tval.data(), sizeof(double) * nnz, hipMemcpyHostToDevice, ctx->stream));
+    }
+    A->trans = csr_wrap(ctx, m, n, nnz, d_rp, d_col, d_val, tp.data());
+    return A->trans;
+}
+
+__global__ void to_fp32_kernel(const double* __restrict__ in, float* __restrict__ out, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = (float)in[i];
+}
+
+void csr_make_fp32(ramd_csr_s* A)
+{
+    if (A->val32) return;
+    A->val32 = dalloc<float>((size_t)A->nnz);
+    if (A->nnz) {
+        int grid = (int)std::min<int64_t>((A->nnz + 255) / 256, 8192);
+        hipLaunchKernelGGL(to_fp32_kernel, dim3(grid), dim3(256), 0, A->ctx->stream, A->val, A->val32, A->nnz);
+        RAMD_HIP(hipGetLastError());
+    }
+}
+
+}  // namespace ramd
+
+void ramd_ctx_s::set_current() const { RAMD_HIP(hipSetDevice(device)); }
+
+using namespace ramd;
+
+extern "C" {
+
+int ramd_last_error(char* buf, size_t len)
+{
+    if (!buf || len == 0) return RAMD_E_ARG;
+    snprintf(buf, len, "%s", g_last_error.c_str());
+    return RAMD_OK;
+}
+
+int ramd_init(int device, ramd_ctx* out)
+{
+    return guarded([&] {
+        RAMD_CHECK(out, RAMD_E_ARG, "ramd_init: out is NULL");
+        int count = 0;
+        hipError_t e = hipGetDeviceCount(&count);
+        if (e != hipSuccess || count == 0) {
+            (void)hipGetLastError();
+            fail(RAMD_E_DEVICE, "ramd_init: no HIP device available (%s)",
+                 e != hipSuccess ? hipGetErrorString(e) : "0 devices");
+        }
+        RAMD_CHECK(device >= 0 && device < count, RAMD_E_ARG, "ramd_init: device %d out of range (%d devices)",
+                   device, count);
+        hipDeviceProp_t prop;
+        RAMD_HIP(hipGetDeviceProperties(&prop, device));
+        RAMD_CHECK(std::strncmp(prop.gcnArchName, "gfx950", 6) == 0, RAMD_E_DEVICE,
+                   "ramd_init: device %d is %s; this library is built for gfx950 only", device,
+                   prop.gcnArchName);
+        ramd_ctx c = new ramd_ctx_s();
+        c->device = device;
+        RAMD_HIP(hipSetDevice(device));
+        RAMD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->red_partials = dalloc<double>(4096);
+        c->red_counter = dalloc<unsigned>(16);
+        c->red_result = dalloc<double>(16);
+        RAMD_HIP(hipMemsetAsync(c->red_counter, 0, 16 * sizeof(unsigned), c->stream));
+        RAMD_HIP(hipHostMalloc((void**)&c->pinned, 16 * sizeof(double), hipHostMallocDefault));
+        RAMD_HIP(hipStreamSynchronize(c->stream));
+        *out = c;
+    });
+}
+
+int ramd_finalize(ramd_ctx ctx)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx, RAMD_E_ARG, "ramd_finalize: NULL context");
+        ctx->set_current();
+        RAMD_HIP(hipStreamSynchronize(ctx->stream));
+        for (const ProfPending& p : ctx->prof_pending) {
+            (void)hipEventDestroy(p.beg);
+            (void)hipEventDestroy(p.end);
+        }
+        for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
+        dfree(ctx->red_partials);
+        dfree(ctx->red_counter);
+        dfree(ctx->red_result);
+        if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+int ramd_sync(ramd_ctx ctx)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx, RAMD_E_ARG, "ramd_sync: NULL context");
+        RAMD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int ramd_malloc(ramd_ctx ctx, size_t bytes, void** out_d)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx && out_d, RAMD_E_ARG, "ramd_malloc: NULL argument");
+        ctx->set_current();
+        *out_d = dmalloc(bytes);
+    });
+}
+
+int ramd_free(ramd_ctx ctx, void* p_d)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx, RAMD_E_ARG, "ramd_free: NULL context");
+        RAMD_HIP(hipStreamSynchronize(ctx->stream));
+        dfree(p_d);
+    });
+}
+
+int ramd_memcpy_h2d(ramd_ctx ctx, void* dst_d, const void* src, size_t bytes)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx && (bytes == 0 || (dst_d && src)), RAMD_E_ARG, "ramd_memcpy_h2d: NULL argument");
+        if (bytes) RAMD_HIP(hipMemcpyAsync(dst_d, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        RAMD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int ramd_memcpy_d2h(ramd_ctx ctx, void* dst, const void* src_d, size_t bytes)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx && (bytes == 0 || (dst && src_d)), RAMD_E_ARG, "ramd_memcpy_d2h: NULL argument");
+        if (bytes) RAMD_HIP(hipMemcpyAsync(dst, src_d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        RAMD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int ramd_memcpy_d2d(ramd_ctx ctx, void* dst_d, const void* src_d, size_t bytes)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx && (bytes == 0 || (dst_d && src_d)), RAMD_E_ARG, "ramd_memcpy_d2d: NULL argument");
+        if (bytes) RAMD_HIP(hipMemcpyAsync(dst_d, src_d, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    });
+}
+
+int ramd_memset_zero(ramd_ctx ctx, void* p_d, size_t bytes)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx && (bytes == 0 || p_d), RAMD_E_ARG, "ramd_memset_zero: NULL argument");
+        if (bytes) RAMD_HIP(hipMemsetAsync(p_d, 0, bytes, ctx->stream));
+    });
+}
+
+int ramd_csr_create(ramd_ctx ctx, int32_t n_rows, int32_t n_cols, int64_t nnz, const int32_t* rowptr,
+                    const int32_t* col, const double* val, ramd_csr* out)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx && out && rowptr, RAMD_E_ARG, "ramd_csr_create: NULL argument");
+        RAMD_CHECK(n_rows >= 0 && n_cols >= 0 && nnz >= 0 && nnz <= INT32_MAX, RAMD_E_ARG,
+                   "ramd_csr_create: bad sizes n_rows=%d n_cols=%d nnz=%lld", n_rows, n_cols, (long long)nnz);
+        RAMD_CHECK(nnz == 0 || (col && val), RAMD_E_ARG, "ramd_csr_create: NULL col/val");
+        RAMD_CHECK(rowptr[0] == 0 && rowptr[n_rows] == nnz, RAMD_E_ARG,
+                   "ramd_csr_create: rowptr[0]=%d rowptr[n]=%d, expected 0 and %lld", rowptr[0],
+                   rowptr[n_rows], (long long)nnz);
+        for (int i = 0; i < n_rows; i++)
+            RAMD_CHECK(rowptr[i + 1] >= rowptr[i], RAMD_E_ARG, "ramd_csr_create: rowptr decreases at row %d", i);
+        for (int64_t j = 0; j < nnz; j++)
+            RAMD_CHECK(col[j] >= 0 && col[j] < n_cols, RAMD_E_ARG,
+                       "ramd_csr_create: column %d of entry %lld outside [0,%d)", col[j], (long long)j, n_cols);
+        ctx->set_current();
+        int* d_rp = dalloc<int>((size_t)n_rows + 1);
+        int* d_col = dalloc<int>((size_t)nnz);
+        double* d_val = dalloc<double>((size_t)nnz);
+        RAMD_HIP(hipMemcpyAsync(d_rp, rowptr, sizeof(int) * (n_rows + 1), hipMemcpyHostToDevice, ctx->stream));
+        if (nnz) {
+            RAMD_HIP(hipMemcpyAsync(d_col, col, sizeof(int) * nnz, hipMemcpyHostToDevice, ctx->stream));
+            RAMD_HIP(hipMemcpyAsync(d_val, val, sizeof(double) * nnz, hipMemcpyHostToDevice, ctx->stream));
+        }
+        *out = csr_wrap(ctx, n_rows, n_cols, nnz, d_rp, d_col, d_val, rowptr);
+    });
+}
+
+int ramd_csr_info(ramd_csr A, int32_t* n_rows, int32_t* n_cols, int64_t* nnz)
+{
+    return guarded([&] {
+        RAMD_CHECK(A, RAMD_E_ARG, "ramd_csr_info: NULL handle");
+        if (n_rows) *n_rows = A->n_rows;
+        if (n_cols) *n_cols = A->n_cols;
+        if (nnz) *nnz = A->nnz;
+    });
+}
+
+int ramd_csr_download(ramd_csr A, int32_t* rowptr, int32_t* col, double* val)
+{
+    return guarded([&] {
+        RAMD_CHECK(A, RAMD_E_ARG, "ramd_csr_download: NULL handle");
+        hipStream_t s = A->ctx->stream;
+        if (rowptr)
+            RAMD_HIP(hipMemcpyAsync(rowptr, A->rowptr, sizeof(int) * (A->n_rows + 1), hipMemcpyDeviceToHost, s));
+        if (col && A->nnz) RAMD_HIP(hipMemcpyAsync(col, A->col, sizeof(int) * A->nnz, hipMemcpyDeviceToHost, s));
+        if (val && A->nnz) RAMD_HIP(hipMemcpyAsync(val, A->val, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, s));
+        RAMD_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int ramd_csr_destroy(ramd_csr A)
+{
+    return guarded([&] {
+        if (!A) return;
+        RAMD_CHECK(!A->borrowed, RAMD_E_STATE, "ramd_csr_destroy: handle is owned by a hierarchy");
+        (void)hipStreamSynchronize(A->ctx->stream);
+        csr_free(A);
+    });
+}
+
+}  // extern "C"

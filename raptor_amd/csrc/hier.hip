@@ -1,0 +1,836 @@
+// Device-resident AMG hierarchy: setup driver (host strength / splitting /
+// interpolation, device Galerkin products), the V-cycle in FP64 and FP32,
+// the coarse dense LU solve, stationary AMG, PCG with the reference's rules
+// and the mixed-precision flexible PCG.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "common.hpp"
+#include "host/setup.hpp"
+
+namespace ramd {
+
+ramd_csr_s* spgemm(ramd_csr_s* A, ramd_csr_s* B);
+ramd_csr_s* galerkin(ramd_csr_s* A, ramd_csr_s* P);
+
+struct Level {
+    ramd_csr_s* A = nullptr;  // diagonal first
+    ramd_csr_s* P = nullptr;  // n_l x n_{l+1}
+    ramd_csr_s* R = nullptr;  // P^T (owned by P)
+    double* diag = nullptr;   // a_ii (0 where absent)
+    double *x = nullptr, *b = nullptr, *t = nullptr, *r = nullptr;
+    float* diag32 = nullptr;
+    float *x32 = nullptr, *b32 = nullptr, *t32 = nullptr, *r32 = nullptr;
+    std::vector<int> states;
+    int n = 0;
+};
+
+}  // namespace ramd
+
+struct ramd_hier_s {
+    ramd_ctx ctx = nullptr;
+    ramd_amg_opts opts{};
+    std::vector<ramd::Level> levels;
+    int coarse_n = 0;
+    double* lu = nullptr;  // column-major LU of the coarse matrix read column-major (A_c^T)
+    int* piv = nullptr;
+    double* cb = nullptr;  // FP64 staging for the FP32 cycle's coarse solve
+    double* cx = nullptr;
+    double times[5] = {0, 0, 0, 0, 0};
+    bool fp32 = false;
+    // PCG work vectors (fine level)
+    double *pr = nullptr, *pz = nullptr, *pp = nullptr, *pAp = nullptr, *pzo = nullptr;
+    float *pr32 = nullptr, *pz32 = nullptr;
+};
+
+namespace ramd {
+
+using host::Csr;
+
+static double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static ramd_csr_s* upload(ramd_ctx ctx, const Csr& M)
+{
+    int* rp = dalloc<int>((size_t)M.n_rows + 1);
+    int* c = dalloc<int>((size_t)M.nnz());
+    double* v = dalloc<double>((size_t)M.nnz());
+    RAMD_HIP(hipMemcpyAsync(rp, M.rowptr.data(), sizeof(int) * (M.n_rows + 1), hipMemcpyHostToDevice, ctx->stream));
+    if (M.nnz()) {
+        RAMD_HIP(hipMemcpyAsync(c, M.col.data(), sizeof(int) * M.nnz(), hipMemcpyHostToDevice, ctx->stream));
+        RAMD_HIP(hipMemcpyAsync(v, M.val.data(), sizeof(double) * M.nnz(), hipMemcpyHostToDevice, ctx->stream));
+    }
+    return csr_wrap(ctx, M.n_rows, M.n_cols, M.nnz(), rp, c, v, M.rowptr.data());
+}
+
+static Csr download(ramd_csr_s* A)
+{
+    Csr M;
+    M.n_rows = A->n_rows;
+    M.n_cols = A->n_cols;
+    M.rowptr = A->h_rowptr;
+    M.col.resize((size_t)A->nnz);
+    M.val.resize((size_t)A->nnz);
+    if (A->nnz) {
+        RAMD_HIP(hipMemcpyAsync(M.col.data(), A->col, sizeof(int) * A->nnz, hipMemcpyDeviceToHost, A->ctx->stream));
+        RAMD_HIP(hipMemcpyAsync(M.val.data(), A->val, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, A->ctx->stream));
+    }
+    RAMD_HIP(hipStreamSynchronize(A->ctx->stream));
+    return M;
+}
+
+// Dense LU with partial pivoting of the row-major coarse matrix read in
+// column-major order, i.e. LAPACK dgetrf_ applied as
+// multilevel/par_multilevel.hpp:295-331 does.  Returns LAPACK's info.
+static int lu_factor_colmajor(int n, std::vector<double>& a, std::vector<int>& piv)
+{
+    int info = 0;
+    piv.assign((size_t)n, 0);
+    for (int j = 0; j < n; j++) {
+        int p = j;
+        double amax = std::fabs(a[j + (size_t)j * n]);
+        for (int i = j + 1; i < n; i++) {
+            const double t = std::fabs(a[i + (size_t)j * n]);
+            if (t > amax) {
+                amax = t;
+                p = i;
+            }
+        }
+        piv[j] = p;
+        if (a[p + (size_t)j * n] != 0.0) {
+            if (p != j)
+                for (int k = 0; k < n; k++) std::swap(a[j + (size_t)k * n], a[p + (size_t)k * n]);
+            const double r = 1.0 / a[j + (size_t)j * n];
+            for (int i = j + 1; i < n; i++) a[i + (size_t)j * n] *= r;
+        } else if (info == 0) {
+            info = j + 1;
+        }
+        for (int k = j + 1; k < n; k++) {
+            const double u = a[j + (size_t)k * n];
+            for (int i = j + 1; i < n; i++) a[i + (size_t)k * n] -= a[i + (size_t)j * n] * u;
+        }
+    }
+    return info;
+}
+
+constexpr int kCoarseMax = 8192;
+
+// dgetrs_('N') on one right-hand side by one workgroup: row swaps, unit-lower
+// then upper substitution in column order (the reference BLAS dtrsm order).
+__global__ __launch_bounds__(256) void coarse_solve_kernel(int n, const double* __restrict__ lu,
+                                                           const int* __restrict__ piv, const double* __restrict__ b,
+                                                           double* __restrict__ x)
+{
+    extern __shared__ double sb[];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) sb[i] = b[i];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int j = 0; j < n; j++) {
+            const int p = piv[j];
+            if (p != j) {
+                const double t = sb[j];
+                sb[j] = sb[p];
+                sb[p] = t;
+            }
+        }
+    __syncthreads();
+    for (int j = 0; j < n; j++) {
+        const double bj = sb[j];
+        for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) sb[i] -= bj * lu[i + (size_t)j * n];
+        __syncthreads();
+    }
+    for (int j = n - 1; j >= 0; j--) {
+        if (threadIdx.x == 0) sb[j] /= lu[j + (size_t)j * n];
+        __syncthreads();
+        const double bj = sb[j];
+        for (int i = threadIdx.x; i < j; i += blockDim.x) sb[i] -= bj * lu[i + (size_t)j * n];
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = sb[i];
+}
+
+__global__ void extract_diag(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                             const double* __restrict__ val, double* __restrict__ d)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = 0.0;
+    for (int j = rp[i]; j < rp[i + 1]; j++)
+        if (col[j] == i) {
+            v = val[j];
+            break;
+        }
+    d[i] = v;
+}
+
+// First Jacobi sweep from x == 0 (relax.cpp:40 with tmp = 0 and the
+// off-diagonal sum of zeros equal to +0): x = (1-w)*0 + w*((b - 0)/d).
+// Rows without a usable diagonal (|d| <= 1e-16, or empty rows) keep x = 0.
+template <typename X>
+__global__ void jacobi_from_zero(int n, const X* __restrict__ d, const X* __restrict__ b, X* __restrict__ x, X omega,
+                                 const int* __restrict__ rp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const X di = d[i];
+    const X zero = X(0.0);
+    X v = zero;
+    if (rp[i + 1] > rp[i] && fabs(di) > X(1e-16)) v = ((X(1.0) - omega) * zero) + (omega * ((b[i] - zero) / di));
+    x[i] = v;
+}
+
+template <typename X>
+__global__ void add_into(int n, const X* __restrict__ xin, const X* __restrict__ inc, X* __restrict__ y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = xin[i] + inc[i];
+}
+
+__global__ void to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b)
+{
+    const int64_t s = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += s) b[i] = (float)a[i];
+}
+
+__global__ void to_f64(int64_t n, const float* __restrict__ a, double* __restrict__ b)
+{
+    const int64_t s = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += s) b[i] = (double)a[i];
+}
+
+static dim3 grid1(int64_t n, int bs = 256)
+{
+    return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + bs - 1) / bs, 1 << 20)));
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle
+// ---------------------------------------------------------------------------
+static void coarse_solve64(ramd_hier_s* h, const double* b, double* x)
+{
+    const int n = h->coarse_n;
+    hipLaunchKernelGGL(coarse_solve_kernel, dim3(1), dim3(256), sizeof(double) * n, h->ctx->stream, n, h->lu, h->piv,
+                       b, x);
+    RAMD_HIP(hipGetLastError());
+}
+
+// The off-diagonal Jacobi kernel reads `cur` and writes the other buffer.
+static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_zero)
+{
+    Level& L = h->levels[l];
+    hipStream_t s = h->ctx->stream;
+    if (l == (int)h->levels.size() - 1) {
+        coarse_solve64(h, b, x);
+        return;
+    }
+    const double w = h->opts.relax_weight;
+    const int sweeps = h->opts.num_smooth_sweeps;
+    double* cur = x;
+    double* other = L.t;
+    for (int it = 0; it < sweeps; it++) {
+        if (it == 0 && x_zero) {
+            hipLaunchKernelGGL((jacobi_from_zero<double>), grid1(L.n), dim3(256), 0, s, L.n, L.diag, b, cur, w,
+                               L.A->rowptr);
+            RAMD_HIP(hipGetLastError());
+        } else {
+            launch_stream_op(L.A, OP_JACOBI, cur, b, other, w, s);
+            std::swap(cur, other);
+        }
+    }
+    launch_stream_op(L.A, OP_RESIDUAL, cur, b, L.r, 0.0, s);
+    Level& C = h->levels[l + 1];
+    launch_stream_op(L.R, OP_SPMV, L.r, nullptr, C.b, 0.0, s);
+    cycle64(h, l + 1, C.x, C.b, true);
+    // x <- x + P e (CSR_append: row sum first, then added), into the other buffer
+    launch_stream_op(L.P, OP_SPMV, C.x, nullptr, L.r, 0.0, s);
+    hipLaunchKernelGGL((add_into<double>), grid1(L.n), dim3(256), 0, s, L.n, cur, L.r, other);
+    RAMD_HIP(hipGetLastError());
+    std::swap(cur, other);
+    for (int it = 0; it < sweeps; it++) {
+        launch_stream_op(L.A, OP_JACOBI, cur, b, other, w, s);
+        std::swap(cur, other);
+    }
+    if (cur != x) RAMD_HIP(hipMemcpyAsync(x, cur, sizeof(double) * L.n, hipMemcpyDeviceToDevice, s));
+}
+
+static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero)
+{
+    Level& L = h->levels[l];
+    hipStream_t s = h->ctx->stream;
+    if (l == (int)h->levels.size() - 1) {
+        const int n = h->coarse_n;
+        hipLaunchKernelGGL(to_f64, grid1(n), dim3(256), 0, s, (int64_t)n, b, h->cb);
+        coarse_solve64(h, h->cb, h->cx);
+        hipLaunchKernelGGL(to_f32, grid1(n), dim3(256), 0, s, (int64_t)n, h->cx, x);
+        RAMD_HIP(hipGetLastError());
+        return;
+    }
+    const float w = (float)h->opts.relax_weight;
+    const int sweeps = h->opts.num_smooth_sweeps;
+    float* cur = x;
+    float* other = L.t32;
+    for (int it = 0; it < sweeps; it++) {
+        if (it == 0 && x_zero) {
+            hipLaunchKernelGGL((jacobi_from_zero<float>), grid1(L.n), dim3(256), 0, s, L.n, L.diag32, b, cur, w,
+                               L.A->rowptr);
+            RAMD_HIP(hipGetLastError());
+        } else {
+            launch_stream_op32(L.A, OP_JACOBI, cur, b, other, w, s);
+            std::swap(cur, other);
+        }
+    }
+    launch_stream_op32(L.A, OP_RESIDUAL, cur, b, L.r32, 0.f, s);
+    Level& C = h->levels[l + 1];
+    launch_stream_op32(L.R, OP_SPMV, L.r32, nullptr, C.b32, 0.f, s);
+    cycle32(h, l + 1, C.x32, C.b32, true);
+    launch_stream_op32(L.P, OP_SPMV, C.x32, nullptr, L.r32, 0.f, s);
+    hipLaunchKernelGGL((add_into<float>), grid1(L.n), dim3(256), 0, s, L.n, cur, L.r32, other);
+    RAMD_HIP(hipGetLastError());
+    std::swap(cur, other);
+    for (int it = 0; it < sweeps; it++) {
+        launch_stream_op32(L.A, OP_JACOBI, cur, b, other, w, s);
+        std::swap(cur, other);
+    }
+    if (cur != x) RAMD_HIP(hipMemcpyAsync(x, cur, sizeof(float) * L.n, hipMemcpyDeviceToDevice, s));
+}
+
+// ---------------------------------------------------------------------------
+// setup
+// ---------------------------------------------------------------------------
+static void alloc_level_vectors(ramd_hier_s* h, Level& L)
+{
+    L.n = L.A->n_rows;
+    L.x = dalloc<double>((size_t)L.n);
+    L.b = dalloc<double>((size_t)L.n);
+    L.t = dalloc<double>((size_t)L.n);
+    L.r = dalloc<double>((size_t)L.n);
+    L.diag = dalloc<double>((size_t)L.n);
+    if (L.n) {
+        hipLaunchKernelGGL(extract_diag, grid1(L.n), dim3(256), 0, h->ctx->stream, L.n, L.A->rowptr, L.A->col,
+                           L.A->val, L.diag);
+        RAMD_HIP(hipGetLastError());
+    }
+}
+
+static void free_level(Level& L)
+{
+    if (L.A) csr_free(L.A);
+    if (L.P) csr_free(L.P);  // frees R (its transpose)
+    for (void* p : {(void*)L.diag, (void*)L.x, (void*)L.b, (void*)L.t, (void*)L.r, (void*)L.diag32, (void*)L.x32,
+                    (void*)L.b32, (void*)L.t32, (void*)L.r32})
+        dfree(p);
+    L = Level();
+}
+
+static void hier_free(ramd_hier_s* h)
+{
+    if (!h) return;
+    if (h->ctx) (void)hipStreamSynchronize(h->ctx->stream);
+    for (Level& L : h->levels) free_level(L);
+    for (void* p : {(void*)h->lu, (void*)h->piv, (void*)h->cb, (void*)h->cx, (void*)h->pr, (void*)h->pz, (void*)h->pp,
+                    (void*)h->pAp, (void*)h->pzo, (void*)h->pr32, (void*)h->pz32})
+        dfree(p);
+    delete h;
+}
+
+// ParRugeStubenSolver::extend_hierarchy, ruge_stuben/par_ruge_stuben_solver.hpp:57-162
+static Csr rs_level(ramd_hier_s* h, Csr& A, int level, const double* weights, std::vector<int>& states)
+{
+    const ramd_amg_opts& o = h->opts;
+    double t0 = now_s();
+    Csr S = o.strength_type == RAMD_SYMMETRIC ? host::symmetric_strength(A, o.strong_threshold)
+                                              : host::classical_strength(A, o.strong_threshold);
+    double t1 = now_s();
+    switch (o.coarsen_type) {
+    case RAMD_CLJP: host::split_cljp(S, weights, states); break;
+    case RAMD_PMIS: host::split_pmis(S, weights, states); break;
+    default: fail(RAMD_E_ARG, "coarsen_type %d is not supported by this build", o.coarsen_type);
+    }
+    double t2 = now_s();
+    Csr P;
+    switch (o.interp_type) {
+    case RAMD_DIRECT: P = host::direct_interpolation(A, S, states); break;
+    case RAMD_MOD_CLASSICAL: P = host::mod_classical_interpolation(A, S, states); break;
+    case RAMD_EXTENDED: P = host::extended_interpolation(A, S, states, o.interp_filter); break;
+    default: fail(RAMD_E_ARG, "interp_type %d is not supported", o.interp_type);
+    }
+    double t3 = now_s();
+    h->times[0] += t1 - t0;
+    h->times[1] += t2 - t1;
+    h->times[2] += t3 - t2;
+    (void)level;
+    return P;
+}
+
+static void hier_setup(ramd_hier_s* h, Csr Af)
+{
+    const ramd_amg_opts& o = h->opts;
+    ramd_ctx ctx = h->ctx;
+    double tstart = now_s();
+    // levels[0]->A = Af->copy(); sort(); move_diag()  (par_multilevel.hpp:141-143)
+    host::sort_rows(Af);
+    host::move_diag(Af);
+    std::vector<double> weights;
+    if (o.weights) weights.assign(o.weights, o.weights + Af.n_rows);
+    else weights = host::reference_weights(Af.n_rows, 0);
+
+    h->levels.emplace_back();
+    h->levels[0].A = upload(ctx, Af);
+    alloc_level_vectors(h, h->levels[0]);
+    Csr A = std::move(Af);
+    while (A.n_rows > o.max_coarse && (o.max_levels == -1 || (int)h->levels.size() < o.max_levels)) {
+        const int l = (int)h->levels.size() - 1;
+        Level& L = h->levels[l];
+        Csr P = rs_level(h, A, l, weights.data(), L.states);
+        if (P.n_cols == 0) fail(RAMD_E_NUMERIC, "level %d produced no coarse points", l);
+        double tg = now_s();
+        L.P = upload(ctx, P);
+        L.R = csr_transpose(L.P);
+        ramd_csr_s* Ac = galerkin(L.A, L.P);
+        h->times[3] += now_s() - tg;
+        h->levels.emplace_back();
+        h->levels.back().A = Ac;
+        alloc_level_vectors(h, h->levels.back());
+        A = download(Ac);
+    }
+    // duplicate_coarse (par_multilevel.hpp:229-339): dense row-major A_c, dgetrf_
+    const int n = A.n_rows;
+    RAMD_CHECK(n <= kCoarseMax, RAMD_E_ARG, "coarsest level has %d rows; the dense solve supports <= %d", n,
+               kCoarseMax);
+    std::vector<double> dense((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int j = A.rowptr[i]; j < A.rowptr[i + 1]; j++) dense[(size_t)i * n + A.col[j]] = A.val[j];
+    std::vector<int> piv;
+    const int info = lu_factor_colmajor(n, dense, piv);
+    RAMD_CHECK(info == 0, RAMD_E_NUMERIC, "coarse matrix (%d rows) is singular (info=%d)", n, info);
+    h->coarse_n = n;
+    h->lu = dalloc<double>((size_t)n * n);
+    h->piv = dalloc<int>((size_t)n);
+    h->cb = dalloc<double>((size_t)n);
+    h->cx = dalloc<double>((size_t)n);
+    if (n) {
+        RAMD_HIP(hipMemcpyAsync(h->lu, dense.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
+        RAMD_HIP(hipMemcpyAsync(h->piv, piv.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+    }
+    const int n0 = h->levels[0].n;
+    h->pr = dalloc<double>((size_t)n0);
+    h->pz = dalloc<double>((size_t)n0);
+    h->pp = dalloc<double>((size_t)n0);
+    h->pAp = dalloc<double>((size_t)n0);
+    h->pzo = dalloc<double>((size_t)n0);
+    RAMD_HIP(hipStreamSynchronize(ctx->stream));
+    for (Level& L : h->levels) L.A->borrowed = true;
+    for (Level& L : h->levels)
+        if (L.P) L.P->borrowed = true;
+    double total = now_s() - tstart;
+    h->times[4] = total - h->times[0] - h->times[1] - h->times[2] - h->times[3];
+}
+
+static void enable_fp32(ramd_hier_s* h)
+{
+    if (h->fp32) return;
+    hipStream_t s = h->ctx->stream;
+    for (Level& L : h->levels) {
+        csr_make_fp32(L.A);
+        if (L.P) {
+            csr_make_fp32(L.P);
+            csr_make_fp32(L.R);
+        }
+        L.x32 = dalloc<float>((size_t)L.n);
+        L.b32 = dalloc<float>((size_t)L.n);
+        L.t32 = dalloc<float>((size_t)L.n);
+        L.r32 = dalloc<float>((size_t)L.n);
+        L.diag32 = dalloc<float>((size_t)L.n);
+        if (L.n) hipLaunchKernelGGL(to_f32, grid1(L.n), dim3(256), 0, s, (int64_t)L.n, L.diag, L.diag32);
+        RAMD_HIP(hipGetLastError());
+    }
+    const int n0 = h->levels[0].n;
+    h->pr32 = dalloc<float>((size_t)n0);
+    h->pz32 = dalloc<float>((size_t)n0);
+    RAMD_HIP(hipStreamSynchronize(s));
+    h->fp32 = true;
+}
+
+// ---------------------------------------------------------------------------
+// PCG vector kernels (reference operand order, one rounding per op)
+// ---------------------------------------------------------------------------
+__global__ void pcg_update_xr(int n, double alpha, const double* __restrict__ p, const double* __restrict__ Ap,
+                              double* __restrict__ x, double* __restrict__ r, int update_r)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] += p[i] * alpha;
+    if (update_r) r[i] += Ap[i] * (-1.0 * alpha);
+}
+
+__global__ void pcg_update_p(int n, double beta, const double* __restrict__ z, double* __restrict__ p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double t = p[i] * beta;
+    p[i] = t + z[i] * 1.0;
+}
+
+static double dot64(ramd_ctx ctx, int n, const double* a, const double* b)
+{
+    launch_dot(ctx, n, a, b, ctx->red_result);
+    return fetch_scalar(ctx, ctx->red_result);
+}
+
+static void vcycle_zero(ramd_hier_s* h, double* z, const double* r) { cycle64(h, 0, z, r, true); }
+
+// z = M^{-1} r with the V-cycle in the requested precision (r, z FP64).
+static void precondition(ramd_hier_s* h, int prec, const double* r, double* z)
+{
+    hipStream_t s = h->ctx->stream;
+    const int n = h->levels[0].n;
+    if (prec == RAMD_PREC_FP64) {
+        cycle64(h, 0, z, r, true);
+        return;
+    }
+    hipLaunchKernelGGL(to_f32, grid1(n), dim3(256), 0, s, (int64_t)n, r, h->pr32);
+    cycle32(h, 0, h->pz32, h->pr32, true);
+    hipLaunchKernelGGL(to_f64, grid1(n), dim3(256), 0, s, (int64_t)n, h->pz32, z);
+    RAMD_HIP(hipGetLastError());
+}
+
+// x += alpha p; r -= alpha Ap; keep the previous r for the flexible beta
+__global__ void fpcg_update(int n, double alpha, const double* __restrict__ p, const double* __restrict__ Ap,
+                            double* __restrict__ x, double* __restrict__ r, double* __restrict__ r_old)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] += alpha * p[i];
+    const double ri = r[i];
+    r_old[i] = ri;
+    r[i] = ri - alpha * Ap[i];
+}
+
+__global__ void fpcg_dir(int n, double beta, const double* __restrict__ z, double* __restrict__ p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+// plain 2-norm (no small-entry skipping, NaN propagates)
+static double norm2_64(ramd_ctx ctx, int n, const double* a) { return std::sqrt(dot64(ctx, n, a, a)); }
+
+}  // namespace ramd
+
+using namespace ramd;
+
+extern "C" {
+
+int ramd_amg_opts_default(int solver, ramd_amg_opts* o)
+{
+    return guarded([&] {
+        RAMD_CHECK(o, RAMD_E_ARG, "ramd_amg_opts_default: NULL opts");
+        RAMD_CHECK(solver == RAMD_SOLVER_RS || solver == RAMD_SOLVER_SA, RAMD_E_ARG, "unknown solver %d", solver);
+        std::memset(o, 0, sizeof *o);
+        o->solver = solver;
+        if (solver == RAMD_SOLVER_RS) {
+            // ParRugeStubenSolver(0.0, RS, Direct, Classical, SOR) defaults, relax set to Jacobi
+            o->strong_threshold = 0.0;
+            o->coarsen_type = RAMD_RS;
+            o->interp_type = RAMD_DIRECT;
+            o->strength_type = RAMD_CLASSICAL;
+        } else {
+            o->strong_threshold = 0.0;
+            o->strength_type = RAMD_SYMMETRIC;
+        }
+        o->interp_filter = 0.3;
+        o->prolong_weight = 4.0 / 3;
+        o->num_smooth_sweeps = 1;
+        o->relax_weight = 1.0;
+        o->max_coarse = 50;
+        o->max_levels = 25;
+        o->weights = nullptr;
+    });
+}
+
+int ramd_hier_setup(ramd_ctx ctx, int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t* col,
+                    const double* val, const ramd_amg_opts* opts, ramd_hier* out)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx && out && opts && rowptr && (nnz == 0 || (col && val)), RAMD_E_ARG,
+                   "ramd_hier_setup: NULL argument");
+        RAMD_CHECK(n > 0 && nnz >= 0 && rowptr[0] == 0 && rowptr[n] == nnz, RAMD_E_ARG,
+                   "ramd_hier_setup: inconsistent CSR (n=%d nnz=%lld)", n, (long long)nnz);
+        RAMD_CHECK(opts->solver == RAMD_SOLVER_RS, RAMD_E_ARG, "ramd_hier_setup: solver %d not supported",
+                   opts->solver);
+        RAMD_CHECK(opts->num_smooth_sweeps >= 0 && opts->max_coarse >= 1, RAMD_E_ARG, "ramd_hier_setup: bad options");
+        for (int64_t j = 0; j < nnz; j++)
+            RAMD_CHECK(col[j] >= 0 && col[j] < n, RAMD_E_ARG, "ramd_hier_setup: column %d out of range", col[j]);
+        ctx->set_current();
+        Csr A;
+        A.n_rows = A.n_cols = n;
+        A.rowptr.assign(rowptr, rowptr + n + 1);
+        A.col.assign(col, col + nnz);
+        A.val.assign(val, val + nnz);
+        std::unique_ptr<ramd_hier_s, void (*)(ramd_hier_s*)> h(new ramd_hier_s(), hier_free);
+        h->ctx = ctx;
+        h->opts = *opts;
+        h->opts.weights = nullptr;
+        if (opts->weights) {
+            h->opts.weights = opts->weights;
+        }
+        hier_setup(h.get(), std::move(A));
+        h->opts.weights = nullptr;
+        *out = h.release();
+    });
+}
+
+int ramd_hier_destroy(ramd_hier h)
+{
+    return guarded([&] { hier_free(h); });
+}
+
+int ramd_hier_info(ramd_hier h, int* num_levels, int32_t* n_rows, int64_t* nnz_A, int64_t* nnz_P)
+{
+    return guarded([&] {
+        RAMD_CHECK(h, RAMD_E_ARG, "ramd_hier_info: NULL handle");
+        const int L = (int)h->levels.size();
+        if (num_levels) *num_levels = L;
+        for (int l = 0; l < L; l++) {
+            if (n_rows) n_rows[l] = h->levels[l].n;
+            if (nnz_A) nnz_A[l] = h->levels[l].A->nnz;
+            if (nnz_P) nnz_P[l] = h->levels[l].P ? h->levels[l].P->nnz : 0;
+        }
+    });
+}
+
+int ramd_hier_level(ramd_hier h, int level, ramd_csr* A, ramd_csr* P)
+{
+    return guarded([&] {
+        RAMD_CHECK(h && level >= 0 && level < (int)h->levels.size(), RAMD_E_ARG, "ramd_hier_level: bad level %d",
+                   level);
+        if (A) *A = h->levels[level].A;
+        if (P) *P = h->levels[level].P;
+    });
+}
+
+int ramd_hier_states(ramd_hier h, int level, int32_t* states)
+{
+    return guarded([&] {
+        RAMD_CHECK(h && states && level >= 0 && level < (int)h->levels.size(), RAMD_E_ARG,
+                   "ramd_hier_states: bad argument");
+        const std::vector<int>& s = h->levels[level].states;
+        RAMD_CHECK(!s.empty() || h->levels[level].n == 0, RAMD_E_STATE, "level %d has no splitting (coarsest)", level);
+        std::copy(s.begin(), s.end(), states);
+    });
+}
+
+int ramd_hier_setup_times(ramd_hier h, double* t)
+{
+    return guarded([&] {
+        RAMD_CHECK(h && t, RAMD_E_ARG, "ramd_hier_setup_times: NULL argument");
+        for (int i = 0; i < 5; i++) t[i] = h->times[i];
+    });
+}
+
+int ramd_hier_enable_fp32(ramd_hier h)
+{
+    return guarded([&] {
+        RAMD_CHECK(h, RAMD_E_ARG, "ramd_hier_enable_fp32: NULL handle");
+        h->ctx->set_current();
+        enable_fp32(h);
+    });
+}
+
+int ramd_vcycle(ramd_hier h, double* x_d, const double* b_d, int x_is_zero)
+{
+    return guarded([&] {
+        RAMD_CHECK(h && x_d && b_d, RAMD_E_ARG, "ramd_vcycle: NULL argument");
+        h->ctx->set_current();
+        cycle64(h, 0, x_d, b_d, x_is_zero != 0);
+    });
+}
+
+int ramd_vcycle_fp32(ramd_hier h, float* x_d, const float* b_d)
+{
+    return guarded([&] {
+        RAMD_CHECK(h && x_d && b_d, RAMD_E_ARG, "ramd_vcycle_fp32: NULL argument");
+        h->ctx->set_current();
+        enable_fp32(h);
+        cycle32(h, 0, x_d, b_d, true);
+    });
+}
+
+// ParMultilevel::solve multilevel/par_multilevel.hpp:459-538
+int ramd_amg_solve(ramd_hier h, double* x_d, const double* b_d, double tol, int max_iter, double* residuals,
+                   int32_t* iters)
+{
+    return guarded([&] {
+        RAMD_CHECK(h && x_d && b_d && iters, RAMD_E_ARG, "ramd_amg_solve: NULL argument");
+        ramd_ctx ctx = h->ctx;
+        ctx->set_current();
+        Level& L0 = h->levels[0];
+        auto norm = [&](const double* v) {
+            launch_norm2sq(ctx, L0.n, v, ctx->red_result);
+            double r = fetch_scalar(ctx, ctx->red_result);
+            r = pow(r, 1.0 / 2);
+            r = pow(r, 2);
+            return pow(r, 1. / 2);
+        };
+        const double b_norm = norm(b_d);
+        launch_stream_op(L0.A, OP_RESIDUAL, x_d, b_d, h->pr, 0.0, ctx->stream);
+        double r_norm = std::fabs(b_norm) > 1e-16 ? norm(h->pr) / b_norm : norm(h->pr);
+        int iter = 0;
+        if (residuals) residuals[0] = r_norm;
+        while (r_norm > tol && iter < max_iter) {
+            cycle64(h, 0, x_d, b_d, false);
+            iter++;
+            launch_stream_op(L0.A, OP_RESIDUAL, x_d, b_d, h->pr, 0.0, ctx->stream);
+            r_norm = std::fabs(b_norm) > 1e-16 ? norm(h->pr) / b_norm : norm(h->pr);
+            if (residuals) residuals[iter] = r_norm;
+        }
+        *iters = iter;
+    });
+}
+
+// PCG krylov/par_cg.cpp:121-263
+int ramd_pcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double tol, int max_iter, double* res,
+             int32_t max_res, int32_t* n_res, int32_t* iters)
+{
+    int rc = RAMD_OK;
+    int st = guarded([&] {
+        RAMD_CHECK(h && A && x_d && b_d && n_res && iters && (res || max_res == 0), RAMD_E_ARG,
+                   "ramd_pcg: NULL argument");
+        ramd_ctx ctx = h->ctx;
+        ctx->set_current();
+        hipStream_t s = ctx->stream;
+        const int n = A->n_rows;
+        RAMD_CHECK(n == h->levels[0].n, RAMD_E_ARG, "ramd_pcg: A has %d rows, hierarchy %d", n, h->levels[0].n);
+        double *r = h->pr, *z = h->pz, *p = h->pp, *Ap = h->pAp;
+        int nres = 0;
+        auto push = [&](double v) {
+            if (nres < max_res) res[nres] = v;
+            nres++;
+        };
+        if (max_iter <= 0) max_iter = ((int)(1.3 * n)) + 2;
+        vcycle_zero(h, z, b_d);
+        const double b_inner = dot64(ctx, n, b_d, z);
+        const double norm_b = sqrt(b_inner);
+        if (norm_b > 1e-16) tol = tol * norm_b;
+        launch_stream_op(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
+        vcycle_zero(h, z, r);
+        RAMD_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        double rz = dot64(ctx, n, r, z);
+        push(sqrt(rz));
+        int iter = 0;
+        const int recompute_r = 8;
+        while (iter < max_iter) {
+            iter++;
+            launch_stream_op(A, OP_SPMV, p, nullptr, Ap, 0.0, s);
+            const double App = dot64(ctx, n, Ap, p);
+            if (App < 0.0) {
+                rc = RAMD_E_INDEFINITE;
+                break;
+            }
+            const double alpha = rz / App;
+            const bool full_r = recompute_r && iter % recompute_r == 0;
+            hipLaunchKernelGGL(pcg_update_xr, grid1(n), dim3(256), 0, s, n, alpha, p, Ap, x_d, r, full_r ? 0 : 1);
+            RAMD_HIP(hipGetLastError());
+            if (full_r) launch_stream_op(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
+            vcycle_zero(h, z, r);
+            const double next = dot64(ctx, n, r, z);
+            const double beta = next / rz;
+            push(next / b_inner);
+            if (next < tol) break;
+            if (full_r) {
+                RAMD_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            } else {
+                hipLaunchKernelGGL(pcg_update_p, grid1(n), dim3(256), 0, s, n, beta, z, p);
+                RAMD_HIP(hipGetLastError());
+            }
+            rz = next;
+        }
+        RAMD_HIP(hipStreamSynchronize(s));
+        *n_res = nres;
+        *iters = iter;
+        if (rc == RAMD_OK && iter == max_iter) rc = RAMD_OK;  // reference prints "Max Iterations Reached"
+    });
+    if (st != RAMD_OK) return st;
+    if (rc == RAMD_E_INDEFINITE) set_error("ramd_pcg: indefinite matrix detected (p.Ap < 0)");
+    return rc;
+}
+
+// Flexible PCG (Polak-Ribiere beta = z_{k+1}.(r_{k+1} - r_k) / z_k.r_k) in FP64
+// with a V-cycle preconditioner in `cycle_prec`; stops on the true FP64
+// relative residual ||b - A x|| / ||b|| <= rtol.
+int ramd_fpcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double rtol, int max_iter, int cycle_prec,
+              double* relres_hist, int32_t* iters, double* final_relres)
+{
+    int rc = RAMD_OK;
+    int st = guarded([&] {
+        RAMD_CHECK(h && A && x_d && b_d && iters && final_relres, RAMD_E_ARG, "ramd_fpcg: NULL argument");
+        RAMD_CHECK(cycle_prec == RAMD_PREC_FP64 || cycle_prec == RAMD_PREC_FP32, RAMD_E_ARG,
+                   "ramd_fpcg: unknown precision %d", cycle_prec);
+        ramd_ctx ctx = h->ctx;
+        ctx->set_current();
+        hipStream_t s = ctx->stream;
+        const int n = A->n_rows;
+        RAMD_CHECK(n == h->levels[0].n, RAMD_E_ARG, "ramd_fpcg: A has %d rows, hierarchy %d", n, h->levels[0].n);
+        if (cycle_prec == RAMD_PREC_FP32) enable_fp32(h);
+        double *r = h->pr, *z = h->pz, *p = h->pp, *Ap = h->pAp, *r_old = h->pzo;
+        const double bnorm = norm2_64(ctx, n, b_d);
+        const double scale = bnorm > 0 ? bnorm : 1.0;
+        launch_stream_op(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
+        double relres = norm2_64(ctx, n, r) / scale;
+        if (relres_hist) relres_hist[0] = relres;
+        int it = 0;
+        if (relres > rtol) {
+            precondition(h, cycle_prec, r, z);
+            RAMD_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            double rz = dot64(ctx, n, r, z);
+            while (it < max_iter) {
+                launch_stream_op(A, OP_SPMV, p, nullptr, Ap, 0.0, s);
+                const double pAp = dot64(ctx, n, p, Ap);
+                if (!(pAp > 0.0)) {
+                    rc = std::isfinite(pAp) ? RAMD_E_INDEFINITE : RAMD_E_NUMERIC;
+                    break;
+                }
+                const double alpha = rz / pAp;
+                hipLaunchKernelGGL(fpcg_update, grid1(n), dim3(256), 0, s, n, alpha, p, Ap, x_d, r, r_old);
+                RAMD_HIP(hipGetLastError());
+                it++;
+                relres = norm2_64(ctx, n, r) / scale;
+                if (relres <= rtol) {
+                    // confirm with the true residual; continue from it if the recurrence drifted
+                    launch_stream_op(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
+                    relres = norm2_64(ctx, n, r) / scale;
+                    if (relres_hist) relres_hist[it] = relres;
+                    if (relres <= rtol) break;
+                } else if (relres_hist) {
+                    relres_hist[it] = relres;
+                }
+                if (!std::isfinite(relres)) {
+                    rc = RAMD_E_NUMERIC;
+                    break;
+                }
+                precondition(h, cycle_prec, r, z);
+                const double rz_new = dot64(ctx, n, r, z);
+                const double rz_old = dot64(ctx, n, r_old, z);
+                const double beta = (rz_new - rz_old) / rz;
+                hipLaunchKernelGGL(fpcg_dir, grid1(n), dim3(256), 0, s, n, beta, z, p);
+                RAMD_HIP(hipGetLastError());
+                rz = rz_new;
+            }
+        }
+        RAMD_HIP(hipStreamSynchronize(s));
+        *iters = it;
+        *final_relres = relres;
+        if (rc == RAMD_OK && relres > rtol) rc = RAMD_E_NOT_CONVERGED;
+    });
+    if (st != RAMD_OK) return st;
+    if (rc == RAMD_E_INDEFINITE) set_error("ramd_fpcg: p.Ap <= 0 (matrix or preconditioner not SPD)");
+    if (rc == RAMD_E_NUMERIC) set_error("ramd_fpcg: non-finite value in the iteration");
+    if (rc == RAMD_E_NOT_CONVERGED) set_error("ramd_fpcg: %d iterations did not reach rtol", max_iter);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,942 @@
+// Host AMG setup: layout helpers, the reference's random stream, strength
+// of connection, C/F splitting, interpolation and stencil generators.
+//
+// All routines are the one-rank specialisation of the reference's parallel
+// code: every off_proc block, halo exchange and transpose exchange is empty
+// at one rank (core/comm_pkg.hpp: num_msgs == 0), so only the on_proc
+// arithmetic remains, restated here in the reference's visiting order.
+#include "setup.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <numeric>
+#include <stdexcept>
+
+namespace ramd {
+namespace host {
+
+// ---------------------------------------------------------------------------
+// layout
+// ---------------------------------------------------------------------------
+void sort_rows(Csr& A)
+{
+    std::vector<int> perm;
+    std::vector<int> c2;
+    std::vector<double> v2;
+    const bool has_val = !A.val.empty();
+    for (int i = 0; i < A.n_rows; i++) {
+        const int s = A.rowptr[i], e = A.rowptr[i + 1];
+        bool sorted = true;
+        for (int j = s + 1; j < e && sorted; j++) sorted = A.col[j - 1] <= A.col[j];
+        if (sorted) continue;
+        perm.resize(e - s);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return A.col[s + a] < A.col[s + b]; });
+        c2.resize(e - s);
+        v2.resize(e - s);
+        for (int k = 0; k < e - s; k++) {
+            c2[k] = A.col[s + perm[k]];
+            if (has_val) v2[k] = A.val[s + perm[k]];
+        }
+        std::copy(c2.begin(), c2.end(), A.col.begin() + s);
+        if (has_val) std::copy(v2.begin(), v2.end(), A.val.begin() + s);
+    }
+}
+
+void move_diag(Csr& A)
+{
+    const bool has_val = !A.val.empty();
+    for (int i = 0; i < A.n_rows; i++) {
+        const int s = A.rowptr[i], e = A.rowptr[i + 1];
+        for (int j = s; j < e; j++) {
+            if (A.col[j] == i) {
+                const double t = has_val ? A.val[j] : 0.0;
+                for (int k = j; k > s; k--) {
+                    A.col[k] = A.col[k - 1];
+                    if (has_val) A.val[k] = A.val[k - 1];
+                }
+                A.col[s] = i;
+                if (has_val) A.val[s] = t;
+                break;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// glibc rand(): srandom_r fills 31 words with 16807*x mod (2^31-1) (Schrage's
+// method), the front pointer starts 3 words ahead, 310 outputs are discarded,
+// and each output is (state[f] += state[r]) >> 1 over uint32 arithmetic.
+// ---------------------------------------------------------------------------
+GlibcRand::GlibcRand(unsigned seed)
+{
+    if (seed == 0) seed = 1;
+    r_[0] = (int32_t)seed;
+    int32_t word = (int32_t)seed;
+    for (int i = 1; i < 31; i++) {
+        long hi = word / 127773, lo = word % 127773;
+        word = (int32_t)(16807 * lo - 2836 * hi);
+        if (word < 0) word += 2147483647;
+        r_[i] = word;
+    }
+    f_ = 3;
+    b_ = 0;
+    for (int i = 0; i < 310; i++) next();
+}
+
+int GlibcRand::next()
+{
+    uint32_t v = (uint32_t)r_[f_] + (uint32_t)r_[b_];
+    r_[f_] = (int32_t)v;
+    f_ = (f_ + 1) % 31;
+    b_ = (b_ + 1) % 31;
+    return (int)(v >> 1);
+}
+
+std::vector<double> reference_weights(int n, int first_row)
+{
+    std::vector<double> w((size_t)n);
+    GlibcRand g((unsigned)(2448422 + first_row));
+    for (int i = 0; i < n; i++) w[i] = double(g.next()) / GlibcRand::kRandMax;
+    return w;
+}
+
+// ---------------------------------------------------------------------------
+// strength, par_strength.cpp
+// ---------------------------------------------------------------------------
+static void sort_and_diag_first(Csr& A)
+{
+    sort_rows(A);
+    move_diag(A);
+}
+
+// classical_strength par_strength.cpp:8-340 (num_variables == 1)
+Csr classical_strength(Csr& A, double theta)
+{
+    sort_and_diag_first(A);
+    Csr S;
+    S.n_rows = A.n_rows;
+    S.n_cols = A.n_cols;
+    S.rowptr.assign((size_t)A.n_rows + 1, 0);
+    S.col.reserve(A.col.size());
+    S.val.reserve(A.col.size());
+    for (int i = 0; i < A.n_rows; i++) {
+        int s = A.rowptr[i];
+        const int e = A.rowptr[i + 1];
+        if (e - s) {
+            double diag;
+            if (A.col[s] == i) {
+                diag = A.val[s];
+                s++;
+            } else {
+                diag = 0.0;
+            }
+            double row_scale;
+            if (diag < 0.0) {
+                row_scale = -RAND_MAX;
+                for (int j = s; j < e; j++)
+                    if (A.val[j] > row_scale) row_scale = A.val[j];
+            } else {
+                row_scale = RAND_MAX;
+                for (int j = s; j < e; j++)
+                    if (A.val[j] < row_scale) row_scale = A.val[j];
+            }
+            const double threshold = row_scale * theta;
+            S.col.push_back(i);
+            S.val.push_back(diag);
+            if (diag < 0) {
+                for (int j = s; j < e; j++)
+                    if (A.val[j] > threshold) {
+                        S.col.push_back(A.col[j]);
+                        S.val.push_back(A.val[j]);
+                    }
+            } else {
+                for (int j = s; j < e; j++)
+                    if (A.val[j] < threshold) {
+                        S.col.push_back(A.col[j]);
+                        S.val.push_back(A.val[j]);
+                    }
+            }
+        }
+        S.rowptr[i + 1] = (int)S.col.size();
+    }
+    return S;
+}
+
+// symmetric_strength par_strength.cpp:342-534
+Csr symmetric_strength(Csr& A, double theta)
+{
+    sort_and_diag_first(A);
+    const int n = A.n_rows;
+    std::vector<int> neg_diags((size_t)n, 0);
+    std::vector<double> row_scales((size_t)n, 0.0);
+    for (int i = 0; i < n; i++) {
+        int s = A.rowptr[i];
+        const int e = A.rowptr[i + 1];
+        if (!(e - s)) continue;
+        double diag;
+        if (A.col[s] == i) {
+            diag = A.val[s];
+            s++;
+        } else {
+            diag = 0.0;
+        }
+        double row_scale;
+        if (diag < 0.0) {
+            neg_diags[i] = 1;
+            row_scale = -RAND_MAX;
+            for (int j = s; j < e; j++)
+                if (A.val[j] > row_scale) row_scale = A.val[j];
+        } else {
+            neg_diags[i] = 0;
+            row_scale = RAND_MAX;
+            for (int j = s; j < e; j++)
+                if (A.val[j] < row_scale) row_scale = A.val[j];
+        }
+        row_scales[i] = row_scale * theta;
+    }
+    Csr S;
+    S.n_rows = n;
+    S.n_cols = A.n_cols;
+    S.rowptr.assign((size_t)n + 1, 0);
+    S.col.reserve(A.col.size());
+    S.val.reserve(A.col.size());
+    for (int i = 0; i < n; i++) {
+        int s = A.rowptr[i];
+        const int e = A.rowptr[i + 1];
+        if (e - s) {
+            const bool neg_diag = neg_diags[i];
+            const double threshold = row_scales[i];
+            // the first stored entry is taken as the diagonal (:466-468)
+            S.col.push_back(i);
+            S.val.push_back(A.val[s++]);
+            for (int j = s; j < e; j++) {
+                const double v = A.val[j];
+                const int c = A.col[j];
+                if ((neg_diag && v > threshold) || (!neg_diag && v < threshold) ||
+                    (neg_diags[c] && v > row_scales[c]) || (!neg_diags[c] && v < row_scales[c])) {
+                    S.col.push_back(c);
+                    S.val.push_back(v);
+                }
+            }
+        }
+        S.rowptr[i + 1] = (int)S.col.size();
+    }
+    return S;
+}
+
+// ---------------------------------------------------------------------------
+// C/F splitting, ruge_stuben/par_cf_splitting.cpp
+// ---------------------------------------------------------------------------
+
+// set_initial_states :115-131
+static void set_initial_states(const Csr& S, std::vector<int>& states)
+{
+    states.assign((size_t)S.n_rows, kUnassigned);
+    for (int i = 0; i < S.n_rows; i++)
+        states[i] = (S.rowptr[i + 1] - S.rowptr[i] > 1) ? kUnassigned : kNoNeighbors;
+}
+
+// transpose :150-230 -- CSC of S without each row's leading diagonal, rows
+// ascending within a column.
+static void transpose_s(const Csr& S, std::vector<int>& col_ptr, std::vector<int>& col_idx)
+{
+    const int n = S.n_cols;
+    col_ptr.assign((size_t)n + 1, 0);
+    std::vector<int> sizes((size_t)n, 0);
+    for (int i = 0; i < S.n_rows; i++) {
+        int s = S.rowptr[i];
+        const int e = S.rowptr[i + 1];
+        if (s < e && S.col[s] == i) s++;
+        for (int j = s; j < e; j++) sizes[S.col[j]]++;
+    }
+    for (int i = 0; i < n; i++) {
+        col_ptr[i + 1] = col_ptr[i] + sizes[i];
+        sizes[i] = 0;
+    }
+    col_idx.assign((size_t)col_ptr[n], 0);
+    for (int i = 0; i < S.n_rows; i++) {
+        int s = S.rowptr[i];
+        const int e = S.rowptr[i + 1];
+        if (s < e && S.col[s] == i) s++;
+        for (int j = s; j < e; j++) {
+            const int c = S.col[j];
+            col_idx[col_ptr[c] + sizes[c]++] = i;
+        }
+    }
+}
+
+// initial_weights :232-289 with rand_vals given: w_i = rand_i + #{rows that
+// strongly depend on i}.
+static void initial_weights(const Csr& S, const double* rand_vals, std::vector<double>& w)
+{
+    for (int i = 0; i < S.n_cols; i++) w[i] = rand_vals[i];
+    for (int i = 0; i < S.n_rows; i++) {
+        int s = S.rowptr[i];
+        const int e = S.rowptr[i + 1];
+        if (s < e && S.col[s] == i) s++;
+        for (int j = s; j < e; j++) w[S.col[j]] += 1;
+    }
+}
+
+// select_independent_set :403-495 (one rank: max_off_weights are all 0)
+static int select_independent_set(const Csr& S, int remaining, const std::vector<int>& unassigned,
+                                  const std::vector<double>& w, const std::vector<int>& col_ptr,
+                                  const std::vector<int>& col_idx, std::vector<int>& states,
+                                  std::vector<int>& new_coarse)
+{
+    int num = 0;
+    for (int i = 0; i < remaining; i++) {
+        const int u = unassigned[i];
+        const double weight = w[u];
+        if (0.0 > weight) continue;
+        int s = S.rowptr[u];
+        const int e = S.rowptr[u + 1];
+        if (s < e && S.col[s] == u) s++;
+        int j;
+        for (j = s; j < e; j++)
+            if (w[S.col[j]] > weight) break;
+        if (j != e) continue;
+        for (j = col_ptr[u]; j < col_ptr[u + 1]; j++)
+            if (w[col_idx[j]] > weight) break;
+        if (j != col_ptr[u + 1]) continue;
+        states[u] = kNewSelection;
+        new_coarse[num++] = u;
+    }
+    return num;
+}
+
+// update_states :1200-1227
+static int update_states(std::vector<double>& w, std::vector<int>& states, int remaining,
+                         std::vector<int>& unassigned)
+{
+    int ctr = 0;
+    for (int i = 0; i < remaining; i++) {
+        const int u = unassigned[i];
+        if (states[u] == kNewSelection) {
+            w[u] = 0.0;
+            states[u] = kSelected;
+        } else if (w[u] < 1.0 || states[u] == kNewUnselection) {
+            w[u] = 0.0;
+            states[u] = kUnselected;
+        } else {
+            unassigned[ctr++] = unassigned[i];
+        }
+    }
+    return ctr;
+}
+
+// update_row_weights :497-536 (on_proc part)
+static void update_row_weights(const Csr& S, int num_new, const std::vector<int>& new_coarse,
+                               std::vector<char>& edgemark, const std::vector<int>& states,
+                               std::vector<double>& w)
+{
+    for (int i = 0; i < num_new; i++) {
+        const int c = new_coarse[i];
+        int s = S.rowptr[c];
+        const int e = S.rowptr[c + 1];
+        if (s < e && S.col[s] == c) s++;
+        for (int j = s; j < e; j++) {
+            const int idx = S.col[j];
+            if (states[idx] == kUnassigned && edgemark[j]) {
+                edgemark[j] = 0;
+                w[idx]--;
+            }
+        }
+    }
+}
+
+// update_local_dist2_weights :538-640 (on_proc new coarse points)
+static void update_local_dist2_weights(const Csr& S, int num_new, const std::vector<int>& new_coarse,
+                                       const std::vector<int>& col_ptr, const std::vector<int>& col_idx,
+                                       std::vector<char>& edgemark, const std::vector<int>& states,
+                                       std::vector<double>& w, std::vector<int>& c_dep_cache)
+{
+    for (int i = 0; i < num_new; i++) {
+        const int c = new_coarse[i];
+        const int s = col_ptr[c], e = col_ptr[c + 1];
+        for (int j = s; j < e; j++) {
+            const int idx = col_idx[j];
+            if (states[idx] == kUnassigned) c_dep_cache[idx] = c;
+        }
+        for (int j = s; j < e; j++) {
+            const int idx = col_idx[j];
+            if (states[idx] == kSelected) continue;
+            int is = S.rowptr[idx];
+            const int ie = S.rowptr[idx + 1];
+            if (is < ie && S.col[is] == idx) is++;
+            for (int k = is; k < ie; k++) {
+                const int ik = S.col[k];
+                if (states[ik] == kUnassigned && edgemark[k] && c_dep_cache[ik] == c) {
+                    edgemark[k] = 0;
+                    w[ik]--;
+                }
+            }
+        }
+    }
+}
+
+// cljp_main_loop :1383-1607
+static void cljp_main_loop(Csr& S, std::vector<int>& states, const double* rand_vals)
+{
+    const int n = S.n_rows;
+    std::vector<double> w((size_t)n, 0.0);
+    std::vector<int> new_coarse((size_t)n), unassigned((size_t)n), c_dep((size_t)n, kUnassigned);
+    std::vector<char> edgemark(S.col.size(), 1);
+    std::vector<int> col_ptr, col_idx;
+    initial_weights(S, rand_vals, w);
+    transpose_s(S, col_ptr, col_idx);
+    int remaining = 0, num_new = 0;
+    for (int i = 0; i < S.n_cols; i++) {
+        if (states[i] == kUnassigned) {
+            unassigned[remaining++] = i;
+        } else {
+            w[i] = 0.0;
+            if (states[i] == kSelected) new_coarse[num_new++] = i;
+        }
+    }
+    if (num_new) {
+        update_row_weights(S, num_new, new_coarse, edgemark, states, w);
+        update_local_dist2_weights(S, num_new, new_coarse, col_ptr, col_idx, edgemark, states, w, c_dep);
+    }
+    bool first_pass = true;
+    while (remaining || first_pass) {
+        num_new = select_independent_set(S, remaining, unassigned, w, col_ptr, col_idx, states, new_coarse);
+        update_row_weights(S, num_new, new_coarse, edgemark, states, w);
+        update_local_dist2_weights(S, num_new, new_coarse, col_ptr, col_idx, edgemark, states, w, c_dep);
+        remaining = update_states(w, states, remaining, unassigned);
+        first_pass = false;
+    }
+}
+
+// pmis_main_loop :1229-1381
+static void pmis_main_loop(Csr& S, std::vector<int>& states, const double* rand_vals)
+{
+    const int n = S.n_rows;
+    std::vector<double> w((size_t)n, 0.0);
+    std::vector<int> new_coarse((size_t)n), unassigned((size_t)n);
+    std::vector<int> col_ptr, col_idx;
+    transpose_s(S, col_ptr, col_idx);
+    initial_weights(S, rand_vals, w);
+    for (int i = 0; i < S.n_cols; i++) {
+        if (states[i] == kSelected) {
+            for (int j = col_ptr[i]; j < col_ptr[i + 1]; j++) {
+                const int row = col_idx[j];
+                if (states[row] == kUnassigned) states[row] = kUnselected;
+            }
+        }
+    }
+    int remaining = 0;
+    for (int i = 0; i < n; i++) {
+        if (states[i] == kUnassigned && w[i] < 1) {
+            states[i] = kUnselected;
+            w[i] = 0.0;
+        } else if (states[i] == kUnassigned) {
+            unassigned[remaining++] = i;
+        } else {
+            w[i] = 0.0;
+        }
+    }
+    bool first_pass = true;
+    while (remaining || first_pass) {
+        const int num_new =
+            select_independent_set(S, remaining, unassigned, w, col_ptr, col_idx, states, new_coarse);
+        for (int i = 0; i < num_new; i++) {
+            const int idx = new_coarse[i];
+            for (int j = col_ptr[idx]; j < col_ptr[idx + 1]; j++) {
+                const int row = col_idx[j];
+                if (states[row] == kUnassigned) states[row] = kNewUnselection;
+            }
+        }
+        remaining = update_states(w, states, remaining, unassigned);
+        first_pass = false;
+    }
+}
+
+// split_cljp :30-46
+void split_cljp(Csr& S, const double* weights, std::vector<int>& states)
+{
+    move_diag(S);
+    set_initial_states(S, states);
+    cljp_main_loop(S, states, weights);
+}
+
+// split_pmis :69-84
+void split_pmis(Csr& S, const double* weights, std::vector<int>& states)
+{
+    move_diag(S);
+    set_initial_states(S, states);
+    pmis_main_loop(S, states, weights);
+}
+
+// ---------------------------------------------------------------------------
+// interpolation, ruge_stuben/par_interpolation.cpp
+// ---------------------------------------------------------------------------
+static std::vector<int> coarse_index(const std::vector<int>& states, bool any_nonzero, int& n_coarse)
+{
+    std::vector<int> cmap(states.size(), -1);
+    n_coarse = 0;
+    for (size_t i = 0; i < states.size(); i++)
+        if (any_nonzero ? (states[i] != 0) : (states[i] == kSelected)) cmap[i] = n_coarse++;
+    return cmap;
+}
+
+// direct_interpolation :1453-1754.  Every state != 0 (C points and points
+// without neighbours) gets a column (:1527-1533), as in the reference.
+Csr direct_interpolation(Csr& A, Csr& S, const std::vector<int>& states)
+{
+    sort_and_diag_first(A);
+    sort_and_diag_first(S);
+    const int n = A.n_rows;
+    // values of A on S's pattern (:1470-1488)
+    std::vector<double> sa(S.col.size());
+    for (int i = 0; i < n; i++) {
+        int ctr = A.rowptr[i];
+        for (int j = S.rowptr[i]; j < S.rowptr[i + 1]; j++) {
+            const int c = S.col[j];
+            while (A.col[ctr] != c) ctr++;
+            sa[j] = A.val[ctr];
+        }
+    }
+    int nc;
+    std::vector<int> cmap = coarse_index(states, true, nc);
+    Csr P;
+    P.n_rows = n;
+    P.n_cols = nc;
+    P.rowptr.assign((size_t)n + 1, 0);
+    for (int i = 0; i < n; i++) {
+        if (states[i] == 1) {
+            P.col.push_back(cmap[i]);
+            P.val.push_back(1);
+        } else {
+            double sum_strong_pos = 0, sum_strong_neg = 0, sum_all_pos = 0, sum_all_neg = 0;
+            int s = S.rowptr[i];
+            const int e = S.rowptr[i + 1];
+            if (S.col[s] == i) s++;
+            for (int j = s; j < e; j++) {
+                if (states[S.col[j]] == 1) {
+                    const double v = sa[j];
+                    if (v < 0) sum_strong_neg += v;
+                    else sum_strong_pos += v;
+                }
+            }
+            int as = A.rowptr[i];
+            const int ae = A.rowptr[i + 1];
+            double diag = A.val[as];
+            as++;
+            for (int j = as; j < ae; j++) {
+                const double v = A.val[j];
+                if (v < 0) sum_all_neg += v;
+                else sum_all_pos += v;
+            }
+            const double alpha = sum_all_neg / sum_strong_neg;
+            double beta;
+            if (sum_strong_pos == 0) {
+                diag += sum_all_pos;
+                beta = 0;
+            } else {
+                beta = sum_all_pos / sum_strong_pos;
+            }
+            const double neg_coeff = -alpha / diag;
+            const double pos_coeff = -beta / diag;
+            for (int j = s; j < e; j++) {
+                const int c = S.col[j];
+                if (states[c] == 1) {
+                    const double v = sa[j];
+                    P.col.push_back(cmap[c]);
+                    P.val.push_back(v < 0 ? neg_coeff * v : pos_coeff * v);
+                }
+            }
+        }
+        P.rowptr[i + 1] = (int)P.col.size();
+    }
+    return P;
+}
+
+// mod_classical_interpolation :991-1451 (num_variables == 1)
+Csr mod_classical_interpolation(Csr& A, Csr& S, const std::vector<int>& states)
+{
+    sort_and_diag_first(A);
+    sort_and_diag_first(S);
+    const int n = A.n_rows;
+    int nc;
+    std::vector<int> cmap = coarse_index(states, false, nc);
+    std::vector<int> pos((size_t)A.n_cols, -1);
+    Csr P;
+    P.n_rows = n;
+    P.n_cols = nc;
+    P.rowptr.assign((size_t)n + 1, 0);
+    for (int i = 0; i < n; i++) {
+        if (states[i] == kSelected) {
+            P.col.push_back(cmap[i]);
+            P.val.push_back(1);
+            P.rowptr[i + 1] = (int)P.col.size();
+            continue;
+        }
+        const int row_start = (int)P.col.size();
+        for (int j = S.rowptr[i] + 1; j < S.rowptr[i + 1]; j++) {
+            const int c = S.col[j];
+            if (states[c] == kSelected) {
+                pos[c] = (int)P.col.size();
+                P.col.push_back(cmap[c]);
+                P.val.push_back(S.val[j]);
+            }
+        }
+        int start = A.rowptr[i];
+        const int end = A.rowptr[i + 1];
+        int ctr = S.rowptr[i] + 1;
+        const int end_S = S.rowptr[i + 1];
+        double weak_sum = A.val[start++];
+        for (int j = start; j < end; j++) {
+            const int c = A.col[j];
+            const double v = A.val[j];
+            if (ctr < end_S && S.col[ctr] == c) {
+                ctr++;
+                if (states[c] == kSelected) continue;
+                double coarse_sum = 0;
+                int sk = A.rowptr[c];
+                const int ek = A.rowptr[c + 1];
+                const double diag = A.val[sk++];
+                const int sign = diag > 0 ? 1 : -1;
+                for (int k = sk; k < ek; k++) {
+                    const int ck = A.col[k];
+                    if (states[ck] == kSelected) {
+                        const double vk = A.val[k];
+                        if (vk * sign < 0 && pos[ck] >= row_start) coarse_sum += vk;
+                    }
+                }
+                if (fabs(coarse_sum) < kZeroTol) weak_sum += v;
+                else coarse_sum = v / coarse_sum;
+                if (coarse_sum) {
+                    for (int k = A.rowptr[c] + 1; k < ek; k++) {
+                        const int ck = A.col[k];
+                        if (states[ck] == kSelected) {
+                            const double vk = A.val[k];
+                            const int idx = pos[ck];
+                            if (vk * sign < 0 && idx >= row_start) P.val[idx] += (coarse_sum * vk);
+                        }
+                    }
+                }
+            } else if (states[c] != kNoNeighbors) {
+                weak_sum += v;
+            }
+        }
+        P.rowptr[i + 1] = (int)P.col.size();
+        for (int j = P.rowptr[i]; j < P.rowptr[i + 1]; j++) P.val[j] /= -weak_sum;
+    }
+    return P;
+}
+
+// filter_interp :175-278
+void filter_interp(Csr& P, const double filter_threshold)
+{
+    if (!(filter_threshold > kZeroTol && filter_threshold <= 1)) return;
+    int row_start = 0, ctr = 0;
+    for (int i = 0; i < P.n_rows; i++) {
+        const int prev_ctr = ctr;
+        const int row_end = P.rowptr[i + 1];
+        double row_max = 0;
+        for (int j = row_start; j < row_end; j++) {
+            const double a = fabs(P.val[j]);
+            if (a > row_max) row_max = a;
+        }
+        row_max *= filter_threshold;
+        double row_sum = 0, remain_sum = 0;
+        for (int j = row_start; j < row_end; j++) {
+            const double v = P.val[j];
+            row_sum += v;
+            if (fabs(v) >= row_max) {
+                P.col[ctr] = P.col[j];
+                P.val[ctr] = v;
+                ctr++;
+                remain_sum += v;
+            }
+        }
+        if (fabs(remain_sum) > kZeroTol && fabs(row_sum - remain_sum) > kZeroTol) {
+            const double row_scale = row_sum / remain_sum;
+            for (int j = prev_ctr; j < ctr; j++) P.val[j] *= row_scale;
+        }
+        P.rowptr[i + 1] = ctr;
+        row_start = row_end;
+    }
+    P.col.resize(ctr);
+    P.val.resize(ctr);
+}
+
+// extended_interpolation :280-989 (num_variables == 1).  Row entries are
+// kept in discovery order, which later sets the order of every sum over P.
+Csr extended_interpolation(Csr& A, Csr& S, const std::vector<int>& states, double filter)
+{
+    sort_and_diag_first(A);
+    sort_and_diag_first(S);
+    const int n = A.n_rows;
+    int nc;
+    std::vector<int> cmap = coarse_index(states, false, nc);
+    std::vector<int> pos((size_t)A.n_cols, -1);
+    Csr P;
+    P.n_rows = n;
+    P.n_cols = nc;
+    P.rowptr.assign((size_t)n + 1, 0);
+    // upper bound of P's size (:531-566)
+    int64_t bound = 0;
+    for (int i = 0; i < n; i++) {
+        if (states[i] == kUnselected) {
+            bound += S.rowptr[i + 1] - S.rowptr[i];
+            for (int j = S.rowptr[i] + 1; j < S.rowptr[i + 1]; j++) {
+                const int c = S.col[j];
+                if (states[c] == kUnselected) bound += S.rowptr[c + 1] - S.rowptr[c];
+            }
+        } else {
+            bound++;
+        }
+    }
+    P.col.reserve((size_t)bound);
+    P.val.reserve((size_t)bound);
+    for (int i = 0; i < n; i++) {
+        if (states[i] != kUnselected) {
+            if (states[i] == kSelected) {
+                P.col.push_back(cmap[i]);
+                P.val.push_back(1);
+            }
+            P.rowptr[i + 1] = (int)P.col.size();
+            continue;
+        }
+        const int row_start = (int)P.col.size();
+        for (int j = S.rowptr[i] + 1; j < S.rowptr[i + 1]; j++) {
+            const int c = S.col[j];
+            if (states[c] == kSelected) {
+                if (pos[c] < row_start) {
+                    pos[c] = (int)P.col.size();
+                    P.col.push_back(cmap[c]);
+                    P.val.push_back(0.0);
+                }
+            } else if (states[c] == kUnselected) {
+                for (int k = S.rowptr[c] + 1; k < S.rowptr[c + 1]; k++) {
+                    const int ck = S.col[k];
+                    if (states[ck] == kSelected && pos[ck] < row_start) {
+                        pos[ck] = (int)P.col.size();
+                        P.col.push_back(cmap[ck]);
+                        P.val.push_back(0.0);
+                    }
+                }
+            }
+        }
+        pos[i] = (int)P.col.size();
+        const int row_end = (int)P.col.size();
+        int start = A.rowptr[i];
+        const int end = A.rowptr[i + 1];
+        double weak_sum = A.val[start++];
+        int ctr = S.rowptr[i] + 1;
+        const int end_S = S.rowptr[i + 1];
+        for (int j = start; j < end; j++) {
+            const int c = A.col[j];
+            const double v = A.val[j];
+            const int idx = pos[c];
+            if (idx >= row_start) {
+                P.val[idx] += v;
+                if (ctr < end_S && S.col[ctr] == c) ctr++;
+            } else if (ctr < end_S && S.col[ctr] == c) {
+                ctr++;
+                if (states[c] != kUnselected) continue;
+                double col_sum = 0;
+                int sk = A.rowptr[c];
+                const int ek = A.rowptr[c + 1];
+                const double diag = A.val[sk++];
+                const int sign = diag > 0 ? 1 : -1;
+                for (int k = sk; k < ek; k++) {
+                    const double vk = A.val[k];
+                    if (vk * sign < 0 && pos[A.col[k]] >= row_start) col_sum += vk;
+                }
+                if (fabs(col_sum) < kZeroTol) {
+                    weak_sum += v;
+                } else {
+                    col_sum = v / col_sum;
+                    for (int k = A.rowptr[c] + 1; k < ek; k++) {
+                        const int ck = A.col[k];
+                        const double vk = A.val[k];
+                        const int pk = pos[ck];
+                        if (vk * sign < 0 && pk >= row_start) {
+                            if (ck == i) weak_sum += (col_sum * vk);
+                            else P.val[pk] += (col_sum * vk);
+                        }
+                    }
+                }
+            } else {
+                if (states[c] != kNoNeighbors) weak_sum += v;
+            }
+        }
+        if (fabs(weak_sum) > kZeroTol)
+            for (int j = row_start; j < row_end; j++) P.val[j] /= -weak_sum;
+        pos[i] = -1;
+        P.rowptr[i + 1] = row_end;
+    }
+    filter_interp(P, filter);
+    return P;
+}
+
+// ---------------------------------------------------------------------------
+// stencil generators
+// ---------------------------------------------------------------------------
+
+// par_stencil_grid gallery/par_stencil.hpp:16-236 at one rank.  Row r takes,
+// for its d-th nonzero stencil offset (ascending), the value of nonzero
+// stencil entry N_s-1-d (:220), dropped when the neighbour leaves the grid
+// in any dimension (the boundary zeroing of :137-209 for the symmetric
+// stencils the gallery provides).
+Csr stencil_grid(const double* stencil, const int* grid, int dim)
+{
+    int len = 1;
+    for (int i = 0; i < dim; i++) len *= 3;
+    int64_t nv = 1;
+    for (int i = 0; i < dim; i++) nv *= grid[i];
+    if (nv > INT32_MAX) throw std::length_error("stencil grid exceeds int32 rows");
+    // extents and strides: dimension k (0 = fastest) has stride prod grid[dim-1..dim-k]
+    std::vector<int64_t> stride(dim), extent(dim);
+    stride[0] = 1;
+    for (int k = 0; k < dim; k++) {
+        extent[k] = grid[dim - 1 - k];
+        if (k + 1 < dim) stride[k + 1] = stride[k] * extent[k];
+    }
+    std::vector<std::vector<int>> off;  // per nonzero entry: offset digit per dimension k
+    std::vector<double> nz;
+    std::vector<int64_t> diag;
+    for (int i = 0; i < len; i++) {
+        if (fabs(stencil[i]) > kZeroTol) {
+            std::vector<int> o(dim);
+            int64_t d = 0;
+            int t = i;
+            for (int k = 0; k < dim; k++) {
+                o[k] = (t % 3) - 1;
+                t /= 3;
+                d += stride[k] * o[k];
+            }
+            off.push_back(o);
+            nz.push_back(stencil[i]);
+            diag.push_back(d);
+        }
+    }
+    const int ns = (int)nz.size();
+    Csr A;
+    A.n_rows = A.n_cols = (int)nv;
+    A.rowptr.assign((size_t)nv + 1, 0);
+    A.col.reserve((size_t)nv * ns);
+    A.val.reserve((size_t)nv * ns);
+    std::vector<int64_t> coord(dim);
+    for (int64_t r = 0; r < nv; r++) {
+        int64_t t = r;
+        for (int k = 0; k < dim; k++) {
+            coord[k] = t % extent[k];
+            t /= extent[k];
+        }
+        for (int d = 0; d < ns; d++) {
+            bool inside = true;
+            for (int k = 0; k < dim && inside; k++) {
+                const int64_t c = coord[k] + off[d][k];
+                inside = c >= 0 && c < extent[k];
+            }
+            const double v = nz[ns - 1 - d];
+            if (inside && fabs(v) > kZeroTol) {
+                A.col.push_back((int)(r + diag[d]));
+                A.val.push_back(v);
+            }
+        }
+        A.rowptr[r + 1] = (int)A.col.size();
+    }
+    return A;
+}
+
+// laplace_stencil_27pt gallery/laplacian27pt.hpp:27-39
+std::vector<double> laplace_stencil_27pt()
+{
+    std::vector<double> s(27, -1.0);
+    s[13] = 26;
+    return s;
+}
+
+// diffusion_stencil_2d gallery/diffusion.hpp:58-85
+std::vector<double> diffusion_stencil_2d(double eps, double theta)
+{
+    const double C = cos(theta), S = sin(theta);
+    const double CS = C * S, CC = C * C, SS = S * S;
+    const double v1 = ((-1 * eps - 1) * CC + (-1 * eps - 1) * SS + (3 * eps - 3) * CS) / 6.0;
+    const double v2 = ((2 * eps - 4) * CC + (-4 * eps + 2) * SS) / 6.0;
+    const double v3 = ((-1 * eps - 1) * CC + (-1 * eps - 1) * SS + (-3 * eps + 3) * CS) / 6.0;
+    const double v4 = ((-4 * eps + 2) * CC + (2 * eps - 4) * SS) / 6.0;
+    const double v5 = ((8 * eps + 8) * CC + (8 * eps + 8) * SS) / 6.0;
+    return {v1, v2, v3, v4, v5, v4, v3, v2, v1};
+}
+
+// SplitMix64 -> uniform (u >> 11) * 2^-53 -> Box-Muller.
+static uint64_t splitmix64(uint64_t& s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static double uniform01(uint64_t& s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+static double gaussian(uint64_t& s)
+{
+    double u1 = uniform01(s), u2 = uniform01(s);
+    if (u1 < 1e-300) u1 = 1e-300;
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+Csr fv7_variable(int m, uint64_t seed)
+{
+    const int64_t n = (int64_t)m * m * m;
+    if (n > INT32_MAX) throw std::length_error("fv7 grid exceeds int32 rows");
+    std::vector<double> k((size_t)n);
+    uint64_t s = seed;
+    for (int64_t i = 0; i < n; i++) k[i] = exp(gaussian(s));
+    auto face = [&](int64_t p, int64_t q) { return 2.0 * k[p] * k[q] / (k[p] + k[q]); };
+    Csr A;
+    A.n_rows = A.n_cols = (int)n;
+    A.rowptr.assign((size_t)n + 1, 0);
+    A.col.reserve((size_t)n * 7);
+    A.val.reserve((size_t)n * 7);
+    const int64_t st[3] = {1, m, (int64_t)m * m};
+    for (int64_t p = 0; p < n; p++) {
+        const int64_t c[3] = {p % m, (p / m) % m, p / ((int64_t)m * m)};
+        double off_vals[6];
+        int64_t off_cols[6];
+        double diag = 0;
+        // neighbours in ascending column order: -z, -y, -x, +x, +y, +z
+        const int dims[6] = {2, 1, 0, 0, 1, 2};
+        const int sgn[6] = {-1, -1, -1, 1, 1, 1};
+        for (int f = 0; f < 6; f++) {
+            const int d = dims[f];
+            const int64_t cc = c[d] + sgn[f];
+            if (cc < 0 || cc >= m) {
+                diag += 2.0 * k[p];  // Dirichlet wall at half-cell distance
+                off_cols[f] = -1;
+            } else {
+                const int64_t q = p + sgn[f] * st[d];
+                const double w = face(p, q);
+                diag += w;
+                off_cols[f] = q;
+                off_vals[f] = -w;
+            }
+        }
+        for (int f = 0; f < 3; f++)
+            if (off_cols[f] >= 0) {
+                A.col.push_back((int)off_cols[f]);
+                A.val.push_back(off_vals[f]);
+            }
+        A.col.push_back((int)p);
+        A.val.push_back(diag);
+        for (int f = 3; f < 6; f++)
+            if (off_cols[f] >= 0) {
+                A.col.push_back((int)off_cols[f]);
+                A.val.push_back(off_vals[f]);
+            }
+        A.rowptr[p + 1] = (int)A.col.size();
+    }
+    return A;
+}
+
+}  // namespace host
+}  // namespace ramd

@@ -1,0 +1,402 @@
+// Row-block ("CSR-stream") kernels for the SpMV family and the
+// deterministic vector reductions.
+//
+// Each workgroup owns a block of consecutive rows whose nonzeros fit in
+// kStreamCap products.  Phase 1 streams the block's values and column
+// indices with fully coalesced loads and stages a_ij * x_j in LDS; phase 2
+// gives every row to one lane, which adds its products left to right.  The
+// per-row order of operations is therefore exactly the reference's
+// sequential loop (util/linalg/spmv.cpp:52-103, util/linalg/relax.cpp:23-41),
+// so FP64 results are bit-identical to it (the file is compiled with
+// -ffp-contract=off).  A row longer than kStreamCap is processed in chunks
+// by its own workgroup.
+#include <algorithm>
+#include <cmath>
+
+#include "common.hpp"
+
+namespace ramd {
+
+template <int OP, typename V, typename X>
+__global__ __launch_bounds__(kBlockThreads) void stream_kernel(
+    const int* __restrict__ rowptr, const int* __restrict__ col, const V* __restrict__ val,
+    const int* __restrict__ blocks, int nb, const X* __restrict__ x, const X* __restrict__ b,
+    X* __restrict__ y, X omega)
+{
+    __shared__ X prod[kStreamCap];
+    const int blk = xcd_remap(blockIdx.x, nb);
+    const int r0 = blocks[blk], r1 = blocks[blk + 1];
+    const int s = rowptr[r0], e = rowptr[r1];
+    const int tid = threadIdx.x;
+
+    if (e - s > kStreamCap) {
+        // one long row, processed in LDS-sized chunks, summed by lane 0
+        X acc = (OP == OP_RESIDUAL) ? b[r0] : X(0);
+        X diag = 0;
+        for (int c0 = s; c0 < e; c0 += kStreamCap) {
+            const int cn = min(kStreamCap, e - c0);
+            for (int j = tid; j < cn; j += kBlockThreads) prod[j] = X(val[c0 + j]) * x[col[c0 + j]];
+            __syncthreads();
+            if (tid == 0) {
+                for (int k = 0; k < cn; k++) {
+                    if (OP == OP_JACOBI) {
+                        if (col[c0 + k] == r0) diag = X(val[c0 + k]);
+                        else acc += prod[k];
+                    } else if (OP == OP_RESIDUAL) {
+                        acc -= prod[k];
+                    } else {
+                        acc += prod[k];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            if (OP == OP_SPMV || OP == OP_RESIDUAL) y[r0] = acc;
+            else if (OP == OP_APPEND) y[r0] += acc;
+            else {
+                const X xo = x[r0];
+                y[r0] = (fabs(diag) > X(1e-16)) ? ((X(1.0) - omega) * xo) + (omega * ((b[r0] - acc) / diag)) : xo;
+            }
+        }
+        return;
+    }
+
+    for (int j = tid; j < e - s; j += kBlockThreads) prod[j] = X(val[s + j]) * x[col[s + j]];
+    __syncthreads();
+
+    const int row = r0 + tid;
+    if (row >= r1) return;
+    const int k0 = rowptr[row] - s, k1 = rowptr[row + 1] - s;
+    if (OP == OP_SPMV) {
+        X acc = 0;
+        for (int k = k0; k < k1; k++) acc += prod[k];
+        y[row] = acc;
+    } else if (OP == OP_APPEND) {
+        X acc = 0;
+        for (int k = k0; k < k1; k++) acc += prod[k];
+        y[row] += acc;
+    } else if (OP == OP_RESIDUAL) {
+        X acc = b[row];
+        for (int k = k0; k < k1; k++) acc -= prod[k];
+        y[row] = acc;
+    } else {
+        // relax.cpp:23-41: tmp (= x here) holds the old iterate, y the new
+        const X xo = x[row];
+        if (k0 == k1) {
+            y[row] = xo;
+            return;
+        }
+        X acc = 0, diag = 0;
+        for (int k = k0; k < k1; k++) {
+            if (col[s + k] == row) diag = X(val[s + k]);
+            else acc += prod[k];
+        }
+        y[row] = (fabs(diag) > X(1e-16)) ? ((X(1.0) - omega) * xo) + (omega * ((b[row] - acc) / diag)) : xo;
+    }
+}
+
+static hipEvent_t prof_event(ramd_ctx ctx)
+{
+    if (!ctx->prof_pool.empty()) {
+        hipEvent_t e = ctx->prof_pool.back();
+        ctx->prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    RAMD_HIP(hipEventCreate(&e));
+    return e;
+}
+
+// Algorithmic bytes of one launch: the matrix streamed once, x gathered with
+// ideal reuse (n_cols entries), the output written once; residual, Jacobi
+// and append read one more n-vector (b, or y for append).
+static double op_bytes(const ramd_csr_s* A, int op, int sv)
+{
+    const double si = 4.0;
+    double b = (double)A->nnz * (sv + si) + 4.0 * (A->n_rows + 1) + (double)sv * A->n_cols + (double)sv * A->n_rows;
+    if (op != OP_SPMV) b += (double)sv * A->n_rows;
+    return b;
+}
+
+template <typename V, typename X>
+static void launch_stream(const ramd_csr_s* A, const V* val, int op, const X* x, const X* b, X* y, X omega,
+                          hipStream_t s)
+{
+    if (A->n_rows == 0) return;
+    const int nb = A->n_blocks;
+    dim3 grid(nb), block(kBlockThreads);
+    ramd_ctx ctx = A->ctx;
+    const bool prof = ctx->prof_min_nnz >= 0 && A->nnz >= ctx->prof_min_nnz;
+    ProfPending pend{};
+    if (prof) {
+        pend.beg = prof_event(ctx);
+        pend.end = prof_event(ctx);
+        pend.slot = op * 2 + (sizeof(X) == 4 ? 1 : 0);
+        RAMD_HIP(hipEventRecord(pend.beg, s));
+        ctx->prof[pend.slot].bytes += op_bytes(A, op, (int)sizeof(V));
+        ctx->prof[pend.slot].launches++;
+    }
+    switch (op) {
+    case OP_SPMV:
+        hipLaunchKernelGGL((stream_kernel<OP_SPMV, V, X>), grid, block, 0, s, A->rowptr, A->col, val, A->blocks,
+                           nb, x, b, y, omega);
+        break;
+    case OP_APPEND:
+        hipLaunchKernelGGL((stream_kernel<OP_APPEND, V, X>), grid, block, 0, s, A->rowptr, A->col, val,
+                           A->blocks, nb, x, b, y, omega);
+        break;
+    case OP_RESIDUAL:
+        hipLaunchKernelGGL((stream_kernel<OP_RESIDUAL, V, X>), grid, block, 0, s, A->rowptr, A->col, val,
+                           A->blocks, nb, x, b, y, omega);
+        break;
+    case OP_JACOBI:
+        hipLaunchKernelGGL((stream_kernel<OP_JACOBI, V, X>), grid, block, 0, s, A->rowptr, A->col, val,
+                           A->blocks, nb, x, b, y, omega);
+        break;
+    default:
+        fail(RAMD_E_ARG, "unknown stream op %d", op);
+    }
+    RAMD_HIP(hipGetLastError());
+    if (prof) {
+        RAMD_HIP(hipEventRecord(pend.end, s));
+        ctx->prof_pending.push_back(pend);
+    }
+}
+
+// Resolves recorded event pairs into the per-slot totals.
+static void prof_resolve(ramd_ctx ctx)
+{
+    if (ctx->prof_pending.empty()) return;
+    RAMD_HIP(hipStreamSynchronize(ctx->stream));
+    for (const ProfPending& p : ctx->prof_pending) {
+        float ms = 0;
+        RAMD_HIP(hipEventElapsedTime(&ms, p.beg, p.end));
+        ctx->prof[p.slot].ms += ms;
+        ctx->prof_pool.push_back(p.beg);
+        ctx->prof_pool.push_back(p.end);
+    }
+    ctx->prof_pending.clear();
+}
+
+void launch_stream_op(const ramd_csr_s* A, int op, const double* x, const double* b, double* y, double omega,
+                      hipStream_t s)
+{
+    launch_stream<double, double>(A, A->val, op, x, b, y, omega, s);
+}
+
+void launch_stream_op32(const ramd_csr_s* A, int op, const float* x, const float* b, float* y, float omega,
+                        hipStream_t s)
+{
+    RAMD_CHECK(A->val32, RAMD_E_STATE, "FP32 values of the matrix were not built");
+    launch_stream<float, float>(A, A->val32, op, x, b, y, omega, s);
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic reductions: a fixed grid of workgroups, each summing a fixed
+// strided subset in a fixed order, then one workgroup combining the partials
+// in index order.  Same inputs -> same bits, every run.
+// ---------------------------------------------------------------------------
+constexpr int kRedBlocks = 1024;
+
+__device__ __forceinline__ double block_sum(double v)
+{
+    __shared__ double wsum[kBlockThreads / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wsum[w] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kBlockThreads / 64; i++) t += wsum[i];
+    __syncthreads();
+    return t;
+}
+
+template <int MODE>  // 0: x.y   1: sum x^2 over |x| > 1e-16
+__global__ __launch_bounds__(kBlockThreads) void reduce_partial(const double* __restrict__ x,
+                                                                const double* __restrict__ y, int64_t n,
+                                                                double* __restrict__ partials)
+{
+    double s = 0;
+    const int64_t stride = (int64_t)gridDim.x * kBlockThreads;
+    for (int64_t i = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x; i < n; i += stride) {
+        if (MODE == 0) {
+            s += x[i] * y[i];
+        } else {
+            const double v = x[i];
+            if (fabs(v) > 1e-16) s += v * v;
+        }
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlockThreads) void reduce_final(const double* __restrict__ partials, int np,
+                                                              double* __restrict__ out)
+{
+    double s = 0;
+    for (int i = threadIdx.x; i < np; i += kBlockThreads) s += partials[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) *out = s;
+}
+
+static int red_blocks(int64_t n)
+{
+    int64_t b = (n + kBlockThreads - 1) / kBlockThreads;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, kRedBlocks));
+}
+
+void launch_dot(ramd_ctx ctx, int64_t n, const double* x, const double* y, double* out_d)
+{
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL(reduce_partial<0>, dim3(nb), dim3(kBlockThreads), 0, ctx->stream, x, y, n,
+                       ctx->red_partials);
+    hipLaunchKernelGGL(reduce_final, dim3(1), dim3(kBlockThreads), 0, ctx->stream, ctx->red_partials, nb, out_d);
+    RAMD_HIP(hipGetLastError());
+}
+
+void launch_norm2sq(ramd_ctx ctx, int64_t n, const double* x, double* out_d)
+{
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL(reduce_partial<1>, dim3(nb), dim3(kBlockThreads), 0, ctx->stream, x, x, n,
+                       ctx->red_partials);
+    hipLaunchKernelGGL(reduce_final, dim3(1), dim3(kBlockThreads), 0, ctx->stream, ctx->red_partials, nb, out_d);
+    RAMD_HIP(hipGetLastError());
+}
+
+double fetch_scalar(ramd_ctx ctx, const double* d)
+{
+    RAMD_HIP(hipMemcpyAsync(ctx->pinned, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RAMD_HIP(hipStreamSynchronize(ctx->stream));
+    return ctx->pinned[0];
+}
+
+__global__ void axpy_kernel(int64_t n, double alpha, const double* __restrict__ x, double* __restrict__ y)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) y[i] += x[i] * alpha;
+}
+
+}  // namespace ramd
+
+using namespace ramd;
+
+static void check_vec_op(ramd_csr A, const void* a, const void* b, const char* who)
+{
+    RAMD_CHECK(A, RAMD_E_ARG, "%s: NULL matrix", who);
+    RAMD_CHECK((a && b) || (A->n_rows == 0), RAMD_E_ARG, "%s: NULL vector", who);
+    A->ctx->set_current();
+}
+
+extern "C" {
+
+int ramd_spmv(ramd_csr A, const double* x_d, double* y_d)
+{
+    return guarded([&] {
+        check_vec_op(A, x_d, y_d, "ramd_spmv");
+        launch_stream_op(A, OP_SPMV, x_d, nullptr, y_d, 0.0, A->ctx->stream);
+    });
+}
+
+int ramd_spmv_append(ramd_csr A, const double* x_d, double* y_d)
+{
+    return guarded([&] {
+        check_vec_op(A, x_d, y_d, "ramd_spmv_append");
+        launch_stream_op(A, OP_APPEND, x_d, nullptr, y_d, 0.0, A->ctx->stream);
+    });
+}
+
+int ramd_residual(ramd_csr A, const double* x_d, const double* b_d, double* r_d)
+{
+    return guarded([&] {
+        check_vec_op(A, x_d, r_d, "ramd_residual");
+        RAMD_CHECK(b_d || A->n_rows == 0, RAMD_E_ARG, "ramd_residual: NULL b");
+        launch_stream_op(A, OP_RESIDUAL, x_d, b_d, r_d, 0.0, A->ctx->stream);
+    });
+}
+
+int ramd_spmv_t(ramd_csr A, const double* x_d, double* y_d)
+{
+    return guarded([&] {
+        check_vec_op(A, x_d, y_d, "ramd_spmv_t");
+        ramd_csr_s* T = csr_transpose(A);
+        launch_stream_op(T, OP_SPMV, x_d, nullptr, y_d, 0.0, A->ctx->stream);
+    });
+}
+
+int ramd_jacobi(ramd_csr A, double* x_d, const double* b_d, double* tmp_d, int sweeps, double omega)
+{
+    return guarded([&] {
+        check_vec_op(A, x_d, b_d, "ramd_jacobi");
+        RAMD_CHECK(tmp_d || A->n_rows == 0, RAMD_E_ARG, "ramd_jacobi: NULL tmp");
+        RAMD_CHECK(A->n_rows == A->n_cols, RAMD_E_ARG, "ramd_jacobi: matrix is %dx%d, not square", A->n_rows,
+                   A->n_cols);
+        for (int it = 0; it < sweeps; it++) {
+            RAMD_HIP(hipMemcpyAsync(tmp_d, x_d, sizeof(double) * A->n_rows, hipMemcpyDeviceToDevice,
+                                    A->ctx->stream));
+            launch_stream_op(A, OP_JACOBI, tmp_d, b_d, x_d, omega, A->ctx->stream);
+        }
+    });
+}
+
+int ramd_dot(ramd_ctx ctx, int64_t n, const double* x_d, const double* y_d, double* out)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx && out && n >= 0 && (n == 0 || (x_d && y_d)), RAMD_E_ARG, "ramd_dot: bad argument");
+        ctx->set_current();
+        launch_dot(ctx, n, x_d, y_d, ctx->red_result);
+        *out = fetch_scalar(ctx, ctx->red_result);
+    });
+}
+
+int ramd_norm2(ramd_ctx ctx, int64_t n, const double* x_d, double* out)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx && out && n >= 0 && (n == 0 || x_d), RAMD_E_ARG, "ramd_norm2: bad argument");
+        ctx->set_current();
+        launch_norm2sq(ctx, n, x_d, ctx->red_result);
+        double r = fetch_scalar(ctx, ctx->red_result);
+        // core/vector.cpp:117 and core/par_vector.cpp:94-97: root, undo, root
+        r = pow(r, 1.0 / 2);
+        r = pow(r, 2);
+        *out = pow(r, 1. / 2);
+    });
+}
+
+int ramd_prof_enable(ramd_ctx ctx, int64_t min_nnz)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx, RAMD_E_ARG, "ramd_prof_enable: NULL context");
+        prof_resolve(ctx);
+        ctx->prof_min_nnz = min_nnz;
+        for (ProfSlot& s : ctx->prof) s = ProfSlot();
+    });
+}
+
+int ramd_prof_read(ramd_ctx ctx, int op, int prec, int64_t* launches, double* ms, double* bytes)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx && op >= 0 && op < 4 && (prec == RAMD_PREC_FP64 || prec == RAMD_PREC_FP32), RAMD_E_ARG,
+                   "ramd_prof_read: bad argument");
+        prof_resolve(ctx);
+        const ProfSlot& s = ctx->prof[op * 2 + prec];
+        if (launches) *launches = s.launches;
+        if (ms) *ms = s.ms;
+        if (bytes) *bytes = s.bytes;
+    });
+}
+
+int ramd_axpy(ramd_ctx ctx, int64_t n, double alpha, const double* x_d, double* y_d)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx && n >= 0 && (n == 0 || (x_d && y_d)), RAMD_E_ARG, "ramd_axpy: bad argument");
+        ctx->set_current();
+        if (n == 0) return;
+        int grid = (int)std::min<int64_t>((n + 255) / 256, 16384);
+        hipLaunchKernelGGL(axpy_kernel, dim3(grid), dim3(256), 0, ctx->stream, n, alpha, x_d, y_d);
+        RAMD_HIP(hipGetLastError());
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,37 @@
+"""The C-ABI library loads, exports every symbol include/raptor_amd.h
+declares, and fails cleanly (no CPU fallback) where no GPU is usable."""
+import ctypes as C
+
+import pytest
+
+import raptor_amd as ra
+from raptor_amd import _lib
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    declared = _lib.declared_symbols()
+    assert len(declared) > 30
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_error_codes_without_device_or_arguments():
+    L = _lib.lib()
+    assert L.ramd_init(0, None) == _lib.RAMD_E_ARG
+    assert "NULL" in _lib.last_error()
+    assert L.ramd_csr_info(None, None, None, None) == _lib.RAMD_E_ARG
+
+
+def test_opts_defaults_match_reference():
+    o = ra.default_opts()
+    # multilevel/par_multilevel.hpp:87-99, ruge_stuben/par_ruge_stuben_solver.hpp:24
+    assert o.num_smooth_sweeps == 1 and o.relax_weight == 1.0
+    assert o.max_coarse == 50 and o.max_levels == 25 and o.interp_filter == 0.3
+
+
+def test_host_entry_points_reject_bad_input():
+    with pytest.raises(ra.RamdError):
+        ra.strength([0, 1], [5], [1.0], ra.Classical, 0.25)  # column out of range
+    with pytest.raises(ra.RamdError):
+        ra.split([0, 1], [0], 99, None)                       # unknown coarsening

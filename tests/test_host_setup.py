@@ -1,0 +1,179 @@
+"""Host setup pieces of the product (strength, C/F splitting, interpolation,
+stencil generators, the weight stream) against the reference's golden data
+(tests/test_par_strength.cpp, ruge_stuben/tests/test_par_splitting.cpp,
+ruge_stuben/tests/test_par_interpolation.cpp, gallery/tests/*) and against
+the reference compiled from source.  These run on the CPU by design: the
+reference's hierarchy construction stays on the host in this build."""
+import ctypes as C
+import ctypes.util
+import math
+
+import numpy as np
+import pytest
+
+import checkers as K
+import raptor_amd as ra
+
+needs_ref = pytest.mark.skipif(not K.have_ref(), reason="oracle/_ref not built (reference sources absent)")
+
+
+def as_sorted(A):
+    return K.sort_rows(A)
+
+
+def compare_sorted(rp, col, val, E, diag_first=True, atol=1e-6):
+    """tests/par_compare.hpp:14-53: sort both, diagonal first, exact pattern."""
+    n = len(rp) - 1
+    M = K.Csr(rp, col, val, E.n_cols)
+    M = K.sort_rows(M)
+    E = K.sort_rows(E)
+    if diag_first:
+        M, E = K.move_diag(M), K.move_diag(E)
+    assert np.array_equal(M.rowptr, E.rowptr)
+    assert np.array_equal(M.col, E.col)
+    assert np.allclose(M.val, E.val, rtol=0, atol=atol)
+
+
+def test_glibc_rand_stream():
+    """The library's rand() restatement equals libc's srand/rand output."""
+    libc = C.CDLL(ctypes.util.find_library("c"))
+    libc.srand(2448422)
+    expect = np.array([libc.rand() / 2147483647.0 for _ in range(2000)])
+    # the weight stream is observable through CLJP on an edgeless graph:
+    # use the host splitter's default weights via a tiny S where only the
+    # ordering by weight matters -- instead compare directly through PMIS on
+    # a path graph, whose C points are the local maxima of w_i + degree.
+    n = 2000
+    rp = [0]
+    col = []
+    for i in range(n):
+        nb = [j for j in (i - 1, i, i + 1) if 0 <= j < n]
+        col += [i] + [j for j in nb if j != i]
+        rp.append(len(col))
+    st_default = ra.split(rp, col, ra.PMIS, None)
+    st_libc = ra.split(rp, col, ra.PMIS, expect)
+    assert np.array_equal(st_default, st_libc)
+
+
+@pytest.mark.parametrize("name,stype,fixture", [("aniso", ra.Classical, "aniso_S"),
+                                                ("laplacian", ra.Classical, "laplacian_S"),
+                                                ("aniso", ra.Symmetric, "aniso_SS"),
+                                                ("laplacian", ra.Symmetric, "laplacian_SS")])
+def test_strength_golden(golden, name, stype, fixture):
+    """tests/test_par_strength.cpp (theta = 0.25)."""
+    A = K.fixture_csr(golden, name)
+    srp, scol = ra.strength(A.rowptr, A.col, A.val, stype, 0.25)
+    E = K.fixture_csr(golden, fixture)
+    S = K.Csr(srp, scol, np.ones(scol.size))
+    S, E2 = K.move_diag(K.sort_rows(S)), K.move_diag(E)
+    assert np.array_equal(S.rowptr, E2.rowptr) and np.array_equal(S.col, E2.col)
+
+
+@pytest.mark.parametrize("level", [0, 1])
+@pytest.mark.parametrize("ctype,suffix", [(ra.CLJP, ""), (ra.PMIS, "_pmis")])
+def test_split_golden(golden, level, ctype, suffix):
+    """ruge_stuben/tests/test_par_splitting.cpp with weights.txt."""
+    S = K.fixture_csr(golden, f"rss_S{level}")
+    w = golden["weights"][:S.n_rows]
+    st = ra.split(S.rowptr, S.col, ctype, w)
+    assert np.array_equal(st, golden[f"rss_cf{level}{suffix}"])
+
+
+@pytest.mark.parametrize("name", ["aniso", "laplacian"])
+@pytest.mark.parametrize("itype,suffix", [(ra.Direct, "direct"), (ra.ModClassical, "mod_class"),
+                                          (ra.Extended, "extend")])
+def test_interpolation_golden(golden, name, itype, suffix):
+    """ruge_stuben/tests/test_par_interpolation.cpp (extended with filter 0)."""
+    A = K.fixture_csr(golden, name)
+    S = K.fixture_csr(golden, name + "_S")
+    states = golden[name + "_split"]
+    prp, pcol, pval, nc = ra.interpolation(A.rowptr, A.col, A.val, S.rowptr, S.col, states, itype, 0.0)
+    E = K.fixture_csr(golden, f"{name}_P_{suffix}")
+    assert nc == E.n_cols
+    compare_sorted(prp, pcol, pval, E, diag_first=False)
+
+
+def test_gallery_matches_fixture(golden):
+    rp, col, val = ra.gallery_laplace27(10)
+    E = K.fixture_csr(golden, "laplacian27")
+    assert np.array_equal(rp, E.rowptr) and np.array_equal(col, E.col) and np.array_equal(val, E.val)
+    rp, col, val = ra.gallery_aniso2d(25, 0.001, math.pi / 8)
+    E = K.fixture_csr(golden, "aniso")
+    assert np.array_equal(rp, E.rowptr) and np.array_equal(col, E.col)
+    assert np.allclose(val, E.val, rtol=1e-13, atol=0)
+
+
+def test_gallery_fv7_properties():
+    rp, col, val = ra.gallery_fv7(6, seed=3)
+    A = K.Csr(rp, col, val)
+    D = A.dense()
+    assert np.allclose(D, D.T, rtol=0, atol=1e-13)          # symmetric
+    assert np.all(np.linalg.eigvalsh(D) > 0)                 # SPD (Dirichlet)
+    assert np.all(np.diff(rp) <= 7) and rp[-1] == 6 ** 3 * 7 - 6 * 6 * 6
+    rp2, _, val2 = ra.gallery_fv7(6, seed=3)
+    assert np.array_equal(val, val2)                         # seeded, reproducible
+
+
+# ---------------------------------------------------------------------------
+# against the reference compiled from source (bit-exact)
+# ---------------------------------------------------------------------------
+def _ref_csr(nnz):
+    rp, col, val, nc = K.ref_fetch(nnz)
+    return K.Csr(rp, col, val, nc)
+
+
+@needs_ref
+@pytest.mark.parametrize("kind,m", [(0, 12), (1, 40)])
+def test_gallery_bit_exact_vs_ref(kind, m):
+    L = K.ref()
+    E = _ref_csr(L.ref_gallery(kind, m, C.c_double(0.001), C.c_double(math.pi / 8)))
+    rp, col, val = ra.gallery_laplace27(m) if kind == 0 else ra.gallery_aniso2d(m, 0.001, math.pi / 8)
+    assert np.array_equal(rp, E.rowptr) and np.array_equal(col, E.col) and np.array_equal(val, E.val)
+
+
+def _problems():
+    return [("aniso100", ra.gallery_aniso2d(100, 0.001, math.pi / 8)), ("lap27_14", ra.gallery_laplace27(14))]
+
+
+@needs_ref
+@pytest.mark.parametrize("stype", [ra.Classical, ra.Symmetric])
+def test_strength_bit_exact_vs_ref(stype):
+    for _, (rp, col, val) in _problems():
+        A = K.Csr(rp, col, val)
+        E = _ref_csr(K.ref().ref_strength(A.n_rows, K.ip(A.rowptr), K.ip(A.col), K.dp(A.val), stype,
+                                          C.c_double(0.25)))
+        srp, scol = ra.strength(rp, col, val, stype, 0.25)
+        assert np.array_equal(srp, E.rowptr) and np.array_equal(scol, E.col)
+
+
+@needs_ref
+@pytest.mark.parametrize("ctype", [ra.CLJP, ra.PMIS])
+def test_split_bit_exact_vs_ref(ctype):
+    for _, (rp, col, val) in _problems():
+        srp, scol = ra.strength(rp, col, val, ra.Classical, 0.25)
+        n = srp.size - 1
+        w = np.random.default_rng(7).random(n)
+        st = np.empty(n, np.int32)
+        assert K.ref().ref_split(n, K.ip(srp), K.ip(scol), ctype, K.dp(w), K.ip(st)) == 0
+        assert np.array_equal(ra.split(srp, scol, ctype, w), st)
+
+
+@needs_ref
+@pytest.mark.parametrize("itype", [ra.Direct, ra.ModClassical, ra.Extended])
+@pytest.mark.parametrize("ctype", [ra.CLJP, ra.PMIS])
+def test_interp_bit_exact_vs_ref(itype, ctype):
+    for _, (rp, col, val) in _problems():
+        A = K.Csr(rp, col, val)
+        srp, scol = ra.strength(rp, col, val, ra.Classical, 0.25)
+        n = A.n_rows
+        st = ra.split(srp, scol, ctype, np.random.default_rng(8).random(n))
+        # S values are A's values on S's pattern
+        S = K.Csr(srp, scol, np.zeros(scol.size))
+        Ad = {(i, c): v for i in range(n) for c, v in zip(col[rp[i]:rp[i + 1]], val[rp[i]:rp[i + 1]])}
+        S.val = np.array([Ad[(i, c)] for i in range(n) for c in scol[srp[i]:srp[i + 1]]])
+        E = _ref_csr(K.ref().ref_interp(n, K.ip(A.rowptr), K.ip(A.col), K.dp(A.val), K.ip(S.rowptr),
+                                        K.ip(S.col), K.dp(S.val), K.ip(st), itype, C.c_double(0.3)))
+        prp, pcol, pval, nc = ra.interpolation(rp, col, val, srp, scol, st, itype, 0.3)
+        assert nc == E.n_cols
+        # stored order matters downstream (it fixes the summation order of A*P)
+        assert np.array_equal(prp, E.rowptr) and np.array_equal(pcol, E.col) and np.array_equal(pval, E.val)
