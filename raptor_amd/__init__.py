@@ -21,6 +21,7 @@ libraptor_amd.so (include/raptor_amd.h):
 Every compute call runs the HIP library; nothing falls back to a CPU path.
 """
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -49,19 +50,33 @@ def _f64(a):
 
 
 class Context:
-    """One GPU and its stream (ramd_init)."""
+    """One GPU and its stream (ramd_init).  Closing it first releases every
+    live object created on it, so nothing is freed after its context."""
 
     def __init__(self, device=0):
         self.h = C.c_void_p()
+        self._children = weakref.WeakSet()
         check(lib().ramd_init(device, C.byref(self.h)))
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     def sync(self):
         check(lib().ramd_sync(self.h))
 
     def close(self):
         if self.h:
+            # hierarchies own level matrices: free them after borrowed views
+            for obj in sorted(list(self._children), key=lambda o: isinstance(o, ParRugeStubenSolver)):
+                obj.free()
             lib().ramd_finalize(self.h)
             self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class ParVector:
@@ -71,6 +86,7 @@ class ParVector:
         self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
         self.ptr = C.c_void_p()
         check(lib().ramd_malloc(ctx.h, max(1, self.n) * self.dtype.itemsize, C.byref(self.ptr)))
+        ctx._adopt(self)
 
     @classmethod
     def from_numpy(cls, ctx, a):
@@ -113,9 +129,9 @@ class ParVector:
         check(lib().ramd_axpy(self.ctx.h, self.n, float(alpha), x.ptr, self.ptr))
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.ctx.h:
             lib().ramd_free(self.ctx.h, self.ptr)
-            self.ptr = C.c_void_p()
+        self.ptr = C.c_void_p()
 
     def __del__(self):
         try:
@@ -129,6 +145,7 @@ class ParCSRMatrix:
 
     def __init__(self, ctx, handle, owned=True):
         self.ctx, self.h, self.owned = ctx, handle, owned
+        ctx._adopt(self)
         nr, nc, nnz = C.c_int32(), C.c_int32(), C.c_int64()
         check(lib().ramd_csr_info(self.h, C.byref(nr), C.byref(nc), C.byref(nnz)))
         self.n_rows, self.n_cols, self.nnz = nr.value, nc.value, nnz.value
@@ -181,7 +198,7 @@ class ParCSRMatrix:
         check(lib().ramd_jacobi(self.h, x.ptr, b.ptr, tmp.ptr, num_sweeps, float(omega)))
 
     def free(self):
-        if self.h and self.owned:
+        if self.h and self.owned and self.ctx.h:
             lib().ramd_csr_destroy(self.h)
         self.h = C.c_void_p()
 
@@ -245,6 +262,7 @@ class ParRugeStubenSolver:
         check(lib().ramd_hier_setup(self.ctx.h, rowptr.size - 1, col.size, _ptr(rowptr, C.c_int32),
                                     _ptr(col, C.c_int32), _ptr(val, C.c_double), C.byref(o), C.byref(h)))
         self.h = h
+        self.ctx._adopt(self)
         nl = C.c_int()
         check(lib().ramd_hier_info(self.h, C.byref(nl), None, None, None))
         self.num_levels = nl.value
@@ -291,9 +309,9 @@ class ParRugeStubenSolver:
         return it.value
 
     def free(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib().ramd_hier_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -783,7 +783,9 @@ int ramd_fpcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double rt
         double relres = norm2_64(ctx, n, r) / scale;
         if (relres_hist) relres_hist[0] = relres;
         int it = 0;
-        if (relres > rtol) {
+        if (!std::isfinite(bnorm) || !std::isfinite(relres)) {
+            rc = RAMD_E_NUMERIC;
+        } else if (relres > rtol) {
             precondition(h, cycle_prec, r, z);
             RAMD_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
             double rz = dot64(ctx, n, r, z);
