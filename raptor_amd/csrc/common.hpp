@@ -69,7 +69,7 @@ struct ramd_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
     int64_t prof_min_nnz = -1;  // < 0: off
-    ProfSlot prof[8];
+    ProfSlot prof[10];
     std::vector<ProfPending> prof_pending;
     std::vector<hipEvent_t> prof_pool;
     // scratch for reductions: partial sums and a completion counter
@@ -97,6 +97,9 @@ struct ramd_csr_s {
     // host copy of rowptr (kept: used to build blocks and by setup)
     std::vector<int> h_rowptr;
     bool borrowed = false;   // owned by a hierarchy; ramd_csr_destroy refuses it
+    // every row stores its diagonal first or has none (move_diag layout,
+    // core/matrix.cpp:729-787); lets Jacobi read one column per row
+    bool diag_first = false;
 };
 
 namespace ramd {
@@ -104,12 +107,16 @@ namespace ramd {
 ramd_csr_s* csr_wrap(ramd_ctx ctx, int n_rows, int n_cols, int64_t nnz, int* rowptr_d, int* col_d,
                      double* val_d, const int* rowptr_host);
 void csr_free(ramd_csr_s* A);
+// True when every row has its diagonal first or no diagonal at all.
+bool host_diag_first(int n_rows, const int* rowptr, const int* col);
 // Builds (once) A->trans, the CSR of A^T with rows in ascending source-row order.
 ramd_csr_s* csr_transpose(ramd_csr_s* A);
 void csr_make_fp32(ramd_csr_s* A);
 
 // Kernel launchers (stream-ordered), FP64 and FP32.
-enum StreamOp { OP_SPMV = 0, OP_APPEND = 1, OP_RESIDUAL = 2, OP_JACOBI = 3 };
+// OP_ADD_INTO: y = b + A x with b a separate input (x <- x + P e written to a
+// second buffer); same arithmetic as OP_APPEND.
+enum StreamOp { OP_SPMV = 0, OP_APPEND = 1, OP_RESIDUAL = 2, OP_JACOBI = 3, OP_ADD_INTO = 4 };
 void launch_stream_op(const ramd_csr_s* A, int op, const double* x, const double* b, double* y,
                       double omega, hipStream_t s);
 void launch_stream_op32(const ramd_csr_s* A, int op, const float* x, const float* b, float* y,

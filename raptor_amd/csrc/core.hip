@@ -98,6 +98,14 @@ ramd_csr_s* csr_wrap(ramd_ctx ctx, int n_rows, int n_cols, int64_t nnz, int* row
     return A;
 }
 
+bool host_diag_first(int n_rows, const int* rp, const int* col)
+{
+    for (int i = 0; i < n_rows; i++)
+        for (int j = rp[i] + 1; j < rp[i + 1]; j++)
+            if (col[j] == i) return false;
+    return true;
+}
+
 void csr_free(ramd_csr_s* A)
 {
     if (!A) return;
@@ -321,6 +329,7 @@ int ramd_csr_create(ramd_ctx ctx, int32_t n_rows, int32_t n_cols, int64_t nnz, c
             RAMD_HIP(hipMemcpyAsync(d_val, val, sizeof(double) * nnz, hipMemcpyHostToDevice, ctx->stream));
         }
         *out = csr_wrap(ctx, n_rows, n_cols, nnz, d_rp, d_col, d_val, rowptr);
+        (*out)->diag_first = host_diag_first(n_rows, rowptr, col);
     });
 }
 

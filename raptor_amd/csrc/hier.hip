@@ -64,7 +64,9 @@ static ramd_csr_s* upload(ramd_ctx ctx, const Csr& M)
         RAMD_HIP(hipMemcpyAsync(c, M.col.data(), sizeof(int) * M.nnz(), hipMemcpyHostToDevice, ctx->stream));
         RAMD_HIP(hipMemcpyAsync(v, M.val.data(), sizeof(double) * M.nnz(), hipMemcpyHostToDevice, ctx->stream));
     }
-    return csr_wrap(ctx, M.n_rows, M.n_cols, M.nnz(), rp, c, v, M.rowptr.data());
+    ramd_csr_s* D = csr_wrap(ctx, M.n_rows, M.n_cols, M.nnz(), rp, c, v, M.rowptr.data());
+    D->diag_first = host_diag_first(M.n_rows, M.rowptr.data(), M.col.data());
+    return D;
 }
 
 static Csr download(ramd_csr_s* A)
@@ -246,9 +248,7 @@ static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_ze
     launch_stream_op(L.R, OP_SPMV, L.r, nullptr, C.b, 0.0, s);
     cycle64(h, l + 1, C.x, C.b, true);
     // x <- x + P e (CSR_append: row sum first, then added), into the other buffer
-    launch_stream_op(L.P, OP_SPMV, C.x, nullptr, L.r, 0.0, s);
-    hipLaunchKernelGGL((add_into<double>), grid1(L.n), dim3(256), 0, s, L.n, cur, L.r, other);
-    RAMD_HIP(hipGetLastError());
+    launch_stream_op(L.P, OP_ADD_INTO, C.x, cur, other, 0.0, s);
     std::swap(cur, other);
     for (int it = 0; it < sweeps; it++) {
         launch_stream_op(L.A, OP_JACOBI, cur, b, other, w, s);
@@ -287,9 +287,7 @@ static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero
     Level& C = h->levels[l + 1];
     launch_stream_op32(L.R, OP_SPMV, L.r32, nullptr, C.b32, 0.f, s);
     cycle32(h, l + 1, C.x32, C.b32, true);
-    launch_stream_op32(L.P, OP_SPMV, C.x32, nullptr, L.r32, 0.f, s);
-    hipLaunchKernelGGL((add_into<float>), grid1(L.n), dim3(256), 0, s, L.n, cur, L.r32, other);
-    RAMD_HIP(hipGetLastError());
+    launch_stream_op32(L.P, OP_ADD_INTO, C.x32, cur, other, 0.f, s);
     std::swap(cur, other);
     for (int it = 0; it < sweeps; it++) {
         launch_stream_op32(L.A, OP_JACOBI, cur, b, other, w, s);

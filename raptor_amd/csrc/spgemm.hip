@@ -476,6 +476,7 @@ ramd_csr_s* galerkin(ramd_csr_s* A, ramd_csr_s* P)
         RAMD_HIP(hipGetLastError());
     }
     RAMD_HIP(hipStreamSynchronize(Ac->ctx->stream));
+    Ac->diag_first = true;
     return Ac;
 }
 

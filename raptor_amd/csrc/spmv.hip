@@ -21,7 +21,7 @@ template <int OP, typename V, typename X>
 __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
     const int* __restrict__ rowptr, const int* __restrict__ col, const V* __restrict__ val,
     const int* __restrict__ blocks, int nb, const X* __restrict__ x, const X* __restrict__ b,
-    X* __restrict__ y, X omega)
+    X* __restrict__ y, X omega, int diag_first)
 {
     __shared__ X prod[kStreamCap];
     const int blk = xcd_remap(blockIdx.x, nb);
@@ -54,6 +54,7 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
         if (tid == 0) {
             if (OP == OP_SPMV || OP == OP_RESIDUAL) y[r0] = acc;
             else if (OP == OP_APPEND) y[r0] += acc;
+            else if (OP == OP_ADD_INTO) y[r0] = b[r0] + acc;
             else {
                 const X xo = x[r0];
                 y[r0] = (fabs(diag) > X(1e-16)) ? ((X(1.0) - omega) * xo) + (omega * ((b[r0] - acc) / diag)) : xo;
@@ -62,9 +63,34 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
         return;
     }
 
-    for (int j = tid; j < e - s; j += kBlockThreads) prod[j] = X(val[s + j]) * x[col[s + j]];
+    // phase 1: every lane issues all of its column/value loads, then all of
+    // its x gathers, before the first multiply (16 independent loads in
+    // flight per lane instead of one dependent chain)
+    constexpr int U = kStreamCap / kBlockThreads;
+    const int len = e - s;
+    int cc[U];
+    X vv[U], xx[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int j = tid + u * kBlockThreads;
+        if (j < len) {
+            cc[u] = col[s + j];
+            vv[u] = X(val[s + j]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int j = tid + u * kBlockThreads;
+        if (j < len) xx[u] = x[cc[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int j = tid + u * kBlockThreads;
+        if (j < len) prod[j] = vv[u] * xx[u];
+    }
     __syncthreads();
 
+    // phase 2: one lane per row, products added left to right
     const int row = r0 + tid;
     if (row >= r1) return;
     const int k0 = rowptr[row] - s, k1 = rowptr[row + 1] - s;
@@ -72,25 +98,34 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
         X acc = 0;
         for (int k = k0; k < k1; k++) acc += prod[k];
         y[row] = acc;
-    } else if (OP == OP_APPEND) {
+    } else if (OP == OP_APPEND || OP == OP_ADD_INTO) {
+        // CSR_append: the row sum is formed first, then added (spmv.cpp:96-101)
         X acc = 0;
         for (int k = k0; k < k1; k++) acc += prod[k];
-        y[row] += acc;
+        y[row] = (OP == OP_APPEND ? y[row] : b[row]) + acc;
     } else if (OP == OP_RESIDUAL) {
         X acc = b[row];
         for (int k = k0; k < k1; k++) acc -= prod[k];
         y[row] = acc;
     } else {
-        // relax.cpp:23-41: tmp (= x here) holds the old iterate, y the new
+        // relax.cpp:23-41: x holds the old iterate (tmp), y receives the new
         const X xo = x[row];
         if (k0 == k1) {
             y[row] = xo;
             return;
         }
         X acc = 0, diag = 0;
-        for (int k = k0; k < k1; k++) {
-            if (col[s + k] == row) diag = X(val[s + k]);
-            else acc += prod[k];
+        if (diag_first) {
+            // diagonal stored first, or absent (then the row keeps its value)
+            if (col[s + k0] == row) {
+                diag = X(val[s + k0]);
+                for (int k = k0 + 1; k < k1; k++) acc += prod[k];
+            }
+        } else {
+            for (int k = k0; k < k1; k++) {
+                if (col[s + k] == row) diag = X(val[s + k]);
+                else acc += prod[k];
+            }
         }
         y[row] = (fabs(diag) > X(1e-16)) ? ((X(1.0) - omega) * xo) + (omega * ((b[row] - acc) / diag)) : xo;
     }
@@ -140,19 +175,23 @@ static void launch_stream(const ramd_csr_s* A, const V* val, int op, const X* x,
     switch (op) {
     case OP_SPMV:
         hipLaunchKernelGGL((stream_kernel<OP_SPMV, V, X>), grid, block, 0, s, A->rowptr, A->col, val, A->blocks,
-                           nb, x, b, y, omega);
+                           nb, x, b, y, omega, (int)A->diag_first);
         break;
     case OP_APPEND:
         hipLaunchKernelGGL((stream_kernel<OP_APPEND, V, X>), grid, block, 0, s, A->rowptr, A->col, val,
-                           A->blocks, nb, x, b, y, omega);
+                           A->blocks, nb, x, b, y, omega, (int)A->diag_first);
         break;
     case OP_RESIDUAL:
         hipLaunchKernelGGL((stream_kernel<OP_RESIDUAL, V, X>), grid, block, 0, s, A->rowptr, A->col, val,
-                           A->blocks, nb, x, b, y, omega);
+                           A->blocks, nb, x, b, y, omega, (int)A->diag_first);
         break;
     case OP_JACOBI:
         hipLaunchKernelGGL((stream_kernel<OP_JACOBI, V, X>), grid, block, 0, s, A->rowptr, A->col, val,
-                           A->blocks, nb, x, b, y, omega);
+                           A->blocks, nb, x, b, y, omega, (int)A->diag_first);
+        break;
+    case OP_ADD_INTO:
+        hipLaunchKernelGGL((stream_kernel<OP_ADD_INTO, V, X>), grid, block, 0, s, A->rowptr, A->col, val,
+                           A->blocks, nb, x, b, y, omega, (int)A->diag_first);
         break;
     default:
         fail(RAMD_E_ARG, "unknown stream op %d", op);
@@ -377,7 +416,7 @@ int ramd_prof_enable(ramd_ctx ctx, int64_t min_nnz)
 int ramd_prof_read(ramd_ctx ctx, int op, int prec, int64_t* launches, double* ms, double* bytes)
 {
     return guarded([&] {
-        RAMD_CHECK(ctx && op >= 0 && op < 4 && (prec == RAMD_PREC_FP64 || prec == RAMD_PREC_FP32), RAMD_E_ARG,
+        RAMD_CHECK(ctx && op >= 0 && op < 5 && (prec == RAMD_PREC_FP64 || prec == RAMD_PREC_FP32), RAMD_E_ARG,
                    "ramd_prof_read: bad argument");
         prof_resolve(ctx);
         const ProfSlot& s = ctx->prof[op * 2 + prec];
