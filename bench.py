@@ -44,13 +44,16 @@ def vcycle_bytes(ml, sv):
       restrict  b_c = R r                          nnzP (sv+4) + 4(nc+1) + sv nf + sv nc
       interpolate x += P e                         nnzP (sv+4) + 4(nf+1) + sv nc + 2 sv nf
       post-smooth Jacobi                           nnz (sv+4) + 4(n+1) + 3 sv n
-    plus the coarse FP64 LU solve 8 nc^2 + 16 nc."""
+    plus the coarse FP64 LU solve 8 nc^2 + 16 nc.  A level held in the
+    stencil-diagonal form streams nd*n values and a 4-byte mask per row
+    instead of nnz (sv+4) + 4(n+1)."""
     n, nnzA, nnzP = ml.level_sizes()
     total = 0.0
     L = len(n)
     for l in range(L - 1):
         nf, nc = int(n[l]), int(n[l + 1])
-        a = nnzA[l] * (sv + 4) + 4 * (nf + 1)
+        nd = ml.level(l)[0].diagonals()
+        a = nd * nf * sv + 4 * nf if nd else nnzA[l] * (sv + 4) + 4 * (nf + 1)
         p = nnzP[l] * (sv + 4)
         total += 3 * sv * nf
         total += a + 3 * sv * nf
@@ -140,14 +143,16 @@ def main():
     t32 = (time.perf_counter() - t0) / args.steps
     prof = {}
     import ctypes as C
-    names = {0: "spmv", 1: "append", 2: "residual", 3: "jacobi"}
-    for op in range(4):
+    names = {0: "spmv", 1: "append", 2: "residual", 3: "jacobi", 4: "add_into"}
+    for op in range(5):
         for prec in (ra.PREC_FP64, ra.PREC_FP32):
-            nl_, ms, by = C.c_int64(), C.c_double(), C.c_double()
-            check(L.ramd_prof_read(ctx.h, op, prec, C.byref(nl_), C.byref(ms), C.byref(by)))
+            nl_, nd_, ms, by = C.c_int64(), C.c_int64(), C.c_double(), C.c_double()
+            check(L.ramd_prof_read(ctx.h, op, prec, C.byref(nl_), C.byref(nd_), C.byref(ms), C.byref(by)))
             if nl_.value:
+                kern = "dia_kernel" if nd_.value == nl_.value else ("stream_kernel" if nd_.value == 0 else "mixed")
                 prof[f"{names[op]}_{'fp32' if prec else 'fp64'}"] = dict(
-                    launches=nl_.value, ms=ms.value, bytes=by.value)
+                    kernel=kern, launches=nl_.value, ms=ms.value, bytes=by.value,
+                    gbs=round(by.value / (ms.value * 1e-3) / 1e9, 1))
     check(L.ramd_prof_enable(ctx.h, -1))
     log(f"FP32-cycle solve: {t32:.4f}s, iterations {its32}, relres {rel32:.3e}")
 
@@ -191,7 +196,7 @@ def main():
     per_launch_s = dom["ms"] / 1e3 / dom["launches"]
     achieved = per_launch_bytes / per_launch_s / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
-                    frac=round(achieved / PEAK_HBM_GBS, 4), traffic=None, kernel=f"stream_kernel:{dom_name}",
+                    frac=round(achieved / PEAK_HBM_GBS, 4), traffic=None, kernel=f"{dom['kernel']}:{dom_name}",
                     bytes_per_launch=per_launch_bytes, us_per_launch=round(per_launch_s * 1e6, 2))
     log(f"dominant kernel {dom_name}: {achieved:.0f} GB/s ({achieved / PEAK_HBM_GBS:.1%} of 8 TB/s)")
 

@@ -79,6 +79,12 @@ int ramd_csr_info(ramd_csr A, int32_t* n_rows, int32_t* n_cols, int64_t* nnz);
 /* Copies the device arrays back (any output pointer may be NULL). */
 int ramd_csr_download(ramd_csr A, int32_t* rowptr, int32_t* col, double* val);
 int ramd_csr_destroy(ramd_csr A);
+/* Number of stored diagonals when the matrix also has the stencil-diagonal
+ * form the SpMV family uses for stencil-structured operators (values per
+ * diagonal, a per-row presence mask, no column indices; built automatically
+ * for square matrices of >= 4096 rows with <= 32 distinct column offsets,
+ * <= 15% padding and one common per-row order); 0 when it is CSR only. */
+int ramd_csr_diagonals(ramd_csr A, int* nd);
 
 /* ------------------------------------------------------------------ */
 /* SpMV family (FP64; each row summed left to right in stored order, as  */
@@ -121,11 +127,13 @@ int ramd_axpy(ramd_ctx ctx, int64_t n, double alpha, const double* x_d, double* 
  * matrix with at least min_nnz stored entries is bracketed by hipEvents on
  * the context's stream; min_nnz < 0 turns timing off.  Resets the totals. */
 int ramd_prof_enable(ramd_ctx ctx, int64_t min_nnz);
-/* Totals for op (0 SpMV, 1 append, 2 residual, 3 Jacobi) and precision
- * (RAMD_PREC_FP64 / RAMD_PREC_FP32): launches, summed kernel milliseconds
- * and summed algorithmic bytes (matrix once, x once per column, vectors
- * once).  Synchronises the stream. */
-int ramd_prof_read(ramd_ctx ctx, int op, int prec, int64_t* launches, double* ms, double* bytes);
+/* Totals for op (0 SpMV, 1 append, 2 residual, 3 Jacobi, 4 x+Pe into a second
+ * vector) and precision (RAMD_PREC_FP64 / RAMD_PREC_FP32): launches, of which
+ * in the stencil-diagonal form, summed kernel milliseconds and summed
+ * algorithmic bytes (matrix once, x once per column, vectors once).
+ * Synchronises the stream. */
+int ramd_prof_read(ramd_ctx ctx, int op, int prec, int64_t* launches, int64_t* dia_launches, double* ms,
+                   double* bytes);
 
 /* ------------------------------------------------------------------ */
 /* Galerkin product (two-phase SpGEMM on the device)                     */
@@ -255,6 +263,11 @@ int ramd_host_strength(int32_t n, const int32_t* rowptr, const int32_t* col,
  * optional weights (NULL = the reference's rand() stream). */
 int ramd_host_split(int32_t n, const int32_t* s_rowptr, const int32_t* s_col,
                     int coarsen_type, const double* weights, int32_t* states);
+/* Serial Ruge-Stuben splitting split_rs(CSRMatrix*, states, false, second_pass),
+ * ruge_stuben/cf_splitting.cpp:277-322 (the building block of RS, Falgout
+ * and HMIS coarsening). */
+int ramd_host_split_rs_serial(int32_t n, const int32_t* s_rowptr, const int32_t* s_col,
+                              int second_pass, int32_t* states);
 /* Interpolation P (ruge_stuben/par_interpolation.cpp): direct :1453-1754,
  * mod_classical :991-1451, extended :280-989 (+ filter_interp :175-278). */
 int ramd_host_interp(int32_t n, const int32_t* rowptr, const int32_t* col, const double* val,

@@ -160,6 +160,12 @@ class ParCSRMatrix:
                                     _ptr(val, C.c_double), C.byref(h)))
         return cls(ctx, h)
 
+    def diagonals(self):
+        """Stored diagonals of the stencil-diagonal form (0: CSR only)."""
+        nd = C.c_int()
+        check(lib().ramd_csr_diagonals(self.h, C.byref(nd)))
+        return nd.value
+
     def to_csr(self):
         rp = np.empty(self.n_rows + 1, np.int32)
         col = np.empty(max(1, self.nnz), np.int32)
@@ -367,6 +373,16 @@ def split(s_rowptr, s_col, coarsen_type, weights=None):
     w = None if weights is None else _f64(weights)
     check(lib().ramd_host_split(n, _ptr(s_rowptr, C.c_int32), _ptr(s_col, C.c_int32), coarsen_type,
                                 None if w is None else _ptr(w, C.c_double), _ptr(states, C.c_int32)))
+    return states
+
+
+def split_rs_serial(s_rowptr, s_col, second_pass=True):
+    """split_rs(CSRMatrix*, states, false, second_pass) of the reference."""
+    s_rowptr, s_col = _i32(s_rowptr), _i32(s_col)
+    n = s_rowptr.size - 1
+    states = np.empty(n, np.int32)
+    check(lib().ramd_host_split_rs_serial(n, _ptr(s_rowptr, C.c_int32), _ptr(s_col, C.c_int32),
+                                          1 if second_pass else 0, _ptr(states, C.c_int32)))
     return states
 
 

@@ -79,6 +79,7 @@ _SIGS = {
     "ramd_csr_info": (P, PI32, PI32, PI64),
     "ramd_csr_download": (P, PI32, PI32, PD),
     "ramd_csr_destroy": (P,),
+    "ramd_csr_diagonals": (P, C.POINTER(C.c_int)),
     "ramd_spmv": (P, P, P),
     "ramd_spmv_append": (P, P, P),
     "ramd_residual": (P, P, P, P),
@@ -88,7 +89,7 @@ _SIGS = {
     "ramd_norm2": (P, I64, P, PD),
     "ramd_axpy": (P, I64, D, P, P),
     "ramd_prof_enable": (P, I64),
-    "ramd_prof_read": (P, C.c_int, C.c_int, PI64, PD, PD),
+    "ramd_prof_read": (P, C.c_int, C.c_int, PI64, PI64, PD, PD),
     "ramd_spgemm": (P, P, C.POINTER(P)),
     "ramd_spgemm_tn": (P, P, C.POINTER(P)),
     "ramd_galerkin": (P, P, C.POINTER(P)),
@@ -110,6 +111,7 @@ _SIGS = {
     "ramd_host_free": None,
     "ramd_host_strength": (I32, PI32, PI32, PD, C.c_int, D, PPI32, PPI32, PI64),
     "ramd_host_split": (I32, PI32, PI32, C.c_int, PD, PI32),
+    "ramd_host_split_rs_serial": (I32, PI32, PI32, C.c_int, PI32),
     "ramd_host_interp": (I32, PI32, PI32, PD, PI32, PI32, PI32, C.c_int, D, PI32, PPI32, PPI32, PPD, PI64),
     "ramd_host_gallery": (C.c_int, I32, D, D, C.c_uint64, PI32, PPI32, PPI32, PPD, PI64),
 }

@@ -57,6 +57,7 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb)
 // matrices, accumulated per (operation, precision).
 struct ProfSlot {
     int64_t launches = 0;
+    int64_t dia_launches = 0;  // of which in the stencil-diagonal form
     double bytes = 0;  // algorithmic bytes, summed over launches
     double ms = 0;     // resolved event time
 };
@@ -80,8 +81,30 @@ struct ramd_ctx_s {
     void set_current() const;
 };
 
+// Stencil-diagonal form of a matrix whose rows use few distinct column
+// offsets (col - row): values stored per diagonal (val[k * n + row], zero
+// where absent), a per-row presence mask, and the order in which every row
+// stores its diagonals (ascending, or diagonal first then ascending).  The
+// kernels visit a row's entries in that order, so sums are bit-identical to
+// the CSR ones; column indices are never read.
+constexpr int kDiaMax = 32;
+struct DiaArgs {
+    int off[kDiaMax];  // offset of diagonal k (ascending in k)
+    int ord[kDiaMax];  // visiting order: ord[kk] = diagonal index
+};
+struct Dia {
+    int nd = 0;
+    int center = -1;            // diagonal index of offset 0 (-1 if none)
+    DiaArgs args{};
+    unsigned* mask = nullptr;   // device, n
+    double* v64 = nullptr;      // device, nd * n
+    float* v32 = nullptr;       // device, nd * n (built with the FP32 copy)
+    bool v32_exact = false;     // every value is exactly representable in FP32
+};
+
 struct ramd_csr_s {
     ramd_ctx ctx = nullptr;
+    Dia* dia = nullptr;         // stencil-diagonal form (fine-level operators)
     int32_t n_rows = 0, n_cols = 0;
     int64_t nnz = 0;
     int* rowptr = nullptr;   // device, n_rows+1
@@ -109,6 +132,12 @@ ramd_csr_s* csr_wrap(ramd_ctx ctx, int n_rows, int n_cols, int64_t nnz, int* row
 void csr_free(ramd_csr_s* A);
 // True when every row has its diagonal first or no diagonal at all.
 bool host_diag_first(int n_rows, const int* rowptr, const int* col);
+// Builds A->dia from host arrays when the matrix qualifies (square, at least
+// kDiaMinRows rows, <= kDiaMax distinct offsets, padding <= 15%, every row in
+// one common diagonal order).  Returns whether it was built.
+constexpr int kDiaMinRows = 4096;
+bool dia_build(ramd_csr_s* A, const int* rowptr, const int* col, const double* val);
+void dia_free(Dia* d);
 // Builds (once) A->trans, the CSR of A^T with rows in ascending source-row order.
 ramd_csr_s* csr_transpose(ramd_csr_s* A);
 void csr_make_fp32(ramd_csr_s* A);
@@ -121,6 +150,11 @@ void launch_stream_op(const ramd_csr_s* A, int op, const double* x, const double
                       double omega, hipStream_t s);
 void launch_stream_op32(const ramd_csr_s* A, int op, const float* x, const float* b, float* y,
                         float omega, hipStream_t s);
+// FP64 arithmetic reading the FP32 copy of the values where every value is
+// exactly representable in FP32 (same results as launch_stream_op, half the
+// value bytes); falls back to launch_stream_op otherwise.
+void launch_stream_op_c(const ramd_csr_s* A, int op, const double* x, const double* b, double* y,
+                        double omega, hipStream_t s);
 
 // Deterministic device reductions; result left in *out_d (device scalar).
 void launch_dot(ramd_ctx ctx, int64_t n, const double* x, const double* y, double* out_d);

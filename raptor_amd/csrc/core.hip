@@ -1,5 +1,6 @@
 // Context, device memory, device CSR handles and the C entry points that
 // manage them.
+#include <algorithm>
 #include <cstdarg>
 #include <cstring>
 #include <new>
@@ -98,6 +99,103 @@ ramd_csr_s* csr_wrap(ramd_ctx ctx, int n_rows, int n_cols, int64_t nnz, int* row
     return A;
 }
 
+void dia_free(Dia* d)
+{
+    if (!d) return;
+    dfree(d->mask);
+    dfree(d->v64);
+    dfree(d->v32);
+    delete d;
+}
+
+bool dia_build(ramd_csr_s* A, const int* rp, const int* col, const double* val)
+{
+    const int n = A->n_rows;
+    if (A->n_rows != A->n_cols || n < kDiaMinRows || A->nnz == 0) return false;
+    // distinct offsets through a small open-addressing table
+    constexpr int T = 128;
+    long long key[T];
+    bool used[T] = {false};
+    int nd = 0;
+    std::vector<long long> offs;
+    auto find = [&](long long d) -> int {
+        unsigned h = (unsigned)((unsigned long long)(d * 0x9E3779B97F4A7C15ull) >> 57) & (T - 1);
+        while (used[h] && key[h] != d) h = (h + 1) & (T - 1);
+        return (int)h;
+    };
+    std::vector<int> slot_of(T, -1);
+    for (int i = 0; i < n; i++)
+        for (int j = rp[i]; j < rp[i + 1]; j++) {
+            const long long d = (long long)col[j] - i;
+            const int h = find(d);
+            if (!used[h]) {
+                if (nd == kDiaMax) return false;
+                used[h] = true;
+                key[h] = d;
+                offs.push_back(d);
+                nd++;
+            }
+        }
+    if ((double)nd * n > 1.15 * (double)A->nnz) return false;
+    std::sort(offs.begin(), offs.end());
+    for (int k = 0; k < nd; k++) slot_of[find(offs[k])] = k;
+    int center = -1;
+    for (int k = 0; k < nd; k++)
+        if (offs[k] == 0) center = k;
+    // rank of diagonal k in the ascending and the diagonal-first orders
+    std::vector<int> rank_asc(nd), rank_dfirst(nd);
+    for (int k = 0; k < nd; k++) {
+        rank_asc[k] = k;
+        rank_dfirst[k] = (k == center) ? -1 : k;
+    }
+    bool all_asc = true, all_dfirst = true;
+    std::vector<unsigned> mask((size_t)n, 0u);
+    std::vector<double> v((size_t)nd * n, 0.0);
+    bool exact32 = true;
+    for (int i = 0; i < n; i++) {
+        int prev_a = -2, prev_d = -2;
+        for (int j = rp[i]; j < rp[i + 1]; j++) {
+            const int k = slot_of[find((long long)col[j] - i)];
+            if (mask[i] & (1u << k)) return false;  // repeated column
+            mask[i] |= 1u << k;
+            v[(size_t)k * n + i] = val[j];
+            if ((double)(float)val[j] != val[j]) exact32 = false;
+            if (rank_asc[k] <= prev_a) all_asc = false;
+            if (rank_dfirst[k] <= prev_d) all_dfirst = false;
+            prev_a = rank_asc[k];
+            prev_d = rank_dfirst[k];
+        }
+    }
+    if (!all_asc && !all_dfirst) return false;
+    Dia* d = new Dia();
+    d->nd = nd;
+    d->center = center;
+    for (int k = 0; k < kDiaMax; k++) d->args.off[k] = k < nd ? (int)offs[k] : 0;
+    int kk = 0;
+    if (all_asc) {
+        for (int k = 0; k < nd; k++) d->args.ord[kk++] = k;
+    } else {
+        d->args.ord[kk++] = center;
+        for (int k = 0; k < nd; k++)
+            if (k != center) d->args.ord[kk++] = k;
+    }
+    for (; kk < kDiaMax; kk++) d->args.ord[kk] = 0;
+    d->v32_exact = exact32;
+    try {
+        d->mask = dalloc<unsigned>((size_t)n);
+        d->v64 = dalloc<double>((size_t)nd * n);
+        RAMD_HIP(hipMemcpyAsync(d->mask, mask.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice, A->ctx->stream));
+        RAMD_HIP(hipMemcpyAsync(d->v64, v.data(), sizeof(double) * (size_t)nd * n, hipMemcpyHostToDevice,
+                                A->ctx->stream));
+        RAMD_HIP(hipStreamSynchronize(A->ctx->stream));
+    } catch (...) {
+        dia_free(d);
+        throw;
+    }
+    A->dia = d;
+    return true;
+}
+
 bool host_diag_first(int n_rows, const int* rp, const int* col)
 {
     for (int i = 0; i < n_rows; i++)
@@ -115,6 +213,7 @@ void csr_free(ramd_csr_s* A)
     dfree(A->val);
     dfree(A->blocks);
     dfree(A->val32);
+    dia_free(A->dia);
     delete A;
 }
 
@@ -172,6 +271,13 @@ void csr_make_fp32(ramd_csr_s* A)
     if (A->nnz) {
         int grid = (int)std::min<int64_t>((A->nnz + 255) / 256, 8192);
         hipLaunchKernelGGL(to_fp32_kernel, dim3(grid), dim3(256), 0, A->ctx->stream, A->val, A->val32, A->nnz);
+        RAMD_HIP(hipGetLastError());
+    }
+    if (A->dia && !A->dia->v32) {
+        const int64_t m = (int64_t)A->dia->nd * A->n_rows;
+        A->dia->v32 = dalloc<float>((size_t)m);
+        int grid = (int)std::min<int64_t>((m + 255) / 256, 8192);
+        hipLaunchKernelGGL(to_fp32_kernel, dim3(grid), dim3(256), 0, A->ctx->stream, A->dia->v64, A->dia->v32, m);
         RAMD_HIP(hipGetLastError());
     }
 }
@@ -330,6 +436,7 @@ int ramd_csr_create(ramd_ctx ctx, int32_t n_rows, int32_t n_cols, int64_t nnz, c
         }
         *out = csr_wrap(ctx, n_rows, n_cols, nnz, d_rp, d_col, d_val, rowptr);
         (*out)->diag_first = host_diag_first(n_rows, rowptr, col);
+        dia_build(*out, rowptr, col, val);
     });
 }
 
@@ -353,6 +460,14 @@ int ramd_csr_download(ramd_csr A, int32_t* rowptr, int32_t* col, double* val)
         if (col && A->nnz) RAMD_HIP(hipMemcpyAsync(col, A->col, sizeof(int) * A->nnz, hipMemcpyDeviceToHost, s));
         if (val && A->nnz) RAMD_HIP(hipMemcpyAsync(val, A->val, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, s));
         RAMD_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int ramd_csr_diagonals(ramd_csr A, int* nd)
+{
+    return guarded([&] {
+        RAMD_CHECK(A && nd, RAMD_E_ARG, "ramd_csr_diagonals: NULL argument");
+        *nd = A->dia ? A->dia->nd : 0;
     });
 }
 

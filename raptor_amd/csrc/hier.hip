@@ -66,6 +66,7 @@ static ramd_csr_s* upload(ramd_ctx ctx, const Csr& M)
     }
     ramd_csr_s* D = csr_wrap(ctx, M.n_rows, M.n_cols, M.nnz(), rp, c, v, M.rowptr.data());
     D->diag_first = host_diag_first(M.n_rows, M.rowptr.data(), M.col.data());
+    dia_build(D, M.rowptr.data(), M.col.data(), M.val.data());
     return D;
 }
 
@@ -344,9 +345,16 @@ static Csr rs_level(ramd_hier_s* h, Csr& A, int level, const double* weights, st
                                               : host::classical_strength(A, o.strong_threshold);
     double t1 = now_s();
     switch (o.coarsen_type) {
+    case RAMD_RS:
+        // RS on the first three levels, Falgout below (par_ruge_stuben_solver.hpp:76-86)
+        if (level < 3) host::split_rs(S, states);
+        else host::split_falgout(S, weights, states);
+        break;
     case RAMD_CLJP: host::split_cljp(S, weights, states); break;
+    case RAMD_FALGOUT: host::split_falgout(S, weights, states); break;
     case RAMD_PMIS: host::split_pmis(S, weights, states); break;
-    default: fail(RAMD_E_ARG, "coarsen_type %d is not supported by this build", o.coarsen_type);
+    case RAMD_HMIS: host::split_hmis(S, weights, states); break;
+    default: fail(RAMD_E_ARG, "unknown coarsen_type %d", o.coarsen_type);
     }
     double t2 = now_s();
     Csr P;
@@ -773,11 +781,17 @@ int ramd_fpcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double rt
         hipStream_t s = ctx->stream;
         const int n = A->n_rows;
         RAMD_CHECK(n == h->levels[0].n, RAMD_E_ARG, "ramd_fpcg: A has %d rows, hierarchy %d", n, h->levels[0].n);
-        if (cycle_prec == RAMD_PREC_FP32) enable_fp32(h);
+        // FP32 form: the outer FP64 operator reads FP32-stored values where
+        // that is lossless (same FP64 results, half the value bytes)
+        auto outer = cycle_prec == RAMD_PREC_FP32 ? launch_stream_op_c : launch_stream_op;
+        if (cycle_prec == RAMD_PREC_FP32) {
+            enable_fp32(h);
+            csr_make_fp32(A);
+        }
         double *r = h->pr, *z = h->pz, *p = h->pp, *Ap = h->pAp, *r_old = h->pzo;
         const double bnorm = norm2_64(ctx, n, b_d);
         const double scale = bnorm > 0 ? bnorm : 1.0;
-        launch_stream_op(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
+        outer(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
         double relres = norm2_64(ctx, n, r) / scale;
         if (relres_hist) relres_hist[0] = relres;
         int it = 0;
@@ -788,7 +802,7 @@ int ramd_fpcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double rt
             RAMD_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
             double rz = dot64(ctx, n, r, z);
             while (it < max_iter) {
-                launch_stream_op(A, OP_SPMV, p, nullptr, Ap, 0.0, s);
+                outer(A, OP_SPMV, p, nullptr, Ap, 0.0, s);
                 const double pAp = dot64(ctx, n, p, Ap);
                 if (!(pAp > 0.0)) {
                     rc = std::isfinite(pAp) ? RAMD_E_INDEFINITE : RAMD_E_NUMERIC;
@@ -801,7 +815,7 @@ int ramd_fpcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double rt
                 relres = norm2_64(ctx, n, r) / scale;
                 if (relres <= rtol) {
                     // confirm with the true residual; continue from it if the recurrence drifted
-                    launch_stream_op(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
+                    outer(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
                     relres = norm2_64(ctx, n, r) / scale;
                     if (relres_hist) relres_hist[it] = relres;
                     if (relres <= rtol) break;

@@ -66,10 +66,25 @@ int ramd_host_split(int32_t n, const int32_t* s_rowptr, const int32_t* s_col, in
         else w = host::reference_weights(n, 0);
         std::vector<int> st;
         switch (coarsen_type) {
+        case RAMD_RS: host::split_rs(S, st); break;
         case RAMD_CLJP: host::split_cljp(S, w.data(), st); break;
+        case RAMD_FALGOUT: host::split_falgout(S, w.data(), st); break;
         case RAMD_PMIS: host::split_pmis(S, w.data(), st); break;
-        default: fail(RAMD_E_ARG, "ramd_host_split: coarsen_type %d not supported by this build", coarsen_type);
+        case RAMD_HMIS: host::split_hmis(S, w.data(), st); break;
+        default: fail(RAMD_E_ARG, "ramd_host_split: unknown coarsen_type %d", coarsen_type);
         }
+        std::copy(st.begin(), st.end(), states);
+    });
+}
+
+int ramd_host_split_rs_serial(int32_t n, const int32_t* s_rowptr, const int32_t* s_col, int second_pass,
+                              int32_t* states)
+{
+    return guarded([&] {
+        RAMD_CHECK(states, RAMD_E_ARG, "ramd_host_split_rs_serial: NULL states");
+        Csr S = make_csr(n, n, s_rowptr, s_col, nullptr);
+        std::vector<int> st;
+        host::split_rs_serial(S, st, false, second_pass != 0);
         std::copy(st.begin(), st.end(), states);
     });
 }

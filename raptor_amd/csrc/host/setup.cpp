@@ -10,22 +10,99 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
+#include <exception>
 #include <numeric>
 #include <stdexcept>
+#include <thread>
 
 namespace ramd {
 namespace host {
+
+// ---------------------------------------------------------------------------
+// row-parallel drivers: rows are independent in every routine below except
+// the splittings, so contiguous row ranges go to threads with private
+// scratch, and the pieces are concatenated in row order (same result as one
+// sequential pass).  RAMD_HOST_THREADS caps the thread count (default: the
+// hardware threads, at most 16).
+// ---------------------------------------------------------------------------
+static int host_threads(int64_t work)
+{
+    if (work < (1 << 16)) return 1;
+    int t = (int)std::thread::hardware_concurrency();
+    if (const char* e = std::getenv("RAMD_HOST_THREADS")) t = std::atoi(e);
+    return std::max(1, std::min(t, 16));
+}
+
+// runs fn(t, r0, r1) on T contiguous ranges of [0, n)
+template <class Fn>
+static void parallel_ranges(int n, int T, Fn fn)
+{
+    if (T <= 1) {
+        fn(0, 0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::vector<std::exception_ptr> err((size_t)T);
+    for (int t = 0; t < T; t++) {
+        const int r0 = (int)((int64_t)n * t / T), r1 = (int)((int64_t)n * (t + 1) / T);
+        th.emplace_back([&, t, r0, r1] {
+            try {
+                fn(t, r0, r1);
+            } catch (...) {
+                err[t] = std::current_exception();
+            }
+        });
+    }
+    for (auto& x : th) x.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+}
+
+struct RowChunk {
+    std::vector<int> len, col;
+    std::vector<double> val;
+};
+
+// Concatenates per-range row pieces into out (rowptr/col/val).
+static void assemble(int n, std::vector<RowChunk>& ch, Csr& out)
+{
+    const int T = (int)ch.size();
+    out.rowptr.assign((size_t)n + 1, 0);
+    std::vector<int64_t> base((size_t)T + 1, 0);
+    int r = 0;
+    for (int t = 0; t < T; t++) {
+        for (int l : ch[t].len) {
+            out.rowptr[r + 1] = out.rowptr[r] + l;
+            r++;
+        }
+        base[t + 1] = base[t] + (int64_t)ch[t].col.size();
+    }
+    if (base[T] > INT32_MAX) throw std::length_error("matrix exceeds 2^31 stored entries");
+    out.col.resize((size_t)base[T]);
+    out.val.resize((size_t)base[T]);
+    parallel_ranges(T, T, [&](int, int t0, int t1) {
+        for (int t = t0; t < t1; t++) {
+            if (ch[t].col.empty()) continue;
+            std::memcpy(out.col.data() + base[t], ch[t].col.data(), sizeof(int) * ch[t].col.size());
+            std::memcpy(out.val.data() + base[t], ch[t].val.data(), sizeof(double) * ch[t].val.size());
+            std::vector<int>().swap(ch[t].col);
+            std::vector<double>().swap(ch[t].val);
+        }
+    });
+}
 
 // ---------------------------------------------------------------------------
 // layout
 // ---------------------------------------------------------------------------
 void sort_rows(Csr& A)
 {
+    parallel_ranges(A.n_rows, host_threads(A.nnz()), [&](int, int r0, int r1) {
     std::vector<int> perm;
     std::vector<int> c2;
     std::vector<double> v2;
     const bool has_val = !A.val.empty();
-    for (int i = 0; i < A.n_rows; i++) {
+    for (int i = r0; i < r1; i++) {
         const int s = A.rowptr[i], e = A.rowptr[i + 1];
         bool sorted = true;
         for (int j = s + 1; j < e && sorted; j++) sorted = A.col[j - 1] <= A.col[j];
@@ -42,12 +119,14 @@ void sort_rows(Csr& A)
         std::copy(c2.begin(), c2.end(), A.col.begin() + s);
         if (has_val) std::copy(v2.begin(), v2.end(), A.val.begin() + s);
     }
+    });
 }
 
 void move_diag(Csr& A)
 {
     const bool has_val = !A.val.empty();
-    for (int i = 0; i < A.n_rows; i++) {
+    parallel_ranges(A.n_rows, host_threads(A.nnz()), [&](int, int r0, int r1) {
+    for (int i = r0; i < r1; i++) {
         const int s = A.rowptr[i], e = A.rowptr[i + 1];
         for (int j = s; j < e; j++) {
             if (A.col[j] == i) {
@@ -62,6 +141,7 @@ void move_diag(Csr& A)
             }
         }
     }
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -118,49 +198,56 @@ Csr classical_strength(Csr& A, double theta)
     Csr S;
     S.n_rows = A.n_rows;
     S.n_cols = A.n_cols;
-    S.rowptr.assign((size_t)A.n_rows + 1, 0);
-    S.col.reserve(A.col.size());
-    S.val.reserve(A.col.size());
-    for (int i = 0; i < A.n_rows; i++) {
-        int s = A.rowptr[i];
-        const int e = A.rowptr[i + 1];
-        if (e - s) {
-            double diag;
-            if (A.col[s] == i) {
-                diag = A.val[s];
-                s++;
-            } else {
-                diag = 0.0;
+    const int T = host_threads(A.nnz());
+    std::vector<RowChunk> ch((size_t)T);
+    parallel_ranges(A.n_rows, T, [&](int t, int r0, int r1) {
+        RowChunk& c = ch[t];
+        c.len.resize((size_t)(r1 - r0));
+        c.col.reserve((size_t)(A.rowptr[r1] - A.rowptr[r0]));
+        c.val.reserve((size_t)(A.rowptr[r1] - A.rowptr[r0]));
+        for (int i = r0; i < r1; i++) {
+            const size_t before = c.col.size();
+            int s = A.rowptr[i];
+            const int e = A.rowptr[i + 1];
+            if (e - s) {
+                double diag;
+                if (A.col[s] == i) {
+                    diag = A.val[s];
+                    s++;
+                } else {
+                    diag = 0.0;
+                }
+                double row_scale;
+                if (diag < 0.0) {
+                    row_scale = -RAND_MAX;
+                    for (int j = s; j < e; j++)
+                        if (A.val[j] > row_scale) row_scale = A.val[j];
+                } else {
+                    row_scale = RAND_MAX;
+                    for (int j = s; j < e; j++)
+                        if (A.val[j] < row_scale) row_scale = A.val[j];
+                }
+                const double threshold = row_scale * theta;
+                c.col.push_back(i);
+                c.val.push_back(diag);
+                if (diag < 0) {
+                    for (int j = s; j < e; j++)
+                        if (A.val[j] > threshold) {
+                            c.col.push_back(A.col[j]);
+                            c.val.push_back(A.val[j]);
+                        }
+                } else {
+                    for (int j = s; j < e; j++)
+                        if (A.val[j] < threshold) {
+                            c.col.push_back(A.col[j]);
+                            c.val.push_back(A.val[j]);
+                        }
+                }
             }
-            double row_scale;
-            if (diag < 0.0) {
-                row_scale = -RAND_MAX;
-                for (int j = s; j < e; j++)
-                    if (A.val[j] > row_scale) row_scale = A.val[j];
-            } else {
-                row_scale = RAND_MAX;
-                for (int j = s; j < e; j++)
-                    if (A.val[j] < row_scale) row_scale = A.val[j];
-            }
-            const double threshold = row_scale * theta;
-            S.col.push_back(i);
-            S.val.push_back(diag);
-            if (diag < 0) {
-                for (int j = s; j < e; j++)
-                    if (A.val[j] > threshold) {
-                        S.col.push_back(A.col[j]);
-                        S.val.push_back(A.val[j]);
-                    }
-            } else {
-                for (int j = s; j < e; j++)
-                    if (A.val[j] < threshold) {
-                        S.col.push_back(A.col[j]);
-                        S.val.push_back(A.val[j]);
-                    }
-            }
+            c.len[i - r0] = (int)(c.col.size() - before);
         }
-        S.rowptr[i + 1] = (int)S.col.size();
-    }
+    });
+    assemble(A.n_rows, ch, S);
     return S;
 }
 
@@ -230,7 +317,7 @@ Csr symmetric_strength(Csr& A, double theta)
 // C/F splitting, ruge_stuben/par_cf_splitting.cpp
 // ---------------------------------------------------------------------------
 
-// set_initial_states :115-131
+// set_initial_states :114-131
 static void set_initial_states(const Csr& S, std::vector<int>& states)
 {
     states.assign((size_t)S.n_rows, kUnassigned);
@@ -238,7 +325,7 @@ static void set_initial_states(const Csr& S, std::vector<int>& states)
         states[i] = (S.rowptr[i + 1] - S.rowptr[i] > 1) ? kUnassigned : kNoNeighbors;
 }
 
-// transpose :150-230 -- CSC of S without each row's leading diagonal, rows
+// transpose :155-228 -- CSC of S without each row's leading diagonal, rows
 // ascending within a column.
 static void transpose_s(const Csr& S, std::vector<int>& col_ptr, std::vector<int>& col_idx)
 {
@@ -454,7 +541,150 @@ static void pmis_main_loop(Csr& S, std::vector<int>& states, const double* rand_
     }
 }
 
-// split_cljp :30-46
+// ---------------------------------------------------------------------------
+// serial Ruge-Stuben splitting, ruge_stuben/cf_splitting.cpp
+// ---------------------------------------------------------------------------
+
+// rs_first_pass cf_splitting.cpp:69-209: columns kept in buckets by weight
+// (number of rows that strongly depend on them); positions are visited from
+// the heaviest end, and each new C point moves the affected columns between
+// adjacent buckets by swapping them to the bucket boundary.
+static void rs_first_pass(const Csr& S, const std::vector<int>& col_ptr, const std::vector<int>& col_idx,
+                          std::vector<int>& w, std::vector<int>& states)
+{
+    const int nc = S.n_cols;
+    std::vector<int> wptr((size_t)nc + 1, 0), wsize((size_t)nc, 0), pos_col((size_t)nc), col_pos((size_t)nc);
+    for (int i = 0; i < nc; i++) wsize[w[i]]++;
+    for (int i = 0; i < nc; i++) {
+        wptr[i + 1] = wptr[i] + wsize[i];
+        wsize[i] = 0;
+    }
+    for (int i = 0; i < nc; i++) {
+        const int idx = wptr[w[i]] + wsize[w[i]]++;
+        pos_col[idx] = i;
+        col_pos[i] = idx;
+    }
+    auto swap_pos = [&](int a, int b) {
+        col_pos[pos_col[a]] = b;
+        col_pos[pos_col[b]] = a;
+        std::swap(pos_col[a], pos_col[b]);
+    };
+    for (int i = nc - 1; i >= 0; i--) {
+        const int c = pos_col[i];
+        wsize[w[c]]--;
+        if (states[c] != kUnassigned) continue;
+        states[c] = kSelected;
+        for (int j = col_ptr[c]; j < col_ptr[c + 1]; j++) {
+            const int idx = col_idx[j];
+            if (states[idx] != kUnassigned) continue;
+            states[idx] = kUnselected;
+            int ks = S.rowptr[idx];
+            const int ke = S.rowptr[idx + 1];
+            if (ks < ke && S.col[ks] == idx) ks++;
+            for (int k = ks; k < ke; k++) {
+                const int ik = S.col[k];
+                if (states[ik] != kUnassigned) continue;
+                const int wk = w[ik];
+                if (wk >= nc - 1) continue;
+                const int old_pos = col_pos[ik];
+                const int new_pos = wptr[wk] + wsize[wk] - 1;
+                swap_pos(old_pos, new_pos);
+                wsize[wk] -= 1;
+                wsize[wk + 1] += 1;
+                wptr[wk + 1] = new_pos;
+                w[ik]++;
+            }
+        }
+        int s = S.rowptr[c];
+        const int e = S.rowptr[c + 1];
+        if (s < e && S.col[s] == c) s++;
+        for (int j = s; j < e; j++) {
+            const int idx = S.col[j];
+            if (states[idx] != kUnassigned) continue;
+            const int wi = w[idx];
+            if (wi == 0) continue;
+            const int old_pos = col_pos[idx];
+            const int new_pos = wptr[wi];
+            swap_pos(old_pos, new_pos);
+            wsize[wi] -= 1;
+            wsize[wi - 1] += 1;
+            wptr[wi] += 1;
+            wptr[wi - 1] = wptr[wi] - wsize[wi - 1];
+            w[idx]--;
+        }
+    }
+}
+
+// rs_second_pass cf_splitting.cpp:211-261: every F point whose strong F
+// neighbours share no C point with it promotes them (rows include their
+// diagonal here, so an F point without strong C neighbours promotes itself).
+static void rs_second_pass(const Csr& S, std::vector<int>& states)
+{
+    std::vector<int> row_coarse((size_t)S.n_rows, -1);
+    for (int i = 0; i < S.n_rows; i++) {
+        if (states[i] == kSelected) continue;
+        const int s = S.rowptr[i], e = S.rowptr[i + 1];
+        for (int j = s; j < e; j++)
+            if (states[S.col[j]] == kSelected) row_coarse[S.col[j]] = i;
+        for (int j = s; j < e; j++) {
+            const int c = S.col[j];
+            if (states[c] != kUnselected) continue;
+            const int ks = S.rowptr[c], ke = S.rowptr[c + 1];
+            if (ks == ke) continue;
+            bool connection = false;
+            for (int k = ks; k < ke; k++)
+                if (row_coarse[S.col[k]] == i) {
+                    connection = true;
+                    break;
+                }
+            if (!connection) {
+                row_coarse[c] = i;
+                states[c] = kSelected;
+            }
+        }
+    }
+}
+
+// split_rs(CSRMatrix*) cf_splitting.cpp:277-322
+void split_rs_serial(Csr& S, std::vector<int>& states, bool prev_states, bool second_pass)
+{
+    move_diag(S);
+    if (!prev_states) states.assign((size_t)S.n_rows, kUnassigned);
+    std::vector<int> col_ptr, col_idx;
+    transpose_s(S, col_ptr, col_idx);
+    std::vector<int> w((size_t)S.n_rows, 0);
+    for (int i = 0; i < S.n_cols; i++) w[i] = col_ptr[i + 1] - col_ptr[i];
+    rs_first_pass(S, col_ptr, col_idx, w, states);
+    if (second_pass) rs_second_pass(S, states);
+}
+
+// split_rs(ParCSRMatrix*) par_cf_splitting.cpp:7-23
+void split_rs(Csr& S, std::vector<int>& states)
+{
+    set_initial_states(S, states);
+    split_rs_serial(S, states, true, true);
+}
+
+// split_falgout par_cf_splitting.cpp:50-73 (reset_boundaries marks nothing
+// at one rank: there are no off_proc columns or send indices)
+void split_falgout(Csr& S, const double* weights, std::vector<int>& states)
+{
+    move_diag(S);
+    set_initial_states(S, states);
+    split_rs_serial(S, states, true, true);
+    cljp_main_loop(S, states, weights);
+}
+
+// split_hmis par_cf_splitting.cpp:91-112
+void split_hmis(Csr& S, const double* weights, std::vector<int>& states)
+{
+    move_diag(S);
+    set_initial_states(S, states);
+    split_rs_serial(S, states, true, false);
+    pmis_main_loop(S, states, weights);
+}
+
+// split_cljp :32-48
 void split_cljp(Csr& S, const double* weights, std::vector<int>& states)
 {
     move_diag(S);
@@ -462,7 +692,7 @@ void split_cljp(Csr& S, const double* weights, std::vector<int>& states)
     cljp_main_loop(S, states, weights);
 }
 
-// split_pmis :69-84
+// split_pmis :75-89
 void split_pmis(Csr& S, const double* weights, std::vector<int>& states)
 {
     move_diag(S);
@@ -562,25 +792,30 @@ Csr mod_classical_interpolation(Csr& A, Csr& S, const std::vector<int>& states)
     const int n = A.n_rows;
     int nc;
     std::vector<int> cmap = coarse_index(states, false, nc);
-    std::vector<int> pos((size_t)A.n_cols, -1);
     Csr P;
     P.n_rows = n;
     P.n_cols = nc;
-    P.rowptr.assign((size_t)n + 1, 0);
-    for (int i = 0; i < n; i++) {
+    const int T = host_threads(A.nnz());
+    std::vector<RowChunk> ch((size_t)T);
+    parallel_ranges(n, T, [&](int t, int r0, int r1) {
+    RowChunk& Pc = ch[t];
+    Pc.len.resize((size_t)(r1 - r0));
+    std::vector<int> pos((size_t)A.n_cols, -1);
+    for (int i = r0; i < r1; i++) {
+        const int row_first = (int)Pc.col.size();
         if (states[i] == kSelected) {
-            P.col.push_back(cmap[i]);
-            P.val.push_back(1);
-            P.rowptr[i + 1] = (int)P.col.size();
+            Pc.col.push_back(cmap[i]);
+            Pc.val.push_back(1);
+            Pc.len[i - r0] = 1;
             continue;
         }
-        const int row_start = (int)P.col.size();
+        const int row_start = (int)Pc.col.size();
         for (int j = S.rowptr[i] + 1; j < S.rowptr[i + 1]; j++) {
             const int c = S.col[j];
             if (states[c] == kSelected) {
-                pos[c] = (int)P.col.size();
-                P.col.push_back(cmap[c]);
-                P.val.push_back(S.val[j]);
+                pos[c] = (int)Pc.col.size();
+                Pc.col.push_back(cmap[c]);
+                Pc.val.push_back(S.val[j]);
             }
         }
         int start = A.rowptr[i];
@@ -614,7 +849,7 @@ Csr mod_classical_interpolation(Csr& A, Csr& S, const std::vector<int>& states)
                         if (states[ck] == kSelected) {
                             const double vk = A.val[k];
                             const int idx = pos[ck];
-                            if (vk * sign < 0 && idx >= row_start) P.val[idx] += (coarse_sum * vk);
+                            if (vk * sign < 0 && idx >= row_start) Pc.val[idx] += (coarse_sum * vk);
                         }
                     }
                 }
@@ -622,46 +857,60 @@ Csr mod_classical_interpolation(Csr& A, Csr& S, const std::vector<int>& states)
                 weak_sum += v;
             }
         }
-        P.rowptr[i + 1] = (int)P.col.size();
-        for (int j = P.rowptr[i]; j < P.rowptr[i + 1]; j++) P.val[j] /= -weak_sum;
+        const int row_end = (int)Pc.col.size();
+        for (int j = row_first; j < row_end; j++) Pc.val[j] /= -weak_sum;
+        Pc.len[i - r0] = row_end - row_first;
     }
+    });
+    assemble(n, ch, P);
     return P;
 }
 
-// filter_interp :175-278
-void filter_interp(Csr& P, const double filter_threshold)
+// filter_interp :175-278 over rows given by their lengths, compacting in
+// place: entries below filter * max|p_ij| are dropped and the survivors are
+// rescaled to keep the row sum.
+static void filter_rows(std::vector<int>& col, std::vector<double>& val, std::vector<int>& len,
+                        const double filter_threshold)
 {
     if (!(filter_threshold > kZeroTol && filter_threshold <= 1)) return;
     int row_start = 0, ctr = 0;
-    for (int i = 0; i < P.n_rows; i++) {
+    for (size_t i = 0; i < len.size(); i++) {
         const int prev_ctr = ctr;
-        const int row_end = P.rowptr[i + 1];
+        const int row_end = row_start + len[i];
         double row_max = 0;
         for (int j = row_start; j < row_end; j++) {
-            const double a = fabs(P.val[j]);
+            const double a = fabs(val[j]);
             if (a > row_max) row_max = a;
         }
         row_max *= filter_threshold;
         double row_sum = 0, remain_sum = 0;
         for (int j = row_start; j < row_end; j++) {
-            const double v = P.val[j];
+            const double v = val[j];
             row_sum += v;
             if (fabs(v) >= row_max) {
-                P.col[ctr] = P.col[j];
-                P.val[ctr] = v;
+                col[ctr] = col[j];
+                val[ctr] = v;
                 ctr++;
                 remain_sum += v;
             }
         }
         if (fabs(remain_sum) > kZeroTol && fabs(row_sum - remain_sum) > kZeroTol) {
             const double row_scale = row_sum / remain_sum;
-            for (int j = prev_ctr; j < ctr; j++) P.val[j] *= row_scale;
+            for (int j = prev_ctr; j < ctr; j++) val[j] *= row_scale;
         }
-        P.rowptr[i + 1] = ctr;
+        len[i] = ctr - prev_ctr;
         row_start = row_end;
     }
-    P.col.resize(ctr);
-    P.val.resize(ctr);
+    col.resize(ctr);
+    val.resize(ctr);
+}
+
+void filter_interp(Csr& P, const double filter_threshold)
+{
+    std::vector<int> len((size_t)P.n_rows);
+    for (int i = 0; i < P.n_rows; i++) len[i] = P.rowptr[i + 1] - P.rowptr[i];
+    filter_rows(P.col, P.val, len, filter_threshold);
+    for (int i = 0; i < P.n_rows; i++) P.rowptr[i + 1] = P.rowptr[i] + len[i];
 }
 
 // extended_interpolation :280-989 (num_variables == 1).  Row entries are
@@ -673,14 +922,18 @@ Csr extended_interpolation(Csr& A, Csr& S, const std::vector<int>& states, doubl
     const int n = A.n_rows;
     int nc;
     std::vector<int> cmap = coarse_index(states, false, nc);
-    std::vector<int> pos((size_t)A.n_cols, -1);
     Csr P;
     P.n_rows = n;
     P.n_cols = nc;
-    P.rowptr.assign((size_t)n + 1, 0);
-    // upper bound of P's size (:531-566)
+    const int T = host_threads(A.nnz());
+    std::vector<RowChunk> ch((size_t)T);
+    parallel_ranges(n, T, [&](int t, int r0, int r1) {
+    RowChunk& Pc = ch[t];
+    Pc.len.resize((size_t)(r1 - r0));
+    std::vector<int> pos((size_t)A.n_cols, -1);
+    // upper bound of this range's entries (:531-566)
     int64_t bound = 0;
-    for (int i = 0; i < n; i++) {
+    for (int i = r0; i < r1; i++) {
         if (states[i] == kUnselected) {
             bound += S.rowptr[i + 1] - S.rowptr[i];
             for (int j = S.rowptr[i] + 1; j < S.rowptr[i + 1]; j++) {
@@ -691,39 +944,41 @@ Csr extended_interpolation(Csr& A, Csr& S, const std::vector<int>& states, doubl
             bound++;
         }
     }
-    P.col.reserve((size_t)bound);
-    P.val.reserve((size_t)bound);
-    for (int i = 0; i < n; i++) {
+    Pc.col.reserve((size_t)bound);
+    Pc.val.reserve((size_t)bound);
+    for (int i = r0; i < r1; i++) {
         if (states[i] != kUnselected) {
             if (states[i] == kSelected) {
-                P.col.push_back(cmap[i]);
-                P.val.push_back(1);
+                Pc.col.push_back(cmap[i]);
+                Pc.val.push_back(1);
+                Pc.len[i - r0] = 1;
+            } else {
+                Pc.len[i - r0] = 0;
             }
-            P.rowptr[i + 1] = (int)P.col.size();
             continue;
         }
-        const int row_start = (int)P.col.size();
+        const int row_start = (int)Pc.col.size();
         for (int j = S.rowptr[i] + 1; j < S.rowptr[i + 1]; j++) {
             const int c = S.col[j];
             if (states[c] == kSelected) {
                 if (pos[c] < row_start) {
-                    pos[c] = (int)P.col.size();
-                    P.col.push_back(cmap[c]);
-                    P.val.push_back(0.0);
+                    pos[c] = (int)Pc.col.size();
+                    Pc.col.push_back(cmap[c]);
+                    Pc.val.push_back(0.0);
                 }
             } else if (states[c] == kUnselected) {
                 for (int k = S.rowptr[c] + 1; k < S.rowptr[c + 1]; k++) {
                     const int ck = S.col[k];
                     if (states[ck] == kSelected && pos[ck] < row_start) {
-                        pos[ck] = (int)P.col.size();
-                        P.col.push_back(cmap[ck]);
-                        P.val.push_back(0.0);
+                        pos[ck] = (int)Pc.col.size();
+                        Pc.col.push_back(cmap[ck]);
+                        Pc.val.push_back(0.0);
                     }
                 }
             }
         }
-        pos[i] = (int)P.col.size();
-        const int row_end = (int)P.col.size();
+        pos[i] = (int)Pc.col.size();
+        const int row_end = (int)Pc.col.size();
         int start = A.rowptr[i];
         const int end = A.rowptr[i + 1];
         double weak_sum = A.val[start++];
@@ -734,7 +989,7 @@ Csr extended_interpolation(Csr& A, Csr& S, const std::vector<int>& states, doubl
             const double v = A.val[j];
             const int idx = pos[c];
             if (idx >= row_start) {
-                P.val[idx] += v;
+                Pc.val[idx] += v;
                 if (ctr < end_S && S.col[ctr] == c) ctr++;
             } else if (ctr < end_S && S.col[ctr] == c) {
                 ctr++;
@@ -758,7 +1013,7 @@ Csr extended_interpolation(Csr& A, Csr& S, const std::vector<int>& states, doubl
                         const int pk = pos[ck];
                         if (vk * sign < 0 && pk >= row_start) {
                             if (ck == i) weak_sum += (col_sum * vk);
-                            else P.val[pk] += (col_sum * vk);
+                            else Pc.val[pk] += (col_sum * vk);
                         }
                     }
                 }
@@ -767,11 +1022,15 @@ Csr extended_interpolation(Csr& A, Csr& S, const std::vector<int>& states, doubl
             }
         }
         if (fabs(weak_sum) > kZeroTol)
-            for (int j = row_start; j < row_end; j++) P.val[j] /= -weak_sum;
+            for (int j = row_start; j < row_end; j++) Pc.val[j] /= -weak_sum;
         pos[i] = -1;
-        P.rowptr[i + 1] = row_end;
+        Pc.len[i - r0] = row_end - row_start;
     }
-    filter_interp(P, filter);
+    // the filter runs after every row of the range exists (row-local, like
+    // the reference's post-pass), so no stored position is invalidated early
+    filter_rows(Pc.col, Pc.val, Pc.len, filter);
+    });
+    assemble(n, ch, P);
     return P;
 }
 

@@ -131,6 +131,100 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
     }
 }
 
+// Stencil-diagonal kernels: one lane per row, diagonals visited in the
+// row's stored order (a.ord).  Every load is issued unconditionally -- a
+// diagonal absent from the row reads its own plane (zero) and x[row] -- and
+// the accumulation skips it with a select, so the operation sequence of each
+// row is exactly the CSR row loop's and no out-of-range x is touched.
+template <int OP, typename V, typename X, int ND>
+__global__ __launch_bounds__(kBlockThreads) void dia_kernel(int n, DiaArgs a, int nd, int center,
+                                                           const unsigned* __restrict__ mask, const V* __restrict__ val,
+                                                           const X* __restrict__ x, const X* __restrict__ b,
+                                                           X* __restrict__ y, X omega)
+{
+    const int row = xcd_remap(blockIdx.x, gridDim.x) * kBlockThreads + threadIdx.x;
+    if (row >= n) return;
+    const unsigned m = mask[row];
+    X acc = (OP == OP_RESIDUAL) ? b[row] : X(0);
+    X diag = 0;
+    constexpr int CH = 9;
+#pragma unroll
+    for (int c0 = 0; c0 < ND; c0 += CH) {
+        V vv[CH];
+        X xx[CH];
+        bool on[CH];
+        int kd[CH];
+#pragma unroll
+        for (int t = 0; t < CH; t++) {
+            const int kk = c0 + t;
+            if (kk < ND) {
+                const bool live = (ND == kDiaMax) ? (kk < nd) : true;
+                const int k = live ? a.ord[kk] : 0;
+                kd[t] = k;
+                on[t] = live && ((m >> k) & 1u);
+                vv[t] = val[(size_t)k * n + row];
+                xx[t] = x[on[t] ? row + a.off[k] : row];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < CH; t++) {
+            if (c0 + t < ND) {
+                const X p = X(vv[t]) * xx[t];
+                if (OP == OP_JACOBI) {
+                    if (kd[t] == center) diag = on[t] ? X(vv[t]) : diag;
+                    else acc = on[t] ? acc + p : acc;
+                } else if (OP == OP_RESIDUAL) {
+                    acc = on[t] ? acc - p : acc;
+                } else {
+                    acc = on[t] ? acc + p : acc;
+                }
+            }
+        }
+    }
+    if (OP == OP_SPMV || OP == OP_RESIDUAL) {
+        y[row] = acc;
+    } else if (OP == OP_APPEND) {
+        y[row] += acc;
+    } else if (OP == OP_ADD_INTO) {
+        y[row] = b[row] + acc;
+    } else {
+        const X xo = x[row];
+        y[row] = (m != 0u && fabs(diag) > X(1e-16)) ? ((X(1.0) - omega) * xo) + (omega * ((b[row] - acc) / diag))
+                                                   : xo;
+    }
+}
+
+template <int OP, typename V, typename X>
+static void launch_dia_op(const ramd_csr_s* A, const V* val, const X* x, const X* b, X* y, X omega, hipStream_t s)
+{
+    const Dia* d = A->dia;
+    const int n = A->n_rows;
+    dim3 grid((n + kBlockThreads - 1) / kBlockThreads), block(kBlockThreads);
+#define DIA_LAUNCH(NDV)                                                                                         \
+    hipLaunchKernelGGL((dia_kernel<OP, V, X, NDV>), grid, block, 0, s, n, d->args, d->nd, d->center, d->mask, val, \
+                       x, b, y, omega)
+    if (d->nd == 27) DIA_LAUNCH(27);
+    else if (d->nd == 9) DIA_LAUNCH(9);
+    else if (d->nd == 7) DIA_LAUNCH(7);
+    else DIA_LAUNCH(kDiaMax);
+#undef DIA_LAUNCH
+    RAMD_HIP(hipGetLastError());
+}
+
+template <typename V, typename X>
+static void launch_dia(const ramd_csr_s* A, const V* val, int op, const X* x, const X* b, X* y, X omega,
+                       hipStream_t s)
+{
+    switch (op) {
+    case OP_SPMV: launch_dia_op<OP_SPMV>(A, val, x, b, y, omega, s); break;
+    case OP_APPEND: launch_dia_op<OP_APPEND>(A, val, x, b, y, omega, s); break;
+    case OP_RESIDUAL: launch_dia_op<OP_RESIDUAL>(A, val, x, b, y, omega, s); break;
+    case OP_JACOBI: launch_dia_op<OP_JACOBI>(A, val, x, b, y, omega, s); break;
+    case OP_ADD_INTO: launch_dia_op<OP_ADD_INTO>(A, val, x, b, y, omega, s); break;
+    default: fail(RAMD_E_ARG, "unknown stream op %d", op);
+    }
+}
+
 static hipEvent_t prof_event(ramd_ctx ctx)
 {
     if (!ctx->prof_pool.empty()) {
@@ -146,17 +240,22 @@ static hipEvent_t prof_event(ramd_ctx ctx)
 // Algorithmic bytes of one launch: the matrix streamed once, x gathered with
 // ideal reuse (n_cols entries), the output written once; residual, Jacobi
 // and append read one more n-vector (b, or y for append).
-static double op_bytes(const ramd_csr_s* A, int op, int sv)
+// In the stencil-diagonal form the matrix costs nd * n values (padding
+// included) plus a 4-byte mask per row, and no column indices.
+static double op_bytes(const ramd_csr_s* A, int op, int sv, int sx, bool dia)
 {
-    const double si = 4.0;
-    double b = (double)A->nnz * (sv + si) + 4.0 * (A->n_rows + 1) + (double)sv * A->n_cols + (double)sv * A->n_rows;
-    if (op != OP_SPMV) b += (double)sv * A->n_rows;
+    double b;
+    if (dia) b = (double)A->dia->nd * A->n_rows * sv + 4.0 * A->n_rows;
+    else b = (double)A->nnz * (sv + 4.0) + 4.0 * (A->n_rows + 1);
+    b += (double)sx * A->n_cols + (double)sx * A->n_rows;
+    if (op != OP_SPMV) b += (double)sx * A->n_rows;
     return b;
 }
 
+// val: CSR values; dval: stencil-diagonal values (nullptr -> CSR kernel)
 template <typename V, typename X>
-static void launch_stream(const ramd_csr_s* A, const V* val, int op, const X* x, const X* b, X* y, X omega,
-                          hipStream_t s)
+static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int op, const X* x, const X* b, X* y,
+                          X omega, hipStream_t s)
 {
     if (A->n_rows == 0) return;
     const int nb = A->n_blocks;
@@ -169,8 +268,17 @@ static void launch_stream(const ramd_csr_s* A, const V* val, int op, const X* x,
         pend.end = prof_event(ctx);
         pend.slot = op * 2 + (sizeof(X) == 4 ? 1 : 0);
         RAMD_HIP(hipEventRecord(pend.beg, s));
-        ctx->prof[pend.slot].bytes += op_bytes(A, op, (int)sizeof(V));
+        ctx->prof[pend.slot].bytes += op_bytes(A, op, (int)sizeof(V), (int)sizeof(X), dval != nullptr);
         ctx->prof[pend.slot].launches++;
+        if (dval) ctx->prof[pend.slot].dia_launches++;
+    }
+    if (dval) {
+        launch_dia(A, dval, op, x, b, y, omega, s);
+        if (prof) {
+            RAMD_HIP(hipEventRecord(pend.end, s));
+            ctx->prof_pending.push_back(pend);
+        }
+        return;
     }
     switch (op) {
     case OP_SPMV:
@@ -221,14 +329,24 @@ static void prof_resolve(ramd_ctx ctx)
 void launch_stream_op(const ramd_csr_s* A, int op, const double* x, const double* b, double* y, double omega,
                       hipStream_t s)
 {
-    launch_stream<double, double>(A, A->val, op, x, b, y, omega, s);
+    launch_stream<double, double>(A, A->val, A->dia ? A->dia->v64 : nullptr, op, x, b, y, omega, s);
+}
+
+void launch_stream_op_c(const ramd_csr_s* A, int op, const double* x, const double* b, double* y, double omega,
+                        hipStream_t s)
+{
+    if (A->dia && A->dia->v32 && A->dia->v32_exact)
+        launch_stream<float, double>(A, nullptr, A->dia->v32, op, x, b, y, omega, s);
+    else
+        launch_stream_op(A, op, x, b, y, omega, s);
 }
 
 void launch_stream_op32(const ramd_csr_s* A, int op, const float* x, const float* b, float* y, float omega,
                         hipStream_t s)
 {
     RAMD_CHECK(A->val32, RAMD_E_STATE, "FP32 values of the matrix were not built");
-    launch_stream<float, float>(A, A->val32, op, x, b, y, omega, s);
+    RAMD_CHECK(!A->dia || A->dia->v32, RAMD_E_STATE, "FP32 diagonal values of the matrix were not built");
+    launch_stream<float, float>(A, A->val32, A->dia ? A->dia->v32 : nullptr, op, x, b, y, omega, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -413,7 +531,8 @@ int ramd_prof_enable(ramd_ctx ctx, int64_t min_nnz)
     });
 }
 
-int ramd_prof_read(ramd_ctx ctx, int op, int prec, int64_t* launches, double* ms, double* bytes)
+int ramd_prof_read(ramd_ctx ctx, int op, int prec, int64_t* launches, int64_t* dia_launches, double* ms,
+                   double* bytes)
 {
     return guarded([&] {
         RAMD_CHECK(ctx && op >= 0 && op < 5 && (prec == RAMD_PREC_FP64 || prec == RAMD_PREC_FP32), RAMD_E_ARG,
@@ -421,6 +540,7 @@ int ramd_prof_read(ramd_ctx ctx, int op, int prec, int64_t* launches, double* ms
         prof_resolve(ctx);
         const ProfSlot& s = ctx->prof[op * 2 + prec];
         if (launches) *launches = s.launches;
+        if (dia_launches) *dia_launches = s.dia_launches;
         if (ms) *ms = s.ms;
         if (bytes) *bytes = s.bytes;
     });

@@ -79,6 +79,14 @@ def test_split_golden(golden, level, ctype, suffix):
     assert np.array_equal(st, golden[f"rss_cf{level}{suffix}"])
 
 
+@pytest.mark.parametrize("level,second_pass", [(0, True), (1, False)])
+def test_split_rs_serial_golden(golden, level, second_pass):
+    """ruge_stuben/tests/test_splitting.cpp: split_rs(S0) and split_rs(S1, false, false)."""
+    S = K.fixture_csr(golden, f"rss_S{level}", sort=False)
+    st = ra.split_rs_serial(S.rowptr, S.col, second_pass)
+    assert np.array_equal(st, golden[f"rss_cf{level}_rs"])
+
+
 @pytest.mark.parametrize("name", ["aniso", "laplacian"])
 @pytest.mark.parametrize("itype,suffix", [(ra.Direct, "direct"), (ra.ModClassical, "mod_class"),
                                           (ra.Extended, "extend")])
@@ -147,7 +155,7 @@ def test_strength_bit_exact_vs_ref(stype):
 
 
 @needs_ref
-@pytest.mark.parametrize("ctype", [ra.CLJP, ra.PMIS])
+@pytest.mark.parametrize("ctype", [ra.RS, ra.CLJP, ra.Falgout, ra.PMIS, ra.HMIS])
 def test_split_bit_exact_vs_ref(ctype):
     for _, (rp, col, val) in _problems():
         srp, scol = ra.strength(rp, col, val, ra.Classical, 0.25)
