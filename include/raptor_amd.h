@@ -79,6 +79,11 @@ int ramd_csr_info(ramd_csr A, int32_t* n_rows, int32_t* n_cols, int64_t* nnz);
 /* Copies the device arrays back (any output pointer may be NULL). */
 int ramd_csr_download(ramd_csr A, int32_t* rowptr, int32_t* col, double* val);
 int ramd_csr_destroy(ramd_csr A);
+/* New values (device array, nnz, A's stored order) for A's pattern; the
+ * derived copies (FP32 values, diagonal planes, transpose) follow.  Refused
+ * for matrices owned by a hierarchy.  No reference counterpart beyond
+ * writing ParCSRMatrix::on_proc->vals. */
+int ramd_csr_set_values(ramd_csr A, const double* val_d);
 /* Number of stored diagonals when the matrix also has the stencil-diagonal
  * form the SpMV family uses for stencil-structured operators (values per
  * diagonal, a per-row presence mask, no column indices; built automatically
@@ -180,9 +185,11 @@ typedef struct {
     double relax_weight;    /* default 1.0 (configs use 2/3)                    */
     int max_coarse;         /* default 50                                       */
     int max_levels;         /* default 25                                       */
-    const double* weights;  /* optional CLJP/PMIS random weights (n entries);
-                               NULL = the reference's srand(2448422)+rand()
-                               stream (multilevel/par_multilevel.hpp:215-225)  */
+    const double* weights;  /* optional CLJP/PMIS/MIS-2 random weights (n
+                               entries); NULL = the reference's
+                               srand(2448422)+rand() stream
+                               (multilevel/par_multilevel.hpp:215-225)          */
+    int prolong_smooth_steps;/* SA Jacobi prolongation steps (default 1)       */
 } ramd_amg_opts;
 
 /* Fills opts with the reference's defaults for `solver`. */
@@ -203,7 +210,8 @@ int ramd_hier_info(ramd_hier h, int* num_levels, int32_t* n_rows, int64_t* nnz_A
 /* Borrowed handles of level l's A (diagonal first) and P (NULL at the
  * coarsest level); valid until ramd_hier_destroy. */
 int ramd_hier_level(ramd_hier h, int level, ramd_csr* A, ramd_csr* P);
-/* C/F states (1 = C, 0 = F, -2 = no neighbours) or SA aggregates of level l. */
+/* C/F states (1 = C, 0 = F, -2 = no neighbours) of level l, or for SA the
+ * aggregate of each row (its root row, -1 for none). */
 int ramd_hier_states(ramd_hier h, int level, int32_t* states);
 /* Host-seconds spent per setup phase: [strength, coarsen, interp, galerkin, other]. */
 int ramd_hier_setup_times(ramd_hier h, double* times5);
@@ -250,6 +258,16 @@ int ramd_fpcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double rt
               int max_iter, int cycle_prec, double* relres_hist, int32_t* iters,
               double* final_relres);
 
+/* Adjoint of x = A^-1 b for a symmetric positive definite A (no reference
+ * counterpart: the reference has no differentiation).  Given the solution
+ * x and g = dL/dx, solves A^T lambda = g by the same flexible PCG and
+ * hierarchy (A^T = A), then writes dL/db = lambda into db (n) and, when dA
+ * is not NULL, dL/dA_k = -lambda_row(k) * x_col(k) for every stored entry k
+ * of A in A's stored order (nnz).  All arrays are device arrays. */
+int ramd_adjoint(ramd_hier h, ramd_csr A, const double* x_d, const double* g_d, double rtol,
+                 int max_iter, int cycle_prec, double* dA_d, double* db_d, int32_t* iters,
+                 double* final_relres);
+
 /* ------------------------------------------------------------------ */
 /* host setup pieces, exposed for parity tests                          */
 /* ------------------------------------------------------------------ */
@@ -281,6 +299,30 @@ int ramd_host_interp(int32_t n, const int32_t* rowptr, const int32_t* col, const
  * diffusion on m^3 with log-normal cell coefficients (seed). */
 int ramd_host_gallery(int kind, int32_t m, double eps, double theta, uint64_t seed,
                       int32_t* n, int32_t** rowptr, int32_t** col, double** val, int64_t* nnz);
+/* mis2(S, states, off_proc_states, false, weights), aggregation/par_mis.cpp:
+ * 204-645.  weights NULL = the srand(2448422) stream.  states: 1 selected,
+ * 0 not selected. */
+int ramd_host_mis2(int32_t n, const int32_t* s_rowptr, const int32_t* s_col, const double* weights,
+                   int32_t* states);
+/* aggregate(A, S, states, off, aggregates, false, rand_vals),
+ * aggregation/par_aggregate.cpp:5-185.  aggregates[i] = root row or -1;
+ * rand_vals may be NULL (zeros, as the solver calls it). */
+int ramd_host_aggregate(int32_t n, const int32_t* rowptr, const int32_t* col, const double* val,
+                        const int32_t* s_rowptr, const int32_t* s_col, const int32_t* states,
+                        const double* rand_vals, int32_t* aggregates, int32_t* n_aggs);
+/* fit_candidates(A, n_aggs, aggregates, B, R, 1, false, tol),
+ * aggregation/par_candidates.cpp:6-216: tentative prolongator T (n x n_aggs)
+ * and the aggregate norms R (n_aggs). */
+int ramd_host_fit_candidates(int32_t n, const int32_t* aggregates, int32_t n_aggs, const double* B, double* R,
+                             int32_t** t_rowptr, int32_t** t_col, double** t_val, int64_t* t_nnz);
+/* jacobi_prolongation(A, T, false, omega, steps),
+ * aggregation/par_prolongation.cpp:7-103: P = T - (omega D^-1 A) T per step
+ * (D = absolute row sums), the sparse products on the device of ctx.  A is
+ * taken in its stored order (the solver passes it sorted, diagonal first). */
+int ramd_sa_prolongation(ramd_ctx ctx, int32_t n, const int32_t* rowptr, const int32_t* col, const double* val,
+                         int32_t n_aggs, const int32_t* t_rowptr, const int32_t* t_col, const double* t_val,
+                         double omega, int steps, int32_t** p_rowptr, int32_t** p_col, double** p_val,
+                         int64_t* p_nnz);
 
 #ifdef __cplusplus
 }

@@ -10,7 +10,8 @@ using namespace raptor;
 static std::vector<int> g_rp, g_col;
 static std::vector<double> g_val;
 static int g_nrows = 0, g_ncols = 0;
-static ParRugeStubenSolver* g_ml = nullptr;
+static ParMultilevel* g_ml = nullptr;
+static bool g_sa = false;
 static std::vector<double> g_weights;
 
 static ParCSRMatrix* make_par(int n_rows, int n_cols, const int* rp, const int* col, const double* val)
@@ -178,6 +179,7 @@ int ref_hier_setup(int n, const int* rp, const int* col, const double* val, doub
     interp_t it[] = {Direct, ModClassical, Extended};
     relax_t rt[] = {Jacobi, SOR, SSOR};
     ParCSRMatrix* A = make_par(n, n, rp, col, val);
+    g_sa = false;
     g_ml = new ParRugeStubenSolver(theta, ct[coarsen], it[interp], strength == 0 ? Classical : Symmetric, rt[relax]);
     g_ml->relax_weight = relax_weight;
     g_ml->max_coarse = max_coarse;
@@ -212,7 +214,17 @@ int ref_hier_states(int l, int* states)
     ParCSRMatrix* A = g_ml->levels[l]->A;
     ParCSRMatrix* S = A->strength(g_ml->strength_type, g_ml->strong_threshold);
     aligned_vector<int> st, off;
-    switch (g_ml->coarsen_type) {
+    if (g_sa) {
+        // aggregates, recomputed as extend_hierarchy does
+        aligned_vector<int> agg;
+        mis2(S, st, off, false, g_weights.data());
+        aggregate(A, S, st, off, agg, false);
+        std::copy(agg.begin(), agg.end(), states);
+        delete S;
+        return (int)agg.size();
+    }
+    ParRugeStubenSolver* rs = (ParRugeStubenSolver*)g_ml;
+    switch (rs->coarsen_type) {
     case RS: if (l < 3) split_rs(S, st, off); else split_falgout(S, st, off, false, g_weights.data()); break;
     case CLJP: split_cljp(S, st, off, false, g_weights.data()); break;
     case Falgout: split_falgout(S, st, off, false, g_weights.data()); break;
@@ -305,6 +317,89 @@ int ref_cg(int n, const int* rp, const int* col, const double* val, double* x, c
     for (int i = 0; i < k; i++) res[i] = r[i];
     delete A;
     return (int)r.size();
+}
+
+// ParSmoothedAggregationSolver setup (relax as in ref_hier_setup)
+int ref_hier_setup_sa(int n, const int* rp, const int* col, const double* val, double theta, int strength,
+                      int relax, double relax_weight, int max_coarse, int max_levels, const double* weights,
+                      double prolong_weight, int prolong_steps)
+{
+    delete g_ml;
+    relax_t rt[] = {Jacobi, SOR, SSOR};
+    ParCSRMatrix* A = make_par(n, n, rp, col, val);
+    g_sa = true;
+    g_ml = new ParSmoothedAggregationSolver(theta, MIS, JacobiProlongation, strength == 0 ? Classical : Symmetric,
+                                            rt[relax], prolong_steps, prolong_weight);
+    g_ml->relax_weight = relax_weight;
+    g_ml->max_coarse = max_coarse;
+    g_ml->max_levels = max_levels;
+    g_weights.clear();
+    if (weights) {
+        g_weights.assign(weights, weights + n);
+        g_ml->weights = new double[n];
+        std::copy(weights, weights + n, g_ml->weights);
+    } else {
+        g_weights.resize(n);
+        srand(2448422);
+        for (int i = 0; i < n; i++) g_weights[i] = double(rand()) / RAND_MAX;
+    }
+    g_ml->setup(A);
+    delete A;
+    return g_ml->num_levels;
+}
+
+int ref_mis2(int n, const int* srp, const int* scol, const double* weights, int* states)
+{
+    ParCSRMatrix* S = make_par(n, n, srp, scol, nullptr);
+    aligned_vector<int> st, off;
+    std::vector<double> w(weights, weights + n);
+    mis2(S, st, off, false, w.data());
+    std::copy(st.begin(), st.end(), states);
+    delete S;
+    return 0;
+}
+
+int ref_aggregate(int n, const int* rp, const int* col, const double* val, const int* srp, const int* scol,
+                  const int* states, const double* rand_vals, int* aggs)
+{
+    ParCSRMatrix* A = make_par(n, n, rp, col, val);
+    ParCSRMatrix* S = make_par(n, n, srp, scol, nullptr);
+    aligned_vector<int> st(states, states + n), off, agg;
+    std::vector<double> r;
+    if (rand_vals) r.assign(rand_vals, rand_vals + n);
+    int na = aggregate(A, S, st, off, agg, false, rand_vals ? r.data() : nullptr);
+    std::copy(agg.begin(), agg.end(), aggs);
+    delete S;
+    delete A;
+    return na;
+}
+
+// T of fit_candidates (stashed); R (n_aggs) written out
+long ref_fit_candidates(int n, const int* rp, const int* col, const double* val, const int* aggs, int n_aggs,
+                        const double* B, double* R)
+{
+    ParCSRMatrix* A = make_par(n, n, rp, col, val);
+    aligned_vector<int> agg(aggs, aggs + n);
+    aligned_vector<double> b(B, B + n), r;
+    ParCSRMatrix* T = fit_candidates(A, n_aggs, agg, b, r, 1, false, 1e-10);
+    std::copy(r.begin(), r.end(), R);
+    long nnz = stash(T->on_proc);
+    delete T;
+    delete A;
+    return nnz;
+}
+
+long ref_jacobi_prolongation(int n, const int* rp, const int* col, const double* val, int n_aggs, const int* trp,
+                             const int* tcol, const double* tval, double omega, int steps)
+{
+    ParCSRMatrix* A = make_par(n, n, rp, col, val);
+    ParCSRMatrix* T = make_par(n, n_aggs, trp, tcol, tval);
+    ParCSRMatrix* P = jacobi_prolongation(A, T, false, omega, steps);
+    long nnz = stash(P->on_proc);
+    delete P;
+    delete T;
+    delete A;
+    return nnz;
 }
 
 }  // extern "C"

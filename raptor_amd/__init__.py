@@ -8,15 +8,18 @@ libraptor_amd.so (include/raptor_amd.h):
                       (core/par_matrix.hpp, util/linalg/par_spmv.cpp,
                       util/linalg/par_matmult.cpp)
     ParVector         device FP64 vector (core/par_vector.hpp)
-    ParRugeStubenSolver
-                      ParMultilevel with RS setup: setup / cycle / solve
-                      (multilevel/par_multilevel.hpp,
-                      ruge_stuben/par_ruge_stuben_solver.hpp)
+    ParRugeStubenSolver, ParSmoothedAggregationSolver
+                      ParMultilevel with RS or SA setup: setup / cycle /
+                      solve (multilevel/par_multilevel.hpp,
+                      ruge_stuben/par_ruge_stuben_solver.hpp,
+                      aggregation/par_smoothed_aggregation_solver.hpp)
     PCG               krylov/par_cg.cpp:121-263
-    strength / split / interpolation / gallery
-                      host setup pieces (par_strength.cpp,
+    strength / split / interpolation / mis2 / aggregate /
+    fit_candidates / jacobi_prolongation / gallery
+                      setup pieces (par_strength.cpp,
                       ruge_stuben/par_cf_splitting.cpp,
-                      ruge_stuben/par_interpolation.cpp, gallery/)
+                      ruge_stuben/par_interpolation.cpp, aggregation/,
+                      gallery/)
 
 Every compute call runs the HIP library; nothing falls back to a CPU path.
 """
@@ -30,8 +33,9 @@ from ._lib import (CLJP, HMIS, PMIS, RS, Classical, Direct, Extended, Falgout, J
                    PREC_FP32, PREC_FP64, RamdError, Symmetric, check, lib)
 
 __all__ = [
-    "Context", "ParVector", "ParCSRMatrix", "ParRugeStubenSolver", "PCG", "strength", "split",
-    "interpolation", "gallery_laplace27", "gallery_aniso2d", "gallery_fv7", "RamdError",
+    "Context", "ParVector", "ParCSRMatrix", "ParMultilevel", "ParRugeStubenSolver",
+    "ParSmoothedAggregationSolver", "PCG", "FPCG", "adjoint", "strength", "split", "interpolation", "mis2",
+    "aggregate", "fit_candidates", "jacobi_prolongation", "gallery_laplace27", "gallery_aniso2d", "gallery_fv7", "RamdError",
     "RS", "CLJP", "Falgout", "PMIS", "HMIS", "Direct", "ModClassical", "Extended", "Classical",
     "Symmetric", "Jacobi", "PREC_FP64", "PREC_FP32",
 ]
@@ -200,6 +204,10 @@ class ParCSRMatrix:
         check(lib().ramd_galerkin(self.h, P.h, C.byref(h)))
         return ParCSRMatrix(self.ctx, h)
 
+    def set_values(self, vals):
+        """New values (a device ParVector of nnz entries, stored order) for the same pattern."""
+        check(lib().ramd_csr_set_values(self.h, vals.ptr))
+
     def jacobi(self, x, b, tmp, num_sweeps=1, omega=1.0):
         check(lib().ramd_jacobi(self.h, x.ptr, b.ptr, tmp.ptr, num_sweeps, float(omega)))
 
@@ -221,45 +229,43 @@ def default_opts(solver=_lib.SOLVER_RS):
     return o
 
 
-class ParRugeStubenSolver:
-    """ParRugeStubenSolver(strong_threshold, coarsen, interp, strength, relax)
-    (ruge_stuben/par_ruge_stuben_solver.hpp:15-25) with the public fields of
-    ParMultilevel (multilevel/par_multilevel.hpp:626-658)."""
+class ParMultilevel:
+    """Fields and solve path shared by both solvers (ParMultilevel,
+    multilevel/par_multilevel.hpp:79-100,626-658)."""
 
-    def __init__(self, ctx, strong_threshold=0.0, coarsen_type=RS, interp_type=Direct,
-                 strength_type=Classical, relax_type=Jacobi):
+    _solver = _lib.SOLVER_RS
+
+    def _init_common(self, ctx, strong_threshold, strength_type, relax_type):
         if relax_type != Jacobi:
             raise ValueError("the device V-cycle smooths with weighted Jacobi only")
         self.ctx = ctx
         self.strong_threshold = strong_threshold
-        self.coarsen_type = coarsen_type
-        self.interp_type = interp_type
         self.strength_type = strength_type
         self.relax_type = relax_type
-        o = default_opts()
+        o = default_opts(self._solver)
         self.num_smooth_sweeps = o.num_smooth_sweeps
         self.relax_weight = o.relax_weight
         self.max_coarse = o.max_coarse
         self.max_levels = o.max_levels
-        self.interp_filter = o.interp_filter
         self.solve_tol = 1e-7
         self.max_iterations = 100
         self.weights = None
         self.h = None
 
-    def setup(self, rowptr, col, val):
-        """ParMultilevel::setup(Af): A is given as host CSR arrays."""
-        rowptr, col, val = _i32(rowptr), _i32(col), _f64(val)
-        o = default_opts()
+    def _opts(self):
+        o = default_opts(self._solver)
         o.strong_threshold = self.strong_threshold
-        o.coarsen_type = self.coarsen_type
-        o.interp_type = self.interp_type
         o.strength_type = self.strength_type
-        o.interp_filter = self.interp_filter
         o.num_smooth_sweeps = self.num_smooth_sweeps
         o.relax_weight = self.relax_weight
         o.max_coarse = self.max_coarse
         o.max_levels = self.max_levels
+        return o
+
+    def setup(self, rowptr, col, val):
+        """ParMultilevel::setup(Af): A is given as host CSR arrays."""
+        rowptr, col, val = _i32(rowptr), _i32(col), _f64(val)
+        o = self._opts()
         w = None
         if self.weights is not None:
             w = _f64(self.weights)
@@ -326,6 +332,47 @@ class ParRugeStubenSolver:
             pass
 
 
+
+class ParRugeStubenSolver(ParMultilevel):
+    """ParRugeStubenSolver(strong_threshold, coarsen, interp, strength, relax)
+    (ruge_stuben/par_ruge_stuben_solver.hpp:15-25)."""
+
+    _solver = _lib.SOLVER_RS
+
+    def __init__(self, ctx, strong_threshold=0.0, coarsen_type=RS, interp_type=Direct,
+                 strength_type=Classical, relax_type=Jacobi):
+        self._init_common(ctx, strong_threshold, strength_type, relax_type)
+        self.coarsen_type = coarsen_type
+        self.interp_type = interp_type
+        self.interp_filter = default_opts(self._solver).interp_filter
+
+    def _opts(self):
+        o = super()._opts()
+        o.coarsen_type = self.coarsen_type
+        o.interp_type = self.interp_type
+        o.interp_filter = self.interp_filter
+        return o
+
+
+class ParSmoothedAggregationSolver(ParMultilevel):
+    """ParSmoothedAggregationSolver(strong_threshold, MIS, JacobiProlongation,
+    strength, relax, prolong_smooth_steps, prolong_weight)
+    (aggregation/par_smoothed_aggregation_solver.hpp:17-32)."""
+
+    _solver = _lib.SOLVER_SA
+
+    def __init__(self, ctx, strong_threshold=0.0, strength_type=Symmetric, relax_type=Jacobi,
+                 prolong_smooth_steps=1, prolong_weight=4.0 / 3):
+        self._init_common(ctx, strong_threshold, strength_type, relax_type)
+        self.prolong_smooth_steps = prolong_smooth_steps
+        self.prolong_weight = prolong_weight
+
+    def _opts(self):
+        o = super()._opts()
+        o.prolong_smooth_steps = self.prolong_smooth_steps
+        o.prolong_weight = self.prolong_weight
+        return o
+
 def PCG(A, ml, x, b, tol=1e-5, max_iter=-1, max_res=10000):
     """PCG(A, ml, x, b, res, tol, max_iter), krylov/par_cg.cpp:121-263.
     Returns (residual history, iteration count)."""
@@ -343,6 +390,19 @@ def FPCG(A, ml, x, b, rtol=1e-8, max_iter=200, cycle_prec=PREC_FP64):
     check(lib().ramd_fpcg(ml.h, A.h, x.ptr, b.ptr, float(rtol), int(max_iter), int(cycle_prec),
                           _ptr(hist, C.c_double), C.byref(it), C.byref(fin)))
     return hist[:it.value + 1], it.value, fin.value
+
+
+def adjoint(A, ml, x, g, rtol=1e-10, max_iter=500, cycle_prec=PREC_FP64, with_dA=True):
+    """Gradients of L(x) for x = A^-1 b from g = dL/dx (ramd_adjoint): solves
+    A^T lambda = g through ml's hierarchy; returns (dL/dA on A's pattern as a
+    device ParVector of nnz entries or None, dL/db = lambda, iterations)."""
+    nnz = A.nnz
+    db = ParVector(A.ctx, x.n)
+    dA = ParVector(A.ctx, nnz) if with_dA else None
+    it, rel = C.c_int32(), C.c_double()
+    check(lib().ramd_adjoint(ml.h, A.h, x.ptr, g.ptr, float(rtol), int(max_iter), int(cycle_prec),
+                             dA.ptr if dA is not None else None, db.ptr, C.byref(it), C.byref(rel)))
+    return dA, db, it.value
 
 
 # ---------------------------------------------------------------------------
@@ -421,3 +481,58 @@ def gallery_aniso2d(m, eps=0.001, theta=np.pi / 8):
 def gallery_fv7(m, seed=3):
     """7-point finite-volume diffusion, log-normal cell coefficients (seeded)."""
     return _gallery(2, m, seed=seed)
+
+
+def mis2(s_rowptr, s_col, weights=None):
+    """mis2(S, states, off_proc_states, false, weights), aggregation/par_mis.cpp:204-645."""
+    s_rowptr, s_col = _i32(s_rowptr), _i32(s_col)
+    n = s_rowptr.size - 1
+    states = np.empty(n, np.int32)
+    w = None if weights is None else _f64(weights)
+    check(lib().ramd_host_mis2(n, _ptr(s_rowptr, C.c_int32), _ptr(s_col, C.c_int32),
+                               None if w is None else _ptr(w, C.c_double), _ptr(states, C.c_int32)))
+    return states
+
+
+def aggregate(rowptr, col, val, s_rowptr, s_col, states, rand_vals=None):
+    """aggregate(A, S, states, ...), aggregation/par_aggregate.cpp:5-185.
+    Returns (aggregates: root row per row or -1, n_aggs)."""
+    rowptr, col, val = _i32(rowptr), _i32(col), _f64(val)
+    s_rowptr, s_col, states = _i32(s_rowptr), _i32(s_col), _i32(states)
+    n = rowptr.size - 1
+    agg = np.empty(n, np.int32)
+    na = C.c_int32()
+    r = None if rand_vals is None else _f64(rand_vals)
+    check(lib().ramd_host_aggregate(n, _ptr(rowptr, C.c_int32), _ptr(col, C.c_int32), _ptr(val, C.c_double),
+                                    _ptr(s_rowptr, C.c_int32), _ptr(s_col, C.c_int32), _ptr(states, C.c_int32),
+                                    None if r is None else _ptr(r, C.c_double), _ptr(agg, C.c_int32),
+                                    C.byref(na)))
+    return agg, na.value
+
+
+def fit_candidates(aggregates, n_aggs, B):
+    """fit_candidates(A, n_aggs, aggregates, B, R, 1, ...), aggregation/par_candidates.cpp:6-216.
+    Returns (T rowptr, col, val, R)."""
+    aggregates, B = _i32(aggregates), _f64(B)
+    n = aggregates.size
+    R = np.zeros(max(n_aggs, 1))
+    trp, tcol, tval, tnnz = C.POINTER(C.c_int32)(), C.POINTER(C.c_int32)(), C.POINTER(C.c_double)(), C.c_int64()
+    check(lib().ramd_host_fit_candidates(n, _ptr(aggregates, C.c_int32), int(n_aggs), _ptr(B, C.c_double),
+                                         _ptr(R, C.c_double), C.byref(trp), C.byref(tcol), C.byref(tval),
+                                         C.byref(tnnz)))
+    return (_take(trp, n + 1, np.int32), _take(tcol, tnnz.value, np.int32), _take(tval, tnnz.value, np.float64),
+            R[:n_aggs])
+
+
+def jacobi_prolongation(ctx, rowptr, col, val, t_rowptr, t_col, t_val, n_aggs, omega=4.0 / 3, steps=1):
+    """jacobi_prolongation(A, T, false, omega, steps), aggregation/par_prolongation.cpp:7-103
+    (the sparse products on the device of ctx).  Returns (P rowptr, col, val)."""
+    rowptr, col, val = _i32(rowptr), _i32(col), _f64(val)
+    t_rowptr, t_col, t_val = _i32(t_rowptr), _i32(t_col), _f64(t_val)
+    n = rowptr.size - 1
+    prp, pcol, pval, pnnz = C.POINTER(C.c_int32)(), C.POINTER(C.c_int32)(), C.POINTER(C.c_double)(), C.c_int64()
+    check(lib().ramd_sa_prolongation(ctx.h, n, _ptr(rowptr, C.c_int32), _ptr(col, C.c_int32),
+                                     _ptr(val, C.c_double), int(n_aggs), _ptr(t_rowptr, C.c_int32),
+                                     _ptr(t_col, C.c_int32), _ptr(t_val, C.c_double), float(omega), int(steps),
+                                     C.byref(prp), C.byref(pcol), C.byref(pval), C.byref(pnnz)))
+    return _take(prp, n + 1, np.int32), _take(pcol, pnnz.value, np.int32), _take(pval, pnnz.value, np.float64)

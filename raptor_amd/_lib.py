@@ -49,6 +49,7 @@ class AmgOpts(C.Structure):
         ("max_coarse", C.c_int),
         ("max_levels", C.c_int),
         ("weights", C.POINTER(C.c_double)),
+        ("prolong_smooth_steps", C.c_int),
     ]
 
 
@@ -106,14 +107,18 @@ _SIGS = {
     "ramd_amg_solve": (P, P, P, D, C.c_int, PD, PI32),
     "ramd_pcg": (P, P, P, P, D, C.c_int, PD, I32, PI32, PI32),
     "ramd_fpcg": (P, P, P, P, D, C.c_int, C.c_int, PD, PI32, PD),
-    "ramd_fpcg_block": (P, P, I32, P, P, D, C.c_int, C.c_int, C.c_int, PI32, PD),
-    "ramd_adjoint": (P, P, P, P, D, C.c_int, P, P, PI32),
+    "ramd_adjoint": (P, P, P, P, D, C.c_int, C.c_int, P, P, PI32, PD),
+    "ramd_csr_set_values": (P, P),
     "ramd_host_free": None,
     "ramd_host_strength": (I32, PI32, PI32, PD, C.c_int, D, PPI32, PPI32, PI64),
     "ramd_host_split": (I32, PI32, PI32, C.c_int, PD, PI32),
     "ramd_host_split_rs_serial": (I32, PI32, PI32, C.c_int, PI32),
     "ramd_host_interp": (I32, PI32, PI32, PD, PI32, PI32, PI32, C.c_int, D, PI32, PPI32, PPI32, PPD, PI64),
     "ramd_host_gallery": (C.c_int, I32, D, D, C.c_uint64, PI32, PPI32, PPI32, PPD, PI64),
+    "ramd_host_mis2": (I32, PI32, PI32, PD, PI32),
+    "ramd_host_aggregate": (I32, PI32, PI32, PD, PI32, PI32, PI32, PD, PI32, PI32),
+    "ramd_host_fit_candidates": (I32, PI32, I32, PD, PD, PPI32, PPI32, PPD, PI64),
+    "ramd_sa_prolongation": (P, I32, PI32, PI32, PD, I32, PI32, PI32, PD, D, C.c_int, PPI32, PPI32, PPD, PI64),
 }
 
 

@@ -51,6 +51,23 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb)
     return x * q + (x < r ? x : r) + k;
 }
 
+// Workgroup sum in a fixed order (wavefront shuffle tree, then the four
+// wavefront sums in index order); the result is valid in thread 0.  Every
+// thread of the workgroup must call it.
+__device__ __forceinline__ double block_sum(double v)
+{
+    __shared__ double wsum[kBlockThreads / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wsum[w] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kBlockThreads / 64; i++) t += wsum[i];
+    __syncthreads();
+    return t;
+}
+
 }  // namespace ramd
 
 // Live kernel timing: hipEvent pairs around the row-block kernels of large
@@ -78,6 +95,9 @@ struct ramd_ctx_s {
     unsigned* red_counter = nullptr;
     double* red_result = nullptr;  // device scalars (16 slots)
     double* pinned = nullptr;      // pinned host staging for scalars (16 slots)
+    // per-workgroup partial sums of fused kernels (grown on demand)
+    double* big_partials = nullptr;
+    size_t big_cap = 0;
     void set_current() const;
 };
 
@@ -141,6 +161,9 @@ void dia_free(Dia* d);
 // Builds (once) A->trans, the CSR of A^T with rows in ascending source-row order.
 ramd_csr_s* csr_transpose(ramd_csr_s* A);
 void csr_make_fp32(ramd_csr_s* A);
+// New values for the same pattern (device array); refreshes every derived
+// copy (FP32 values, stencil-diagonal planes, the cached transpose).
+void csr_set_values(ramd_csr_s* A, const double* val_d);
 
 // Kernel launchers (stream-ordered), FP64 and FP32.
 // OP_ADD_INTO: y = b + A x with b a separate input (x <- x + P e written to a
@@ -156,8 +179,22 @@ void launch_stream_op32(const ramd_csr_s* A, int op, const float* x, const float
 void launch_stream_op_c(const ramd_csr_s* A, int op, const double* x, const double* b, double* y,
                         double omega, hipStream_t s);
 
+// y = A x (FP64 vectors) with x.y summed per workgroup in the same pass and
+// reduced into *out_d (fixed order: deterministic).  values_fp32 selects the
+// lossless FP32-value form of launch_stream_op_c.
+void launch_spmv_dot(const ramd_csr_s* A, bool values_fp32, const double* x, double* y, double* out_d,
+                     hipStream_t s);
+
 // Deterministic device reductions; result left in *out_d (device scalar).
 void launch_dot(ramd_ctx ctx, int64_t n, const double* x, const double* y, double* out_d);
 void launch_norm2sq(ramd_ctx ctx, int64_t n, const double* x, double* out_d);
+// Sums partials[0..np) into *out_d in a fixed order (one workgroup).
+void launch_reduce_final(ramd_ctx ctx, const double* partials, int np, double* out_d);
+// Device scratch of at least `count` doubles for per-workgroup partials.
+double* ctx_partials(ramd_ctx ctx, size_t count);
 double fetch_scalar(ramd_ctx ctx, const double* d);
+// Copies k (<= 16) device scalars to out after the stream drains.
+void fetch_scalars(ramd_ctx ctx, const double* d, int k, double* out);
+// Fixed grid of the deterministic grid-stride reductions.
+constexpr int kRedBlocks = 2048;
 }  // namespace ramd

@@ -282,6 +282,71 @@ void csr_make_fp32(ramd_csr_s* A)
     }
 }
 
+// Rewrites the stencil-diagonal planes from new CSR values (same pattern):
+// entry (row, col) lands on the plane of offset col - row.
+__global__ void dia_scatter_kernel(int n, DiaArgs a, int nd, const int* __restrict__ rp, const int* __restrict__ col,
+                                   const double* __restrict__ val, double* __restrict__ v64)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    for (int j = rp[row]; j < rp[row + 1]; j++) {
+        const int d = col[j] - row;
+        int k = 0;
+        while (k < nd - 1 && a.off[k] != d) k++;
+        v64[(size_t)k * n + row] = val[j];
+    }
+}
+
+// per-workgroup count of values that FP32 does not hold exactly
+__global__ __launch_bounds__(kBlockThreads) void inexact32_kernel(const double* __restrict__ v, int64_t m,
+                                                                  double* __restrict__ partials)
+{
+    double c = 0;
+    const int64_t stride = (int64_t)gridDim.x * kBlockThreads;
+    for (int64_t i = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x; i < m; i += stride)
+        c += ((double)(float)v[i] != v[i]) ? 1.0 : 0.0;
+    c = block_sum(c);
+    if (threadIdx.x == 0) partials[blockIdx.x] = c;
+}
+
+void csr_set_values(ramd_csr_s* A, const double* val_d)
+{
+    ramd_ctx ctx = A->ctx;
+    hipStream_t s = ctx->stream;
+    if (A->nnz) RAMD_HIP(hipMemcpyAsync(A->val, val_d, sizeof(double) * A->nnz, hipMemcpyDeviceToDevice, s));
+    if (A->trans) {  // stale values: rebuilt on next use
+        RAMD_HIP(hipStreamSynchronize(s));
+        csr_free(A->trans);
+        A->trans = nullptr;
+    }
+    if (A->val32 && A->nnz) {
+        int grid = (int)std::min<int64_t>((A->nnz + 255) / 256, 8192);
+        hipLaunchKernelGGL(to_fp32_kernel, dim3(grid), dim3(256), 0, s, A->val, A->val32, A->nnz);
+        RAMD_HIP(hipGetLastError());
+    }
+    if (Dia* d = A->dia) {
+        const int n = A->n_rows;
+        hipLaunchKernelGGL(dia_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, d->args, d->nd, A->rowptr,
+                           A->col, A->val, d->v64);
+        RAMD_HIP(hipGetLastError());
+        const int64_t m = (int64_t)d->nd * n;
+        const int g = (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (m + 255) / 256));
+        double* part = ctx_partials(ctx, (size_t)g);
+        hipLaunchKernelGGL(inexact32_kernel, dim3(g), dim3(kBlockThreads), 0, s, d->v64, m, part);
+        RAMD_HIP(hipGetLastError());
+        launch_reduce_final(ctx, part, g, ctx->red_result + 4);
+        double inexact = 0;
+        fetch_scalars(ctx, ctx->red_result + 4, 1, &inexact);
+        d->v32_exact = inexact == 0.0;
+        if (d->v32) {
+            int grid = (int)std::min<int64_t>((m + 255) / 256, 8192);
+            hipLaunchKernelGGL(to_fp32_kernel, dim3(grid), dim3(256), 0, s, d->v64, d->v32, m);
+            RAMD_HIP(hipGetLastError());
+        }
+    }
+    RAMD_HIP(hipStreamSynchronize(s));
+}
+
 }  // namespace ramd
 
 void ramd_ctx_s::set_current() const { RAMD_HIP(hipSetDevice(device)); }
@@ -343,6 +408,7 @@ int ramd_finalize(ramd_ctx ctx)
         dfree(ctx->red_partials);
         dfree(ctx->red_counter);
         dfree(ctx->red_result);
+        dfree(ctx->big_partials);
         if (ctx->pinned) (void)hipHostFree(ctx->pinned);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
@@ -460,6 +526,16 @@ int ramd_csr_download(ramd_csr A, int32_t* rowptr, int32_t* col, double* val)
         if (col && A->nnz) RAMD_HIP(hipMemcpyAsync(col, A->col, sizeof(int) * A->nnz, hipMemcpyDeviceToHost, s));
         if (val && A->nnz) RAMD_HIP(hipMemcpyAsync(val, A->val, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, s));
         RAMD_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int ramd_csr_set_values(ramd_csr A, const double* val_d)
+{
+    return guarded([&] {
+        RAMD_CHECK(A && (val_d || A->nnz == 0), RAMD_E_ARG, "ramd_csr_set_values: NULL argument");
+        RAMD_CHECK(!A->borrowed, RAMD_E_STATE, "ramd_csr_set_values: handle is owned by a hierarchy");
+        A->ctx->set_current();
+        csr_set_values(A, val_d);
     });
 }
 

@@ -41,7 +41,7 @@ struct ramd_hier_s {
     double times[5] = {0, 0, 0, 0, 0};
     bool fp32 = false;
     // PCG work vectors (fine level)
-    double *pr = nullptr, *pz = nullptr, *pp = nullptr, *pAp = nullptr, *pzo = nullptr;
+    double *pr = nullptr, *pz = nullptr, *pp = nullptr, *pAp = nullptr;
     float *pr32 = nullptr, *pz32 = nullptr;
 };
 
@@ -54,7 +54,9 @@ static double now_s()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-static ramd_csr_s* upload(ramd_ctx ctx, const Csr& M)
+// dia = false skips the stencil-diagonal form (matrices that are only
+// multiplied by other sparse matrices).
+static ramd_csr_s* upload(ramd_ctx ctx, const Csr& M, bool dia = true)
 {
     int* rp = dalloc<int>((size_t)M.n_rows + 1);
     int* c = dalloc<int>((size_t)M.nnz());
@@ -66,7 +68,7 @@ static ramd_csr_s* upload(ramd_ctx ctx, const Csr& M)
     }
     ramd_csr_s* D = csr_wrap(ctx, M.n_rows, M.n_cols, M.nnz(), rp, c, v, M.rowptr.data());
     D->diag_first = host_diag_first(M.n_rows, M.rowptr.data(), M.col.data());
-    dia_build(D, M.rowptr.data(), M.col.data(), M.val.data());
+    if (dia) dia_build(D, M.rowptr.data(), M.col.data(), M.val.data());
     return D;
 }
 
@@ -331,7 +333,7 @@ static void hier_free(ramd_hier_s* h)
     if (h->ctx) (void)hipStreamSynchronize(h->ctx->stream);
     for (Level& L : h->levels) free_level(L);
     for (void* p : {(void*)h->lu, (void*)h->piv, (void*)h->cb, (void*)h->cx, (void*)h->pr, (void*)h->pz, (void*)h->pp,
-                    (void*)h->pAp, (void*)h->pzo, (void*)h->pr32, (void*)h->pz32})
+                    (void*)h->pAp, (void*)h->pr32, (void*)h->pz32})
         dfree(p);
     delete h;
 }
@@ -372,6 +374,65 @@ static Csr rs_level(ramd_hier_s* h, Csr& A, int level, const double* weights, st
     return P;
 }
 
+// jacobi_prolongation aggregation/par_prolongation.cpp:7-103: P = T, then
+// `steps` times P = P - (omega D^-1 A) P with D the absolute row sums; the
+// products run on the device, the row scaling and the subtraction's layout
+// pass on the host.
+static Csr sa_prolongation(ramd_ctx ctx, const Csr& A, Csr P, double omega, int steps)
+{
+    if (steps <= 0) return P;
+    Csr scaled = host::sa_scaled_operator(A, omega);
+    ramd_csr_s* Sd = upload(ctx, scaled, false);
+    try {
+        for (int s = 0; s < steps; s++) {
+            ramd_csr_s* Pd = upload(ctx, P, false);
+            ramd_csr_s* Qd = nullptr;
+            try {
+                Qd = spgemm(Sd, Pd);
+            } catch (...) {
+                csr_free(Pd);
+                throw;
+            }
+            csr_free(Pd);
+            Csr Q = download(Qd);
+            csr_free(Qd);
+            P = host::sa_subtract(P, Q);
+        }
+    } catch (...) {
+        csr_free(Sd);
+        throw;
+    }
+    csr_free(Sd);
+    return P;
+}
+
+// ParSmoothedAggregationSolver::extend_hierarchy,
+// aggregation/par_smoothed_aggregation_solver.hpp:52-129: strength, MIS-2,
+// aggregation, tentative prolongator from the candidate vector B (replaced
+// by the aggregate norms for the next level), Jacobi smoothing.
+static Csr sa_level(ramd_hier_s* h, Csr& A, const double* weights, std::vector<int>& aggregates,
+                    std::vector<double>& B)
+{
+    const ramd_amg_opts& o = h->opts;
+    double t0 = now_s();
+    Csr S = o.strength_type == RAMD_SYMMETRIC ? host::symmetric_strength(A, o.strong_threshold)
+                                              : host::classical_strength(A, o.strong_threshold);
+    double t1 = now_s();
+    std::vector<int> states;
+    host::mis2(S, weights, states);
+    const int n_aggs = host::aggregate(A, S, states, nullptr, aggregates);
+    double t2 = now_s();
+    std::vector<double> R;
+    Csr T = host::fit_candidates(n_aggs, aggregates, B, R);
+    Csr P = sa_prolongation(h->ctx, A, std::move(T), o.prolong_weight, o.prolong_smooth_steps);
+    B = std::move(R);
+    double t3 = now_s();
+    h->times[0] += t1 - t0;
+    h->times[1] += t2 - t1;
+    h->times[2] += t3 - t2;
+    return P;
+}
+
 static void hier_setup(ramd_hier_s* h, Csr Af)
 {
     const ramd_amg_opts& o = h->opts;
@@ -388,10 +449,14 @@ static void hier_setup(ramd_hier_s* h, Csr Af)
     h->levels[0].A = upload(ctx, Af);
     alloc_level_vectors(h, h->levels[0]);
     Csr A = std::move(Af);
+    // SA near-nullspace candidate (aggregation/par_smoothed_aggregation_solver.hpp:38-49)
+    std::vector<double> B;
+    if (o.solver == RAMD_SOLVER_SA) B.assign((size_t)A.n_rows, 1.0);
     while (A.n_rows > o.max_coarse && (o.max_levels == -1 || (int)h->levels.size() < o.max_levels)) {
         const int l = (int)h->levels.size() - 1;
         Level& L = h->levels[l];
-        Csr P = rs_level(h, A, l, weights.data(), L.states);
+        Csr P = o.solver == RAMD_SOLVER_SA ? sa_level(h, A, weights.data(), L.states, B)
+                                           : rs_level(h, A, l, weights.data(), L.states);
         if (P.n_cols == 0) fail(RAMD_E_NUMERIC, "level %d produced no coarse points", l);
         double tg = now_s();
         L.P = upload(ctx, P);
@@ -427,7 +492,6 @@ static void hier_setup(ramd_hier_s* h, Csr Af)
     h->pz = dalloc<double>((size_t)n0);
     h->pp = dalloc<double>((size_t)n0);
     h->pAp = dalloc<double>((size_t)n0);
-    h->pzo = dalloc<double>((size_t)n0);
     RAMD_HIP(hipStreamSynchronize(ctx->stream));
     for (Level& L : h->levels) L.A->borrowed = true;
     for (Level& L : h->levels)
@@ -489,41 +553,201 @@ static double dot64(ramd_ctx ctx, int n, const double* a, const double* b)
 
 static void vcycle_zero(ramd_hier_s* h, double* z, const double* r) { cycle64(h, 0, z, r, true); }
 
-// z = M^{-1} r with the V-cycle in the requested precision (r, z FP64).
-static void precondition(ramd_hier_s* h, int prec, const double* r, double* z)
+// Flexible-PCG vector kernels.  The reductions run on a fixed grid of
+// kRedBlocks workgroups, each summing a fixed strided subset, so every run
+// gives the same bits.
+
+// x += alpha p; r -= alpha Ap; r32 = (float) r when given; partial sums of r^2
+__global__ __launch_bounds__(kBlockThreads) void fpcg_update_norm(int n, double alpha, const double* __restrict__ p,
+                                                                  const double* __restrict__ Ap, double* __restrict__ x,
+                                                                  double* __restrict__ r, float* __restrict__ r32,
+                                                                  double* __restrict__ partials)
 {
-    hipStream_t s = h->ctx->stream;
-    const int n = h->levels[0].n;
-    if (prec == RAMD_PREC_FP64) {
-        cycle64(h, 0, z, r, true);
-        return;
+    double s = 0;
+    const int stride = gridDim.x * kBlockThreads;
+    for (int i = blockIdx.x * kBlockThreads + threadIdx.x; i < n; i += stride) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * Ap[i];
+        r[i] = ri;
+        if (r32) r32[i] = (float)ri;
+        s += ri * ri;
     }
-    hipLaunchKernelGGL(to_f32, grid1(n), dim3(256), 0, s, (int64_t)n, r, h->pr32);
-    cycle32(h, 0, h->pz32, h->pr32, true);
-    hipLaunchKernelGGL(to_f64, grid1(n), dim3(256), 0, s, (int64_t)n, h->pz32, z);
+    s = block_sum(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// partial sums of r.z (first gridDim.x entries) and z.Ap (next gridDim.x)
+template <typename Z>
+__global__ __launch_bounds__(kBlockThreads) void fpcg_dots(int n, const double* __restrict__ r,
+                                                           const Z* __restrict__ z, const double* __restrict__ Ap,
+                                                           double* __restrict__ partials)
+{
+    double s1 = 0, s2 = 0;
+    const int stride = gridDim.x * kBlockThreads;
+    for (int i = blockIdx.x * kBlockThreads + threadIdx.x; i < n; i += stride) {
+        const double zi = (double)z[i];
+        s1 += r[i] * zi;
+        s2 += zi * Ap[i];
+    }
+    s1 = block_sum(s1);
+    s2 = block_sum(s2);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s1;
+        partials[gridDim.x + blockIdx.x] = s2;
+    }
+}
+
+// p = z + beta p
+template <typename Z>
+__global__ void fpcg_dir(int n, double beta, const Z* __restrict__ z, double* __restrict__ p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (double)z[i] + beta * p[i];
+}
+
+static int fixed_blocks(int n) { return std::max(1, std::min(kRedBlocks, (n + kBlockThreads - 1) / kBlockThreads)); }
+
+// r.z and z.Ap of the flexible-PCG step, in one pass
+template <typename Z>
+static void fpcg_rz_zap(ramd_ctx ctx, int n, const double* r, const Z* z, const double* Ap, double out[2])
+{
+    const int g = fixed_blocks(n);
+    double* part = ctx_partials(ctx, 2 * (size_t)g);
+    hipLaunchKernelGGL((fpcg_dots<Z>), dim3(g), dim3(kBlockThreads), 0, ctx->stream, n, r, z, Ap, part);
     RAMD_HIP(hipGetLastError());
-}
-
-// x += alpha p; r -= alpha Ap; keep the previous r for the flexible beta
-__global__ void fpcg_update(int n, double alpha, const double* __restrict__ p, const double* __restrict__ Ap,
-                            double* __restrict__ x, double* __restrict__ r, double* __restrict__ r_old)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    x[i] += alpha * p[i];
-    const double ri = r[i];
-    r_old[i] = ri;
-    r[i] = ri - alpha * Ap[i];
-}
-
-__global__ void fpcg_dir(int n, double beta, const double* __restrict__ z, double* __restrict__ p)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = z[i] + beta * p[i];
+    launch_reduce_final(ctx, part, g, ctx->red_result + 2);
+    launch_reduce_final(ctx, part + g, g, ctx->red_result + 3);
+    fetch_scalars(ctx, ctx->red_result + 2, 2, out);
 }
 
 // plain 2-norm (no small-entry skipping, NaN propagates)
 static double norm2_64(ramd_ctx ctx, int n, const double* a) { return std::sqrt(dot64(ctx, n, a, a)); }
+
+// Flexible PCG in FP64 (true residual, inner products, coarse LU in FP64)
+// preconditioned by one V-cycle in `cycle_prec`; stops on the true relative
+// residual.  Returns RAMD_OK, RAMD_E_INDEFINITE, RAMD_E_NUMERIC or
+// RAMD_E_NOT_CONVERGED; throws on argument and device errors.
+static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_d, double rtol, int max_iter,
+                    int cycle_prec, double* relres_hist, int32_t* iters, double* final_relres)
+{
+    int rc = RAMD_OK;
+    ramd_ctx ctx = h->ctx;
+    ctx->set_current();
+    hipStream_t s = ctx->stream;
+    const int n = A->n_rows;
+    RAMD_CHECK(n == h->levels[0].n, RAMD_E_ARG, "ramd_fpcg: A has %d rows, hierarchy %d", n, h->levels[0].n);
+    const bool f32 = cycle_prec == RAMD_PREC_FP32;
+    // FP32 form: the outer FP64 operator reads FP32-stored values where
+    // that is lossless (same FP64 results, half the value bytes)
+    auto outer = f32 ? launch_stream_op_c : launch_stream_op;
+    if (f32) {
+        enable_fp32(h);
+        csr_make_fp32(A);
+    }
+    double *r = h->pr, *z = h->pz, *p = h->pp, *Ap = h->pAp;
+    float *r32 = h->pr32, *z32 = h->pz32;
+    const int g = fixed_blocks(n);
+    const dim3 gv = grid1(n), bv(256);
+    double* part = nullptr;
+    // z = M^-1 r from the current r (FP32 form: from r32, which must hold (float) r)
+    auto cycle = [&] {
+        if (f32) cycle32(h, 0, z32, r32, true);
+        else cycle64(h, 0, z, r, true);
+    };
+    auto dots = [&](const double* ap, double out[2]) {
+        if (f32) fpcg_rz_zap(ctx, n, r, (const float*)z32, ap, out);
+        else fpcg_rz_zap(ctx, n, r, (const double*)z, ap, out);
+    };
+    const double bnorm = norm2_64(ctx, n, b_d);
+    const double scale = bnorm > 0 ? bnorm : 1.0;
+    outer(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
+    double relres = norm2_64(ctx, n, r) / scale;
+    if (relres_hist) relres_hist[0] = relres;
+    int it = 0;
+    if (!std::isfinite(bnorm) || !std::isfinite(relres)) {
+        rc = RAMD_E_NUMERIC;
+    } else if (relres > rtol) {
+        if (f32) {
+            hipLaunchKernelGGL(to_f32, gv, bv, 0, s, (int64_t)n, r, r32);
+            RAMD_HIP(hipGetLastError());
+        }
+        cycle();
+        if (f32) hipLaunchKernelGGL(to_f64, gv, bv, 0, s, (int64_t)n, z32, p);
+        else RAMD_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        RAMD_HIP(hipGetLastError());
+        double d2[2];
+        dots(r, d2);
+        double rz = d2[0];
+        while (it < max_iter) {
+            // Ap = A p and p.Ap in one pass over A
+            double pAp;
+            launch_spmv_dot(A, f32, p, Ap, ctx->red_result, s);
+            fetch_scalars(ctx, ctx->red_result, 1, &pAp);
+            if (!(pAp > 0.0)) {
+                rc = std::isfinite(pAp) ? RAMD_E_INDEFINITE : RAMD_E_NUMERIC;
+                break;
+            }
+            const double alpha = rz / pAp;
+            // x += alpha p, r -= alpha Ap, ||r||^2 (and r32) in one pass
+            part = ctx_partials(ctx, (size_t)g);
+            hipLaunchKernelGGL(fpcg_update_norm, dim3(g), dim3(kBlockThreads), 0, s, n, alpha, p, Ap, x_d, r,
+                               f32 ? r32 : nullptr, part);
+            RAMD_HIP(hipGetLastError());
+            launch_reduce_final(ctx, part, g, ctx->red_result + 1);
+            double rr;
+            fetch_scalars(ctx, ctx->red_result + 1, 1, &rr);
+            it++;
+            relres = std::sqrt(rr) / scale;
+            if (relres <= rtol) {
+                // confirm with the true residual; continue from it if the recurrence drifted
+                outer(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
+                relres = norm2_64(ctx, n, r) / scale;
+                if (relres_hist) relres_hist[it] = relres;
+                if (relres <= rtol) break;
+                if (f32) {
+                    hipLaunchKernelGGL(to_f32, gv, bv, 0, s, (int64_t)n, r, r32);
+                    RAMD_HIP(hipGetLastError());
+                }
+            } else if (relres_hist) {
+                relres_hist[it] = relres;
+            }
+            if (!std::isfinite(relres)) {
+                rc = RAMD_E_NUMERIC;
+                break;
+            }
+            cycle();
+            // Polak-Ribiere beta of flexible CG with r_new - r_old = -alpha Ap
+            dots(Ap, d2);
+            const double beta = -alpha * d2[1] / rz;
+            if (f32) hipLaunchKernelGGL((fpcg_dir<float>), gv, bv, 0, s, n, beta, z32, p);
+            else hipLaunchKernelGGL((fpcg_dir<double>), gv, bv, 0, s, n, beta, z, p);
+            RAMD_HIP(hipGetLastError());
+            rz = d2[0];
+        }
+    }
+    RAMD_HIP(hipStreamSynchronize(s));
+    *iters = it;
+    *final_relres = relres;
+    if (rc == RAMD_OK && relres > rtol) rc = RAMD_E_NOT_CONVERGED;
+    return rc;
+}
+
+static void fpcg_error(const char* who, int rc, int max_iter)
+{
+    if (rc == RAMD_E_INDEFINITE) set_error("%s: p.Ap <= 0 (matrix or preconditioner not SPD)", who);
+    if (rc == RAMD_E_NUMERIC) set_error("%s: non-finite value in the iteration", who);
+    if (rc == RAMD_E_NOT_CONVERGED) set_error("%s: %d iterations did not reach rtol", who, max_iter);
+}
+
+// dL/dA on A's pattern: g_k = -lambda_row * x_col (adjoint of A x = b)
+__global__ void adjoint_outer(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                              const double* __restrict__ lam, const double* __restrict__ x, double* __restrict__ g)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    const double l = lam[row];
+    for (int j = rp[row]; j < rp[row + 1]; j++) g[j] = -(l * x[col[j]]);
+}
 
 }  // namespace ramd
 
@@ -555,6 +779,7 @@ int ramd_amg_opts_default(int solver, ramd_amg_opts* o)
         o->max_coarse = 50;
         o->max_levels = 25;
         o->weights = nullptr;
+        o->prolong_smooth_steps = 1;
     });
 }
 
@@ -566,9 +791,10 @@ int ramd_hier_setup(ramd_ctx ctx, int32_t n, int64_t nnz, const int32_t* rowptr,
                    "ramd_hier_setup: NULL argument");
         RAMD_CHECK(n > 0 && nnz >= 0 && rowptr[0] == 0 && rowptr[n] == nnz, RAMD_E_ARG,
                    "ramd_hier_setup: inconsistent CSR (n=%d nnz=%lld)", n, (long long)nnz);
-        RAMD_CHECK(opts->solver == RAMD_SOLVER_RS, RAMD_E_ARG, "ramd_hier_setup: solver %d not supported",
-                   opts->solver);
-        RAMD_CHECK(opts->num_smooth_sweeps >= 0 && opts->max_coarse >= 1, RAMD_E_ARG, "ramd_hier_setup: bad options");
+        RAMD_CHECK(opts->solver == RAMD_SOLVER_RS || opts->solver == RAMD_SOLVER_SA, RAMD_E_ARG,
+                   "ramd_hier_setup: solver %d unknown", opts->solver);
+        RAMD_CHECK(opts->num_smooth_sweeps >= 0 && opts->max_coarse >= 1 && opts->prolong_smooth_steps >= 0,
+                   RAMD_E_ARG, "ramd_hier_setup: bad options");
         for (int64_t j = 0; j < nnz; j++)
             RAMD_CHECK(col[j] >= 0 && col[j] < n, RAMD_E_ARG, "ramd_hier_setup: column %d out of range", col[j]);
         ctx->set_current();
@@ -587,6 +813,50 @@ int ramd_hier_setup(ramd_ctx ctx, int32_t n, int64_t nnz, const int32_t* rowptr,
         hier_setup(h.get(), std::move(A));
         h->opts.weights = nullptr;
         *out = h.release();
+    });
+}
+
+int ramd_sa_prolongation(ramd_ctx ctx, int32_t n, const int32_t* rowptr, const int32_t* col, const double* val,
+                         int32_t n_aggs, const int32_t* t_rowptr, const int32_t* t_col, const double* t_val,
+                         double omega, int steps, int32_t** p_rowptr, int32_t** p_col, double** p_val,
+                         int64_t* p_nnz)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx && rowptr && t_rowptr && p_rowptr && p_col && p_val && p_nnz && n >= 0 && n_aggs >= 0 &&
+                       steps >= 0,
+                   RAMD_E_ARG, "ramd_sa_prolongation: bad argument");
+        auto take = [](int32_t nr, int32_t nc, const int32_t* rp, const int32_t* c, const double* v) {
+            RAMD_CHECK(rp[0] == 0, RAMD_E_ARG, "ramd_sa_prolongation: rowptr[0] != 0");
+            for (int i = 0; i < nr; i++)
+                RAMD_CHECK(rp[i + 1] >= rp[i], RAMD_E_ARG, "ramd_sa_prolongation: rowptr decreases at %d", i);
+            const int64_t nnz = rp[nr];
+            RAMD_CHECK(nnz == 0 || (c && v), RAMD_E_ARG, "ramd_sa_prolongation: NULL entries");
+            for (int64_t j = 0; j < nnz; j++)
+                RAMD_CHECK(c[j] >= 0 && c[j] < nc, RAMD_E_ARG, "ramd_sa_prolongation: column %d out of range",
+                           c[j]);
+            Csr M;
+            M.n_rows = nr;
+            M.n_cols = nc;
+            M.rowptr.assign(rp, rp + nr + 1);
+            M.col.assign(c, c + nnz);
+            M.val.assign(v, v + nnz);
+            return M;
+        };
+        Csr A = take(n, n, rowptr, col, val);
+        Csr T = take(n, n_aggs, t_rowptr, t_col, t_val);
+        ctx->set_current();
+        Csr P = sa_prolongation(ctx, A, std::move(T), omega, steps);
+        auto out = [](const auto& v, auto** dst) {
+            using E = typename std::decay_t<decltype(v)>::value_type;
+            E* p = (E*)std::malloc(sizeof(E) * (v.size() ? v.size() : 1));
+            if (!p) throw std::bad_alloc();
+            if (!v.empty()) std::memcpy(p, v.data(), sizeof(E) * v.size());
+            *dst = p;
+        };
+        out(P.rowptr, p_rowptr);
+        out(P.col, p_col);
+        out(P.val, p_val);
+        *p_nnz = P.nnz();
     });
 }
 
@@ -776,74 +1046,36 @@ int ramd_fpcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double rt
         RAMD_CHECK(h && A && x_d && b_d && iters && final_relres, RAMD_E_ARG, "ramd_fpcg: NULL argument");
         RAMD_CHECK(cycle_prec == RAMD_PREC_FP64 || cycle_prec == RAMD_PREC_FP32, RAMD_E_ARG,
                    "ramd_fpcg: unknown precision %d", cycle_prec);
-        ramd_ctx ctx = h->ctx;
-        ctx->set_current();
-        hipStream_t s = ctx->stream;
-        const int n = A->n_rows;
-        RAMD_CHECK(n == h->levels[0].n, RAMD_E_ARG, "ramd_fpcg: A has %d rows, hierarchy %d", n, h->levels[0].n);
-        // FP32 form: the outer FP64 operator reads FP32-stored values where
-        // that is lossless (same FP64 results, half the value bytes)
-        auto outer = cycle_prec == RAMD_PREC_FP32 ? launch_stream_op_c : launch_stream_op;
-        if (cycle_prec == RAMD_PREC_FP32) {
-            enable_fp32(h);
-            csr_make_fp32(A);
-        }
-        double *r = h->pr, *z = h->pz, *p = h->pp, *Ap = h->pAp, *r_old = h->pzo;
-        const double bnorm = norm2_64(ctx, n, b_d);
-        const double scale = bnorm > 0 ? bnorm : 1.0;
-        outer(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
-        double relres = norm2_64(ctx, n, r) / scale;
-        if (relres_hist) relres_hist[0] = relres;
-        int it = 0;
-        if (!std::isfinite(bnorm) || !std::isfinite(relres)) {
-            rc = RAMD_E_NUMERIC;
-        } else if (relres > rtol) {
-            precondition(h, cycle_prec, r, z);
-            RAMD_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-            double rz = dot64(ctx, n, r, z);
-            while (it < max_iter) {
-                outer(A, OP_SPMV, p, nullptr, Ap, 0.0, s);
-                const double pAp = dot64(ctx, n, p, Ap);
-                if (!(pAp > 0.0)) {
-                    rc = std::isfinite(pAp) ? RAMD_E_INDEFINITE : RAMD_E_NUMERIC;
-                    break;
-                }
-                const double alpha = rz / pAp;
-                hipLaunchKernelGGL(fpcg_update, grid1(n), dim3(256), 0, s, n, alpha, p, Ap, x_d, r, r_old);
-                RAMD_HIP(hipGetLastError());
-                it++;
-                relres = norm2_64(ctx, n, r) / scale;
-                if (relres <= rtol) {
-                    // confirm with the true residual; continue from it if the recurrence drifted
-                    outer(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
-                    relres = norm2_64(ctx, n, r) / scale;
-                    if (relres_hist) relres_hist[it] = relres;
-                    if (relres <= rtol) break;
-                } else if (relres_hist) {
-                    relres_hist[it] = relres;
-                }
-                if (!std::isfinite(relres)) {
-                    rc = RAMD_E_NUMERIC;
-                    break;
-                }
-                precondition(h, cycle_prec, r, z);
-                const double rz_new = dot64(ctx, n, r, z);
-                const double rz_old = dot64(ctx, n, r_old, z);
-                const double beta = (rz_new - rz_old) / rz;
-                hipLaunchKernelGGL(fpcg_dir, grid1(n), dim3(256), 0, s, n, beta, z, p);
-                RAMD_HIP(hipGetLastError());
-                rz = rz_new;
-            }
-        }
-        RAMD_HIP(hipStreamSynchronize(s));
-        *iters = it;
-        *final_relres = relres;
-        if (rc == RAMD_OK && relres > rtol) rc = RAMD_E_NOT_CONVERGED;
+        rc = fpcg_run(h, A, x_d, b_d, rtol, max_iter, cycle_prec, relres_hist, iters, final_relres);
     });
     if (st != RAMD_OK) return st;
-    if (rc == RAMD_E_INDEFINITE) set_error("ramd_fpcg: p.Ap <= 0 (matrix or preconditioner not SPD)");
-    if (rc == RAMD_E_NUMERIC) set_error("ramd_fpcg: non-finite value in the iteration");
-    if (rc == RAMD_E_NOT_CONVERGED) set_error("ramd_fpcg: %d iterations did not reach rtol", max_iter);
+    fpcg_error("ramd_fpcg", rc, max_iter);
+    return rc;
+}
+
+int ramd_adjoint(ramd_hier h, ramd_csr A, const double* x_d, const double* g_d, double rtol, int max_iter,
+                 int cycle_prec, double* dA_d, double* db_d, int32_t* iters, double* final_relres)
+{
+    int rc = RAMD_OK;
+    int st = guarded([&] {
+        RAMD_CHECK(h && A && x_d && g_d && db_d && iters && final_relres, RAMD_E_ARG, "ramd_adjoint: NULL argument");
+        RAMD_CHECK(cycle_prec == RAMD_PREC_FP64 || cycle_prec == RAMD_PREC_FP32, RAMD_E_ARG,
+                   "ramd_adjoint: unknown precision %d", cycle_prec);
+        RAMD_CHECK(A->n_rows == A->n_cols, RAMD_E_ARG, "ramd_adjoint: A is not square");
+        hipStream_t s = h->ctx->stream;
+        h->ctx->set_current();
+        // A^T lambda = dL/dx; A is symmetric (the SPD operator the hierarchy was built for)
+        RAMD_HIP(hipMemsetAsync(db_d, 0, sizeof(double) * A->n_rows, s));
+        rc = fpcg_run(h, A, db_d, g_d, rtol, max_iter, cycle_prec, nullptr, iters, final_relres);
+        if (rc == RAMD_OK && dA_d && A->n_rows) {
+            hipLaunchKernelGGL(adjoint_outer, grid1(A->n_rows), dim3(256), 0, s, A->n_rows, A->rowptr, A->col, db_d,
+                               x_d, dA_d);
+            RAMD_HIP(hipGetLastError());
+        }
+        RAMD_HIP(hipStreamSynchronize(s));
+    });
+    if (st != RAMD_OK) return st;
+    fpcg_error("ramd_adjoint", rc, max_iter);
     return rc;
 }
 

@@ -156,4 +156,53 @@ int ramd_host_gallery(int kind, int32_t m, double eps, double theta, uint64_t se
     });
 }
 
+int ramd_host_mis2(int32_t n, const int32_t* s_rowptr, const int32_t* s_col, const double* weights,
+                   int32_t* states)
+{
+    return guarded([&] {
+        RAMD_CHECK(states, RAMD_E_ARG, "ramd_host_mis2: NULL states");
+        Csr S = make_csr(n, n, s_rowptr, s_col, nullptr);
+        std::vector<double> w;
+        if (weights) w.assign(weights, weights + n);
+        else w = host::reference_weights(n, 0);
+        std::vector<int> st;
+        host::mis2(S, w.data(), st);
+        std::copy(st.begin(), st.end(), states);
+    });
+}
+
+int ramd_host_aggregate(int32_t n, const int32_t* rowptr, const int32_t* col, const double* val,
+                        const int32_t* s_rowptr, const int32_t* s_col, const int32_t* states,
+                        const double* rand_vals, int32_t* aggregates, int32_t* n_aggs)
+{
+    return guarded([&] {
+        RAMD_CHECK(val && states && aggregates && n_aggs, RAMD_E_ARG, "ramd_host_aggregate: NULL argument");
+        Csr A = make_csr(n, n, rowptr, col, val);
+        Csr S = make_csr(n, n, s_rowptr, s_col, nullptr);
+        std::vector<int> st(states, states + n), agg;
+        *n_aggs = host::aggregate(A, S, st, rand_vals, agg);
+        std::copy(agg.begin(), agg.end(), aggregates);
+    });
+}
+
+int ramd_host_fit_candidates(int32_t n, const int32_t* aggregates, int32_t n_aggs, const double* B, double* R,
+                             int32_t** t_rowptr, int32_t** t_col, double** t_val, int64_t* t_nnz)
+{
+    return guarded([&] {
+        RAMD_CHECK(n >= 0 && n_aggs >= 0 && aggregates && B && R && t_rowptr && t_col && t_val && t_nnz,
+                   RAMD_E_ARG, "ramd_host_fit_candidates: bad argument");
+        for (int i = 0; i < n; i++)
+            RAMD_CHECK(aggregates[i] >= -1 && aggregates[i] < n, RAMD_E_ARG,
+                       "ramd_host_fit_candidates: aggregate %d of row %d out of range", aggregates[i], i);
+        std::vector<int> agg(aggregates, aggregates + n);
+        std::vector<double> b(B, B + n), r;
+        Csr T = host::fit_candidates(n_aggs, agg, b, r);
+        std::copy(r.begin(), r.end(), R);
+        *t_rowptr = export_array(T.rowptr);
+        *t_col = export_array(T.col);
+        *t_val = export_array(T.val);
+        *t_nnz = T.nnz();
+    });
+}
+
 }  // extern "C"

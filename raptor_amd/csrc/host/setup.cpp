@@ -1035,6 +1035,323 @@ Csr extended_interpolation(Csr& A, Csr& S, const std::vector<int>& states, doubl
 }
 
 // ---------------------------------------------------------------------------
+// smoothed aggregation, aggregation/par_*.cpp
+// ---------------------------------------------------------------------------
+constexpr int kTmpSelection = 4;  // core/types.hpp:31
+
+// mis2 aggregation/par_mis.cpp:204-645 at one rank.  Distance-two maximal
+// independent set on S, driven by the random values r = weights.  The
+// reference's quirks are kept: a vertex marked TmpSelection keeps that mark
+// until it is selected or unselected, and the reset of the "touched" list
+// sets next[] to 0 instead of Unassigned, so a vertex touched in two
+// iterations keeps C[w] == Selected from then on (:587-593).
+void mis2(const Csr& S, const double* r, std::vector<int>& states)
+{
+    const int n = S.n_rows;
+    states.assign((size_t)n, kUnassigned);
+    if (!n) return;
+    std::vector<int> V((size_t)n);
+    std::iota(V.begin(), V.end(), 0);
+    std::vector<int> C((size_t)n, kUnselected), next((size_t)n, kUnassigned);
+    std::vector<double> row_max((size_t)n);
+
+    // directed graph: entries of S toward smaller random values (:326-357)
+    std::vector<int> d_ptr((size_t)n + 1, 0), d_col;
+    d_col.reserve(S.col.size() / 2);
+    for (int i = 0; i < n; i++) {
+        for (int j = S.rowptr[i]; j < S.rowptr[i + 1]; j++)
+            if (r[i] > r[S.col[j]]) d_col.push_back(S.col[j]);
+        d_ptr[i + 1] = (int)d_col.size();
+    }
+    // columns of S (CSR_to_CSC, core/matrix.cpp:492-538)
+    std::vector<int> c_ptr((size_t)n + 1, 0), c_row(S.col.size());
+    for (int c : S.col) c_ptr[c + 1]++;
+    for (int i = 0; i < n; i++) c_ptr[i + 1] += c_ptr[i];
+    {
+        std::vector<int> pos(c_ptr.begin(), c_ptr.end() - 1);
+        for (int i = 0; i < n; i++)
+            for (int j = S.rowptr[i]; j < S.rowptr[i + 1]; j++) c_row[pos[S.col[j]]++] = i;
+    }
+
+    int remaining = n;
+    bool first_pass = true;
+    int64_t guard = 0;
+    while (remaining || first_pass) {
+        if (++guard > (int64_t)n + 2) throw std::runtime_error("mis2: independent-set loop does not terminate");
+        // tentative selection (:378-415)
+        for (int i = 0; i < remaining; i++) {
+            const int v = V[i];
+            bool found = false;
+            for (int j = d_ptr[v]; j < d_ptr[v + 1]; j++) {
+                const int w = d_col[j];
+                if (states[w] == kUnassigned || states[w] > kSelected) {
+                    found = true;
+                    break;
+                }
+            }
+            if (!found) states[v] = kTmpSelection;
+        }
+        // largest tentative random value in each row (:420-446)
+        for (int i = 0; i < n; i++) {
+            double max_val = -1.0;
+            for (int j = S.rowptr[i]; j < S.rowptr[i + 1]; j++) {
+                const int u = S.col[j];
+                if (states[u] > kSelected && r[u] > max_val) max_val = r[u];
+            }
+            row_max[i] = max_val;
+        }
+        // conflicts at distance two (:449-473)
+        for (int i = 0; i < remaining; i++) {
+            const int v = V[i];
+            if (states[v] != kTmpSelection) continue;
+            bool found = false;
+            for (int j = S.rowptr[v]; j < S.rowptr[v + 1]; j++)
+                if (row_max[S.col[j]] > r[v]) {
+                    found = true;
+                    break;
+                }
+            if (!found) states[v] = kNewSelection;
+        }
+        // mark the neighbours of new selections (:505-526)
+        int head = kTmpSelection, length = 0;
+        for (int i = 0; i < remaining; i++) {
+            const int v = V[i];
+            if (states[v] != kNewSelection) continue;
+            for (int j = c_ptr[v]; j < c_ptr[v + 1]; j++) {
+                const int w = c_row[j];
+                C[w] = kSelected;
+                if (next[w] == kUnassigned) {
+                    next[w] = head;
+                    head = w;
+                    length++;
+                }
+            }
+        }
+        // unselect everything within distance two of a new selection (:555-586)
+        for (int i = 0; i < remaining; i++) {
+            const int v = V[i];
+            if (states[v] == kNewSelection) continue;
+            bool found = false;
+            for (int j = S.rowptr[v]; j < S.rowptr[v + 1]; j++) {
+                const int w = S.col[j];
+                if (states[w] == kNewSelection || C[w] == kSelected) {
+                    found = true;
+                    break;
+                }
+            }
+            if (found) states[v] = kNewUnselection;
+        }
+        for (int i = 0; i < length; i++) {
+            const int t = head;
+            head = next[head];
+            C[t] = kUnselected;
+            next[t] = 0;
+        }
+        // settle this iteration's decisions (:595-612)
+        int ctr = 0;
+        for (int i = 0; i < remaining; i++) {
+            const int v = V[i];
+            if (states[v] == kNewSelection)
+                states[v] = kSelected;
+            else if (states[v] == kNewUnselection)
+                states[v] = kUnselected;
+            else
+                V[ctr++] = v;
+        }
+        remaining = ctr;
+        first_pass = false;
+    }
+}
+
+// aggregate aggregation/par_aggregate.cpp:5-185 at one rank.  S and A are
+// sorted with the diagonal first in place (:14-17).  aggregates[i] is the
+// root row of i's aggregate, or -1; returns the number of aggregates.
+// rand_vals (may be NULL: zeros, as the solver calls it) break ties in the
+// second pass.
+int aggregate(Csr& A, Csr& S, const std::vector<int>& states, const double* rand_vals,
+              std::vector<int>& aggregates)
+{
+    sort_and_diag_first(S);
+    sort_and_diag_first(A);
+    const int n = S.n_rows;
+    const int N = n;  // global rows at one rank
+    aggregates.assign((size_t)n, -1);
+    std::vector<double> r((size_t)n, 0.0);
+    if (rand_vals) std::copy(rand_vals, rand_vals + n, r.begin());
+    int n_aggs = 0;
+    for (int i = 0; i < n; i++) {
+        if (S.rowptr[i + 1] - S.rowptr[i] <= 1)
+            aggregates[i] = -N;
+        else if (states[i] == kSelected) {
+            aggregates[i] = i;
+            n_aggs++;
+        }
+    }
+    // pass 1: join the first selected strong neighbour (:96-126)
+    for (int i = 0; i < n; i++) {
+        if (states[i] == kSelected) continue;
+        for (int j = S.rowptr[i]; j < S.rowptr[i + 1]; j++) {
+            const int c = S.col[j];
+            if (states[c] == kSelected && aggregates[c] >= 0) {
+                aggregates[i] = aggregates[c];
+                break;
+            }
+        }
+    }
+    // pass 2: join the aggregate of the strongest aggregated neighbour,
+    // encoded as -(agg+1) until every row is done (:134-172)
+    for (int i = 0; i < n; i++) {
+        if (aggregates[i] != -1) continue;
+        int ctr = A.rowptr[i];
+        const int a_end = A.rowptr[i + 1];
+        double max_val = 0.0;
+        int max_agg = -N;
+        for (int j = S.rowptr[i]; j < S.rowptr[i + 1]; j++) {
+            const int c = S.col[j];
+            while (ctr < a_end && A.col[ctr] != c) ctr++;
+            if (ctr == a_end) throw std::runtime_error("aggregate: strength entry missing from A");
+            const double val = std::fabs(A.val[ctr]) + r[c];
+            if (val > max_val && aggregates[c] >= 0) {
+                max_val = val;
+                max_agg = aggregates[c];
+            }
+        }
+        aggregates[i] = -(max_agg + 1);
+    }
+    for (int i = 0; i < n; i++) {
+        if (aggregates[i] <= -N)
+            aggregates[i] = -1;
+        else if (aggregates[i] < 0)
+            aggregates[i] = -(aggregates[i] + 1);
+    }
+    return n_aggs;
+}
+
+// fit_candidates aggregation/par_candidates.cpp:6-216 (one candidate) at one
+// rank.  Aggregates become columns in increasing root order; column k holds
+// B over its rows, scaled by 1/sqrt(sum of B^2 in ascending row order); R
+// gets the norms (the next level's B).
+Csr fit_candidates(int n_aggs, const std::vector<int>& aggregates, const std::vector<double>& B,
+                   std::vector<double>& R)
+{
+    const int n = (int)aggregates.size();
+    std::vector<int> colmap((size_t)n, 0);
+    for (int a : aggregates)
+        if (a >= 0) colmap[a] = 1;
+    int nc = 0;
+    for (int i = 0; i < n; i++)
+        if (colmap[i]) colmap[i] = nc++;
+    if (nc != n_aggs)
+        throw std::runtime_error("fit_candidates: an aggregate has no selected root (the reference's aggregation "
+                                 "leaves such a point undefined)");
+    R.assign((size_t)n_aggs, 0.0);
+    for (int i = 0; i < n; i++) {
+        const int a = aggregates[i];
+        if (a < 0) continue;
+        R[colmap[a]] += B[i] * B[i];
+    }
+    std::vector<double> scale((size_t)n_aggs);
+    for (int k = 0; k < n_aggs; k++) {
+        R[k] = std::sqrt(R[k]);
+        scale[k] = 1.0 / R[k];
+    }
+    Csr T;
+    T.n_rows = n;
+    T.n_cols = n_aggs;
+    T.rowptr.assign((size_t)n + 1, 0);
+    for (int i = 0; i < n; i++) {
+        const int a = aggregates[i];
+        if (a >= 0) {
+            const int k = colmap[a];
+            T.col.push_back(k);
+            T.val.push_back(B[i] * scale[k]);
+        }
+        T.rowptr[i + 1] = (int)T.col.size();
+    }
+    return T;
+}
+
+// The row scaling of jacobi_prolongation aggregation/par_prolongation.cpp:
+// 7-103: row i of A times omega / sum_j |a_ij| (the absolute row sum in
+// stored order; rows summing to zero become zero).  A keeps its layout.
+Csr sa_scaled_operator(const Csr& A, double omega)
+{
+    Csr M = A;
+    parallel_ranges(A.n_rows, host_threads(A.nnz()), [&](int, int r0, int r1) {
+        for (int i = r0; i < r1; i++) {
+            double row_sum = 0.0;
+            for (int j = A.rowptr[i]; j < A.rowptr[i + 1]; j++) row_sum += std::fabs(A.val[j]);
+            double inv = 0.0;
+            if (row_sum != 0.0) inv = (1.0 / std::fabs(row_sum)) * omega;
+            for (int j = A.rowptr[i]; j < A.rowptr[i + 1]; j++) M.val[j] = A.val[j] * inv;
+        }
+    });
+    return M;
+}
+
+// ParCSRMatrix::subtract util/linalg/par_add.cpp:184-310 at one rank: the
+// rows of P followed by the negated rows of Q, sorted by column, equal
+// columns summed in order, |v| < 1e-16 dropped (core/matrix.cpp:922-986),
+// then the diagonal (column == row) moved first.  A row of P has at most
+// one entry and the rows of Q have distinct columns, so no column appears
+// more than twice and the sum does not depend on how the sort orders ties.
+Csr sa_subtract(const Csr& P, const Csr& Q)
+{
+    const int n = P.n_rows;
+    Csr C;
+    C.n_rows = n;
+    C.n_cols = P.n_cols;
+    std::vector<RowChunk> ch((size_t)host_threads(P.nnz() + Q.nnz()));
+    parallel_ranges(n, (int)ch.size(), [&](int t, int r0, int r1) {
+        RowChunk& c = ch[t];
+        std::vector<std::pair<int, double>> row;
+        for (int i = r0; i < r1; i++) {
+            row.clear();
+            for (int j = P.rowptr[i]; j < P.rowptr[i + 1]; j++) row.emplace_back(P.col[j], P.val[j]);
+            for (int j = Q.rowptr[i]; j < Q.rowptr[i + 1]; j++) row.emplace_back(Q.col[j], -Q.val[j]);
+            std::stable_sort(row.begin(), row.end(),
+                             [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
+                                 return a.first < b.first;
+                             });
+            // remove_duplicates
+            const size_t base = c.col.size();
+            for (size_t k = 0; k < row.size(); k++) {
+                if (k > 0 && row[k].first == row[k - 1].first) {
+                    c.val.back() += row[k].second;
+                    continue;
+                }
+                if (c.col.size() > base && std::fabs(c.val.back()) < kZeroTol) {
+                    c.col.pop_back();
+                    c.val.pop_back();
+                }
+                c.col.push_back(row[k].first);
+                c.val.push_back(row[k].second);
+            }
+            if (c.col.size() > base && std::fabs(c.val.back()) < kZeroTol) {
+                c.col.pop_back();
+                c.val.pop_back();
+            }
+            // move_diag
+            for (size_t k = base; k < c.col.size(); k++) {
+                if (c.col[k] == i) {
+                    const double v = c.val[k];
+                    for (size_t m = k; m > base; m--) {
+                        c.col[m] = c.col[m - 1];
+                        c.val[m] = c.val[m - 1];
+                    }
+                    c.col[base] = i;
+                    c.val[base] = v;
+                    break;
+                }
+            }
+            c.len.push_back((int)(c.col.size() - base));
+        }
+    });
+    assemble(n, ch, C);
+    return C;
+}
+
+// ---------------------------------------------------------------------------
 // stencil generators
 // ---------------------------------------------------------------------------
 

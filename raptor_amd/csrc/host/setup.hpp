@@ -68,12 +68,16 @@ Csr mod_classical_interpolation(Csr& A, Csr& S, const std::vector<int>& states);
 Csr extended_interpolation(Csr& A, Csr& S, const std::vector<int>& states, double filter);
 void filter_interp(Csr& P, double filter_threshold);
 
-// Smoothed aggregation (aggregation/*.cpp).
+// Smoothed aggregation (aggregation/*.cpp).  The Jacobi prolongation
+// P = T - (omega D^-1 A) T is sa_scaled_operator, a sparse product (done
+// on the device) and sa_subtract.
 void mis2(const Csr& S, const double* weights, std::vector<int>& states);
-int aggregate(Csr& A, Csr& S, std::vector<int>& states, std::vector<int>& aggregates);
+int aggregate(Csr& A, Csr& S, const std::vector<int>& states, const double* rand_vals,
+              std::vector<int>& aggregates);
 Csr fit_candidates(int n_aggs, const std::vector<int>& aggregates, const std::vector<double>& B,
                    std::vector<double>& R);
-Csr jacobi_prolongation_scaled(Csr& A, const Csr& T, double omega, std::vector<double>& scale);
+Csr sa_scaled_operator(const Csr& A, double omega);
+Csr sa_subtract(const Csr& P, const Csr& Q);
 
 // Stencil generators (gallery/par_stencil.hpp:16-236).  stencil has 3^dim
 // entries, grid has dim entries.

@@ -21,7 +21,7 @@ template <int OP, typename V, typename X>
 __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
     const int* __restrict__ rowptr, const int* __restrict__ col, const V* __restrict__ val,
     const int* __restrict__ blocks, int nb, const X* __restrict__ x, const X* __restrict__ b,
-    X* __restrict__ y, X omega, int diag_first)
+    X* __restrict__ y, X omega, int diag_first, double* __restrict__ dotp)
 {
     __shared__ X prod[kStreamCap];
     const int blk = xcd_remap(blockIdx.x, nb);
@@ -50,6 +50,10 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
                 }
             }
             __syncthreads();
+        }
+        if (OP == OP_SPMV && dotp) {
+            const double v = block_sum(tid == 0 ? (double)x[r0] * (double)acc : 0.0);
+            if (tid == 0) dotp[blk] = v;
         }
         if (tid == 0) {
             if (OP == OP_SPMV || OP == OP_RESIDUAL) y[r0] = acc;
@@ -92,13 +96,22 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
 
     // phase 2: one lane per row, products added left to right
     const int row = r0 + tid;
-    if (row >= r1) return;
-    const int k0 = rowptr[row] - s, k1 = rowptr[row + 1] - s;
     if (OP == OP_SPMV) {
         X acc = 0;
-        for (int k = k0; k < k1; k++) acc += prod[k];
-        y[row] = acc;
-    } else if (OP == OP_APPEND || OP == OP_ADD_INTO) {
+        if (row < r1) {
+            const int k0 = rowptr[row] - s, k1 = rowptr[row + 1] - s;
+            for (int k = k0; k < k1; k++) acc += prod[k];
+            y[row] = acc;
+        }
+        if (dotp) {
+            const double v = block_sum(row < r1 ? (double)x[row] * (double)acc : 0.0);
+            if (tid == 0) dotp[blk] = v;
+        }
+        return;
+    }
+    if (row >= r1) return;
+    const int k0 = rowptr[row] - s, k1 = rowptr[row + 1] - s;
+    if (OP == OP_APPEND || OP == OP_ADD_INTO) {
         // CSR_append: the row sum is formed first, then added (spmv.cpp:96-101)
         X acc = 0;
         for (int k = k0; k < k1; k++) acc += prod[k];
@@ -136,13 +149,57 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
 // diagonal absent from the row reads its own plane (zero) and x[row] -- and
 // the accumulation skips it with a select, so the operation sequence of each
 // row is exactly the CSR row loop's and no out-of-range x is touched.
+// y_row of the plain product (OP_SPMV) in the stencil-diagonal form: the
+// same load and select sequence as dia_kernel's body.
+template <int OP, typename V, typename X, int ND>
+__device__ __forceinline__ X dia_row(int row, int n, const DiaArgs& a, int nd, int center,
+                                     const unsigned* __restrict__ mask, const V* __restrict__ val,
+                                     const X* __restrict__ x, const X* __restrict__ b)
+{
+    const unsigned m = mask[row];
+    X acc = X(0);
+    constexpr int CH = 9;
+#pragma unroll
+    for (int c0 = 0; c0 < ND; c0 += CH) {
+        V vv[CH];
+        X xx[CH];
+        bool on[CH];
+#pragma unroll
+        for (int t = 0; t < CH; t++) {
+            const int kk = c0 + t;
+            if (kk < ND) {
+                const bool live = (ND == kDiaMax) ? (kk < nd) : true;
+                const int k = live ? a.ord[kk] : 0;
+                on[t] = live && ((m >> k) & 1u);
+                vv[t] = val[(size_t)k * n + row];
+                xx[t] = x[on[t] ? row + a.off[k] : row];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < CH; t++)
+            if (c0 + t < ND) acc = on[t] ? acc + X(vv[t]) * xx[t] : acc;
+    }
+    (void)b;
+    (void)center;
+    return acc;
+}
+
 template <int OP, typename V, typename X, int ND>
 __global__ __launch_bounds__(kBlockThreads) void dia_kernel(int n, DiaArgs a, int nd, int center,
                                                            const unsigned* __restrict__ mask, const V* __restrict__ val,
                                                            const X* __restrict__ x, const X* __restrict__ b,
-                                                           X* __restrict__ y, X omega)
+                                                           X* __restrict__ y, X omega, double* __restrict__ dotp)
 {
-    const int row = xcd_remap(blockIdx.x, gridDim.x) * kBlockThreads + threadIdx.x;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int row = blk * kBlockThreads + threadIdx.x;
+    if (OP == OP_SPMV && dotp) {
+        // all lanes take part in the workgroup sum; rows past n add zero
+        const X acc = row < n ? dia_row<OP, V, X, ND>(row, n, a, nd, center, mask, val, x, b) : X(0);
+        if (row < n) y[row] = acc;
+        const double v = block_sum(row < n ? (double)x[row] * (double)acc : 0.0);
+        if (threadIdx.x == 0) dotp[blk] = v;
+        return;
+    }
     if (row >= n) return;
     const unsigned m = mask[row];
     X acc = (OP == OP_RESIDUAL) ? b[row] : X(0);
@@ -195,14 +252,15 @@ __global__ __launch_bounds__(kBlockThreads) void dia_kernel(int n, DiaArgs a, in
 }
 
 template <int OP, typename V, typename X>
-static void launch_dia_op(const ramd_csr_s* A, const V* val, const X* x, const X* b, X* y, X omega, hipStream_t s)
+static void launch_dia_op(const ramd_csr_s* A, const V* val, const X* x, const X* b, X* y, X omega, hipStream_t s,
+                          double* dotp)
 {
     const Dia* d = A->dia;
     const int n = A->n_rows;
     dim3 grid((n + kBlockThreads - 1) / kBlockThreads), block(kBlockThreads);
 #define DIA_LAUNCH(NDV)                                                                                         \
     hipLaunchKernelGGL((dia_kernel<OP, V, X, NDV>), grid, block, 0, s, n, d->args, d->nd, d->center, d->mask, val, \
-                       x, b, y, omega)
+                       x, b, y, omega, dotp)
     if (d->nd == 27) DIA_LAUNCH(27);
     else if (d->nd == 9) DIA_LAUNCH(9);
     else if (d->nd == 7) DIA_LAUNCH(7);
@@ -213,14 +271,14 @@ static void launch_dia_op(const ramd_csr_s* A, const V* val, const X* x, const X
 
 template <typename V, typename X>
 static void launch_dia(const ramd_csr_s* A, const V* val, int op, const X* x, const X* b, X* y, X omega,
-                       hipStream_t s)
+                       hipStream_t s, double* dotp)
 {
     switch (op) {
-    case OP_SPMV: launch_dia_op<OP_SPMV>(A, val, x, b, y, omega, s); break;
-    case OP_APPEND: launch_dia_op<OP_APPEND>(A, val, x, b, y, omega, s); break;
-    case OP_RESIDUAL: launch_dia_op<OP_RESIDUAL>(A, val, x, b, y, omega, s); break;
-    case OP_JACOBI: launch_dia_op<OP_JACOBI>(A, val, x, b, y, omega, s); break;
-    case OP_ADD_INTO: launch_dia_op<OP_ADD_INTO>(A, val, x, b, y, omega, s); break;
+    case OP_SPMV: launch_dia_op<OP_SPMV>(A, val, x, b, y, omega, s, dotp); break;
+    case OP_APPEND: launch_dia_op<OP_APPEND>(A, val, x, b, y, omega, s, nullptr); break;
+    case OP_RESIDUAL: launch_dia_op<OP_RESIDUAL>(A, val, x, b, y, omega, s, nullptr); break;
+    case OP_JACOBI: launch_dia_op<OP_JACOBI>(A, val, x, b, y, omega, s, nullptr); break;
+    case OP_ADD_INTO: launch_dia_op<OP_ADD_INTO>(A, val, x, b, y, omega, s, nullptr); break;
     default: fail(RAMD_E_ARG, "unknown stream op %d", op);
     }
 }
@@ -255,7 +313,7 @@ static double op_bytes(const ramd_csr_s* A, int op, int sv, int sx, bool dia)
 // val: CSR values; dval: stencil-diagonal values (nullptr -> CSR kernel)
 template <typename V, typename X>
 static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int op, const X* x, const X* b, X* y,
-                          X omega, hipStream_t s)
+                          X omega, hipStream_t s, double* dotp = nullptr)
 {
     if (A->n_rows == 0) return;
     const int nb = A->n_blocks;
@@ -273,7 +331,7 @@ static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int 
         if (dval) ctx->prof[pend.slot].dia_launches++;
     }
     if (dval) {
-        launch_dia(A, dval, op, x, b, y, omega, s);
+        launch_dia(A, dval, op, x, b, y, omega, s, dotp);
         if (prof) {
             RAMD_HIP(hipEventRecord(pend.end, s));
             ctx->prof_pending.push_back(pend);
@@ -283,23 +341,23 @@ static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int 
     switch (op) {
     case OP_SPMV:
         hipLaunchKernelGGL((stream_kernel<OP_SPMV, V, X>), grid, block, 0, s, A->rowptr, A->col, val, A->blocks,
-                           nb, x, b, y, omega, (int)A->diag_first);
+                           nb, x, b, y, omega, (int)A->diag_first, dotp);
         break;
     case OP_APPEND:
         hipLaunchKernelGGL((stream_kernel<OP_APPEND, V, X>), grid, block, 0, s, A->rowptr, A->col, val,
-                           A->blocks, nb, x, b, y, omega, (int)A->diag_first);
+                           A->blocks, nb, x, b, y, omega, (int)A->diag_first, nullptr);
         break;
     case OP_RESIDUAL:
         hipLaunchKernelGGL((stream_kernel<OP_RESIDUAL, V, X>), grid, block, 0, s, A->rowptr, A->col, val,
-                           A->blocks, nb, x, b, y, omega, (int)A->diag_first);
+                           A->blocks, nb, x, b, y, omega, (int)A->diag_first, nullptr);
         break;
     case OP_JACOBI:
         hipLaunchKernelGGL((stream_kernel<OP_JACOBI, V, X>), grid, block, 0, s, A->rowptr, A->col, val,
-                           A->blocks, nb, x, b, y, omega, (int)A->diag_first);
+                           A->blocks, nb, x, b, y, omega, (int)A->diag_first, nullptr);
         break;
     case OP_ADD_INTO:
         hipLaunchKernelGGL((stream_kernel<OP_ADD_INTO, V, X>), grid, block, 0, s, A->rowptr, A->col, val,
-                           A->blocks, nb, x, b, y, omega, (int)A->diag_first);
+                           A->blocks, nb, x, b, y, omega, (int)A->diag_first, nullptr);
         break;
     default:
         fail(RAMD_E_ARG, "unknown stream op %d", op);
@@ -349,27 +407,39 @@ void launch_stream_op32(const ramd_csr_s* A, int op, const float* x, const float
     launch_stream<float, float>(A, A->val32, A->dia ? A->dia->v32 : nullptr, op, x, b, y, omega, s);
 }
 
+double* ctx_partials(ramd_ctx ctx, size_t count)
+{
+    if (count > ctx->big_cap) {
+        RAMD_HIP(hipStreamSynchronize(ctx->stream));
+        dfree(ctx->big_partials);
+        ctx->big_partials = nullptr;
+        ctx->big_cap = 0;
+        ctx->big_partials = dalloc<double>(count);
+        ctx->big_cap = count;
+    }
+    return ctx->big_partials;
+}
+
+void launch_spmv_dot(const ramd_csr_s* A, bool values_fp32, const double* x, double* y, double* out_d,
+                     hipStream_t s)
+{
+    ramd_ctx ctx = A->ctx;
+    const bool dia32 = values_fp32 && A->dia && A->dia->v32 && A->dia->v32_exact;
+    const int np = A->dia ? (A->n_rows + kBlockThreads - 1) / kBlockThreads : A->n_blocks;
+    double* part = ctx_partials(ctx, (size_t)std::max(np, 1));
+    if (dia32)
+        launch_stream<float, double>(A, nullptr, A->dia->v32, OP_SPMV, x, nullptr, y, 0.0, s, part);
+    else
+        launch_stream<double, double>(A, A->val, A->dia ? A->dia->v64 : nullptr, OP_SPMV, x, nullptr, y, 0.0, s,
+                                      part);
+    launch_reduce_final(ctx, part, np, out_d);
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic reductions: a fixed grid of workgroups, each summing a fixed
 // strided subset in a fixed order, then one workgroup combining the partials
 // in index order.  Same inputs -> same bits, every run.
 // ---------------------------------------------------------------------------
-constexpr int kRedBlocks = 1024;
-
-__device__ __forceinline__ double block_sum(double v)
-{
-    __shared__ double wsum[kBlockThreads / 64];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) wsum[w] = v;
-    __syncthreads();
-    double t = 0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < kBlockThreads / 64; i++) t += wsum[i];
-    __syncthreads();
-    return t;
-}
-
 template <int MODE>  // 0: x.y   1: sum x^2 over |x| > 1e-16
 __global__ __launch_bounds__(kBlockThreads) void reduce_partial(const double* __restrict__ x,
                                                                 const double* __restrict__ y, int64_t n,
@@ -420,6 +490,20 @@ void launch_norm2sq(ramd_ctx ctx, int64_t n, const double* x, double* out_d)
                        ctx->red_partials);
     hipLaunchKernelGGL(reduce_final, dim3(1), dim3(kBlockThreads), 0, ctx->stream, ctx->red_partials, nb, out_d);
     RAMD_HIP(hipGetLastError());
+}
+
+void launch_reduce_final(ramd_ctx ctx, const double* partials, int np, double* out_d)
+{
+    hipLaunchKernelGGL(reduce_final, dim3(1), dim3(kBlockThreads), 0, ctx->stream, partials, np, out_d);
+    RAMD_HIP(hipGetLastError());
+}
+
+void fetch_scalars(ramd_ctx ctx, const double* d, int k, double* out)
+{
+    RAMD_CHECK(k >= 1 && k <= 16, RAMD_E_ARG, "fetch_scalars: %d scalars", k);
+    RAMD_HIP(hipMemcpyAsync(ctx->pinned, d, sizeof(double) * k, hipMemcpyDeviceToHost, ctx->stream));
+    RAMD_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < k; i++) out[i] = ctx->pinned[i];
 }
 
 double fetch_scalar(ramd_ctx ctx, const double* d)

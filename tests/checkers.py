@@ -72,7 +72,8 @@ def ref():
     global _ref
     if _ref is None:
         L = C.CDLL(os.path.join(REF_DIR, "libraptor_ref.so"))
-        for f in ("ref_gallery", "ref_strength", "ref_interp", "ref_spgemm", "ref_rap", "ref_hier_level"):
+        for f in ("ref_gallery", "ref_strength", "ref_interp", "ref_spgemm", "ref_rap", "ref_hier_level",
+                  "ref_fit_candidates", "ref_jacobi_prolongation"):
             getattr(L, f).restype = C.c_long
         L.ref_init()
         _ref = L

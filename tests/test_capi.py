@@ -35,3 +35,11 @@ def test_host_entry_points_reject_bad_input():
         ra.strength([0, 1], [5], [1.0], ra.Classical, 0.25)  # column out of range
     with pytest.raises(ra.RamdError):
         ra.split([0, 1], [0], 99, None)                       # unknown coarsening
+
+
+def test_torch_extension_loads():
+    """The libtorch autograd wrapper links against this library and exports amg_solve."""
+    pytest.importorskip("torch")
+    from raptor_amd import autograd
+    mod = autograd._load()
+    assert hasattr(mod, "amg_solve")

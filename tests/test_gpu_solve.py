@@ -16,8 +16,12 @@ needs_ref = pytest.mark.skipif(not K.have_ref(), reason="oracle/_ref not built")
 OMEGA = 2.0 / 3.0
 
 
-def build(ctx, rp, col, val, coarsen, interp, theta=0.25, max_coarse=50):
-    ml = ra.ParRugeStubenSolver(ctx, theta, coarsen, interp, ra.Classical, ra.Jacobi)
+def build(ctx, rp, col, val, cfg, max_coarse=50):
+    if cfg["solver"] == "sa":
+        ml = ra.ParSmoothedAggregationSolver(ctx, cfg["theta"], ra.Symmetric, ra.Jacobi)
+    else:
+        ml = ra.ParRugeStubenSolver(ctx, cfg.get("theta", 0.25), cfg["coarsen"], cfg["interp"], ra.Classical,
+                                    ra.Jacobi)
     ml.relax_weight = OMEGA
     ml.max_coarse = max_coarse
     ml.setup(rp, col, val)
@@ -34,22 +38,35 @@ def levels(ml):
     return As, Ps
 
 
+def rs(coarsen, interp):
+    return dict(solver="rs", coarsen=coarsen, interp=interp)
+
+
+def sa(theta):
+    return dict(solver="sa", theta=theta)
+
+
 CONFIGS = [
     # C1: aniso 100^2, CLJP + ModClassical
-    ("c1_aniso100", lambda: ra.gallery_aniso2d(100, 0.001, math.pi / 8), ra.CLJP, ra.ModClassical),
-    # C2 shape at reduced size: 27-pt, PMIS + Extended
-    ("lap27_16_pmis_ext", lambda: ra.gallery_laplace27(16), ra.PMIS, ra.Extended),
-    ("aniso60_pmis_direct", lambda: ra.gallery_aniso2d(60, 0.001, math.pi / 8), ra.PMIS, ra.Direct),
+    ("c1_aniso100", lambda: ra.gallery_aniso2d(100, 0.001, math.pi / 8), rs(ra.CLJP, ra.ModClassical)),
+    # C2 shape at reduced size: 27-pt, PMIS + Extended and HMIS + Extended
+    ("lap27_16_pmis_ext", lambda: ra.gallery_laplace27(16), rs(ra.PMIS, ra.Extended)),
+    ("lap27_12_hmis_ext", lambda: ra.gallery_laplace27(12), rs(ra.HMIS, ra.Extended)),
+    ("aniso60_pmis_direct", lambda: ra.gallery_aniso2d(60, 0.001, math.pi / 8), rs(ra.PMIS, ra.Direct)),
+    ("aniso50_rs_modclass", lambda: ra.gallery_aniso2d(50, 0.001, math.pi / 8), rs(ra.RS, ra.ModClassical)),
+    # C5 shape at reduced size: 7-point FV, smoothed aggregation theta 0.08
+    ("c5_fv7_20_sa", lambda: ra.gallery_fv7(20, seed=3), sa(0.08)),
+    ("aniso60_sa", lambda: ra.gallery_aniso2d(60, 0.001, math.pi / 8), sa(0.0)),
 ]
 
 
 @pytest.fixture(scope="module", params=CONFIGS, ids=[c[0] for c in CONFIGS])
 def problem(request, ctx):
-    name, gen, coarsen, interp = request.param
+    name, gen, cfg = request.param
     rp, col, val = gen()
-    ml = build(ctx, rp, col, val, coarsen, interp)
+    ml = build(ctx, rp, col, val, cfg)
     As, Ps = levels(ml)
-    return dict(name=name, rp=rp, col=col, val=val, ml=ml, As=As, Ps=Ps, coarsen=coarsen, interp=interp)
+    return dict(name=name, rp=rp, col=col, val=val, ml=ml, As=As, Ps=Ps, cfg=cfg)
 
 
 def test_hierarchy_galerkin_bit_exact(problem):
@@ -68,19 +85,31 @@ def test_hierarchy_galerkin_bit_exact(problem):
 
 @needs_ref
 def test_hierarchy_matches_reference(problem):
-    """Same number of levels, same C/F splittings, same P and A_l patterns
-    (exact) and values (1e-12 relative) as the reference's setup."""
+    """Same number of levels, same C/F splittings (or SA aggregates), same P
+    and A_l patterns (exact) and values (1e-12 relative) as the reference's
+    setup."""
     rp, col, val = problem["rp"], problem["col"], problem["val"]
     L = K.ref()
     rpa, cola, vala = K.i32(rp), K.i32(col), K.f64(val)
-    nl = L.ref_hier_setup(rpa.size - 1, K.ip(rpa), K.ip(cola), K.dp(vala), C.c_double(0.25), problem["coarsen"],
-                          problem["interp"], 0, 0, C.c_double(OMEGA), 50, 25, None)
+    cfg = problem["cfg"]
+    if cfg["solver"] == "sa":
+        nl = L.ref_hier_setup_sa(rpa.size - 1, K.ip(rpa), K.ip(cola), K.dp(vala), C.c_double(cfg["theta"]), 1, 0,
+                                 C.c_double(OMEGA), 50, 25, None, C.c_double(4.0 / 3), 1)
+    else:
+        nl = L.ref_hier_setup(rpa.size - 1, K.ip(rpa), K.ip(cola), K.dp(vala), C.c_double(0.25), cfg["coarsen"],
+                              cfg["interp"], 0, 0, C.c_double(OMEGA), 50, 25, None)
     ml, As, Ps = problem["ml"], problem["As"], problem["Ps"]
     assert nl == ml.num_levels
     for l in range(nl):
-        rpr, colr, valr, nc = K.ref_fetch(L.ref_hier_level(l, 0))
-        assert np.array_equal(As[l].rowptr, rpr) and np.array_equal(As[l].col, colr), l
-        assert np.allclose(As[l].val, valr, rtol=1e-12, atol=1e-300), l
+        R = K.Csr(*K.ref_fetch(L.ref_hier_level(l, 0)))
+        M = As[l]
+        if cfg["solver"] == "sa" and l == nl - 1:
+            # the reference leaves the coarsest SA operator in the product's
+            # ascending order (only the next level's strength would move its
+            # diagonal first); the dense coarse solve does not see the order
+            R, M = K.move_diag(K.sort_rows(R)), K.move_diag(K.sort_rows(M))
+        assert np.array_equal(M.rowptr, R.rowptr) and np.array_equal(M.col, R.col), l
+        assert np.allclose(M.val, R.val, rtol=1e-12, atol=1e-300), l
         if l < nl - 1:
             rpr, colr, valr, nc = K.ref_fetch(L.ref_hier_level(l, 1))
             assert nc == Ps[l].n_cols
@@ -162,7 +191,7 @@ def test_fpcg_reaches_true_residual(problem, ctx, prec):
 
 def test_pcg_indefinite_reports_error(ctx):
     rp, col, val = ra.gallery_aniso2d(30, 1.0, 0.0)
-    ml = build(ctx, rp, col, val, ra.CLJP, ra.ModClassical)
+    ml = build(ctx, rp, col, val, rs(ra.CLJP, ra.ModClassical))
     A = ra.ParCSRMatrix.from_csr(ctx, rp, col, -np.asarray(val))
     n = rp.size - 1
     x = ra.ParVector.from_numpy(ctx, np.zeros(n))
@@ -173,7 +202,7 @@ def test_pcg_indefinite_reports_error(ctx):
 
 def test_fpcg_nan_rhs_reports_error(ctx):
     rp, col, val = ra.gallery_aniso2d(30, 1.0, 0.0)
-    ml = build(ctx, rp, col, val, ra.CLJP, ra.ModClassical)
+    ml = build(ctx, rp, col, val, rs(ra.CLJP, ra.ModClassical))
     A = ra.ParCSRMatrix.from_csr(ctx, rp, col, val)
     n = rp.size - 1
     b = np.ones(n)
@@ -182,3 +211,38 @@ def test_fpcg_nan_rhs_reports_error(ctx):
     with pytest.raises(ra.RamdError) as e:
         ra.FPCG(A, ml, x, ra.ParVector.from_numpy(ctx, b), 1e-8, 50)
     assert e.value.code in (-7, -5)
+
+
+# ---------------------------------------------------------------------------
+# smoothed-aggregation prolongation (device products) in isolation
+# ---------------------------------------------------------------------------
+def test_sa_prolongation_golden(golden, ctx):
+    """aggregation/tests/test_par_smoothed_aggregation.cpp: jacobi_prolongation
+    of sas_A0 with sas_T0 (omega 4/3, one step) gives sas_P0, and of sas_A1
+    with sas_T1 gives sas_P1 (pattern exact, values 1e-10)."""
+    for a, t, p in (("sas_A0", "sas_T0", "sas_P0"), ("sas_A1", "sas_T1", "sas_P1")):
+        A = K.move_diag(K.fixture_csr(golden, a))
+        T = K.fixture_csr(golden, t)
+        prp, pcol, pval = ra.jacobi_prolongation(ctx, A.rowptr, A.col, A.val, T.rowptr, T.col, T.val, T.n_cols)
+        E = K.fixture_csr(golden, p)
+        M = K.sort_rows(K.Csr(prp, pcol, pval, T.n_cols))
+        assert np.array_equal(M.rowptr, E.rowptr) and np.array_equal(M.col, E.col), a
+        assert np.allclose(M.val, E.val, rtol=0, atol=1e-10), a
+
+
+@needs_ref
+@pytest.mark.parametrize("steps", [1, 2])
+def test_sa_prolongation_bit_exact_vs_ref(ctx, steps):
+    L = K.ref()
+    for rp, col, val in (ra.gallery_fv7(14, seed=5), ra.gallery_aniso2d(40, 0.001, math.pi / 8)):
+        A = K.move_diag(K.sort_rows(K.Csr(rp, col, val)))
+        n = A.n_rows
+        srp, scol = ra.strength(A.rowptr, A.col, A.val, ra.Symmetric, 0.08)
+        st = ra.mis2(srp, scol, np.random.default_rng(1).random(n))
+        agg, na = ra.aggregate(A.rowptr, A.col, A.val, srp, scol, st)
+        trp, tcol, tval, _ = ra.fit_candidates(agg, na, np.ones(n))
+        prp, pcol, pval = ra.jacobi_prolongation(ctx, A.rowptr, A.col, A.val, trp, tcol, tval, na, 4.0 / 3, steps)
+        nnz = L.ref_jacobi_prolongation(n, K.ip(A.rowptr), K.ip(A.col), K.dp(A.val), na, K.ip(trp), K.ip(tcol),
+                                        K.dp(tval), C.c_double(4.0 / 3), steps)
+        E = K.ref_fetch(nnz)
+        assert np.array_equal(prp, E[0]) and np.array_equal(pcol, E[1]) and np.array_equal(pval, E[2])

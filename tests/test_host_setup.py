@@ -185,3 +185,86 @@ def test_interp_bit_exact_vs_ref(itype, ctype):
         assert nc == E.n_cols
         # stored order matters downstream (it fixes the summation order of A*P)
         assert np.array_equal(prp, E.rowptr) and np.array_equal(pcol, E.col) and np.array_equal(pval, E.val)
+
+
+# ---------------------------------------------------------------------------
+# smoothed aggregation (aggregation/tests/test_par_smoothed_aggregation.cpp)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("level", [0, 1])
+def test_sa_pieces_golden(golden, level):
+    """Symmetric strength (0.25) -> mis2 with weights.txt -> aggregate ->
+    fit_candidates with B = 1, against sas_S/mis/agg/T of the reference's test
+    (aggregate numbers there index the selected rows in increasing order)."""
+    A = K.fixture_csr(golden, f"sas_A{level}", sort=False)
+    n = A.n_rows
+    w = golden["weights"][:n]
+    srp, scol = ra.strength(A.rowptr, A.col, A.val, ra.Symmetric, 0.25)
+    S, E = K.move_diag(K.sort_rows(K.Csr(srp, scol, np.ones(scol.size)))), K.fixture_csr(golden, f"sas_S{level}")
+    E = K.move_diag(E)
+    assert np.array_equal(S.rowptr, E.rowptr) and np.array_equal(S.col, E.col)
+    st = ra.mis2(srp, scol, w)
+    assert np.array_equal(st, golden[f"sas_mis{level}"])
+    agg, na = ra.aggregate(A.rowptr, A.col, A.val, srp, scol, st, w)
+    roots = np.flatnonzero(st > 0)
+    assert na == roots.size
+    assert np.array_equal(roots[golden[f"sas_agg{level}"]], agg)
+    trp, tcol, tval, R = ra.fit_candidates(agg, na, np.ones(n))
+    T = K.fixture_csr(golden, f"sas_T{level}")
+    assert np.array_equal(trp, T.rowptr) and np.array_equal(tcol, T.col)
+    assert np.allclose(tval, T.val, rtol=0, atol=1e-14)
+    # closed form: column k holds 1/sqrt(|aggregate k|); R holds sqrt(|aggregate k|)
+    sizes = np.bincount(tcol, minlength=na)
+    assert np.array_equal(tval, 1.0 / np.sqrt(sizes[tcol].astype(float)))
+    assert np.array_equal(R, np.sqrt(sizes.astype(float)))
+
+
+def test_mis2_properties():
+    """Distance-two independence and maximality on a 2-D 5-point graph."""
+    m = 30
+    n = m * m
+    rp, col = [0], []
+    for i in range(n):
+        x, y = divmod(i, m)
+        nb = [i] + [(x + dx) * m + (y + dy) for dx, dy in ((-1, 0), (1, 0), (0, -1), (0, 1))
+                    if 0 <= x + dx < m and 0 <= y + dy < m]
+        col += nb
+        rp.append(len(col))
+    rp, col = np.array(rp), np.array(col)
+    st = ra.mis2(rp, col, np.random.default_rng(5).random(n))
+    adj = [set(col[rp[i]:rp[i + 1]]) - {i} for i in range(n)]
+    sel = set(np.flatnonzero(st == 1))
+    for v in sel:
+        d2 = set(adj[v]).union(*[adj[u] for u in adj[v]]) - {v}
+        assert not (d2 & sel)
+    for v in set(range(n)) - sel:
+        d2 = set(adj[v]).union(*[adj[u] for u in adj[v]])
+        assert d2 & sel
+
+
+@needs_ref
+@pytest.mark.parametrize("theta", [0.0, 0.08, 0.25])
+def test_sa_pieces_bit_exact_vs_ref(theta):
+    L = K.ref()
+    for rp, col, val in (ra.gallery_aniso2d(60, 0.001, math.pi / 8), ra.gallery_laplace27(12),
+                         ra.gallery_fv7(14, seed=3)):
+        A = K.Csr(rp, col, val)
+        n = A.n_rows
+        srp, scol = ra.strength(rp, col, val, ra.Symmetric, theta)
+        w = np.random.default_rng(3).random(n)
+        st = ra.mis2(srp, scol, w)
+        st_ref = np.empty(n, np.int32)
+        L.ref_mis2(n, K.ip(srp), K.ip(scol), K.dp(w), K.ip(st_ref))
+        assert np.array_equal(st, st_ref)
+        for rv in (None, w):
+            agg, na = ra.aggregate(rp, col, val, srp, scol, st, rv)
+            agg_ref = np.empty(n, np.int32)
+            na_ref = L.ref_aggregate(n, K.ip(A.rowptr), K.ip(A.col), K.dp(A.val), K.ip(srp), K.ip(scol),
+                                     K.ip(st_ref), None if rv is None else K.dp(rv), K.ip(agg_ref))
+            assert na == na_ref and np.array_equal(agg, agg_ref)
+        B = np.random.default_rng(4).random(n) + 0.5
+        trp, tcol, tval, R = ra.fit_candidates(agg, na, B)
+        R_ref = np.zeros(na)
+        E = K.ref_fetch(L.ref_fit_candidates(n, K.ip(A.rowptr), K.ip(A.col), K.dp(A.val), K.ip(agg_ref), na_ref,
+                                             K.dp(B), K.dp(R_ref)))
+        assert np.array_equal(trp, E[0]) and np.array_equal(tcol, E[1]) and np.array_equal(tval, E[2])
+        assert np.array_equal(R, R_ref)
