@@ -258,6 +258,27 @@ int ramd_fpcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double rt
               int max_iter, int cycle_prec, double* relres_hist, int32_t* iters,
               double* final_relres);
 
+/* k right-hand sides at once (1 <= k <= 64).  Blocks are n x k row-major
+ * ("row-interleaved": entry (i, c) at i*k + c), FP64 unless stated.
+ * No reference counterpart (the reference solves one vector at a time). */
+/* Lockstep flexible PCG: column c is solved as ramd_fpcg would solve it
+ * (own scalars, own stopping test on its true relative residual), all
+ * columns sharing each operator and V-cycle launch.  use_graph != 0
+ * replays the iteration as two captured HIP graphs.  iters_col and
+ * relres_col (k entries, may be NULL) receive each column's iteration count
+ * and final relative residual; *iters_total the number of lockstep
+ * iterations. */
+int ramd_fpcg_block(ramd_hier h, ramd_csr A, int32_t k, double* X_d, const double* B_d, double rtol,
+                    int max_iter, int cycle_prec, int use_graph, int32_t* iters_col, double* relres_col,
+                    int32_t* iters_total);
+/* One V-cycle from X = 0 for k columns in precision prec (X, B float* for
+ * RAMD_PREC_FP32); column c is bit-identical to ramd_vcycle (FP64) or
+ * ramd_vcycle_fp32 applied to column c. */
+int ramd_vcycle_block(ramd_hier h, int32_t k, int prec, void* X_d, const void* B_d);
+/* Lanes-per-row chosen by the multi-vector autotuner for k columns: three
+ * entries per level (A, P, R; 0 where absent). */
+int ramd_block_tuning(ramd_hier h, int32_t k, int32_t* lpr);
+
 /* Adjoint of x = A^-1 b for a symmetric positive definite A (no reference
  * counterpart: the reference has no differentiation).  Given the solution
  * x and g = dL/dx, solves A^T lambda = g by the same flexible PCG and

@@ -3,6 +3,8 @@
 // to obtain the reference's own results at one MPI rank; the product never
 // links or calls it.  Outputs of variable size are stashed and fetched with
 // ref_fetch().
+#include <map>
+
 #include "raptor.hpp"
 
 using namespace raptor;
@@ -219,7 +221,13 @@ int ref_hier_states(int l, int* states)
         aligned_vector<int> agg;
         mis2(S, st, off, false, g_weights.data());
         aggregate(A, S, st, off, agg, false);
-        std::copy(agg.begin(), agg.end(), states);
+        // aggregates are global column ids of A; below the fine level the
+        // coarse operators keep the fine root rows as their global column
+        // ids (fit_candidates copies them into T's column map), so map them
+        // back to local indices
+        std::map<int, int> local;
+        for (int i = 0; i < (int)A->on_proc_column_map.size(); i++) local[A->on_proc_column_map[i]] = i;
+        for (size_t i = 0; i < agg.size(); i++) states[i] = agg[i] < 0 ? agg[i] : local.at(agg[i]);
         delete S;
         return (int)agg.size();
     }

@@ -34,7 +34,7 @@ from ._lib import (CLJP, HMIS, PMIS, RS, Classical, Direct, Extended, Falgout, J
 
 __all__ = [
     "Context", "ParVector", "ParCSRMatrix", "ParMultilevel", "ParRugeStubenSolver",
-    "ParSmoothedAggregationSolver", "PCG", "FPCG", "adjoint", "strength", "split", "interpolation", "mis2",
+    "ParSmoothedAggregationSolver", "PCG", "FPCG", "FPCG_block", "vcycle_block", "block_tuning", "adjoint", "strength", "split", "interpolation", "mis2",
     "aggregate", "fit_candidates", "jacobi_prolongation", "gallery_laplace27", "gallery_aniso2d", "gallery_fv7", "RamdError",
     "RS", "CLJP", "Falgout", "PMIS", "HMIS", "Direct", "ModClassical", "Extended", "Classical",
     "Symmetric", "Jacobi", "PREC_FP64", "PREC_FP32",
@@ -390,6 +390,31 @@ def FPCG(A, ml, x, b, rtol=1e-8, max_iter=200, cycle_prec=PREC_FP64):
     check(lib().ramd_fpcg(ml.h, A.h, x.ptr, b.ptr, float(rtol), int(max_iter), int(cycle_prec),
                           _ptr(hist, C.c_double), C.byref(it), C.byref(fin)))
     return hist[:it.value + 1], it.value, fin.value
+
+
+def FPCG_block(A, ml, X, B, k, rtol=1e-8, max_iter=200, cycle_prec=PREC_FP64, use_graph=True):
+    """k right-hand sides (row-interleaved n x k device blocks) solved in
+    lockstep (ramd_fpcg_block).  Returns (per-column iterations, per-column
+    final relative residuals, lockstep iterations)."""
+    its = np.zeros(k, np.int32)
+    rel = np.zeros(k)
+    tot = C.c_int32()
+    check(lib().ramd_fpcg_block(ml.h, A.h, int(k), X.ptr, B.ptr, float(rtol), int(max_iter), int(cycle_prec),
+                                1 if use_graph else 0, _ptr(its, C.c_int32), _ptr(rel, C.c_double),
+                                C.byref(tot)))
+    return its, rel, tot.value
+
+
+def vcycle_block(ml, X, B, k, prec=PREC_FP64):
+    """One V-cycle from X = 0 on k row-interleaved columns (ramd_vcycle_block)."""
+    check(lib().ramd_vcycle_block(ml.h, int(k), int(prec), X.ptr, B.ptr))
+
+
+def block_tuning(ml, k):
+    """Lanes per row picked by the autotuner: array (levels, 3) for A, P, R."""
+    out = np.zeros(3 * ml.num_levels, np.int32)
+    check(lib().ramd_block_tuning(ml.h, int(k), _ptr(out, C.c_int32)))
+    return out.reshape(-1, 3)
 
 
 def adjoint(A, ml, x, g, rtol=1e-10, max_iter=500, cycle_prec=PREC_FP64, with_dA=True):

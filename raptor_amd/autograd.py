@@ -13,6 +13,9 @@ Python or CPU fallback.
     x = raptor_amd.autograd.amg_solve(vals, b, ml, A)
     loss = 0.5 * ((x - x_star) ** 2).sum(); loss.backward()   # vals.grad, b.grad
 
+Import torch before the first raptor_amd call: the library then binds to
+torch's HIP runtime (one process cannot use two HIP runtimes).
+
 The hierarchy stays that of the original values: when vals moves far from
 them, the solve still converges to rtol (the hierarchy is only the
 preconditioner) but needs more iterations.
@@ -26,15 +29,32 @@ from ._lib import PREC_FP64
 _ext = None
 
 
+def _hip_runtimes():
+    """Paths of the HIP runtime libraries mapped into this process."""
+    paths = set()
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                p = line.split()[-1] if len(line.split()) >= 6 else ""
+                if "libamdhip64" in os.path.basename(p):
+                    paths.add(os.path.realpath(p))
+    except OSError:
+        pass
+    return paths
+
+
 def _load():
     global _ext
     if _ext is None:
+        import torch  # noqa: F401  (libtorch and its HIP runtime before the library)
         _lib.lib()  # libraptor_amd.so first, so the extension binds to the same instance
+        if len(_hip_runtimes()) > 1:
+            raise RuntimeError("two HIP runtimes are loaded (libraptor_amd.so was loaded before torch): "
+                               "import torch before the first raptor_amd call when using raptor_amd.autograd")
         path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_torch_ext.so")
         if not os.path.exists(path):
             raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() "
                                "(make -C raptor_amd/csrc torch_ext ...)")
-        import torch  # noqa: F401  (libtorch must be loaded before the extension)
         spec = importlib.util.spec_from_file_location("raptor_amd._torch_ext", path)
         mod = importlib.util.module_from_spec(spec)
         spec.loader.exec_module(mod)

@@ -1,0 +1,882 @@
+// Multi-vector path: k right-hand sides (k <= 64) held row-interleaved
+// (entry (i, c) at i*k + c), one V-cycle for all of them per launch, and a
+// lockstep flexible PCG with one set of scalars per right-hand side.
+//
+// Every row is handled by a group of LPR lanes; lane g of the group owns
+// columns g, g+LPR, ...  Each column's sum runs over the row in stored order
+// with exactly the single-vector kernels' operation sequence, so column c
+// of a block V-cycle is bit-identical to the single-vector V-cycle applied
+// to column c.  The matrix entries are read once per row for all k columns.
+//
+// LPR is picked per matrix by a small autotuner (timed trial launches on
+// the real operator, see block_tune) and cached in the hierarchy.
+//
+// The flexible-PCG iteration is split into two captured HIP graphs (the
+// operator step with the convergence test, then the preconditioner step);
+// all scalars live on the device, and one 16-byte read per iteration tells
+// the host whether to continue.  No memset node is recorded: buffers are
+// cleared by kernels.
+#include <cmath>
+#include <cstring>
+#include <functional>
+
+#include "hier.hpp"
+
+namespace ramd {
+
+constexpr int kMaxRhs = 64;
+
+// ---------------------------------------------------------------------------
+// SpMM family (CSR and stencil-diagonal forms)
+// ---------------------------------------------------------------------------
+template <int OP, typename V, typename X, int LPR>
+__global__ __launch_bounds__(256) void bcsr_kernel(int n, int k, const int* __restrict__ rp,
+                                                   const int* __restrict__ col, const V* __restrict__ val,
+                                                   const X* __restrict__ x, const X* __restrict__ b,
+                                                   X* __restrict__ y, X omega)
+{
+    constexpr int RPW = 256 / LPR;
+    constexpr int CPL = kMaxRhs / LPR;  // columns per lane (upper bound)
+    const int row = xcd_remap(blockIdx.x, gridDim.x) * RPW + threadIdx.x / LPR;
+    const int g = threadIdx.x % LPR;
+    if (row >= n || g >= k) return;
+    const int s = rp[row], e = rp[row + 1];
+    const size_t o = (size_t)row * k;
+    X acc[CPL], diag = 0;
+#pragma unroll
+    for (int q = 0; q < CPL; q++) {
+        const int c = g + q * LPR;
+        acc[q] = (OP == OP_RESIDUAL && c < k) ? b[o + c] : X(0);
+    }
+    for (int j = s; j < e; j++) {
+        const int cj = col[j];
+        const X a = X(val[j]);
+        const size_t oj = (size_t)cj * k;
+        if (OP == OP_JACOBI && cj == row) {
+            diag = a;
+            continue;
+        }
+#pragma unroll
+        for (int q = 0; q < CPL; q++) {
+            const int c = g + q * LPR;
+            if (c < k) {
+                const X p = a * x[oj + c];
+                if (OP == OP_RESIDUAL) acc[q] -= p;
+                else acc[q] += p;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < CPL; q++) {
+        const int c = g + q * LPR;
+        if (c >= k) continue;
+        if (OP == OP_SPMV || OP == OP_RESIDUAL) y[o + c] = acc[q];
+        else if (OP == OP_APPEND) y[o + c] += acc[q];
+        else if (OP == OP_ADD_INTO) y[o + c] = b[o + c] + acc[q];
+        else {
+            const X xo = x[o + c];
+            y[o + c] = (s != e && fabs(diag) > X(1e-16))
+                           ? ((X(1.0) - omega) * xo) + (omega * ((b[o + c] - acc[q]) / diag))
+                           : xo;
+        }
+    }
+}
+
+template <int OP, typename V, typename X, int LPR, int ND>
+__global__ __launch_bounds__(256) void bdia_kernel(int n, int k, DiaArgs a, int nd, int center,
+                                                   const unsigned* __restrict__ mask, const V* __restrict__ val,
+                                                   const X* __restrict__ x, const X* __restrict__ b,
+                                                   X* __restrict__ y, X omega)
+{
+    constexpr int RPW = 256 / LPR;
+    constexpr int CPL = kMaxRhs / LPR;
+    const int row = xcd_remap(blockIdx.x, gridDim.x) * RPW + threadIdx.x / LPR;
+    const int g = threadIdx.x % LPR;
+    if (row >= n || g >= k) return;
+    const unsigned m = mask[row];
+    const size_t o = (size_t)row * k;
+    X acc[CPL], diag = 0;
+#pragma unroll
+    for (int q = 0; q < CPL; q++) {
+        const int c = g + q * LPR;
+        acc[q] = (OP == OP_RESIDUAL && c < k) ? b[o + c] : X(0);
+    }
+#pragma unroll
+    for (int kk = 0; kk < ND; kk++) {
+        const bool live = (ND == kDiaMax) ? (kk < nd) : true;
+        const int d = live ? a.ord[kk] : 0;
+        const bool on = live && ((m >> d) & 1u);
+        const X v = X(val[(size_t)d * n + row]);
+        const size_t oj = (size_t)(on ? row + a.off[d] : row) * k;
+        if (OP == OP_JACOBI && d == center) {
+            diag = on ? v : diag;
+            continue;
+        }
+#pragma unroll
+        for (int q = 0; q < CPL; q++) {
+            const int c = g + q * LPR;
+            if (c < k) {
+                const X p = v * x[oj + c];
+                if (OP == OP_RESIDUAL) acc[q] = on ? acc[q] - p : acc[q];
+                else acc[q] = on ? acc[q] + p : acc[q];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < CPL; q++) {
+        const int c = g + q * LPR;
+        if (c >= k) continue;
+        if (OP == OP_SPMV || OP == OP_RESIDUAL) y[o + c] = acc[q];
+        else if (OP == OP_APPEND) y[o + c] += acc[q];
+        else if (OP == OP_ADD_INTO) y[o + c] = b[o + c] + acc[q];
+        else {
+            const X xo = x[o + c];
+            y[o + c] = (m != 0u && fabs(diag) > X(1e-16))
+                           ? ((X(1.0) - omega) * xo) + (omega * ((b[o + c] - acc[q]) / diag))
+                           : xo;
+        }
+    }
+}
+
+template <int OP, typename V, typename X, int LPR>
+static void blaunch_op(const ramd_csr_s* A, const V* val, const V* dval, int k, const X* x, const X* b, X* y,
+                       X omega, hipStream_t s)
+{
+    const int n = A->n_rows;
+    if (n == 0) return;
+    const dim3 grid((unsigned)((n + 256 / LPR - 1) / (256 / LPR))), block(256);
+    if (dval) {
+        const Dia* d = A->dia;
+#define BDIA(NDV)                                                                                                  \
+    hipLaunchKernelGGL((bdia_kernel<OP, V, X, LPR, NDV>), grid, block, 0, s, n, k, d->args, d->nd, d->center, \
+                       d->mask, dval, x, b, y, omega)
+        if (d->nd == 27) BDIA(27);
+        else if (d->nd == 9) BDIA(9);
+        else if (d->nd == 7) BDIA(7);
+        else BDIA(kDiaMax);
+#undef BDIA
+    } else {
+        hipLaunchKernelGGL((bcsr_kernel<OP, V, X, LPR>), grid, block, 0, s, n, k, A->rowptr, A->col, val, x, b, y,
+                           omega);
+    }
+    RAMD_HIP(hipGetLastError());
+}
+
+template <typename V, typename X, int LPR>
+static void blaunch_lpr(const ramd_csr_s* A, const V* val, const V* dval, int op, int k, const X* x, const X* b,
+                        X* y, X omega, hipStream_t s)
+{
+    switch (op) {
+    case OP_SPMV: blaunch_op<OP_SPMV, V, X, LPR>(A, val, dval, k, x, b, y, omega, s); break;
+    case OP_RESIDUAL: blaunch_op<OP_RESIDUAL, V, X, LPR>(A, val, dval, k, x, b, y, omega, s); break;
+    case OP_JACOBI: blaunch_op<OP_JACOBI, V, X, LPR>(A, val, dval, k, x, b, y, omega, s); break;
+    case OP_ADD_INTO: blaunch_op<OP_ADD_INTO, V, X, LPR>(A, val, dval, k, x, b, y, omega, s); break;
+    default: fail(RAMD_E_ARG, "block op %d not supported", op);
+    }
+}
+
+// lanes per row: 8, 16, 32 or 64 (at least k/8, since a lane holds <= 8 columns)
+static bool lpr_ok(int lpr, int k) { return lpr * 8 >= k && (lpr <= 8 || lpr / 2 < k); }
+
+template <typename X>
+static void blaunch(const ramd_csr_s* A, int lpr, int op, int k, const X* x, const X* b, X* y, X omega,
+                    hipStream_t s)
+{
+    // FP64 vectors read FP64 values; FP32 vectors read the FP32 copies
+    using V = X;
+    const V* val;
+    const V* dval;
+    if constexpr (sizeof(X) == 8) {
+        val = (const V*)A->val;
+        dval = A->dia ? (const V*)A->dia->v64 : nullptr;
+    } else {
+        RAMD_CHECK(A->val32 && (!A->dia || A->dia->v32), RAMD_E_STATE, "FP32 copies of the matrix were not built");
+        val = (const V*)A->val32;
+        dval = A->dia ? (const V*)A->dia->v32 : nullptr;
+    }
+    switch (lpr) {
+    case 8: blaunch_lpr<V, X, 8>(A, val, dval, op, k, x, b, y, omega, s); break;
+    case 16: blaunch_lpr<V, X, 16>(A, val, dval, op, k, x, b, y, omega, s); break;
+    case 32: blaunch_lpr<V, X, 32>(A, val, dval, op, k, x, b, y, omega, s); break;
+    case 64: blaunch_lpr<V, X, 64>(A, val, dval, op, k, x, b, y, omega, s); break;
+    default: fail(RAMD_E_ARG, "lanes per row %d", lpr);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// elementwise and per-column kernels.  Layout of the per-column kernels:
+// thread t of a workgroup owns column t % 64 and row offset t / 64; rows
+// advance by 4 * gridDim.x.  Sums are combined in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr int kColBlocks = 512;
+
+template <typename X>
+__global__ void bjacobi_from_zero(int n, int k, const X* __restrict__ d, const X* __restrict__ b, X* __restrict__ x,
+                                  X omega, const int* __restrict__ rp)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * k) return;
+    const int i = (int)(e / k);
+    const X di = d[i];
+    const X zero = X(0.0);
+    X v = zero;
+    if (rp[i + 1] > rp[i] && fabs(di) > X(1e-16)) v = ((X(1.0) - omega) * zero) + (omega * ((b[e] - zero) / di));
+    x[e] = v;
+}
+
+template <typename A, typename B>
+__global__ void bconvert(int64_t m, const A* __restrict__ a, B* __restrict__ b)
+{
+    const int64_t s = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += s) b[i] = (B)a[i];
+}
+
+__global__ void bfill(int64_t m, double v, double* __restrict__ a)
+{
+    const int64_t s = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += s) a[i] = v;
+}
+
+// dgetrs_('N') for column blockIdx.x of a row-interleaved block (see coarse_solve_kernel)
+template <typename X>
+__global__ __launch_bounds__(256) void bcoarse_solve(int n, int k, const double* __restrict__ lu,
+                                                     const int* __restrict__ piv, const X* __restrict__ b,
+                                                     X* __restrict__ x)
+{
+    extern __shared__ double sb[];
+    const int c = blockIdx.x;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) sb[i] = (double)b[(size_t)i * k + c];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int j = 0; j < n; j++) {
+            const int p = piv[j];
+            if (p != j) {
+                const double t = sb[j];
+                sb[j] = sb[p];
+                sb[p] = t;
+            }
+        }
+    __syncthreads();
+    for (int j = 0; j < n; j++) {
+        const double bj = sb[j];
+        for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) sb[i] -= bj * lu[i + (size_t)j * n];
+        __syncthreads();
+    }
+    for (int j = n - 1; j >= 0; j--) {
+        if (threadIdx.x == 0) sb[j] /= lu[j + (size_t)j * n];
+        __syncthreads();
+        const double bj = sb[j];
+        for (int i = threadIdx.x; i < j; i += blockDim.x) sb[i] -= bj * lu[i + (size_t)j * n];
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) x[(size_t)i * k + c] = (X)sb[i];
+}
+
+// Combines the four row-offset sums of each column in LDS and stores the
+// workgroup's partial for column c at partials[blockIdx.x * k + c].
+__device__ __forceinline__ void col_partial(double v, int k, double* __restrict__ partials)
+{
+    __shared__ double sh[256];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x < k)
+        partials[(size_t)blockIdx.x * k + threadIdx.x] =
+            ((sh[threadIdx.x] + sh[threadIdx.x + 64]) + sh[threadIdx.x + 128]) + sh[threadIdx.x + 192];
+    __syncthreads();
+}
+
+// per-column a.b (MODE 0) or r.z and z.w (MODE 1: two partial arrays)
+template <int MODE, typename Z>
+__global__ __launch_bounds__(256) void bcol_dots(int n, int k, const double* __restrict__ a, const Z* __restrict__ z,
+                                                 const double* __restrict__ w, double* __restrict__ partials)
+{
+    const int c = threadIdx.x & 63;
+    double s1 = 0, s2 = 0;
+    if (c < k)
+        for (int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * 4) {
+            const size_t e = (size_t)i * k + c;
+            const double zi = (double)z[e];
+            s1 += a[e] * zi;
+            if (MODE == 1) s2 += zi * w[e];
+        }
+    col_partial(s1, k, partials);
+    if (MODE == 1) col_partial(s2, k, partials + (size_t)gridDim.x * k);
+}
+
+// x += alpha p; r -= alpha Ap; r32 = (float) r; per-column sums of r^2
+__global__ __launch_bounds__(256) void bupdate(int n, int k, const double* __restrict__ alpha,
+                                               const double* __restrict__ p, const double* __restrict__ Ap,
+                                               double* __restrict__ x, double* __restrict__ r,
+                                               float* __restrict__ r32, double* __restrict__ partials)
+{
+    const int c = threadIdx.x & 63;
+    double s = 0;
+    if (c < k) {
+        const double al = alpha[c];
+        for (int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * 4) {
+            const size_t e = (size_t)i * k + c;
+            x[e] += al * p[e];
+            const double ri = r[e] - al * Ap[e];
+            r[e] = ri;
+            if (r32) r32[e] = (float)ri;
+            s += ri * ri;
+        }
+    }
+    col_partial(s, k, partials);
+}
+
+// p = z + beta p
+template <typename Z>
+__global__ __launch_bounds__(256) void bdir(int n, int k, const double* __restrict__ beta, const Z* __restrict__ z,
+                                            double* __restrict__ p)
+{
+    const int c = threadIdx.x & 63;
+    if (c >= k) return;
+    const double be = beta[c];
+    for (int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * 4) {
+        const size_t e = (size_t)i * k + c;
+        p[e] = (double)z[e] + be * p[e];
+    }
+}
+
+// out[c] = sum over g of partials[g * k + c], in g order (one workgroup)
+__global__ void bcol_final(int np, int k, const double* __restrict__ partials, double* __restrict__ out)
+{
+    const int c = threadIdx.x;
+    if (c >= k) return;
+    double s = 0;
+    for (int g = 0; g < np; g++) s += partials[(size_t)g * k + c];
+    out[c] = s;
+}
+
+// Device scalars of the lockstep flexible PCG, one entry per column.
+struct BScal {
+    double *bb, *rr, *rz, *rzn, *zap, *pap, *alpha, *beta, *scale, *relres, *active, *iters;
+    double* flags;  // [0] active count, [1] error (1 indefinite, 2 non-finite)
+};
+
+__global__ void bscal_init(int k, double rtol, BScal S)
+{
+    const int c = threadIdx.x;
+    if (c < k) {
+        const double bn = sqrt(S.bb[c]);
+        S.scale[c] = bn > 0 ? bn : 1.0;
+        S.relres[c] = sqrt(S.rr[c]) / S.scale[c];
+        S.active[c] = S.relres[c] > rtol ? 1.0 : 0.0;
+        S.iters[c] = 0;
+        if (!isfinite(bn) || !isfinite(S.relres[c])) S.flags[1] = 2;
+    }
+    __syncthreads();
+    if (c == 0) {
+        double a = 0;
+        for (int i = 0; i < k; i++) a += S.active[i];
+        S.flags[0] = a;
+    }
+}
+
+__global__ void bscal_alpha(int k, BScal S)
+{
+    const int c = threadIdx.x;
+    if (c >= k) return;
+    double al = 0;
+    if (S.active[c] != 0) {
+        if (!(S.pap[c] > 0)) S.flags[1] = isfinite(S.pap[c]) ? 1 : 2;
+        else al = S.rz[c] / S.pap[c];
+    }
+    S.alpha[c] = al;
+}
+
+__global__ void bscal_relres(int k, double rtol, BScal S)
+{
+    const int c = threadIdx.x;
+    if (c < k && S.active[c] != 0) {
+        S.iters[c] += 1;
+        const double rel = sqrt(S.rr[c]) / S.scale[c];
+        S.relres[c] = rel;
+        if (!isfinite(rel)) S.flags[1] = 2;
+        if (rel <= rtol) S.active[c] = 0;
+    }
+    __syncthreads();
+    if (c == 0) {
+        double a = 0;
+        for (int i = 0; i < k; i++) a += S.active[i];
+        S.flags[0] = a;
+    }
+}
+
+__global__ void bscal_beta(int k, BScal S)
+{
+    const int c = threadIdx.x;
+    if (c >= k) return;
+    double be = 0;
+    if (S.active[c] != 0) {
+        be = -S.alpha[c] * S.zap[c] / S.rz[c];
+        S.rz[c] = S.rzn[c];
+    }
+    S.beta[c] = be;
+}
+
+// after the true-residual check: reactivate columns that are not below rtol
+__global__ void bscal_recheck(int k, double rtol, BScal S)
+{
+    const int c = threadIdx.x;
+    if (c < k) {
+        const double rel = sqrt(S.rr[c]) / S.scale[c];
+        S.relres[c] = rel;
+        S.active[c] = rel > rtol ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (c == 0) {
+        double a = 0;
+        for (int i = 0; i < k; i++) a += S.active[i];
+        S.flags[0] = a;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// work space
+// ---------------------------------------------------------------------------
+struct BlockLevel {
+    double *x = nullptr, *b = nullptr, *t = nullptr, *r = nullptr;
+    float *x32 = nullptr, *b32 = nullptr, *t32 = nullptr, *r32 = nullptr;
+    int lpr_A = 0, lpr_P = 0, lpr_R = 0;
+};
+
+struct BlockState {
+    int k = 0;
+    std::vector<BlockLevel> lv;
+    double *X = nullptr, *B = nullptr, *R = nullptr, *Z = nullptr, *P = nullptr, *AP = nullptr;
+    float *R32 = nullptr, *Z32 = nullptr;
+    double* scal = nullptr;     // 16 arrays of kMaxRhs
+    double* partials = nullptr;  // 2 * kColBlocks * kMaxRhs
+    double* pinned = nullptr;    // host, 2 + 2 * kMaxRhs
+    BScal S{};
+    // captured iteration (operator step, preconditioner step) for one (A, precision)
+    hipGraphExec_t gA = nullptr, gB = nullptr;
+    const ramd_csr_s* g_A = nullptr;
+    const void* g_sig[4] = {nullptr, nullptr, nullptr, nullptr};  // buffers of the captured operator
+    int g_prec = -1;
+    double g_rtol = -1;
+};
+
+static void free_graphs(BlockState* b)
+{
+    if (b->gA) (void)hipGraphExecDestroy(b->gA);
+    if (b->gB) (void)hipGraphExecDestroy(b->gB);
+    b->gA = b->gB = nullptr;
+    b->g_A = nullptr;
+}
+
+void block_free(BlockState* b)
+{
+    if (!b) return;
+    free_graphs(b);
+    for (BlockLevel& L : b->lv)
+        for (void* p : {(void*)L.x, (void*)L.b, (void*)L.t, (void*)L.r, (void*)L.x32, (void*)L.b32, (void*)L.t32,
+                        (void*)L.r32})
+            dfree(p);
+    for (void* p : {(void*)b->X, (void*)b->B, (void*)b->R, (void*)b->Z, (void*)b->P, (void*)b->AP, (void*)b->R32,
+                    (void*)b->Z32, (void*)b->scal, (void*)b->partials})
+        dfree(p);
+    if (b->pinned) (void)hipHostFree(b->pinned);
+    delete b;
+}
+
+static double time_ms(hipStream_t s, const std::function<void()>& fn, int reps)
+{
+    hipEvent_t e0, e1;
+    RAMD_HIP(hipEventCreate(&e0));
+    RAMD_HIP(hipEventCreate(&e1));
+    fn();  // warm
+    RAMD_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; i++) fn();
+    RAMD_HIP(hipEventRecord(e1, s));
+    RAMD_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    RAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / reps;
+}
+
+// Autotuner: lanes per row for one operator and k, by timing the residual
+// (FP64) or the FP32 residual on the level's own block vectors.
+template <typename X>
+static int block_tune(ramd_hier_s* h, const ramd_csr_s* M, int k, const X* x, const X* b, X* y)
+{
+    hipStream_t s = h->ctx->stream;
+    int best = 0;
+    double best_ms = 1e30;
+    for (int lpr : {8, 16, 32, 64}) {
+        if (!lpr_ok(lpr, k)) continue;
+        const double ms =
+            time_ms(s, [&] { blaunch<X>(M, lpr, M->n_rows == M->n_cols ? OP_RESIDUAL : OP_SPMV, k, x, b, y, X(0), s); },
+                    3);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = lpr;
+        }
+    }
+    return best;
+}
+
+static BlockState* block_state(ramd_hier_s* h, int k, bool fp32)
+{
+    BlockState*& b = h->blk;
+    if (b && b->k != k) {
+        RAMD_HIP(hipStreamSynchronize(h->ctx->stream));
+        block_free(b);
+        b = nullptr;
+    }
+    if (!b) {
+        b = new BlockState();
+        b->k = k;
+        b->lv.resize(h->levels.size());
+        for (size_t l = 0; l < h->levels.size(); l++) {
+            const size_t m = (size_t)h->levels[l].n * k;
+            BlockLevel& L = b->lv[l];
+            L.x = dalloc<double>(m);
+            L.b = dalloc<double>(m);
+            L.t = dalloc<double>(m);
+            L.r = dalloc<double>(m);
+        }
+        const size_t m0 = (size_t)h->levels[0].n * k;
+        b->X = dalloc<double>(m0);
+        b->B = dalloc<double>(m0);
+        b->R = dalloc<double>(m0);
+        b->Z = dalloc<double>(m0);
+        b->P = dalloc<double>(m0);
+        b->AP = dalloc<double>(m0);
+        b->scal = dalloc<double>(16 * kMaxRhs);
+        b->partials = dalloc<double>(2 * (size_t)kColBlocks * kMaxRhs);
+        RAMD_HIP(hipHostMalloc((void**)&b->pinned, sizeof(double) * (2 + 2 * kMaxRhs), hipHostMallocDefault));
+        double** f[] = {&b->S.bb, &b->S.rr, &b->S.rz, &b->S.rzn, &b->S.zap, &b->S.pap, &b->S.alpha, &b->S.beta,
+                        &b->S.scale, &b->S.relres, &b->S.active, &b->S.iters, &b->S.flags};
+        for (int i = 0; i < 13; i++) *f[i] = b->scal + i * kMaxRhs;
+        hipLaunchKernelGGL(bfill, dim3(4), dim3(256), 0, h->ctx->stream, (int64_t)16 * kMaxRhs, 0.0, b->scal);
+        RAMD_HIP(hipGetLastError());
+        // tune every level's operators in FP64 on the level vectors
+        for (size_t l = 0; l < h->levels.size(); l++) {
+            const int64_t m = (int64_t)h->levels[l].n * k;
+            for (double* v : {b->lv[l].x, b->lv[l].b, b->lv[l].t, b->lv[l].r})
+                hipLaunchKernelGGL(bfill, dim3(256), dim3(256), 0, h->ctx->stream, m, 1.0, v);
+            RAMD_HIP(hipGetLastError());
+        }
+        for (size_t l = 0; l + 1 < h->levels.size(); l++) {
+            Level& Lv = h->levels[l];
+            BlockLevel& L = b->lv[l];
+            L.lpr_A = block_tune<double>(h, Lv.A, k, L.x, L.b, L.r);
+            L.lpr_P = block_tune<double>(h, Lv.P, k, b->lv[l + 1].x, L.b, L.r);
+            L.lpr_R = block_tune<double>(h, Lv.R, k, L.x, b->lv[l + 1].b, b->lv[l + 1].r);
+        }
+    }
+    if (fp32 && !b->R32) {
+        enable_fp32(h);
+        for (size_t l = 0; l < h->levels.size(); l++) {
+            const size_t m = (size_t)h->levels[l].n * k;
+            BlockLevel& L = b->lv[l];
+            L.x32 = dalloc<float>(m);
+            L.b32 = dalloc<float>(m);
+            L.t32 = dalloc<float>(m);
+            L.r32 = dalloc<float>(m);
+        }
+        const size_t m0 = (size_t)h->levels[0].n * k;
+        b->R32 = dalloc<float>(m0);
+        b->Z32 = dalloc<float>(m0);
+    }
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// block V-cycle (x = 0 on entry at every level, as inside PCG)
+// ---------------------------------------------------------------------------
+template <typename X>
+static void bvec(BlockLevel& L, X*& x, X*& b, X*& t, X*& r)
+{
+    if constexpr (sizeof(X) == 8) {
+        x = L.x;
+        b = L.b;
+        t = L.t;
+        r = L.r;
+    } else {
+        x = L.x32;
+        b = L.b32;
+        t = L.t32;
+        r = L.r32;
+    }
+}
+
+template <typename X>
+static void bcycle(ramd_hier_s* h, BlockState* bs, int l, X* x, const X* b)
+{
+    Level& L = h->levels[l];
+    BlockLevel& BL = bs->lv[l];
+    const int k = bs->k;
+    hipStream_t s = h->ctx->stream;
+    if (l == (int)h->levels.size() - 1) {
+        const int n = h->coarse_n;
+        if (n)
+            hipLaunchKernelGGL((bcoarse_solve<X>), dim3(k), dim3(256), sizeof(double) * n, s, n, k, h->lu, h->piv, b,
+                               x);
+        RAMD_HIP(hipGetLastError());
+        return;
+    }
+    const X w = (X)h->opts.relax_weight;
+    const int sweeps = h->opts.num_smooth_sweeps;
+    X *lx, *lb, *lt, *lr;
+    bvec<X>(BL, lx, lb, lt, lr);
+    (void)lx;
+    (void)lb;
+    X* cur = x;
+    X* other = lt;
+    const X* diag;
+    if constexpr (sizeof(X) == 8) diag = L.diag;
+    else diag = L.diag32;
+    for (int it = 0; it < sweeps; it++) {
+        if (it == 0) {
+            hipLaunchKernelGGL((bjacobi_from_zero<X>), grid1((int64_t)L.n * k), dim3(256), 0, s, L.n, k, diag, b, cur,
+                               w, L.A->rowptr);
+            RAMD_HIP(hipGetLastError());
+        } else {
+            blaunch<X>(L.A, BL.lpr_A, OP_JACOBI, k, cur, b, other, w, s);
+            std::swap(cur, other);
+        }
+    }
+    blaunch<X>(L.A, BL.lpr_A, OP_RESIDUAL, k, cur, b, lr, X(0), s);
+    BlockLevel& BC = bs->lv[l + 1];
+    X *cx, *cb, *ct, *cr;
+    bvec<X>(BC, cx, cb, ct, cr);
+    (void)ct;
+    (void)cr;
+    blaunch<X>(L.R, BL.lpr_R, OP_SPMV, k, lr, (const X*)nullptr, cb, X(0), s);
+    bcycle<X>(h, bs, l + 1, cx, cb);
+    blaunch<X>(L.P, BL.lpr_P, OP_ADD_INTO, k, cx, cur, other, X(0), s);
+    std::swap(cur, other);
+    for (int it = 0; it < sweeps; it++) {
+        blaunch<X>(L.A, BL.lpr_A, OP_JACOBI, k, cur, b, other, w, s);
+        std::swap(cur, other);
+    }
+    if (cur != x) RAMD_HIP(hipMemcpyAsync(x, cur, sizeof(X) * (size_t)L.n * k, hipMemcpyDeviceToDevice, s));
+}
+
+static void col_dots(ramd_ctx ctx, BlockState* bs, int n, int mode, const double* a, const void* z, bool z32,
+                     const double* w, double* out1, double* out2)
+{
+    hipStream_t s = ctx->stream;
+    const int k = bs->k;
+    const int g = std::max(1, std::min(kColBlocks, (n + 3) / 4));
+    if (mode == 0) {
+        if (z32) hipLaunchKernelGGL((bcol_dots<0, float>), dim3(g), dim3(256), 0, s, n, k, a, (const float*)z, w, bs->partials);
+        else hipLaunchKernelGGL((bcol_dots<0, double>), dim3(g), dim3(256), 0, s, n, k, a, (const double*)z, w, bs->partials);
+    } else {
+        if (z32) hipLaunchKernelGGL((bcol_dots<1, float>), dim3(g), dim3(256), 0, s, n, k, a, (const float*)z, w, bs->partials);
+        else hipLaunchKernelGGL((bcol_dots<1, double>), dim3(g), dim3(256), 0, s, n, k, a, (const double*)z, w, bs->partials);
+    }
+    RAMD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(bcol_final, dim3(1), dim3(64), 0, s, g, k, bs->partials, out1);
+    if (mode == 1) hipLaunchKernelGGL(bcol_final, dim3(1), dim3(64), 0, s, g, k, bs->partials + (size_t)g * k, out2);
+    RAMD_HIP(hipGetLastError());
+}
+
+// outer operator of the block solve: FP64 values (the caller's matrix has
+// the fine level's pattern, so the fine level's tuning applies)
+static void bouter(ramd_hier_s* h, BlockState* bs, const ramd_csr_s* A, int op, const double* x, const double* b,
+                   double* y)
+{
+    const int lpr = bs->lv.size() > 1 ? bs->lv[0].lpr_A : (bs->k <= 8 ? 8 : 32);
+    blaunch<double>(A, lpr, op, bs->k, x, b, y, 0.0, h->ctx->stream);
+}
+
+static void bprecondition(ramd_hier_s* h, BlockState* bs, bool f32)
+{
+    if (f32) bcycle<float>(h, bs, 0, bs->Z32, bs->R32);
+    else bcycle<double>(h, bs, 0, bs->Z, bs->R);
+}
+
+static hipGraphExec_t capture(hipStream_t s, const std::function<void()>& fn)
+{
+    hipGraph_t g = nullptr;
+    RAMD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        fn();
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    RAMD_HIP(hipStreamEndCapture(s, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    RAMD_HIP(e);
+    return ex;
+}
+
+static int fpcg_block_run(ramd_hier_s* h, ramd_csr_s* A, int k, double* X_d, const double* B_d, double rtol,
+                          int max_iter, int prec, int use_graph, int32_t* iters_col, double* relres_col,
+                          int32_t* iters_total)
+{
+    ramd_ctx ctx = h->ctx;
+    ctx->set_current();
+    hipStream_t s = ctx->stream;
+    const int n = A->n_rows;
+    RAMD_CHECK(n == h->levels[0].n, RAMD_E_ARG, "ramd_fpcg_block: A has %d rows, hierarchy %d", n, h->levels[0].n);
+    const bool f32 = prec == RAMD_PREC_FP32;
+    BlockState* bs = block_state(h, k, f32);
+    const int64_t m = (int64_t)n * k;
+    const BScal S = bs->S;
+    const dim3 ge(std::max<unsigned>(1, (unsigned)std::min<int64_t>((m + 255) / 256, 65536)));
+    const int gc = std::max(1, std::min(kColBlocks, (n + 3) / 4));
+    RAMD_HIP(hipMemcpyAsync(bs->X, X_d, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
+    RAMD_HIP(hipMemcpyAsync(bs->B, B_d, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(bfill, dim3(4), dim3(256), 0, s, (int64_t)16 * kMaxRhs, 0.0, bs->scal);
+    // R = B - A X; per-column ||B||, ||R||
+    bouter(h, bs, A, OP_RESIDUAL, bs->X, bs->B, bs->R);
+    col_dots(ctx, bs, n, 0, bs->B, bs->B, false, nullptr, S.bb, nullptr);
+    col_dots(ctx, bs, n, 0, bs->R, bs->R, false, nullptr, S.rr, nullptr);
+    hipLaunchKernelGGL(bscal_init, dim3(1), dim3(64), 0, s, k, rtol, S);
+    RAMD_HIP(hipGetLastError());
+    auto read_flags = [&](double f[2]) {
+        RAMD_HIP(hipMemcpyAsync(bs->pinned, S.flags, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        RAMD_HIP(hipStreamSynchronize(s));
+        f[0] = bs->pinned[0];
+        f[1] = bs->pinned[1];
+    };
+    double fl[2];
+    read_flags(fl);
+    int rc = RAMD_OK;
+    int it = 0;
+    if (fl[1] != 0) rc = RAMD_E_NUMERIC;
+    if (rc == RAMD_OK && fl[0] > 0) {
+        // first direction: Z = M^-1 R, P = Z, rz = R.Z
+        if (f32) hipLaunchKernelGGL((bconvert<double, float>), ge, dim3(256), 0, s, m, bs->R, bs->R32);
+        bprecondition(h, bs, f32);
+        if (f32) hipLaunchKernelGGL((bconvert<float, double>), ge, dim3(256), 0, s, m, bs->Z32, bs->P);
+        else RAMD_HIP(hipMemcpyAsync(bs->P, bs->Z, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
+        col_dots(ctx, bs, n, 0, bs->R, f32 ? (const void*)bs->Z32 : (const void*)bs->Z, f32, nullptr, S.rz, nullptr);
+
+        auto stepA = [&] {
+            bouter(h, bs, A, OP_SPMV, bs->P, nullptr, bs->AP);
+            col_dots(ctx, bs, n, 0, bs->P, bs->AP, false, nullptr, S.pap, nullptr);
+            hipLaunchKernelGGL(bscal_alpha, dim3(1), dim3(64), 0, s, k, S);
+            hipLaunchKernelGGL(bupdate, dim3(gc), dim3(256), 0, s, n, k, S.alpha, bs->P, bs->AP, bs->X, bs->R,
+                               f32 ? bs->R32 : nullptr, bs->partials);
+            hipLaunchKernelGGL(bcol_final, dim3(1), dim3(64), 0, s, gc, k, bs->partials, S.rr);
+            hipLaunchKernelGGL(bscal_relres, dim3(1), dim3(64), 0, s, k, rtol, S);
+            RAMD_HIP(hipGetLastError());
+        };
+        auto stepB = [&] {
+            bprecondition(h, bs, f32);
+            col_dots(ctx, bs, n, 1, bs->R, f32 ? (const void*)bs->Z32 : (const void*)bs->Z, f32, bs->AP, S.rzn,
+                     S.zap);
+            hipLaunchKernelGGL(bscal_beta, dim3(1), dim3(64), 0, s, k, S);
+            if (f32) hipLaunchKernelGGL((bdir<float>), dim3(gc), dim3(256), 0, s, n, k, S.beta, bs->Z32, bs->P);
+            else hipLaunchKernelGGL((bdir<double>), dim3(gc), dim3(256), 0, s, n, k, S.beta, bs->Z, bs->P);
+            RAMD_HIP(hipGetLastError());
+        };
+        const void* sig[4] = {A->val, A->col, A->rowptr, A->dia ? (const void*)A->dia->v64 : nullptr};
+        if (use_graph && (bs->g_A != A || std::memcmp(sig, bs->g_sig, sizeof sig) != 0 || bs->g_prec != prec ||
+                          bs->g_rtol != rtol || !bs->gA)) {
+            free_graphs(bs);
+            bs->gA = capture(s, stepA);
+            bs->gB = capture(s, stepB);
+            bs->g_A = A;
+            std::memcpy(bs->g_sig, sig, sizeof sig);
+            bs->g_prec = prec;
+            bs->g_rtol = rtol;
+        }
+        while (it < max_iter) {
+            if (use_graph) RAMD_HIP(hipGraphLaunch(bs->gA, s));
+            else stepA();
+            it++;
+            read_flags(fl);
+            if (fl[1] != 0) {
+                rc = fl[1] == 1 ? RAMD_E_INDEFINITE : RAMD_E_NUMERIC;
+                break;
+            }
+            if (fl[0] == 0) {
+                // confirm every column with its true residual
+                bouter(h, bs, A, OP_RESIDUAL, bs->X, bs->B, bs->R);
+                col_dots(ctx, bs, n, 0, bs->R, bs->R, false, nullptr, S.rr, nullptr);
+                hipLaunchKernelGGL(bscal_recheck, dim3(1), dim3(64), 0, s, k, rtol, S);
+                RAMD_HIP(hipGetLastError());
+                read_flags(fl);
+                if (fl[0] == 0) break;
+                if (f32) hipLaunchKernelGGL((bconvert<double, float>), ge, dim3(256), 0, s, m, bs->R, bs->R32);
+            }
+            if (use_graph) RAMD_HIP(hipGraphLaunch(bs->gB, s));
+            else stepB();
+        }
+    }
+    RAMD_HIP(hipMemcpyAsync(X_d, bs->X, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
+    RAMD_HIP(hipMemcpyAsync(bs->pinned + 2, S.iters, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    RAMD_HIP(hipMemcpyAsync(bs->pinned + 2 + k, S.relres, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    RAMD_HIP(hipStreamSynchronize(s));
+    bool all = true;
+    for (int c = 0; c < k; c++) {
+        if (iters_col) iters_col[c] = (int32_t)bs->pinned[2 + c];
+        if (relres_col) relres_col[c] = bs->pinned[2 + k + c];
+        if (!(bs->pinned[2 + k + c] <= rtol)) all = false;
+    }
+    *iters_total = it;
+    if (rc == RAMD_OK && !all) rc = RAMD_E_NOT_CONVERGED;
+    return rc;
+}
+
+}  // namespace ramd
+
+using namespace ramd;
+
+extern "C" {
+
+int ramd_fpcg_block(ramd_hier h, ramd_csr A, int32_t k, double* X_d, const double* B_d, double rtol, int max_iter,
+                    int cycle_prec, int use_graph, int32_t* iters_col, double* relres_col, int32_t* iters_total)
+{
+    int rc = RAMD_OK;
+    int st = guarded([&] {
+        RAMD_CHECK(h && A && X_d && B_d && iters_total, RAMD_E_ARG, "ramd_fpcg_block: NULL argument");
+        RAMD_CHECK(k >= 1 && k <= kMaxRhs, RAMD_E_ARG, "ramd_fpcg_block: k = %d (1..%d)", k, kMaxRhs);
+        RAMD_CHECK(cycle_prec == RAMD_PREC_FP64 || cycle_prec == RAMD_PREC_FP32, RAMD_E_ARG,
+                   "ramd_fpcg_block: unknown precision %d", cycle_prec);
+        rc = fpcg_block_run(h, A, k, X_d, B_d, rtol, max_iter, cycle_prec, use_graph, iters_col, relres_col,
+                            iters_total);
+    });
+    if (st != RAMD_OK) return st;
+    if (rc == RAMD_E_INDEFINITE) set_error("ramd_fpcg_block: p.Ap <= 0 in a column (not SPD)");
+    if (rc == RAMD_E_NUMERIC) set_error("ramd_fpcg_block: non-finite value in the iteration");
+    if (rc == RAMD_E_NOT_CONVERGED) set_error("ramd_fpcg_block: %d iterations did not reach rtol", max_iter);
+    return rc;
+}
+
+int ramd_vcycle_block(ramd_hier h, int32_t k, int prec, void* X_d, const void* B_d)
+{
+    return guarded([&] {
+        RAMD_CHECK(h && X_d && B_d, RAMD_E_ARG, "ramd_vcycle_block: NULL argument");
+        RAMD_CHECK(k >= 1 && k <= kMaxRhs, RAMD_E_ARG, "ramd_vcycle_block: k = %d (1..%d)", k, kMaxRhs);
+        RAMD_CHECK(prec == RAMD_PREC_FP64 || prec == RAMD_PREC_FP32, RAMD_E_ARG, "ramd_vcycle_block: precision %d",
+                   prec);
+        h->ctx->set_current();
+        BlockState* bs = block_state(h, k, prec == RAMD_PREC_FP32);
+        if (prec == RAMD_PREC_FP32) bcycle<float>(h, bs, 0, (float*)X_d, (const float*)B_d);
+        else bcycle<double>(h, bs, 0, (double*)X_d, (const double*)B_d);
+        RAMD_HIP(hipStreamSynchronize(h->ctx->stream));
+    });
+}
+
+int ramd_block_tuning(ramd_hier h, int32_t k, int32_t* lpr)
+{
+    return guarded([&] {
+        RAMD_CHECK(h && lpr, RAMD_E_ARG, "ramd_block_tuning: NULL argument");
+        RAMD_CHECK(k >= 1 && k <= kMaxRhs, RAMD_E_ARG, "ramd_block_tuning: k = %d", k);
+        h->ctx->set_current();
+        BlockState* bs = block_state(h, k, false);
+        RAMD_HIP(hipStreamSynchronize(h->ctx->stream));
+        for (size_t l = 0; l < h->levels.size(); l++) {
+            lpr[3 * l] = bs->lv[l].lpr_A;
+            lpr[3 * l + 1] = bs->lv[l].lpr_P;
+            lpr[3 * l + 2] = bs->lv[l].lpr_R;
+        }
+    });
+}
+
+}  // extern "C"

@@ -10,40 +10,7 @@
 #include "common.hpp"
 #include "host/setup.hpp"
 
-namespace ramd {
-
-ramd_csr_s* spgemm(ramd_csr_s* A, ramd_csr_s* B);
-ramd_csr_s* galerkin(ramd_csr_s* A, ramd_csr_s* P);
-
-struct Level {
-    ramd_csr_s* A = nullptr;  // diagonal first
-    ramd_csr_s* P = nullptr;  // n_l x n_{l+1}
-    ramd_csr_s* R = nullptr;  // P^T (owned by P)
-    double* diag = nullptr;   // a_ii (0 where absent)
-    double *x = nullptr, *b = nullptr, *t = nullptr, *r = nullptr;
-    float* diag32 = nullptr;
-    float *x32 = nullptr, *b32 = nullptr, *t32 = nullptr, *r32 = nullptr;
-    std::vector<int> states;
-    int n = 0;
-};
-
-}  // namespace ramd
-
-struct ramd_hier_s {
-    ramd_ctx ctx = nullptr;
-    ramd_amg_opts opts{};
-    std::vector<ramd::Level> levels;
-    int coarse_n = 0;
-    double* lu = nullptr;  // column-major LU of the coarse matrix read column-major (A_c^T)
-    int* piv = nullptr;
-    double* cb = nullptr;  // FP64 staging for the FP32 cycle's coarse solve
-    double* cx = nullptr;
-    double times[5] = {0, 0, 0, 0, 0};
-    bool fp32 = false;
-    // PCG work vectors (fine level)
-    double *pr = nullptr, *pz = nullptr, *pp = nullptr, *pAp = nullptr;
-    float *pr32 = nullptr, *pz32 = nullptr;
-};
+#include "hier.hpp"
 
 namespace ramd {
 
@@ -207,11 +174,6 @@ __global__ void to_f64(int64_t n, const float* __restrict__ a, double* __restric
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += s) b[i] = (double)a[i];
 }
 
-static dim3 grid1(int64_t n, int bs = 256)
-{
-    return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + bs - 1) / bs, 1 << 20)));
-}
-
 // ---------------------------------------------------------------------------
 // V-cycle
 // ---------------------------------------------------------------------------
@@ -331,6 +293,7 @@ static void hier_free(ramd_hier_s* h)
 {
     if (!h) return;
     if (h->ctx) (void)hipStreamSynchronize(h->ctx->stream);
+    block_free(h->blk);
     for (Level& L : h->levels) free_level(L);
     for (void* p : {(void*)h->lu, (void*)h->piv, (void*)h->cb, (void*)h->cx, (void*)h->pr, (void*)h->pz, (void*)h->pp,
                     (void*)h->pAp, (void*)h->pr32, (void*)h->pz32})
@@ -500,7 +463,7 @@ static void hier_setup(ramd_hier_s* h, Csr Af)
     h->times[4] = total - h->times[0] - h->times[1] - h->times[2] - h->times[3];
 }
 
-static void enable_fp32(ramd_hier_s* h)
+void enable_fp32(ramd_hier_s* h)
 {
     if (h->fp32) return;
     hipStream_t s = h->ctx->stream;

@@ -38,8 +38,16 @@ def test_host_entry_points_reject_bad_input():
 
 
 def test_torch_extension_loads():
-    """The libtorch autograd wrapper links against this library and exports amg_solve."""
+    """The libtorch autograd wrapper links against this library, binds to
+    torch's HIP runtime when torch is imported first, and exports amg_solve
+    (own interpreter: this one has already loaded the library)."""
     pytest.importorskip("torch")
-    from raptor_amd import autograd
-    mod = autograd._load()
-    assert hasattr(mod, "amg_solve")
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = ("import sys, torch; sys.path.insert(0, sys.argv[1]); from raptor_amd import autograd; "
+              "m = autograd._load(); assert hasattr(m, 'amg_solve'); assert len(autograd._hip_runtimes()) == 1; "
+              "print('EXT_OK')")
+    r = subprocess.run([sys.executable, "-c", script, root], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "EXT_OK" in r.stdout, r.stderr[-3000:]

@@ -110,22 +110,61 @@ def test_adjoint_fp32_cycle_same_gradient(problem, ctx):
     assert np.allclose(dA32, dA64, rtol=1e-9, atol=1e-11 * np.abs(dA64).max())
 
 
-def test_torch_autograd_function(problem, ctx):
-    torch = pytest.importorskip("torch")
-    from raptor_amd import autograd
-    dev = torch.device("cuda", 0)
-    vals = torch.tensor(problem["val"], dtype=torch.float64, device=dev, requires_grad=True)
-    b = torch.tensor(problem["b"], dtype=torch.float64, device=dev, requires_grad=True)
-    xs = torch.tensor(problem["xs"], dtype=torch.float64, device=dev)
-    # a separate operator handle: the autograd path rewrites its values
-    Aop = ra.ParCSRMatrix.from_csr(ctx, problem["rp"], problem["col"], problem["val"])
-    x = autograd.amg_solve(vals, b, problem["ml"], Aop, rtol=1e-13)
-    loss = 0.5 * ((x - xs) ** 2).sum()
-    loss.backward()
-    xh, dA, db = device_grad(problem, ctx)
-    assert np.array_equal(x.detach().cpu().numpy(), xh)
-    assert np.array_equal(vals.grad.cpu().numpy(), dA)
-    assert np.array_equal(b.grad.cpu().numpy(), db)
+_TORCH_SCRIPT = r"""
+import sys
+import numpy as np
+import torch                      # first: the library then binds to torch's HIP runtime
+sys.path.insert(0, sys.argv[1])
+import raptor_amd as ra
+from raptor_amd import autograd
+ctx = ra.Context(0)
+rp, col, val = ra.gallery_fv7(10, seed=3)
+n = rp.size - 1
+ml = ra.ParSmoothedAggregationSolver(ctx, 0.08)
+ml.relax_weight = 2.0 / 3.0
+ml.max_coarse = 20
+ml.setup(rp, col, val)
+rng = np.random.default_rng(4)
+b_h, xs_h = rng.standard_normal(n), rng.standard_normal(n)
+dev = torch.device("cuda", 0)
+vals = torch.tensor(val, dtype=torch.float64, device=dev, requires_grad=True)
+b = torch.tensor(b_h, dtype=torch.float64, device=dev, requires_grad=True)
+xs = torch.tensor(xs_h, dtype=torch.float64, device=dev)
+Aop = ra.ParCSRMatrix.from_csr(ctx, rp, col, val)   # the autograd path rewrites its values
+x = autograd.amg_solve(vals, b, ml, Aop, rtol=1e-13)
+loss = 0.5 * ((x - xs) ** 2).sum()
+loss.backward()
+# the same gradient through the C ABI directly
+A = ra.ParCSRMatrix.from_csr(ctx, rp, col, val)
+xd = ra.ParVector.from_numpy(ctx, np.zeros(n))
+ra.FPCG(A, ml, xd, ra.ParVector.from_numpy(ctx, b_h), 1e-13, 500)
+xh = xd.numpy()
+dA, db, _ = ra.adjoint(A, ml, xd, ra.ParVector.from_numpy(ctx, xh - xs_h), rtol=1e-13)
+assert np.array_equal(x.detach().cpu().numpy(), xh), "x"
+assert np.array_equal(vals.grad.cpu().numpy(), dA.numpy()), "dA"
+assert np.array_equal(b.grad.cpu().numpy(), db.numpy()), "db"
+# a second forward with moved values converges and differs
+with torch.no_grad():
+    v2 = vals * 1.01
+x2 = autograd.amg_solve(v2, b.detach(), ml, Aop, rtol=1e-13)
+assert torch.allclose(x2 * 1.01, x.detach(), rtol=1e-6, atol=0), "scaled A"
+ctx.close()
+print("TORCH_AUTOGRAD_OK")
+"""
+
+
+def test_torch_autograd_function():
+    """The libtorch autograd Function gives exactly the C ABI's solution and
+    adjoint gradient.  Runs in its own interpreter, which imports torch
+    before the library so that both use torch's HIP runtime (two HIP
+    runtimes cannot share one process)."""
+    pytest.importorskip("torch")
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", _TORCH_SCRIPT, root], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0 and "TORCH_AUTOGRAD_OK" in r.stdout, r.stdout[-2000:] + r.stderr[-4000:]
 
 
 def test_set_values_refreshes_operator(ctx):
