@@ -248,6 +248,12 @@ int ramd_amg_solve(ramd_hier h, double* x_d, const double* b_d, double tol,
 int ramd_pcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double tol,
              int max_iter, double* res, int32_t max_res, int32_t* n_res, int32_t* iters);
 
+/* CG(A, x, b, res, tol, max_iter), krylov/par_cg.cpp:8-118: unpreconditioned
+ * CG with the reference's rules (tol *= ||r0||, res[k] = ||r_k|| / ||b||,
+ * true residual every 8th iteration).  res must hold max_res entries. */
+int ramd_cg(ramd_csr A, double* x_d, const double* b_d, double tol, int max_iter, double* res,
+            int32_t max_res, int32_t* n_res, int32_t* iters);
+
 /* Mixed-precision solve: flexible PCG in FP64 (true residual b-Ax, inner
  * products and the coarse LU in FP64) preconditioned by one V-cycle in the
  * precision `cycle_prec`, iterated until ||b-Ax||_2/||b||_2 <= rtol (true,

@@ -400,7 +400,11 @@ long ref_fit_candidates(int n, const int* rp, const int* col, const double* val,
 long ref_jacobi_prolongation(int n, const int* rp, const int* col, const double* val, int n_aggs, const int* trp,
                              const int* tcol, const double* tval, double omega, int steps)
 {
+    // finalize() leaves the rows sorted; the solver hands jacobi_prolongation
+    // its level matrix with the diagonal first (strength() and aggregate()
+    // sort it and move the diagonal, par_strength.cpp:369-370)
     ParCSRMatrix* A = make_par(n, n, rp, col, val);
+    A->on_proc->move_diag();
     ParCSRMatrix* T = make_par(n, n_aggs, trp, tcol, tval);
     ParCSRMatrix* P = jacobi_prolongation(A, T, false, omega, steps);
     long nnz = stash(P->on_proc);

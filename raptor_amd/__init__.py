@@ -34,7 +34,7 @@ from ._lib import (CLJP, HMIS, PMIS, RS, Classical, Direct, Extended, Falgout, J
 
 __all__ = [
     "Context", "ParVector", "ParCSRMatrix", "ParMultilevel", "ParRugeStubenSolver",
-    "ParSmoothedAggregationSolver", "PCG", "FPCG", "FPCG_block", "vcycle_block", "block_tuning", "adjoint", "strength", "split", "interpolation", "mis2",
+    "ParSmoothedAggregationSolver", "PCG", "CG", "FPCG", "FPCG_block", "vcycle_block", "block_tuning", "adjoint", "strength", "split", "interpolation", "mis2",
     "aggregate", "fit_candidates", "jacobi_prolongation", "gallery_laplace27", "gallery_aniso2d", "gallery_fv7", "RamdError",
     "RS", "CLJP", "Falgout", "PMIS", "HMIS", "Direct", "ModClassical", "Extended", "Classical",
     "Symmetric", "Jacobi", "PREC_FP64", "PREC_FP32",
@@ -380,6 +380,16 @@ def PCG(A, ml, x, b, tol=1e-5, max_iter=-1, max_res=10000):
     nres, it = C.c_int32(), C.c_int32()
     check(lib().ramd_pcg(ml.h, A.h, x.ptr, b.ptr, float(tol), int(max_iter), _ptr(res, C.c_double), max_res,
                          C.byref(nres), C.byref(it)))
+    return res[:min(nres.value, max_res)], it.value
+
+
+def CG(A, x, b, tol=1e-5, max_iter=-1, max_res=100000):
+    """CG(A, x, b, res, tol, max_iter), krylov/par_cg.cpp:8-118.  Returns
+    (residual history, iteration count)."""
+    res = np.zeros(max_res)
+    nres, it = C.c_int32(), C.c_int32()
+    check(lib().ramd_cg(A.h, x.ptr, b.ptr, float(tol), int(max_iter), _ptr(res, C.c_double), max_res,
+                        C.byref(nres), C.byref(it)))
     return res[:min(nres.value, max_res)], it.value
 
 

@@ -998,6 +998,79 @@ int ramd_pcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double tol
     return rc;
 }
 
+int ramd_cg(ramd_csr A, double* x_d, const double* b_d, double tol, int max_iter, double* res, int32_t max_res,
+            int32_t* n_res, int32_t* iters)
+{
+    int rc = RAMD_OK;
+    int st = guarded([&] {
+        RAMD_CHECK(A && x_d && b_d && n_res && iters && (res || max_res == 0), RAMD_E_ARG, "ramd_cg: NULL argument");
+        RAMD_CHECK(A->n_rows == A->n_cols, RAMD_E_ARG, "ramd_cg: A is not square");
+        ramd_ctx ctx = A->ctx;
+        ctx->set_current();
+        hipStream_t s = ctx->stream;
+        const int n = A->n_rows;
+        struct Bufs {
+            double *r = nullptr, *p = nullptr, *Ap = nullptr;
+            ~Bufs()
+            {
+                dfree(r);
+                dfree(p);
+                dfree(Ap);
+            }
+        } w;
+        w.r = dalloc<double>((size_t)std::max(n, 1));
+        w.p = dalloc<double>((size_t)std::max(n, 1));
+        w.Ap = dalloc<double>((size_t)std::max(n, 1));
+        int nres = 0;
+        auto push = [&](double v) {
+            if (nres < max_res) res[nres] = v;
+            nres++;
+        };
+        // b.norm(2): entries |v| <= 1e-16 skipped, root / power / root (core/vector.cpp:107-118)
+        launch_norm2sq(ctx, n, b_d, ctx->red_result);
+        double b_norm = fetch_scalar(ctx, ctx->red_result);
+        b_norm = pow(pow(pow(b_norm, 1.0 / 2), 2), 1.0 / 2);
+        if (b_norm < 1e-16) b_norm = 1.0;
+        if (max_iter <= 0) max_iter = ((int)(1.3 * n)) + 2;
+        launch_stream_op(A, OP_RESIDUAL, x_d, b_d, w.r, 0.0, s);
+        RAMD_HIP(hipMemcpyAsync(w.p, w.r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        double rr = dot64(ctx, n, w.r, w.r);
+        double norm_r = sqrt(rr);
+        push(norm_r / b_norm);
+        if (norm_r != 0.0) tol = tol * norm_r;
+        const int recompute_r = 8;
+        int iter = 0;
+        while (norm_r > tol && iter < max_iter) {
+            launch_stream_op(A, OP_SPMV, w.p, nullptr, w.Ap, 0.0, s);
+            const double App = dot64(ctx, n, w.Ap, w.p);
+            if (App < 0.0) {
+                rc = RAMD_E_INDEFINITE;
+                break;
+            }
+            const double alpha = rr / App;
+            const bool update_r = (iter % recompute_r) && iter > 0;
+            hipLaunchKernelGGL(pcg_update_xr, grid1(n), dim3(256), 0, s, n, alpha, w.p, w.Ap, x_d, w.r,
+                               update_r ? 1 : 0);
+            RAMD_HIP(hipGetLastError());
+            if (!update_r) launch_stream_op(A, OP_RESIDUAL, x_d, b_d, w.r, 0.0, s);
+            const double next = dot64(ctx, n, w.r, w.r);
+            const double beta = next / rr;
+            hipLaunchKernelGGL(pcg_update_p, grid1(n), dim3(256), 0, s, n, beta, w.r, w.p);
+            RAMD_HIP(hipGetLastError());
+            rr = next;
+            norm_r = sqrt(rr);
+            push(norm_r / b_norm);
+            iter++;
+        }
+        RAMD_HIP(hipStreamSynchronize(s));
+        *n_res = nres;
+        *iters = iter;
+    });
+    if (st != RAMD_OK) return st;
+    if (rc == RAMD_E_INDEFINITE) set_error("ramd_cg: indefinite matrix detected (A p . p < 0)");
+    return rc;
+}
+
 // Flexible PCG (Polak-Ribiere beta = z_{k+1}.(r_{k+1} - r_k) / z_k.r_k) in FP64
 // with a V-cycle preconditioner in `cycle_prec`; stops on the true FP64
 // relative residual ||b - A x|| / ||b|| <= rtol.

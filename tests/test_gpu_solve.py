@@ -246,3 +246,23 @@ def test_sa_prolongation_bit_exact_vs_ref(ctx, steps):
                                         K.dp(tval), C.c_double(4.0 / 3), steps)
         E = K.ref_fetch(nnz)
         assert np.array_equal(prp, E[0]) and np.array_equal(pcol, E[1]) and np.array_equal(pval, E[2])
+
+
+def test_cg_golden_history(golden, ctx):
+    """krylov/tests/test_par_cg.cpp: CG on the 50x50 anisotropic operator,
+    b = A*1, x0 = 0: the residual history of the reference (cg_res.txt,
+    161 entries, scaled by ||b|| there) and the oracle's, iteration for
+    iteration."""
+    rp, col, val = ra.gallery_aniso2d(50, 0.001, math.pi / 8)
+    A = ra.ParCSRMatrix.from_csr(ctx, rp, col, val)
+    Ah = K.Csr(rp, col, val)
+    n = Ah.n_rows
+    b = K.spmv(Ah, np.ones(n))
+    x = ra.ParVector.from_numpy(ctx, np.zeros(n))
+    res, it = ra.CG(A, x, ra.ParVector.from_numpy(ctx, b))
+    ref = golden["cg_res"]
+    assert len(res) == len(ref)
+    assert np.allclose(res * K.norm2(b), ref, rtol=0, atol=1e-6)
+    xo, reso, ito = K.cg(Ah, np.zeros(n), b)
+    # CG amplifies the rounding of the (differently ordered) inner products
+    assert it == ito and np.allclose(res * K.norm2(b), reso * K.norm2(b), rtol=0, atol=1e-6)
