@@ -112,6 +112,8 @@ def main():
     log(f"setup {setup_s:.1f}s (strength {tphase[0]:.1f} split {tphase[1]:.1f} interp {tphase[2]:.1f} "
         f"galerkin {tphase[3]:.1f} other {tphase[4]:.1f}); levels {list(map(int, nl))}")
     ml.enable_fp32()
+    caps = ml.row_caps()
+    log(f"row-block caps [A, P, R] x [fp64, fp32] by level: {caps.tolist()}")
     A = ra.ParCSRMatrix.from_csr(ctx, rp, col, val)
 
     if cfg["rhs"] == "normal":
@@ -204,9 +206,12 @@ def main():
     per_launch_bytes = dom["bytes"] / dom["launches"]
     per_launch_s = dom["ms"] / 1e3 / dom["launches"]
     achieved = per_launch_bytes / per_launch_s / 1e9
+    kname = kernel_template(dom_name, dom["kernel"], ml.level(0)[0].diagonals())
+    traffic = measured_traffic(m, kname)
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
-                    frac=round(achieved / PEAK_HBM_GBS, 4), traffic=None, kernel=f"{dom['kernel']}:{dom_name}",
-                    bytes_per_launch=per_launch_bytes, us_per_launch=round(per_launch_s * 1e6, 2))
+                    frac=round(achieved / PEAK_HBM_GBS, 4), traffic=traffic, kernel=f"{dom['kernel']}:{dom_name}",
+                    kernel_symbol=kname, bytes_per_launch=per_launch_bytes,
+                    us_per_launch=round(per_launch_s * 1e6, 2))
     log(f"dominant kernel {dom_name}: {achieved:.0f} GB/s ({achieved / PEAK_HBM_GBS:.1%} of 8 TB/s)")
 
     # ---- CPU baseline: the oracle (plain C, one core) on the same hierarchy
@@ -241,6 +246,7 @@ def main():
         "setup_s": round(setup_s, 2),
         "vcycle": vc,
         "kernels": prof,
+        "row_caps": caps.tolist(),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -248,6 +254,28 @@ def main():
         calibrate_copy(ctx)
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def kernel_template(slot, kind, nd):
+    """Symbol of the kernel behind a timing slot of the FP32-form solve: the
+    FP64-vector slots there read the lossless FP32 values."""
+    op = {"spmv": 0, "append": 1, "residual": 2, "jacobi": 3, "add_into": 4}[slot.rsplit("_", 1)[0]]
+    vec = "float" if slot.endswith("fp32") else "double"
+    if kind == "dia_kernel":
+        return f"ramd::dia_kernel<{op}, float, {vec}, {nd}>"
+    return f"ramd::stream_kernel<{op}, {vec}, {vec}>"
+
+
+def measured_traffic(m, kname):
+    """HBM bytes per launch of kname from the committed counter summary of
+    the same workload (profiles/pmc_traffic_27pt-<m>cubed_fpcg.md), or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_27pt-{m}cubed_fpcg.md")
+    try:
+        txt = open(path).read()
+        block = txt.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block).get(kname)
+    except (OSError, IndexError, ValueError):
+        return None
 
 
 def calibrate_copy(ctx):

@@ -213,6 +213,11 @@ int ramd_hier_level(ramd_hier h, int level, ramd_csr* A, ramd_csr* P);
 /* C/F states (1 = C, 0 = F, -2 = no neighbours) of level l, or for SA the
  * aggregate of each row (its root row, -1 for none). */
 int ramd_hier_states(ramd_hier h, int level, int32_t* states);
+/* Row-block caps (products per workgroup) the autotuner chose for the
+ * CSR kernels: six entries per level, [A, P, R] x [FP64, FP32 vectors];
+ * 0 for a stencil-diagonal operator or an absent one.  No reference
+ * counterpart. */
+int ramd_hier_row_caps(ramd_hier h, int32_t* caps);
 /* Host-seconds spent per setup phase: [strength, coarsen, interp, galerkin, other]. */
 int ramd_hier_setup_times(ramd_hier h, double* times5);
 /* Builds the FP32 copies of every level used by RAMD_PREC_FP32 cycles. */

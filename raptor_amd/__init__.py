@@ -301,6 +301,13 @@ class ParMultilevel:
         check(lib().ramd_hier_states(self.h, l, _ptr(s, C.c_int32)))
         return s
 
+    def row_caps(self):
+        """Row-block caps chosen by the autotuner: array (levels, 3, 2) for
+        A, P, R with FP64 and FP32 vectors (0: stencil-diagonal or absent)."""
+        c = np.zeros(6 * self.num_levels, np.int32)
+        check(lib().ramd_hier_row_caps(self.h, _ptr(c, C.c_int32)))
+        return c.reshape(-1, 3, 2)
+
     def setup_times(self):
         t = np.zeros(5)
         check(lib().ramd_hier_setup_times(self.h, _ptr(t, C.c_double)))

@@ -101,6 +101,7 @@ _SIGS = {
     "ramd_hier_level": (P, C.c_int, C.POINTER(P), C.POINTER(P)),
     "ramd_hier_states": (P, C.c_int, PI32),
     "ramd_hier_setup_times": (P, PD),
+    "ramd_hier_row_caps": (P, PI32),
     "ramd_hier_enable_fp32": (P,),
     "ramd_vcycle": (P, P, P, C.c_int),
     "ramd_vcycle_fp32": (P, P, P),

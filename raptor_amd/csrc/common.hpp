@@ -130,9 +130,17 @@ struct ramd_csr_s {
     int* rowptr = nullptr;   // device, n_rows+1
     int* col = nullptr;      // device, nnz
     double* val = nullptr;   // device, nnz
-    // row blocks of the LDS-staged kernels (device, n_blocks+1)
+    // row blocks of the LDS-staged kernels (device, n_blocks+1) for the
+    // default cap of kStreamCap products per workgroup
     int* blocks = nullptr;
     int n_blocks = 0;
+    // blocks used per vector precision ([0] FP64, [1] FP32 vectors): the
+    // default ones, or those of the cap the autotuner measured fastest
+    struct RowBlocks {
+        int cap = 0;
+        int n = 0;
+        int* d = nullptr;  // == blocks for the default cap
+    } rb[2];
     // lazily built transpose (gather form of A^T x), same drop-free pattern
     ramd_csr_s* trans = nullptr;
     // FP32 copy of the values (mixed-precision cycle)
@@ -161,6 +169,13 @@ void dia_free(Dia* d);
 // Builds (once) A->trans, the CSR of A^T with rows in ascending source-row order.
 ramd_csr_s* csr_transpose(ramd_csr_s* A);
 void csr_make_fp32(ramd_csr_s* A);
+// Row-block caps the LDS-staged kernels can use (products per workgroup).
+constexpr int kStreamCaps[3] = {2048, 4096, 8192};
+// Times the CSR kernels of A with every cap for vectors of precision prec
+// (0 FP64, 1 FP32; the FP32 values must exist) and keeps the fastest row
+// blocks.  The cap changes only which workgroup owns a row, never the order
+// of a row's sum, so results are identical for every choice.
+void csr_autotune(ramd_csr_s* A, int prec);
 // New values for the same pattern (device array); refreshes every derived
 // copy (FP32 values, stencil-diagonal planes, the cached transpose).
 void csr_set_values(ramd_csr_s* A, const double* val_d);
@@ -169,10 +184,12 @@ void csr_set_values(ramd_csr_s* A, const double* val_d);
 // OP_ADD_INTO: y = b + A x with b a separate input (x <- x + P e written to a
 // second buffer); same arithmetic as OP_APPEND.
 enum StreamOp { OP_SPMV = 0, OP_APPEND = 1, OP_RESIDUAL = 2, OP_JACOBI = 3, OP_ADD_INTO = 4 };
+// dvec (Jacobi only, may be NULL): the matrix's diagonal (0 where absent);
+// on a diagonal-first matrix it replaces re-reading each row's first entry.
 void launch_stream_op(const ramd_csr_s* A, int op, const double* x, const double* b, double* y,
-                      double omega, hipStream_t s);
+                      double omega, hipStream_t s, const double* dvec = nullptr);
 void launch_stream_op32(const ramd_csr_s* A, int op, const float* x, const float* b, float* y,
-                        float omega, hipStream_t s);
+                        float omega, hipStream_t s, const float* dvec = nullptr);
 // FP64 arithmetic reading the FP32 copy of the values where every value is
 // exactly representable in FP32 (same results as launch_stream_op, half the
 // value bytes); falls back to launch_stream_op otherwise.

@@ -96,6 +96,11 @@ ramd_csr_s* csr_wrap(ramd_ctx ctx, int n_rows, int n_cols, int64_t nnz, int* row
     RAMD_HIP(hipMemcpyAsync(A->blocks, blocks.data(), sizeof(int) * blocks.size(),
                             hipMemcpyHostToDevice, ctx->stream));
     RAMD_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto& rb : A->rb) {
+        rb.cap = kStreamCap;
+        rb.n = A->n_blocks;
+        rb.d = A->blocks;
+    }
     return A;
 }
 
@@ -211,6 +216,8 @@ void csr_free(ramd_csr_s* A)
     dfree(A->rowptr);
     dfree(A->col);
     dfree(A->val);
+    for (auto& rb : A->rb)
+        if (rb.d != A->blocks) dfree(rb.d);
     dfree(A->blocks);
     dfree(A->val32);
     dia_free(A->dia);
@@ -334,9 +341,9 @@ void csr_set_values(ramd_csr_s* A, const double* val_d)
         double* part = ctx_partials(ctx, (size_t)g);
         hipLaunchKernelGGL(inexact32_kernel, dim3(g), dim3(kBlockThreads), 0, s, d->v64, m, part);
         RAMD_HIP(hipGetLastError());
-        launch_reduce_final(ctx, part, g, ctx->red_result + 4);
+        launch_reduce_final(ctx, part, g, ctx->red_result + 8);
         double inexact = 0;
-        fetch_scalars(ctx, ctx->red_result + 4, 1, &inexact);
+        fetch_scalars(ctx, ctx->red_result + 8, 1, &inexact);
         d->v32_exact = inexact == 0.0;
         if (d->v32) {
             int grid = (int)std::min<int64_t>((m + 255) / 256, 8192);

@@ -204,7 +204,7 @@ static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_ze
                                L.A->rowptr);
             RAMD_HIP(hipGetLastError());
         } else {
-            launch_stream_op(L.A, OP_JACOBI, cur, b, other, w, s);
+            launch_stream_op(L.A, OP_JACOBI, cur, b, other, w, s, L.diag);
             std::swap(cur, other);
         }
     }
@@ -216,7 +216,7 @@ static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_ze
     launch_stream_op(L.P, OP_ADD_INTO, C.x, cur, other, 0.0, s);
     std::swap(cur, other);
     for (int it = 0; it < sweeps; it++) {
-        launch_stream_op(L.A, OP_JACOBI, cur, b, other, w, s);
+        launch_stream_op(L.A, OP_JACOBI, cur, b, other, w, s, L.diag);
         std::swap(cur, other);
     }
     if (cur != x) RAMD_HIP(hipMemcpyAsync(x, cur, sizeof(double) * L.n, hipMemcpyDeviceToDevice, s));
@@ -244,7 +244,7 @@ static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero
                                L.A->rowptr);
             RAMD_HIP(hipGetLastError());
         } else {
-            launch_stream_op32(L.A, OP_JACOBI, cur, b, other, w, s);
+            launch_stream_op32(L.A, OP_JACOBI, cur, b, other, w, s, L.diag32);
             std::swap(cur, other);
         }
     }
@@ -255,7 +255,7 @@ static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero
     launch_stream_op32(L.P, OP_ADD_INTO, C.x32, cur, other, 0.f, s);
     std::swap(cur, other);
     for (int it = 0; it < sweeps; it++) {
-        launch_stream_op32(L.A, OP_JACOBI, cur, b, other, w, s);
+        launch_stream_op32(L.A, OP_JACOBI, cur, b, other, w, s, L.diag32);
         std::swap(cur, other);
     }
     if (cur != x) RAMD_HIP(hipMemcpyAsync(x, cur, sizeof(float) * L.n, hipMemcpyDeviceToDevice, s));
@@ -455,6 +455,14 @@ static void hier_setup(ramd_hier_s* h, Csr Af)
     h->pz = dalloc<double>((size_t)n0);
     h->pp = dalloc<double>((size_t)n0);
     h->pAp = dalloc<double>((size_t)n0);
+    // row-block caps of every CSR operator of the cycle, FP64 vectors
+    for (Level& L : h->levels) {
+        if (&L != &h->levels.back()) csr_autotune(L.A, 0);
+        if (L.P) {
+            csr_autotune(L.P, 0);
+            csr_autotune(L.R, 0);
+        }
+    }
     RAMD_HIP(hipStreamSynchronize(ctx->stream));
     for (Level& L : h->levels) L.A->borrowed = true;
     for (Level& L : h->levels)
@@ -472,6 +480,11 @@ void enable_fp32(ramd_hier_s* h)
         if (L.P) {
             csr_make_fp32(L.P);
             csr_make_fp32(L.R);
+        }
+        if (&L != &h->levels.back()) csr_autotune(L.A, 1);
+        if (L.P) {
+            csr_autotune(L.P, 1);
+            csr_autotune(L.R, 1);
         }
         L.x32 = dalloc<float>((size_t)L.n);
         L.b32 = dalloc<float>((size_t)L.n);
@@ -520,12 +533,19 @@ static void vcycle_zero(ramd_hier_s* h, double* z, const double* r) { cycle64(h,
 // kRedBlocks workgroups, each summing a fixed strided subset, so every run
 // gives the same bits.
 
-// x += alpha p; r -= alpha Ap; r32 = (float) r when given; partial sums of r^2
-__global__ __launch_bounds__(kBlockThreads) void fpcg_update_norm(int n, double alpha, const double* __restrict__ p,
+// x += alpha p; r -= alpha Ap; r32 = (float) r when given; partial sums of
+// r^2.  alpha = rz / pAp is formed from the device scalars (every thread
+// computes the same value; workgroup 0 stores it for the direction update).
+__global__ __launch_bounds__(kBlockThreads) void fpcg_update_norm(int n, const double* __restrict__ sc_rz,
+                                                                  const double* __restrict__ sc_pap,
+                                                                  double* __restrict__ sc_alpha,
+                                                                  const double* __restrict__ p,
                                                                   const double* __restrict__ Ap, double* __restrict__ x,
                                                                   double* __restrict__ r, float* __restrict__ r32,
                                                                   double* __restrict__ partials)
 {
+    const double alpha = *sc_rz / *sc_pap;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *sc_alpha = alpha;
     double s = 0;
     const int stride = gridDim.x * kBlockThreads;
     for (int i = blockIdx.x * kBlockThreads + threadIdx.x; i < n; i += stride) {
@@ -560,27 +580,30 @@ __global__ __launch_bounds__(kBlockThreads) void fpcg_dots(int n, const double* 
     }
 }
 
-// p = z + beta p
+// p = z + beta p with the flexible (Polak-Ribiere) beta = -alpha (z.Ap) / rz_old
+// formed from the device scalars
 template <typename Z>
-__global__ void fpcg_dir(int n, double beta, const Z* __restrict__ z, double* __restrict__ p)
+__global__ void fpcg_dir(int n, const double* __restrict__ sc_alpha, const double* __restrict__ sc_zap,
+                         const double* __restrict__ sc_rz_old, const Z* __restrict__ z, double* __restrict__ p)
 {
+    const double beta = -(*sc_alpha) * (*sc_zap) / (*sc_rz_old);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = (double)z[i] + beta * p[i];
 }
 
 static int fixed_blocks(int n) { return std::max(1, std::min(kRedBlocks, (n + kBlockThreads - 1) / kBlockThreads)); }
 
-// r.z and z.Ap of the flexible-PCG step, in one pass
+// r.z and z.Ap of the flexible-PCG step, in one pass, into device scalars
 template <typename Z>
-static void fpcg_rz_zap(ramd_ctx ctx, int n, const double* r, const Z* z, const double* Ap, double out[2])
+static void fpcg_rz_zap(ramd_ctx ctx, int n, const double* r, const Z* z, const double* Ap, double* rz_d,
+                        double* zap_d)
 {
     const int g = fixed_blocks(n);
     double* part = ctx_partials(ctx, 2 * (size_t)g);
     hipLaunchKernelGGL((fpcg_dots<Z>), dim3(g), dim3(kBlockThreads), 0, ctx->stream, n, r, z, Ap, part);
     RAMD_HIP(hipGetLastError());
-    launch_reduce_final(ctx, part, g, ctx->red_result + 2);
-    launch_reduce_final(ctx, part + g, g, ctx->red_result + 3);
-    fetch_scalars(ctx, ctx->red_result + 2, 2, out);
+    launch_reduce_final(ctx, part, g, rz_d);
+    launch_reduce_final(ctx, part + g, g, zap_d);
 }
 
 // plain 2-norm (no small-entry skipping, NaN propagates)
@@ -602,7 +625,11 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
     const bool f32 = cycle_prec == RAMD_PREC_FP32;
     // FP32 form: the outer FP64 operator reads FP32-stored values where
     // that is lossless (same FP64 results, half the value bytes)
-    auto outer = f32 ? launch_stream_op_c : launch_stream_op;
+    auto outer = [f32](const ramd_csr_s* M, int op, const double* x, const double* b, double* y, double w,
+                       hipStream_t st) {
+        if (f32) launch_stream_op_c(M, op, x, b, y, w, st);
+        else launch_stream_op(M, op, x, b, y, w, st);
+    };
     if (f32) {
         enable_fp32(h);
         csr_make_fp32(A);
@@ -617,9 +644,12 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
         if (f32) cycle32(h, 0, z32, r32, true);
         else cycle64(h, 0, z, r, true);
     };
-    auto dots = [&](const double* ap, double out[2]) {
-        if (f32) fpcg_rz_zap(ctx, n, r, (const float*)z32, ap, out);
-        else fpcg_rz_zap(ctx, n, r, (const double*)z, ap, out);
+    // device scalars: [0] p.Ap, [1] r.r, [2]/[5] r.z (current / next), [3] z.Ap, [4] alpha
+    double* sc = ctx->red_result;
+    int cur = 2, nxt = 5;
+    auto dots = [&](const double* ap, double* rz_d) {
+        if (f32) fpcg_rz_zap(ctx, n, r, (const float*)z32, ap, rz_d, sc + 3);
+        else fpcg_rz_zap(ctx, n, r, (const double*)z, ap, rz_d, sc + 3);
     };
     const double bnorm = norm2_64(ctx, n, b_d);
     const double scale = bnorm > 0 ? bnorm : 1.0;
@@ -638,29 +668,25 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
         if (f32) hipLaunchKernelGGL(to_f64, gv, bv, 0, s, (int64_t)n, z32, p);
         else RAMD_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         RAMD_HIP(hipGetLastError());
-        double d2[2];
-        dots(r, d2);
-        double rz = d2[0];
+        dots(r, sc + cur);
+        part = ctx_partials(ctx, (size_t)g);
         while (it < max_iter) {
-            // Ap = A p and p.Ap in one pass over A
-            double pAp;
-            launch_spmv_dot(A, f32, p, Ap, ctx->red_result, s);
-            fetch_scalars(ctx, ctx->red_result, 1, &pAp);
-            if (!(pAp > 0.0)) {
-                rc = std::isfinite(pAp) ? RAMD_E_INDEFINITE : RAMD_E_NUMERIC;
+            // Ap = A p and p.Ap in one pass over A; then x += alpha p,
+            // r -= alpha Ap, ||r||^2 (and r32) in one pass; one host read
+            launch_spmv_dot(A, f32, p, Ap, sc, s);
+            hipLaunchKernelGGL(fpcg_update_norm, dim3(g), dim3(kBlockThreads), 0, s, n, sc + cur, sc, sc + 4, p,
+                               Ap, x_d, r, f32 ? r32 : nullptr, part);
+            RAMD_HIP(hipGetLastError());
+            launch_reduce_final(ctx, part, g, sc + 1);
+            double pr[2];
+            fetch_scalars(ctx, sc, 2, pr);
+            if (!(pr[0] > 0.0)) {
+                // x and r took the step already; the error is reported instead of a result
+                rc = std::isfinite(pr[0]) ? RAMD_E_INDEFINITE : RAMD_E_NUMERIC;
                 break;
             }
-            const double alpha = rz / pAp;
-            // x += alpha p, r -= alpha Ap, ||r||^2 (and r32) in one pass
-            part = ctx_partials(ctx, (size_t)g);
-            hipLaunchKernelGGL(fpcg_update_norm, dim3(g), dim3(kBlockThreads), 0, s, n, alpha, p, Ap, x_d, r,
-                               f32 ? r32 : nullptr, part);
-            RAMD_HIP(hipGetLastError());
-            launch_reduce_final(ctx, part, g, ctx->red_result + 1);
-            double rr;
-            fetch_scalars(ctx, ctx->red_result + 1, 1, &rr);
             it++;
-            relres = std::sqrt(rr) / scale;
+            relres = std::sqrt(pr[1]) / scale;
             if (relres <= rtol) {
                 // confirm with the true residual; continue from it if the recurrence drifted
                 outer(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
@@ -679,13 +705,12 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
                 break;
             }
             cycle();
-            // Polak-Ribiere beta of flexible CG with r_new - r_old = -alpha Ap
-            dots(Ap, d2);
-            const double beta = -alpha * d2[1] / rz;
-            if (f32) hipLaunchKernelGGL((fpcg_dir<float>), gv, bv, 0, s, n, beta, z32, p);
-            else hipLaunchKernelGGL((fpcg_dir<double>), gv, bv, 0, s, n, beta, z, p);
+            // flexible beta with r_new - r_old = -alpha Ap
+            dots(Ap, sc + nxt);
+            if (f32) hipLaunchKernelGGL((fpcg_dir<float>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + cur, z32, p);
+            else hipLaunchKernelGGL((fpcg_dir<double>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + cur, z, p);
             RAMD_HIP(hipGetLastError());
-            rz = d2[0];
+            std::swap(cur, nxt);
         }
     }
     RAMD_HIP(hipStreamSynchronize(s));
@@ -860,6 +885,20 @@ int ramd_hier_states(ramd_hier h, int level, int32_t* states)
         const std::vector<int>& s = h->levels[level].states;
         RAMD_CHECK(!s.empty() || h->levels[level].n == 0, RAMD_E_STATE, "level %d has no splitting (coarsest)", level);
         std::copy(s.begin(), s.end(), states);
+    });
+}
+
+int ramd_hier_row_caps(ramd_hier h, int32_t* caps)
+{
+    return guarded([&] {
+        RAMD_CHECK(h && caps, RAMD_E_ARG, "ramd_hier_row_caps: NULL argument");
+        for (size_t l = 0; l < h->levels.size(); l++) {
+            const Level& L = h->levels[l];
+            const ramd_csr_s* M[3] = {L.A, L.P, L.R};
+            for (int m = 0; m < 3; m++)
+                for (int p = 0; p < 2; p++)
+                    caps[6 * l + 2 * m + p] = (M[m] && !M[m]->dia) ? M[m]->rb[p].cap : 0;
+        }
     });
 }
 

@@ -17,24 +17,24 @@
 
 namespace ramd {
 
-template <int OP, typename V, typename X>
+template <int OP, typename V, typename X, int CAP>
 __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
     const int* __restrict__ rowptr, const int* __restrict__ col, const V* __restrict__ val,
     const int* __restrict__ blocks, int nb, const X* __restrict__ x, const X* __restrict__ b,
-    X* __restrict__ y, X omega, int diag_first, double* __restrict__ dotp)
+    X* __restrict__ y, X omega, int diag_first, double* __restrict__ dotp, const X* __restrict__ dvec)
 {
-    __shared__ X prod[kStreamCap];
+    __shared__ X prod[CAP];
     const int blk = xcd_remap(blockIdx.x, nb);
     const int r0 = blocks[blk], r1 = blocks[blk + 1];
     const int s = rowptr[r0], e = rowptr[r1];
     const int tid = threadIdx.x;
 
-    if (e - s > kStreamCap) {
+    if (e - s > CAP) {
         // one long row, processed in LDS-sized chunks, summed by lane 0
         X acc = (OP == OP_RESIDUAL) ? b[r0] : X(0);
         X diag = 0;
-        for (int c0 = s; c0 < e; c0 += kStreamCap) {
-            const int cn = min(kStreamCap, e - c0);
+        for (int c0 = s; c0 < e; c0 += CAP) {
+            const int cn = min(CAP, e - c0);
             for (int j = tid; j < cn; j += kBlockThreads) prod[j] = X(val[c0 + j]) * x[col[c0 + j]];
             __syncthreads();
             if (tid == 0) {
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
     // phase 1: every lane issues all of its column/value loads, then all of
     // its x gathers, before the first multiply (16 independent loads in
     // flight per lane instead of one dependent chain)
-    constexpr int U = kStreamCap / kBlockThreads;
+    constexpr int U = CAP / kBlockThreads;
     const int len = e - s;
     int cc[U];
     X vv[U], xx[U];
@@ -128,7 +128,16 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
             return;
         }
         X acc = 0, diag = 0;
-        if (diag_first) {
+        if (diag_first && dvec) {
+            // the level's diagonal array holds a_ii (0 where absent); a row
+            // whose |a_ii| > 1e-16 stores it first, any other row keeps its
+            // value -- the row's first entry is not read again
+            const X dg = dvec[row];
+            if (fabs(dg) > X(1e-16)) {
+                diag = dg;
+                for (int k = k0 + 1; k < k1; k++) acc += prod[k];
+            }
+        } else if (diag_first) {
             // diagonal stored first, or absent (then the row keeps its value)
             if (col[s + k0] == row) {
                 diag = X(val[s + k0]);
@@ -310,14 +319,45 @@ static double op_bytes(const ramd_csr_s* A, int op, int sv, int sx, bool dia)
     return b;
 }
 
+template <typename V, typename X, int CAP>
+static void launch_stream_cap(const ramd_csr_s* A, const ramd_csr_s::RowBlocks& rb, const V* val, int op,
+                              const X* x, const X* b, X* y, X omega, hipStream_t s, double* dotp, const X* dvec)
+{
+    const int nb = rb.n;
+    const dim3 grid(nb), block(kBlockThreads);
+    const int df = (int)A->diag_first;
+    switch (op) {
+    case OP_SPMV:
+        hipLaunchKernelGGL((stream_kernel<OP_SPMV, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d, nb,
+                           x, b, y, omega, df, dotp, nullptr);
+        break;
+    case OP_APPEND:
+        hipLaunchKernelGGL((stream_kernel<OP_APPEND, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d,
+                           nb, x, b, y, omega, df, nullptr, nullptr);
+        break;
+    case OP_RESIDUAL:
+        hipLaunchKernelGGL((stream_kernel<OP_RESIDUAL, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d,
+                           nb, x, b, y, omega, df, nullptr, nullptr);
+        break;
+    case OP_JACOBI:
+        hipLaunchKernelGGL((stream_kernel<OP_JACOBI, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d,
+                           nb, x, b, y, omega, df, nullptr, dvec);
+        break;
+    case OP_ADD_INTO:
+        hipLaunchKernelGGL((stream_kernel<OP_ADD_INTO, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d,
+                           nb, x, b, y, omega, df, nullptr, nullptr);
+        break;
+    default:
+        fail(RAMD_E_ARG, "unknown stream op %d", op);
+    }
+}
+
 // val: CSR values; dval: stencil-diagonal values (nullptr -> CSR kernel)
 template <typename V, typename X>
 static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int op, const X* x, const X* b, X* y,
-                          X omega, hipStream_t s, double* dotp = nullptr)
+                          X omega, hipStream_t s, double* dotp = nullptr, const X* dvec = nullptr)
 {
     if (A->n_rows == 0) return;
-    const int nb = A->n_blocks;
-    dim3 grid(nb), block(kBlockThreads);
     ramd_ctx ctx = A->ctx;
     const bool prof = ctx->prof_min_nnz >= 0 && A->nnz >= ctx->prof_min_nnz;
     ProfPending pend{};
@@ -338,29 +378,15 @@ static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int 
         }
         return;
     }
-    switch (op) {
-    case OP_SPMV:
-        hipLaunchKernelGGL((stream_kernel<OP_SPMV, V, X>), grid, block, 0, s, A->rowptr, A->col, val, A->blocks,
-                           nb, x, b, y, omega, (int)A->diag_first, dotp);
+    const ramd_csr_s::RowBlocks& rb = A->rb[sizeof(X) == 4 ? 1 : 0];
+    switch (rb.cap) {
+    case 2048: launch_stream_cap<V, X, 2048>(A, rb, val, op, x, b, y, omega, s, dotp, dvec); break;
+    case 8192:
+        // 8192 FP64 products would fill the whole 64 KiB of LDS a workgroup may use
+        if constexpr (sizeof(X) == 4) launch_stream_cap<V, X, 8192>(A, rb, val, op, x, b, y, omega, s, dotp, dvec);
+        else fail(RAMD_E_STATE, "row-block cap 8192 is for FP32 vectors only");
         break;
-    case OP_APPEND:
-        hipLaunchKernelGGL((stream_kernel<OP_APPEND, V, X>), grid, block, 0, s, A->rowptr, A->col, val,
-                           A->blocks, nb, x, b, y, omega, (int)A->diag_first, nullptr);
-        break;
-    case OP_RESIDUAL:
-        hipLaunchKernelGGL((stream_kernel<OP_RESIDUAL, V, X>), grid, block, 0, s, A->rowptr, A->col, val,
-                           A->blocks, nb, x, b, y, omega, (int)A->diag_first, nullptr);
-        break;
-    case OP_JACOBI:
-        hipLaunchKernelGGL((stream_kernel<OP_JACOBI, V, X>), grid, block, 0, s, A->rowptr, A->col, val,
-                           A->blocks, nb, x, b, y, omega, (int)A->diag_first, nullptr);
-        break;
-    case OP_ADD_INTO:
-        hipLaunchKernelGGL((stream_kernel<OP_ADD_INTO, V, X>), grid, block, 0, s, A->rowptr, A->col, val,
-                           A->blocks, nb, x, b, y, omega, (int)A->diag_first, nullptr);
-        break;
-    default:
-        fail(RAMD_E_ARG, "unknown stream op %d", op);
+    default: launch_stream_cap<V, X, 4096>(A, rb, val, op, x, b, y, omega, s, dotp, dvec); break;
     }
     RAMD_HIP(hipGetLastError());
     if (prof) {
@@ -385,9 +411,9 @@ static void prof_resolve(ramd_ctx ctx)
 }
 
 void launch_stream_op(const ramd_csr_s* A, int op, const double* x, const double* b, double* y, double omega,
-                      hipStream_t s)
+                      hipStream_t s, const double* dvec)
 {
-    launch_stream<double, double>(A, A->val, A->dia ? A->dia->v64 : nullptr, op, x, b, y, omega, s);
+    launch_stream<double, double>(A, A->val, A->dia ? A->dia->v64 : nullptr, op, x, b, y, omega, s, nullptr, dvec);
 }
 
 void launch_stream_op_c(const ramd_csr_s* A, int op, const double* x, const double* b, double* y, double omega,
@@ -400,11 +426,86 @@ void launch_stream_op_c(const ramd_csr_s* A, int op, const double* x, const doub
 }
 
 void launch_stream_op32(const ramd_csr_s* A, int op, const float* x, const float* b, float* y, float omega,
-                        hipStream_t s)
+                        hipStream_t s, const float* dvec)
 {
     RAMD_CHECK(A->val32, RAMD_E_STATE, "FP32 values of the matrix were not built");
     RAMD_CHECK(!A->dia || A->dia->v32, RAMD_E_STATE, "FP32 diagonal values of the matrix were not built");
-    launch_stream<float, float>(A, A->val32, A->dia ? A->dia->v32 : nullptr, op, x, b, y, omega, s);
+    launch_stream<float, float>(A, A->val32, A->dia ? A->dia->v32 : nullptr, op, x, b, y, omega, s, nullptr, dvec);
+}
+
+void csr_autotune(ramd_csr_s* A, int prec)
+{
+    if (A->n_rows == 0 || A->nnz == 0 || A->dia) return;
+    RAMD_CHECK(prec == 0 || A->val32, RAMD_E_STATE, "csr_autotune: FP32 values were not built");
+    ramd_ctx ctx = A->ctx;
+    hipStream_t s = ctx->stream;
+    RAMD_HIP(hipStreamSynchronize(s));
+    if (A->rb[prec].d != A->blocks) {  // tuned before: start from the default blocks
+        dfree(A->rb[prec].d);
+        A->rb[prec] = {kStreamCap, A->n_blocks, A->blocks};
+    }
+    const size_t w = prec ? sizeof(float) : sizeof(double);
+    struct Scratch {
+        void *x = nullptr, *b = nullptr, *y = nullptr;
+        std::vector<int*> blocks;
+        ~Scratch()
+        {
+            dfree(x);
+            dfree(b);
+            dfree(y);
+            for (int* p : blocks) dfree(p);
+        }
+    } t;
+    t.x = dmalloc(w * std::max(A->n_cols, 1));
+    t.b = dmalloc(w * std::max(A->n_rows, 1));
+    t.y = dmalloc(w * std::max(A->n_rows, 1));
+    RAMD_HIP(hipMemsetAsync(t.x, 0, w * std::max(A->n_cols, 1), s));
+    RAMD_HIP(hipMemsetAsync(t.b, 0, w * std::max(A->n_rows, 1), s));
+    const int op = A->n_rows == A->n_cols ? OP_RESIDUAL : OP_SPMV;
+    hipEvent_t e0, e1;
+    RAMD_HIP(hipEventCreate(&e0));
+    RAMD_HIP(hipEventCreate(&e1));
+    ramd_csr_s::RowBlocks best = A->rb[prec];
+    float best_ms = 1e30f;
+    for (int cap : kStreamCaps) {
+        if (prec == 0 && cap > 4096) continue;
+        ramd_csr_s::RowBlocks rb;
+        rb.cap = cap;
+        if (cap == kStreamCap) {
+            rb.n = A->n_blocks;
+            rb.d = A->blocks;
+        } else {
+            std::vector<int> blk = make_row_blocks(A->h_rowptr.data(), A->n_rows, cap);
+            rb.n = (int)blk.size() - 1;
+            rb.d = dalloc<int>(blk.size());
+            t.blocks.push_back(rb.d);
+            RAMD_HIP(hipMemcpyAsync(rb.d, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice, s));
+        }
+        A->rb[prec] = rb;
+        auto run = [&] {
+            if (prec) launch_stream<float, float>(A, A->val32, nullptr, op, (const float*)t.x, (const float*)t.b,
+                                                  (float*)t.y, 0.f, s);
+            else launch_stream<double, double>(A, A->val, nullptr, op, (const double*)t.x, (const double*)t.b,
+                                               (double*)t.y, 0.0, s);
+        };
+        run();
+        RAMD_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; r++) run();
+        RAMD_HIP(hipEventRecord(e1, s));
+        RAMD_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        RAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = rb;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // keep the winner's blocks, release the others
+    for (int*& p : t.blocks)
+        if (p == best.d) p = nullptr;
+    A->rb[prec] = best;
 }
 
 double* ctx_partials(ramd_ctx ctx, size_t count)
@@ -425,7 +526,7 @@ void launch_spmv_dot(const ramd_csr_s* A, bool values_fp32, const double* x, dou
 {
     ramd_ctx ctx = A->ctx;
     const bool dia32 = values_fp32 && A->dia && A->dia->v32 && A->dia->v32_exact;
-    const int np = A->dia ? (A->n_rows + kBlockThreads - 1) / kBlockThreads : A->n_blocks;
+    const int np = A->dia ? (A->n_rows + kBlockThreads - 1) / kBlockThreads : A->rb[0].n;
     double* part = ctx_partials(ctx, (size_t)std::max(np, 1));
     if (dia32)
         launch_stream<float, double>(A, nullptr, A->dia->v32, OP_SPMV, x, nullptr, y, 0.0, s, part);
@@ -492,9 +593,30 @@ void launch_norm2sq(ramd_ctx ctx, int64_t n, const double* x, double* out_d)
     RAMD_HIP(hipGetLastError());
 }
 
+// First stage for long partial arrays: workgroup g sums the contiguous
+// chunk [g*chunk, (g+1)*chunk) in a fixed order.
+__global__ __launch_bounds__(kBlockThreads) void reduce_chunks(const double* __restrict__ partials, int np, int chunk,
+                                                               double* __restrict__ out)
+{
+    const int b0 = blockIdx.x * chunk, b1 = min(np, b0 + chunk);
+    double s = 0;
+    for (int i = b0 + threadIdx.x; i < b1; i += kBlockThreads) s += partials[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
 void launch_reduce_final(ramd_ctx ctx, const double* partials, int np, double* out_d)
 {
-    hipLaunchKernelGGL(reduce_final, dim3(1), dim3(kBlockThreads), 0, ctx->stream, partials, np, out_d);
+    constexpr int kChunks = 256;
+    if (np > 4096) {
+        // one workgroup would read every partial serially: split in two stages
+        const int chunk = (np + kChunks - 1) / kChunks;
+        double* mid = ctx->red_partials + 3072;  // 256 doubles past the grid-stride partials
+        hipLaunchKernelGGL(reduce_chunks, dim3(kChunks), dim3(kBlockThreads), 0, ctx->stream, partials, np, chunk, mid);
+        hipLaunchKernelGGL(reduce_final, dim3(1), dim3(kBlockThreads), 0, ctx->stream, mid, kChunks, out_d);
+    } else {
+        hipLaunchKernelGGL(reduce_final, dim3(1), dim3(kBlockThreads), 0, ctx->stream, partials, np, out_d);
+    }
     RAMD_HIP(hipGetLastError());
 }
 

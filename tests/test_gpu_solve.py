@@ -266,3 +266,16 @@ def test_cg_golden_history(golden, ctx):
     xo, reso, ito = K.cg(Ah, np.zeros(n), b)
     # CG amplifies the rounding of the (differently ordered) inner products
     assert it == ito and np.allclose(res * K.norm2(b), reso * K.norm2(b), rtol=0, atol=1e-6)
+
+
+def test_row_cap_autotuner_choices(problem):
+    """Every CSR operator of the cycle got one of the caps the kernels are
+    built for (FP32-only cap 8192 never for FP64 vectors); the bit-exact
+    V-cycle tests above ran with these choices."""
+    ml = problem["ml"]
+    ml.enable_fp32()
+    caps = ml.row_caps()
+    assert caps.shape == (ml.num_levels, 3, 2)
+    assert set(np.unique(caps)) <= {0, 2048, 4096, 8192}
+    assert not np.any(caps[:, :, 0] == 8192)
+    assert np.all(caps[-1, 1:] == 0)  # no P, R at the coarsest level
