@@ -166,6 +166,15 @@ def main():
                     gbs=round(by.value / (ms.value * 1e-3) / 1e9, 1))
     check(L.ramd_prof_enable(ctx.h, -1))
     log(f"FP32-cycle solve: {t32:.4f}s, iterations {its32}, relres {rel32:.3e}")
+    x32_h = x.numpy()
+
+    # ---- error of the FP32-cycle solution against an FP64 reference solve
+    # (FP64 cycles, whose V-cycle is bit-identical to the oracle's, to 1e-12)
+    x.set_const_value(0.0)
+    ra.FPCG(A, ml, x, b, 1e-12, 500, ra.PREC_FP64)
+    xref = x.numpy()
+    err32 = float(np.linalg.norm(x32_h - xref) / np.linalg.norm(xref))
+    log(f"FP32-cycle solution vs FP64 reference (1e-12): relative error {err32:.3e}")
 
     # ---- FP64-cycle solves (same protocol)
     t64 = None
@@ -243,6 +252,7 @@ def main():
         "iterations_fp64": its64,
         "relres_fp32": rel32,
         "relres_fp64": rel64,
+        "error_fp32_vs_fp64_reference": err32,
         "setup_s": round(setup_s, 2),
         "vcycle": vc,
         "kernels": prof,

@@ -90,6 +90,12 @@ int ramd_csr_set_values(ramd_csr A, const double* val_d);
  * for square matrices of >= 4096 rows with <= 32 distinct column offsets,
  * <= 15% padding and one common per-row order); 0 when it is CSR only. */
 int ramd_csr_diagonals(ramd_csr A, int* nd);
+/* Kernel variant of the stencil-diagonal form for the precision pair pp
+ * (0 FP64 values and vectors, 1 FP32/FP32, 2 FP32 values/FP64 vectors):
+ * mode 0 gathers x from memory, 1 stages each workgroup's x window in LDS,
+ * -1 lets the autotuner time both at the next hierarchy setup or solve.
+ * Both variants give identical bits.  No reference counterpart. */
+int ramd_csr_set_dia_staging(ramd_csr A, int pp, int mode);
 
 /* ------------------------------------------------------------------ */
 /* SpMV family (FP64; each row summed left to right in stored order, as  */

@@ -204,6 +204,11 @@ class ParCSRMatrix:
         check(lib().ramd_galerkin(self.h, P.h, C.byref(h)))
         return ParCSRMatrix(self.ctx, h)
 
+    def set_dia_staging(self, pp, mode):
+        """Stencil-diagonal kernel variant for precision pair pp (0 FP64, 1 FP32,
+        2 FP32 values with FP64 vectors): 0 gather, 1 LDS-staged x, -1 autotune."""
+        check(lib().ramd_csr_set_dia_staging(self.h, int(pp), int(mode)))
+
     def set_values(self, vals):
         """New values (a device ParVector of nnz entries, stored order) for the same pattern."""
         check(lib().ramd_csr_set_values(self.h, vals.ptr))

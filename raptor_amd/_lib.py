@@ -81,6 +81,7 @@ _SIGS = {
     "ramd_csr_download": (P, PI32, PI32, PD),
     "ramd_csr_destroy": (P,),
     "ramd_csr_diagonals": (P, C.POINTER(C.c_int)),
+    "ramd_csr_set_dia_staging": (P, C.c_int, C.c_int),
     "ramd_spmv": (P, P, P),
     "ramd_spmv_append": (P, P, P),
     "ramd_residual": (P, P, P, P),

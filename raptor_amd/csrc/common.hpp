@@ -112,10 +112,28 @@ struct DiaArgs {
     int off[kDiaMax];  // offset of diagonal k (ascending in k)
     int ord[kDiaMax];  // visiting order: ord[kk] = diagonal index
 };
+// x window of a 256-row workgroup for the LDS-staged kernels: the offsets
+// form runs of consecutive values; run j needs x[r0 + lo[j] .. r0 + 255 +
+// lo[j] + len[j] - 1], stored at base[j] in LDS, and diagonal k is read at
+// lofs[k] + (row - r0).
+constexpr int kDiaWin = 2560;  // LDS values per workgroup (20 KiB in FP64)
+struct DiaWindow {
+    int nr = 0;
+    int win = 0;            // total values
+    int lo[kDiaMax];
+    int len[kDiaMax];
+    int base[kDiaMax];
+    int lofs[kDiaMax];
+};
 struct Dia {
     int nd = 0;
     int center = -1;            // diagonal index of offset 0 (-1 if none)
     DiaArgs args{};
+    DiaWindow window{};
+    // per (values, vectors) precision ([0] FP64/FP64, [1] FP32/FP32,
+    // [2] FP32/FP64): 1 = stage x in LDS, 0 = gather, -1 = not timed yet
+    // (gather).  Chosen by timing both; the result bits are the same.
+    int lds_x[3] = {-1, -1, -1};
     unsigned* mask = nullptr;   // device, n
     double* v64 = nullptr;      // device, nd * n
     float* v32 = nullptr;       // device, nd * n (built with the FP32 copy)
@@ -176,6 +194,11 @@ constexpr int kStreamCaps[3] = {2048, 4096, 8192};
 // blocks.  The cap changes only which workgroup owns a row, never the order
 // of a row's sum, so results are identical for every choice.
 void csr_autotune(ramd_csr_s* A, int prec);
+// For a matrix with the stencil-diagonal form whose x window fits kDiaWin:
+// times the gather and the LDS-staged kernels for the precision pair `pp`
+// (0 FP64 values/vectors, 1 FP32/FP32, 2 FP32 values/FP64 vectors) and keeps
+// the faster one (same result bits either way).  Timed once per pair.
+void dia_autotune(ramd_csr_s* A, int pp);
 // New values for the same pattern (device array); refreshes every derived
 // copy (FP32 values, stencil-diagonal planes, the cached transpose).
 void csr_set_values(ramd_csr_s* A, const double* val_d);

@@ -185,6 +185,24 @@ bool dia_build(ramd_csr_s* A, const int* rp, const int* col, const double* val)
             if (k != center) d->args.ord[kk++] = k;
     }
     for (; kk < kDiaMax; kk++) d->args.ord[kk] = 0;
+    // runs of consecutive offsets and the LDS window they need
+    {
+        DiaWindow& w = d->window;
+        int tot = 0;
+        for (int k = 0; k < nd; k++) {
+            if (k == 0 || offs[k] != offs[k - 1] + 1) {
+                w.lo[w.nr] = (int)offs[k];
+                w.len[w.nr] = 0;
+                w.base[w.nr] = tot;
+                w.nr++;
+            }
+            const int j = w.nr - 1;
+            w.lofs[k] = w.base[j] + (int)(offs[k] - w.lo[j]);
+            w.len[j]++;
+            if (k + 1 == nd || offs[k + 1] != offs[k] + 1) tot += 255 + w.len[j];
+        }
+        w.win = tot;
+    }
     d->v32_exact = exact32;
     try {
         d->mask = dalloc<unsigned>((size_t)n);
@@ -543,6 +561,18 @@ int ramd_csr_set_values(ramd_csr A, const double* val_d)
         RAMD_CHECK(!A->borrowed, RAMD_E_STATE, "ramd_csr_set_values: handle is owned by a hierarchy");
         A->ctx->set_current();
         csr_set_values(A, val_d);
+    });
+}
+
+int ramd_csr_set_dia_staging(ramd_csr A, int pp, int mode)
+{
+    return guarded([&] {
+        RAMD_CHECK(A && pp >= 0 && pp < 3 && mode >= -1 && mode <= 1, RAMD_E_ARG,
+                   "ramd_csr_set_dia_staging: bad argument");
+        RAMD_CHECK(A->dia, RAMD_E_STATE, "ramd_csr_set_dia_staging: matrix has no stencil-diagonal form");
+        RAMD_CHECK(mode != 1 || A->dia->window.win <= kDiaWin, RAMD_E_ARG,
+                   "ramd_csr_set_dia_staging: x window of %d values exceeds %d", A->dia->window.win, kDiaWin);
+        A->dia->lds_x[pp] = mode;
     });
 }
 

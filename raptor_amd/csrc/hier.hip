@@ -455,9 +455,13 @@ static void hier_setup(ramd_hier_s* h, Csr Af)
     h->pz = dalloc<double>((size_t)n0);
     h->pp = dalloc<double>((size_t)n0);
     h->pAp = dalloc<double>((size_t)n0);
-    // row-block caps of every CSR operator of the cycle, FP64 vectors
+    // row-block caps of every CSR operator of the cycle and the x staging of
+    // the diagonal-form ones, FP64 vectors
     for (Level& L : h->levels) {
-        if (&L != &h->levels.back()) csr_autotune(L.A, 0);
+        if (&L != &h->levels.back()) {
+            csr_autotune(L.A, 0);
+            dia_autotune(L.A, 0);
+        }
         if (L.P) {
             csr_autotune(L.P, 0);
             csr_autotune(L.R, 0);
@@ -481,7 +485,10 @@ void enable_fp32(ramd_hier_s* h)
             csr_make_fp32(L.P);
             csr_make_fp32(L.R);
         }
-        if (&L != &h->levels.back()) csr_autotune(L.A, 1);
+        if (&L != &h->levels.back()) {
+            csr_autotune(L.A, 1);
+            dia_autotune(L.A, 1);
+        }
         if (L.P) {
             csr_autotune(L.P, 1);
             csr_autotune(L.R, 1);
@@ -633,6 +640,9 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
     if (f32) {
         enable_fp32(h);
         csr_make_fp32(A);
+        dia_autotune(A, 2);  // the outer operator reads FP32 values into FP64 vectors
+    } else {
+        dia_autotune(A, 0);
     }
     double *r = h->pr, *z = h->pz, *p = h->pp, *Ap = h->pAp;
     float *r32 = h->pr32, *z32 = h->pz32;

@@ -260,6 +260,88 @@ __global__ __launch_bounds__(kBlockThreads) void dia_kernel(int n, DiaArgs a, in
     }
 }
 
+// The same operations with the workgroup's x window staged in LDS: the
+// 256 rows of a workgroup read x only at row + offset, i.e. inside a few
+// runs of consecutive addresses, which are loaded once with coalesced loads
+// (indices clamped into [0, n): a clamped value is only ever read for an
+// absent diagonal, whose product the select drops).  Same operation
+// sequence per row as dia_kernel, so the same bits.
+template <int OP, typename V, typename X, int ND>
+__global__ __launch_bounds__(kBlockThreads) void dia_lds_kernel(int n, DiaArgs a, DiaWindow w, int nd, int center,
+                                                               const unsigned* __restrict__ mask,
+                                                               const V* __restrict__ val, const X* __restrict__ x,
+                                                               const X* __restrict__ b, X* __restrict__ y, X omega,
+                                                               double* __restrict__ dotp)
+{
+    __shared__ X xs[kDiaWin];
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int r0 = blk * kBlockThreads;
+    const int tid = threadIdx.x;
+    for (int j = 0; j < w.nr; j++) {
+        const int len = kBlockThreads - 1 + w.len[j];
+        for (int t = tid; t < len; t += kBlockThreads) {
+            int g = r0 + w.lo[j] + t;
+            g = g < 0 ? 0 : (g >= n ? n - 1 : g);
+            xs[w.base[j] + t] = x[g];
+        }
+    }
+    __syncthreads();
+    const int row = r0 + tid;
+    const bool valid = row < n;
+    X acc = (OP == OP_RESIDUAL && valid) ? b[row] : X(0);
+    X diag = 0;
+    const unsigned m = valid ? mask[row] : 0u;
+    const int vrow = valid ? row : 0;
+    constexpr int CH = 9;
+#pragma unroll
+    for (int c0 = 0; c0 < ND; c0 += CH) {
+        V vv[CH];
+        bool on[CH];
+        int kd[CH];
+#pragma unroll
+        for (int t = 0; t < CH; t++) {
+            const int kk = c0 + t;
+            if (kk < ND) {
+                const bool live = (ND == kDiaMax) ? (kk < nd) : true;
+                const int k = live ? a.ord[kk] : 0;
+                kd[t] = k;
+                on[t] = live && ((m >> k) & 1u);
+                vv[t] = val[(size_t)k * n + vrow];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < CH; t++) {
+            if (c0 + t < ND) {
+                const X p = X(vv[t]) * xs[w.lofs[kd[t]] + tid];
+                if (OP == OP_JACOBI) {
+                    if (kd[t] == center) diag = on[t] ? X(vv[t]) : diag;
+                    else acc = on[t] ? acc + p : acc;
+                } else if (OP == OP_RESIDUAL) {
+                    acc = on[t] ? acc - p : acc;
+                } else {
+                    acc = on[t] ? acc + p : acc;
+                }
+            }
+        }
+    }
+    if (OP == OP_SPMV && dotp) {
+        const double v = block_sum(valid ? (double)x[row] * (double)acc : 0.0);
+        if (tid == 0) dotp[blk] = v;
+    }
+    if (!valid) return;
+    if (OP == OP_SPMV || OP == OP_RESIDUAL) {
+        y[row] = acc;
+    } else if (OP == OP_APPEND) {
+        y[row] += acc;
+    } else if (OP == OP_ADD_INTO) {
+        y[row] = b[row] + acc;
+    } else {
+        const X xo = x[row];
+        y[row] = (m != 0u && fabs(diag) > X(1e-16)) ? ((X(1.0) - omega) * xo) + (omega * ((b[row] - acc) / diag))
+                                                   : xo;
+    }
+}
+
 template <int OP, typename V, typename X>
 static void launch_dia_op(const ramd_csr_s* A, const V* val, const X* x, const X* b, X* y, X omega, hipStream_t s,
                           double* dotp)
@@ -267,14 +349,26 @@ static void launch_dia_op(const ramd_csr_s* A, const V* val, const X* x, const X
     const Dia* d = A->dia;
     const int n = A->n_rows;
     dim3 grid((n + kBlockThreads - 1) / kBlockThreads), block(kBlockThreads);
+    const int pp = sizeof(X) == 4 ? 1 : (sizeof(V) == 4 ? 2 : 0);
+    if (d->lds_x[pp] == 1 && d->window.win <= kDiaWin) {
+#define DIA_LDS(NDV)                                                                                            \
+    hipLaunchKernelGGL((dia_lds_kernel<OP, V, X, NDV>), grid, block, 0, s, n, d->args, d->window, d->nd, d->center, \
+                       d->mask, val, x, b, y, omega, dotp)
+        if (d->nd == 27) DIA_LDS(27);
+        else if (d->nd == 9) DIA_LDS(9);
+        else if (d->nd == 7) DIA_LDS(7);
+        else DIA_LDS(kDiaMax);
+#undef DIA_LDS
+    } else {
 #define DIA_LAUNCH(NDV)                                                                                         \
     hipLaunchKernelGGL((dia_kernel<OP, V, X, NDV>), grid, block, 0, s, n, d->args, d->nd, d->center, d->mask, val, \
                        x, b, y, omega, dotp)
-    if (d->nd == 27) DIA_LAUNCH(27);
-    else if (d->nd == 9) DIA_LAUNCH(9);
-    else if (d->nd == 7) DIA_LAUNCH(7);
-    else DIA_LAUNCH(kDiaMax);
+        if (d->nd == 27) DIA_LAUNCH(27);
+        else if (d->nd == 9) DIA_LAUNCH(9);
+        else if (d->nd == 7) DIA_LAUNCH(7);
+        else DIA_LAUNCH(kDiaMax);
 #undef DIA_LAUNCH
+    }
     RAMD_HIP(hipGetLastError());
 }
 
@@ -506,6 +600,58 @@ void csr_autotune(ramd_csr_s* A, int prec)
     for (int*& p : t.blocks)
         if (p == best.d) p = nullptr;
     A->rb[prec] = best;
+}
+
+void dia_autotune(ramd_csr_s* A, int pp)
+{
+    Dia* d = A->dia;
+    if (!d || d->window.win > kDiaWin || A->n_rows == 0 || d->lds_x[pp] >= 0) return;
+    RAMD_CHECK(pp == 0 || d->v32, RAMD_E_STATE, "dia_autotune: FP32 values were not built");
+    ramd_ctx ctx = A->ctx;
+    hipStream_t s = ctx->stream;
+    const size_t w = pp == 1 ? sizeof(float) : sizeof(double);
+    const size_t bytes = w * (size_t)A->n_rows;
+    struct Scratch {
+        void *x = nullptr, *b = nullptr, *y = nullptr;
+        ~Scratch()
+        {
+            dfree(x);
+            dfree(b);
+            dfree(y);
+        }
+    } t;
+    t.x = dmalloc(bytes);
+    t.b = dmalloc(bytes);
+    t.y = dmalloc(bytes);
+    RAMD_HIP(hipMemsetAsync(t.x, 0, bytes, s));
+    RAMD_HIP(hipMemsetAsync(t.b, 0, bytes, s));
+    hipEvent_t e0, e1;
+    RAMD_HIP(hipEventCreate(&e0));
+    RAMD_HIP(hipEventCreate(&e1));
+    float ms[2] = {0, 0};
+    for (int mode = 0; mode < 2; mode++) {
+        d->lds_x[pp] = mode;
+        auto run = [&] {
+            if (pp == 1)
+                launch_dia<float, float>(A, d->v32, OP_RESIDUAL, (const float*)t.x, (const float*)t.b, (float*)t.y,
+                                         0.f, s, nullptr);
+            else if (pp == 2)
+                launch_dia<float, double>(A, d->v32, OP_SPMV, (const double*)t.x, (const double*)t.b, (double*)t.y,
+                                          0.0, s, nullptr);
+            else
+                launch_dia<double, double>(A, d->v64, OP_RESIDUAL, (const double*)t.x, (const double*)t.b,
+                                           (double*)t.y, 0.0, s, nullptr);
+        };
+        run();
+        RAMD_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; r++) run();
+        RAMD_HIP(hipEventRecord(e1, s));
+        RAMD_HIP(hipEventSynchronize(e1));
+        RAMD_HIP(hipEventElapsedTime(&ms[mode], e0, e1));
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    d->lds_x[pp] = ms[1] < ms[0] ? 1 : 0;
 }
 
 double* ctx_partials(ramd_ctx ctx, size_t count)

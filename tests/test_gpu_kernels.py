@@ -65,6 +65,35 @@ def test_spmv_family_bit_exact(ctx, name, A):
     assert np.array_equal(t.numpy(), K.spmv_t(A, xr))
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("name,gen", [("aniso64", lambda: ra.gallery_aniso2d(64, 0.001, math.pi / 8)),
+                                      ("lap27_20", lambda: ra.gallery_laplace27(20)),
+                                      ("fv7_18", lambda: ra.gallery_fv7(18, seed=2))])
+def test_dia_staging_variants_bit_exact(ctx, name, gen, mode):
+    """Both kernels of the stencil-diagonal form (x gathered, x window staged
+    in LDS), with FP64 values and with the exact FP32 copy, give the CSR
+    oracle's bits for every operation."""
+    A = K.move_diag(K.Csr(*gen()))
+    D = to_dev(ctx, A)
+    assert D.diagonals() > 0
+    for pp in (0, 2):
+        D.set_dia_staging(pp, mode)
+    rng = np.random.default_rng(14)
+    x, b = rng.standard_normal(A.n_rows), rng.standard_normal(A.n_rows)
+    y = ra.ParVector(ctx, A.n_rows)
+    D.mult(dev(ctx, x), y)
+    assert np.array_equal(y.numpy(), K.spmv(A, x))
+    r = ra.ParVector(ctx, A.n_rows)
+    D.residual(dev(ctx, x), dev(ctx, b), r)
+    assert np.array_equal(r.numpy(), K.residual(A, x, b))
+    y2 = dev(ctx, b)
+    D.mult_append(dev(ctx, x), y2)
+    assert np.array_equal(y2.numpy(), K.spmv_append(A, x, b))
+    xj = dev(ctx, x)
+    D.jacobi(xj, dev(ctx, b), ra.ParVector(ctx, A.n_rows), 2, 2.0 / 3.0)
+    assert np.array_equal(xj.numpy(), K.jacobi(A, x, b, 2, 2.0 / 3.0))
+
+
 @pytest.mark.parametrize("name,A", cases()[:2], ids=[c[0] for c in cases()[:2]])
 def test_jacobi_bit_exact(ctx, name, A):
     rng = np.random.default_rng(13)
