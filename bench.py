@@ -76,6 +76,7 @@ def main():
     ap.add_argument("--m", type=int, default=0, help="override the grid edge")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-cycle comparison solves")
+    ap.add_argument("--no-graph", action="store_true", help="c4: launch the block iteration eagerly (no HIP graphs)")
     ap.add_argument("--calibrate", action="store_true",
                     help="end with one axpy over 2^27 doubles (2 GiB read, 1 GiB written), the reference "
                          "dispatch for scaling hardware byte counters")
@@ -351,7 +352,7 @@ def bench_batch(args, m, cfg):
 
     def block(prec):
         X.set_const_value(0.0)
-        return ra.FPCG_block(A, ml, X, B, k, 1e-8, 500, prec, use_graph=True)
+        return ra.FPCG_block(A, ml, X, B, k, 1e-8, 500, prec, use_graph=not args.no_graph)
 
     def sequential(prec):
         its = []
@@ -406,7 +407,7 @@ def bench_batch(args, m, cfg):
         "data": "synthetic",
         "config": dict(workload=f"aniso9pt-{m}^2 RS(CLJP, mod-classical, theta=0.25) Jacobi(2/3) "
                                 f"{k}-RHS lockstep FPCG to 1e-8", n=n, nnz=int(col.size), k=k,
-                       levels=list(map(int, nl))),
+                       levels=list(map(int, nl)), graphs=not args.no_graph),
         "setup_s": round(setup_s, 2),
         "solves": res,
         "block_speedup_fp32": round(res["sequential_fp32"]["s"] / res["block_fp32"]["s"], 3),

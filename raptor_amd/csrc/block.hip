@@ -205,8 +205,9 @@ static void blaunch(const ramd_csr_s* A, int lpr, int op, int k, const X* x, con
 
 // ---------------------------------------------------------------------------
 // elementwise and per-column kernels.  Layout of the per-column kernels:
-// thread t of a workgroup owns column t % 64 and row offset t / 64; rows
-// advance by 4 * gridDim.x.  Sums are combined in a fixed order.
+// with kp = the power of two >= k, thread t of a workgroup owns column
+// t % kp and row offset t / kp; rows advance by (256 / kp) * gridDim.x.
+// Sums are combined in a fixed order.
 // ---------------------------------------------------------------------------
 constexpr int kColBlocks = 512;
 
@@ -272,48 +273,54 @@ __global__ __launch_bounds__(256) void bcoarse_solve(int n, int k, const double*
     for (int i = threadIdx.x; i < n; i += blockDim.x) x[(size_t)i * k + c] = (X)sb[i];
 }
 
-// Combines the four row-offset sums of each column in LDS and stores the
-// workgroup's partial for column c at partials[blockIdx.x * k + c].
-__device__ __forceinline__ void col_partial(double v, int k, double* __restrict__ partials)
+// Combines the 256 / kp row-offset sums of each column in LDS (in offset
+// order) and stores the workgroup's partial for column c at
+// partials[blockIdx.x * k + c].
+__device__ __forceinline__ void col_partial(double v, int k, int kp, double* __restrict__ partials)
 {
     __shared__ double sh[256];
     sh[threadIdx.x] = v;
     __syncthreads();
-    if (threadIdx.x < k)
-        partials[(size_t)blockIdx.x * k + threadIdx.x] =
-            ((sh[threadIdx.x] + sh[threadIdx.x + 64]) + sh[threadIdx.x + 128]) + sh[threadIdx.x + 192];
+    if (threadIdx.x < k) {
+        double t = sh[threadIdx.x];
+        for (int o = kp; o < 256; o += kp) t += sh[threadIdx.x + o];
+        partials[(size_t)blockIdx.x * k + threadIdx.x] = t;
+    }
     __syncthreads();
 }
 
 // per-column a.b (MODE 0) or r.z and z.w (MODE 1: two partial arrays)
 template <int MODE, typename Z>
-__global__ __launch_bounds__(256) void bcol_dots(int n, int k, const double* __restrict__ a, const Z* __restrict__ z,
-                                                 const double* __restrict__ w, double* __restrict__ partials)
+__global__ __launch_bounds__(256) void bcol_dots(int n, int k, int kp, const double* __restrict__ a,
+                                                 const Z* __restrict__ z, const double* __restrict__ w,
+                                                 double* __restrict__ partials)
 {
-    const int c = threadIdx.x & 63;
+    const int c = threadIdx.x & (kp - 1);
+    const int rpb = 256 / kp;
     double s1 = 0, s2 = 0;
     if (c < k)
-        for (int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * 4) {
+        for (int64_t i = (int64_t)blockIdx.x * rpb + threadIdx.x / kp; i < n; i += (int64_t)gridDim.x * rpb) {
             const size_t e = (size_t)i * k + c;
             const double zi = (double)z[e];
             s1 += a[e] * zi;
             if (MODE == 1) s2 += zi * w[e];
         }
-    col_partial(s1, k, partials);
-    if (MODE == 1) col_partial(s2, k, partials + (size_t)gridDim.x * k);
+    col_partial(s1, k, kp, partials);
+    if (MODE == 1) col_partial(s2, k, kp, partials + (size_t)gridDim.x * k);
 }
 
 // x += alpha p; r -= alpha Ap; r32 = (float) r; per-column sums of r^2
-__global__ __launch_bounds__(256) void bupdate(int n, int k, const double* __restrict__ alpha,
+__global__ __launch_bounds__(256) void bupdate(int n, int k, int kp, const double* __restrict__ alpha,
                                                const double* __restrict__ p, const double* __restrict__ Ap,
                                                double* __restrict__ x, double* __restrict__ r,
                                                float* __restrict__ r32, double* __restrict__ partials)
 {
-    const int c = threadIdx.x & 63;
+    const int c = threadIdx.x & (kp - 1);
+    const int rpb = 256 / kp;
     double s = 0;
     if (c < k) {
         const double al = alpha[c];
-        for (int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * 4) {
+        for (int64_t i = (int64_t)blockIdx.x * rpb + threadIdx.x / kp; i < n; i += (int64_t)gridDim.x * rpb) {
             const size_t e = (size_t)i * k + c;
             x[e] += al * p[e];
             const double ri = r[e] - al * Ap[e];
@@ -322,18 +329,19 @@ __global__ __launch_bounds__(256) void bupdate(int n, int k, const double* __res
             s += ri * ri;
         }
     }
-    col_partial(s, k, partials);
+    col_partial(s, k, kp, partials);
 }
 
 // p = z + beta p
 template <typename Z>
-__global__ __launch_bounds__(256) void bdir(int n, int k, const double* __restrict__ beta, const Z* __restrict__ z,
-                                            double* __restrict__ p)
+__global__ __launch_bounds__(256) void bdir(int n, int k, int kp, const double* __restrict__ beta,
+                                            const Z* __restrict__ z, double* __restrict__ p)
 {
-    const int c = threadIdx.x & 63;
+    const int c = threadIdx.x & (kp - 1);
+    const int rpb = 256 / kp;
     if (c >= k) return;
     const double be = beta[c];
-    for (int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * 4) {
+    for (int64_t i = (int64_t)blockIdx.x * rpb + threadIdx.x / kp; i < n; i += (int64_t)gridDim.x * rpb) {
         const size_t e = (size_t)i * k + c;
         p[e] = (double)z[e] + be * p[e];
     }
@@ -659,18 +667,33 @@ static void bcycle(ramd_hier_s* h, BlockState* bs, int l, X* x, const X* b)
     if (cur != x) RAMD_HIP(hipMemcpyAsync(x, cur, sizeof(X) * (size_t)L.n * k, hipMemcpyDeviceToDevice, s));
 }
 
+static int pow2_at_least(int k)
+{
+    int p = 1;
+    while (p < k) p <<= 1;
+    return p;
+}
+
+// workgroups of the per-column kernels for n rows
+static int col_blocks(int n, int kp) { return std::max(1, std::min(kColBlocks, (n + 256 / kp - 1) / (256 / kp))); }
+
 static void col_dots(ramd_ctx ctx, BlockState* bs, int n, int mode, const double* a, const void* z, bool z32,
                      const double* w, double* out1, double* out2)
 {
     hipStream_t s = ctx->stream;
     const int k = bs->k;
-    const int g = std::max(1, std::min(kColBlocks, (n + 3) / 4));
+    const int kp = pow2_at_least(k);
+    const int g = col_blocks(n, kp);
     if (mode == 0) {
-        if (z32) hipLaunchKernelGGL((bcol_dots<0, float>), dim3(g), dim3(256), 0, s, n, k, a, (const float*)z, w, bs->partials);
-        else hipLaunchKernelGGL((bcol_dots<0, double>), dim3(g), dim3(256), 0, s, n, k, a, (const double*)z, w, bs->partials);
+        if (z32) hipLaunchKernelGGL((bcol_dots<0, float>), dim3(g), dim3(256), 0, s, n, k, kp, a, (const float*)z, w,
+                                    bs->partials);
+        else hipLaunchKernelGGL((bcol_dots<0, double>), dim3(g), dim3(256), 0, s, n, k, kp, a, (const double*)z, w,
+                                bs->partials);
     } else {
-        if (z32) hipLaunchKernelGGL((bcol_dots<1, float>), dim3(g), dim3(256), 0, s, n, k, a, (const float*)z, w, bs->partials);
-        else hipLaunchKernelGGL((bcol_dots<1, double>), dim3(g), dim3(256), 0, s, n, k, a, (const double*)z, w, bs->partials);
+        if (z32) hipLaunchKernelGGL((bcol_dots<1, float>), dim3(g), dim3(256), 0, s, n, k, kp, a, (const float*)z, w,
+                                    bs->partials);
+        else hipLaunchKernelGGL((bcol_dots<1, double>), dim3(g), dim3(256), 0, s, n, k, kp, a, (const double*)z, w,
+                                bs->partials);
     }
     RAMD_HIP(hipGetLastError());
     hipLaunchKernelGGL(bcol_final, dim3(1), dim3(64), 0, s, g, k, bs->partials, out1);
@@ -726,7 +749,8 @@ static int fpcg_block_run(ramd_hier_s* h, ramd_csr_s* A, int k, double* X_d, con
     const int64_t m = (int64_t)n * k;
     const BScal S = bs->S;
     const dim3 ge(std::max<unsigned>(1, (unsigned)std::min<int64_t>((m + 255) / 256, 65536)));
-    const int gc = std::max(1, std::min(kColBlocks, (n + 3) / 4));
+    const int kp = pow2_at_least(k);
+    const int gc = col_blocks(n, kp);
     RAMD_HIP(hipMemcpyAsync(bs->X, X_d, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
     RAMD_HIP(hipMemcpyAsync(bs->B, B_d, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(bfill, dim3(4), dim3(256), 0, s, (int64_t)16 * kMaxRhs, 0.0, bs->scal);
@@ -759,7 +783,7 @@ static int fpcg_block_run(ramd_hier_s* h, ramd_csr_s* A, int k, double* X_d, con
             bouter(h, bs, A, OP_SPMV, bs->P, nullptr, bs->AP);
             col_dots(ctx, bs, n, 0, bs->P, bs->AP, false, nullptr, S.pap, nullptr);
             hipLaunchKernelGGL(bscal_alpha, dim3(1), dim3(64), 0, s, k, S);
-            hipLaunchKernelGGL(bupdate, dim3(gc), dim3(256), 0, s, n, k, S.alpha, bs->P, bs->AP, bs->X, bs->R,
+            hipLaunchKernelGGL(bupdate, dim3(gc), dim3(256), 0, s, n, k, kp, S.alpha, bs->P, bs->AP, bs->X, bs->R,
                                f32 ? bs->R32 : nullptr, bs->partials);
             hipLaunchKernelGGL(bcol_final, dim3(1), dim3(64), 0, s, gc, k, bs->partials, S.rr);
             hipLaunchKernelGGL(bscal_relres, dim3(1), dim3(64), 0, s, k, rtol, S);
@@ -770,8 +794,8 @@ static int fpcg_block_run(ramd_hier_s* h, ramd_csr_s* A, int k, double* X_d, con
             col_dots(ctx, bs, n, 1, bs->R, f32 ? (const void*)bs->Z32 : (const void*)bs->Z, f32, bs->AP, S.rzn,
                      S.zap);
             hipLaunchKernelGGL(bscal_beta, dim3(1), dim3(64), 0, s, k, S);
-            if (f32) hipLaunchKernelGGL((bdir<float>), dim3(gc), dim3(256), 0, s, n, k, S.beta, bs->Z32, bs->P);
-            else hipLaunchKernelGGL((bdir<double>), dim3(gc), dim3(256), 0, s, n, k, S.beta, bs->Z, bs->P);
+            if (f32) hipLaunchKernelGGL((bdir<float>), dim3(gc), dim3(256), 0, s, n, k, kp, S.beta, bs->Z32, bs->P);
+            else hipLaunchKernelGGL((bdir<double>), dim3(gc), dim3(256), 0, s, n, k, kp, S.beta, bs->Z, bs->P);
             RAMD_HIP(hipGetLastError());
         };
         const void* sig[4] = {A->val, A->col, A->rowptr, A->dia ? (const void*)A->dia->v64 : nullptr};
