@@ -226,6 +226,13 @@ int ramd_hier_states(ramd_hier h, int level, int32_t* states);
 int ramd_hier_row_caps(ramd_hier h, int32_t* caps);
 /* Host-seconds spent per setup phase: [strength, coarsen, interp, galerkin, other]. */
 int ramd_hier_setup_times(ramd_hier h, double* times5);
+/* Writes the hierarchy (options, every level's A in its stored order, P,
+ * the splitting or aggregates, the coarse LU factors and pivots) to a file,
+ * and rebuilds a device hierarchy from one without redoing the setup; the
+ * loaded hierarchy cycles bit-identically.  The file is this library's own
+ * (native byte order); the reference keeps its hierarchy in memory only. */
+int ramd_hier_save(ramd_hier h, const char* path);
+int ramd_hier_load(ramd_ctx ctx, const char* path, ramd_hier* out);
 /* Builds the FP32 copies of every level used by RAMD_PREC_FP32 cycles. */
 int ramd_hier_enable_fp32(ramd_hier h);
 
@@ -337,6 +344,14 @@ int ramd_host_interp(int32_t n, const int32_t* rowptr, const int32_t* col, const
  * diffusion on m^3 with log-normal cell coefficients (seed). */
 int ramd_host_gallery(int kind, int32_t m, double eps, double theta, uint64_t seed,
                       int32_t* n, int32_t** rowptr, int32_t** col, double** val, int64_t* nnz);
+/* readMatrix(filename), gallery/matrix_IO.cpp:24-110: a PETSc binary matrix
+ * file (big-endian as PETSc writes it; a little-endian file is detected by
+ * its class id).  Arrays are malloc'd; release them with ramd_host_free. */
+int ramd_host_read_petsc(const char* path, int32_t* n_rows, int32_t* n_cols, int32_t** rowptr,
+                         int32_t** col, double** val, int64_t* nnz);
+/* The matching writer (the reference has none): big-endian PETSc binary. */
+int ramd_host_write_petsc(const char* path, int32_t n_rows, int32_t n_cols, const int32_t* rowptr,
+                          const int32_t* col, const double* val);
 /* mis2(S, states, off_proc_states, false, weights), aggregation/par_mis.cpp:
  * 204-645.  weights NULL = the srand(2448422) stream.  states: 1 selected,
  * 0 not selected. */

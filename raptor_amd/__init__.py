@@ -24,6 +24,7 @@ libraptor_amd.so (include/raptor_amd.h):
 Every compute call runs the HIP library; nothing falls back to a CPU path.
 """
 import ctypes as C
+import os
 import weakref
 
 import numpy as np
@@ -35,7 +36,7 @@ from ._lib import (CLJP, HMIS, PMIS, RS, Classical, Direct, Extended, Falgout, J
 __all__ = [
     "Context", "ParVector", "ParCSRMatrix", "ParMultilevel", "ParRugeStubenSolver",
     "ParSmoothedAggregationSolver", "PCG", "CG", "FPCG", "FPCG_block", "vcycle_block", "block_tuning", "adjoint", "strength", "split", "interpolation", "mis2",
-    "aggregate", "fit_candidates", "jacobi_prolongation", "gallery_laplace27", "gallery_aniso2d", "gallery_fv7", "RamdError",
+    "aggregate", "fit_candidates", "jacobi_prolongation", "read_petsc", "write_petsc", "gallery_laplace27", "gallery_aniso2d", "gallery_fv7", "RamdError",
     "RS", "CLJP", "Falgout", "PMIS", "HMIS", "Direct", "ModClassical", "Extended", "Classical",
     "Symmetric", "Jacobi", "PREC_FP64", "PREC_FP32",
 ]
@@ -305,6 +306,25 @@ class ParMultilevel:
         s = np.empty(int(n[l]), np.int32)
         check(lib().ramd_hier_states(self.h, l, _ptr(s, C.c_int32)))
         return s
+
+    def save(self, path):
+        """Writes the hierarchy to `path` (ramd_hier_save)."""
+        check(lib().ramd_hier_save(self.h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, ctx, path):
+        """A hierarchy read back from `path` (ramd_hier_load); cycles and
+        solves exactly as the one that was saved."""
+        ml = cls.__new__(cls)
+        ml._init_common(ctx, 0.0, Classical, Jacobi)
+        h = C.c_void_p()
+        check(lib().ramd_hier_load(ctx.h, os.fsencode(path), C.byref(h)))
+        ml.h = h
+        ctx._adopt(ml)
+        nl = C.c_int()
+        check(lib().ramd_hier_info(ml.h, C.byref(nl), None, None, None))
+        ml.num_levels = nl.value
+        return ml
 
     def row_caps(self):
         """Row-block caps chosen by the autotuner: array (levels, 3, 2) for
@@ -583,3 +603,21 @@ def jacobi_prolongation(ctx, rowptr, col, val, t_rowptr, t_col, t_val, n_aggs, o
                                      _ptr(t_col, C.c_int32), _ptr(t_val, C.c_double), float(omega), int(steps),
                                      C.byref(prp), C.byref(pcol), C.byref(pval), C.byref(pnnz)))
     return _take(prp, n + 1, np.int32), _take(pcol, pnnz.value, np.int32), _take(pval, pnnz.value, np.float64)
+
+
+def read_petsc(path):
+    """readMatrix(filename) of gallery/matrix_IO.cpp: (rowptr, col, val, n_cols)."""
+    nr, nc, nnz = C.c_int32(), C.c_int32(), C.c_int64()
+    rp, col, val = C.POINTER(C.c_int32)(), C.POINTER(C.c_int32)(), C.POINTER(C.c_double)()
+    check(lib().ramd_host_read_petsc(os.fsencode(path), C.byref(nr), C.byref(nc), C.byref(rp), C.byref(col),
+                                     C.byref(val), C.byref(nnz)))
+    return (_take(rp, nr.value + 1, np.int32), _take(col, nnz.value, np.int32), _take(val, nnz.value, np.float64),
+            nc.value)
+
+
+def write_petsc(path, rowptr, col, val, n_cols=None):
+    """Big-endian PETSc binary matrix file."""
+    rowptr, col, val = _i32(rowptr), _i32(col), _f64(val)
+    n = rowptr.size - 1
+    check(lib().ramd_host_write_petsc(os.fsencode(path), n, n if n_cols is None else int(n_cols),
+                                      _ptr(rowptr, C.c_int32), _ptr(col, C.c_int32), _ptr(val, C.c_double)))

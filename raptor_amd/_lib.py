@@ -103,6 +103,10 @@ _SIGS = {
     "ramd_hier_states": (P, C.c_int, PI32),
     "ramd_hier_setup_times": (P, PD),
     "ramd_hier_row_caps": (P, PI32),
+    "ramd_hier_save": (P, C.c_char_p),
+    "ramd_hier_load": (P, C.c_char_p, C.POINTER(P)),
+    "ramd_host_read_petsc": (C.c_char_p, PI32, PI32, PPI32, PPI32, PPD, PI64),
+    "ramd_host_write_petsc": (C.c_char_p, I32, I32, PI32, PI32, PD),
     "ramd_hier_enable_fp32": (P,),
     "ramd_vcycle": (P, P, P, C.c_int),
     "ramd_vcycle_fp32": (P, P, P),

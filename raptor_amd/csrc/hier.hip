@@ -3,6 +3,7 @@
 // the coarse dense LU solve, stationary AMG, PCG with the reference's rules
 // and the mixed-precision flexible PCG.
 #include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -396,6 +397,44 @@ static Csr sa_level(ramd_hier_s* h, Csr& A, const double* weights, std::vector<i
     return P;
 }
 
+// Device state every hierarchy needs once its levels exist: the coarse LU
+// (column-major factors of A_c^T and pivots), the PCG work vectors, the
+// kernel tuning of every cycle operator and the ownership marks.
+static void finish_hierarchy(ramd_hier_s* h, int n, const std::vector<double>& lu, const std::vector<int>& piv)
+{
+    ramd_ctx ctx = h->ctx;
+    h->coarse_n = n;
+    h->lu = dalloc<double>((size_t)n * n);
+    h->piv = dalloc<int>((size_t)n);
+    h->cb = dalloc<double>((size_t)n);
+    h->cx = dalloc<double>((size_t)n);
+    if (n) {
+        RAMD_HIP(hipMemcpyAsync(h->lu, lu.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
+        RAMD_HIP(hipMemcpyAsync(h->piv, piv.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+    }
+    const int n0 = h->levels[0].n;
+    h->pr = dalloc<double>((size_t)n0);
+    h->pz = dalloc<double>((size_t)n0);
+    h->pp = dalloc<double>((size_t)n0);
+    h->pAp = dalloc<double>((size_t)n0);
+    // row-block caps of every CSR operator of the cycle and the x staging of
+    // the diagonal-form ones, FP64 vectors
+    for (Level& L : h->levels) {
+        if (&L != &h->levels.back()) {
+            csr_autotune(L.A, 0);
+            dia_autotune(L.A, 0);
+        }
+        if (L.P) {
+            csr_autotune(L.P, 0);
+            csr_autotune(L.R, 0);
+        }
+    }
+    RAMD_HIP(hipStreamSynchronize(ctx->stream));
+    for (Level& L : h->levels) L.A->borrowed = true;
+    for (Level& L : h->levels)
+        if (L.P) L.P->borrowed = true;
+}
+
 static void hier_setup(ramd_hier_s* h, Csr Af)
 {
     const ramd_amg_opts& o = h->opts;
@@ -441,36 +480,7 @@ static void hier_setup(ramd_hier_s* h, Csr Af)
     std::vector<int> piv;
     const int info = lu_factor_colmajor(n, dense, piv);
     RAMD_CHECK(info == 0, RAMD_E_NUMERIC, "coarse matrix (%d rows) is singular (info=%d)", n, info);
-    h->coarse_n = n;
-    h->lu = dalloc<double>((size_t)n * n);
-    h->piv = dalloc<int>((size_t)n);
-    h->cb = dalloc<double>((size_t)n);
-    h->cx = dalloc<double>((size_t)n);
-    if (n) {
-        RAMD_HIP(hipMemcpyAsync(h->lu, dense.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, ctx->stream));
-        RAMD_HIP(hipMemcpyAsync(h->piv, piv.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
-    }
-    const int n0 = h->levels[0].n;
-    h->pr = dalloc<double>((size_t)n0);
-    h->pz = dalloc<double>((size_t)n0);
-    h->pp = dalloc<double>((size_t)n0);
-    h->pAp = dalloc<double>((size_t)n0);
-    // row-block caps of every CSR operator of the cycle and the x staging of
-    // the diagonal-form ones, FP64 vectors
-    for (Level& L : h->levels) {
-        if (&L != &h->levels.back()) {
-            csr_autotune(L.A, 0);
-            dia_autotune(L.A, 0);
-        }
-        if (L.P) {
-            csr_autotune(L.P, 0);
-            csr_autotune(L.R, 0);
-        }
-    }
-    RAMD_HIP(hipStreamSynchronize(ctx->stream));
-    for (Level& L : h->levels) L.A->borrowed = true;
-    for (Level& L : h->levels)
-        if (L.P) L.P->borrowed = true;
+    finish_hierarchy(h, n, dense, piv);
     double total = now_s() - tstart;
     h->times[4] = total - h->times[0] - h->times[1] - h->times[2] - h->times[3];
 }
@@ -747,6 +757,186 @@ __global__ void adjoint_outer(int n, const int* __restrict__ rp, const int* __re
     for (int j = rp[row]; j < rp[row + 1]; j++) g[j] = -(l * x[col[j]]);
 }
 
+// ---------------------------------------------------------------------------
+// hierarchy files: every level's A (stored order), P, splitting/aggregates,
+// the coarse LU factors and the options, so a solve can skip the setup
+// ---------------------------------------------------------------------------
+static const char kHierMagic[8] = {'R', 'A', 'M', 'D', 'H', 'I', 'E', 'R'};
+constexpr int32_t kHierVersion = 1;
+
+struct HierFile {
+    FILE* f = nullptr;
+    const char* path = "";
+    ~HierFile()
+    {
+        if (f) std::fclose(f);
+    }
+    void put_bytes(const void* p, size_t n)
+    {
+        if (n && std::fwrite(p, 1, n, f) != n) fail(RAMD_E_ARG, "%s: write failed", path);
+    }
+    void get_bytes(void* p, size_t n)
+    {
+        if (n && std::fread(p, 1, n, f) != n) fail(RAMD_E_ARG, "%s: file ends early", path);
+    }
+    template <class T>
+    void put(T v)
+    {
+        put_bytes(&v, sizeof v);
+    }
+    template <class T>
+    T get()
+    {
+        T v;
+        get_bytes(&v, sizeof v);
+        return v;
+    }
+    template <class T>
+    void put_vec(const std::vector<T>& v)
+    {
+        put<int64_t>((int64_t)v.size());
+        put_bytes(v.data(), v.size() * sizeof(T));
+    }
+    template <class T>
+    std::vector<T> get_vec(int64_t max_n)
+    {
+        const int64_t n = get<int64_t>();
+        RAMD_CHECK(n >= 0 && n <= max_n, RAMD_E_ARG, "%s: corrupt array length %lld", path, (long long)n);
+        std::vector<T> v((size_t)n);
+        get_bytes(v.data(), (size_t)n * sizeof(T));
+        return v;
+    }
+    void put_csr(const Csr& M)
+    {
+        put<int32_t>(M.n_rows);
+        put<int32_t>(M.n_cols);
+        put_vec(M.rowptr);
+        put_vec(M.col);
+        put_vec(M.val);
+    }
+    Csr get_csr()
+    {
+        Csr M;
+        M.n_rows = get<int32_t>();
+        M.n_cols = get<int32_t>();
+        RAMD_CHECK(M.n_rows >= 0 && M.n_cols >= 0, RAMD_E_ARG, "%s: corrupt matrix header", path);
+        M.rowptr = get_vec<int>((int64_t)M.n_rows + 1);
+        M.col = get_vec<int>(INT64_MAX / 8);
+        M.val = get_vec<double>(INT64_MAX / 8);
+        RAMD_CHECK((int64_t)M.rowptr.size() == (int64_t)M.n_rows + 1 && M.rowptr[0] == 0 &&
+                       M.rowptr[M.n_rows] == (int)M.col.size() && M.col.size() == M.val.size(),
+                   RAMD_E_ARG, "%s: inconsistent matrix", path);
+        for (int i = 0; i < M.n_rows; i++)
+            RAMD_CHECK(M.rowptr[i + 1] >= M.rowptr[i], RAMD_E_ARG, "%s: corrupt row pointers", path);
+        for (int c : M.col) RAMD_CHECK(c >= 0 && c < M.n_cols, RAMD_E_ARG, "%s: column out of range", path);
+        return M;
+    }
+};
+
+static void hier_save(ramd_hier_s* h, const char* path)
+{
+    HierFile w;
+    w.path = path;
+    w.f = std::fopen(path, "wb");
+    RAMD_CHECK(w.f, RAMD_E_ARG, "%s: cannot create", path);
+    w.put_bytes(kHierMagic, 8);
+    w.put<int32_t>(kHierVersion);
+    const ramd_amg_opts& o = h->opts;
+    w.put<int32_t>(o.solver);
+    w.put<double>(o.strong_threshold);
+    w.put<int32_t>(o.coarsen_type);
+    w.put<int32_t>(o.interp_type);
+    w.put<int32_t>(o.strength_type);
+    w.put<double>(o.interp_filter);
+    w.put<double>(o.prolong_weight);
+    w.put<int32_t>(o.num_smooth_sweeps);
+    w.put<double>(o.relax_weight);
+    w.put<int32_t>(o.max_coarse);
+    w.put<int32_t>(o.max_levels);
+    w.put<int32_t>(o.prolong_smooth_steps);
+    for (double t : h->times) w.put<double>(t);
+    w.put<int32_t>((int32_t)h->levels.size());
+    for (Level& L : h->levels) {
+        w.put_csr(download(L.A));
+        w.put<int32_t>(L.P ? 1 : 0);
+        if (L.P) w.put_csr(download(L.P));
+        w.put_vec(L.states);
+    }
+    const int n = h->coarse_n;
+    std::vector<double> lu((size_t)n * n);
+    std::vector<int> piv((size_t)n);
+    if (n) {
+        RAMD_HIP(hipMemcpyAsync(lu.data(), h->lu, sizeof(double) * lu.size(), hipMemcpyDeviceToHost, h->ctx->stream));
+        RAMD_HIP(hipMemcpyAsync(piv.data(), h->piv, sizeof(int) * piv.size(), hipMemcpyDeviceToHost, h->ctx->stream));
+        RAMD_HIP(hipStreamSynchronize(h->ctx->stream));
+    }
+    w.put<int32_t>(n);
+    w.put_vec(lu);
+    w.put_vec(piv);
+    RAMD_CHECK(std::fflush(w.f) == 0, RAMD_E_ARG, "%s: write failed", path);
+}
+
+static void hier_load(ramd_hier_s* h, const char* path)
+{
+    HierFile r;
+    r.path = path;
+    r.f = std::fopen(path, "rb");
+    RAMD_CHECK(r.f, RAMD_E_ARG, "%s: cannot open", path);
+    char magic[8];
+    r.get_bytes(magic, 8);
+    RAMD_CHECK(std::memcmp(magic, kHierMagic, 8) == 0, RAMD_E_ARG, "%s: not a hierarchy file", path);
+    const int32_t version = r.get<int32_t>();
+    RAMD_CHECK(version == kHierVersion, RAMD_E_ARG, "%s: version %d (expected %d)", path, version, kHierVersion);
+    ramd_amg_opts& o = h->opts;
+    std::memset(&o, 0, sizeof o);
+    o.solver = r.get<int32_t>();
+    o.strong_threshold = r.get<double>();
+    o.coarsen_type = r.get<int32_t>();
+    o.interp_type = r.get<int32_t>();
+    o.strength_type = r.get<int32_t>();
+    o.interp_filter = r.get<double>();
+    o.prolong_weight = r.get<double>();
+    o.num_smooth_sweeps = r.get<int32_t>();
+    o.relax_weight = r.get<double>();
+    o.max_coarse = r.get<int32_t>();
+    o.max_levels = r.get<int32_t>();
+    o.prolong_smooth_steps = r.get<int32_t>();
+    o.weights = nullptr;
+    RAMD_CHECK(o.num_smooth_sweeps >= 0, RAMD_E_ARG, "%s: corrupt options", path);
+    for (double& t : h->times) t = r.get<double>();
+    const int32_t nl = r.get<int32_t>();
+    RAMD_CHECK(nl >= 1 && nl <= 1000, RAMD_E_ARG, "%s: %d levels", path, nl);
+    for (int l = 0; l < nl; l++) {
+        Csr A = r.get_csr();
+        RAMD_CHECK(A.n_rows == A.n_cols && (l == 0 || A.n_rows == h->levels.back().P->n_cols), RAMD_E_ARG,
+                   "%s: level %d has the wrong size", path, l);
+        h->levels.emplace_back();
+        Level& L = h->levels.back();
+        // only the fine level gets the stencil-diagonal form, as in the setup
+        L.A = upload(h->ctx, A, l == 0);
+        alloc_level_vectors(h, L);
+        if (r.get<int32_t>()) {
+            RAMD_CHECK(l + 1 < nl, RAMD_E_ARG, "%s: prolongator below the coarsest level", path);
+            Csr P = r.get_csr();
+            RAMD_CHECK(P.n_rows == A.n_rows, RAMD_E_ARG, "%s: level %d prolongator has the wrong size", path, l);
+            L.P = upload(h->ctx, P, false);
+            L.R = csr_transpose(L.P);
+        } else {
+            RAMD_CHECK(l + 1 == nl, RAMD_E_ARG, "%s: level %d has no prolongator", path, l);
+        }
+        std::vector<int> st = r.get_vec<int>(A.n_rows);
+        L.states = std::move(st);
+    }
+    const int32_t n = r.get<int32_t>();
+    RAMD_CHECK(n == h->levels.back().n && n <= kCoarseMax, RAMD_E_ARG, "%s: coarse size %d", path, n);
+    std::vector<double> lu = r.get_vec<double>((int64_t)n * n);
+    std::vector<int> piv = r.get_vec<int>(n);
+    RAMD_CHECK((int64_t)lu.size() == (int64_t)n * n && (int)piv.size() == n, RAMD_E_ARG, "%s: corrupt coarse LU",
+               path);
+    for (int p : piv) RAMD_CHECK(p >= 0 && p < n, RAMD_E_ARG, "%s: corrupt pivots", path);
+    finish_hierarchy(h, n, lu, piv);
+}
+
 }  // namespace ramd
 
 using namespace ramd;
@@ -855,6 +1045,27 @@ int ramd_sa_prolongation(ramd_ctx ctx, int32_t n, const int32_t* rowptr, const i
         out(P.col, p_col);
         out(P.val, p_val);
         *p_nnz = P.nnz();
+    });
+}
+
+int ramd_hier_save(ramd_hier h, const char* path)
+{
+    return guarded([&] {
+        RAMD_CHECK(h && path, RAMD_E_ARG, "ramd_hier_save: NULL argument");
+        h->ctx->set_current();
+        hier_save(h, path);
+    });
+}
+
+int ramd_hier_load(ramd_ctx ctx, const char* path, ramd_hier* out)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx && path && out, RAMD_E_ARG, "ramd_hier_load: NULL argument");
+        ctx->set_current();
+        std::unique_ptr<ramd_hier_s, void (*)(ramd_hier_s*)> h(new ramd_hier_s(), hier_free);
+        h->ctx = ctx;
+        hier_load(h.get(), path);
+        *out = h.release();
     });
 }
 

@@ -51,3 +51,32 @@ def test_torch_extension_loads():
               "print('EXT_OK')")
     r = subprocess.run([sys.executable, "-c", script, root], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and "EXT_OK" in r.stdout, r.stderr[-3000:]
+
+
+def test_petsc_matrix_round_trip(tmp_path, golden):
+    """write_petsc writes PETSc's big-endian binary layout (the format of the
+    reference's test_data/*.pm, gallery/matrix_IO.cpp) and read_petsc reads
+    it back, also from a little-endian file."""
+    import numpy as np
+    import checkers as K
+    A = K.fixture_csr(golden, "aniso", sort=False)
+    p = tmp_path / "a.pm"
+    ra.write_petsc(p, A.rowptr, A.col, A.val, A.n_cols)
+    raw = p.read_bytes()
+    head = np.frombuffer(raw, dtype=">i4", count=4)
+    assert list(head) == [1211216, A.n_rows, A.n_cols, A.col.size]
+    rp, col, val, nc = ra.read_petsc(p)
+    assert nc == A.n_cols and np.array_equal(rp, A.rowptr) and np.array_equal(col, A.col)
+    assert np.array_equal(val, A.val)
+    # the same matrix stored little-endian
+    lens = np.diff(A.rowptr).astype("<i4")
+    q = tmp_path / "le.pm"
+    q.write_bytes(np.array([1211216, A.n_rows, A.n_cols, A.col.size], "<i4").tobytes() + lens.tobytes()
+                  + A.col.astype("<i4").tobytes() + A.val.astype("<f8").tobytes())
+    rp2, col2, val2, nc2 = ra.read_petsc(q)
+    assert np.array_equal(rp2, rp) and np.array_equal(col2, col) and np.array_equal(val2, val)
+    with pytest.raises(ra.RamdError):
+        ra.read_petsc(tmp_path / "missing.pm")
+    q.write_bytes(raw[:40])
+    with pytest.raises(ra.RamdError):
+        ra.read_petsc(q)

@@ -279,3 +279,32 @@ def test_row_cap_autotuner_choices(problem):
     assert set(np.unique(caps)) <= {0, 2048, 4096, 8192}
     assert not np.any(caps[:, :, 0] == 8192)
     assert np.all(caps[-1, 1:] == 0)  # no P, R at the coarsest level
+
+
+def test_hierarchy_save_load_round_trip(problem, ctx, tmp_path):
+    """A saved and reloaded hierarchy has the same levels, splittings and
+    coarse factors, and cycles bit-identically."""
+    ml = problem["ml"]
+    path = tmp_path / "h.bin"
+    ml.save(path)
+    ml2 = type(ml).load(ctx, path)
+    assert ml2.num_levels == ml.num_levels
+    for l in range(ml.num_levels):
+        A1, P1 = ml.level(l)
+        A2, P2 = ml2.level(l)
+        for M1, M2 in ((A1, A2), (P1, P2)):
+            if M1 is None:
+                assert M2 is None
+                continue
+            a, b = M1.to_csr(), M2.to_csr()
+            assert all(np.array_equal(u, v) for u, v in zip(a, b))
+        if l < ml.num_levels - 1:
+            assert np.array_equal(ml.states(l), ml2.states(l))
+    n = problem["As"][0].n_rows
+    b = np.random.default_rng(31).standard_normal(n)
+    x1 = ra.ParVector.from_numpy(ctx, np.zeros(n))
+    x2 = ra.ParVector.from_numpy(ctx, np.zeros(n))
+    ml.cycle(x1, ra.ParVector.from_numpy(ctx, b), x_is_zero=True)
+    ml2.cycle(x2, ra.ParVector.from_numpy(ctx, b), x_is_zero=True)
+    assert np.array_equal(x1.numpy(), x2.numpy())
+    ml2.free()
