@@ -96,6 +96,12 @@ int ramd_csr_diagonals(ramd_csr A, int* nd);
  * -1 lets the autotuner time both at the next hierarchy setup or solve.
  * Both variants give identical bits.  No reference counterpart. */
 int ramd_csr_set_dia_staging(ramd_csr A, int pp, int mode);
+/* CSR kernel of the operator for FP64 (prec 0) or FP32 (prec 1) vectors:
+ * on = 1 builds and selects the sliced-row form (64 rows per slice, one
+ * lane per row), 0 returns to the row-block kernel.  Both give identical
+ * bits; the autotuner picks one per operator at hierarchy setup and at
+ * FP32 enable.  No reference counterpart. */
+int ramd_csr_set_sliced(ramd_csr A, int prec, int on);
 
 /* ------------------------------------------------------------------ */
 /* SpMV family (FP64; each row summed left to right in stored order, as  */
@@ -221,8 +227,9 @@ int ramd_hier_level(ramd_hier h, int level, ramd_csr* A, ramd_csr* P);
 int ramd_hier_states(ramd_hier h, int level, int32_t* states);
 /* Row-block caps (products per workgroup) the autotuner chose for the
  * CSR kernels: six entries per level, [A, P, R] x [FP64, FP32 vectors];
- * 0 for a stencil-diagonal operator or an absent one.  No reference
- * counterpart. */
+ * -1 where the sliced-row kernel (64 rows per slice, one lane per row) was
+ * measured faster, 0 for a stencil-diagonal operator or an absent one.  No
+ * reference counterpart. */
 int ramd_hier_row_caps(ramd_hier h, int32_t* caps);
 /* Host-seconds spent per setup phase: [strength, coarsen, interp, galerkin, other]. */
 int ramd_hier_setup_times(ramd_hier h, double* times5);

@@ -448,6 +448,71 @@ static void cycle_level(const orc_hier* h, int l, double* x, const double* b)
 
 void orc_cycle(const orc_hier* h, double* x, const double* b) { cycle_level(h, 0, x, b); }
 
+/* ---- single-precision V-cycle ----
+ * The reference has no single-precision cycle; this is cycle_level's
+ * operation sequence (multilevel/par_multilevel.hpp:341-457) with every
+ * matrix value rounded once to float and every vector, product and sum in
+ * float, one rounding per + and per *.  The coarsest system is solved in
+ * double on the widened right-hand side (dgetrs_ as above) and the solution
+ * rounded once to float.  Jacobi's diagonal test compares with 1e-16
+ * rounded to float. */
+static void cycle_level_f(const orc_hier* h, int l, float* x, const float* b)
+{
+    const orc_csr* A = &h->A[l];
+    if (l == h->num_levels - 1) {
+        int n = h->coarse_n;
+        double* bd = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+        for (int i = 0; i < n; i++) bd[i] = (double)b[i];
+        orc_lu_solve(n, h->coarse_lu, h->coarse_piv, bd);
+        for (int i = 0; i < n; i++) x[i] = (float)bd[i];
+        free(bd);
+        return;
+    }
+    const orc_csr* P = &h->P[l];
+    const int n = A->n_rows, nc = P->n_cols;
+    const float w = (float)h->relax_weight, tol = (float)ZERO_TOL;
+    float* tmp = (float*)malloc((size_t)(n > 0 ? n : 1) * sizeof(float));
+    float* xc = (float*)calloc((size_t)(nc > 0 ? nc : 1), sizeof(float));
+    float* bc = (float*)malloc((size_t)(nc > 0 ? nc : 1) * sizeof(float));
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) {
+            /* Pt r into bc, recursion, then x += P xc (row sum first) */
+            for (int i = 0; i < n; i++) {
+                float s = b[i];
+                for (int j = A->rowptr[i]; j < A->rowptr[i + 1]; j++) s -= (float)A->val[j] * x[A->col[j]];
+                tmp[i] = s;
+            }
+            for (int c = 0; c < nc; c++) bc[c] = 0.0f;
+            for (int i = 0; i < P->n_rows; i++)
+                for (int j = P->rowptr[i]; j < P->rowptr[i + 1]; j++) bc[P->col[j]] += (float)P->val[j] * tmp[i];
+            cycle_level_f(h, l + 1, xc, bc);
+            for (int i = 0; i < P->n_rows; i++) {
+                float s = 0.0f;
+                for (int j = P->rowptr[i]; j < P->rowptr[i + 1]; j++) s += (float)P->val[j] * xc[P->col[j]];
+                x[i] += s;
+            }
+        }
+        for (int it = 0; it < h->num_sweeps; it++) {
+            for (int i = 0; i < n; i++) tmp[i] = x[i];
+            for (int i = 0; i < n; i++) {
+                int s0 = A->rowptr[i], e0 = A->rowptr[i + 1];
+                if (s0 == e0) continue;
+                float row_sum = 0.0f, diag = 0.0f;
+                for (int j = s0; j < e0; j++) {
+                    if (A->col[j] == i) diag = (float)A->val[j];
+                    else row_sum += (float)A->val[j] * tmp[A->col[j]];
+                }
+                if (fabsf(diag) > tol) x[i] = ((1.0f - w) * tmp[i]) + (w * ((b[i] - row_sum) / diag));
+            }
+        }
+    }
+    free(tmp);
+    free(xc);
+    free(bc);
+}
+
+void orc_cycle_fp32(const orc_hier* h, float* x, const float* b) { cycle_level_f(h, 0, x, b); }
+
 /* ParMultilevel::solve multilevel/par_multilevel.hpp:459-538 */
 int orc_amg_solve(const orc_hier* h, double* x, const double* b, double solve_tol,
                   int max_iterations, double* residuals)

@@ -77,6 +77,8 @@ typedef struct {
 } orc_hier;
 
 void orc_cycle(const orc_hier* h, double* x, const double* b);
+/* the same cycle in single precision (matrix values rounded once to float) */
+void orc_cycle_fp32(const orc_hier* h, float* x, const float* b);
 int  orc_amg_solve(const orc_hier* h, double* x, const double* b, double solve_tol,
                    int max_iterations, double* residuals);
 int  orc_pcg(const orc_csr* A, const orc_hier* h, double* x, const double* b,

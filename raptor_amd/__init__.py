@@ -210,6 +210,11 @@ class ParCSRMatrix:
         2 FP32 values with FP64 vectors): 0 gather, 1 LDS-staged x, -1 autotune."""
         check(lib().ramd_csr_set_dia_staging(self.h, int(pp), int(mode)))
 
+    def set_sliced(self, prec, on):
+        """CSR kernel for FP64 (prec 0) or FP32 (prec 1) vectors: True selects
+        the sliced-row form, False the row-block kernel (identical bits)."""
+        check(lib().ramd_csr_set_sliced(self.h, int(prec), int(bool(on))))
+
     def set_values(self, vals):
         """New values (a device ParVector of nnz entries, stored order) for the same pattern."""
         check(lib().ramd_csr_set_values(self.h, vals.ptr))
@@ -343,6 +348,11 @@ class ParMultilevel:
 
     def cycle(self, x, b, x_is_zero=False):
         check(lib().ramd_vcycle(self.h, x.ptr, b.ptr, 1 if x_is_zero else 0))
+
+    def cycle_fp32(self, x, b):
+        """One V-cycle from x = 0 on the FP32 copies (float32 ParVectors)."""
+        assert x.dtype == np.float32 and b.dtype == np.float32
+        check(lib().ramd_vcycle_fp32(self.h, x.ptr, b.ptr))
 
     def solve(self, x, b):
         res = np.zeros(self.max_iterations + 1)

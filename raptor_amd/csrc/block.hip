@@ -213,7 +213,7 @@ constexpr int kColBlocks = 512;
 
 template <typename X>
 __global__ void bjacobi_from_zero(int n, int k, const X* __restrict__ d, const X* __restrict__ b, X* __restrict__ x,
-                                  X omega, const int* __restrict__ rp)
+                                  X omega)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)n * k) return;
@@ -221,7 +221,7 @@ __global__ void bjacobi_from_zero(int n, int k, const X* __restrict__ d, const X
     const X di = d[i];
     const X zero = X(0.0);
     X v = zero;
-    if (rp[i + 1] > rp[i] && fabs(di) > X(1e-16)) v = ((X(1.0) - omega) * zero) + (omega * ((b[e] - zero) / di));
+    if (fabs(di) > X(1e-16)) v = ((X(1.0) - omega) * zero) + (omega * ((b[e] - zero) / di));  // d = 0 without a diagonal
     x[e] = v;
 }
 
@@ -239,6 +239,17 @@ __global__ void bfill(int64_t m, double v, double* __restrict__ a)
 }
 
 // dgetrs_('N') for column blockIdx.x of a row-interleaved block (see coarse_solve_kernel)
+// Column c of the block by one wavefront (n <= kCoarseWave).
+template <typename X>
+__global__ __launch_bounds__(64) void bcoarse_wave(int n, int k, const double* __restrict__ lu,
+                                                   const int* __restrict__ piv, const X* __restrict__ b,
+                                                   X* __restrict__ x)
+{
+    extern __shared__ double ls[];
+    const int c = blockIdx.x;
+    coarse_solve_wave<X, X>(n, lu, piv, b + c, x + c, (size_t)k, ls);
+}
+
 template <typename X>
 __global__ __launch_bounds__(256) void bcoarse_solve(int n, int k, const double* __restrict__ lu,
                                                      const int* __restrict__ piv, const X* __restrict__ b,
@@ -623,7 +634,10 @@ static void bcycle(ramd_hier_s* h, BlockState* bs, int l, X* x, const X* b)
     hipStream_t s = h->ctx->stream;
     if (l == (int)h->levels.size() - 1) {
         const int n = h->coarse_n;
-        if (n)
+        if (n && n <= kCoarseWave)
+            hipLaunchKernelGGL((bcoarse_wave<X>), dim3(k), dim3(64), sizeof(double) * n * n, s, n, k, h->lu, h->piv,
+                               b, x);
+        else if (n)
             hipLaunchKernelGGL((bcoarse_solve<X>), dim3(k), dim3(256), sizeof(double) * n, s, n, k, h->lu, h->piv, b,
                                x);
         RAMD_HIP(hipGetLastError());
@@ -643,7 +657,7 @@ static void bcycle(ramd_hier_s* h, BlockState* bs, int l, X* x, const X* b)
     for (int it = 0; it < sweeps; it++) {
         if (it == 0) {
             hipLaunchKernelGGL((bjacobi_from_zero<X>), grid1((int64_t)L.n * k), dim3(256), 0, s, L.n, k, diag, b, cur,
-                               w, L.A->rowptr);
+                               w);
             RAMD_HIP(hipGetLastError());
         } else {
             blaunch<X>(L.A, BL.lpr_A, OP_JACOBI, k, cur, b, other, w, s);

@@ -34,6 +34,10 @@ T* dalloc(size_t n)
 
 // Launch geometry of the row-block ("CSR-stream") kernels.
 constexpr int kBlockThreads = 256;   // one workgroup = 4 wavefronts of 64
+// the sliced-row form is tried for square operators averaging at least this
+// many entries per row, when padding adds at most half the entries again
+constexpr double kSlicedMinRowLen = 12.0;
+constexpr double kSlicedMaxPad = 1.5;
 constexpr int kStreamCap = 4096;     // products staged in LDS per workgroup
 constexpr int kMaxRowsPerBlock = 256;
 
@@ -140,9 +144,27 @@ struct Dia {
     bool v32_exact = false;     // every value is exactly representable in FP32
 };
 
+namespace ramd {
+// Sliced rows: 64 consecutive rows per slice (one wavefront, one lane per
+// row), the slice's entries stored column-major (entry k of lane l at
+// sptr[q] + 64k + l) and padded to its longest row with column -1.  The
+// one-lane-per-row form of operators with long rows: every load of a
+// wavefront is one contiguous run.
+struct Sliced {
+    int n_slices = 0;
+    int64_t entries = 0;
+    int64_t* sptr = nullptr;  // device, n_slices + 1
+    int* col = nullptr;       // device, entries
+    double* v64 = nullptr;    // device, entries (built on demand)
+    float* v32 = nullptr;
+};
+}  // namespace ramd
+
 struct ramd_csr_s {
     ramd_ctx ctx = nullptr;
     Dia* dia = nullptr;         // stencil-diagonal form (fine-level operators)
+    ramd::Sliced* sl = nullptr; // sliced-row form (long-row operators)
+    bool sl_on[2] = {false, false};  // per vector precision: use the sliced form
     int32_t n_rows = 0, n_cols = 0;
     int64_t nnz = 0;
     int* rowptr = nullptr;   // device, n_rows+1
@@ -176,6 +198,11 @@ namespace ramd {
 ramd_csr_s* csr_wrap(ramd_ctx ctx, int n_rows, int n_cols, int64_t nnz, int* rowptr_d, int* col_d,
                      double* val_d, const int* rowptr_host);
 void csr_free(ramd_csr_s* A);
+// Sliced-row form: pattern (sliced_build), values of one precision copied in
+// from A->val / A->val32 (sliced_values), release (sliced_free).
+void sliced_build(ramd_csr_s* A);
+void sliced_values(ramd_csr_s* A, int prec);
+void sliced_free(ramd_csr_s* A);
 // True when every row has its diagonal first or no diagonal at all.
 bool host_diag_first(int n_rows, const int* rowptr, const int* col);
 // Builds A->dia from host arrays when the matrix qualifies (square, at least

@@ -227,9 +227,108 @@ bool host_diag_first(int n_rows, const int* rp, const int* col)
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Sliced-row form
+// ---------------------------------------------------------------------------
+__global__ void fill_int_kernel(int* __restrict__ p, int64_t n, int v)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+template <typename T>
+__global__ void fill_zero_kernel(T* __restrict__ p, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = T(0);
+}
+
+// entries of row r, in stored order, to lane r % 64 of slice r / 64; the
+// padding keeps what the fill wrote
+template <typename T>
+__global__ void sliced_scatter_kernel(int n, const int* __restrict__ rp, const int64_t* __restrict__ sptr,
+                                      const T* __restrict__ in, T* __restrict__ out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t base = sptr[r >> 6] + (r & 63);
+    const int s = rp[r], e = rp[r + 1];
+    for (int k = s; k < e; k++) out[base + (int64_t)64 * (k - s)] = in[k];
+}
+
+void sliced_free(ramd_csr_s* A)
+{
+    if (!A->sl) return;
+    dfree(A->sl->sptr);
+    dfree(A->sl->col);
+    dfree(A->sl->v64);
+    dfree(A->sl->v32);
+    delete A->sl;
+    A->sl = nullptr;
+    A->sl_on[0] = A->sl_on[1] = false;
+}
+
+void sliced_build(ramd_csr_s* A)
+{
+    if (A->sl) return;
+    const int n = A->n_rows;
+    const int nq = (n + 63) / 64;
+    const int* rp = A->h_rowptr.data();
+    std::vector<int64_t> sptr((size_t)nq + 1, 0);
+    for (int q = 0; q < nq; q++) {
+        int w = 0;
+        for (int r = 64 * q; r < std::min(n, 64 * q + 64); r++) w = std::max(w, rp[r + 1] - rp[r]);
+        sptr[q + 1] = sptr[q] + (int64_t)64 * w;
+    }
+    hipStream_t s = A->ctx->stream;
+    Sliced* S = new Sliced;
+    S->n_slices = nq;
+    S->entries = sptr[nq];
+    A->sl = S;
+    S->sptr = dalloc<int64_t>(sptr.size());
+    S->col = dalloc<int>((size_t)std::max<int64_t>(S->entries, 1));
+    RAMD_HIP(hipMemcpyAsync(S->sptr, sptr.data(), sizeof(int64_t) * sptr.size(), hipMemcpyHostToDevice, s));
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>((S->entries + 255) / 256, 8192));
+    hipLaunchKernelGGL(fill_int_kernel, dim3(g), dim3(256), 0, s, S->col, S->entries, -1);
+    if (n)
+        hipLaunchKernelGGL((sliced_scatter_kernel<int>), dim3((n + 255) / 256), dim3(256), 0, s, n, A->rowptr,
+                           S->sptr, A->col, S->col);
+    RAMD_HIP(hipGetLastError());
+    RAMD_HIP(hipStreamSynchronize(s));  // sptr's host copy goes out of scope
+}
+
+template <typename T>
+static void sliced_copy(ramd_csr_s* A, const T* in, T* out)
+{
+    hipStream_t s = A->ctx->stream;
+    const Sliced* S = A->sl;
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>((S->entries + 255) / 256, 8192));
+    hipLaunchKernelGGL(fill_zero_kernel<T>, dim3(g), dim3(256), 0, s, out, S->entries);
+    if (A->n_rows)
+        hipLaunchKernelGGL((sliced_scatter_kernel<T>), dim3((A->n_rows + 255) / 256), dim3(256), 0, s, A->n_rows,
+                           A->rowptr, S->sptr, in, out);
+    RAMD_HIP(hipGetLastError());
+}
+
+void sliced_values(ramd_csr_s* A, int prec)
+{
+    RAMD_CHECK(A->sl, RAMD_E_STATE, "sliced form not built");
+    Sliced* S = A->sl;
+    const size_t m = (size_t)std::max<int64_t>(S->entries, 1);
+    if (prec == 0) {
+        if (!S->v64) S->v64 = dalloc<double>(m);
+        sliced_copy<double>(A, A->val, S->v64);
+    } else {
+        RAMD_CHECK(A->val32, RAMD_E_STATE, "FP32 values of the matrix were not built");
+        if (!S->v32) S->v32 = dalloc<float>(m);
+        sliced_copy<float>(A, A->val32, S->v32);
+    }
+}
+
 void csr_free(ramd_csr_s* A)
 {
     if (!A) return;
+    sliced_free(A);
     if (A->trans) csr_free(A->trans);
     dfree(A->rowptr);
     dfree(A->col);
@@ -348,6 +447,10 @@ void csr_set_values(ramd_csr_s* A, const double* val_d)
         int grid = (int)std::min<int64_t>((A->nnz + 255) / 256, 8192);
         hipLaunchKernelGGL(to_fp32_kernel, dim3(grid), dim3(256), 0, s, A->val, A->val32, A->nnz);
         RAMD_HIP(hipGetLastError());
+    }
+    if (A->sl) {
+        if (A->sl->v64) sliced_values(A, 0);
+        if (A->sl->v32) sliced_values(A, 1);
     }
     if (Dia* d = A->dia) {
         const int n = A->n_rows;
@@ -573,6 +676,24 @@ int ramd_csr_set_dia_staging(ramd_csr A, int pp, int mode)
         RAMD_CHECK(mode != 1 || A->dia->window.win <= kDiaWin, RAMD_E_ARG,
                    "ramd_csr_set_dia_staging: x window of %d values exceeds %d", A->dia->window.win, kDiaWin);
         A->dia->lds_x[pp] = mode;
+    });
+}
+
+int ramd_csr_set_sliced(ramd_csr A, int prec, int on)
+{
+    return guarded([&] {
+        RAMD_CHECK(A && (prec == 0 || prec == 1) && (on == 0 || on == 1), RAMD_E_ARG,
+                   "ramd_csr_set_sliced: bad argument");
+        RAMD_CHECK(!A->dia, RAMD_E_STATE, "ramd_csr_set_sliced: matrix uses its stencil-diagonal form");
+        A->ctx->set_current();
+        if (on) {
+            RAMD_CHECK(prec == 0 || A->val32, RAMD_E_STATE, "ramd_csr_set_sliced: FP32 values were not built");
+            RAMD_CHECK(A->nnz > 0, RAMD_E_ARG, "ramd_csr_set_sliced: matrix has no entries");
+            sliced_build(A);
+            sliced_values(A, prec);
+            RAMD_HIP(hipStreamSynchronize(A->ctx->stream));
+        }
+        A->sl_on[prec] = on != 0;
     });
 }
 

@@ -144,15 +144,14 @@ __global__ void extract_diag(int n, const int* __restrict__ rp, const int* __res
 // off-diagonal sum of zeros equal to +0): x = (1-w)*0 + w*((b - 0)/d).
 // Rows without a usable diagonal (|d| <= 1e-16, or empty rows) keep x = 0.
 template <typename X>
-__global__ void jacobi_from_zero(int n, const X* __restrict__ d, const X* __restrict__ b, X* __restrict__ x, X omega,
-                                 const int* __restrict__ rp)
+__global__ void jacobi_from_zero(int n, const X* __restrict__ d, const X* __restrict__ b, X* __restrict__ x, X omega)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const X di = d[i];
+    const X di = d[i];  // 0 for a row without a stored diagonal (empty rows included)
     const X zero = X(0.0);
     X v = zero;
-    if (rp[i + 1] > rp[i] && fabs(di) > X(1e-16)) v = ((X(1.0) - omega) * zero) + (omega * ((b[i] - zero) / di));
+    if (fabs(di) > X(1e-16)) v = ((X(1.0) - omega) * zero) + (omega * ((b[i] - zero) / di));
     x[i] = v;
 }
 
@@ -178,11 +177,24 @@ __global__ void to_f64(int64_t n, const float* __restrict__ a, double* __restric
 // ---------------------------------------------------------------------------
 // V-cycle
 // ---------------------------------------------------------------------------
+template <typename TB, typename TX>
+__global__ __launch_bounds__(64) void coarse_wave_kernel(int n, const double* __restrict__ lu,
+                                                         const int* __restrict__ piv, const TB* __restrict__ b,
+                                                         TX* __restrict__ x)
+{
+    extern __shared__ double ls[];
+    coarse_solve_wave<TB, TX>(n, lu, piv, b, x, 1, ls);
+}
+
 static void coarse_solve64(ramd_hier_s* h, const double* b, double* x)
 {
     const int n = h->coarse_n;
-    hipLaunchKernelGGL(coarse_solve_kernel, dim3(1), dim3(256), sizeof(double) * n, h->ctx->stream, n, h->lu, h->piv,
-                       b, x);
+    if (n <= kCoarseWave)
+        hipLaunchKernelGGL((coarse_wave_kernel<double, double>), dim3(1), dim3(64), sizeof(double) * n * n,
+                           h->ctx->stream, n, h->lu, h->piv, b, x);
+    else
+        hipLaunchKernelGGL(coarse_solve_kernel, dim3(1), dim3(256), sizeof(double) * n, h->ctx->stream, n, h->lu,
+                           h->piv, b, x);
     RAMD_HIP(hipGetLastError());
 }
 
@@ -201,8 +213,7 @@ static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_ze
     double* other = L.t;
     for (int it = 0; it < sweeps; it++) {
         if (it == 0 && x_zero) {
-            hipLaunchKernelGGL((jacobi_from_zero<double>), grid1(L.n), dim3(256), 0, s, L.n, L.diag, b, cur, w,
-                               L.A->rowptr);
+            hipLaunchKernelGGL((jacobi_from_zero<double>), grid1(L.n), dim3(256), 0, s, L.n, L.diag, b, cur, w);
             RAMD_HIP(hipGetLastError());
         } else {
             launch_stream_op(L.A, OP_JACOBI, cur, b, other, w, s, L.diag);
@@ -229,9 +240,15 @@ static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero
     hipStream_t s = h->ctx->stream;
     if (l == (int)h->levels.size() - 1) {
         const int n = h->coarse_n;
-        hipLaunchKernelGGL(to_f64, grid1(n), dim3(256), 0, s, (int64_t)n, b, h->cb);
-        coarse_solve64(h, h->cb, h->cx);
-        hipLaunchKernelGGL(to_f32, grid1(n), dim3(256), 0, s, (int64_t)n, h->cx, x);
+        if (n <= kCoarseWave) {
+            // FP64 solve of the widened right-hand side, rounded once on output
+            hipLaunchKernelGGL((coarse_wave_kernel<float, float>), dim3(1), dim3(64), sizeof(double) * n * n, s, n,
+                               h->lu, h->piv, b, x);
+        } else {
+            hipLaunchKernelGGL(to_f64, grid1(n), dim3(256), 0, s, (int64_t)n, b, h->cb);
+            coarse_solve64(h, h->cb, h->cx);
+            hipLaunchKernelGGL(to_f32, grid1(n), dim3(256), 0, s, (int64_t)n, h->cx, x);
+        }
         RAMD_HIP(hipGetLastError());
         return;
     }
@@ -241,8 +258,7 @@ static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero
     float* other = L.t32;
     for (int it = 0; it < sweeps; it++) {
         if (it == 0 && x_zero) {
-            hipLaunchKernelGGL((jacobi_from_zero<float>), grid1(L.n), dim3(256), 0, s, L.n, L.diag32, b, cur, w,
-                               L.A->rowptr);
+            hipLaunchKernelGGL((jacobi_from_zero<float>), grid1(L.n), dim3(256), 0, s, L.n, L.diag32, b, cur, w);
             RAMD_HIP(hipGetLastError());
         } else {
             launch_stream_op32(L.A, OP_JACOBI, cur, b, other, w, s, L.diag32);
@@ -1118,7 +1134,8 @@ int ramd_hier_row_caps(ramd_hier h, int32_t* caps)
             const ramd_csr_s* M[3] = {L.A, L.P, L.R};
             for (int m = 0; m < 3; m++)
                 for (int p = 0; p < 2; p++)
-                    caps[6 * l + 2 * m + p] = (M[m] && !M[m]->dia) ? M[m]->rb[p].cap : 0;
+                    caps[6 * l + 2 * m + p] =
+                        (M[m] && !M[m]->dia) ? (M[m]->sl_on[p] ? -1 : M[m]->rb[p].cap) : 0;
         }
     });
 }

@@ -23,6 +23,45 @@ struct Level {
     int n = 0;
 };
 
+// Largest coarse system solved by one wavefront (LU staged in LDS).
+constexpr int kCoarseWave = 64;
+
+// dgetrs_('N') on one right-hand side by one wavefront, n <= kCoarseWave:
+// lane i holds b_i; the row swaps, the unit-lower and the upper substitution
+// run in the reference BLAS column order with the same operations per entry
+// as the workgroup kernel (b_i -= b_j * l_ij; b_j /= u_jj), so the result is
+// bit-identical.  b and x are read and written with element stride `stride`;
+// `ls` is n*n doubles of LDS.  All 64 lanes must call it.
+template <typename TB, typename TX>
+__device__ __forceinline__ void coarse_solve_wave(int n, const double* __restrict__ lu, const int* __restrict__ piv,
+                                                  const TB* __restrict__ b, TX* __restrict__ x, size_t stride,
+                                                  double* ls)
+{
+    const int lane = threadIdx.x;
+    for (int t = lane; t < n * n; t += 64) ls[t] = lu[t];
+    const int pv = lane < n ? piv[lane] : 0;
+    double v = lane < n ? (double)b[(size_t)lane * stride] : 0.0;
+    __syncthreads();
+    for (int j = 0; j < n; j++) {
+        const int p = __shfl(pv, j);
+        if (p != j) {
+            const double vj = __shfl(v, j), vp = __shfl(v, p);
+            if (lane == j) v = vp;
+            else if (lane == p) v = vj;
+        }
+    }
+    for (int j = 0; j < n; j++) {
+        const double bj = __shfl(v, j);
+        if (lane > j && lane < n) v -= bj * ls[lane + j * n];
+    }
+    for (int j = n - 1; j >= 0; j--) {
+        if (lane == j) v /= ls[j + j * n];
+        const double bj = __shfl(v, j);
+        if (lane < j) v -= bj * ls[lane + j * n];
+    }
+    if (lane < n) x[(size_t)lane * stride] = (TX)v;
+}
+
 struct BlockState;  // multi-vector work space (block.hip)
 void block_free(BlockState* b);
 

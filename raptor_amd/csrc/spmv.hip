@@ -153,6 +153,88 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
     }
 }
 
+// Sliced-row kernels: one lane per row over the sliced form; a lane adds its
+// row's entries in stored order with the same operations as phase 2 of
+// stream_kernel, so the bits are the same.  Every wavefront reads its
+// slice's columns and values as contiguous runs, eight entries in flight per
+// lane; padding (column -1) is loaded and dropped by a select.
+template <int OP, typename V, typename X>
+__global__ __launch_bounds__(kBlockThreads) void sliced_kernel(
+    int n, const int64_t* __restrict__ sptr, const int* __restrict__ col, const V* __restrict__ val,
+    const X* __restrict__ x, const X* __restrict__ b, X* __restrict__ y, X omega, int diag_first,
+    double* __restrict__ dotp, const X* __restrict__ dvec)
+{
+    constexpr int B = 8;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int row = blk * kBlockThreads + threadIdx.x;
+    const bool valid = row < n;
+    const int lane = threadIdx.x & 63;
+    // lanes past n belong to no slice when n is not a multiple of 64 * 4
+    const int64_t base = valid ? sptr[row >> 6] + lane : 0;
+    const int w = valid ? (int)((sptr[(row >> 6) + 1] - sptr[row >> 6]) >> 6) : 0;
+    X acc = (OP == OP_RESIDUAL && valid) ? b[row] : X(0);
+    X diag = 0;
+    // Jacobi: first entry of the row to add (w: none -- the row keeps x)
+    int k_first = 0;
+    if (OP == OP_JACOBI && valid && diag_first) {
+        if (dvec) {
+            const X dg = dvec[row];
+            if (fabs(dg) > X(1e-16)) {
+                diag = dg;
+                k_first = 1;
+            } else {
+                k_first = w;
+            }
+        } else if (w > 0 && col[base] == row) {
+            diag = X(val[base]);
+            k_first = 1;
+        } else {
+            k_first = w;
+        }
+    }
+    for (int k0 = 0; k0 < w; k0 += B) {
+        int cc[B];
+        V vv[B];
+        X xx[B];
+#pragma unroll
+        for (int t = 0; t < B; t++) {
+            const bool in = k0 + t < w;
+            cc[t] = in ? col[base + (int64_t)64 * (k0 + t)] : -1;
+            vv[t] = in ? val[base + (int64_t)64 * (k0 + t)] : V(0);
+        }
+#pragma unroll
+        for (int t = 0; t < B; t++) xx[t] = x[cc[t] < 0 ? 0 : cc[t]];
+#pragma unroll
+        for (int t = 0; t < B; t++) {
+            const X p = X(vv[t]) * xx[t];
+            const bool use = cc[t] >= 0 && k0 + t >= k_first;
+            if (OP == OP_JACOBI && !diag_first) {
+                if (use && cc[t] == row) diag = X(vv[t]);
+                else acc = use ? acc + p : acc;
+            } else if (OP == OP_RESIDUAL) {
+                acc = use ? acc - p : acc;
+            } else {
+                acc = use ? acc + p : acc;
+            }
+        }
+    }
+    if (OP == OP_SPMV && dotp) {
+        const double v = block_sum(valid ? (double)x[row] * (double)acc : 0.0);
+        if (threadIdx.x == 0) dotp[blk] = v;
+    }
+    if (!valid) return;
+    if (OP == OP_SPMV || OP == OP_RESIDUAL) {
+        y[row] = acc;
+    } else if (OP == OP_APPEND) {
+        y[row] += acc;
+    } else if (OP == OP_ADD_INTO) {
+        y[row] = b[row] + acc;
+    } else {
+        const X xo = x[row];
+        y[row] = (fabs(diag) > X(1e-16)) ? ((X(1.0) - omega) * xo) + (omega * ((b[row] - acc) / diag)) : xo;
+    }
+}
+
 // Stencil-diagonal kernels: one lane per row, diagonals visited in the
 // row's stored order (a.ord).  Every load is issued unconditionally -- a
 // diagonal absent from the row reads its own plane (zero) and x[row] -- and
@@ -446,6 +528,33 @@ static void launch_stream_cap(const ramd_csr_s* A, const ramd_csr_s::RowBlocks& 
     }
 }
 
+template <typename V, typename X>
+static void launch_sliced(const ramd_csr_s* A, int op, const X* x, const X* b, X* y, X omega, hipStream_t s,
+                          double* dotp, const X* dvec)
+{
+    const Sliced* S = A->sl;
+    const V* val;
+    if constexpr (sizeof(V) == 4) val = S->v32;
+    else val = S->v64;
+    RAMD_CHECK(val, RAMD_E_STATE, "sliced values of this precision were not built");
+    const int n = A->n_rows;
+    const dim3 grid((n + kBlockThreads - 1) / kBlockThreads), block(kBlockThreads);
+    const int df = (int)A->diag_first;
+#define SLICED(OPV, DOT, DV)                                                                                     \
+    hipLaunchKernelGGL((sliced_kernel<OPV, V, X>), grid, block, 0, s, n, S->sptr, S->col, val, x, b, y, omega, df, \
+                       DOT, DV)
+    switch (op) {
+    case OP_SPMV: SLICED(OP_SPMV, dotp, nullptr); break;
+    case OP_APPEND: SLICED(OP_APPEND, nullptr, nullptr); break;
+    case OP_RESIDUAL: SLICED(OP_RESIDUAL, nullptr, nullptr); break;
+    case OP_JACOBI: SLICED(OP_JACOBI, nullptr, dvec); break;
+    case OP_ADD_INTO: SLICED(OP_ADD_INTO, nullptr, nullptr); break;
+    default: fail(RAMD_E_ARG, "unknown stream op %d", op);
+    }
+#undef SLICED
+    RAMD_HIP(hipGetLastError());
+}
+
 // val: CSR values; dval: stencil-diagonal values (nullptr -> CSR kernel)
 template <typename V, typename X>
 static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int op, const X* x, const X* b, X* y,
@@ -466,6 +575,14 @@ static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int 
     }
     if (dval) {
         launch_dia(A, dval, op, x, b, y, omega, s, dotp);
+        if (prof) {
+            RAMD_HIP(hipEventRecord(pend.end, s));
+            ctx->prof_pending.push_back(pend);
+        }
+        return;
+    }
+    if (A->sl && A->sl_on[sizeof(X) == 4 ? 1 : 0]) {
+        launch_sliced<V, X>(A, op, x, b, y, omega, s, dotp, dvec);
         if (prof) {
             RAMD_HIP(hipEventRecord(pend.end, s));
             ctx->prof_pending.push_back(pend);
@@ -538,6 +655,7 @@ void csr_autotune(ramd_csr_s* A, int prec)
         dfree(A->rb[prec].d);
         A->rb[prec] = {kStreamCap, A->n_blocks, A->blocks};
     }
+    A->sl_on[prec] = false;
     const size_t w = prec ? sizeof(float) : sizeof(double);
     struct Scratch {
         void *x = nullptr, *b = nullptr, *y = nullptr;
@@ -594,12 +712,45 @@ void csr_autotune(ramd_csr_s* A, int prec)
             best = rb;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     // keep the winner's blocks, release the others
     for (int*& p : t.blocks)
         if (p == best.d) p = nullptr;
     A->rb[prec] = best;
+    // square operators with long rows: the sliced form against the best cap
+    if (A->n_rows == A->n_cols && (double)A->nnz >= kSlicedMinRowLen * (double)A->n_rows) {
+        sliced_build(A);
+        if ((double)A->sl->entries <= kSlicedMaxPad * (double)A->nnz) {
+            sliced_values(A, prec);
+            A->sl_on[prec] = true;
+            auto run = [&] {
+                if (prec) launch_stream<float, float>(A, A->val32, nullptr, op, (const float*)t.x, (const float*)t.b,
+                                                      (float*)t.y, 0.f, s);
+                else launch_stream<double, double>(A, A->val, nullptr, op, (const double*)t.x, (const double*)t.b,
+                                                   (double*)t.y, 0.0, s);
+            };
+            run();
+            RAMD_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < 3; r++) run();
+            RAMD_HIP(hipEventRecord(e1, s));
+            RAMD_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            RAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+            A->sl_on[prec] = ms < best_ms;
+        }
+        if (!A->sl_on[prec]) {  // release what this precision built
+            RAMD_HIP(hipStreamSynchronize(s));
+            if (prec) {
+                dfree(A->sl->v32);
+                A->sl->v32 = nullptr;
+            } else {
+                dfree(A->sl->v64);
+                A->sl->v64 = nullptr;
+            }
+            if (!A->sl_on[0] && !A->sl_on[1]) sliced_free(A);
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
 }
 
 void dia_autotune(ramd_csr_s* A, int pp)
@@ -672,7 +823,7 @@ void launch_spmv_dot(const ramd_csr_s* A, bool values_fp32, const double* x, dou
 {
     ramd_ctx ctx = A->ctx;
     const bool dia32 = values_fp32 && A->dia && A->dia->v32 && A->dia->v32_exact;
-    const int np = A->dia ? (A->n_rows + kBlockThreads - 1) / kBlockThreads : A->rb[0].n;
+    const int np = (A->dia || (A->sl && A->sl_on[0])) ? (A->n_rows + kBlockThreads - 1) / kBlockThreads : A->rb[0].n;
     double* part = ctx_partials(ctx, (size_t)std::max(np, 1));
     if (dia32)
         launch_stream<float, double>(A, nullptr, A->dia->v32, OP_SPMV, x, nullptr, y, 0.0, s, part);

@@ -240,6 +240,13 @@ class Hier:
         orc().orc_cycle(C.byref(self.h), dp(x), dp(f64(b)))
         return x
 
+    def cycle_fp32(self, x, b):
+        x = np.ascontiguousarray(x, dtype=np.float32).copy()
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        orc().orc_cycle_fp32(C.byref(self.h), x.ctypes.data_as(C.POINTER(C.c_float)),
+                             b.ctypes.data_as(C.POINTER(C.c_float)))
+        return x
+
     def solve(self, x, b, tol=1e-7, max_iter=100):
         x = f64(x).copy()
         res = np.zeros(max_iter + 1)

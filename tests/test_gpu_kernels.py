@@ -65,6 +65,57 @@ def test_spmv_family_bit_exact(ctx, name, A):
     assert np.array_equal(t.numpy(), K.spmv_t(A, xr))
 
 
+def sliced_cases():
+    rng = np.random.default_rng(31)
+    sq = random_csr(rng, 900, 900, 20)
+    # every row gets a diagonal; a few are (near) zero, one row is emptied
+    rp, col, val = [0], [], []
+    for i in range(sq.n_rows):
+        c = list(sq.col[sq.rowptr[i]:sq.rowptr[i + 1]])
+        v = list(sq.val[sq.rowptr[i]:sq.rowptr[i + 1]])
+        if i == 17:
+            c, v = [], []
+        elif i not in c:
+            c.append(i)
+            v.append(0.0 if i % 97 == 5 else 30.0 + i % 7)
+        o = np.argsort(c)
+        col += [c[j] for j in o]
+        val += [v[j] for j in o]
+        rp.append(len(col))
+    S = K.Csr(rp, col, val, 900)
+    return [("square_sorted", S), ("square_diag_first", K.move_diag(S)),
+            ("rect", random_csr(rng, 700, 333, 9)),
+            ("long_rows", random_csr(rng, 300, 9000, 4, long_rows=(5, 77, 299))),
+            ("empty_rows", K.Csr([0, 0, 2, 2, 3], [1, 3, 0], [1.5, -2.0, 4.0], 4))]
+
+
+@pytest.mark.parametrize("name,A", sliced_cases(), ids=[c[0] for c in sliced_cases()])
+def test_sliced_kernel_bit_exact(ctx, name, A):
+    """The sliced-row kernel (64 rows per slice, one lane per row, padding
+    dropped by a select) gives the oracle's bits for every operation."""
+    rng = np.random.default_rng(32)
+    x, b = rng.standard_normal(A.n_cols), rng.standard_normal(A.n_rows)
+    D = to_dev(ctx, A)
+    D.set_sliced(0, True)
+    y = ra.ParVector(ctx, A.n_rows)
+    D.mult(dev(ctx, x), y)
+    assert np.array_equal(y.numpy(), K.spmv(A, x))
+    y2 = dev(ctx, b)
+    D.mult_append(dev(ctx, x), y2)
+    assert np.array_equal(y2.numpy(), K.spmv_append(A, x, b))
+    r = ra.ParVector(ctx, A.n_rows)
+    D.residual(dev(ctx, x), dev(ctx, b), r)
+    assert np.array_equal(r.numpy(), K.residual(A, x, b))
+    if A.n_rows == A.n_cols:
+        xj = dev(ctx, x)
+        D.jacobi(xj, dev(ctx, b), ra.ParVector(ctx, A.n_rows), 2, 2.0 / 3.0)
+        assert np.array_equal(xj.numpy(), K.jacobi(A, x, b, 2, 2.0 / 3.0))
+    D.set_sliced(0, False)
+    y3 = ra.ParVector(ctx, A.n_rows)
+    D.mult(dev(ctx, x), y3)
+    assert np.array_equal(y3.numpy(), y.numpy())
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("name,gen", [("aniso64", lambda: ra.gallery_aniso2d(64, 0.001, math.pi / 8)),
                                       ("lap27_20", lambda: ra.gallery_laplace27(20)),

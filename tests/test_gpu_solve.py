@@ -137,6 +137,99 @@ def test_vcycle_bit_exact(problem, ctx):
     assert np.array_equal(x.numpy(), H.cycle(x0, b))
 
 
+def test_vcycle_fp32_bit_exact(problem, ctx):
+    """FP32 cycle (the FPCG preconditioner at single precision) against the
+    oracle's single-precision cycle on the same hierarchy: bit-identical."""
+    ml, As, Ps = problem["ml"], problem["As"], problem["Ps"]
+    H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=OMEGA)
+    n = As[0].n_rows
+    b = np.random.default_rng(22).standard_normal(n).astype(np.float32)
+    x = ra.ParVector(ctx, n, np.float32)
+    bd = ra.ParVector.from_numpy(ctx, b)
+    ml.cycle_fp32(x, bd)
+    assert np.array_equal(x.numpy(), H.cycle_fp32(np.zeros(n, np.float32), b))
+
+
+@pytest.mark.parametrize("max_coarse", [400])
+def test_vcycle_large_coarse_system(ctx, max_coarse):
+    """A coarsest system larger than one wavefront takes the workgroup
+    solve; FP64 and FP32 cycles stay bit-identical to the oracle."""
+    rp, col, val = ra.gallery_aniso2d(60, 0.001, math.pi / 8)
+    ml = build(ctx, rp, col, val, rs(ra.PMIS, ra.Direct), max_coarse=max_coarse)
+    As, Ps = levels(ml)
+    assert As[-1].n_rows > 64
+    H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=OMEGA)
+    n = As[0].n_rows
+    b = np.random.default_rng(23).standard_normal(n)
+    x = ra.ParVector.from_numpy(ctx, np.zeros(n))
+    ml.cycle(x, ra.ParVector.from_numpy(ctx, b), x_is_zero=True)
+    assert np.array_equal(x.numpy(), H.cycle(np.zeros(n), b))
+    b32 = b.astype(np.float32)
+    x32 = ra.ParVector(ctx, n, np.float32)
+    b32d = ra.ParVector.from_numpy(ctx, b32)
+    ml.cycle_fp32(x32, b32d)
+    assert np.array_equal(x32.numpy(), H.cycle_fp32(np.zeros(n, np.float32), b32))
+
+
+def test_vcycle_sliced_bit_exact(problem, ctx):
+    """Every CSR level operator switched to the sliced-row kernel (the
+    Jacobi sweeps then read the level's diagonal array): FP64 and FP32 cycles
+    still bit-identical to the oracle.  The tuned choices are restored."""
+    ml, As, Ps = problem["ml"], problem["As"], problem["Ps"]
+    ml.enable_fp32()
+    before = ml.row_caps()
+    mats = []
+    for l in range(ml.num_levels):
+        A = ml.level(l)[0]
+        if A.diagonals() == 0 and A.nnz > 0:
+            mats.append((l, A))
+            for p in (0, 1):
+                A.set_sliced(p, True)
+    if not mats:
+        pytest.skip("no CSR level operator")
+    assert np.all(ml.row_caps()[[l for l, _ in mats], 0, :] == -1)
+    try:
+        H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=OMEGA)
+        n = As[0].n_rows
+        b = np.random.default_rng(24).standard_normal(n)
+        x = ra.ParVector.from_numpy(ctx, np.zeros(n))
+        ml.cycle(x, ra.ParVector.from_numpy(ctx, b), x_is_zero=True)
+        assert np.array_equal(x.numpy(), H.cycle(np.zeros(n), b))
+        b32 = b.astype(np.float32)
+        x32 = ra.ParVector(ctx, n, np.float32)
+        b32d = ra.ParVector.from_numpy(ctx, b32)
+        ml.cycle_fp32(x32, b32d)
+        assert np.array_equal(x32.numpy(), H.cycle_fp32(np.zeros(n, np.float32), b32))
+    finally:
+        for l, A in mats:
+            for p in (0, 1):
+                A.set_sliced(p, bool(before[l, 0, p] == -1))
+    assert np.array_equal(ml.row_caps(), before)
+
+
+def test_fpcg_sliced_outer_operator(ctx):
+    """FPCG on a CSR operator (a coarse Galerkin operator with long rows)
+    gives the same iterates with the outer product on the sliced kernel
+    (fused p.Ap partials per workgroup) as on the row-block kernel."""
+    rp, col, val = ra.gallery_laplace27(14)
+    ml0 = build(ctx, rp, col, val, rs(ra.PMIS, ra.Extended))
+    A1 = ml0.level(1)[0]
+    r1, c1, v1 = A1.to_csr()
+    ml = build(ctx, r1, c1, v1, rs(ra.PMIS, ra.Extended))
+    A = ra.ParCSRMatrix.from_csr(ctx, r1, c1, v1)
+    assert A.diagonals() == 0
+    n = r1.size - 1
+    b = np.random.default_rng(25).standard_normal(n)
+    out = []
+    for sliced in (False, True):
+        A.set_sliced(0, sliced)
+        x = ra.ParVector.from_numpy(ctx, np.zeros(n))
+        hist, it, rel = ra.FPCG(A, ml, x, ra.ParVector.from_numpy(ctx, b), 1e-8, 200, ra.PREC_FP64)
+        out.append((x.numpy(), it, rel))
+    assert out[0][1] == out[1][1] and out[0][2] == out[1][2]
+    assert np.array_equal(out[0][0], out[1][0])
+
+
 def test_amg_solve_matches_oracle(problem, ctx):
     ml, As, Ps = problem["ml"], problem["As"], problem["Ps"]
     H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=OMEGA)
@@ -276,7 +369,7 @@ def test_row_cap_autotuner_choices(problem):
     ml.enable_fp32()
     caps = ml.row_caps()
     assert caps.shape == (ml.num_levels, 3, 2)
-    assert set(np.unique(caps)) <= {0, 2048, 4096, 8192}
+    assert set(np.unique(caps)) <= {-1, 0, 2048, 4096, 8192}  # -1: sliced rows
     assert not np.any(caps[:, :, 0] == 8192)
     assert np.all(caps[-1, 1:] == 0)  # no P, R at the coarsest level
 

@@ -308,6 +308,30 @@ def test_ref_cycle_and_pcg_sor():
 
 
 @needs_ref
+@pytest.mark.parametrize("omega", [2.0 / 3.0, 1.0])
+def test_cycle_fp32_tracks_fp64(omega):
+    """The single-precision cycle follows the FP64 cycle (pinned above) to
+    FP32 accuracy on the reference's own C1 hierarchy: a dropped or
+    sign-flipped term would leave an O(1) difference, not 1e-6."""
+    A = stencil_grid_np(diffusion_2d(0.001, math.pi / 8), [60, 60])
+    As, Ps = _ref_hierarchy(A, 0.25, 1, 1, 0, omega)
+    assert len(As) >= 3
+    H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=omega)
+    b = np.random.default_rng(5).standard_normal(A.n_rows).astype(np.float32)
+    x64 = H.cycle(np.zeros(A.n_rows), b.astype(np.float64))
+    x32 = H.cycle_fp32(np.zeros(A.n_rows, np.float32), b)
+    assert x32.dtype == np.float32
+    err = np.linalg.norm(x32 - x64) / np.linalg.norm(x64)
+    assert 0 < err < 2e-6, err
+    # a nonzero float32 input on the coarsest level alone is solved in FP64
+    # and rounded once
+    Hc = K.Hier(As[-1:], [], relax_type=0, sweeps=1, omega=omega)
+    bc = np.random.default_rng(6).standard_normal(As[-1].n_rows).astype(np.float32)
+    xc = Hc.cycle_fp32(np.zeros_like(bc), bc)
+    assert np.array_equal(xc, Hc.cycle(np.zeros(bc.size), bc.astype(np.float64)).astype(np.float32))
+
+
+@needs_ref
 def test_ref_coarse_lu_matches_oracle():
     A = stencil_grid_np(diffusion_2d(0.001, math.pi / 8), [40, 40])
     As, Ps = _ref_hierarchy(A, 0.25, 1, 1, 1, 1.0)
