@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void bdia_kernel(int n, int k, DiaArgs a, int 
         const bool live = (ND == kDiaMax) ? (kk < nd) : true;
         const int d = live ? a.ord[kk] : 0;
         const bool on = live && ((m >> d) & 1u);
-        const X v = X(val[(size_t)d * n + row]);
+        const X v = X(val[dia_index<V>(live ? kk : 0, row, n)]);
         const size_t oj = (size_t)(on ? row + a.off[d] : row) * k;
         if (OP == OP_JACOBI && d == center) {
             diag = on ? v : diag;

@@ -115,7 +115,25 @@ constexpr int kDiaMax = 32;
 struct DiaArgs {
     int off[kDiaMax];  // offset of diagonal k (ascending in k)
     int ord[kDiaMax];  // visiting order: ord[kk] = diagonal index
+    int pos[kDiaMax];  // slot of diagonal k: ord[pos[k]] == k
 };
+// Stencil-diagonal values are stored by slot (position in the rows' common
+// stored order) in 16-byte groups: slot t of row r sits at
+// ((t / G) * n + r) * G + t % G with G = 16 / sizeof(V), so one lane reads
+// G consecutive slots of its row with one 16-byte load and a wavefront's
+// load is one contiguous run.  Slots past nd (up to a multiple of G) hold 0.
+template <typename V>
+__host__ __device__ __forceinline__ size_t dia_index(int slot, int row, int n)
+{
+    constexpr int G = 16 / (int)sizeof(V);
+    return ((size_t)(slot / G) * (size_t)n + (size_t)row) * G + (size_t)(slot % G);
+}
+template <typename V>
+__host__ __device__ inline int dia_slots(int nd)
+{
+    constexpr int G = 16 / (int)sizeof(V);
+    return (nd + G - 1) / G * G;
+}
 // x window of a 256-row workgroup for the LDS-staged kernels: the offsets
 // form runs of consecutive values; run j needs x[r0 + lo[j] .. r0 + 255 +
 // lo[j] + len[j] - 1], stored at base[j] in LDS, and diagonal k is read at
@@ -139,8 +157,8 @@ struct Dia {
     // (gather).  Chosen by timing both; the result bits are the same.
     int lds_x[3] = {-1, -1, -1};
     unsigned* mask = nullptr;   // device, n
-    double* v64 = nullptr;      // device, nd * n
-    float* v32 = nullptr;       // device, nd * n (built with the FP32 copy)
+    double* v64 = nullptr;      // device, dia_slots<double>(nd) * n, dia_index layout
+    float* v32 = nullptr;       // device, dia_slots<float>(nd) * n (built with the FP32 copy)
     bool v32_exact = false;     // every value is exactly representable in FP32
 };
 

@@ -104,6 +104,41 @@ ramd_csr_s* csr_wrap(ramd_ctx ctx, int n_rows, int n_cols, int64_t nnz, int* row
     return A;
 }
 
+// diagonal-indexed planes (plane k of row r at k * n + r) to the slot-grouped
+// layout; slots past nd get 0
+__global__ void dia_group_kernel(int n, int nd, int nslots, DiaArgs a, const double* __restrict__ planes,
+                                 double* __restrict__ out)
+{
+    const int64_t m = (int64_t)nslots * n;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        const int t = (int)(i / n), r = (int)(i % n);
+        out[dia_index<double>(t, r, n)] = t < nd ? planes[(size_t)a.ord[t] * n + r] : 0.0;
+    }
+}
+
+// FP32 copy of the slot-grouped FP64 values (each value rounded once)
+__global__ void dia_fp32_kernel(int n, int nd, int nslots, const double* __restrict__ v64, float* __restrict__ v32)
+{
+    const int64_t m = (int64_t)nslots * n;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        const int t = (int)(i / n), r = (int)(i % n);
+        v32[dia_index<float>(t, r, n)] = t < nd ? (float)v64[dia_index<double>(t, r, n)] : 0.0f;
+    }
+}
+
+static void dia_make_fp32(ramd_csr_s* A)
+{
+    Dia* d = A->dia;
+    const int n = A->n_rows;
+    const int ns = dia_slots<float>(d->nd);
+    if (!d->v32) d->v32 = dalloc<float>((size_t)ns * n);
+    const int grid = (int)std::min<int64_t>(((int64_t)ns * n + 255) / 256, 8192);
+    hipLaunchKernelGGL(dia_fp32_kernel, dim3(grid), dim3(256), 0, A->ctx->stream, n, d->nd, ns, d->v64, d->v32);
+    RAMD_HIP(hipGetLastError());
+}
+
 void dia_free(Dia* d)
 {
     if (!d) return;
@@ -185,6 +220,8 @@ bool dia_build(ramd_csr_s* A, const int* rp, const int* col, const double* val)
             if (k != center) d->args.ord[kk++] = k;
     }
     for (; kk < kDiaMax; kk++) d->args.ord[kk] = 0;
+    for (int k = 0; k < kDiaMax; k++) d->args.pos[k] = 0;
+    for (int t = 0; t < nd; t++) d->args.pos[d->args.ord[t]] = t;
     // runs of consecutive offsets and the LDS window they need
     {
         DiaWindow& w = d->window;
@@ -204,14 +241,23 @@ bool dia_build(ramd_csr_s* A, const int* rp, const int* col, const double* val)
         w.win = tot;
     }
     d->v32_exact = exact32;
+    double* planes = nullptr;
     try {
+        hipStream_t s = A->ctx->stream;
         d->mask = dalloc<unsigned>((size_t)n);
-        d->v64 = dalloc<double>((size_t)nd * n);
-        RAMD_HIP(hipMemcpyAsync(d->mask, mask.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice, A->ctx->stream));
-        RAMD_HIP(hipMemcpyAsync(d->v64, v.data(), sizeof(double) * (size_t)nd * n, hipMemcpyHostToDevice,
-                                A->ctx->stream));
-        RAMD_HIP(hipStreamSynchronize(A->ctx->stream));
+        const int ns = dia_slots<double>(nd);
+        d->v64 = dalloc<double>((size_t)ns * n);
+        planes = dalloc<double>((size_t)nd * n);
+        RAMD_HIP(hipMemcpyAsync(d->mask, mask.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice, s));
+        RAMD_HIP(hipMemcpyAsync(planes, v.data(), sizeof(double) * (size_t)nd * n, hipMemcpyHostToDevice, s));
+        const int grid = (int)std::min<int64_t>(((int64_t)ns * n + 255) / 256, 8192);
+        hipLaunchKernelGGL(dia_group_kernel, dim3(grid), dim3(256), 0, s, n, nd, ns, d->args, planes, d->v64);
+        RAMD_HIP(hipGetLastError());
+        RAMD_HIP(hipStreamSynchronize(s));
+        dfree(planes);
+        planes = nullptr;
     } catch (...) {
+        dfree(planes);
         dia_free(d);
         throw;
     }
@@ -397,13 +443,7 @@ void csr_make_fp32(ramd_csr_s* A)
         hipLaunchKernelGGL(to_fp32_kernel, dim3(grid), dim3(256), 0, A->ctx->stream, A->val, A->val32, A->nnz);
         RAMD_HIP(hipGetLastError());
     }
-    if (A->dia && !A->dia->v32) {
-        const int64_t m = (int64_t)A->dia->nd * A->n_rows;
-        A->dia->v32 = dalloc<float>((size_t)m);
-        int grid = (int)std::min<int64_t>((m + 255) / 256, 8192);
-        hipLaunchKernelGGL(to_fp32_kernel, dim3(grid), dim3(256), 0, A->ctx->stream, A->dia->v64, A->dia->v32, m);
-        RAMD_HIP(hipGetLastError());
-    }
+    if (A->dia && !A->dia->v32) dia_make_fp32(A);
 }
 
 // Rewrites the stencil-diagonal planes from new CSR values (same pattern):
@@ -417,7 +457,7 @@ __global__ void dia_scatter_kernel(int n, DiaArgs a, int nd, const int* __restri
         const int d = col[j] - row;
         int k = 0;
         while (k < nd - 1 && a.off[k] != d) k++;
-        v64[(size_t)k * n + row] = val[j];
+        v64[dia_index<double>(a.pos[k], row, n)] = val[j];
     }
 }
 
@@ -457,7 +497,7 @@ void csr_set_values(ramd_csr_s* A, const double* val_d)
         hipLaunchKernelGGL(dia_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, d->args, d->nd, A->rowptr,
                            A->col, A->val, d->v64);
         RAMD_HIP(hipGetLastError());
-        const int64_t m = (int64_t)d->nd * n;
+        const int64_t m = (int64_t)dia_slots<double>(d->nd) * n;  // padding slots hold exact zeros
         const int g = (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (m + 255) / 256));
         double* part = ctx_partials(ctx, (size_t)g);
         hipLaunchKernelGGL(inexact32_kernel, dim3(g), dim3(kBlockThreads), 0, s, d->v64, m, part);
@@ -466,11 +506,7 @@ void csr_set_values(ramd_csr_s* A, const double* val_d)
         double inexact = 0;
         fetch_scalars(ctx, ctx->red_result + 8, 1, &inexact);
         d->v32_exact = inexact == 0.0;
-        if (d->v32) {
-            int grid = (int)std::min<int64_t>((m + 255) / 256, 8192);
-            hipLaunchKernelGGL(to_fp32_kernel, dim3(grid), dim3(256), 0, s, d->v64, d->v32, m);
-            RAMD_HIP(hipGetLastError());
-        }
+        if (d->v32) dia_make_fp32(A);
     }
     RAMD_HIP(hipStreamSynchronize(s));
 }

@@ -12,6 +12,7 @@
 // by its own workgroup.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -236,10 +237,43 @@ __global__ __launch_bounds__(kBlockThreads) void sliced_kernel(
 }
 
 // Stencil-diagonal kernels: one lane per row, diagonals visited in the
-// row's stored order (a.ord).  Every load is issued unconditionally -- a
-// diagonal absent from the row reads its own plane (zero) and x[row] -- and
-// the accumulation skips it with a select, so the operation sequence of each
-// row is exactly the CSR row loop's and no out-of-range x is touched.
+// row's stored order (slot t holds diagonal a.ord[t]).  A lane reads its
+// row's values a 16-byte group of slots at a time (dia_index layout).  Every
+// load is issued unconditionally -- a diagonal absent from the row reads its
+// own zero and x[row] -- and the accumulation skips it with a select, so the
+// operation sequence of each row is exactly the CSR row loop's and no
+// out-of-range x is touched.
+
+// slots per chunk: all of a 27-point row's FP32 values, ten FP64 values;
+// x is gathered and added in sub-chunks of dia_xchunk slots (nine FP64
+// values against FP32 matrix values, to hold fewer registers)
+template <typename V>
+constexpr int dia_chunk() { return sizeof(V) == 4 ? 28 : 10; }
+template <typename V, typename X>
+constexpr int dia_xchunk() { return (sizeof(V) == 4 && sizeof(X) == 8) ? 9 : dia_chunk<V>(); }
+
+// Values of slots [c0, c0 + CH) of `row`; groups past the stored slots (nsl,
+// a multiple of the group size) read nothing and give 0.
+template <typename V, int CH, int ND>
+__device__ __forceinline__ void dia_load(const V* __restrict__ val, int c0, int row, int n, int nsl, V (&vv)[CH])
+{
+    constexpr int G = 16 / (int)sizeof(V);
+    constexpr int NSL = (ND + G - 1) / G * G;
+    static_assert(CH % G == 0, "chunk must hold whole groups");
+    using VG = typename std::conditional<sizeof(V) == 4, float4, double2>::type;
+    const VG* vg = reinterpret_cast<const VG*>(val);
+#pragma unroll
+    for (int g = 0; g < CH / G; g++) {
+        const int s0 = c0 + g * G;
+        const bool ok = (ND == kDiaMax) ? s0 < nsl : s0 < NSL;
+        VG v{};
+        if (ok) v = vg[(size_t)(s0 / G) * n + row];
+        const V* pv = reinterpret_cast<const V*>(&v);
+#pragma unroll
+        for (int t = 0; t < G; t++) vv[g * G + t] = pv[t];
+    }
+}
+
 // y_row of the plain product (OP_SPMV) in the stencil-diagonal form: the
 // same load and select sequence as dia_kernel's body.
 template <int OP, typename V, typename X, int ND>
@@ -249,26 +283,31 @@ __device__ __forceinline__ X dia_row(int row, int n, const DiaArgs& a, int nd, i
 {
     const unsigned m = mask[row];
     X acc = X(0);
-    constexpr int CH = 9;
+    constexpr int CH = dia_chunk<V>();
+    const int nsl = dia_slots<V>(nd);
+    constexpr int XC = dia_xchunk<V, X>();
 #pragma unroll
     for (int c0 = 0; c0 < ND; c0 += CH) {
         V vv[CH];
-        X xx[CH];
-        bool on[CH];
+        dia_load<V, CH, ND>(val, c0, row, n, nsl, vv);
 #pragma unroll
-        for (int t = 0; t < CH; t++) {
-            const int kk = c0 + t;
-            if (kk < ND) {
-                const bool live = (ND == kDiaMax) ? (kk < nd) : true;
-                const int k = live ? a.ord[kk] : 0;
-                on[t] = live && ((m >> k) & 1u);
-                vv[t] = val[(size_t)k * n + row];
-                xx[t] = x[on[t] ? row + a.off[k] : row];
+        for (int x0 = 0; x0 < CH; x0 += XC) {
+            X xx[XC];
+            bool on[XC];
+#pragma unroll
+            for (int t = 0; t < XC; t++) {
+                const int kk = c0 + x0 + t;
+                if (x0 + t < CH && kk < ND) {
+                    const bool live = (ND == kDiaMax) ? (kk < nd) : true;
+                    const int k = live ? a.ord[kk] : 0;
+                    on[t] = live && ((m >> k) & 1u);
+                    xx[t] = x[on[t] ? row + a.off[k] : row];
+                }
             }
-        }
 #pragma unroll
-        for (int t = 0; t < CH; t++)
-            if (c0 + t < ND) acc = on[t] ? acc + X(vv[t]) * xx[t] : acc;
+            for (int t = 0; t < XC; t++)
+                if (x0 + t < CH && c0 + x0 + t < ND) acc = on[t] ? acc + X(vv[x0 + t]) * xx[t] : acc;
+        }
     }
     (void)b;
     (void)center;
@@ -295,36 +334,42 @@ __global__ __launch_bounds__(kBlockThreads) void dia_kernel(int n, DiaArgs a, in
     const unsigned m = mask[row];
     X acc = (OP == OP_RESIDUAL) ? b[row] : X(0);
     X diag = 0;
-    constexpr int CH = 9;
+    constexpr int CH = dia_chunk<V>();
+    const int nsl = dia_slots<V>(nd);
+    constexpr int XC = dia_xchunk<V, X>();
 #pragma unroll
     for (int c0 = 0; c0 < ND; c0 += CH) {
         V vv[CH];
-        X xx[CH];
-        bool on[CH];
-        int kd[CH];
+        dia_load<V, CH, ND>(val, c0, row, n, nsl, vv);
 #pragma unroll
-        for (int t = 0; t < CH; t++) {
-            const int kk = c0 + t;
-            if (kk < ND) {
-                const bool live = (ND == kDiaMax) ? (kk < nd) : true;
-                const int k = live ? a.ord[kk] : 0;
-                kd[t] = k;
-                on[t] = live && ((m >> k) & 1u);
-                vv[t] = val[(size_t)k * n + row];
-                xx[t] = x[on[t] ? row + a.off[k] : row];
+        for (int x0 = 0; x0 < CH; x0 += XC) {
+            X xx[XC];
+            bool on[XC];
+            int kd[XC];
+#pragma unroll
+            for (int t = 0; t < XC; t++) {
+                const int kk = c0 + x0 + t;
+                if (x0 + t < CH && kk < ND) {
+                    const bool live = (ND == kDiaMax) ? (kk < nd) : true;
+                    const int k = live ? a.ord[kk] : 0;
+                    kd[t] = k;
+                    on[t] = live && ((m >> k) & 1u);
+                    xx[t] = x[on[t] ? row + a.off[k] : row];
+                }
             }
-        }
 #pragma unroll
-        for (int t = 0; t < CH; t++) {
-            if (c0 + t < ND) {
-                const X p = X(vv[t]) * xx[t];
-                if (OP == OP_JACOBI) {
-                    if (kd[t] == center) diag = on[t] ? X(vv[t]) : diag;
-                    else acc = on[t] ? acc + p : acc;
-                } else if (OP == OP_RESIDUAL) {
-                    acc = on[t] ? acc - p : acc;
-                } else {
-                    acc = on[t] ? acc + p : acc;
+            for (int t = 0; t < XC; t++) {
+                if (x0 + t < CH && c0 + x0 + t < ND) {
+                    const X v = X(vv[x0 + t]);
+                    const X p = v * xx[t];
+                    if (OP == OP_JACOBI) {
+                        if (kd[t] == center) diag = on[t] ? v : diag;
+                        else acc = on[t] ? acc + p : acc;
+                    } else if (OP == OP_RESIDUAL) {
+                        acc = on[t] ? acc - p : acc;
+                    } else {
+                        acc = on[t] ? acc + p : acc;
+                    }
                 }
             }
         }
@@ -374,12 +419,14 @@ __global__ __launch_bounds__(kBlockThreads) void dia_lds_kernel(int n, DiaArgs a
     X diag = 0;
     const unsigned m = valid ? mask[row] : 0u;
     const int vrow = valid ? row : 0;
-    constexpr int CH = 9;
+    constexpr int CH = dia_chunk<V>();
+    const int nsl = dia_slots<V>(nd);
 #pragma unroll
     for (int c0 = 0; c0 < ND; c0 += CH) {
         V vv[CH];
         bool on[CH];
         int kd[CH];
+        dia_load<V, CH, ND>(val, c0, vrow, n, nsl, vv);
 #pragma unroll
         for (int t = 0; t < CH; t++) {
             const int kk = c0 + t;
@@ -388,7 +435,6 @@ __global__ __launch_bounds__(kBlockThreads) void dia_lds_kernel(int n, DiaArgs a
                 const int k = live ? a.ord[kk] : 0;
                 kd[t] = k;
                 on[t] = live && ((m >> k) & 1u);
-                vv[t] = val[(size_t)k * n + vrow];
             }
         }
 #pragma unroll
