@@ -306,9 +306,17 @@ int ramd_fpcg_block(ramd_hier h, ramd_csr A, int32_t k, double* X_d, const doubl
  * RAMD_PREC_FP32); column c is bit-identical to ramd_vcycle (FP64) or
  * ramd_vcycle_fp32 applied to column c. */
 int ramd_vcycle_block(ramd_hier h, int32_t k, int prec, void* X_d, const void* B_d);
-/* Lanes-per-row chosen by the multi-vector autotuner for k columns: three
- * entries per level (A, P, R; 0 where absent). */
-int ramd_block_tuning(ramd_hier h, int32_t k, int32_t* lpr);
+/* Kernel choice of the multi-vector autotuner for k columns and vector
+ * precision prec: three entries per level (A, P, R; 0 where absent).  A
+ * positive entry is the lanes per row of the one-column-per-lane-step kernel,
+ * a negative one -(lanes per row) of the kernel reading 16-byte units of
+ * consecutive columns (k a multiple of 16 / sizeof(element)).  No reference
+ * counterpart. */
+int ramd_block_tuning(ramd_hier h, int32_t k, int32_t prec, int32_t* codes);
+/* Sets those kernel choices (same encoding; entries of the coarsest level
+ * are ignored), e.g. to compare kernels; every column stays bit-identical
+ * whichever kernel runs.  No reference counterpart. */
+int ramd_block_set_tuning(ramd_hier h, int32_t k, int32_t prec, const int32_t* codes);
 
 /* Adjoint of x = A^-1 b for a symmetric positive definite A (no reference
  * counterpart: the reference has no differentiation).  Given the solution

@@ -35,7 +35,7 @@ from ._lib import (CLJP, HMIS, PMIS, RS, Classical, Direct, Extended, Falgout, J
 
 __all__ = [
     "Context", "ParVector", "ParCSRMatrix", "ParMultilevel", "ParRugeStubenSolver",
-    "ParSmoothedAggregationSolver", "PCG", "CG", "FPCG", "FPCG_block", "vcycle_block", "block_tuning", "adjoint", "strength", "split", "interpolation", "mis2",
+    "ParSmoothedAggregationSolver", "PCG", "CG", "FPCG", "FPCG_block", "vcycle_block", "block_tuning", "block_set_tuning", "adjoint", "strength", "split", "interpolation", "mis2",
     "aggregate", "fit_candidates", "jacobi_prolongation", "read_petsc", "write_petsc", "gallery_laplace27", "gallery_aniso2d", "gallery_fv7", "RamdError",
     "RS", "CLJP", "Falgout", "PMIS", "HMIS", "Direct", "ModClassical", "Extended", "Classical",
     "Symmetric", "Jacobi", "PREC_FP64", "PREC_FP32",
@@ -462,11 +462,21 @@ def vcycle_block(ml, X, B, k, prec=PREC_FP64):
     check(lib().ramd_vcycle_block(ml.h, int(k), int(prec), X.ptr, B.ptr))
 
 
-def block_tuning(ml, k):
-    """Lanes per row picked by the autotuner: array (levels, 3) for A, P, R."""
+def block_tuning(ml, k, prec=PREC_FP64):
+    """Multi-vector kernels picked by the autotuner for k columns and vector
+    precision prec: array (levels, 3) for A, P, R -- lanes per row, negative
+    for the kernel that reads 16-byte units of consecutive columns."""
     out = np.zeros(3 * ml.num_levels, np.int32)
-    check(lib().ramd_block_tuning(ml.h, int(k), _ptr(out, C.c_int32)))
+    check(lib().ramd_block_tuning(ml.h, int(k), int(prec), _ptr(out, C.c_int32)))
     return out.reshape(-1, 3)
+
+
+def block_set_tuning(ml, k, codes, prec=PREC_FP64):
+    """Overrides the multi-vector kernel choices (array (levels, 3), same
+    encoding as block_tuning)."""
+    codes = np.ascontiguousarray(codes, dtype=np.int32).ravel()
+    assert codes.size == 3 * ml.num_levels
+    check(lib().ramd_block_set_tuning(ml.h, int(k), int(prec), _ptr(codes, C.c_int32)))
 
 
 def adjoint(A, ml, x, g, rtol=1e-10, max_iter=500, cycle_prec=PREC_FP64, with_dA=True):

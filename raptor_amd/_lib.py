@@ -118,7 +118,8 @@ _SIGS = {
     "ramd_adjoint": (P, P, P, P, D, C.c_int, C.c_int, P, P, PI32, PD),
     "ramd_fpcg_block": (P, P, I32, P, P, D, C.c_int, C.c_int, C.c_int, PI32, PD, PI32),
     "ramd_vcycle_block": (P, I32, C.c_int, P, P),
-    "ramd_block_tuning": (P, I32, PI32),
+    "ramd_block_tuning": (P, I32, I32, PI32),
+    "ramd_block_set_tuning": (P, I32, I32, PI32),
     "ramd_csr_set_values": (P, P),
     "ramd_host_free": None,
     "ramd_host_strength": (I32, PI32, PI32, PD, C.c_int, D, PPI32, PPI32, PI64),

@@ -138,6 +138,213 @@ __global__ __launch_bounds__(256) void bdia_kernel(int n, int k, DiaArgs a, int 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Vectorized SpMM: the k columns of a row are read and written as 16-byte
+// units of VW = 16 / sizeof(X) consecutive columns (k a multiple of VW); lane
+// g of a row's LPR lanes owns units g, g + LPR, ... (at most kMaxUnits).  A
+// lane group's load of one x row is then one contiguous run of whole cache
+// lines.  Every column still runs over the row in stored order with the
+// scalar kernels' operation sequence, so the bits are the same.
+// ---------------------------------------------------------------------------
+constexpr int kMaxUnits = 4;
+
+template <typename X>
+struct Unit16;
+template <>
+struct Unit16<float> {
+    using T = float4;
+};
+template <>
+struct Unit16<double> {
+    using T = double2;
+};
+
+template <typename X>
+__device__ __forceinline__ void unit_load(const X* __restrict__ p, X (&v)[16 / sizeof(X)])
+{
+    using XV = typename Unit16<X>::T;
+    const XV t = *reinterpret_cast<const XV*>(p);
+    const X* pt = reinterpret_cast<const X*>(&t);
+#pragma unroll
+    for (int i = 0; i < (int)(16 / sizeof(X)); i++) v[i] = pt[i];
+}
+
+template <typename X>
+__device__ __forceinline__ void unit_store(X* __restrict__ p, const X (&v)[16 / sizeof(X)])
+{
+    using XV = typename Unit16<X>::T;
+    XV t;
+    X* pt = reinterpret_cast<X*>(&t);
+#pragma unroll
+    for (int i = 0; i < (int)(16 / sizeof(X)); i++) pt[i] = v[i];
+    *reinterpret_cast<XV*>(p) = t;
+}
+
+// accumulators <- b (residual) or 0
+template <int OP, typename X, int LPR>
+__device__ __forceinline__ void bvec_init(X (&acc)[kMaxUnits][16 / sizeof(X)], int g, int units,
+                                          const X* __restrict__ brow)
+{
+    constexpr int VW = 16 / sizeof(X);
+#pragma unroll
+    for (int u = 0; u < kMaxUnits; u++) {
+        const int un = g + u * LPR;
+#pragma unroll
+        for (int t = 0; t < VW; t++) acc[u][t] = X(0);
+        if (OP == OP_RESIDUAL && un < units) unit_load<X>(brow + un * VW, acc[u]);
+    }
+}
+
+// one stored entry a * x_row for every owned column (on: the entry exists)
+template <int OP, typename X, int LPR>
+__device__ __forceinline__ void bvec_step(X (&acc)[kMaxUnits][16 / sizeof(X)], int g, int units, X a, bool on,
+                                          const X* __restrict__ xrow)
+{
+    constexpr int VW = 16 / sizeof(X);
+#pragma unroll
+    for (int u = 0; u < kMaxUnits; u++) {
+        const int un = g + u * LPR;
+        if (un < units) {
+            X xv[VW];
+            unit_load<X>(xrow + un * VW, xv);
+#pragma unroll
+            for (int t = 0; t < VW; t++) {
+                const X p = a * xv[t];
+                if (OP == OP_RESIDUAL) acc[u][t] = on ? acc[u][t] - p : acc[u][t];
+                else acc[u][t] = on ? acc[u][t] + p : acc[u][t];
+            }
+        }
+    }
+}
+
+// y row from the accumulators (Jacobi: usable = the row has a usable diagonal)
+template <int OP, typename X, int LPR>
+__device__ __forceinline__ void bvec_finish(const X (&acc)[kMaxUnits][16 / sizeof(X)], int g, int units,
+                                            const X* __restrict__ xrow, const X* __restrict__ brow,
+                                            X* __restrict__ yrow, X diag, bool usable, X omega)
+{
+    constexpr int VW = 16 / sizeof(X);
+#pragma unroll
+    for (int u = 0; u < kMaxUnits; u++) {
+        const int un = g + u * LPR;
+        if (un >= units) continue;
+        X out[VW];
+        if (OP == OP_SPMV || OP == OP_RESIDUAL) {
+#pragma unroll
+            for (int t = 0; t < VW; t++) out[t] = acc[u][t];
+        } else if (OP == OP_ADD_INTO) {
+            X bv[VW];
+            unit_load<X>(brow + un * VW, bv);
+#pragma unroll
+            for (int t = 0; t < VW; t++) out[t] = bv[t] + acc[u][t];
+        } else {
+            X xo[VW], bv[VW];
+            unit_load<X>(xrow + un * VW, xo);
+            unit_load<X>(brow + un * VW, bv);
+#pragma unroll
+            for (int t = 0; t < VW; t++)
+                out[t] = usable ? ((X(1.0) - omega) * xo[t]) + (omega * ((bv[t] - acc[u][t]) / diag)) : xo[t];
+        }
+        unit_store<X>(yrow + un * VW, out);
+    }
+}
+
+template <int OP, typename V, typename X, int LPR>
+__global__ __launch_bounds__(256) void bcsr_vec_kernel(int n, int k, const int* __restrict__ rp,
+                                                       const int* __restrict__ col, const V* __restrict__ val,
+                                                       const X* __restrict__ x, const X* __restrict__ b,
+                                                       X* __restrict__ y, X omega)
+{
+    constexpr int VW = 16 / sizeof(X);
+    constexpr int RPW = 256 / LPR;
+    const int row = xcd_remap(blockIdx.x, gridDim.x) * RPW + threadIdx.x / LPR;
+    const int g = threadIdx.x % LPR;
+    const int units = k / VW;
+    if (row >= n || g >= units) return;
+    const int s = rp[row], e = rp[row + 1];
+    const size_t o = (size_t)row * k;
+    X acc[kMaxUnits][VW], diag = 0;
+    bvec_init<OP, X, LPR>(acc, g, units, b + o);
+    for (int j = s; j < e; j++) {
+        const int cj = col[j];
+        const X a = X(val[j]);
+        if (OP == OP_JACOBI && cj == row) {
+            diag = a;
+            continue;
+        }
+        bvec_step<OP, X, LPR>(acc, g, units, a, true, x + (size_t)cj * k);
+    }
+    bvec_finish<OP, X, LPR>(acc, g, units, x + o, b + o, y + o, diag, s != e && fabs(diag) > X(1e-16), omega);
+}
+
+template <int OP, typename V, typename X, int LPR, int ND>
+__global__ __launch_bounds__(256) void bdia_vec_kernel(int n, int k, DiaArgs a, int nd, int center,
+                                                       const unsigned* __restrict__ mask, const V* __restrict__ val,
+                                                       const X* __restrict__ x, const X* __restrict__ b,
+                                                       X* __restrict__ y, X omega)
+{
+    constexpr int VW = 16 / sizeof(X);
+    constexpr int RPW = 256 / LPR;
+    const int row = xcd_remap(blockIdx.x, gridDim.x) * RPW + threadIdx.x / LPR;
+    const int g = threadIdx.x % LPR;
+    const int units = k / VW;
+    if (row >= n || g >= units) return;
+    const unsigned m = mask[row];
+    const size_t o = (size_t)row * k;
+    X acc[kMaxUnits][VW], diag = 0;
+    bvec_init<OP, X, LPR>(acc, g, units, b + o);
+#pragma unroll
+    for (int kk = 0; kk < ND; kk++) {
+        const bool live = (ND == kDiaMax) ? (kk < nd) : true;
+        const int d = live ? a.ord[kk] : 0;
+        const bool on = live && ((m >> d) & 1u);
+        const X v = X(val[dia_index<V>(live ? kk : 0, row, n)]);
+        if (OP == OP_JACOBI && d == center) {
+            diag = on ? v : diag;
+            continue;
+        }
+        bvec_step<OP, X, LPR>(acc, g, units, v, on, x + (size_t)(on ? row + a.off[d] : row) * k);
+    }
+    bvec_finish<OP, X, LPR>(acc, g, units, x + o, b + o, y + o, diag, m != 0u && fabs(diag) > X(1e-16), omega);
+}
+
+template <int OP, typename V, typename X, int LPR>
+static void blaunch_vec_op(const ramd_csr_s* A, const V* val, const V* dval, int k, const X* x, const X* b, X* y,
+                           X omega, hipStream_t s)
+{
+    const int n = A->n_rows;
+    if (n == 0) return;
+    const dim3 grid((unsigned)((n + 256 / LPR - 1) / (256 / LPR))), block(256);
+    if (dval) {
+        const Dia* d = A->dia;
+#define BDIAV(NDV)                                                                                                 \
+    hipLaunchKernelGGL((bdia_vec_kernel<OP, V, X, LPR, NDV>), grid, block, 0, s, n, k, d->args, d->nd, d->center, \
+                       d->mask, dval, x, b, y, omega)
+        if (d->nd == 27) BDIAV(27);
+        else if (d->nd == 9) BDIAV(9);
+        else if (d->nd == 7) BDIAV(7);
+        else BDIAV(kDiaMax);
+#undef BDIAV
+    } else {
+        hipLaunchKernelGGL((bcsr_vec_kernel<OP, V, X, LPR>), grid, block, 0, s, n, k, A->rowptr, A->col, val, x, b,
+                           y, omega);
+    }
+    RAMD_HIP(hipGetLastError());
+}
+
+template <typename V, typename X, int LPR>
+static void blaunch_vec_lpr(const ramd_csr_s* A, const V* val, const V* dval, int op, int k, const X* x,
+                            const X* b, X* y, X omega, hipStream_t s)
+{
+    switch (op) {
+    case OP_SPMV: blaunch_vec_op<OP_SPMV, V, X, LPR>(A, val, dval, k, x, b, y, omega, s); break;
+    case OP_RESIDUAL: blaunch_vec_op<OP_RESIDUAL, V, X, LPR>(A, val, dval, k, x, b, y, omega, s); break;
+    case OP_JACOBI: blaunch_vec_op<OP_JACOBI, V, X, LPR>(A, val, dval, k, x, b, y, omega, s); break;
+    case OP_ADD_INTO: blaunch_vec_op<OP_ADD_INTO, V, X, LPR>(A, val, dval, k, x, b, y, omega, s); break;
+    default: fail(RAMD_E_ARG, "block op %d not supported", op);
+    }
+}
+
 template <int OP, typename V, typename X, int LPR>
 static void blaunch_op(const ramd_csr_s* A, const V* val, const V* dval, int k, const X* x, const X* b, X* y,
                        X omega, hipStream_t s)
@@ -178,10 +385,27 @@ static void blaunch_lpr(const ramd_csr_s* A, const V* val, const V* dval, int op
 // lanes per row: 8, 16, 32 or 64 (at least k/8, since a lane holds <= 8 columns)
 static bool lpr_ok(int lpr, int k) { return lpr * 8 >= k && (lpr <= 8 || lpr / 2 < k); }
 
+// vectorized kernel with lpr lanes per row for k columns of X: whole 16-byte
+// units, at most kMaxUnits per lane, no lane without a unit
 template <typename X>
-static void blaunch(const ramd_csr_s* A, int lpr, int op, int k, const X* x, const X* b, X* y, X omega,
+static bool vec_ok(int lpr, int k)
+{
+    constexpr int VW = 16 / sizeof(X);
+    if (k % VW) return false;
+    const int units = k / VW;
+    return lpr <= units && (units + lpr - 1) / lpr <= kMaxUnits;
+}
+
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+// code > 0: scalar kernel with `code` lanes per row; code < 0: vectorized
+// kernel with -code lanes per row (falls back to 32 scalar lanes when the
+// vectors are not 16-byte aligned)
+template <typename X>
+static void blaunch(const ramd_csr_s* A, int code, int op, int k, const X* x, const X* b, X* y, X omega,
                     hipStream_t s)
 {
+    if (code < 0 && !(aligned16(x) && aligned16(y) && (!b || aligned16(b)))) code = lpr_ok(32, k) ? 32 : 64;
     // FP64 vectors read FP64 values; FP32 vectors read the FP32 copies
     using V = X;
     const V* val;
@@ -194,12 +418,17 @@ static void blaunch(const ramd_csr_s* A, int lpr, int op, int k, const X* x, con
         val = (const V*)A->val32;
         dval = A->dia ? (const V*)A->dia->v32 : nullptr;
     }
-    switch (lpr) {
+    switch (code) {
     case 8: blaunch_lpr<V, X, 8>(A, val, dval, op, k, x, b, y, omega, s); break;
     case 16: blaunch_lpr<V, X, 16>(A, val, dval, op, k, x, b, y, omega, s); break;
     case 32: blaunch_lpr<V, X, 32>(A, val, dval, op, k, x, b, y, omega, s); break;
     case 64: blaunch_lpr<V, X, 64>(A, val, dval, op, k, x, b, y, omega, s); break;
-    default: fail(RAMD_E_ARG, "lanes per row %d", lpr);
+    case -2: blaunch_vec_lpr<V, X, 2>(A, val, dval, op, k, x, b, y, omega, s); break;
+    case -4: blaunch_vec_lpr<V, X, 4>(A, val, dval, op, k, x, b, y, omega, s); break;
+    case -8: blaunch_vec_lpr<V, X, 8>(A, val, dval, op, k, x, b, y, omega, s); break;
+    case -16: blaunch_vec_lpr<V, X, 16>(A, val, dval, op, k, x, b, y, omega, s); break;
+    case -32: blaunch_vec_lpr<V, X, 32>(A, val, dval, op, k, x, b, y, omega, s); break;
+    default: fail(RAMD_E_ARG, "block kernel code %d", code);
     }
 }
 
@@ -209,20 +438,24 @@ static void blaunch(const ramd_csr_s* A, int lpr, int op, int k, const X* x, con
 // t % kp and row offset t / kp; rows advance by (256 / kp) * gridDim.x.
 // Sums are combined in a fixed order.
 // ---------------------------------------------------------------------------
-constexpr int kColBlocks = 512;
+constexpr int kColBlocks = 4096;
 
+// first Jacobi sweep from X = 0 (the single-vector jacobi_from_zero per column)
 template <typename X>
-__global__ void bjacobi_from_zero(int n, int k, const X* __restrict__ d, const X* __restrict__ b, X* __restrict__ x,
-                                  X omega)
+__global__ __launch_bounds__(256) void bjacobi_from_zero(int n, int k, int kp, const X* __restrict__ d,
+                                                         const X* __restrict__ b, X* __restrict__ x, X omega)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (int64_t)n * k) return;
-    const int i = (int)(e / k);
-    const X di = d[i];
+    const int c = threadIdx.x & (kp - 1);
+    const int rpb = 256 / kp;
+    if (c >= k) return;
     const X zero = X(0.0);
-    X v = zero;
-    if (fabs(di) > X(1e-16)) v = ((X(1.0) - omega) * zero) + (omega * ((b[e] - zero) / di));  // d = 0 without a diagonal
-    x[e] = v;
+    for (int64_t i = (int64_t)blockIdx.x * rpb + threadIdx.x / kp; i < n; i += (int64_t)gridDim.x * rpb) {
+        const size_t e = (size_t)i * k + c;
+        const X di = d[i];  // 0 for a row without a stored diagonal
+        X v = zero;
+        if (fabs(di) > X(1e-16)) v = ((X(1.0) - omega) * zero) + (omega * ((b[e] - zero) / di));
+        x[e] = v;
+    }
 }
 
 template <typename A, typename B>
@@ -233,6 +466,12 @@ __global__ void bconvert(int64_t m, const A* __restrict__ a, B* __restrict__ b)
 }
 
 __global__ void bfill(int64_t m, double v, double* __restrict__ a)
+{
+    const int64_t s = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += s) a[i] = v;
+}
+
+__global__ void bfill32(int64_t m, float v, float* __restrict__ a)
 {
     const int64_t s = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += s) a[i] = v;
@@ -283,6 +522,16 @@ __global__ __launch_bounds__(256) void bcoarse_solve(int n, int k, const double*
     }
     for (int i = threadIdx.x; i < n; i += blockDim.x) x[(size_t)i * k + c] = (X)sb[i];
 }
+
+static int pow2_at_least(int k)
+{
+    int p = 1;
+    while (p < k) p <<= 1;
+    return p;
+}
+
+// workgroups of the per-column kernels for n rows
+static int col_blocks(int n, int kp) { return std::max(1, std::min(kColBlocks, (n + 256 / kp - 1) / (256 / kp))); }
 
 // Combines the 256 / kp row-offset sums of each column in LDS (in offset
 // order) and stores the workgroup's partial for column c at
@@ -358,14 +607,22 @@ __global__ __launch_bounds__(256) void bdir(int n, int k, int kp, const double* 
     }
 }
 
-// out[c] = sum over g of partials[g * k + c], in g order (one workgroup)
-__global__ void bcol_final(int np, int k, const double* __restrict__ partials, double* __restrict__ out)
+// out[c] = sum over g of partials[g * k + c] (workgroup c): thread t adds
+// g = t, t + 256, ... in order, then the 256 sums combine in a fixed tree
+__global__ __launch_bounds__(256) void bcol_final(int np, int k, const double* __restrict__ partials,
+                                                  double* __restrict__ out)
 {
-    const int c = threadIdx.x;
-    if (c >= k) return;
+    __shared__ double sh[256];
+    const int c = blockIdx.x;
     double s = 0;
-    for (int g = 0; g < np; g++) s += partials[(size_t)g * k + c];
-    out[c] = s;
+    for (int g = threadIdx.x; g < np; g += 256) s += partials[(size_t)g * k + c];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = sh[0];
 }
 
 // Device scalars of the lockstep flexible PCG, one entry per column.
@@ -458,7 +715,8 @@ __global__ void bscal_recheck(int k, double rtol, BScal S)
 struct BlockLevel {
     double *x = nullptr, *b = nullptr, *t = nullptr, *r = nullptr;
     float *x32 = nullptr, *b32 = nullptr, *t32 = nullptr, *r32 = nullptr;
-    int lpr_A = 0, lpr_P = 0, lpr_R = 0;
+    // kernel codes (see blaunch) per vector precision, [0] FP64, [1] FP32
+    int code_A[2] = {0, 0}, code_P[2] = {0, 0}, code_R[2] = {0, 0};
 };
 
 struct BlockState {
@@ -518,22 +776,27 @@ static double time_ms(hipStream_t s, const std::function<void()>& fn, int reps)
     return ms / reps;
 }
 
-// Autotuner: lanes per row for one operator and k, by timing the residual
-// (FP64) or the FP32 residual on the level's own block vectors.
+// Autotuner: kernel and lanes per row for one operator, k and vector
+// precision, by timing the residual (square operators) or the product on
+// the level's own block vectors.
 template <typename X>
 static int block_tune(ramd_hier_s* h, const ramd_csr_s* M, int k, const X* x, const X* b, X* y)
 {
     hipStream_t s = h->ctx->stream;
     int best = 0;
     double best_ms = 1e30;
-    for (int lpr : {8, 16, 32, 64}) {
-        if (!lpr_ok(lpr, k)) continue;
-        const double ms =
-            time_ms(s, [&] { blaunch<X>(M, lpr, M->n_rows == M->n_cols ? OP_RESIDUAL : OP_SPMV, k, x, b, y, X(0), s); },
-                    3);
+    std::vector<int> codes;
+    for (int lpr : {8, 16, 32, 64})
+        if (lpr_ok(lpr, k)) codes.push_back(lpr);
+    if (aligned16(x) && aligned16(b) && aligned16(y))
+        for (int lpr : {2, 4, 8, 16, 32})
+            if (vec_ok<X>(lpr, k)) codes.push_back(-lpr);
+    for (int code : codes) {
+        const double ms = time_ms(
+            s, [&] { blaunch<X>(M, code, M->n_rows == M->n_cols ? OP_RESIDUAL : OP_SPMV, k, x, b, y, X(0), s); }, 3);
         if (ms < best_ms) {
             best_ms = ms;
-            best = lpr;
+            best = code;
         }
     }
     return best;
@@ -584,9 +847,9 @@ static BlockState* block_state(ramd_hier_s* h, int k, bool fp32)
         for (size_t l = 0; l + 1 < h->levels.size(); l++) {
             Level& Lv = h->levels[l];
             BlockLevel& L = b->lv[l];
-            L.lpr_A = block_tune<double>(h, Lv.A, k, L.x, L.b, L.r);
-            L.lpr_P = block_tune<double>(h, Lv.P, k, b->lv[l + 1].x, L.b, L.r);
-            L.lpr_R = block_tune<double>(h, Lv.R, k, L.x, b->lv[l + 1].b, b->lv[l + 1].r);
+            L.code_A[0] = block_tune<double>(h, Lv.A, k, L.x, L.b, L.r);
+            L.code_P[0] = block_tune<double>(h, Lv.P, k, b->lv[l + 1].x, L.b, L.r);
+            L.code_R[0] = block_tune<double>(h, Lv.R, k, L.x, b->lv[l + 1].b, b->lv[l + 1].r);
         }
     }
     if (fp32 && !b->R32) {
@@ -602,6 +865,21 @@ static BlockState* block_state(ramd_hier_s* h, int k, bool fp32)
         const size_t m0 = (size_t)h->levels[0].n * k;
         b->R32 = dalloc<float>(m0);
         b->Z32 = dalloc<float>(m0);
+        // tune every level's operators for FP32 vectors on the level vectors
+        for (size_t l = 0; l < h->levels.size(); l++) {
+            const int64_t m = (int64_t)h->levels[l].n * k;
+            for (float* v : {b->lv[l].x32, b->lv[l].b32, b->lv[l].t32, b->lv[l].r32})
+                hipLaunchKernelGGL(bfill32, dim3(256), dim3(256), 0, h->ctx->stream, m, 1.0f, v);
+            RAMD_HIP(hipGetLastError());
+        }
+        for (size_t l = 0; l + 1 < h->levels.size(); l++) {
+            Level& Lv = h->levels[l];
+            BlockLevel& L = b->lv[l];
+            BlockLevel& C = b->lv[l + 1];
+            L.code_A[1] = block_tune<float>(h, Lv.A, k, L.x32, L.b32, L.r32);
+            L.code_P[1] = block_tune<float>(h, Lv.P, k, C.x32, L.b32, L.r32);
+            L.code_R[1] = block_tune<float>(h, Lv.R, k, L.x32, C.b32, C.r32);
+        }
     }
     return b;
 }
@@ -656,40 +934,32 @@ static void bcycle(ramd_hier_s* h, BlockState* bs, int l, X* x, const X* b)
     else diag = L.diag32;
     for (int it = 0; it < sweeps; it++) {
         if (it == 0) {
-            hipLaunchKernelGGL((bjacobi_from_zero<X>), grid1((int64_t)L.n * k), dim3(256), 0, s, L.n, k, diag, b, cur,
-                               w);
+            const int kp = pow2_at_least(k);
+            hipLaunchKernelGGL((bjacobi_from_zero<X>), dim3(col_blocks(L.n, kp)), dim3(256), 0, s, L.n, k, kp, diag,
+                               b, cur, w);
             RAMD_HIP(hipGetLastError());
         } else {
-            blaunch<X>(L.A, BL.lpr_A, OP_JACOBI, k, cur, b, other, w, s);
+            blaunch<X>(L.A, BL.code_A[(sizeof(X) == 4 ? 1 : 0)], OP_JACOBI, k, cur, b, other, w, s);
             std::swap(cur, other);
         }
     }
-    blaunch<X>(L.A, BL.lpr_A, OP_RESIDUAL, k, cur, b, lr, X(0), s);
+    blaunch<X>(L.A, BL.code_A[(sizeof(X) == 4 ? 1 : 0)], OP_RESIDUAL, k, cur, b, lr, X(0), s);
     BlockLevel& BC = bs->lv[l + 1];
     X *cx, *cb, *ct, *cr;
     bvec<X>(BC, cx, cb, ct, cr);
     (void)ct;
     (void)cr;
-    blaunch<X>(L.R, BL.lpr_R, OP_SPMV, k, lr, (const X*)nullptr, cb, X(0), s);
+    blaunch<X>(L.R, BL.code_R[(sizeof(X) == 4 ? 1 : 0)], OP_SPMV, k, lr, (const X*)nullptr, cb, X(0), s);
     bcycle<X>(h, bs, l + 1, cx, cb);
-    blaunch<X>(L.P, BL.lpr_P, OP_ADD_INTO, k, cx, cur, other, X(0), s);
+    blaunch<X>(L.P, BL.code_P[(sizeof(X) == 4 ? 1 : 0)], OP_ADD_INTO, k, cx, cur, other, X(0), s);
     std::swap(cur, other);
     for (int it = 0; it < sweeps; it++) {
-        blaunch<X>(L.A, BL.lpr_A, OP_JACOBI, k, cur, b, other, w, s);
+        blaunch<X>(L.A, BL.code_A[(sizeof(X) == 4 ? 1 : 0)], OP_JACOBI, k, cur, b, other, w, s);
         std::swap(cur, other);
     }
     if (cur != x) RAMD_HIP(hipMemcpyAsync(x, cur, sizeof(X) * (size_t)L.n * k, hipMemcpyDeviceToDevice, s));
 }
 
-static int pow2_at_least(int k)
-{
-    int p = 1;
-    while (p < k) p <<= 1;
-    return p;
-}
-
-// workgroups of the per-column kernels for n rows
-static int col_blocks(int n, int kp) { return std::max(1, std::min(kColBlocks, (n + 256 / kp - 1) / (256 / kp))); }
 
 static void col_dots(ramd_ctx ctx, BlockState* bs, int n, int mode, const double* a, const void* z, bool z32,
                      const double* w, double* out1, double* out2)
@@ -710,8 +980,8 @@ static void col_dots(ramd_ctx ctx, BlockState* bs, int n, int mode, const double
                                 bs->partials);
     }
     RAMD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(bcol_final, dim3(1), dim3(64), 0, s, g, k, bs->partials, out1);
-    if (mode == 1) hipLaunchKernelGGL(bcol_final, dim3(1), dim3(64), 0, s, g, k, bs->partials + (size_t)g * k, out2);
+    hipLaunchKernelGGL(bcol_final, dim3(k), dim3(256), 0, s, g, k, bs->partials, out1);
+    if (mode == 1) hipLaunchKernelGGL(bcol_final, dim3(k), dim3(256), 0, s, g, k, bs->partials + (size_t)g * k, out2);
     RAMD_HIP(hipGetLastError());
 }
 
@@ -720,8 +990,8 @@ static void col_dots(ramd_ctx ctx, BlockState* bs, int n, int mode, const double
 static void bouter(ramd_hier_s* h, BlockState* bs, const ramd_csr_s* A, int op, const double* x, const double* b,
                    double* y)
 {
-    const int lpr = bs->lv.size() > 1 ? bs->lv[0].lpr_A : (bs->k <= 8 ? 8 : 32);
-    blaunch<double>(A, lpr, op, bs->k, x, b, y, 0.0, h->ctx->stream);
+    const int code = bs->lv.size() > 1 ? bs->lv[0].code_A[0] : (bs->k <= 8 ? 8 : 32);
+    blaunch<double>(A, code, op, bs->k, x, b, y, 0.0, h->ctx->stream);
 }
 
 static void bprecondition(ramd_hier_s* h, BlockState* bs, bool f32)
@@ -799,7 +1069,7 @@ static int fpcg_block_run(ramd_hier_s* h, ramd_csr_s* A, int k, double* X_d, con
             hipLaunchKernelGGL(bscal_alpha, dim3(1), dim3(64), 0, s, k, S);
             hipLaunchKernelGGL(bupdate, dim3(gc), dim3(256), 0, s, n, k, kp, S.alpha, bs->P, bs->AP, bs->X, bs->R,
                                f32 ? bs->R32 : nullptr, bs->partials);
-            hipLaunchKernelGGL(bcol_final, dim3(1), dim3(64), 0, s, gc, k, bs->partials, S.rr);
+            hipLaunchKernelGGL(bcol_final, dim3(k), dim3(256), 0, s, gc, k, bs->partials, S.rr);
             hipLaunchKernelGGL(bscal_relres, dim3(1), dim3(64), 0, s, k, rtol, S);
             RAMD_HIP(hipGetLastError());
         };
@@ -901,18 +1171,51 @@ int ramd_vcycle_block(ramd_hier h, int32_t k, int prec, void* X_d, const void* B
     });
 }
 
-int ramd_block_tuning(ramd_hier h, int32_t k, int32_t* lpr)
+int ramd_block_tuning(ramd_hier h, int32_t k, int32_t prec, int32_t* codes)
 {
     return guarded([&] {
-        RAMD_CHECK(h && lpr, RAMD_E_ARG, "ramd_block_tuning: NULL argument");
+        RAMD_CHECK(h && codes, RAMD_E_ARG, "ramd_block_tuning: NULL argument");
         RAMD_CHECK(k >= 1 && k <= kMaxRhs, RAMD_E_ARG, "ramd_block_tuning: k = %d", k);
+        RAMD_CHECK(prec == RAMD_PREC_FP64 || prec == RAMD_PREC_FP32, RAMD_E_ARG, "ramd_block_tuning: precision %d",
+                   prec);
         h->ctx->set_current();
-        BlockState* bs = block_state(h, k, false);
+        const int p = prec == RAMD_PREC_FP32 ? 1 : 0;
+        BlockState* bs = block_state(h, k, p == 1);
         RAMD_HIP(hipStreamSynchronize(h->ctx->stream));
         for (size_t l = 0; l < h->levels.size(); l++) {
-            lpr[3 * l] = bs->lv[l].lpr_A;
-            lpr[3 * l + 1] = bs->lv[l].lpr_P;
-            lpr[3 * l + 2] = bs->lv[l].lpr_R;
+            codes[3 * l] = bs->lv[l].code_A[p];
+            codes[3 * l + 1] = bs->lv[l].code_P[p];
+            codes[3 * l + 2] = bs->lv[l].code_R[p];
+        }
+    });
+}
+
+int ramd_block_set_tuning(ramd_hier h, int32_t k, int32_t prec, const int32_t* codes)
+{
+    return guarded([&] {
+        RAMD_CHECK(h && codes, RAMD_E_ARG, "ramd_block_set_tuning: NULL argument");
+        RAMD_CHECK(k >= 1 && k <= kMaxRhs, RAMD_E_ARG, "ramd_block_set_tuning: k = %d", k);
+        RAMD_CHECK(prec == RAMD_PREC_FP64 || prec == RAMD_PREC_FP32, RAMD_E_ARG,
+                   "ramd_block_set_tuning: precision %d", prec);
+        h->ctx->set_current();
+        const int p = prec == RAMD_PREC_FP32 ? 1 : 0;
+        const size_t nl = h->levels.size();
+        for (size_t l = 0; l + 1 < nl; l++)
+            for (int m = 0; m < 3; m++) {
+                const int c = codes[3 * l + m];
+                const bool ok = c > 0 ? lpr_ok(c, k)
+                                      : (c < 0 && (p ? vec_ok<float>(-c, k) : vec_ok<double>(-c, k)) &&
+                                         (-c == 2 || -c == 4 || -c == 8 || -c == 16 || -c == 32));
+                RAMD_CHECK(ok && (c < 0 || c == 8 || c == 16 || c == 32 || c == 64), RAMD_E_ARG,
+                           "ramd_block_set_tuning: code %d for k = %d at level %zu", c, k, l);
+            }
+        BlockState* bs = block_state(h, k, p == 1);
+        RAMD_HIP(hipStreamSynchronize(h->ctx->stream));
+        free_graphs(bs);  // captured launches used the old kernels
+        for (size_t l = 0; l + 1 < nl; l++) {
+            bs->lv[l].code_A[p] = codes[3 * l];
+            bs->lv[l].code_P[p] = codes[3 * l + 1];
+            bs->lv[l].code_R[p] = codes[3 * l + 2];
         }
     });
 }

@@ -61,14 +61,57 @@ def test_block_vcycle_columns_bit_exact(problem, ctx, k):
     for c in range(k):
         x = ra.ParVector(ctx, n, np.float32)
         b = ra.ParVector.from_numpy(ctx, B32[:, c].copy())
-        _lib.check(_lib.lib().ramd_vcycle_fp32(ml.h, x.ptr, b.ptr))
+        ml.cycle_fp32(x, b)
         assert np.array_equal(X32h[:, c], x.numpy()), ("fp32", c)
 
 
-def test_block_tuning_choices(problem):
-    t = ra.block_tuning(problem["ml"], 32)
+@pytest.mark.parametrize("kind", ["scalar", "units"])
+def test_block_vcycle_kernel_variants_bit_exact(problem, ctx, kind):
+    """k = 32 with every operator forced onto the one-column-per-lane-step
+    kernel or onto the 16-byte-unit kernel: columns still bit-identical to
+    the single-vector cycles (FP64 and FP32).  The tuned choice is restored."""
+    ml, n = problem["ml"], problem["n"]
+    k = 32
+    B = np.random.default_rng(41).standard_normal((n, k))
+    from raptor_amd import _lib
+    for prec, units_lpr in ((ra.PREC_FP64, -8), (ra.PREC_FP32, -4)):
+        tuned = ra.block_tuning(ml, k, prec)
+        codes = np.zeros_like(tuned)
+        codes[:-1] = 32 if kind == "scalar" else units_lpr
+        ra.block_set_tuning(ml, k, codes, prec)
+        try:
+            if prec == ra.PREC_FP64:
+                X = ra.ParVector(ctx, n * k)
+                ra.vcycle_block(ml, X, ra.ParVector.from_numpy(ctx, B.ravel()), k, prec)
+                Xh = X.numpy().reshape(n, k)
+                for c in (0, 13, 31):
+                    x = ra.ParVector.from_numpy(ctx, np.zeros(n))
+                    bc = ra.ParVector.from_numpy(ctx, B[:, c].copy())
+                    ml.cycle(x, bc, x_is_zero=True)
+                    assert np.array_equal(Xh[:, c], x.numpy()), (kind, "fp64", c)
+            else:
+                B32 = B.astype(np.float32)
+                X32 = ra.ParVector(ctx, n * k, np.float32)
+                ra.vcycle_block(ml, X32, ra.ParVector.from_numpy(ctx, B32.ravel()), k, prec)
+                X32h = X32.numpy().reshape(n, k)
+                for c in (0, 13, 31):
+                    x = ra.ParVector(ctx, n, np.float32)
+                    bc = ra.ParVector.from_numpy(ctx, B32[:, c].copy())
+                    ml.cycle_fp32(x, bc)
+                    assert np.array_equal(X32h[:, c], x.numpy()), (kind, "fp32", c)
+        finally:
+            ra.block_set_tuning(ml, k, tuned, prec)
+        assert np.array_equal(ra.block_tuning(ml, k, prec), tuned)
+
+
+@pytest.mark.parametrize("prec", [ra.PREC_FP64, ra.PREC_FP32])
+def test_block_tuning_choices(problem, prec):
+    """k = 32: every operator got a kernel the launcher knows (lanes per row of
+    the scalar kernel, or minus those of the 16-byte-unit kernel)."""
+    t = ra.block_tuning(problem["ml"], 32, prec)
     assert t.shape == (problem["ml"].num_levels, 3)
-    assert set(np.unique(t[:-1])) <= {8, 16, 32}
+    vec = {-2, -4, -8} if prec == ra.PREC_FP32 else {-4, -8, -16}
+    assert set(np.unique(t[:-1])) <= {8, 16, 32} | vec
     assert np.all(t[-1] == 0)
 
 
