@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void bdia_kernel(int n, int k, DiaArgs a, int 
         const int d = live ? a.ord[kk] : 0;
         const bool on = live && ((m >> d) & 1u);
         const X v = X(val[dia_index<V>(live ? kk : 0, row, n)]);
-        const size_t oj = (size_t)(on ? row + a.off[d] : row) * k;
+        const size_t oj = (size_t)dia_col(row, a.off[d], n) * k;
         if (OP == OP_JACOBI && d == center) {
             diag = on ? v : diag;
             continue;
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void bdia_kernel(int n, int k, DiaArgs a, int 
 // lines.  Every column still runs over the row in stored order with the
 // scalar kernels' operation sequence, so the bits are the same.
 // ---------------------------------------------------------------------------
-constexpr int kMaxUnits = 4;
+constexpr int kMaxUnits = 2;
 
 template <typename X>
 struct Unit16;
@@ -265,14 +265,43 @@ __global__ __launch_bounds__(256) void bcsr_vec_kernel(int n, int k, const int* 
     const size_t o = (size_t)row * k;
     X acc[kMaxUnits][VW], diag = 0;
     bvec_init<OP, X, LPR>(acc, g, units, b + o);
-    for (int j = s; j < e; j++) {
-        const int cj = col[j];
-        const X a = X(val[j]);
-        if (OP == OP_JACOBI && cj == row) {
-            diag = a;
-            continue;
+    // EB entries per step: their column/value loads, then their x-row loads,
+    // are in flight together; entries past the row read column 0 and are
+    // skipped
+    constexpr int EB = 4;
+    for (int j0 = s; j0 < e; j0 += EB) {
+        int cj[EB];
+        X a[EB];
+#pragma unroll
+        for (int t = 0; t < EB; t++) {
+            const bool in = j0 + t < e;
+            cj[t] = in ? col[j0 + t] : 0;
+            a[t] = in ? X(val[j0 + t]) : X(0);
         }
-        bvec_step<OP, X, LPR>(acc, g, units, a, true, x + (size_t)cj * k);
+        X xv[EB][kMaxUnits][VW];
+#pragma unroll
+        for (int t = 0; t < EB; t++)
+#pragma unroll
+            for (int u = 0; u < kMaxUnits; u++)
+                if (g + u * LPR < units) unit_load<X>(x + (size_t)cj[t] * k + (g + u * LPR) * VW, xv[t][u]);
+#pragma unroll
+        for (int t = 0; t < EB; t++) {
+            if (j0 + t >= e) continue;
+            if (OP == OP_JACOBI && cj[t] == row) {
+                diag = a[t];
+                continue;
+            }
+#pragma unroll
+            for (int u = 0; u < kMaxUnits; u++) {
+                if (g + u * LPR >= units) continue;
+#pragma unroll
+                for (int v = 0; v < VW; v++) {
+                    const X p = a[t] * xv[t][u][v];
+                    if (OP == OP_RESIDUAL) acc[u][v] -= p;
+                    else acc[u][v] += p;
+                }
+            }
+        }
     }
     bvec_finish<OP, X, LPR>(acc, g, units, x + o, b + o, y + o, diag, s != e && fabs(diag) > X(1e-16), omega);
 }
@@ -303,7 +332,7 @@ __global__ __launch_bounds__(256) void bdia_vec_kernel(int n, int k, DiaArgs a, 
             diag = on ? v : diag;
             continue;
         }
-        bvec_step<OP, X, LPR>(acc, g, units, v, on, x + (size_t)(on ? row + a.off[d] : row) * k);
+        bvec_step<OP, X, LPR>(acc, g, units, v, on, x + (size_t)dia_col(row, a.off[d], n) * k);
     }
     bvec_finish<OP, X, LPR>(acc, g, units, x + o, b + o, y + o, diag, m != 0u && fabs(diag) > X(1e-16), omega);
 }
@@ -441,6 +470,10 @@ static void blaunch(const ramd_csr_s* A, int code, int op, int k, const X* x, co
 constexpr int kColBlocks = 4096;
 
 // first Jacobi sweep from X = 0 (the single-vector jacobi_from_zero per column)
+// The per-column kernels take kColUnroll rows per thread per step (rows
+// i, i + stride, ...) so that many independent loads are in flight.
+constexpr int kColUnroll = 4;
+
 template <typename X>
 __global__ __launch_bounds__(256) void bjacobi_from_zero(int n, int k, int kp, const X* __restrict__ d,
                                                          const X* __restrict__ b, X* __restrict__ x, X omega)
@@ -449,12 +482,23 @@ __global__ __launch_bounds__(256) void bjacobi_from_zero(int n, int k, int kp, c
     const int rpb = 256 / kp;
     if (c >= k) return;
     const X zero = X(0.0);
-    for (int64_t i = (int64_t)blockIdx.x * rpb + threadIdx.x / kp; i < n; i += (int64_t)gridDim.x * rpb) {
-        const size_t e = (size_t)i * k + c;
-        const X di = d[i];  // 0 for a row without a stored diagonal
-        X v = zero;
-        if (fabs(di) > X(1e-16)) v = ((X(1.0) - omega) * zero) + (omega * ((b[e] - zero) / di));
-        x[e] = v;
+    const int64_t stride = (int64_t)gridDim.x * rpb;
+    for (int64_t i0 = (int64_t)blockIdx.x * rpb + threadIdx.x / kp; i0 < n; i0 += kColUnroll * stride) {
+        X di[kColUnroll], bi[kColUnroll];
+#pragma unroll
+        for (int u = 0; u < kColUnroll; u++) {
+            const int64_t i = i0 + u * stride;
+            di[u] = i < n ? d[i] : X(0);  // 0 for a row without a stored diagonal
+            bi[u] = i < n ? b[(size_t)i * k + c] : X(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kColUnroll; u++) {
+            const int64_t i = i0 + u * stride;
+            if (i >= n) break;
+            X v = zero;
+            if (fabs(di[u]) > X(1e-16)) v = ((X(1.0) - omega) * zero) + (omega * ((bi[u] - zero) / di[u]));
+            x[(size_t)i * k + c] = v;
+        }
     }
 }
 
@@ -558,12 +602,24 @@ __global__ __launch_bounds__(256) void bcol_dots(int n, int k, int kp, const dou
     const int c = threadIdx.x & (kp - 1);
     const int rpb = 256 / kp;
     double s1 = 0, s2 = 0;
+    const int64_t stride = (int64_t)gridDim.x * rpb;
     if (c < k)
-        for (int64_t i = (int64_t)blockIdx.x * rpb + threadIdx.x / kp; i < n; i += (int64_t)gridDim.x * rpb) {
-            const size_t e = (size_t)i * k + c;
-            const double zi = (double)z[e];
-            s1 += a[e] * zi;
-            if (MODE == 1) s2 += zi * w[e];
+        for (int64_t i0 = (int64_t)blockIdx.x * rpb + threadIdx.x / kp; i0 < n; i0 += kColUnroll * stride) {
+            double av[kColUnroll], zv[kColUnroll], wv[kColUnroll];
+#pragma unroll
+            for (int u = 0; u < kColUnroll; u++) {
+                const int64_t i = i0 + u * stride;
+                const size_t e = (size_t)(i < n ? i : 0) * k + c;
+                av[u] = i < n ? a[e] : 0.0;
+                zv[u] = i < n ? (double)z[e] : 0.0;
+                wv[u] = (MODE == 1 && i < n) ? w[e] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kColUnroll; u++) {
+                if (i0 + u * stride >= n) break;
+                s1 += av[u] * zv[u];
+                if (MODE == 1) s2 += zv[u] * wv[u];
+            }
         }
     col_partial(s1, k, kp, partials);
     if (MODE == 1) col_partial(s2, k, kp, partials + (size_t)gridDim.x * k);
@@ -578,15 +634,31 @@ __global__ __launch_bounds__(256) void bupdate(int n, int k, int kp, const doubl
     const int c = threadIdx.x & (kp - 1);
     const int rpb = 256 / kp;
     double s = 0;
+    const int64_t stride = (int64_t)gridDim.x * rpb;
     if (c < k) {
         const double al = alpha[c];
-        for (int64_t i = (int64_t)blockIdx.x * rpb + threadIdx.x / kp; i < n; i += (int64_t)gridDim.x * rpb) {
-            const size_t e = (size_t)i * k + c;
-            x[e] += al * p[e];
-            const double ri = r[e] - al * Ap[e];
-            r[e] = ri;
-            if (r32) r32[e] = (float)ri;
-            s += ri * ri;
+        for (int64_t i0 = (int64_t)blockIdx.x * rpb + threadIdx.x / kp; i0 < n; i0 += kColUnroll * stride) {
+            double xv[kColUnroll], pv[kColUnroll], rv[kColUnroll], av[kColUnroll];
+#pragma unroll
+            for (int u = 0; u < kColUnroll; u++) {
+                const int64_t i = i0 + u * stride;
+                const size_t e = (size_t)(i < n ? i : 0) * k + c;
+                xv[u] = i < n ? x[e] : 0.0;
+                pv[u] = i < n ? p[e] : 0.0;
+                rv[u] = i < n ? r[e] : 0.0;
+                av[u] = i < n ? Ap[e] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kColUnroll; u++) {
+                const int64_t i = i0 + u * stride;
+                if (i >= n) break;
+                const size_t e = (size_t)i * k + c;
+                x[e] = xv[u] + al * pv[u];
+                const double ri = rv[u] - al * av[u];
+                r[e] = ri;
+                if (r32) r32[e] = (float)ri;
+                s += ri * ri;
+            }
         }
     }
     col_partial(s, k, kp, partials);
@@ -601,9 +673,22 @@ __global__ __launch_bounds__(256) void bdir(int n, int k, int kp, const double* 
     const int rpb = 256 / kp;
     if (c >= k) return;
     const double be = beta[c];
-    for (int64_t i = (int64_t)blockIdx.x * rpb + threadIdx.x / kp; i < n; i += (int64_t)gridDim.x * rpb) {
-        const size_t e = (size_t)i * k + c;
-        p[e] = (double)z[e] + be * p[e];
+    const int64_t stride = (int64_t)gridDim.x * rpb;
+    for (int64_t i0 = (int64_t)blockIdx.x * rpb + threadIdx.x / kp; i0 < n; i0 += kColUnroll * stride) {
+        double zv[kColUnroll], pv[kColUnroll];
+#pragma unroll
+        for (int u = 0; u < kColUnroll; u++) {
+            const int64_t i = i0 + u * stride;
+            const size_t e = (size_t)(i < n ? i : 0) * k + c;
+            zv[u] = i < n ? (double)z[e] : 0.0;
+            pv[u] = i < n ? p[e] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kColUnroll; u++) {
+            const int64_t i = i0 + u * stride;
+            if (i >= n) break;
+            p[(size_t)i * k + c] = zv[u] + be * pv[u];
+        }
     }
 }
 

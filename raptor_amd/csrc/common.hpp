@@ -128,6 +128,15 @@ __host__ __device__ __forceinline__ size_t dia_index(int slot, int row, int n)
     constexpr int G = 16 / (int)sizeof(V);
     return ((size_t)(slot / G) * (size_t)n + (size_t)row) * G + (size_t)(slot % G);
 }
+// Column read for diagonal offset off of row: row + off clamped into
+// [0, n).  The address does not wait for the row's mask; a clamped (or
+// absent-diagonal) value is only ever multiplied into a product that the
+// mask's select then drops.
+__device__ __forceinline__ int dia_col(int row, int off, int n)
+{
+    const int c = row + off;
+    return c < 0 ? 0 : (c >= n ? n - 1 : c);
+}
 template <typename V>
 __host__ __device__ inline int dia_slots(int nd)
 {

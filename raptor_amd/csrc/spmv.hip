@@ -239,8 +239,9 @@ __global__ __launch_bounds__(kBlockThreads) void sliced_kernel(
 // Stencil-diagonal kernels: one lane per row, diagonals visited in the
 // row's stored order (slot t holds diagonal a.ord[t]).  A lane reads its
 // row's values a 16-byte group of slots at a time (dia_index layout).  Every
-// load is issued unconditionally -- a diagonal absent from the row reads its
-// own zero and x[row] -- and the accumulation skips it with a select, so the
+// load is issued unconditionally and without waiting for the row's mask --
+// a diagonal absent from the row reads its own zero and x at the clamped
+// column (dia_col) -- and the accumulation skips it with a select, so the
 // operation sequence of each row is exactly the CSR row loop's and no
 // out-of-range x is touched.
 
@@ -301,7 +302,7 @@ __device__ __forceinline__ X dia_row(int row, int n, const DiaArgs& a, int nd, i
                     const bool live = (ND == kDiaMax) ? (kk < nd) : true;
                     const int k = live ? a.ord[kk] : 0;
                     on[t] = live && ((m >> k) & 1u);
-                    xx[t] = x[on[t] ? row + a.off[k] : row];
+                    xx[t] = x[dia_col(row, a.off[k], n)];
                 }
             }
 #pragma unroll
@@ -354,7 +355,7 @@ __global__ __launch_bounds__(kBlockThreads) void dia_kernel(int n, DiaArgs a, in
                     const int k = live ? a.ord[kk] : 0;
                     kd[t] = k;
                     on[t] = live && ((m >> k) & 1u);
-                    xx[t] = x[on[t] ? row + a.off[k] : row];
+                    xx[t] = x[dia_col(row, a.off[k], n)];
                 }
             }
 #pragma unroll
