@@ -141,10 +141,13 @@ __global__ __launch_bounds__(256) void bdia_kernel(int n, int k, DiaArgs a, int 
 // ---------------------------------------------------------------------------
 // Vectorized SpMM: the k columns of a row are read and written as 16-byte
 // units of VW = 16 / sizeof(X) consecutive columns (k a multiple of VW); lane
-// g of a row's LPR lanes owns units g, g + LPR, ... (at most kMaxUnits).  A
-// lane group's load of one x row is then one contiguous run of whole cache
-// lines.  Every column still runs over the row in stored order with the
-// scalar kernels' operation sequence, so the bits are the same.
+// g of a row's LPR lanes owns units g, g + LPR, ... (UPL of them, the last
+// possibly past the row's units: it then re-reads unit g and stores
+// nothing).  A lane group's load of one x row is one contiguous run of whole
+// cache lines, and every load of a step (matrix entries, then x units) is
+// issued before the first use.  Every column runs over the row in stored
+// order with the scalar kernels' operation sequence, so the bits are the
+// same.
 // ---------------------------------------------------------------------------
 constexpr int kMaxUnits = 2;
 
@@ -180,76 +183,73 @@ __device__ __forceinline__ void unit_store(X* __restrict__ p, const X (&v)[16 / 
     *reinterpret_cast<XV*>(p) = t;
 }
 
+// A lane's units: column offset of each (a valid one for every u) and
+// whether it exists.
+template <typename X, int LPR, int UPL>
+struct Units {
+    static constexpr int VW = 16 / sizeof(X);
+    int off[UPL];
+    bool on[UPL];
+    __device__ __forceinline__ Units(int g, int units)
+    {
+#pragma unroll
+        for (int u = 0; u < UPL; u++) {
+            const int un = g + u * LPR;
+            on[u] = un < units;
+            off[u] = (on[u] ? un : g) * VW;
+        }
+    }
+};
+
 // accumulators <- b (residual) or 0
-template <int OP, typename X, int LPR>
-__device__ __forceinline__ void bvec_init(X (&acc)[kMaxUnits][16 / sizeof(X)], int g, int units,
+template <int OP, typename X, int LPR, int UPL>
+__device__ __forceinline__ void bvec_init(X (&acc)[UPL][16 / sizeof(X)], const Units<X, LPR, UPL>& U,
                                           const X* __restrict__ brow)
 {
     constexpr int VW = 16 / sizeof(X);
 #pragma unroll
-    for (int u = 0; u < kMaxUnits; u++) {
-        const int un = g + u * LPR;
+    for (int u = 0; u < UPL; u++) {
+        if (OP == OP_RESIDUAL) {
+            unit_load<X>(brow + U.off[u], acc[u]);
+        } else {
 #pragma unroll
-        for (int t = 0; t < VW; t++) acc[u][t] = X(0);
-        if (OP == OP_RESIDUAL && un < units) unit_load<X>(brow + un * VW, acc[u]);
-    }
-}
-
-// one stored entry a * x_row for every owned column (on: the entry exists)
-template <int OP, typename X, int LPR>
-__device__ __forceinline__ void bvec_step(X (&acc)[kMaxUnits][16 / sizeof(X)], int g, int units, X a, bool on,
-                                          const X* __restrict__ xrow)
-{
-    constexpr int VW = 16 / sizeof(X);
-#pragma unroll
-    for (int u = 0; u < kMaxUnits; u++) {
-        const int un = g + u * LPR;
-        if (un < units) {
-            X xv[VW];
-            unit_load<X>(xrow + un * VW, xv);
-#pragma unroll
-            for (int t = 0; t < VW; t++) {
-                const X p = a * xv[t];
-                if (OP == OP_RESIDUAL) acc[u][t] = on ? acc[u][t] - p : acc[u][t];
-                else acc[u][t] = on ? acc[u][t] + p : acc[u][t];
-            }
+            for (int t = 0; t < VW; t++) acc[u][t] = X(0);
         }
     }
 }
 
 // y row from the accumulators (Jacobi: usable = the row has a usable diagonal)
-template <int OP, typename X, int LPR>
-__device__ __forceinline__ void bvec_finish(const X (&acc)[kMaxUnits][16 / sizeof(X)], int g, int units,
+template <int OP, typename X, int LPR, int UPL>
+__device__ __forceinline__ void bvec_finish(const X (&acc)[UPL][16 / sizeof(X)], const Units<X, LPR, UPL>& U,
                                             const X* __restrict__ xrow, const X* __restrict__ brow,
                                             X* __restrict__ yrow, X diag, bool usable, X omega)
 {
     constexpr int VW = 16 / sizeof(X);
 #pragma unroll
-    for (int u = 0; u < kMaxUnits; u++) {
-        const int un = g + u * LPR;
-        if (un >= units) continue;
+    for (int u = 0; u < UPL; u++) {
+        if (!U.on[u]) continue;
         X out[VW];
         if (OP == OP_SPMV || OP == OP_RESIDUAL) {
 #pragma unroll
             for (int t = 0; t < VW; t++) out[t] = acc[u][t];
         } else if (OP == OP_ADD_INTO) {
             X bv[VW];
-            unit_load<X>(brow + un * VW, bv);
+            unit_load<X>(brow + U.off[u], bv);
 #pragma unroll
             for (int t = 0; t < VW; t++) out[t] = bv[t] + acc[u][t];
         } else {
             X xo[VW], bv[VW];
-            unit_load<X>(xrow + un * VW, xo);
-            unit_load<X>(brow + un * VW, bv);
+            unit_load<X>(xrow + U.off[u], xo);
+            unit_load<X>(brow + U.off[u], bv);
 #pragma unroll
             for (int t = 0; t < VW; t++)
                 out[t] = usable ? ((X(1.0) - omega) * xo[t]) + (omega * ((bv[t] - acc[u][t]) / diag)) : xo[t];
         }
-        unit_store<X>(yrow + un * VW, out);
+        unit_store<X>(yrow + U.off[u], out);
     }
 }
 
-template <int OP, typename V, typename X, int LPR>
+template <int OP, typename V, typename X, int LPR, int UPL>
 __global__ __launch_bounds__(256) void bcsr_vec_kernel(int n, int k, const int* __restrict__ rp,
                                                        const int* __restrict__ col, const V* __restrict__ val,
                                                        const X* __restrict__ x, const X* __restrict__ b,
@@ -257,56 +257,51 @@ __global__ __launch_bounds__(256) void bcsr_vec_kernel(int n, int k, const int* 
 {
     constexpr int VW = 16 / sizeof(X);
     constexpr int RPW = 256 / LPR;
+    constexpr int EB = 8 / UPL;  // entries per step
     const int row = xcd_remap(blockIdx.x, gridDim.x) * RPW + threadIdx.x / LPR;
     const int g = threadIdx.x % LPR;
     const int units = k / VW;
     if (row >= n || g >= units) return;
+    const Units<X, LPR, UPL> U(g, units);
     const int s = rp[row], e = rp[row + 1];
     const size_t o = (size_t)row * k;
-    X acc[kMaxUnits][VW], diag = 0;
-    bvec_init<OP, X, LPR>(acc, g, units, b + o);
-    // EB entries per step: their column/value loads, then their x-row loads,
-    // are in flight together; entries past the row read column 0 and are
-    // skipped
-    constexpr int EB = 4;
+    X acc[UPL][VW], diag = 0;
+    bvec_init<OP, X, LPR, UPL>(acc, U, b + o);
     for (int j0 = s; j0 < e; j0 += EB) {
         int cj[EB];
         X a[EB];
 #pragma unroll
         for (int t = 0; t < EB; t++) {
             const bool in = j0 + t < e;
-            cj[t] = in ? col[j0 + t] : 0;
+            cj[t] = in ? col[j0 + t] : 0;  // past the row: column 0, skipped below
             a[t] = in ? X(val[j0 + t]) : X(0);
         }
-        X xv[EB][kMaxUnits][VW];
+        X xv[EB][UPL][VW];
 #pragma unroll
         for (int t = 0; t < EB; t++)
 #pragma unroll
-            for (int u = 0; u < kMaxUnits; u++)
-                if (g + u * LPR < units) unit_load<X>(x + (size_t)cj[t] * k + (g + u * LPR) * VW, xv[t][u]);
+            for (int u = 0; u < UPL; u++) unit_load<X>(x + (size_t)cj[t] * k + U.off[u], xv[t][u]);
 #pragma unroll
         for (int t = 0; t < EB; t++) {
-            if (j0 + t >= e) continue;
+            if (j0 + t >= e) break;
             if (OP == OP_JACOBI && cj[t] == row) {
                 diag = a[t];
                 continue;
             }
 #pragma unroll
-            for (int u = 0; u < kMaxUnits; u++) {
-                if (g + u * LPR >= units) continue;
+            for (int u = 0; u < UPL; u++)
 #pragma unroll
                 for (int v = 0; v < VW; v++) {
                     const X p = a[t] * xv[t][u][v];
                     if (OP == OP_RESIDUAL) acc[u][v] -= p;
                     else acc[u][v] += p;
                 }
-            }
         }
     }
-    bvec_finish<OP, X, LPR>(acc, g, units, x + o, b + o, y + o, diag, s != e && fabs(diag) > X(1e-16), omega);
+    bvec_finish<OP, X, LPR, UPL>(acc, U, x + o, b + o, y + o, diag, s != e && fabs(diag) > X(1e-16), omega);
 }
 
-template <int OP, typename V, typename X, int LPR, int ND>
+template <int OP, typename V, typename X, int LPR, int UPL, int ND>
 __global__ __launch_bounds__(256) void bdia_vec_kernel(int n, int k, DiaArgs a, int nd, int center,
                                                        const unsigned* __restrict__ mask, const V* __restrict__ val,
                                                        const X* __restrict__ x, const X* __restrict__ b,
@@ -314,30 +309,56 @@ __global__ __launch_bounds__(256) void bdia_vec_kernel(int n, int k, DiaArgs a, 
 {
     constexpr int VW = 16 / sizeof(X);
     constexpr int RPW = 256 / LPR;
+    constexpr int DCH = 9;  // diagonals per step
     const int row = xcd_remap(blockIdx.x, gridDim.x) * RPW + threadIdx.x / LPR;
     const int g = threadIdx.x % LPR;
     const int units = k / VW;
     if (row >= n || g >= units) return;
+    const Units<X, LPR, UPL> U(g, units);
     const unsigned m = mask[row];
     const size_t o = (size_t)row * k;
-    X acc[kMaxUnits][VW], diag = 0;
-    bvec_init<OP, X, LPR>(acc, g, units, b + o);
+    X acc[UPL][VW], diag = 0;
+    bvec_init<OP, X, LPR, UPL>(acc, U, b + o);
 #pragma unroll
-    for (int kk = 0; kk < ND; kk++) {
-        const bool live = (ND == kDiaMax) ? (kk < nd) : true;
-        const int d = live ? a.ord[kk] : 0;
-        const bool on = live && ((m >> d) & 1u);
-        const X v = X(val[dia_index<V>(live ? kk : 0, row, n)]);
-        if (OP == OP_JACOBI && d == center) {
-            diag = on ? v : diag;
-            continue;
+    for (int c0 = 0; c0 < ND; c0 += DCH) {
+        X vv[DCH];
+        int dd[DCH];
+        X xv[DCH][UPL][VW];
+#pragma unroll
+        for (int t = 0; t < DCH; t++) {
+            const int kk = c0 + t;
+            if (kk < ND) {
+                const bool live = (ND == kDiaMax) ? (kk < nd) : true;
+                dd[t] = live ? a.ord[kk] : -1;
+                vv[t] = X(val[dia_index<V>(live ? kk : 0, row, n)]);
+                const X* xr = x + (size_t)dia_col(row, a.off[live ? dd[t] : 0], n) * k;
+#pragma unroll
+                for (int u = 0; u < UPL; u++) unit_load<X>(xr + U.off[u], xv[t][u]);
+            }
         }
-        bvec_step<OP, X, LPR>(acc, g, units, v, on, x + (size_t)dia_col(row, a.off[d], n) * k);
+#pragma unroll
+        for (int t = 0; t < DCH; t++) {
+            if (c0 + t >= ND) break;
+            const int d = dd[t];
+            const bool on = d >= 0 && ((m >> d) & 1u);
+            if (OP == OP_JACOBI && d == center) {
+                diag = on ? vv[t] : diag;
+                continue;
+            }
+#pragma unroll
+            for (int u = 0; u < UPL; u++)
+#pragma unroll
+                for (int v = 0; v < VW; v++) {
+                    const X p = vv[t] * xv[t][u][v];
+                    if (OP == OP_RESIDUAL) acc[u][v] = on ? acc[u][v] - p : acc[u][v];
+                    else acc[u][v] = on ? acc[u][v] + p : acc[u][v];
+                }
+        }
     }
-    bvec_finish<OP, X, LPR>(acc, g, units, x + o, b + o, y + o, diag, m != 0u && fabs(diag) > X(1e-16), omega);
+    bvec_finish<OP, X, LPR, UPL>(acc, U, x + o, b + o, y + o, diag, m != 0u && fabs(diag) > X(1e-16), omega);
 }
 
-template <int OP, typename V, typename X, int LPR>
+template <int OP, typename V, typename X, int LPR, int UPL>
 static void blaunch_vec_op(const ramd_csr_s* A, const V* val, const V* dval, int k, const X* x, const X* b, X* y,
                            X omega, hipStream_t s)
 {
@@ -346,17 +367,17 @@ static void blaunch_vec_op(const ramd_csr_s* A, const V* val, const V* dval, int
     const dim3 grid((unsigned)((n + 256 / LPR - 1) / (256 / LPR))), block(256);
     if (dval) {
         const Dia* d = A->dia;
-#define BDIAV(NDV)                                                                                                 \
-    hipLaunchKernelGGL((bdia_vec_kernel<OP, V, X, LPR, NDV>), grid, block, 0, s, n, k, d->args, d->nd, d->center, \
-                       d->mask, dval, x, b, y, omega)
+#define BDIAV(NDV)                                                                                            \
+    hipLaunchKernelGGL((bdia_vec_kernel<OP, V, X, LPR, UPL, NDV>), grid, block, 0, s, n, k, d->args, d->nd, \
+                       d->center, d->mask, dval, x, b, y, omega)
         if (d->nd == 27) BDIAV(27);
         else if (d->nd == 9) BDIAV(9);
         else if (d->nd == 7) BDIAV(7);
         else BDIAV(kDiaMax);
 #undef BDIAV
     } else {
-        hipLaunchKernelGGL((bcsr_vec_kernel<OP, V, X, LPR>), grid, block, 0, s, n, k, A->rowptr, A->col, val, x, b,
-                           y, omega);
+        hipLaunchKernelGGL((bcsr_vec_kernel<OP, V, X, LPR, UPL>), grid, block, 0, s, n, k, A->rowptr, A->col, val,
+                           x, b, y, omega);
     }
     RAMD_HIP(hipGetLastError());
 }
@@ -365,13 +386,24 @@ template <typename V, typename X, int LPR>
 static void blaunch_vec_lpr(const ramd_csr_s* A, const V* val, const V* dval, int op, int k, const X* x,
                             const X* b, X* y, X omega, hipStream_t s)
 {
-    switch (op) {
-    case OP_SPMV: blaunch_vec_op<OP_SPMV, V, X, LPR>(A, val, dval, k, x, b, y, omega, s); break;
-    case OP_RESIDUAL: blaunch_vec_op<OP_RESIDUAL, V, X, LPR>(A, val, dval, k, x, b, y, omega, s); break;
-    case OP_JACOBI: blaunch_vec_op<OP_JACOBI, V, X, LPR>(A, val, dval, k, x, b, y, omega, s); break;
-    case OP_ADD_INTO: blaunch_vec_op<OP_ADD_INTO, V, X, LPR>(A, val, dval, k, x, b, y, omega, s); break;
-    default: fail(RAMD_E_ARG, "block op %d not supported", op);
+    constexpr int VW = 16 / sizeof(X);
+    const int upl = (k / VW + LPR - 1) / LPR;
+#define BVEC(UPLV)                                                                                            \
+    switch (op) {                                                                                            \
+    case OP_SPMV: blaunch_vec_op<OP_SPMV, V, X, LPR, UPLV>(A, val, dval, k, x, b, y, omega, s); break;       \
+    case OP_RESIDUAL: blaunch_vec_op<OP_RESIDUAL, V, X, LPR, UPLV>(A, val, dval, k, x, b, y, omega, s); break; \
+    case OP_JACOBI: blaunch_vec_op<OP_JACOBI, V, X, LPR, UPLV>(A, val, dval, k, x, b, y, omega, s); break;   \
+    case OP_ADD_INTO: blaunch_vec_op<OP_ADD_INTO, V, X, LPR, UPLV>(A, val, dval, k, x, b, y, omega, s); break; \
+    default: fail(RAMD_E_ARG, "block op %d not supported", op);                                              \
     }
+    if (upl == 1) {
+        BVEC(1)
+    } else if (upl == 2) {
+        BVEC(2)
+    } else {
+        fail(RAMD_E_ARG, "k = %d needs %d units per lane at %d lanes per row", k, upl, LPR);
+    }
+#undef BVEC
 }
 
 template <int OP, typename V, typename X, int LPR>
@@ -452,11 +484,9 @@ static void blaunch(const ramd_csr_s* A, int code, int op, int k, const X* x, co
     case 16: blaunch_lpr<V, X, 16>(A, val, dval, op, k, x, b, y, omega, s); break;
     case 32: blaunch_lpr<V, X, 32>(A, val, dval, op, k, x, b, y, omega, s); break;
     case 64: blaunch_lpr<V, X, 64>(A, val, dval, op, k, x, b, y, omega, s); break;
-    case -2: blaunch_vec_lpr<V, X, 2>(A, val, dval, op, k, x, b, y, omega, s); break;
     case -4: blaunch_vec_lpr<V, X, 4>(A, val, dval, op, k, x, b, y, omega, s); break;
     case -8: blaunch_vec_lpr<V, X, 8>(A, val, dval, op, k, x, b, y, omega, s); break;
     case -16: blaunch_vec_lpr<V, X, 16>(A, val, dval, op, k, x, b, y, omega, s); break;
-    case -32: blaunch_vec_lpr<V, X, 32>(A, val, dval, op, k, x, b, y, omega, s); break;
     default: fail(RAMD_E_ARG, "block kernel code %d", code);
     }
 }
@@ -874,7 +904,7 @@ static int block_tune(ramd_hier_s* h, const ramd_csr_s* M, int k, const X* x, co
     for (int lpr : {8, 16, 32, 64})
         if (lpr_ok(lpr, k)) codes.push_back(lpr);
     if (aligned16(x) && aligned16(b) && aligned16(y))
-        for (int lpr : {2, 4, 8, 16, 32})
+        for (int lpr : {4, 8, 16})
             if (vec_ok<X>(lpr, k)) codes.push_back(-lpr);
     for (int code : codes) {
         const double ms = time_ms(
@@ -1290,7 +1320,7 @@ int ramd_block_set_tuning(ramd_hier h, int32_t k, int32_t prec, const int32_t* c
                 const int c = codes[3 * l + m];
                 const bool ok = c > 0 ? lpr_ok(c, k)
                                       : (c < 0 && (p ? vec_ok<float>(-c, k) : vec_ok<double>(-c, k)) &&
-                                         (-c == 2 || -c == 4 || -c == 8 || -c == 16 || -c == 32));
+                                         (-c == 4 || -c == 8 || -c == 16));
                 RAMD_CHECK(ok && (c < 0 || c == 8 || c == 16 || c == 32 || c == 64), RAMD_E_ARG,
                            "ramd_block_set_tuning: code %d for k = %d at level %zu", c, k, l);
             }

@@ -110,7 +110,7 @@ def test_block_tuning_choices(problem, prec):
     the scalar kernel, or minus those of the 16-byte-unit kernel)."""
     t = ra.block_tuning(problem["ml"], 32, prec)
     assert t.shape == (problem["ml"].num_levels, 3)
-    vec = {-2, -4, -8} if prec == ra.PREC_FP32 else {-4, -8, -16}
+    vec = {-4, -8, -16}
     assert set(np.unique(t[:-1])) <= {8, 16, 32} | vec
     assert np.all(t[-1] == 0)
 
