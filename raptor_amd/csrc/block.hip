@@ -267,20 +267,31 @@ __global__ __launch_bounds__(256) void bcsr_vec_kernel(int n, int k, const int* 
     const size_t o = (size_t)row * k;
     X acc[UPL][VW], diag = 0;
     bvec_init<OP, X, LPR, UPL>(acc, U, b + o);
-    for (int j0 = s; j0 < e; j0 += EB) {
-        int cj[EB];
-        X a[EB];
+    // entries of the first step; each step then issues its x loads and the
+    // next step's entry loads before adding (entries past the row read
+    // column 0 and are skipped)
+    int cj[EB];
+    X a[EB];
 #pragma unroll
-        for (int t = 0; t < EB; t++) {
-            const bool in = j0 + t < e;
-            cj[t] = in ? col[j0 + t] : 0;  // past the row: column 0, skipped below
-            a[t] = in ? X(val[j0 + t]) : X(0);
-        }
+    for (int t = 0; t < EB; t++) {
+        const bool in = s + t < e;
+        cj[t] = in ? col[s + t] : 0;
+        a[t] = in ? X(val[s + t]) : X(0);
+    }
+    for (int j0 = s; j0 < e; j0 += EB) {
         X xv[EB][UPL][VW];
 #pragma unroll
         for (int t = 0; t < EB; t++)
 #pragma unroll
             for (int u = 0; u < UPL; u++) unit_load<X>(x + (size_t)cj[t] * k + U.off[u], xv[t][u]);
+        int cn[EB];
+        X an[EB];
+#pragma unroll
+        for (int t = 0; t < EB; t++) {
+            const bool in = j0 + EB + t < e;
+            cn[t] = in ? col[j0 + EB + t] : 0;
+            an[t] = in ? X(val[j0 + EB + t]) : X(0);
+        }
 #pragma unroll
         for (int t = 0; t < EB; t++) {
             if (j0 + t >= e) break;
@@ -296,6 +307,11 @@ __global__ __launch_bounds__(256) void bcsr_vec_kernel(int n, int k, const int* 
                     if (OP == OP_RESIDUAL) acc[u][v] -= p;
                     else acc[u][v] += p;
                 }
+        }
+#pragma unroll
+        for (int t = 0; t < EB; t++) {
+            cj[t] = cn[t];
+            a[t] = an[t];
         }
     }
     bvec_finish<OP, X, LPR, UPL>(acc, U, x + o, b + o, y + o, diag, s != e && fabs(diag) > X(1e-16), omega);
