@@ -339,7 +339,8 @@ def bench_batch(args, m, cfg):
     ml.setup(rp, col, val)
     setup_s = time.time() - t
     nl, nnzA, nnzP = ml.level_sizes()
-    log(f"setup {setup_s:.1f}s; levels {list(map(int, nl))}")
+    log(f"setup {setup_s:.1f}s; levels {list(map(int, nl))}; nnz(A) {list(map(int, nnzA))}; "
+        f"nnz(P) {list(map(int, nnzP))}")
     ml.enable_fp32()
     A = ra.ParCSRMatrix.from_csr(ctx, rp, col, val)
     Bh = np.stack([synth.normal(cfg["seed"] + c, n) for c in range(k)], axis=1)
@@ -348,7 +349,9 @@ def bench_batch(args, m, cfg):
     x1 = ra.ParVector(ctx, n)
     cols = [ra.ParVector.from_numpy(ctx, Bh[:, c].copy()) for c in range(k)]
     tuning = ra.block_tuning(ml, k)
-    log(f"lanes per row (A, P, R) by level: {tuning.tolist()}")
+    tuning32 = ra.block_tuning(ml, k, ra.PREC_FP32)
+    log(f"block kernels (A, P, R) by level, lanes per row (negative: 16-byte column units): "
+        f"FP64 {tuning.tolist()}; FP32 {tuning32.tolist()}")
 
     def block(prec):
         X.set_const_value(0.0)
@@ -413,7 +416,8 @@ def bench_batch(args, m, cfg):
         "block_speedup_fp32": round(res["sequential_fp32"]["s"] / res["block_fp32"]["s"], 3),
         "block_speedup_fp64": round(res["sequential_fp64"]["s"] / res["block_fp64"]["s"], 3),
         "worst_true_relres": worst,
-        "lanes_per_row": tuning.tolist(),
+        "block_kernels": dict(fp64=tuning.tolist(), fp32=tuning32.tolist()),
+        "nnz_A": list(map(int, nnzA)), "nnz_P": list(map(int, nnzP)),
     }
     print(json.dumps(out), flush=True)
     ctx.close()
