@@ -279,6 +279,22 @@ int ramd_pcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double tol
 int ramd_cg(ramd_csr A, double* x_d, const double* b_d, double tol, int max_iter, double* res,
             int32_t max_res, int32_t* n_res, int32_t* iters);
 
+/* BiCGStab(A, x, b, res, tol, max_iter), krylov/par_bicgstab.cpp:11-110:
+ * res[k] = ||r_k||_2 (ParVector::norm) from k = 0, stop when ||r|| <=
+ * tol * ||r_0|| or after max_iter iterations (max_iter <= 0: 1.3 n + 2).
+ * Vector updates bit-identical to the reference's passes; inner products
+ * are deterministic tree sums.  RAMD_E_NUMERIC when a breakdown makes the
+ * residual norm non-finite (the reference then stops silently). */
+int ramd_bicgstab(ramd_csr A, double* x_d, const double* b_d, double tol, int32_t max_iter, double* res,
+                  int32_t max_res, int32_t* n_res, int32_t* iters);
+/* Pre_BiCGStab(A, x, b, ml, res, tol, max_iter), krylov/par_bicgstab.cpp:
+ * 240-360: the same with p_hat = M^-1 p and s_hat = M^-1 s, M one FP64
+ * V-cycle of h from zero.  The reference's routine reads an uninitialised
+ * iteration counter and unsized p_hat / s_hat; this is the algorithm it
+ * spells out (counter from 0, vectors of n entries). */
+int ramd_pre_bicgstab(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double tol, int32_t max_iter,
+                      double* res, int32_t max_res, int32_t* n_res, int32_t* iters);
+
 /* Mixed-precision solve: flexible PCG in FP64 (true residual b-Ax, inner
  * products and the coarse LU in FP64) preconditioned by one V-cycle in the
  * precision `cycle_prec`, iterated until ||b-Ax||_2/||b||_2 <= rtol (true,

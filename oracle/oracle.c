@@ -641,3 +641,84 @@ int orc_cg(const orc_csr* A, double* x, const double* b, double tol, int max_ite
     free(r); free(p); free(Ap);
     return iter;
 }
+
+/* ---- BiCGStab ----
+ * krylov/par_bicgstab.cpp:11-110 (BiCGStab) and :240-360 (Pre_BiCGStab,
+ * the V-cycle from zero as the preconditioner on p and s) at one rank.
+ * Every vector update is the reference's sequence of copy / axpy / scale
+ * passes (axpy: y_i += a x_i; scale: p_i *= a), the scalars are formed in
+ * its order, res[] holds ||r_i||_2 (ParVector::norm, entries <= 1e-16
+ * skipped) starting with ||r_0||, and the loop runs while
+ * ||r|| > tol ||r_0|| and iter < max_iter (max_iter <= 0: 1.3 n + 2).
+ * Pre_BiCGStab in the reference leaves its iteration counter uninitialised
+ * and its p_hat / s_hat unsized; here the counter starts at 0 and both
+ * vectors have n entries (the algorithm the routine spells out). */
+static int bicgstab_run(const orc_csr* A, const orc_hier* h, double* x, const double* b, double tol,
+                        int max_iter, double* res, int max_res, int* n_res)
+{
+    const int n = A->n_rows;
+    const size_t bytes = (size_t)(n > 0 ? n : 1) * sizeof(double);
+    double *r = (double*)malloc(bytes), *rs = (double*)malloc(bytes), *s = (double*)malloc(bytes);
+    double *p = (double*)malloc(bytes), *Ap = (double*)malloc(bytes), *As = (double*)malloc(bytes);
+    double *ph = (double*)malloc(bytes), *sh = (double*)malloc(bytes);
+    *n_res = 0;
+    if (max_iter <= 0) max_iter = ((int)(1.3 * n)) + 2;
+    orc_residual(n, A->rowptr, A->col, A->val, x, b, r);
+    memcpy(rs, r, bytes);
+    memcpy(p, r, bytes);
+    double rr_inner = orc_dot(n, r, rs);
+    double norm_r = orc_norm2(n, r);
+    push(res, max_res, n_res, norm_r);
+    if (norm_r != 0.0) tol = tol * norm_r;
+    int iter = 0;
+    while (norm_r > tol && iter < max_iter) {
+        const double* pd = p;
+        if (h) {
+            memset(ph, 0, bytes);
+            orc_cycle(h, ph, p);
+            pd = ph;
+        }
+        orc_spmv(n, A->rowptr, A->col, A->val, pd, Ap);
+        const double Apr_inner = orc_dot(n, Ap, rs);
+        const double alpha = rr_inner / Apr_inner;
+        for (int i = 0; i < n; i++) s[i] = r[i];
+        for (int i = 0; i < n; i++) s[i] += (-1.0 * alpha) * Ap[i];
+        const double* sd = s;
+        if (h) {
+            memset(sh, 0, bytes);
+            orc_cycle(h, sh, s);
+            sd = sh;
+        }
+        orc_spmv(n, A->rowptr, A->col, A->val, sd, As);
+        const double As_inner = orc_dot(n, As, s);
+        const double AsAs_inner = orc_dot(n, As, As);
+        const double omega = As_inner / AsAs_inner;
+        for (int i = 0; i < n; i++) x[i] += alpha * pd[i];
+        for (int i = 0; i < n; i++) x[i] += omega * sd[i];
+        for (int i = 0; i < n; i++) r[i] = s[i];
+        for (int i = 0; i < n; i++) r[i] += (-1.0 * omega) * As[i];
+        const double next_inner = orc_dot(n, r, rs);
+        const double beta = (next_inner / rr_inner) * (alpha / omega);
+        for (int i = 0; i < n; i++) p[i] *= beta;
+        for (int i = 0; i < n; i++) p[i] += 1.0 * r[i];
+        for (int i = 0; i < n; i++) p[i] += (-1.0 * beta * omega) * Ap[i];
+        rr_inner = next_inner;
+        norm_r = orc_norm2(n, r);
+        push(res, max_res, n_res, norm_r);
+        iter++;
+    }
+    free(r); free(rs); free(s); free(p); free(Ap); free(As); free(ph); free(sh);
+    return iter;
+}
+
+int orc_bicgstab(const orc_csr* A, double* x, const double* b, double tol, int max_iter, double* res,
+                 int max_res, int* n_res)
+{
+    return bicgstab_run(A, NULL, x, b, tol, max_iter, res, max_res, n_res);
+}
+
+int orc_pre_bicgstab(const orc_csr* A, const orc_hier* h, double* x, const double* b, double tol, int max_iter,
+                     double* res, int max_res, int* n_res)
+{
+    return bicgstab_run(A, h, x, b, tol, max_iter, res, max_res, n_res);
+}

@@ -83,6 +83,11 @@ int  orc_amg_solve(const orc_hier* h, double* x, const double* b, double solve_t
                    int max_iterations, double* residuals);
 int  orc_pcg(const orc_csr* A, const orc_hier* h, double* x, const double* b,
              double tol, int max_iter, double* res, int max_res, int* n_res);
+/* BiCGStab and Pre_BiCGStab (krylov/par_bicgstab.cpp:11-110, 240-360) */
+int  orc_bicgstab(const orc_csr* A, double* x, const double* b, double tol, int max_iter,
+                  double* res, int max_res, int* n_res);
+int  orc_pre_bicgstab(const orc_csr* A, const orc_hier* h, double* x, const double* b, double tol,
+                      int max_iter, double* res, int max_res, int* n_res);
 int  orc_cg(const orc_csr* A, double* x, const double* b, double tol, int max_iter,
             double* res, int max_res, int* n_res);
 

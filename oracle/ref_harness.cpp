@@ -6,6 +6,7 @@
 #include <map>
 
 #include "raptor.hpp"
+#include "krylov/par_bicgstab.hpp"
 
 using namespace raptor;
 
@@ -320,6 +321,24 @@ int ref_cg(int n, const int* rp, const int* col, const double* val, double* x, c
     std::copy(b, b + n, bv.local.values.begin());
     aligned_vector<double> r;
     CG(A, xv, bv, r, tol, max_iter);
+    std::copy(xv.local.values.begin(), xv.local.values.end(), x);
+    int k = std::min((int)r.size(), max_res);
+    for (int i = 0; i < k; i++) res[i] = r[i];
+    delete A;
+    return (int)r.size();
+}
+
+// BiCGStab (krylov/par_bicgstab.cpp:11-110) on one rank; returns the number
+// of residuals the reference recorded
+int ref_bicgstab(int n, const int* rp, const int* col, const double* val, double* x, const double* b, double tol,
+                 int max_iter, double* res, int max_res)
+{
+    ParCSRMatrix* A = make_par(n, n, rp, col, val);
+    ParVector xv(n, n), bv(n, n);
+    std::copy(x, x + n, xv.local.values.begin());
+    std::copy(b, b + n, bv.local.values.begin());
+    aligned_vector<double> r;
+    BiCGStab(A, xv, bv, r, tol, max_iter);
     std::copy(xv.local.values.begin(), xv.local.values.end(), x);
     int k = std::min((int)r.size(), max_res);
     for (int i = 0; i < k; i++) res[i] = r[i];

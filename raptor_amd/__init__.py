@@ -14,6 +14,8 @@ libraptor_amd.so (include/raptor_amd.h):
                       ruge_stuben/par_ruge_stuben_solver.hpp,
                       aggregation/par_smoothed_aggregation_solver.hpp)
     PCG               krylov/par_cg.cpp:121-263
+    BiCGStab, Pre_BiCGStab
+                      krylov/par_bicgstab.cpp:11-110, 240-360
     strength / split / interpolation / mis2 / aggregate /
     fit_candidates / jacobi_prolongation / gallery
                       setup pieces (par_strength.cpp,
@@ -35,7 +37,7 @@ from ._lib import (CLJP, HMIS, PMIS, RS, Classical, Direct, Extended, Falgout, J
 
 __all__ = [
     "Context", "ParVector", "ParCSRMatrix", "ParMultilevel", "ParRugeStubenSolver",
-    "ParSmoothedAggregationSolver", "PCG", "CG", "FPCG", "FPCG_block", "vcycle_block", "block_tuning", "block_set_tuning", "adjoint", "strength", "split", "interpolation", "mis2",
+    "ParSmoothedAggregationSolver", "PCG", "CG", "BiCGStab", "Pre_BiCGStab", "FPCG", "FPCG_block", "vcycle_block", "block_tuning", "block_set_tuning", "adjoint", "strength", "split", "interpolation", "mis2",
     "aggregate", "fit_candidates", "jacobi_prolongation", "read_petsc", "write_petsc", "gallery_laplace27", "gallery_aniso2d", "gallery_fv7", "RamdError",
     "RS", "CLJP", "Falgout", "PMIS", "HMIS", "Direct", "ModClassical", "Extended", "Classical",
     "Symmetric", "Jacobi", "PREC_FP64", "PREC_FP32",
@@ -432,6 +434,27 @@ def CG(A, x, b, tol=1e-5, max_iter=-1, max_res=100000):
     nres, it = C.c_int32(), C.c_int32()
     check(lib().ramd_cg(A.h, x.ptr, b.ptr, float(tol), int(max_iter), _ptr(res, C.c_double), max_res,
                         C.byref(nres), C.byref(it)))
+    return res[:min(nres.value, max_res)], it.value
+
+
+def BiCGStab(A, x, b, tol=1e-5, max_iter=-1, max_res=100000):
+    """BiCGStab(A, x, b, res, tol, max_iter), krylov/par_bicgstab.cpp:11-110.
+    Returns (residual 2-norm history from ||r_0||, iteration count)."""
+    res = np.zeros(max_res)
+    nres, it = C.c_int32(), C.c_int32()
+    check(lib().ramd_bicgstab(A.h, x.ptr, b.ptr, float(tol), int(max_iter), _ptr(res, C.c_double), max_res,
+                              C.byref(nres), C.byref(it)))
+    return res[:min(nres.value, max_res)], it.value
+
+
+def Pre_BiCGStab(A, x, b, ml, tol=1e-5, max_iter=-1, max_res=100000):
+    """Pre_BiCGStab(A, x, b, ml, res, tol, max_iter), krylov/par_bicgstab.cpp:
+    240-360, with one V-cycle of ml as the preconditioner.  Returns
+    (residual history, iteration count)."""
+    res = np.zeros(max_res)
+    nres, it = C.c_int32(), C.c_int32()
+    check(lib().ramd_pre_bicgstab(ml.h, A.h, x.ptr, b.ptr, float(tol), int(max_iter), _ptr(res, C.c_double),
+                                  max_res, C.byref(nres), C.byref(it)))
     return res[:min(nres.value, max_res)], it.value
 
 

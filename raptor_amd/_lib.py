@@ -115,6 +115,8 @@ _SIGS = {
     "ramd_pcg": (P, P, P, P, D, C.c_int, PD, I32, PI32, PI32),
     "ramd_fpcg": (P, P, P, P, D, C.c_int, C.c_int, PD, PI32, PD),
     "ramd_cg": (P, P, P, D, C.c_int, PD, I32, PI32, PI32),
+    "ramd_bicgstab": (P, P, P, D, I32, PD, I32, PI32, PI32),
+    "ramd_pre_bicgstab": (P, P, P, P, D, I32, PD, I32, PI32, PI32),
     "ramd_adjoint": (P, P, P, P, D, C.c_int, C.c_int, P, P, PI32, PD),
     "ramd_fpcg_block": (P, P, I32, P, P, D, C.c_int, C.c_int, C.c_int, PI32, PD, PI32),
     "ramd_vcycle_block": (P, I32, C.c_int, P, P),

@@ -562,6 +562,8 @@ static double dot64(ramd_ctx ctx, int n, const double* a, const double* b)
 
 static void vcycle_zero(ramd_hier_s* h, double* z, const double* r) { cycle64(h, 0, z, r, true); }
 
+void vcycle_from_zero(ramd_hier_s* h, double* x, const double* b) { cycle64(h, 0, x, b, true); }
+
 // Flexible-PCG vector kernels.  The reductions run on a fixed grid of
 // kRedBlocks workgroups, each summing a fixed strided subset, so every run
 // gives the same bits.

@@ -6,6 +6,8 @@
 
 #include "common.hpp"
 
+struct ramd_hier_s;
+
 namespace ramd {
 
 ramd_csr_s* spgemm(ramd_csr_s* A, ramd_csr_s* B);
@@ -61,6 +63,9 @@ __device__ __forceinline__ void coarse_solve_wave(int n, const double* __restric
     }
     if (lane < n) x[(size_t)lane * stride] = (TX)v;
 }
+
+// One FP64 V-cycle from x = 0 (ParMultilevel::cycle on the fine level).
+void vcycle_from_zero(ramd_hier_s* h, double* x, const double* b);
 
 struct BlockState;  // multi-vector work space (block.hip)
 void block_free(BlockState* b);

@@ -272,6 +272,21 @@ def cg(A, x, b, tol=1e-5, max_iter=-1, max_res=5000):
     return x, res[:min(nres.value, max_res)], it
 
 
+def bicgstab(A, x, b, tol=1e-5, max_iter=-1, max_res=5000, hier=None):
+    """orc_bicgstab, or orc_pre_bicgstab with hier (a Hier) as preconditioner."""
+    x = f64(x).copy()
+    res = np.zeros(max_res)
+    nres = C.c_int()
+    cA = A.orc()
+    if hier is None:
+        it = orc().orc_bicgstab(C.byref(cA), dp(x), dp(f64(b)), C.c_double(tol), max_iter, dp(res), max_res,
+                                C.byref(nres))
+    else:
+        it = orc().orc_pre_bicgstab(C.byref(cA), C.byref(hier.h), dp(x), dp(f64(b)), C.c_double(tol), max_iter,
+                                    dp(res), max_res, C.byref(nres))
+    return x, res[:min(nres.value, max_res)], it
+
+
 # ---------------------------------------------------------------------------
 # fixtures
 # ---------------------------------------------------------------------------

@@ -44,6 +44,9 @@ FLOAT_VECTORS = [
     "aniso_ones_b", "aniso_ones_b_T", "aniso_inc_b", "aniso_inc_b_T",
     "random_ones_b", "random_ones_b_T", "random_inc_b", "random_inc_b_T",
     "weights", "cg_res",
+    # BiCGStab on the 50 x 50 rotated anisotropic problem (krylov/tests/test_bicgstab.cpp,
+    # krylov/tests/test_par_bicgstab.cpp)
+    "bicgstab_res", "bicgstab_x",
 ]
 
 

@@ -344,3 +344,54 @@ def test_ref_coarse_lu_matches_oracle():
     assert info == 0
     assert np.array_equal(piv_o, piv_r - 1)
     assert np.allclose(lu_o, lu_r, rtol=1e-12, atol=1e-14)
+
+
+def test_bicgstab_residual_history_golden(golden):
+    """krylov/tests/test_bicgstab.cpp / test_par_bicgstab.cpp: 50x50 aniso,
+    b = A*1, x0 = 0, BiCGStab(A,x,b,res); the reference checks the first 30
+    residuals to 1e-6."""
+    A = stencil_grid_np(diffusion_2d(0.001, math.pi / 8), [50, 50])
+    b = K.spmv(A, np.ones(A.n_rows))
+    x, res, it = K.bicgstab(A, np.zeros(A.n_rows), b)
+    ref = golden["bicgstab_res"]
+    assert np.allclose(res[:30], ref[:30], rtol=0, atol=1e-6)
+    assert len(res) == it + 1
+
+
+@needs_ref
+def test_ref_bicgstab_bit_exact():
+    """orc_bicgstab against the reference's own BiCGStab compiled in _ref:
+    every residual and the solution bit for bit (same passes, same order)."""
+    import ctypes as C
+    A = stencil_grid_np(diffusion_2d(0.001, math.pi / 8), [50, 50])
+    b = K.spmv(A, np.ones(A.n_rows))
+    x, res, it = K.bicgstab(A, np.zeros(A.n_rows), b)
+    L = K.ref()
+    xr = np.zeros(A.n_rows)
+    rr = np.zeros(5000)
+    nr = L.ref_bicgstab(A.n_rows, K.ip(A.rowptr), K.ip(A.col), K.dp(A.val), K.dp(xr), K.dp(b), C.c_double(1e-5),
+                        -1, K.dp(rr), 5000)
+    assert nr == len(res)
+    assert np.array_equal(rr[:nr], res)
+    assert np.array_equal(xr, x)
+
+
+@needs_ref
+@pytest.mark.parametrize("omega", [2.0 / 3.0])
+def test_pre_bicgstab_oracle_invariants(omega):
+    """orc_pre_bicgstab (the reference's routine cannot be run: it reads an
+    uninitialised counter) on the reference's own C1 hierarchy: the
+    recursively updated residual equals b - A x at the end to round-off
+    (a dropped or mis-signed term in the x or r update breaks this), the
+    stop rule holds, and the AMG preconditioner cuts the iterations."""
+    A = stencil_grid_np(diffusion_2d(0.001, math.pi / 8), [60, 60])
+    As, Ps = _ref_hierarchy(A, 0.25, 1, 1, 0, omega)
+    H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=omega)
+    b = K.spmv(A, np.ones(A.n_rows))
+    x, res, it = K.bicgstab(A, np.zeros(A.n_rows), b, tol=1e-8, hier=H)
+    true = np.linalg.norm(b - K.spmv(A, x))
+    assert res[-1] <= 1e-8 * res[0]
+    assert abs(true - res[-1]) <= 1e-6 * res[-1] + 1e-13 * res[0]
+    assert np.allclose(x, 1.0, atol=1e-5)
+    _, _, it_plain = K.bicgstab(A, np.zeros(A.n_rows), b, tol=1e-8)
+    assert it < it_plain / 3
