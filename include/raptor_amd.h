@@ -177,7 +177,15 @@ int ramd_galerkin(ramd_csr A, ramd_csr P, ramd_csr* Ac);
 enum { RAMD_RS = 0, RAMD_CLJP = 1, RAMD_FALGOUT = 2, RAMD_PMIS = 3, RAMD_HMIS = 4 };
 enum { RAMD_DIRECT = 0, RAMD_MOD_CLASSICAL = 1, RAMD_EXTENDED = 2 };
 enum { RAMD_CLASSICAL = 0, RAMD_SYMMETRIC = 1 };
-enum { RAMD_JACOBI = 0 };
+/* relax_t (Jacobi, SOR, SSOR) plus this build's multicolour forms: the
+ * device cycle smooths with RAMD_JACOBI (the intended formula of
+ * util/linalg/relax.cpp:10-43) or with RAMD_MC_SOR / RAMD_MC_SSOR, sweeps
+ * over a greedy colouring of A + A^T with the reference's SOR_forward /
+ * SOR_backward row updates (util/linalg/par_relax.cpp:29-104).  The
+ * reference's sequential SOR/SSOR (row order 0..n-1) is host-only there
+ * and not offered on the device: the multicolour order gives different
+ * results (a labelled non-reference smoother). */
+enum { RAMD_JACOBI = 0, RAMD_SOR = 1, RAMD_SSOR = 2, RAMD_MC_SOR = 3, RAMD_MC_SSOR = 4 };
 enum { RAMD_PREC_FP64 = 0, RAMD_PREC_FP32 = 1 };
 enum { RAMD_SOLVER_RS = 0, RAMD_SOLVER_SA = 1 };
 
@@ -202,6 +210,7 @@ typedef struct {
                                srand(2448422)+rand() stream
                                (multilevel/par_multilevel.hpp:215-225)          */
     int prolong_smooth_steps;/* SA Jacobi prolongation steps (default 1)       */
+    int relax_type;         /* RAMD_JACOBI (default), RAMD_MC_SOR, RAMD_MC_SSOR */
 } ramd_amg_opts;
 
 /* Fills opts with the reference's defaults for `solver`. */
@@ -386,6 +395,11 @@ int ramd_host_write_petsc(const char* path, int32_t n_rows, int32_t n_cols, cons
 /* mis2(S, states, off_proc_states, false, weights), aggregation/par_mis.cpp:
  * 204-645.  weights NULL = the srand(2448422) stream.  states: 1 selected,
  * 0 not selected. */
+/* Greedy colouring of the graph of A + A^T without the diagonal (rows in
+ * ascending order take the smallest colour no coloured neighbour holds),
+ * the colouring the multicolour smoothers use.  No reference counterpart. */
+int ramd_host_greedy_colors(int32_t n, const int32_t* rowptr, const int32_t* col, int32_t* colors,
+                            int32_t* n_colors);
 int ramd_host_mis2(int32_t n, const int32_t* s_rowptr, const int32_t* s_col, const double* weights,
                    int32_t* states);
 /* aggregate(A, S, states, off, aggregates, false, rand_vals),

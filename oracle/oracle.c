@@ -406,11 +406,79 @@ void orc_free(void* p) { free(p); }
 /* Hierarchy drivers                                                        */
 /* ---------------------------------------------------------------------- */
 
+/* ---- multicolour smoothers (this build's labelled non-reference smoothers) ----
+ * Greedy colouring of the graph of A + A^T without the diagonal: rows in
+ * ascending order take the smallest colour no already-coloured neighbour
+ * holds. */
+int orc_greedy_colors(int n, const int* rowptr, const int* col, int* color)
+{
+    int* tp = (int*)calloc((size_t)n + 1, sizeof(int));
+    for (int i = 0; i < n; i++)
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++)
+            if (col[j] != i) tp[col[j] + 1]++;
+    for (int i = 0; i < n; i++) tp[i + 1] += tp[i];
+    int* trow = (int*)malloc((size_t)(tp[n] > 0 ? tp[n] : 1) * sizeof(int));
+    int* pos = (int*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+    for (int i = 0; i < n; i++) pos[i] = tp[i];
+    for (int i = 0; i < n; i++)
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++)
+            if (col[j] != i) trow[pos[col[j]]++] = i;
+    int* taken = (int*)calloc((size_t)n + 1, sizeof(int)); /* taken[c] == i + 1: used next to row i */
+    int ncolors = 0;
+    for (int i = 0; i < n; i++) color[i] = -1;
+    for (int i = 0; i < n; i++) {
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++)
+            if (col[j] != i && color[col[j]] >= 0) taken[color[col[j]]] = i + 1;
+        for (int j = tp[i]; j < tp[i + 1]; j++)
+            if (color[trow[j]] >= 0) taken[color[trow[j]]] = i + 1;
+        int c = 0;
+        while (c < ncolors && taken[c] == i + 1) c++;
+        color[i] = c;
+        if (c == ncolors) ncolors++;
+    }
+    free(tp); free(trow); free(pos); free(taken);
+    return ncolors;
+}
+
+/* Multicolour sweeps: colours in increasing order (then decreasing for the
+ * symmetric form), rows of a colour in ascending order; each row takes
+ * the reference's per-row update with the current x -- SOR_forward's
+ * x_i = (x_i + w (y_i - x_i - sum)) / a_ii on the way up and SOR_backward's
+ * x_i = (1 - w) x_i + w ((y_i - sum) / a_ii) on the way down
+ * (util/linalg/par_relax.cpp:29-104), sum over the off-diagonals in stored
+ * order; a row whose first entry is not its diagonal keeps x.  Rows of one
+ * colour are never neighbours, so any order within a colour gives these
+ * bits. */
+void orc_mc_sor(int n, const int* rowptr, const int* col, const double* val, const double* b, double* x,
+                int sweeps, double omega, int symmetric)
+{
+    int* color = (int*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+    const int nc = orc_greedy_colors(n, rowptr, col, color);
+    for (int it = 0; it < sweeps; it++)
+        for (int pass = 0; pass < (symmetric ? 2 : 1); pass++)
+            for (int k = 0; k < nc; k++) {
+                const int c = pass == 0 ? k : nc - 1 - k;
+                for (int i = 0; i < n; i++) {
+                    if (color[i] != c) continue;
+                    int s0 = rowptr[i], e0 = rowptr[i + 1];
+                    if (s0 == e0 || col[s0] != i) continue;
+                    const double diag = val[s0];
+                    double row_sum = 0;
+                    for (int j = s0 + 1; j < e0; j++) row_sum += val[j] * x[col[j]];
+                    if (pass == 0) x[i] = (x[i] + omega * (b[i] - x[i] - row_sum)) / diag;
+                    else x[i] = ((1.0 - omega) * x[i]) + (omega * ((b[i] - row_sum) / diag));
+                }
+            }
+    free(color);
+}
+
 static void relax(const orc_hier* h, int l, double* x, const double* b, double* tmp)
 {
     const orc_csr* A = &h->A[l];
     if (h->relax_type == 0)
         orc_jacobi(A->n_rows, A->rowptr, A->col, A->val, b, x, tmp, h->num_sweeps, h->relax_weight);
+    else if (h->relax_type == 3 || h->relax_type == 4)
+        orc_mc_sor(A->n_rows, A->rowptr, A->col, A->val, b, x, h->num_sweeps, h->relax_weight, h->relax_type == 4);
     else
         orc_sor(A->n_rows, A->rowptr, A->col, A->val, b, x, h->num_sweeps, h->relax_weight);
 }

@@ -56,6 +56,12 @@ int64_t orc_spgemm_tn(int p_rows, int p_cols, const int* p_ptr, const int* p_col
                       int* c_ptr, int** c_col, double** c_val);
 void orc_free(void* p);
 
+/* multicolour smoothers: greedy colouring of A + A^T, then sweeps colour by
+ * colour with the reference's SOR_forward / SOR_backward row updates */
+int  orc_greedy_colors(int n, const int* rowptr, const int* col, int* color);
+void orc_mc_sor(int n, const int* rowptr, const int* col, const double* val, const double* b, double* x,
+                int sweeps, double omega, int symmetric);
+
 /* ---- hierarchy, V-cycle, stationary solve and PCG ---- */
 typedef struct {
     int n_rows, n_cols;
@@ -71,7 +77,7 @@ typedef struct {
     const double* coarse_lu; /* coarse_n*coarse_n, output of orc_lu_factor */
     const int* coarse_piv;
     int coarse_n;
-    int relax_type;          /* 0 = weighted Jacobi, 1 = hybrid SOR */
+    int relax_type;          /* 0 = weighted Jacobi, 1 = hybrid SOR, 3 / 4 = multicolour SOR / SSOR */
     int num_sweeps;
     double relax_weight;
 } orc_hier;

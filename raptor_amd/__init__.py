@@ -32,15 +32,15 @@ import weakref
 import numpy as np
 
 from . import _lib
-from ._lib import (CLJP, HMIS, PMIS, RS, Classical, Direct, Extended, Falgout, Jacobi, ModClassical,
-                   PREC_FP32, PREC_FP64, RamdError, Symmetric, check, lib)
+from ._lib import (CLJP, HMIS, MC_SOR, MC_SSOR, PMIS, RS, Classical, Direct, Extended, Falgout, Jacobi,
+                   ModClassical, PREC_FP32, PREC_FP64, RamdError, Symmetric, check, lib)
 
 __all__ = [
     "Context", "ParVector", "ParCSRMatrix", "ParMultilevel", "ParRugeStubenSolver",
     "ParSmoothedAggregationSolver", "PCG", "CG", "BiCGStab", "Pre_BiCGStab", "FPCG", "FPCG_block", "vcycle_block", "block_tuning", "block_set_tuning", "adjoint", "strength", "split", "interpolation", "mis2",
     "aggregate", "fit_candidates", "jacobi_prolongation", "read_petsc", "write_petsc", "gallery_laplace27", "gallery_aniso2d", "gallery_fv7", "RamdError",
     "RS", "CLJP", "Falgout", "PMIS", "HMIS", "Direct", "ModClassical", "Extended", "Classical",
-    "Symmetric", "Jacobi", "PREC_FP64", "PREC_FP32",
+    "Symmetric", "Jacobi", "MC_SOR", "MC_SSOR", "greedy_colors", "PREC_FP64", "PREC_FP32",
 ]
 
 
@@ -249,8 +249,9 @@ class ParMultilevel:
     _solver = _lib.SOLVER_RS
 
     def _init_common(self, ctx, strong_threshold, strength_type, relax_type):
-        if relax_type != Jacobi:
-            raise ValueError("the device V-cycle smooths with weighted Jacobi only")
+        if relax_type not in (Jacobi, MC_SOR, MC_SSOR):
+            raise ValueError("the device V-cycle smooths with Jacobi or the multicolour MC_SOR / MC_SSOR "
+                             "(the reference's sequential SOR is host-only)")
         self.ctx = ctx
         self.strong_threshold = strong_threshold
         self.strength_type = strength_type
@@ -273,6 +274,7 @@ class ParMultilevel:
         o.relax_weight = self.relax_weight
         o.max_coarse = self.max_coarse
         o.max_levels = self.max_levels
+        o.relax_type = self.relax_type
         return o
 
     def setup(self, rowptr, col, val):
@@ -435,6 +437,17 @@ def CG(A, x, b, tol=1e-5, max_iter=-1, max_res=100000):
     check(lib().ramd_cg(A.h, x.ptr, b.ptr, float(tol), int(max_iter), _ptr(res, C.c_double), max_res,
                         C.byref(nres), C.byref(it)))
     return res[:min(nres.value, max_res)], it.value
+
+
+def greedy_colors(rowptr, col):
+    """Greedy colouring of A + A^T (ramd_host_greedy_colors): (colour per row, number of colours)."""
+    rowptr, col = _i32(rowptr), _i32(col)
+    n = rowptr.size - 1
+    out = np.empty(max(n, 1), np.int32)
+    nc = C.c_int32()
+    check(lib().ramd_host_greedy_colors(n, _ptr(rowptr, C.c_int32), _ptr(col, C.c_int32),
+                                        _ptr(out, C.c_int32), C.byref(nc)))
+    return out[:n], nc.value
 
 
 def BiCGStab(A, x, b, tol=1e-5, max_iter=-1, max_res=100000):

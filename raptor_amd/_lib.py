@@ -25,6 +25,7 @@ RS, CLJP, Falgout, PMIS, HMIS = 0, 1, 2, 3, 4
 Direct, ModClassical, Extended = 0, 1, 2
 Classical, Symmetric = 0, 1
 Jacobi = 0
+SOR, SSOR, MC_SOR, MC_SSOR = 1, 2, 3, 4
 PREC_FP64, PREC_FP32 = 0, 1
 SOLVER_RS, SOLVER_SA = 0, 1
 
@@ -50,6 +51,7 @@ class AmgOpts(C.Structure):
         ("max_levels", C.c_int),
         ("weights", C.POINTER(C.c_double)),
         ("prolong_smooth_steps", C.c_int),
+        ("relax_type", C.c_int),
     ]
 
 
@@ -116,6 +118,7 @@ _SIGS = {
     "ramd_fpcg": (P, P, P, P, D, C.c_int, C.c_int, PD, PI32, PD),
     "ramd_cg": (P, P, P, D, C.c_int, PD, I32, PI32, PI32),
     "ramd_bicgstab": (P, P, P, D, I32, PD, I32, PI32, PI32),
+    "ramd_host_greedy_colors": (I32, PI32, PI32, PI32, PI32),
     "ramd_pre_bicgstab": (P, P, P, P, D, I32, PD, I32, PI32, PI32),
     "ramd_adjoint": (P, P, P, P, D, C.c_int, C.c_int, P, P, PI32, PD),
     "ramd_fpcg_block": (P, P, I32, P, P, D, C.c_int, C.c_int, C.c_int, PI32, PD, PI32),

@@ -935,6 +935,8 @@ static int block_tune(ramd_hier_s* h, const ramd_csr_s* M, int k, const X* x, co
 
 static BlockState* block_state(ramd_hier_s* h, int k, bool fp32)
 {
+    RAMD_CHECK(h->opts.relax_type == RAMD_JACOBI, RAMD_E_ARG,
+               "the multi-vector cycle smooths with Jacobi only (hierarchy relax_type %d)", h->opts.relax_type);
     BlockState*& b = h->blk;
     if (b && b->k != k) {
         RAMD_HIP(hipStreamSynchronize(h->ctx->stream));

@@ -198,6 +198,99 @@ static void coarse_solve64(ramd_hier_s* h, const double* b, double* x)
     RAMD_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Multicolour smoothers (RAMD_MC_SOR / RAMD_MC_SSOR): colours in increasing
+// order (then decreasing for the symmetric form), one thread per row of the
+// colour with the reference's per-row update on the current x --
+// SOR_forward's x_i = (x_i + w (y_i - x_i - sum)) / a_ii going up,
+// SOR_backward's x_i = (1 - w) x_i + w ((y_i - sum) / a_ii) going down
+// (util/linalg/par_relax.cpp:29-104), the off-diagonal sum in stored order.
+// Rows of one colour are never neighbours, so they update independently.
+// A row whose first entry is not its diagonal keeps x, as there.
+// ---------------------------------------------------------------------------
+template <typename V, typename X, bool BACK>
+__global__ void mc_sor_kernel(const int* __restrict__ rows, int count, const int* __restrict__ rp,
+                              const int* __restrict__ col, const V* __restrict__ val, X* __restrict__ x,
+                              const X* __restrict__ y, X omega)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int i = rows[t];
+    const int s = rp[i], e = rp[i + 1];
+    if (s == e || col[s] != i) return;
+    const X diag = X(val[s]);
+    X row_sum = 0;
+    for (int j = s + 1; j < e; j++) row_sum += X(val[j]) * x[col[j]];
+    if (BACK) x[i] = ((X(1.0) - omega) * x[i]) + (omega * ((y[i] - row_sum) / diag));
+    else x[i] = (x[i] + omega * (y[i] - x[i] - row_sum)) / diag;
+}
+
+template <typename X>
+__global__ void fill_zero(int n, X* __restrict__ x)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = X(0);
+}
+
+static bool multicolour(const ramd_hier_s* h)
+{
+    return h->opts.relax_type == RAMD_MC_SOR || h->opts.relax_type == RAMD_MC_SSOR;
+}
+
+// colouring of every level but the coarsest (built once per hierarchy)
+static void build_colours(ramd_hier_s* h)
+{
+    ramd_ctx ctx = h->ctx;
+    for (size_t l = 0; l + 1 < h->levels.size(); l++) {
+        Level& L = h->levels[l];
+        if (L.color_rows) continue;
+        const ramd_csr_s* A = L.A;
+        std::vector<int> col((size_t)A->nnz);
+        if (A->nnz) {
+            RAMD_HIP(hipMemcpyAsync(col.data(), A->col, sizeof(int) * A->nnz, hipMemcpyDeviceToHost, ctx->stream));
+            RAMD_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        std::vector<int> colour;
+        const int nc = host::greedy_colors(A->n_rows, A->h_rowptr.data(), col.data(), colour);
+        L.color_ptr.assign((size_t)nc + 1, 0);
+        for (int c : colour) L.color_ptr[c + 1]++;
+        for (int c = 0; c < nc; c++) L.color_ptr[c + 1] += L.color_ptr[c];
+        std::vector<int> rows((size_t)A->n_rows), pos(L.color_ptr.begin(), L.color_ptr.end() - 1);
+        for (int i = 0; i < A->n_rows; i++) rows[pos[colour[i]]++] = i;
+        L.color_rows = dalloc<int>((size_t)std::max(A->n_rows, 1));
+        RAMD_HIP(hipMemcpyAsync(L.color_rows, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice,
+                                ctx->stream));
+        RAMD_HIP(hipStreamSynchronize(ctx->stream));
+    }
+}
+
+// num_smooth_sweeps multicolour sweeps in place on x (zeroed first when
+// the cycle enters with x = 0)
+template <typename V, typename X>
+static void mc_relax(ramd_hier_s* h, const Level& L, const V* val, X* x, const X* b, bool x_zero)
+{
+    hipStream_t s = h->ctx->stream;
+    const X w = (X)h->opts.relax_weight;
+    if (x_zero) hipLaunchKernelGGL((fill_zero<X>), grid1(L.n), dim3(256), 0, s, L.n, x);
+    const int nc = (int)L.color_ptr.size() - 1;
+    const bool sym = h->opts.relax_type == RAMD_MC_SSOR;
+    for (int it = 0; it < h->opts.num_smooth_sweeps; it++)
+        for (int pass = 0; pass < (sym ? 2 : 1); pass++)
+            for (int k = 0; k < nc; k++) {
+                const int c = pass == 0 ? k : nc - 1 - k;
+                const int cnt = L.color_ptr[c + 1] - L.color_ptr[c];
+                if (cnt == 0) continue;
+                const int* rows = L.color_rows + L.color_ptr[c];
+                if (pass == 0)
+                    hipLaunchKernelGGL((mc_sor_kernel<V, X, false>), grid1(cnt), dim3(256), 0, s, rows, cnt,
+                                       L.A->rowptr, L.A->col, val, x, b, w);
+                else
+                    hipLaunchKernelGGL((mc_sor_kernel<V, X, true>), grid1(cnt), dim3(256), 0, s, rows, cnt,
+                                       L.A->rowptr, L.A->col, val, x, b, w);
+            }
+    RAMD_HIP(hipGetLastError());
+}
+
 // The off-diagonal Jacobi kernel reads `cur` and writes the other buffer.
 static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_zero)
 {
@@ -209,9 +302,11 @@ static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_ze
     }
     const double w = h->opts.relax_weight;
     const int sweeps = h->opts.num_smooth_sweeps;
+    const bool mc = multicolour(h);
     double* cur = x;
     double* other = L.t;
-    for (int it = 0; it < sweeps; it++) {
+    if (mc) mc_relax<double, double>(h, L, L.A->val, cur, b, x_zero);
+    for (int it = 0; it < sweeps && !mc; it++) {
         if (it == 0 && x_zero) {
             hipLaunchKernelGGL((jacobi_from_zero<double>), grid1(L.n), dim3(256), 0, s, L.n, L.diag, b, cur, w);
             RAMD_HIP(hipGetLastError());
@@ -227,7 +322,8 @@ static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_ze
     // x <- x + P e (CSR_append: row sum first, then added), into the other buffer
     launch_stream_op(L.P, OP_ADD_INTO, C.x, cur, other, 0.0, s);
     std::swap(cur, other);
-    for (int it = 0; it < sweeps; it++) {
+    if (mc) mc_relax<double, double>(h, L, L.A->val, cur, b, false);
+    for (int it = 0; it < sweeps && !mc; it++) {
         launch_stream_op(L.A, OP_JACOBI, cur, b, other, w, s, L.diag);
         std::swap(cur, other);
     }
@@ -254,9 +350,11 @@ static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero
     }
     const float w = (float)h->opts.relax_weight;
     const int sweeps = h->opts.num_smooth_sweeps;
+    const bool mc = multicolour(h);
     float* cur = x;
     float* other = L.t32;
-    for (int it = 0; it < sweeps; it++) {
+    if (mc) mc_relax<float, float>(h, L, L.A->val32, cur, b, x_zero);
+    for (int it = 0; it < sweeps && !mc; it++) {
         if (it == 0 && x_zero) {
             hipLaunchKernelGGL((jacobi_from_zero<float>), grid1(L.n), dim3(256), 0, s, L.n, L.diag32, b, cur, w);
             RAMD_HIP(hipGetLastError());
@@ -271,7 +369,8 @@ static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero
     cycle32(h, l + 1, C.x32, C.b32, true);
     launch_stream_op32(L.P, OP_ADD_INTO, C.x32, cur, other, 0.f, s);
     std::swap(cur, other);
-    for (int it = 0; it < sweeps; it++) {
+    if (mc) mc_relax<float, float>(h, L, L.A->val32, cur, b, false);
+    for (int it = 0; it < sweeps && !mc; it++) {
         launch_stream_op32(L.A, OP_JACOBI, cur, b, other, w, s, L.diag32);
         std::swap(cur, other);
     }
@@ -301,7 +400,7 @@ static void free_level(Level& L)
     if (L.A) csr_free(L.A);
     if (L.P) csr_free(L.P);  // frees R (its transpose)
     for (void* p : {(void*)L.diag, (void*)L.x, (void*)L.b, (void*)L.t, (void*)L.r, (void*)L.diag32, (void*)L.x32,
-                    (void*)L.b32, (void*)L.t32, (void*)L.r32})
+                    (void*)L.b32, (void*)L.t32, (void*)L.r32, (void*)L.color_rows})
         dfree(p);
     L = Level();
 }
@@ -445,6 +544,7 @@ static void finish_hierarchy(ramd_hier_s* h, int n, const std::vector<double>& l
             csr_autotune(L.R, 0);
         }
     }
+    if (multicolour(h)) build_colours(h);
     RAMD_HIP(hipStreamSynchronize(ctx->stream));
     for (Level& L : h->levels) L.A->borrowed = true;
     for (Level& L : h->levels)
@@ -780,7 +880,7 @@ __global__ void adjoint_outer(int n, const int* __restrict__ rp, const int* __re
 // the coarse LU factors and the options, so a solve can skip the setup
 // ---------------------------------------------------------------------------
 static const char kHierMagic[8] = {'R', 'A', 'M', 'D', 'H', 'I', 'E', 'R'};
-constexpr int32_t kHierVersion = 1;
+constexpr int32_t kHierVersion = 2;  // 2: relax_type stored after the prolongation steps
 
 struct HierFile {
     FILE* f = nullptr;
@@ -872,6 +972,7 @@ static void hier_save(ramd_hier_s* h, const char* path)
     w.put<int32_t>(o.max_coarse);
     w.put<int32_t>(o.max_levels);
     w.put<int32_t>(o.prolong_smooth_steps);
+    w.put<int32_t>(o.relax_type);
     for (double t : h->times) w.put<double>(t);
     w.put<int32_t>((int32_t)h->levels.size());
     for (Level& L : h->levels) {
@@ -904,7 +1005,8 @@ static void hier_load(ramd_hier_s* h, const char* path)
     r.get_bytes(magic, 8);
     RAMD_CHECK(std::memcmp(magic, kHierMagic, 8) == 0, RAMD_E_ARG, "%s: not a hierarchy file", path);
     const int32_t version = r.get<int32_t>();
-    RAMD_CHECK(version == kHierVersion, RAMD_E_ARG, "%s: version %d (expected %d)", path, version, kHierVersion);
+    RAMD_CHECK(version == 1 || version == kHierVersion, RAMD_E_ARG, "%s: version %d (expected 1 or %d)", path,
+               version, kHierVersion);
     ramd_amg_opts& o = h->opts;
     std::memset(&o, 0, sizeof o);
     o.solver = r.get<int32_t>();
@@ -919,8 +1021,11 @@ static void hier_load(ramd_hier_s* h, const char* path)
     o.max_coarse = r.get<int32_t>();
     o.max_levels = r.get<int32_t>();
     o.prolong_smooth_steps = r.get<int32_t>();
+    o.relax_type = version >= 2 ? r.get<int32_t>() : RAMD_JACOBI;  // version 1 files predate the field
     o.weights = nullptr;
-    RAMD_CHECK(o.num_smooth_sweeps >= 0, RAMD_E_ARG, "%s: corrupt options", path);
+    RAMD_CHECK(o.num_smooth_sweeps >= 0 &&
+                   (o.relax_type == RAMD_JACOBI || o.relax_type == RAMD_MC_SOR || o.relax_type == RAMD_MC_SSOR),
+               RAMD_E_ARG, "%s: corrupt options", path);
     for (double& t : h->times) t = r.get<double>();
     const int32_t nl = r.get<int32_t>();
     RAMD_CHECK(nl >= 1 && nl <= 1000, RAMD_E_ARG, "%s: %d levels", path, nl);
@@ -986,6 +1091,7 @@ int ramd_amg_opts_default(int solver, ramd_amg_opts* o)
         o->max_levels = 25;
         o->weights = nullptr;
         o->prolong_smooth_steps = 1;
+        o->relax_type = RAMD_JACOBI;
     });
 }
 
@@ -1001,6 +1107,12 @@ int ramd_hier_setup(ramd_ctx ctx, int32_t n, int64_t nnz, const int32_t* rowptr,
                    "ramd_hier_setup: solver %d unknown", opts->solver);
         RAMD_CHECK(opts->num_smooth_sweeps >= 0 && opts->max_coarse >= 1 && opts->prolong_smooth_steps >= 0,
                    RAMD_E_ARG, "ramd_hier_setup: bad options");
+        RAMD_CHECK(opts->relax_type == RAMD_JACOBI || opts->relax_type == RAMD_MC_SOR ||
+                       opts->relax_type == RAMD_MC_SSOR,
+                   RAMD_E_ARG,
+                   "ramd_hier_setup: relax_type %d is not available on the device (RAMD_JACOBI, RAMD_MC_SOR, "
+                   "RAMD_MC_SSOR; the reference's sequential SOR/SSOR is host-only)",
+                   opts->relax_type);
         for (int64_t j = 0; j < nnz; j++)
             RAMD_CHECK(col[j] >= 0 && col[j] < n, RAMD_E_ARG, "ramd_hier_setup: column %d out of range", col[j]);
         ctx->set_current();

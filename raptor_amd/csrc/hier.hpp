@@ -23,6 +23,10 @@ struct Level {
     float *x32 = nullptr, *b32 = nullptr, *t32 = nullptr, *r32 = nullptr;
     std::vector<int> states;
     int n = 0;
+    // multicolour smoothers: rows of colour c are color_rows[color_ptr[c] ..
+    // color_ptr[c+1]) (device), ascending
+    std::vector<int> color_ptr;
+    int* color_rows = nullptr;
 };
 
 // Largest coarse system solved by one wavefront (LU staged in LDS).

@@ -156,6 +156,21 @@ int ramd_host_gallery(int kind, int32_t m, double eps, double theta, uint64_t se
     });
 }
 
+int ramd_host_greedy_colors(int32_t n, const int32_t* rowptr, const int32_t* col, int32_t* colors,
+                            int32_t* n_colors)
+{
+    return guarded([&] {
+        RAMD_CHECK(n >= 0 && rowptr && colors && n_colors && (col || rowptr[n] == 0), RAMD_E_ARG,
+                   "ramd_host_greedy_colors: bad argument");
+        for (int64_t j = 0; j < rowptr[n]; j++)
+            RAMD_CHECK(col[j] >= 0 && col[j] < n, RAMD_E_ARG, "ramd_host_greedy_colors: column %d out of range",
+                       col[j]);
+        std::vector<int> c;
+        *n_colors = host::greedy_colors(n, rowptr, col, c);
+        std::copy(c.begin(), c.end(), colors);
+    });
+}
+
 int ramd_host_mis2(int32_t n, const int32_t* s_rowptr, const int32_t* s_col, const double* weights,
                    int32_t* states)
 {

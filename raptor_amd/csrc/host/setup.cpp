@@ -1514,5 +1514,41 @@ Csr fv7_variable(int m, uint64_t seed)
     return A;
 }
 
+// ---------------------------------------------------------------------------
+// Greedy colouring of A + A^T for the multicolour smoothers
+// ---------------------------------------------------------------------------
+int greedy_colors(int n, const int* rp, const int* col, std::vector<int>& color)
+{
+    // transpose pattern, so that a row sees every row that references it
+    std::vector<int> tp((size_t)n + 1, 0);
+    for (int i = 0; i < n; i++)
+        for (int j = rp[i]; j < rp[i + 1]; j++)
+            if (col[j] != i && col[j] >= 0 && col[j] < n) tp[col[j] + 1]++;
+    for (int i = 0; i < n; i++) tp[i + 1] += tp[i];
+    std::vector<int> trow((size_t)tp[n]);
+    std::vector<int> pos(tp.begin(), tp.end() - 1);
+    for (int i = 0; i < n; i++)
+        for (int j = rp[i]; j < rp[i + 1]; j++)
+            if (col[j] != i && col[j] >= 0 && col[j] < n) trow[pos[col[j]]++] = i;
+    color.assign((size_t)n, -1);
+    std::vector<int> seen;  // seen[c] == i + 1: colour c taken by a neighbour of row i
+    int ncolors = 0;
+    for (int i = 0; i < n; i++) {
+        auto mark = [&](int j) {
+            const int c = color[j];
+            if (c >= 0) seen[c] = i + 1;
+        };
+        if ((int)seen.size() < ncolors + 1) seen.resize((size_t)ncolors + 1, 0);
+        for (int j = rp[i]; j < rp[i + 1]; j++)
+            if (col[j] != i && col[j] >= 0 && col[j] < n) mark(col[j]);
+        for (int j = tp[i]; j < tp[i + 1]; j++) mark(trow[j]);
+        int c = 0;
+        while (c < ncolors && seen[c] == i + 1) c++;
+        color[i] = c;
+        if (c == ncolors) ncolors++;
+    }
+    return ncolors;
+}
+
 }  // namespace host
 }  // namespace ramd

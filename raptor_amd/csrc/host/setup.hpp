@@ -81,6 +81,12 @@ Csr sa_subtract(const Csr& P, const Csr& Q);
 
 // Stencil generators (gallery/par_stencil.hpp:16-236).  stencil has 3^dim
 // entries, grid has dim entries.
+// Greedy colouring of the graph of A + A^T (diagonal ignored): rows in
+// ascending order take the smallest colour no already-coloured neighbour
+// has.  Returns the number of colours.  No reference counterpart (the
+// multicolour smoothers are this build's).
+int greedy_colors(int n, const int* rowptr, const int* col, std::vector<int>& color);
+
 Csr stencil_grid(const double* stencil, const int* grid, int dim);
 std::vector<double> laplace_stencil_27pt();
 std::vector<double> diffusion_stencil_2d(double eps, double theta);

@@ -268,3 +268,25 @@ def test_sa_pieces_bit_exact_vs_ref(theta):
                                              K.dp(B), K.dp(R_ref)))
         assert np.array_equal(trp, E[0]) and np.array_equal(tcol, E[1]) and np.array_equal(tval, E[2])
         assert np.array_equal(R, R_ref)
+
+
+def test_greedy_colouring_matches_oracle():
+    """The product's host colouring (used by the multicolour smoothers) is
+    the oracle's, on stencils and on random unsymmetric patterns."""
+    import ctypes as C
+    rng = np.random.default_rng(8)
+    cases = [ra.gallery_laplace27(7)[:2], ra.gallery_aniso2d(15, 0.001, math.pi / 8)[:2]]
+    for n in (1, 40, 300):
+        rp, col = [0], []
+        for i in range(n):
+            c = np.unique(np.concatenate([[i], rng.choice(n, size=min(n, rng.integers(0, 6)), replace=False)]))
+            col += list(c)
+            rp.append(len(col))
+        cases.append((np.array(rp, np.int32), np.array(col, np.int32)))
+    for rp, col in cases:
+        rp, col = np.asarray(rp, np.int32), np.asarray(col, np.int32)
+        n = rp.size - 1
+        got, nc = ra.greedy_colors(rp, col)
+        want = np.empty(n, np.int32)
+        nco = K.orc().orc_greedy_colors(n, K.ip(rp), K.ip(col), K.ip(want))
+        assert nc == nco and np.array_equal(got, want)

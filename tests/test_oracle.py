@@ -395,3 +395,66 @@ def test_pre_bicgstab_oracle_invariants(omega):
     assert np.allclose(x, 1.0, atol=1e-5)
     _, _, it_plain = K.bicgstab(A, np.zeros(A.n_rows), b, tol=1e-8)
     assert it < it_plain / 3
+
+
+def _colors_oracle(A):
+    import ctypes as C
+    col = np.empty(A.n_rows, np.int32)
+    nc = K.orc().orc_greedy_colors(A.n_rows, K.ip(A.rowptr), K.ip(A.col), K.ip(col))
+    return col, nc
+
+
+def _adjacent_same_colour(A, colour):
+    for i in range(A.n_rows):
+        for j in A.col[A.rowptr[i]:A.rowptr[i + 1]]:
+            if j != i and colour[j] == colour[i]:
+                return True
+    return False
+
+
+def test_greedy_colouring_oracle():
+    """Greedy colouring of A + A^T: a proper colouring (no two neighbours
+    share a colour), 2 colours for the 5-point Laplacian (red-black), 4 for
+    the 9-point stencil, 8 for the 27-point one, and a pattern that is only
+    one-sided (row 0 -> 3) still separates rows 0 and 3."""
+    A5 = stencil_grid_np(np.array([0, -1.0, 0, -1, 4, -1, 0, -1, 0]), [9, 7])
+    A9 = stencil_grid_np(diffusion_2d(0.001, math.pi / 8), [12, 12])
+    for A, want in ((A5, 2), (A9, 4)):
+        c, nc = _colors_oracle(A)
+        assert nc == want and not _adjacent_same_colour(A, c)
+    c5, _ = _colors_oracle(A5)
+    assert np.array_equal(c5, (np.arange(63) % 9 + np.arange(63) // 9) % 2)  # checkerboard
+    one_sided = K.Csr([0, 2, 3, 4, 5], [0, 3, 1, 2, 3], [1.0] * 5, 4)
+    c, nc = _colors_oracle(one_sided)
+    assert nc == 2 and c[0] != c[3]
+
+
+@pytest.mark.parametrize("symmetric", [0, 1])
+def test_mc_sor_oracle_invariants(symmetric):
+    """Multicolour SOR with w = 1 is Gauss-Seidel in colour order: after a
+    forward sweep every row of the last colour satisfies its own equation
+    (its neighbours are all of earlier colours, already final), so its
+    residual is round-off; a symmetric sweep leaves the first colour so.
+    A dropped term, a wrong sign or a stale neighbour value breaks this.
+    Both forms then reduce the error of a stationary iteration on an SPD
+    M-matrix."""
+    import ctypes as C
+    A = K.move_diag(stencil_grid_np(diffusion_2d(0.001, math.pi / 8), [12, 12]))
+    n = A.n_rows
+    c, nc = _colors_oracle(A)
+    b = np.random.default_rng(3).standard_normal(n)
+    x = np.random.default_rng(4).standard_normal(n)
+    K.orc().orc_mc_sor(n, K.ip(A.rowptr), K.ip(A.col), K.dp(A.val), K.dp(b), K.dp(x), 1, C.c_double(1.0),
+                       symmetric)
+    r = b - K.spmv(A, x)
+    last = c == (0 if symmetric else nc - 1)
+    assert np.abs(r[last]).max() < 1e-12 * np.abs(b).max()
+    assert np.abs(r[~last]).max() > 1e-3
+    # stationary iteration x <- S(x) converges for w = 1 and w = 2/3
+    for w in (1.0, 2.0 / 3.0):
+        x = np.zeros(n)
+        r0 = np.linalg.norm(b)
+        for _ in range(30):
+            K.orc().orc_mc_sor(n, K.ip(A.rowptr), K.ip(A.col), K.dp(A.val), K.dp(b), K.dp(x), 1, C.c_double(w),
+                               symmetric)
+        assert np.linalg.norm(b - K.spmv(A, x)) < 0.5 * r0
