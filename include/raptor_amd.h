@@ -395,6 +395,19 @@ int ramd_host_write_petsc(const char* path, int32_t n_rows, int32_t n_cols, cons
 /* mis2(S, states, off_proc_states, false, weights), aggregation/par_mis.cpp:
  * 204-645.  weights NULL = the srand(2448422) stream.  states: 1 selected,
  * 0 not selected. */
+/* Device forms of the setup steps of the 27-point benchmarks' RS path,
+ * bit-identical to the host ones (ramd_host_strength / ramd_host_split /
+ * ramd_host_interp) on a matrix stored diagonal first:
+ * classical strength (par_strength.cpp:8-340) -> S on the device;
+ * PMIS (ruge_stuben/par_cf_splitting.cpp:75-89, 1229-1381) on S with the
+ * given random weights (NULL: the reference's srand(2448422) stream) ->
+ * host states; extended interpolation with filter
+ * (ruge_stuben/par_interpolation.cpp:175-989) -> P on the device
+ * (RAMD_E_ARG when a row would need more than 256 columns). */
+int ramd_strength_gpu(ramd_csr A, double theta, ramd_csr* S);
+int ramd_split_pmis_gpu(ramd_csr S, const double* weights, int32_t* states);
+int ramd_interp_extended_gpu(ramd_csr A, ramd_csr S, const int32_t* states, double filter, ramd_csr* P);
+
 /* Greedy colouring of the graph of A + A^T without the diagonal (rows in
  * ascending order take the smallest colour no coloured neighbour holds),
  * the colouring the multicolour smoothers use.  No reference counterpart. */

@@ -38,7 +38,7 @@ from ._lib import (CLJP, HMIS, MC_SOR, MC_SSOR, PMIS, RS, Classical, Direct, Ext
 __all__ = [
     "Context", "ParVector", "ParCSRMatrix", "ParMultilevel", "ParRugeStubenSolver",
     "ParSmoothedAggregationSolver", "PCG", "CG", "BiCGStab", "Pre_BiCGStab", "FPCG", "FPCG_block", "vcycle_block", "block_tuning", "block_set_tuning", "adjoint", "strength", "split", "interpolation", "mis2",
-    "aggregate", "fit_candidates", "jacobi_prolongation", "read_petsc", "write_petsc", "gallery_laplace27", "gallery_aniso2d", "gallery_fv7", "RamdError",
+    "aggregate", "fit_candidates", "strength_gpu", "split_pmis_gpu", "interp_extended_gpu", "jacobi_prolongation", "read_petsc", "write_petsc", "gallery_laplace27", "gallery_aniso2d", "gallery_fv7", "RamdError",
     "RS", "CLJP", "Falgout", "PMIS", "HMIS", "Direct", "ModClassical", "Extended", "Classical",
     "Symmetric", "Jacobi", "MC_SOR", "MC_SSOR", "greedy_colors", "PREC_FP64", "PREC_FP32",
 ]
@@ -557,6 +557,29 @@ def split(s_rowptr, s_col, coarsen_type, weights=None):
     check(lib().ramd_host_split(n, _ptr(s_rowptr, C.c_int32), _ptr(s_col, C.c_int32), coarsen_type,
                                 None if w is None else _ptr(w, C.c_double), _ptr(states, C.c_int32)))
     return states
+
+
+def strength_gpu(A, theta):
+    """Classical strength of a device matrix on the device (ramd_strength_gpu)."""
+    h = C.c_void_p()
+    check(lib().ramd_strength_gpu(A.h, float(theta), C.byref(h)))
+    return ParCSRMatrix(A.ctx, h)
+
+
+def split_pmis_gpu(S, weights=None):
+    """PMIS C/F states of a device strength matrix (ramd_split_pmis_gpu)."""
+    states = np.empty(S.n_rows, np.int32)
+    w = None if weights is None else _f64(weights)
+    check(lib().ramd_split_pmis_gpu(S.h, None if w is None else _ptr(w, C.c_double), _ptr(states, C.c_int32)))
+    return states
+
+
+def interp_extended_gpu(A, S, states, filter_threshold=0.3):
+    """Extended interpolation (with filter) on the device (ramd_interp_extended_gpu)."""
+    states = _i32(states)
+    h = C.c_void_p()
+    check(lib().ramd_interp_extended_gpu(A.h, S.h, _ptr(states, C.c_int32), float(filter_threshold), C.byref(h)))
+    return ParCSRMatrix(A.ctx, h)
 
 
 def split_rs_serial(s_rowptr, s_col, second_pass=True):

@@ -1,0 +1,569 @@
+// Hierarchy setup on the device for the RS path the 27-point benchmarks use:
+// classical strength (par_strength.cpp:8-340), PMIS splitting
+// (ruge_stuben/par_cf_splitting.cpp:75-89, 1229-1381) and extended
+// interpolation with its filter (ruge_stuben/par_interpolation.cpp:175-989),
+// each with the same decisions and the same per-row arithmetic order as the
+// host restatement (host/setup.cpp), so S, the C/F states and P are
+// bit-identical.  Row pointers are formed on the host from device counts
+// (the CSR handle keeps a host copy of them anyway).
+#include <cmath>
+#include <climits>
+#include <cstring>
+#include <vector>
+
+#include "hier.hpp"
+#include "host/setup.hpp"
+
+namespace ramd {
+
+constexpr int kSelectedState = 1, kUnselectedState = 0, kUnassignedState = -1, kNoNeighborsState = -2;
+constexpr int kNewSelectionState = 3, kNewUnselectionState = 2;
+constexpr double kSetupZeroTol = 1e-16;
+
+// ---------------------------------------------------------------------------
+// classical strength: one thread per row.  The row keeps its diagonal slot
+// first (the stored diagonal, or (i, 0) when the first entry is not the
+// diagonal), then every off-diagonal beyond theta times the row's extreme
+// off-diagonal of the sign opposite to the diagonal, in stored order.
+// ---------------------------------------------------------------------------
+template <bool FILL>
+__global__ void strength_kernel(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                const double* __restrict__ val, double theta, int* __restrict__ cnt,
+                                const int* __restrict__ srp, int* __restrict__ scol, double* __restrict__ sval)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = rp[i];
+    const int e = rp[i + 1];
+    if (e == s) {
+        if (!FILL) cnt[i] = 0;
+        return;
+    }
+    double diag = 0.0;
+    if (col[s] == i) {
+        diag = val[s];
+        s++;
+    }
+    double row_scale;
+    if (diag < 0.0) {
+        row_scale = -(double)INT_MAX;
+        for (int j = s; j < e; j++)
+            if (val[j] > row_scale) row_scale = val[j];
+    } else {
+        row_scale = (double)INT_MAX;
+        for (int j = s; j < e; j++)
+            if (val[j] < row_scale) row_scale = val[j];
+    }
+    const double threshold = row_scale * theta;
+    int o = FILL ? srp[i] : 0;
+    if (FILL) {
+        scol[o] = i;
+        sval[o] = diag;
+    }
+    o++;
+    for (int j = s; j < e; j++) {
+        const double v = val[j];
+        if (diag < 0 ? v > threshold : v < threshold) {
+            if (FILL) {
+                scol[o] = col[j];
+                sval[o] = v;
+            }
+            o++;
+        }
+    }
+    if (!FILL) cnt[i] = o;
+}
+
+// counts -> host exclusive prefix sum -> device row pointers
+static std::vector<int> host_rowptr(ramd_ctx ctx, const int* d_cnt, int n)
+{
+    std::vector<int> cnt((size_t)n), rp((size_t)n + 1, 0);
+    if (n) {
+        RAMD_HIP(hipMemcpyAsync(cnt.data(), d_cnt, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+        RAMD_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    int64_t acc = 0;
+    for (int i = 0; i < n; i++) {
+        acc += cnt[i];
+        RAMD_CHECK(acc <= INT32_MAX, RAMD_E_ARG, "setup product exceeds 2^31 stored entries");
+        rp[i + 1] = (int)acc;
+    }
+    return rp;
+}
+
+static dim3 rows_grid(int n) { return dim3((unsigned)std::max(1, (n + 255) / 256)); }
+
+ramd_csr_s* gpu_classical_strength(ramd_csr_s* A, double theta)
+{
+    ramd_ctx ctx = A->ctx;
+    hipStream_t s = ctx->stream;
+    const int n = A->n_rows;
+    int* cnt = dalloc<int>((size_t)std::max(n, 1));
+    std::vector<int> rp;
+    try {
+        hipLaunchKernelGGL((strength_kernel<false>), rows_grid(n), dim3(256), 0, s, n, A->rowptr, A->col, A->val,
+                           theta, cnt, nullptr, nullptr, nullptr);
+        RAMD_HIP(hipGetLastError());
+        rp = host_rowptr(ctx, cnt, n);
+    } catch (...) {
+        dfree(cnt);
+        throw;
+    }
+    dfree(cnt);
+    const int64_t nnz = rp[n];
+    int* d_rp = dalloc<int>((size_t)n + 1);
+    int* d_col = dalloc<int>((size_t)std::max<int64_t>(nnz, 1));
+    double* d_val = dalloc<double>((size_t)std::max<int64_t>(nnz, 1));
+    RAMD_HIP(hipMemcpyAsync(d_rp, rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((strength_kernel<true>), rows_grid(n), dim3(256), 0, s, n, A->rowptr, A->col, A->val, theta,
+                       nullptr, d_rp, d_col, d_val);
+    RAMD_HIP(hipGetLastError());
+    return csr_wrap(ctx, n, A->n_cols, nnz, d_rp, d_col, d_val, rp.data());
+}
+
+// ---------------------------------------------------------------------------
+// PMIS.  Every step of a round reads only what the previous step wrote:
+// selection reads weights, marking reads the new selections, the update
+// rewrites each vertex from its own state, so one thread per vertex gives
+// the sequential loop's result.  The transpose of S (rows depending on a
+// column) is built with atomics; only its sets are used.
+// ---------------------------------------------------------------------------
+__global__ void pmis_init_kernel(int n, const int* __restrict__ srp, const int* __restrict__ scol,
+                                 int* __restrict__ states, int* __restrict__ tcnt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = srp[i];
+    const int e = srp[i + 1];
+    states[i] = (e - s > 1) ? kUnassignedState : kNoNeighborsState;
+    if (s < e && scol[s] == i) s++;
+    for (int j = s; j < e; j++) atomicAdd(&tcnt[scol[j]], 1);
+}
+
+__global__ void pmis_transpose_kernel(int n, const int* __restrict__ srp, const int* __restrict__ scol,
+                                      const int* __restrict__ tptr, int* __restrict__ tfill, int* __restrict__ tidx)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = srp[i];
+    const int e = srp[i + 1];
+    if (s < e && scol[s] == i) s++;
+    for (int j = s; j < e; j++) {
+        const int c = scol[j];
+        tidx[tptr[c] + atomicAdd(&tfill[c], 1)] = i;
+    }
+}
+
+// initial_weights: the random value plus 1.0 once per row that strongly
+// depends on i (added one at a time, as there); then the first pass over
+// unassigned vertices with weight < 1
+__global__ void pmis_weights_kernel(int n, const double* __restrict__ rnd, const int* __restrict__ tptr,
+                                    int* __restrict__ states, double* __restrict__ w)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = rnd[i];
+    for (int k = tptr[i]; k < tptr[i + 1]; k++) v += 1;
+    if (states[i] == kUnassignedState && v < 1) {
+        states[i] = kUnselectedState;
+        v = 0.0;
+    } else if (states[i] != kUnassignedState) {
+        v = 0.0;
+    }
+    w[i] = v;
+}
+
+__global__ void pmis_select_kernel(int n, const int* __restrict__ srp, const int* __restrict__ scol,
+                                   const int* __restrict__ tptr, const int* __restrict__ tidx,
+                                   const double* __restrict__ w, int* __restrict__ states)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n || states[u] != kUnassignedState) return;
+    const double weight = w[u];
+    if (0.0 > weight) return;
+    int s = srp[u];
+    const int e = srp[u + 1];
+    if (s < e && scol[s] == u) s++;
+    for (int j = s; j < e; j++)
+        if (w[scol[j]] > weight) return;
+    for (int j = tptr[u]; j < tptr[u + 1]; j++)
+        if (w[tidx[j]] > weight) return;
+    states[u] = kNewSelectionState;
+}
+
+__global__ void pmis_mark_kernel(int n, const int* __restrict__ tptr, const int* __restrict__ tidx,
+                                 int* __restrict__ states)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n || states[u] != kNewSelectionState) return;
+    for (int j = tptr[u]; j < tptr[u + 1]; j++) {
+        const int r = tidx[j];
+        if (states[r] == kUnassignedState) states[r] = kNewUnselectionState;
+    }
+}
+
+__global__ void pmis_update_kernel(int n, int* __restrict__ states, double* __restrict__ w,
+                                   int* __restrict__ remaining)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    const int st = states[u];
+    if (st == kNewSelectionState) {
+        w[u] = 0.0;
+        states[u] = kSelectedState;
+    } else if (st == kUnassignedState || st == kNewUnselectionState) {
+        if (w[u] < 1.0 || st == kNewUnselectionState) {
+            w[u] = 0.0;
+            states[u] = kUnselectedState;
+        } else {
+            atomicAdd(remaining, 1);
+        }
+    }
+}
+
+// states (device, n ints, caller-owned) of PMIS on S; weights = the random
+// values (host, n)
+void gpu_split_pmis(const ramd_csr_s* S, const double* weights, int* states)
+{
+    ramd_ctx ctx = S->ctx;
+    hipStream_t s = ctx->stream;
+    const int n = S->n_rows;
+    struct Bufs {
+        int *tcnt = nullptr, *tptr = nullptr, *tidx = nullptr, *rem = nullptr;
+        double *rnd = nullptr, *w = nullptr;
+        ~Bufs()
+        {
+            dfree(tcnt);
+            dfree(tptr);
+            dfree(tidx);
+            dfree(rem);
+            dfree(rnd);
+            dfree(w);
+        }
+    } b;
+    const size_t m = (size_t)std::max(n, 1);
+    b.tcnt = dalloc<int>(m);
+    b.rnd = dalloc<double>(m);
+    b.w = dalloc<double>(m);
+    b.rem = dalloc<int>(1);
+    RAMD_HIP(hipMemsetAsync(b.tcnt, 0, sizeof(int) * m, s));
+    RAMD_HIP(hipMemcpyAsync(b.rnd, weights, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(pmis_init_kernel, rows_grid(n), dim3(256), 0, s, n, S->rowptr, S->col, states, b.tcnt);
+    RAMD_HIP(hipGetLastError());
+    std::vector<int> tp = host_rowptr(ctx, b.tcnt, n);
+    b.tptr = dalloc<int>((size_t)n + 1);
+    b.tidx = dalloc<int>((size_t)std::max(tp[n], 1));
+    RAMD_HIP(hipMemcpyAsync(b.tptr, tp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+    RAMD_HIP(hipMemsetAsync(b.tcnt, 0, sizeof(int) * m, s));  // reused as fill counters
+    hipLaunchKernelGGL(pmis_transpose_kernel, rows_grid(n), dim3(256), 0, s, n, S->rowptr, S->col, b.tptr, b.tcnt,
+                       b.tidx);
+    hipLaunchKernelGGL(pmis_weights_kernel, rows_grid(n), dim3(256), 0, s, n, b.rnd, b.tptr, states, b.w);
+    RAMD_HIP(hipGetLastError());
+    for (int round = 0;; round++) {
+        RAMD_CHECK(round <= n + 1, RAMD_E_NUMERIC, "PMIS did not finish");
+        RAMD_HIP(hipMemsetAsync(b.rem, 0, sizeof(int), s));
+        hipLaunchKernelGGL(pmis_select_kernel, rows_grid(n), dim3(256), 0, s, n, S->rowptr, S->col, b.tptr, b.tidx,
+                           b.w, states);
+        hipLaunchKernelGGL(pmis_mark_kernel, rows_grid(n), dim3(256), 0, s, n, b.tptr, b.tidx, states);
+        hipLaunchKernelGGL(pmis_update_kernel, rows_grid(n), dim3(256), 0, s, n, states, b.w, b.rem);
+        RAMD_HIP(hipGetLastError());
+        int rem = 0;
+        RAMD_HIP(hipMemcpyAsync(&rem, b.rem, sizeof(int), hipMemcpyDeviceToHost, s));
+        RAMD_HIP(hipStreamSynchronize(s));
+        if (rem == 0) break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Extended interpolation, one wavefront per row.  The row's columns are
+// collected in discovery order (strong C neighbours, and the strong C
+// neighbours of strong F neighbours, first occurrence kept) in LDS; every
+// later step walks the same loops as the host in the same order with all
+// lanes in step, so each entry and the weak sum receive their additions in
+// the same sequence.  Lookups are a wavefront-wide compare over the list.
+// A row with more than kExtCols columns sets *overflow (the caller then
+// builds this level on the host).
+// ---------------------------------------------------------------------------
+constexpr int kExtCols = 256;
+constexpr int kExtWaves = 4;
+
+// LDS written by one lane becomes visible to the wavefront's other lanes
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(64 * kExtWaves) void ext_interp_kernel(
+    int n, const int* __restrict__ arp, const int* __restrict__ acol, const double* __restrict__ aval,
+    const int* __restrict__ srp, const int* __restrict__ scol, const int* __restrict__ states,
+    const int* __restrict__ cmap, double filter, int* __restrict__ cnt, const int* __restrict__ prp,
+    int* __restrict__ pcol, double* __restrict__ pval, int* __restrict__ overflow)
+{
+    __shared__ int lcol[kExtWaves][kExtCols];
+    __shared__ double lval[kExtWaves][kExtCols];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * kExtWaves + wv;
+    if (i >= n) return;  // the whole wavefront leaves together
+    int* lc = lcol[wv];
+    double* lv = lval[wv];
+    const int st = states[i];
+    if (st != kUnselectedState) {
+        if (lane == 0) {
+            if (st == kSelectedState) {
+                if (FILL) {
+                    pcol[prp[i]] = cmap[i];
+                    pval[prp[i]] = 1;
+                } else {
+                    cnt[i] = 1;
+                }
+            } else if (!FILL) {
+                cnt[i] = 0;
+            }
+        }
+        return;
+    }
+    int len = 0;
+    bool over = false;
+    // position of c in lc[0, len) or -1 (every lane gets the answer)
+    auto find = [&](int c) -> int {
+        for (int b0 = 0; b0 < len; b0 += 64) {
+            const int k = b0 + lane;
+            const unsigned long long m = __ballot(k < len && lc[k] == c);
+            if (m) return b0 + __ffsll((long long)m) - 1;
+        }
+        return -1;
+    };
+    auto add_col = [&](int c) {
+        if (over || find(c) >= 0) return;
+        if (len == kExtCols) {
+            over = true;
+            return;
+        }
+        if (lane == 0) lc[len] = c;
+        wave_sync();
+        len++;
+    };
+    // discovery (host: the first loop over S row i)
+    const int s_end = srp[i + 1];
+    for (int j = srp[i] + 1; j < s_end; j++) {
+        const int c = scol[j];
+        const int sc = states[c];
+        if (sc == kSelectedState) {
+            add_col(c);
+        } else if (sc == kUnselectedState) {
+            for (int k = srp[c] + 1; k < srp[c + 1]; k++) {
+                const int ck = scol[k];
+                if (states[ck] == kSelectedState) add_col(ck);
+            }
+        }
+    }
+    if (over) {
+        if (lane == 0) {
+            atomicOr(overflow, 1);
+            if (!FILL) cnt[i] = 0;
+        }
+        return;
+    }
+    for (int k = lane; k < len; k += 64) lv[k] = 0.0;
+    wave_sync();
+    // accumulation (host: the loop over A row i); lane (idx % 64) owns entry idx
+    int start = arp[i];
+    const int end = arp[i + 1];
+    double weak_sum = aval[start++];
+    int ctr = srp[i] + 1;
+    for (int j = start; j < end; j++) {
+        const int c = acol[j];
+        const double v = aval[j];
+        const int idx = find(c);
+        if (idx >= 0) {
+            if (lane == (idx & 63)) lv[idx] += v;
+            if (ctr < s_end && scol[ctr] == c) ctr++;
+        } else if (ctr < s_end && scol[ctr] == c) {
+            ctr++;
+            if (states[c] != kUnselectedState) continue;
+            double col_sum = 0;
+            int sk = arp[c];
+            const int ek = arp[c + 1];
+            const double diag = aval[sk++];
+            const int sign = diag > 0 ? 1 : -1;
+            for (int k = sk; k < ek; k++) {
+                const double vk = aval[k];
+                const int ck = acol[k];
+                if (vk * sign < 0 && (ck == i || find(ck) >= 0)) col_sum += vk;
+            }
+            if (fabs(col_sum) < kSetupZeroTol) {
+                weak_sum += v;
+            } else {
+                col_sum = v / col_sum;
+                for (int k = arp[c] + 1; k < ek; k++) {
+                    const int ck = acol[k];
+                    const double vk = aval[k];
+                    if (!(vk * sign < 0)) continue;
+                    if (ck == i) {
+                        weak_sum += (col_sum * vk);
+                    } else {
+                        const int pk = find(ck);
+                        if (pk >= 0 && lane == (pk & 63)) lv[pk] += (col_sum * vk);
+                    }
+                }
+            }
+        } else {
+            if (states[c] != kNoNeighborsState) weak_sum += v;
+        }
+        wave_sync();
+    }
+    if (fabs(weak_sum) > kSetupZeroTol)
+        for (int k = lane; k < len; k += 64) lv[k] /= -weak_sum;
+    wave_sync();
+    // filter (host: filter_rows), one lane, entries in order
+    if (lane == 0) {
+        int kept = len;
+        if (filter > kSetupZeroTol && filter <= 1) {
+            double row_max = 0;
+            for (int k = 0; k < len; k++) {
+                const double a = fabs(lv[k]);
+                if (a > row_max) row_max = a;
+            }
+            row_max *= filter;
+            double row_sum = 0, remain_sum = 0;
+            kept = 0;
+            for (int k = 0; k < len; k++) {
+                const double v = lv[k];
+                row_sum += v;
+                if (fabs(v) >= row_max) {
+                    lc[kept] = lc[k];
+                    lv[kept] = v;
+                    kept++;
+                    remain_sum += v;
+                }
+            }
+            if (fabs(remain_sum) > kSetupZeroTol && fabs(row_sum - remain_sum) > kSetupZeroTol) {
+                const double row_scale = row_sum / remain_sum;
+                for (int k = 0; k < kept; k++) lv[k] *= row_scale;
+            }
+        }
+        if (FILL) {
+            const int o = prp[i];
+            for (int k = 0; k < kept; k++) {
+                pcol[o + k] = cmap[lc[k]];
+                pval[o + k] = lv[k];
+            }
+        } else {
+            cnt[i] = kept;
+        }
+    }
+}
+
+// P on the device, or nullptr when a row needs more than kExtCols columns
+ramd_csr_s* gpu_extended_interpolation(const ramd_csr_s* A, const ramd_csr_s* S, const int* states_d,
+                                       const std::vector<int>& states_h, double filter)
+{
+    ramd_ctx ctx = A->ctx;
+    hipStream_t s = ctx->stream;
+    const int n = A->n_rows;
+    std::vector<int> cmap((size_t)n, -1);
+    int nc = 0;
+    for (int i = 0; i < n; i++)
+        if (states_h[i] == kSelectedState) cmap[i] = nc++;
+    struct Bufs {
+        int *cmap = nullptr, *cnt = nullptr, *over = nullptr;
+        ~Bufs()
+        {
+            dfree(cmap);
+            dfree(cnt);
+            dfree(over);
+        }
+    } b;
+    b.cmap = dalloc<int>((size_t)std::max(n, 1));
+    b.cnt = dalloc<int>((size_t)std::max(n, 1));
+    b.over = dalloc<int>(1);
+    RAMD_HIP(hipMemcpyAsync(b.cmap, cmap.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    RAMD_HIP(hipMemsetAsync(b.over, 0, sizeof(int), s));
+    const dim3 grid((unsigned)std::max(1, (n + kExtWaves - 1) / kExtWaves)), block(64 * kExtWaves);
+    hipLaunchKernelGGL((ext_interp_kernel<false>), grid, block, 0, s, n, A->rowptr, A->col, A->val, S->rowptr,
+                       S->col, states_d, b.cmap, filter, b.cnt, nullptr, nullptr, nullptr, b.over);
+    RAMD_HIP(hipGetLastError());
+    int over = 0;
+    RAMD_HIP(hipMemcpyAsync(&over, b.over, sizeof(int), hipMemcpyDeviceToHost, s));
+    std::vector<int> rp = host_rowptr(ctx, b.cnt, n);  // synchronizes
+    if (over) return nullptr;
+    const int64_t nnz = rp[n];
+    int* d_rp = dalloc<int>((size_t)n + 1);
+    int* d_col = dalloc<int>((size_t)std::max<int64_t>(nnz, 1));
+    double* d_val = dalloc<double>((size_t)std::max<int64_t>(nnz, 1));
+    RAMD_HIP(hipMemcpyAsync(d_rp, rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((ext_interp_kernel<true>), grid, block, 0, s, n, A->rowptr, A->col, A->val, S->rowptr, S->col,
+                       states_d, b.cmap, filter, nullptr, d_rp, d_col, d_val, b.over);
+    RAMD_HIP(hipGetLastError());
+    RAMD_HIP(hipStreamSynchronize(s));
+    return csr_wrap(ctx, n, nc, nnz, d_rp, d_col, d_val, rp.data());
+}
+
+}  // namespace ramd
+
+using namespace ramd;
+
+extern "C" {
+
+int ramd_strength_gpu(ramd_csr A, double theta, ramd_csr* S)
+{
+    return guarded([&] {
+        RAMD_CHECK(A && S, RAMD_E_ARG, "ramd_strength_gpu: NULL argument");
+        RAMD_CHECK(A->n_rows == A->n_cols, RAMD_E_ARG, "ramd_strength_gpu: A is not square");
+        A->ctx->set_current();
+        *S = gpu_classical_strength(A, theta);
+    });
+}
+
+int ramd_split_pmis_gpu(ramd_csr S, const double* weights, int32_t* states)
+{
+    return guarded([&] {
+        RAMD_CHECK(S && states, RAMD_E_ARG, "ramd_split_pmis_gpu: NULL argument");
+        RAMD_CHECK(S->n_rows == S->n_cols, RAMD_E_ARG, "ramd_split_pmis_gpu: S is not square");
+        ramd_ctx ctx = S->ctx;
+        ctx->set_current();
+        const int n = S->n_rows;
+        std::vector<double> w;
+        if (weights) w.assign(weights, weights + n);
+        else w = host::reference_weights(n, 0);
+        int* d = dalloc<int>((size_t)std::max(n, 1));
+        try {
+            gpu_split_pmis(S, w.data(), d);
+            if (n) RAMD_HIP(hipMemcpyAsync(states, d, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+            RAMD_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            dfree(d);
+            throw;
+        }
+        dfree(d);
+    });
+}
+
+int ramd_interp_extended_gpu(ramd_csr A, ramd_csr S, const int32_t* states, double filter, ramd_csr* P)
+{
+    return guarded([&] {
+        RAMD_CHECK(A && S && states && P, RAMD_E_ARG, "ramd_interp_extended_gpu: NULL argument");
+        RAMD_CHECK(A->n_rows == S->n_rows && A->n_rows == A->n_cols, RAMD_E_ARG,
+                   "ramd_interp_extended_gpu: size mismatch");
+        ramd_ctx ctx = A->ctx;
+        ctx->set_current();
+        const int n = A->n_rows;
+        std::vector<int> st(states, states + n);
+        int* d = dalloc<int>((size_t)std::max(n, 1));
+        ramd_csr_s* out = nullptr;
+        try {
+            RAMD_HIP(hipMemcpyAsync(d, st.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+            out = gpu_extended_interpolation(A, S, d, st, filter);
+        } catch (...) {
+            dfree(d);
+            throw;
+        }
+        dfree(d);
+        RAMD_CHECK(out, RAMD_E_ARG, "ramd_interp_extended_gpu: a row needs more than %d columns", kExtCols);
+        *P = out;
+    });
+}
+
+}  // extern "C"

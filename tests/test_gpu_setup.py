@@ -1,0 +1,82 @@
+"""Setup steps on the device (classical strength, PMIS, extended
+interpolation with filter) against the host restatement, which is pinned to
+the reference's fixtures and the compiled reference (test_host_setup.py):
+S patterns, C/F states and P bit-identical."""
+import math
+
+import numpy as np
+import pytest
+
+import checkers as K
+import raptor_amd as ra
+
+pytestmark = pytest.mark.gpu
+
+
+def random_mmatrix(n, seed):
+    """Symmetric M-matrix with a random pattern (row sums >= 0)."""
+    rng = np.random.default_rng(seed)
+    rows = [dict() for _ in range(n)]
+    for i in range(n):
+        for j in rng.choice(n, size=rng.integers(1, 9), replace=False):
+            if j == i:
+                continue
+            v = -rng.uniform(0.1, 2.0)
+            rows[i][j] = v
+            rows[j][i] = v
+    rp, col, val = [0], [], []
+    for i in range(n):
+        d = -sum(rows[i].values()) + rng.uniform(0.0, 0.5)
+        items = sorted(rows[i].items())
+        col += [i] + [j for j, _ in items]
+        val += [d] + [v for _, v in items]
+        rp.append(len(col))
+    return np.array(rp, np.int32), np.array(col, np.int32), np.array(val)
+
+
+CASES = [
+    ("lap27_12", lambda: ra.gallery_laplace27(12)),
+    ("aniso50", lambda: ra.gallery_aniso2d(50, 0.001, math.pi / 8)),
+    ("fv7_10", lambda: ra.gallery_fv7(10, seed=4)),
+    ("random_mm_700", lambda: random_mmatrix(700, 5)),
+]
+
+
+def diag_first(rp, col, val):
+    A = K.move_diag(K.sort_rows(K.Csr(rp, col, val)))
+    return A
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.25, 0.5])
+@pytest.mark.parametrize("name,gen", CASES, ids=[c[0] for c in CASES])
+def test_strength_pmis_extended_match_host(ctx, name, gen, theta):
+    A = diag_first(*gen())
+    n = A.n_rows
+    D = ra.ParCSRMatrix.from_csr(ctx, A.rowptr, A.col, A.val)
+    S = ra.strength_gpu(D, theta)
+    srp, scol, sval = S.to_csr()
+    hrp, hcol = ra.strength(A.rowptr, A.col, A.val, ra.Classical, theta)
+    assert np.array_equal(srp, hrp) and np.array_equal(scol, hcol)
+    # S values are A's kept values (the diagonal slot first)
+    assert np.all(scol[srp[:-1]] == np.arange(n))
+    w = np.random.default_rng(6).random(n)
+    st = ra.split_pmis_gpu(S, w)
+    hst = ra.split(hrp, hcol, ra.PMIS, w)
+    assert np.array_equal(st, hst)
+    assert set(np.unique(st)) <= {-2, 0, 1}
+    for filt in (0.3, 0.0):
+        P = ra.interp_extended_gpu(D, S, st, filt)
+        prp, pcol, pval = P.to_csr()
+        hprp, hpcol, hpval, hnc = ra.interpolation(A.rowptr, A.col, A.val, hrp, hcol, hst, ra.Extended, filt)
+        assert P.n_cols == hnc
+        assert np.array_equal(prp, hprp) and np.array_equal(pcol, hpcol)
+        assert np.array_equal(pval, hpval)
+
+
+def test_pmis_reference_weights(ctx):
+    """Without weights both sides draw the reference's srand(2448422) stream."""
+    A = diag_first(*ra.gallery_laplace27(10))
+    D = ra.ParCSRMatrix.from_csr(ctx, A.rowptr, A.col, A.val)
+    S = ra.strength_gpu(D, 0.25)
+    hrp, hcol = ra.strength(A.rowptr, A.col, A.val, ra.Classical, 0.25)
+    assert np.array_equal(ra.split_pmis_gpu(S), ra.split(hrp, hcol, ra.PMIS))
