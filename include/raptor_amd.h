@@ -211,6 +211,9 @@ typedef struct {
                                (multilevel/par_multilevel.hpp:215-225)          */
     int prolong_smooth_steps;/* SA Jacobi prolongation steps (default 1)       */
     int relax_type;         /* RAMD_JACOBI (default), RAMD_MC_SOR, RAMD_MC_SSOR */
+    int device_setup;       /* 1 (default): classical strength, PMIS and
+                               extended interpolation run on the device
+                               (same results); 0: on the host             */
 } ramd_amg_opts;
 
 /* Fills opts with the reference's defaults for `solver`. */

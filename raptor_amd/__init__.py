@@ -264,6 +264,7 @@ class ParMultilevel:
         self.solve_tol = 1e-7
         self.max_iterations = 100
         self.weights = None
+        self.device_setup = True
         self.h = None
 
     def _opts(self):
@@ -275,6 +276,7 @@ class ParMultilevel:
         o.max_coarse = self.max_coarse
         o.max_levels = self.max_levels
         o.relax_type = self.relax_type
+        o.device_setup = 1 if self.device_setup else 0
         return o
 
     def setup(self, rowptr, col, val):

@@ -52,6 +52,7 @@ class AmgOpts(C.Structure):
         ("weights", C.POINTER(C.c_double)),
         ("prolong_smooth_steps", C.c_int),
         ("relax_type", C.c_int),
+        ("device_setup", C.c_int),
     ]
 
 

@@ -453,6 +453,54 @@ static Csr rs_level(ramd_hier_s* h, Csr& A, int level, const double* weights, st
     return P;
 }
 
+// The same level built on the device (classical strength, PMIS, extended
+// interpolation; results identical to rs_level's), when the options name
+// that combination.  A level whose rows would need more interpolation
+// columns than the device kernel holds is interpolated on the host.
+static bool device_rs(const ramd_amg_opts& o)
+{
+    return o.device_setup && o.solver == RAMD_SOLVER_RS && o.strength_type == RAMD_CLASSICAL &&
+           o.coarsen_type == RAMD_PMIS && o.interp_type == RAMD_EXTENDED;
+}
+
+static ramd_csr_s* rs_level_device(ramd_hier_s* h, int level, const double* weights, std::vector<int>& states)
+{
+    const ramd_amg_opts& o = h->opts;
+    ramd_ctx ctx = h->ctx;
+    Level& L = h->levels[level];
+    const int n = L.n;
+    double t0 = now_s();
+    ramd_csr_s* S = gpu_classical_strength(L.A, o.strong_threshold);
+    int* st = nullptr;
+    ramd_csr_s* P = nullptr;
+    try {
+        double t1 = now_s();
+        st = dalloc<int>((size_t)std::max(n, 1));
+        gpu_split_pmis(S, weights, st);
+        states.resize((size_t)n);
+        if (n) RAMD_HIP(hipMemcpyAsync(states.data(), st, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+        RAMD_HIP(hipStreamSynchronize(ctx->stream));
+        double t2 = now_s();
+        P = gpu_extended_interpolation(L.A, S, st, states, o.interp_filter);
+        if (!P) {
+            Csr Ah = download(L.A), Sh = download(S);
+            Csr Ph = host::extended_interpolation(Ah, Sh, states, o.interp_filter);
+            P = upload(ctx, Ph, false);
+        }
+        double t3 = now_s();
+        h->times[0] += t1 - t0;
+        h->times[1] += t2 - t1;
+        h->times[2] += t3 - t2;
+    } catch (...) {
+        csr_free(S);
+        dfree(st);
+        throw;
+    }
+    csr_free(S);
+    dfree(st);
+    return P;
+}
+
 // jacobi_prolongation aggregation/par_prolongation.cpp:7-103: P = T, then
 // `steps` times P = P - (omega D^-1 A) P with D the absolute row sums; the
 // products run on the device, the row scaling and the subtraction's layout
@@ -567,25 +615,35 @@ static void hier_setup(ramd_hier_s* h, Csr Af)
     h->levels[0].A = upload(ctx, Af);
     alloc_level_vectors(h, h->levels[0]);
     Csr A = std::move(Af);
+    bool host_A = true;  // A holds the current level (the device path skips the copy)
     // SA near-nullspace candidate (aggregation/par_smoothed_aggregation_solver.hpp:38-49)
     std::vector<double> B;
     if (o.solver == RAMD_SOLVER_SA) B.assign((size_t)A.n_rows, 1.0);
-    while (A.n_rows > o.max_coarse && (o.max_levels == -1 || (int)h->levels.size() < o.max_levels)) {
+    while (h->levels.back().n > o.max_coarse &&
+           (o.max_levels == -1 || (int)h->levels.size() < o.max_levels)) {
         const int l = (int)h->levels.size() - 1;
         Level& L = h->levels[l];
-        Csr P = o.solver == RAMD_SOLVER_SA ? sa_level(h, A, weights.data(), L.states, B)
-                                           : rs_level(h, A, l, weights.data(), L.states);
-        if (P.n_cols == 0) fail(RAMD_E_NUMERIC, "level %d produced no coarse points", l);
+        ramd_csr_s* Pd = nullptr;
+        if (device_rs(o)) {
+            Pd = rs_level_device(h, l, weights.data(), L.states);
+        } else {
+            if (!host_A) A = download(L.A);
+            Csr P = o.solver == RAMD_SOLVER_SA ? sa_level(h, A, weights.data(), L.states, B)
+                                               : rs_level(h, A, l, weights.data(), L.states);
+            Pd = upload(ctx, P);
+        }
+        L.P = Pd;
+        if (Pd->n_cols == 0) fail(RAMD_E_NUMERIC, "level %d produced no coarse points", l);
         double tg = now_s();
-        L.P = upload(ctx, P);
         L.R = csr_transpose(L.P);
         ramd_csr_s* Ac = galerkin(L.A, L.P);
         h->times[3] += now_s() - tg;
         h->levels.emplace_back();
         h->levels.back().A = Ac;
         alloc_level_vectors(h, h->levels.back());
-        A = download(Ac);
+        host_A = false;
     }
+    if (!host_A) A = download(h->levels.back().A);
     // duplicate_coarse (par_multilevel.hpp:229-339): dense row-major A_c, dgetrf_
     const int n = A.n_rows;
     RAMD_CHECK(n <= kCoarseMax, RAMD_E_ARG, "coarsest level has %d rows; the dense solve supports <= %d", n,
@@ -1092,6 +1150,7 @@ int ramd_amg_opts_default(int solver, ramd_amg_opts* o)
         o->weights = nullptr;
         o->prolong_smooth_steps = 1;
         o->relax_type = RAMD_JACOBI;
+        o->device_setup = 1;
     });
 }
 

@@ -68,6 +68,12 @@ __device__ __forceinline__ void coarse_solve_wave(int n, const double* __restric
     if (lane < n) x[(size_t)lane * stride] = (TX)v;
 }
 
+// Device setup steps (setup_gpu.hip), bit-identical to the host ones.
+ramd_csr_s* gpu_classical_strength(ramd_csr_s* A, double theta);
+void gpu_split_pmis(const ramd_csr_s* S, const double* weights, int* states);
+ramd_csr_s* gpu_extended_interpolation(const ramd_csr_s* A, const ramd_csr_s* S, const int* states_d,
+                                       const std::vector<int>& states_h, double filter);
+
 // One FP64 V-cycle from x = 0 (ParMultilevel::cycle on the fine level).
 void vcycle_from_zero(ramd_hier_s* h, double* x, const double* b);
 

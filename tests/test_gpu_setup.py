@@ -80,3 +80,27 @@ def test_pmis_reference_weights(ctx):
     S = ra.strength_gpu(D, 0.25)
     hrp, hcol = ra.strength(A.rowptr, A.col, A.val, ra.Classical, 0.25)
     assert np.array_equal(ra.split_pmis_gpu(S), ra.split(hrp, hcol, ra.PMIS))
+
+
+@pytest.mark.parametrize("name,gen", [("lap27_16", lambda: ra.gallery_laplace27(16)),
+                                      ("aniso80", lambda: ra.gallery_aniso2d(80, 0.001, math.pi / 8))])
+def test_hierarchy_device_setup_equals_host(ctx, name, gen):
+    """RS(PMIS, extended) hierarchies built with the device steps and with the
+    host steps: every level's A and P, and the C/F states, identical."""
+    rp, col, val = gen()
+    mls = []
+    for dev in (True, False):
+        ml = ra.ParRugeStubenSolver(ctx, 0.25, ra.PMIS, ra.Extended, ra.Classical, ra.Jacobi)
+        ml.device_setup = dev
+        ml.setup(rp, col, val)
+        mls.append(ml)
+    a, b = mls
+    assert a.num_levels == b.num_levels >= 3
+    for l in range(a.num_levels):
+        for M1, M2 in zip(a.level(l), b.level(l)):
+            if M1 is None:
+                assert M2 is None
+                continue
+            assert all(np.array_equal(u, v) for u, v in zip(M1.to_csr(), M2.to_csr()))
+        if l < a.num_levels - 1:
+            assert np.array_equal(a.states(l), b.states(l))
