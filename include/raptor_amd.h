@@ -102,6 +102,11 @@ int ramd_csr_set_dia_staging(ramd_csr A, int pp, int mode);
  * bits; the autotuner picks one per operator at hierarchy setup and at
  * FP32 enable.  No reference counterpart. */
 int ramd_csr_set_sliced(ramd_csr A, int prec, int on);
+/* The same choice for the lane-per-row kernel over the CSR arrays (one lane
+ * walks its own row; no extra storage): on = 1 selects it (and leaves the
+ * sliced form), 0 returns to the row-block kernel.  Identical bits.  No
+ * reference counterpart. */
+int ramd_csr_set_lane_rows(ramd_csr A, int prec, int on);
 
 /* ------------------------------------------------------------------ */
 /* SpMV family (FP64; each row summed left to right in stored order, as  */
@@ -240,7 +245,8 @@ int ramd_hier_states(ramd_hier h, int level, int32_t* states);
 /* Row-block caps (products per workgroup) the autotuner chose for the
  * CSR kernels: six entries per level, [A, P, R] x [FP64, FP32 vectors];
  * -1 where the sliced-row kernel (64 rows per slice, one lane per row) was
- * measured faster, 0 for a stencil-diagonal operator or an absent one.  No
+ * measured faster, 1 where the lane-per-row kernel over the CSR arrays was,
+ * 0 for a stencil-diagonal operator or an absent one.  No
  * reference counterpart. */
 int ramd_hier_row_caps(ramd_hier h, int32_t* caps);
 /* Host-seconds spent per setup phase: [strength, coarsen, interp, galerkin, other]. */

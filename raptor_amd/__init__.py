@@ -217,6 +217,12 @@ class ParCSRMatrix:
         the sliced-row form, False the row-block kernel (identical bits)."""
         check(lib().ramd_csr_set_sliced(self.h, int(prec), int(bool(on))))
 
+    def set_lane_rows(self, prec, on):
+        """CSR kernel for FP64 (prec 0) or FP32 (prec 1) vectors: True selects
+        one lane per row over the CSR arrays, False the row-block kernel
+        (identical bits)."""
+        check(lib().ramd_csr_set_lane_rows(self.h, int(prec), int(bool(on))))
+
     def set_values(self, vals):
         """New values (a device ParVector of nnz entries, stored order) for the same pattern."""
         check(lib().ramd_csr_set_values(self.h, vals.ptr))

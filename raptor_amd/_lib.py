@@ -86,6 +86,7 @@ _SIGS = {
     "ramd_csr_diagonals": (P, C.POINTER(C.c_int)),
     "ramd_csr_set_dia_staging": (P, C.c_int, C.c_int),
     "ramd_csr_set_sliced": (P, C.c_int, C.c_int),
+    "ramd_csr_set_lane_rows": (P, C.c_int, C.c_int),
     "ramd_spmv": (P, P, P),
     "ramd_spmv_append": (P, P, P),
     "ramd_residual": (P, P, P, P),

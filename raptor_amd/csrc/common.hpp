@@ -55,6 +55,7 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb)
     return x * q + (x < r ? x : r) + k;
 }
 
+
 // Workgroup sum in a fixed order (wavefront shuffle tree, then the four
 // wavefront sums in index order); the result is valid in thread 0.  Every
 // thread of the workgroup must call it.
@@ -192,6 +193,7 @@ struct ramd_csr_s {
     Dia* dia = nullptr;         // stencil-diagonal form (fine-level operators)
     ramd::Sliced* sl = nullptr; // sliced-row form (long-row operators)
     bool sl_on[2] = {false, false};  // per vector precision: use the sliced form
+    bool lr_on[2] = {false, false};  // per vector precision: one lane per CSR row
     int32_t n_rows = 0, n_cols = 0;
     int64_t nnz = 0;
     int* rowptr = nullptr;   // device, n_rows+1

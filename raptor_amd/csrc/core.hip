@@ -152,6 +152,8 @@ bool dia_build(ramd_csr_s* A, const int* rp, const int* col, const double* val)
 {
     const int n = A->n_rows;
     if (A->n_rows != A->n_cols || n < kDiaMinRows || A->nnz == 0) return false;
+    // the kernels gather x through 32-bit byte offsets
+    if ((int64_t)n * (int64_t)sizeof(double) >= ((int64_t)1 << 32)) return false;
     // distinct offsets through a small open-addressing table
     constexpr int T = 128;
     long long key[T];
@@ -730,6 +732,19 @@ int ramd_csr_set_sliced(ramd_csr A, int prec, int on)
             RAMD_HIP(hipStreamSynchronize(A->ctx->stream));
         }
         A->sl_on[prec] = on != 0;
+        if (on) A->lr_on[prec] = false;
+    });
+}
+
+int ramd_csr_set_lane_rows(ramd_csr A, int prec, int on)
+{
+    return guarded([&] {
+        RAMD_CHECK(A && (prec == 0 || prec == 1) && (on == 0 || on == 1), RAMD_E_ARG,
+                   "ramd_csr_set_lane_rows: bad argument");
+        RAMD_CHECK(!A->dia, RAMD_E_STATE, "ramd_csr_set_lane_rows: matrix uses its stencil-diagonal form");
+        RAMD_CHECK(prec == 0 || A->val32 || !on, RAMD_E_STATE, "ramd_csr_set_lane_rows: FP32 values were not built");
+        A->lr_on[prec] = on != 0;
+        if (on) A->sl_on[prec] = false;
     });
 }
 

@@ -724,16 +724,18 @@ void vcycle_from_zero(ramd_hier_s* h, double* x, const double* b) { cycle64(h, 0
 
 // Flexible-PCG vector kernels.  The reductions run on a fixed grid of
 // kRedBlocks workgroups, each summing a fixed strided subset, so every run
-// gives the same bits.
+// gives the same bits.  (A last-workgroup-finishes variant that would save
+// the final reduction's launch needs a device-scope release per workgroup,
+// which writes back the XCD's L2: measured slower.)
 
-// x += alpha p; r -= alpha Ap; r32 = (float) r when given; partial sums of
-// r^2.  alpha = rz / pAp is formed from the device scalars (every thread
-// computes the same value; workgroup 0 stores it for the direction update).
+// r -= alpha Ap; r32 = (float) r when given; partial sums of r^2.  alpha =
+// rz / pAp is formed from the device scalars (every thread computes the
+// same value; workgroup 0 stores it for the direction update, which also
+// takes the step x += alpha p).
 __global__ __launch_bounds__(kBlockThreads) void fpcg_update_norm(int n, const double* __restrict__ sc_rz,
                                                                   const double* __restrict__ sc_pap,
                                                                   double* __restrict__ sc_alpha,
-                                                                  const double* __restrict__ p,
-                                                                  const double* __restrict__ Ap, double* __restrict__ x,
+                                                                  const double* __restrict__ Ap,
                                                                   double* __restrict__ r, float* __restrict__ r32,
                                                                   double* __restrict__ partials)
 {
@@ -742,7 +744,6 @@ __global__ __launch_bounds__(kBlockThreads) void fpcg_update_norm(int n, const d
     double s = 0;
     const int stride = gridDim.x * kBlockThreads;
     for (int i = blockIdx.x * kBlockThreads + threadIdx.x; i < n; i += stride) {
-        x[i] += alpha * p[i];
         const double ri = r[i] - alpha * Ap[i];
         r[i] = ri;
         if (r32) r32[i] = (float)ri;
@@ -773,15 +774,31 @@ __global__ __launch_bounds__(kBlockThreads) void fpcg_dots(int n, const double* 
     }
 }
 
-// p = z + beta p with the flexible (Polak-Ribiere) beta = -alpha (z.Ap) / rz_old
-// formed from the device scalars
+// x += alpha p (when x is given: the step of this iteration, taken here so
+// that p is read once), then p = z + beta p with the flexible
+// (Polak-Ribiere) beta = -alpha (z.Ap) / rz_old from the device scalars
 template <typename Z>
 __global__ void fpcg_dir(int n, const double* __restrict__ sc_alpha, const double* __restrict__ sc_zap,
-                         const double* __restrict__ sc_rz_old, const Z* __restrict__ z, double* __restrict__ p)
+                         const double* __restrict__ sc_rz_old, const Z* __restrict__ z, double* __restrict__ p,
+                         double* __restrict__ x)
 {
-    const double beta = -(*sc_alpha) * (*sc_zap) / (*sc_rz_old);
+    const double alpha = *sc_alpha;
+    const double beta = -alpha * (*sc_zap) / (*sc_rz_old);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = (double)z[i] + beta * p[i];
+    if (i >= n) return;
+    const double pi = p[i];
+    if (x) x[i] += alpha * pi;
+    p[i] = (double)z[i] + beta * pi;
+}
+
+// x += alpha p with alpha from the device scalar (the step not yet taken
+// when the iteration stops or checks the true residual)
+__global__ void fpcg_step_x(int n, const double* __restrict__ sc_alpha, const double* __restrict__ p,
+                            double* __restrict__ x)
+{
+    const double alpha = *sc_alpha;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += alpha * p[i];
 }
 
 static int fixed_blocks(int n) { return std::max(1, std::min(kRedBlocks, (n + kBlockThreads - 1) / kBlockThreads)); }
@@ -834,7 +851,6 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
     float *r32 = h->pr32, *z32 = h->pz32;
     const int g = fixed_blocks(n);
     const dim3 gv = grid1(n), bv(256);
-    double* part = nullptr;
     // z = M^-1 r from the current r (FP32 form: from r32, which must hold (float) r)
     auto cycle = [&] {
         if (f32) cycle32(h, 0, z32, r32, true);
@@ -865,24 +881,37 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
         else RAMD_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         RAMD_HIP(hipGetLastError());
         dots(r, sc + cur);
-        part = ctx_partials(ctx, (size_t)g);
+        // the step x += alpha p of an iteration is taken by its direction
+        // update (one read of p), or here before x is read or returned
+        auto step_x = [&] {
+            hipLaunchKernelGGL(fpcg_step_x, gv, bv, 0, s, n, sc + 4, p, x_d);
+            RAMD_HIP(hipGetLastError());
+        };
         while (it < max_iter) {
-            // Ap = A p and p.Ap in one pass over A; then x += alpha p,
-            // r -= alpha Ap, ||r||^2 (and r32) in one pass; one host read
+            // Ap = A p and p.Ap in one pass over A; then r -= alpha Ap,
+            // ||r||^2 (and r32) in one pass; one host read
             launch_spmv_dot(A, f32, p, Ap, sc, s);
-            hipLaunchKernelGGL(fpcg_update_norm, dim3(g), dim3(kBlockThreads), 0, s, n, sc + cur, sc, sc + 4, p,
-                               Ap, x_d, r, f32 ? r32 : nullptr, part);
+            // (the partials buffer may have grown inside launch_spmv_dot)
+            double* part = ctx_partials(ctx, (size_t)g);
+            hipLaunchKernelGGL(fpcg_update_norm, dim3(g), dim3(kBlockThreads), 0, s, n, sc + cur, sc, sc + 4, Ap, r,
+                               f32 ? r32 : nullptr, part);
             RAMD_HIP(hipGetLastError());
             launch_reduce_final(ctx, part, g, sc + 1);
             double pr[2];
             fetch_scalars(ctx, sc, 2, pr);
             if (!(pr[0] > 0.0)) {
-                // x and r took the step already; the error is reported instead of a result
+                // x and r take the step; the error is reported instead of a result
+                step_x();
                 rc = std::isfinite(pr[0]) ? RAMD_E_INDEFINITE : RAMD_E_NUMERIC;
                 break;
             }
             it++;
             relres = std::sqrt(pr[1]) / scale;
+            bool x_done = false;
+            if (relres <= rtol || !std::isfinite(relres) || it == max_iter) {
+                step_x();
+                x_done = true;
+            }
             if (relres <= rtol) {
                 // confirm with the true residual; continue from it if the recurrence drifted
                 outer(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
@@ -903,8 +932,9 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
             cycle();
             // flexible beta with r_new - r_old = -alpha Ap
             dots(Ap, sc + nxt);
-            if (f32) hipLaunchKernelGGL((fpcg_dir<float>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + cur, z32, p);
-            else hipLaunchKernelGGL((fpcg_dir<double>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + cur, z, p);
+            double* xs = x_done ? nullptr : x_d;
+            if (f32) hipLaunchKernelGGL((fpcg_dir<float>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + cur, z32, p, xs);
+            else hipLaunchKernelGGL((fpcg_dir<double>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + cur, z, p, xs);
             RAMD_HIP(hipGetLastError());
             std::swap(cur, nxt);
         }
@@ -1308,7 +1338,7 @@ int ramd_hier_row_caps(ramd_hier h, int32_t* caps)
             for (int m = 0; m < 3; m++)
                 for (int p = 0; p < 2; p++)
                     caps[6 * l + 2 * m + p] =
-                        (M[m] && !M[m]->dia) ? (M[m]->sl_on[p] ? -1 : M[m]->rb[p].cap) : 0;
+                        (M[m] && !M[m]->dia) ? (M[m]->sl_on[p] ? -1 : M[m]->lr_on[p] ? 1 : M[m]->rb[p].cap) : 0;
         }
     });
 }

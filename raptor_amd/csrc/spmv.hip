@@ -154,25 +154,39 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
     }
 }
 
-// Sliced-row kernels: one lane per row over the sliced form; a lane adds its
-// row's entries in stored order with the same operations as phase 2 of
-// stream_kernel, so the bits are the same.  Every wavefront reads its
-// slice's columns and values as contiguous runs, eight entries in flight per
-// lane; padding (column -1) is loaded and dropped by a select.
-template <int OP, typename V, typename X>
+// One-lane-per-row kernels, over two layouts.  LAYOUT 0, the sliced form:
+// every wavefront reads its slice's columns and values as contiguous runs
+// and padding (column -1) is loaded and dropped by a select.  LAYOUT 1, the
+// CSR arrays themselves (short rows, no LDS staging): a lane walks its own
+// row, entries past the row's end are not loaded and count as padding.  A
+// lane adds its row's entries in stored order with the same operations as
+// phase 2 of stream_kernel, so the bits are the same; eight entries' loads
+// are in flight per lane.
+template <int OP, typename V, typename X, int LAYOUT>
 __global__ __launch_bounds__(kBlockThreads) void sliced_kernel(
-    int n, const int64_t* __restrict__ sptr, const int* __restrict__ col, const V* __restrict__ val,
-    const X* __restrict__ x, const X* __restrict__ b, X* __restrict__ y, X omega, int diag_first,
-    double* __restrict__ dotp, const X* __restrict__ dvec)
+    int n, const int64_t* __restrict__ sptr, const int* __restrict__ rowptr, const int* __restrict__ col,
+    const V* __restrict__ val, const X* __restrict__ x, const X* __restrict__ b, X* __restrict__ y, X omega,
+    int diag_first, double* __restrict__ dotp, const X* __restrict__ dvec)
 {
     constexpr int B = 8;
+    constexpr int64_t STEP = LAYOUT == 0 ? 64 : 1;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int row = blk * kBlockThreads + threadIdx.x;
     const bool valid = row < n;
     const int lane = threadIdx.x & 63;
     // lanes past n belong to no slice when n is not a multiple of 64 * 4
-    const int64_t base = valid ? sptr[row >> 6] + lane : 0;
-    const int w = valid ? (int)((sptr[(row >> 6) + 1] - sptr[row >> 6]) >> 6) : 0;
+    int64_t base = 0;
+    int w = 0;
+    if (valid) {
+        if (LAYOUT == 0) {
+            base = sptr[row >> 6] + lane;
+            w = (int)((sptr[(row >> 6) + 1] - sptr[row >> 6]) >> 6);
+        } else {
+            base = rowptr[row];
+            w = rowptr[row + 1] - (int)base;
+        }
+    }
+    (void)lane;
     X acc = (OP == OP_RESIDUAL && valid) ? b[row] : X(0);
     X diag = 0;
     // Jacobi: first entry of the row to add (w: none -- the row keeps x)
@@ -200,8 +214,8 @@ __global__ __launch_bounds__(kBlockThreads) void sliced_kernel(
 #pragma unroll
         for (int t = 0; t < B; t++) {
             const bool in = k0 + t < w;
-            cc[t] = in ? col[base + (int64_t)64 * (k0 + t)] : -1;
-            vv[t] = in ? val[base + (int64_t)64 * (k0 + t)] : V(0);
+            cc[t] = in ? col[base + STEP * (k0 + t)] : -1;
+            vv[t] = in ? val[base + STEP * (k0 + t)] : V(0);
         }
 #pragma unroll
         for (int t = 0; t < B; t++) xx[t] = x[cc[t] < 0 ? 0 : cc[t]];
@@ -253,6 +267,15 @@ constexpr int dia_chunk() { return sizeof(V) == 4 ? 28 : 10; }
 template <typename V, typename X>
 constexpr int dia_xchunk() { return (sizeof(V) == 4 && sizeof(X) == 8) ? 9 : dia_chunk<V>(); }
 
+// x[c] for 0 <= c < n through a 32-bit byte offset from the uniform base,
+// so the load takes the scalar-base + vector-offset form (one address VGPR
+// per gather); the diagonal form is built only where n * 8 < 2^32.
+template <typename X>
+__device__ __forceinline__ X dia_gather(const X* __restrict__ x, int c)
+{
+    return *reinterpret_cast<const X*>(reinterpret_cast<const char*>(x) + (unsigned)c * (unsigned)sizeof(X));
+}
+
 // Values of slots [c0, c0 + CH) of `row`; groups past the stored slots (nsl,
 // a multiple of the group size) read nothing and give 0.
 template <typename V, int CH, int ND>
@@ -294,20 +317,24 @@ __device__ __forceinline__ X dia_row(int row, int n, const DiaArgs& a, int nd, i
 #pragma unroll
         for (int x0 = 0; x0 < CH; x0 += XC) {
             X xx[XC];
-            bool on[XC];
+            // the gathers first: their addresses do not wait for the mask
 #pragma unroll
             for (int t = 0; t < XC; t++) {
                 const int kk = c0 + x0 + t;
                 if (x0 + t < CH && kk < ND) {
                     const bool live = (ND == kDiaMax) ? (kk < nd) : true;
-                    const int k = live ? a.ord[kk] : 0;
-                    on[t] = live && ((m >> k) & 1u);
-                    xx[t] = x[dia_col(row, a.off[k], n)];
+                    xx[t] = dia_gather(x, dia_col(row, a.off[live ? a.ord[kk] : 0], n));
                 }
             }
 #pragma unroll
-            for (int t = 0; t < XC; t++)
-                if (x0 + t < CH && c0 + x0 + t < ND) acc = on[t] ? acc + X(vv[x0 + t]) * xx[t] : acc;
+            for (int t = 0; t < XC; t++) {
+                const int kk = c0 + x0 + t;
+                if (x0 + t < CH && kk < ND) {
+                    const bool live = (ND == kDiaMax) ? (kk < nd) : true;
+                    const bool on = live && ((m >> (live ? a.ord[kk] : 0)) & 1u);
+                    acc = on ? acc + X(vv[x0 + t]) * xx[t] : acc;
+                }
+            }
         }
     }
     (void)b;
@@ -347,16 +374,20 @@ __global__ __launch_bounds__(kBlockThreads) void dia_kernel(int n, DiaArgs a, in
             X xx[XC];
             bool on[XC];
             int kd[XC];
+            // the gathers first: their addresses do not wait for the mask
 #pragma unroll
             for (int t = 0; t < XC; t++) {
                 const int kk = c0 + x0 + t;
                 if (x0 + t < CH && kk < ND) {
                     const bool live = (ND == kDiaMax) ? (kk < nd) : true;
-                    const int k = live ? a.ord[kk] : 0;
-                    kd[t] = k;
-                    on[t] = live && ((m >> k) & 1u);
-                    xx[t] = x[dia_col(row, a.off[k], n)];
+                    kd[t] = live ? a.ord[kk] : 0;
+                    xx[t] = dia_gather(x, dia_col(row, a.off[kd[t]], n));
                 }
+            }
+#pragma unroll
+            for (int t = 0; t < XC; t++) {
+                const int kk = c0 + x0 + t;
+                if (x0 + t < CH && kk < ND) on[t] = ((ND == kDiaMax) ? (kk < nd) : true) && ((m >> kd[t]) & 1u);
             }
 #pragma unroll
             for (int t = 0; t < XC; t++) {
@@ -575,21 +606,28 @@ static void launch_stream_cap(const ramd_csr_s* A, const ramd_csr_s::RowBlocks& 
     }
 }
 
-template <typename V, typename X>
-static void launch_sliced(const ramd_csr_s* A, int op, const X* x, const X* b, X* y, X omega, hipStream_t s,
-                          double* dotp, const X* dvec)
+// LAYOUT 0: the sliced form; 1: lanes over the CSR rows (val: CSR values)
+template <typename V, typename X, int LAYOUT>
+static void launch_sliced(const ramd_csr_s* A, const V* csr_val, int op, const X* x, const X* b, X* y, X omega,
+                          hipStream_t s, double* dotp, const X* dvec)
 {
     const Sliced* S = A->sl;
-    const V* val;
-    if constexpr (sizeof(V) == 4) val = S->v32;
-    else val = S->v64;
-    RAMD_CHECK(val, RAMD_E_STATE, "sliced values of this precision were not built");
+    const V* val = csr_val;
+    const int64_t* sptr = nullptr;
+    const int* col = A->col;
+    if (LAYOUT == 0) {
+        if constexpr (sizeof(V) == 4) val = S->v32;
+        else val = S->v64;
+        sptr = S->sptr;
+        col = S->col;
+    }
+    RAMD_CHECK(val, RAMD_E_STATE, "values of this precision were not built for the lane-per-row kernel");
     const int n = A->n_rows;
     const dim3 grid((n + kBlockThreads - 1) / kBlockThreads), block(kBlockThreads);
     const int df = (int)A->diag_first;
-#define SLICED(OPV, DOT, DV)                                                                                     \
-    hipLaunchKernelGGL((sliced_kernel<OPV, V, X>), grid, block, 0, s, n, S->sptr, S->col, val, x, b, y, omega, df, \
-                       DOT, DV)
+#define SLICED(OPV, DOT, DV)                                                                                    \
+    hipLaunchKernelGGL((sliced_kernel<OPV, V, X, LAYOUT>), grid, block, 0, s, n, sptr, A->rowptr, col, val, x, b, \
+                       y, omega, df, DOT, DV)
     switch (op) {
     case OP_SPMV: SLICED(OP_SPMV, dotp, nullptr); break;
     case OP_APPEND: SLICED(OP_APPEND, nullptr, nullptr); break;
@@ -628,8 +666,9 @@ static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int 
         }
         return;
     }
-    if (A->sl && A->sl_on[sizeof(X) == 4 ? 1 : 0]) {
-        launch_sliced<V, X>(A, op, x, b, y, omega, s, dotp, dvec);
+    if (A->lr_on[sizeof(X) == 4 ? 1 : 0] || (A->sl && A->sl_on[sizeof(X) == 4 ? 1 : 0])) {
+        if (A->lr_on[sizeof(X) == 4 ? 1 : 0]) launch_sliced<V, X, 1>(A, val, op, x, b, y, omega, s, dotp, dvec);
+        else launch_sliced<V, X, 0>(A, val, op, x, b, y, omega, s, dotp, dvec);
         if (prof) {
             RAMD_HIP(hipEventRecord(pend.end, s));
             ctx->prof_pending.push_back(pend);
@@ -703,6 +742,7 @@ void csr_autotune(ramd_csr_s* A, int prec)
         A->rb[prec] = {kStreamCap, A->n_blocks, A->blocks};
     }
     A->sl_on[prec] = false;
+    A->lr_on[prec] = false;
     const size_t w = prec ? sizeof(float) : sizeof(double);
     struct Scratch {
         void *x = nullptr, *b = nullptr, *y = nullptr;
@@ -726,6 +766,23 @@ void csr_autotune(ramd_csr_s* A, int prec)
     RAMD_HIP(hipEventCreate(&e1));
     ramd_csr_s::RowBlocks best = A->rb[prec];
     float best_ms = 1e30f;
+    // one warm-up launch, then the time of three
+    auto time_ms = [&] {
+        auto run = [&] {
+            if (prec) launch_stream<float, float>(A, A->val32, nullptr, op, (const float*)t.x, (const float*)t.b,
+                                                  (float*)t.y, 0.f, s);
+            else launch_stream<double, double>(A, A->val, nullptr, op, (const double*)t.x, (const double*)t.b,
+                                               (double*)t.y, 0.0, s);
+        };
+        run();
+        RAMD_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; r++) run();
+        RAMD_HIP(hipEventRecord(e1, s));
+        RAMD_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        RAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        return ms;
+    };
     for (int cap : kStreamCaps) {
         if (prec == 0 && cap > 4096) continue;
         ramd_csr_s::RowBlocks rb;
@@ -741,19 +798,7 @@ void csr_autotune(ramd_csr_s* A, int prec)
             RAMD_HIP(hipMemcpyAsync(rb.d, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice, s));
         }
         A->rb[prec] = rb;
-        auto run = [&] {
-            if (prec) launch_stream<float, float>(A, A->val32, nullptr, op, (const float*)t.x, (const float*)t.b,
-                                                  (float*)t.y, 0.f, s);
-            else launch_stream<double, double>(A, A->val, nullptr, op, (const double*)t.x, (const double*)t.b,
-                                               (double*)t.y, 0.0, s);
-        };
-        run();
-        RAMD_HIP(hipEventRecord(e0, s));
-        for (int r = 0; r < 3; r++) run();
-        RAMD_HIP(hipEventRecord(e1, s));
-        RAMD_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        RAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const float ms = time_ms();
         if (ms < best_ms) {
             best_ms = ms;
             best = rb;
@@ -763,26 +808,24 @@ void csr_autotune(ramd_csr_s* A, int prec)
     for (int*& p : t.blocks)
         if (p == best.d) p = nullptr;
     A->rb[prec] = best;
-    // square operators with long rows: the sliced form against the best cap
+    // lanes straight over the CSR rows (no extra storage)
+    A->lr_on[prec] = true;
+    {
+        const float ms = time_ms();
+        A->lr_on[prec] = ms < best_ms;
+        if (ms < best_ms) best_ms = ms;
+    }
+    // square operators with long rows: the sliced form against the best so far
     if (A->n_rows == A->n_cols && (double)A->nnz >= kSlicedMinRowLen * (double)A->n_rows) {
         sliced_build(A);
         if ((double)A->sl->entries <= kSlicedMaxPad * (double)A->nnz) {
             sliced_values(A, prec);
+            const bool lr = A->lr_on[prec];
+            A->lr_on[prec] = false;
             A->sl_on[prec] = true;
-            auto run = [&] {
-                if (prec) launch_stream<float, float>(A, A->val32, nullptr, op, (const float*)t.x, (const float*)t.b,
-                                                      (float*)t.y, 0.f, s);
-                else launch_stream<double, double>(A, A->val, nullptr, op, (const double*)t.x, (const double*)t.b,
-                                                   (double*)t.y, 0.0, s);
-            };
-            run();
-            RAMD_HIP(hipEventRecord(e0, s));
-            for (int r = 0; r < 3; r++) run();
-            RAMD_HIP(hipEventRecord(e1, s));
-            RAMD_HIP(hipEventSynchronize(e1));
-            float ms = 0;
-            RAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+            const float ms = time_ms();
             A->sl_on[prec] = ms < best_ms;
+            A->lr_on[prec] = lr && !A->sl_on[prec];
         }
         if (!A->sl_on[prec]) {  // release what this precision built
             RAMD_HIP(hipStreamSynchronize(s));
@@ -870,7 +913,7 @@ void launch_spmv_dot(const ramd_csr_s* A, bool values_fp32, const double* x, dou
 {
     ramd_ctx ctx = A->ctx;
     const bool dia32 = values_fp32 && A->dia && A->dia->v32 && A->dia->v32_exact;
-    const int np = (A->dia || (A->sl && A->sl_on[0])) ? (A->n_rows + kBlockThreads - 1) / kBlockThreads : A->rb[0].n;
+    const int np = (A->dia || A->lr_on[0] || (A->sl && A->sl_on[0])) ? (A->n_rows + kBlockThreads - 1) / kBlockThreads : A->rb[0].n;
     double* part = ctx_partials(ctx, (size_t)std::max(np, 1));
     if (dia32)
         launch_stream<float, double>(A, nullptr, A->dia->v32, OP_SPMV, x, nullptr, y, 0.0, s, part);

@@ -89,14 +89,19 @@ def sliced_cases():
             ("empty_rows", K.Csr([0, 0, 2, 2, 3], [1, 3, 0], [1.5, -2.0, 4.0], 4))]
 
 
+@pytest.mark.parametrize("layout", ["sliced", "lane_rows"])
 @pytest.mark.parametrize("name,A", sliced_cases(), ids=[c[0] for c in sliced_cases()])
-def test_sliced_kernel_bit_exact(ctx, name, A):
-    """The sliced-row kernel (64 rows per slice, one lane per row, padding
-    dropped by a select) gives the oracle's bits for every operation."""
+def test_sliced_kernel_bit_exact(ctx, name, A, layout):
+    """The one-lane-per-row kernels -- over the sliced form (64 rows per
+    slice, padding dropped by a select) and straight over the CSR rows --
+    give the oracle's bits for every operation."""
     rng = np.random.default_rng(32)
     x, b = rng.standard_normal(A.n_cols), rng.standard_normal(A.n_rows)
     D = to_dev(ctx, A)
-    D.set_sliced(0, True)
+    if layout == "sliced":
+        D.set_sliced(0, True)
+    else:
+        D.set_lane_rows(0, True)
     y = ra.ParVector(ctx, A.n_rows)
     D.mult(dev(ctx, x), y)
     assert np.array_equal(y.numpy(), K.spmv(A, x))
@@ -111,6 +116,7 @@ def test_sliced_kernel_bit_exact(ctx, name, A):
         D.jacobi(xj, dev(ctx, b), ra.ParVector(ctx, A.n_rows), 2, 2.0 / 3.0)
         assert np.array_equal(xj.numpy(), K.jacobi(A, x, b, 2, 2.0 / 3.0))
     D.set_sliced(0, False)
+    D.set_lane_rows(0, False)
     y3 = ra.ParVector(ctx, A.n_rows)
     D.mult(dev(ctx, x), y3)
     assert np.array_equal(y3.numpy(), y.numpy())

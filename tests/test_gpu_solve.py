@@ -204,6 +204,49 @@ def test_vcycle_sliced_bit_exact(problem, ctx):
         for l, A in mats:
             for p in (0, 1):
                 A.set_sliced(p, bool(before[l, 0, p] == -1))
+                if before[l, 0, p] == 1:
+                    A.set_lane_rows(p, True)
+    assert np.array_equal(ml.row_caps(), before)
+
+
+def test_vcycle_lane_rows_bit_exact(problem, ctx):
+    """Every CSR level operator and interpolation switched to the
+    lane-per-row kernel over the CSR arrays: FP64 and FP32 cycles still
+    bit-identical to the oracle.  The tuned choices are restored."""
+    ml, As, Ps = problem["ml"], problem["As"], problem["Ps"]
+    ml.enable_fp32()
+    before = ml.row_caps()
+    mats = []
+    for l in range(ml.num_levels):
+        A, P = ml.level(l)
+        for m, M in ((0, A), (1, P)):
+            if M is not None and M.diagonals() == 0 and M.nnz > 0:
+                mats.append((l, m, M))
+                for p in (0, 1):
+                    M.set_lane_rows(p, True)
+    if not mats:
+        pytest.skip("no CSR operator")
+    caps = ml.row_caps()
+    assert all(np.all(caps[l, m, :] == 1) for l, m, _ in mats)
+    try:
+        H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=OMEGA)
+        n = As[0].n_rows
+        b = np.random.default_rng(25).standard_normal(n)
+        x = ra.ParVector.from_numpy(ctx, np.zeros(n))
+        ml.cycle(x, ra.ParVector.from_numpy(ctx, b), x_is_zero=True)
+        assert np.array_equal(x.numpy(), H.cycle(np.zeros(n), b))
+        b32 = b.astype(np.float32)
+        x32 = ra.ParVector(ctx, n, np.float32)
+        b32d = ra.ParVector.from_numpy(ctx, b32)
+        ml.cycle_fp32(x32, b32d)
+        assert np.array_equal(x32.numpy(), H.cycle_fp32(np.zeros(n, np.float32), b32))
+    finally:
+        for l, m, M in mats:
+            for p in (0, 1):
+                if before[l, m, p] == -1:
+                    M.set_sliced(p, True)
+                else:
+                    M.set_lane_rows(p, bool(before[l, m, p] == 1))
     assert np.array_equal(ml.row_caps(), before)
 
 
@@ -369,7 +412,7 @@ def test_row_cap_autotuner_choices(problem):
     ml.enable_fp32()
     caps = ml.row_caps()
     assert caps.shape == (ml.num_levels, 3, 2)
-    assert set(np.unique(caps)) <= {-1, 0, 2048, 4096, 8192}  # -1: sliced rows
+    assert set(np.unique(caps)) <= {-1, 0, 1, 2048, 4096, 8192}  # -1: sliced, 1: lane rows
     assert not np.any(caps[:, :, 0] == 8192)
     assert np.all(caps[-1, 1:] == 0)  # no P, R at the coarsest level
 
