@@ -238,7 +238,7 @@ bool host_diag_first(int n_rows, const int* rowptr, const int* col);
 // kDiaMinRows rows, <= kDiaMax distinct offsets, padding <= 15%, every row in
 // one common diagonal order).  Returns whether it was built.
 constexpr int kDiaMinRows = 4096;
-bool dia_build(ramd_csr_s* A, const int* rowptr, const int* col, const double* val);
+bool dia_build(ramd_csr_s* A);
 void dia_free(Dia* d);
 // Builds (once) A->trans, the CSR of A^T with rows in ascending source-row order.
 ramd_csr_s* csr_transpose(ramd_csr_s* A);

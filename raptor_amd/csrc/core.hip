@@ -1,6 +1,7 @@
 // Context, device memory, device CSR handles and the C entry points that
 // manage them.
 #include <algorithm>
+#include <climits>
 #include <cstdarg>
 #include <cstring>
 #include <new>
@@ -148,67 +149,144 @@ void dia_free(Dia* d)
     delete d;
 }
 
-bool dia_build(ramd_csr_s* A, const int* rp, const int* col, const double* val)
+__global__ void fill_int_kernel(int* __restrict__ p, int64_t n, int v)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+template <typename T>
+__global__ void fill_zero_kernel(T* __restrict__ p, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = T(0);
+}
+
+// Distinct column offsets of the rows, into an open-addressing table of
+// kDiaTable ints (INT_MIN = empty); *count counts insertions and *fail is set
+// past kDiaMax distinct offsets or a full probe sequence (the form is then
+// not built).
+constexpr int kDiaTable = 256;
+__global__ void dia_offsets_kernel(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                   int* __restrict__ keys, int* __restrict__ count, int* __restrict__ fail)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || __atomic_load_n(fail, __ATOMIC_RELAXED)) return;
+    for (int j = rp[i]; j < rp[i + 1]; j++) {
+        const int d = col[j] - i;
+        int h = (int)(((unsigned)d * 2654435761u) >> 24);
+        bool placed = false;
+        for (int probe = 0; probe < kDiaTable && !placed; probe++, h = (h + 1) & (kDiaTable - 1)) {
+            int k = __atomic_load_n(&keys[h], __ATOMIC_RELAXED);
+            if (k == INT_MIN) {
+                k = atomicCAS(&keys[h], INT_MIN, d);
+                if (k == INT_MIN) {
+                    if (atomicAdd(count, 1) >= kDiaMax) atomicOr(fail, 1);
+                    placed = true;
+                    continue;
+                }
+            }
+            placed = k == d;
+        }
+        if (!placed) {
+            atomicOr(fail, 1);
+            return;
+        }
+    }
+}
+
+// Planes (plane k of row r at k * n + r, k = rank of the offset among the
+// sorted offs) and row masks; flags: 1 a repeated column, 2 a row not in
+// ascending diagonal order, 4 a row not in diagonal-first order, 8 a value
+// not exact in FP32.
+__global__ void dia_fill_kernel(int n, int nd, int center, DiaArgs offs, const int* __restrict__ rp,
+                                const int* __restrict__ col, const double* __restrict__ val,
+                                double* __restrict__ planes, unsigned* __restrict__ mask, int* __restrict__ flags)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned m = 0;
+    int f = 0, prev_a = -2, prev_d = -2;
+    for (int j = rp[i]; j < rp[i + 1]; j++) {
+        const int d = col[j] - i;
+        int lo = 0, hi = nd - 1;  // the offset is one of offs.off[0, nd)
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (offs.off[mid] < d) lo = mid + 1;
+            else hi = mid;
+        }
+        const int k = lo;
+        if (m & (1u << k)) f |= 1;
+        m |= 1u << k;
+        const double v = val[j];
+        planes[(size_t)k * n + i] = v;
+        if ((double)(float)v != v) f |= 8;
+        const int rd = k == center ? -1 : k;
+        if (k <= prev_a) f |= 2;
+        if (rd <= prev_d) f |= 4;
+        prev_a = k;
+        prev_d = rd;
+    }
+    mask[i] = m;
+    if (f) atomicOr(flags, f);
+}
+
+// The stencil-diagonal form, built on the device from the matrix's device
+// CSR arrays (the host reads back only the distinct offsets and four flags).
+bool dia_build(ramd_csr_s* A)
 {
     const int n = A->n_rows;
     if (A->n_rows != A->n_cols || n < kDiaMinRows || A->nnz == 0) return false;
     // the kernels gather x through 32-bit byte offsets
     if ((int64_t)n * (int64_t)sizeof(double) >= ((int64_t)1 << 32)) return false;
-    // distinct offsets through a small open-addressing table
-    constexpr int T = 128;
-    long long key[T];
-    bool used[T] = {false};
-    int nd = 0;
-    std::vector<long long> offs;
-    auto find = [&](long long d) -> int {
-        unsigned h = (unsigned)((unsigned long long)(d * 0x9E3779B97F4A7C15ull) >> 57) & (T - 1);
-        while (used[h] && key[h] != d) h = (h + 1) & (T - 1);
-        return (int)h;
-    };
-    std::vector<int> slot_of(T, -1);
-    for (int i = 0; i < n; i++)
-        for (int j = rp[i]; j < rp[i + 1]; j++) {
-            const long long d = (long long)col[j] - i;
-            const int h = find(d);
-            if (!used[h]) {
-                if (nd == kDiaMax) return false;
-                used[h] = true;
-                key[h] = d;
-                offs.push_back(d);
-                nd++;
-            }
+    hipStream_t s = A->ctx->stream;
+    const dim3 rows((unsigned)((n + 255) / 256)), blk(256);
+    int* keys = dalloc<int>(kDiaTable + 3);  // table, count, fail, flags
+    struct Tmp {
+        int* keys;
+        double* planes = nullptr;
+        unsigned* mask = nullptr;
+        ~Tmp()
+        {
+            dfree(keys);
+            dfree(planes);
+            dfree(mask);
         }
-    if ((double)nd * n > 1.15 * (double)A->nnz) return false;
+    } tmp{keys};
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, keys, (int64_t)kDiaTable, INT_MIN);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, s, keys + kDiaTable, (int64_t)3, 0);
+    hipLaunchKernelGGL(dia_offsets_kernel, rows, blk, 0, s, n, A->rowptr, A->col, keys, keys + kDiaTable,
+                       keys + kDiaTable + 1);
+    RAMD_HIP(hipGetLastError());
+    std::vector<int> tab(kDiaTable + 3);
+    RAMD_HIP(hipMemcpyAsync(tab.data(), keys, sizeof(int) * tab.size(), hipMemcpyDeviceToHost, s));
+    RAMD_HIP(hipStreamSynchronize(s));
+    if (tab[kDiaTable + 1]) return false;
+    std::vector<long long> offs;
+    for (int h = 0; h < kDiaTable; h++)
+        if (tab[h] != INT_MIN) offs.push_back(tab[h]);
+    const int nd = (int)offs.size();
+    if (nd == 0 || nd > kDiaMax || (double)nd * n > 1.15 * (double)A->nnz) return false;
     std::sort(offs.begin(), offs.end());
-    for (int k = 0; k < nd; k++) slot_of[find(offs[k])] = k;
     int center = -1;
     for (int k = 0; k < nd; k++)
         if (offs[k] == 0) center = k;
-    // rank of diagonal k in the ascending and the diagonal-first orders
-    std::vector<int> rank_asc(nd), rank_dfirst(nd);
-    for (int k = 0; k < nd; k++) {
-        rank_asc[k] = k;
-        rank_dfirst[k] = (k == center) ? -1 : k;
+    DiaArgs oa{};
+    for (int k = 0; k < kDiaMax; k++) oa.off[k] = k < nd ? (int)offs[k] : 0;
+    tmp.planes = dalloc<double>((size_t)nd * n);
+    tmp.mask = dalloc<unsigned>((size_t)n);
+    {
+        const int g = (int)std::min<int64_t>(((int64_t)nd * n + 255) / 256, 8192);
+        hipLaunchKernelGGL(fill_zero_kernel<double>, dim3(g), dim3(256), 0, s, tmp.planes, (int64_t)nd * n);
     }
-    bool all_asc = true, all_dfirst = true;
-    std::vector<unsigned> mask((size_t)n, 0u);
-    std::vector<double> v((size_t)nd * n, 0.0);
-    bool exact32 = true;
-    for (int i = 0; i < n; i++) {
-        int prev_a = -2, prev_d = -2;
-        for (int j = rp[i]; j < rp[i + 1]; j++) {
-            const int k = slot_of[find((long long)col[j] - i)];
-            if (mask[i] & (1u << k)) return false;  // repeated column
-            mask[i] |= 1u << k;
-            v[(size_t)k * n + i] = val[j];
-            if ((double)(float)val[j] != val[j]) exact32 = false;
-            if (rank_asc[k] <= prev_a) all_asc = false;
-            if (rank_dfirst[k] <= prev_d) all_dfirst = false;
-            prev_a = rank_asc[k];
-            prev_d = rank_dfirst[k];
-        }
-    }
-    if (!all_asc && !all_dfirst) return false;
+    hipLaunchKernelGGL(dia_fill_kernel, rows, blk, 0, s, n, nd, center, oa, A->rowptr, A->col, A->val, tmp.planes,
+                       tmp.mask, keys + kDiaTable + 2);
+    RAMD_HIP(hipGetLastError());
+    int flags = 0;
+    RAMD_HIP(hipMemcpyAsync(&flags, keys + kDiaTable + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+    RAMD_HIP(hipStreamSynchronize(s));
+    const bool all_asc = !(flags & 2), all_dfirst = !(flags & 4), exact32 = !(flags & 8);
+    if ((flags & 1) || (!all_asc && !all_dfirst)) return false;
     Dia* d = new Dia();
     d->nd = nd;
     d->center = center;
@@ -243,23 +321,16 @@ bool dia_build(ramd_csr_s* A, const int* rp, const int* col, const double* val)
         w.win = tot;
     }
     d->v32_exact = exact32;
-    double* planes = nullptr;
     try {
-        hipStream_t s = A->ctx->stream;
-        d->mask = dalloc<unsigned>((size_t)n);
         const int ns = dia_slots<double>(nd);
         d->v64 = dalloc<double>((size_t)ns * n);
-        planes = dalloc<double>((size_t)nd * n);
-        RAMD_HIP(hipMemcpyAsync(d->mask, mask.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice, s));
-        RAMD_HIP(hipMemcpyAsync(planes, v.data(), sizeof(double) * (size_t)nd * n, hipMemcpyHostToDevice, s));
         const int grid = (int)std::min<int64_t>(((int64_t)ns * n + 255) / 256, 8192);
-        hipLaunchKernelGGL(dia_group_kernel, dim3(grid), dim3(256), 0, s, n, nd, ns, d->args, planes, d->v64);
+        hipLaunchKernelGGL(dia_group_kernel, dim3(grid), dim3(256), 0, s, n, nd, ns, d->args, tmp.planes, d->v64);
         RAMD_HIP(hipGetLastError());
         RAMD_HIP(hipStreamSynchronize(s));
-        dfree(planes);
-        planes = nullptr;
+        d->mask = tmp.mask;
+        tmp.mask = nullptr;
     } catch (...) {
-        dfree(planes);
         dia_free(d);
         throw;
     }
@@ -278,19 +349,6 @@ bool host_diag_first(int n_rows, const int* rp, const int* col)
 // ---------------------------------------------------------------------------
 // Sliced-row form
 // ---------------------------------------------------------------------------
-__global__ void fill_int_kernel(int* __restrict__ p, int64_t n, int v)
-{
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
-}
-
-template <typename T>
-__global__ void fill_zero_kernel(T* __restrict__ p, int64_t n)
-{
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = T(0);
-}
-
 // entries of row r, in stored order, to lane r % 64 of slice r / 64; the
 // padding keeps what the fill wrote
 template <typename T>
@@ -668,7 +726,7 @@ int ramd_csr_create(ramd_ctx ctx, int32_t n_rows, int32_t n_cols, int64_t nnz, c
         }
         *out = csr_wrap(ctx, n_rows, n_cols, nnz, d_rp, d_col, d_val, rowptr);
         (*out)->diag_first = host_diag_first(n_rows, rowptr, col);
-        dia_build(*out, rowptr, col, val);
+        dia_build(*out);
     });
 }
 

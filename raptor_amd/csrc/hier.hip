@@ -36,7 +36,7 @@ static ramd_csr_s* upload(ramd_ctx ctx, const Csr& M, bool dia = true)
     }
     ramd_csr_s* D = csr_wrap(ctx, M.n_rows, M.n_cols, M.nnz(), rp, c, v, M.rowptr.data());
     D->diag_first = host_diag_first(M.n_rows, M.rowptr.data(), M.col.data());
-    if (dia) dia_build(D, M.rowptr.data(), M.col.data(), M.val.data());
+    if (dia) dia_build(D);
     return D;
 }
 

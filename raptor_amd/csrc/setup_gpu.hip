@@ -277,14 +277,19 @@ void gpu_split_pmis(const ramd_csr_s* S, const double* weights, int* states)
 // ---------------------------------------------------------------------------
 // Extended interpolation, one wavefront per row.  The row's columns are
 // collected in discovery order (strong C neighbours, and the strong C
-// neighbours of strong F neighbours, first occurrence kept) in LDS; every
-// later step walks the same loops as the host in the same order with all
-// lanes in step, so each entry and the weak sum receive their additions in
-// the same sequence.  Lookups are a wavefront-wide compare over the list.
-// A row with more than kExtCols columns sets *overflow (the caller then
-// builds this level on the host).
+// neighbours of strong F neighbours, first occurrence kept) in LDS, with an
+// LDS hash table from column to position.  The loops over the row's own
+// neighbours run in order with the wavefront in step; the inner loops over a
+// neighbour's row give its entries to the lanes (one entry each), and what
+// must happen in entry order -- appending new columns, the running column
+// sum -- is done in lane order (prefix counts, a serial pass over a ballot),
+// while the per-entry additions to P's entries go to distinct entries, so
+// every entry and the weak sum receive their additions in the host's
+// sequence.  A row with more than kExtCols columns sets *overflow (the
+// caller then builds this level on the host).
 // ---------------------------------------------------------------------------
 constexpr int kExtCols = 256;
+constexpr int kExtHash = 512;  // load factor <= 1/2
 constexpr int kExtWaves = 4;
 
 // LDS written by one lane becomes visible to the wavefront's other lanes
@@ -292,6 +297,34 @@ __device__ __forceinline__ void wave_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ int ext_hash(int c) { return (int)(((unsigned)c * 2654435761u) >> 23); }
+
+// position of column c in the row's list, or -1
+__device__ __forceinline__ int ext_lookup(const int* hk, const int* hv, int c)
+{
+    for (int h = ext_hash(c);; h = (h + 1) & (kExtHash - 1)) {
+        const int k = hk[h];
+        if (k == c) return hv[h];
+        if (k < 0) return -1;
+    }
+}
+
+// c is not in the table; lanes may insert distinct columns at once
+__device__ __forceinline__ void ext_insert(int* hk, int* hv, int c, int p)
+{
+    for (int h = ext_hash(c);; h = (h + 1) & (kExtHash - 1))
+        if (atomicCAS(&hk[h], -1, c) == -1) {
+            hv[h] = p;
+            return;
+        }
+}
+
+// lanes below this one whose bit is set in m
+__device__ __forceinline__ int lanes_below(unsigned long long m)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
 template <bool FILL>
@@ -303,11 +336,15 @@ __global__ __launch_bounds__(64 * kExtWaves) void ext_interp_kernel(
 {
     __shared__ int lcol[kExtWaves][kExtCols];
     __shared__ double lval[kExtWaves][kExtCols];
+    __shared__ int hkey[kExtWaves][kExtHash];
+    __shared__ int hpos[kExtWaves][kExtHash];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * kExtWaves + wv;
     if (i >= n) return;  // the whole wavefront leaves together
     int* lc = lcol[wv];
     double* lv = lval[wv];
+    int* hk = hkey[wv];
+    int* hv = hpos[wv];
     const int st = states[i];
     if (st != kUnselectedState) {
         if (lane == 0) {
@@ -324,38 +361,47 @@ __global__ __launch_bounds__(64 * kExtWaves) void ext_interp_kernel(
         }
         return;
     }
+    for (int h = lane; h < kExtHash; h += 64) hk[h] = -1;
+    wave_sync();
     int len = 0;
     bool over = false;
-    // position of c in lc[0, len) or -1 (every lane gets the answer)
-    auto find = [&](int c) -> int {
-        for (int b0 = 0; b0 < len; b0 += 64) {
-            const int k = b0 + lane;
-            const unsigned long long m = __ballot(k < len && lc[k] == c);
-            if (m) return b0 + __ffsll((long long)m) - 1;
-        }
-        return -1;
-    };
-    auto add_col = [&](int c) {
-        if (over || find(c) >= 0) return;
-        if (len == kExtCols) {
-            over = true;
-            return;
-        }
-        if (lane == 0) lc[len] = c;
-        wave_sync();
-        len++;
-    };
     // discovery (host: the first loop over S row i)
     const int s_end = srp[i + 1];
-    for (int j = srp[i] + 1; j < s_end; j++) {
+    for (int j = srp[i] + 1; j < s_end && !over; j++) {
         const int c = scol[j];
         const int sc = states[c];
         if (sc == kSelectedState) {
-            add_col(c);
+            if (ext_lookup(hk, hv, c) < 0) {
+                if (len == kExtCols) {
+                    over = true;
+                } else {
+                    if (lane == 0) {
+                        lc[len] = c;
+                        ext_insert(hk, hv, c, len);
+                    }
+                    wave_sync();
+                    len++;
+                }
+            }
         } else if (sc == kUnselectedState) {
-            for (int k = srp[c] + 1; k < srp[c + 1]; k++) {
-                const int ck = scol[k];
-                if (states[ck] == kSelectedState) add_col(ck);
+            const int e = srp[c + 1];
+            for (int b0 = srp[c] + 1; b0 < e; b0 += 64) {
+                const int k = b0 + lane;
+                const int ck = k < e ? scol[k] : -1;
+                const bool cand = ck >= 0 && states[ck] == kSelectedState && ext_lookup(hk, hv, ck) < 0;
+                const unsigned long long m = __ballot(cand);
+                const int add = __popcll(m);
+                if (len + add > kExtCols) {
+                    over = true;
+                    break;
+                }
+                if (cand) {
+                    const int p = len + lanes_below(m);
+                    lc[p] = ck;
+                    ext_insert(hk, hv, ck, p);
+                }
+                wave_sync();
+                len += add;
             }
         }
     }
@@ -368,7 +414,7 @@ __global__ __launch_bounds__(64 * kExtWaves) void ext_interp_kernel(
     }
     for (int k = lane; k < len; k += 64) lv[k] = 0.0;
     wave_sync();
-    // accumulation (host: the loop over A row i); lane (idx % 64) owns entry idx
+    // accumulation (host: the loop over A row i)
     int start = arp[i];
     const int end = arp[i + 1];
     double weak_sum = aval[start++];
@@ -376,37 +422,41 @@ __global__ __launch_bounds__(64 * kExtWaves) void ext_interp_kernel(
     for (int j = start; j < end; j++) {
         const int c = acol[j];
         const double v = aval[j];
-        const int idx = find(c);
+        const int idx = ext_lookup(hk, hv, c);
         if (idx >= 0) {
-            if (lane == (idx & 63)) lv[idx] += v;
+            if (lane == 0) lv[idx] += v;
             if (ctr < s_end && scol[ctr] == c) ctr++;
         } else if (ctr < s_end && scol[ctr] == c) {
             ctr++;
             if (states[c] != kUnselectedState) continue;
+            const int sk = arp[c], ek = arp[c + 1];
+            const int sign = aval[sk] > 0 ? 1 : -1;
+            // entries of row c that count: opposite sign to its diagonal,
+            // column i or a column of the list; their sum in entry order
             double col_sum = 0;
-            int sk = arp[c];
-            const int ek = arp[c + 1];
-            const double diag = aval[sk++];
-            const int sign = diag > 0 ? 1 : -1;
-            for (int k = sk; k < ek; k++) {
-                const double vk = aval[k];
-                const int ck = acol[k];
-                if (vk * sign < 0 && (ck == i || find(ck) >= 0)) col_sum += vk;
+            for (int b0 = sk + 1; b0 < ek; b0 += 64) {
+                const int k = b0 + lane;
+                const bool in = k < ek;
+                const double vk = in ? aval[k] : 0.0;
+                const int ck = in ? acol[k] : -1;
+                const bool use = in && vk * sign < 0 && (ck == i || ext_lookup(hk, hv, ck) >= 0);
+                for (unsigned long long m = __ballot(use); m; m &= m - 1) col_sum += __shfl(vk, __ffsll((long long)m) - 1);
             }
             if (fabs(col_sum) < kSetupZeroTol) {
                 weak_sum += v;
             } else {
                 col_sum = v / col_sum;
-                for (int k = arp[c] + 1; k < ek; k++) {
-                    const int ck = acol[k];
-                    const double vk = aval[k];
-                    if (!(vk * sign < 0)) continue;
-                    if (ck == i) {
-                        weak_sum += (col_sum * vk);
-                    } else {
-                        const int pk = find(ck);
-                        if (pk >= 0 && lane == (pk & 63)) lv[pk] += (col_sum * vk);
-                    }
+                for (int b0 = sk + 1; b0 < ek; b0 += 64) {
+                    const int k = b0 + lane;
+                    const bool in = k < ek;
+                    const double vk = in ? aval[k] : 0.0;
+                    const int ck = in ? acol[k] : -1;
+                    const bool neg = in && vk * sign < 0;
+                    const int pk = (neg && ck != i) ? ext_lookup(hk, hv, ck) : -1;
+                    // distinct columns: each entry of the list gets at most one addition here
+                    if (pk >= 0) lv[pk] += (col_sum * vk);
+                    const unsigned long long mi = __ballot(neg && ck == i);
+                    if (mi) weak_sum += (col_sum * __shfl(vk, __ffsll((long long)mi) - 1));
                 }
             }
         } else {
