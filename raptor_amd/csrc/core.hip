@@ -450,38 +450,116 @@ void csr_free(ramd_csr_s* A)
 // Counting-sort transpose: entries of column c are emitted in ascending
 // source-row order, the order CSR_to_CSC (core/matrix.cpp:491-538) produces
 // and the order CSR_append_T (util/linalg/spmv.cpp:150-163) adds in.
+// Transpose kernels: column counts, a scatter of (row, value) to each
+// column's segment by atomic cursors, then every segment put back in row
+// order (the order of a counting sort over rows ascending, which is the
+// order CSR_to_CSC gives), one thread per column.
+__global__ void transpose_count_kernel(int64_t nnz, const int* __restrict__ col, int* __restrict__ cnt)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nnz; j += stride) atomicAdd(&cnt[col[j]], 1);
+}
+
+__global__ void transpose_scatter_kernel(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                         const double* __restrict__ val, int* __restrict__ cursor,
+                                         int* __restrict__ trow, double* __restrict__ tval)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int j = rp[i]; j < rp[i + 1]; j++) {
+        const int d = atomicAdd(&cursor[col[j]], 1);
+        trow[d] = i;
+        tval[d] = val[j];
+    }
+}
+
+__global__ void transpose_sort_kernel(int m, const int* __restrict__ tp, int* __restrict__ trow,
+                                      double* __restrict__ tval)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m) return;
+    const int s = tp[c], e = tp[c + 1];
+    for (int a = s + 1; a < e; a++) {  // rows of one column are distinct
+        const int r = trow[a];
+        const double v = tval[a];
+        int b = a - 1;
+        while (b >= s && trow[b] > r) {
+            trow[b + 1] = trow[b];
+            tval[b + 1] = tval[b];
+            b--;
+        }
+        trow[b + 1] = r;
+        tval[b + 1] = v;
+    }
+}
+
+// longest column the per-thread insertion sort takes (longer: host sort)
+constexpr int kTransposeMaxCol = 2048;
+
 ramd_csr_s* csr_transpose(ramd_csr_s* A)
 {
     if (A->trans) return A->trans;
     ramd_ctx ctx = A->ctx;
+    hipStream_t s = ctx->stream;
     const int n = A->n_rows, m = A->n_cols;
     const int64_t nnz = A->nnz;
-    std::vector<int> col((size_t)nnz);
-    std::vector<double> val((size_t)nnz);
-    if (nnz) {
-        RAMD_HIP(hipMemcpyAsync(col.data(), A->col, sizeof(int) * nnz, hipMemcpyDeviceToHost, ctx->stream));
-        RAMD_HIP(hipMemcpyAsync(val.data(), A->val, sizeof(double) * nnz, hipMemcpyDeviceToHost, ctx->stream));
-        RAMD_HIP(hipStreamSynchronize(ctx->stream));
-    }
-    std::vector<int> tp((size_t)m + 1, 0), trow((size_t)nnz);
-    std::vector<double> tval((size_t)nnz);
-    for (int64_t j = 0; j < nnz; j++) tp[col[j] + 1]++;
-    for (int c = 0; c < m; c++) tp[c + 1] += tp[c];
-    std::vector<int> pos(tp.begin(), tp.end() - 1);
-    const int* rp = A->h_rowptr.data();
-    for (int i = 0; i < n; i++)
-        for (int j = rp[i]; j < rp[i + 1]; j++) {
-            int d = pos[col[j]]++;
-            trow[d] = i;
-            tval[d] = val[j];
-        }
+    // column counts -> host exclusive sums (the handle keeps them on the host)
+    std::vector<int> tp((size_t)m + 1, 0);
     int* d_rp = dalloc<int>((size_t)m + 1);
-    int* d_col = dalloc<int>((size_t)nnz);
-    double* d_val = dalloc<double>((size_t)nnz);
-    RAMD_HIP(hipMemcpyAsync(d_rp, tp.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, ctx->stream));
-    if (nnz) {
-        RAMD_HIP(hipMemcpyAsync(d_col, trow.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, ctx->stream));
-        RAMD_HIP(hipMemcpyAsync(d_val, tval.data(), sizeof(double) * nnz, hipMemcpyHostToDevice, ctx->stream));
+    int* d_col = nullptr;
+    double* d_val = nullptr;
+    try {
+        int* cnt = d_rp + 1;
+        hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)std::min<int64_t>((m + 256) / 256, 8192)), dim3(256), 0,
+                           s, d_rp, (int64_t)m + 1, 0);
+        if (nnz)
+            hipLaunchKernelGGL(transpose_count_kernel, dim3((unsigned)std::min<int64_t>((nnz + 255) / 256, 8192)),
+                               dim3(256), 0, s, nnz, A->col, cnt);
+        RAMD_HIP(hipGetLastError());
+        RAMD_HIP(hipMemcpyAsync(tp.data(), d_rp, sizeof(int) * (m + 1), hipMemcpyDeviceToHost, s));
+        RAMD_HIP(hipStreamSynchronize(s));
+        int longest = 0;
+        for (int c = 0; c < m; c++) {
+            longest = std::max(longest, tp[c + 1]);
+            tp[c + 1] += tp[c];
+        }
+        d_col = dalloc<int>((size_t)std::max<int64_t>(nnz, 1));
+        d_val = dalloc<double>((size_t)std::max<int64_t>(nnz, 1));
+        RAMD_HIP(hipMemcpyAsync(d_rp, tp.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, s));
+        if (nnz && longest <= kTransposeMaxCol) {
+            int* cursor = dalloc<int>((size_t)std::max(m, 1));
+            RAMD_HIP(hipMemcpyAsync(cursor, d_rp, sizeof(int) * m, hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(transpose_scatter_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                               A->rowptr, A->col, A->val, cursor, d_col, d_val);
+            hipLaunchKernelGGL(transpose_sort_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m, d_rp,
+                               d_col, d_val);
+            RAMD_HIP(hipGetLastError());
+            RAMD_HIP(hipStreamSynchronize(s));
+            dfree(cursor);
+        } else if (nnz) {
+            // a long column: the counting sort on the host
+            std::vector<int> col((size_t)nnz), trow((size_t)nnz);
+            std::vector<double> val((size_t)nnz), tval((size_t)nnz);
+            RAMD_HIP(hipMemcpyAsync(col.data(), A->col, sizeof(int) * nnz, hipMemcpyDeviceToHost, s));
+            RAMD_HIP(hipMemcpyAsync(val.data(), A->val, sizeof(double) * nnz, hipMemcpyDeviceToHost, s));
+            RAMD_HIP(hipStreamSynchronize(s));
+            std::vector<int> pos(tp.begin(), tp.end() - 1);
+            const int* rp = A->h_rowptr.data();
+            for (int i = 0; i < n; i++)
+                for (int j = rp[i]; j < rp[i + 1]; j++) {
+                    const int d = pos[col[j]]++;
+                    trow[d] = i;
+                    tval[d] = val[j];
+                }
+            RAMD_HIP(hipMemcpyAsync(d_col, trow.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, s));
+            RAMD_HIP(hipMemcpyAsync(d_val, tval.data(), sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+            RAMD_HIP(hipStreamSynchronize(s));
+        }
+    } catch (...) {
+        dfree(d_rp);
+        dfree(d_col);
+        dfree(d_val);
+        throw;
     }
     A->trans = csr_wrap(ctx, m, n, nnz, d_rp, d_col, d_val, tp.data());
     return A->trans;
