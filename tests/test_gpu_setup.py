@@ -104,3 +104,35 @@ def test_hierarchy_device_setup_equals_host(ctx, name, gen):
             assert all(np.array_equal(u, v) for u, v in zip(M1.to_csr(), M2.to_csr()))
         if l < a.num_levels - 1:
             assert np.array_equal(a.states(l), b.states(l))
+
+
+@pytest.mark.parametrize("name", ["aniso", "laplacian"])
+def test_device_steps_golden(ctx, golden, name):
+    """The reference's own fixtures (tests/test_par_strength.cpp,
+    ruge_stuben/tests/test_par_interpolation.cpp): classical strength
+    (theta 0.25) and extended interpolation (filter 0) on the device."""
+    A = K.move_diag(K.fixture_csr(golden, name))
+    D = ra.ParCSRMatrix.from_csr(ctx, A.rowptr, A.col, A.val)
+    srp, scol, _ = ra.strength_gpu(D, 0.25).to_csr()
+    E = K.move_diag(K.fixture_csr(golden, name + "_S"))
+    S = K.move_diag(K.sort_rows(K.Csr(srp, scol, np.ones(scol.size))))
+    assert np.array_equal(S.rowptr, E.rowptr) and np.array_equal(S.col, E.col)
+    Sf = ra.ParCSRMatrix.from_csr(ctx, E.rowptr, E.col, E.val)
+    states = golden[name + "_split"].astype(np.int32)
+    P = ra.interp_extended_gpu(D, Sf, states, 0.0)
+    prp, pcol, pval = P.to_csr()
+    F = K.fixture_csr(golden, f"{name}_P_extend")
+    assert P.n_cols == F.n_cols
+    M = K.sort_rows(K.Csr(prp, pcol, pval, F.n_cols))
+    assert np.array_equal(M.rowptr, F.rowptr) and np.array_equal(M.col, F.col)
+    assert np.allclose(M.val, F.val, rtol=0, atol=1e-6)  # the fixture's own bound (par_compare.hpp)
+
+
+@pytest.mark.parametrize("level", [0, 1])
+def test_device_pmis_golden(ctx, golden, level):
+    """ruge_stuben/tests/test_par_splitting.cpp: PMIS on rss_S{0,1} with
+    weights.txt gives rss_cf{0,1}_pmis on the device."""
+    S = K.move_diag(K.fixture_csr(golden, f"rss_S{level}"))
+    D = ra.ParCSRMatrix.from_csr(ctx, S.rowptr, S.col, S.val)
+    w = golden["weights"][:S.n_rows]
+    assert np.array_equal(ra.split_pmis_gpu(D, w), golden[f"rss_cf{level}_pmis"])
