@@ -414,7 +414,14 @@ int ramd_host_write_petsc(const char* path, int32_t n_rows, int32_t n_cols, cons
  * (ruge_stuben/par_interpolation.cpp:175-989) -> P on the device
  * (RAMD_E_ARG when a row would need more than 256 columns). */
 int ramd_strength_gpu(ramd_csr A, double theta, ramd_csr* S);
+/* Symmetric strength (par_strength.cpp:342-534) on the device, bit-identical
+ * to ramd_host_strength(..., RAMD_SYMMETRIC, ...). */
+int ramd_symmetric_strength_gpu(ramd_csr A, double theta, ramd_csr* S);
 int ramd_split_pmis_gpu(ramd_csr S, const double* weights, int32_t* states);
+/* HMIS (ruge_stuben/par_cf_splitting.cpp:91-112): the Ruge-Stuben first
+ * pass on the host (sequential), then the PMIS loop on the device from its
+ * states; bit-identical to ramd_host_split(..., RAMD_HMIS, ...). */
+int ramd_split_hmis_gpu(ramd_csr S, const double* weights, int32_t* states);
 int ramd_interp_extended_gpu(ramd_csr A, ramd_csr S, const int32_t* states, double filter, ramd_csr* P);
 
 /* Greedy colouring of the graph of A + A^T without the diagonal (rows in

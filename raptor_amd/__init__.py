@@ -567,10 +567,12 @@ def split(s_rowptr, s_col, coarsen_type, weights=None):
     return states
 
 
-def strength_gpu(A, theta):
-    """Classical strength of a device matrix on the device (ramd_strength_gpu)."""
+def strength_gpu(A, theta, strength_type=Classical):
+    """Classical (ramd_strength_gpu) or symmetric (ramd_symmetric_strength_gpu)
+    strength of a device matrix on the device."""
     h = C.c_void_p()
-    check(lib().ramd_strength_gpu(A.h, float(theta), C.byref(h)))
+    fn = lib().ramd_symmetric_strength_gpu if strength_type == Symmetric else lib().ramd_strength_gpu
+    check(fn(A.h, float(theta), C.byref(h)))
     return ParCSRMatrix(A.ctx, h)
 
 
@@ -579,6 +581,16 @@ def split_pmis_gpu(S, weights=None):
     states = np.empty(S.n_rows, np.int32)
     w = None if weights is None else _f64(weights)
     check(lib().ramd_split_pmis_gpu(S.h, None if w is None else _ptr(w, C.c_double), _ptr(states, C.c_int32)))
+    return states
+
+
+
+def split_hmis_gpu(S, weights=None):
+    """HMIS C/F states of a device strength matrix (ramd_split_hmis_gpu): the
+    Ruge-Stuben first pass on the host, the PMIS loop on the device."""
+    states = np.empty(S.n_rows, np.int32)
+    w = None if weights is None else _f64(weights)
+    check(lib().ramd_split_hmis_gpu(S.h, None if w is None else _ptr(w, C.c_double), _ptr(states, C.c_int32)))
     return states
 
 

@@ -122,7 +122,9 @@ _SIGS = {
     "ramd_bicgstab": (P, P, P, D, I32, PD, I32, PI32, PI32),
     "ramd_host_greedy_colors": (I32, PI32, PI32, PI32, PI32),
     "ramd_strength_gpu": (P, D, C.POINTER(P)),
+    "ramd_symmetric_strength_gpu": (P, D, C.POINTER(P)),
     "ramd_split_pmis_gpu": (P, PD, PI32),
+    "ramd_split_hmis_gpu": (P, PD, PI32),
     "ramd_interp_extended_gpu": (P, P, PI32, D, C.POINTER(P)),
     "ramd_pre_bicgstab": (P, P, P, P, D, I32, PD, I32, PI32, PI32),
     "ramd_adjoint": (P, P, P, P, D, C.c_int, C.c_int, P, P, PI32, PD),

@@ -453,14 +453,17 @@ static Csr rs_level(ramd_hier_s* h, Csr& A, int level, const double* weights, st
     return P;
 }
 
-// The same level built on the device (classical strength, PMIS, extended
-// interpolation; results identical to rs_level's), when the options name
-// that combination.  A level whose rows would need more interpolation
-// columns than the device kernel holds is interpolated on the host.
+// The same level built on the device (classical or symmetric strength,
+// PMIS or HMIS, extended interpolation; results identical to rs_level's), when the
+// options name that combination.  HMIS's Ruge-Stuben first pass is
+// sequential and runs on the host between the device steps.  A level whose
+// rows would need more interpolation columns than the device kernel holds
+// is interpolated on the host.
 static bool device_rs(const ramd_amg_opts& o)
 {
-    return o.device_setup && o.solver == RAMD_SOLVER_RS && o.strength_type == RAMD_CLASSICAL &&
-           o.coarsen_type == RAMD_PMIS && o.interp_type == RAMD_EXTENDED;
+    return o.device_setup && o.solver == RAMD_SOLVER_RS &&
+           (o.strength_type == RAMD_CLASSICAL || o.strength_type == RAMD_SYMMETRIC) &&
+           (o.coarsen_type == RAMD_PMIS || o.coarsen_type == RAMD_HMIS) && o.interp_type == RAMD_EXTENDED;
 }
 
 static ramd_csr_s* rs_level_device(ramd_hier_s* h, int level, const double* weights, std::vector<int>& states)
@@ -470,13 +473,21 @@ static ramd_csr_s* rs_level_device(ramd_hier_s* h, int level, const double* weig
     Level& L = h->levels[level];
     const int n = L.n;
     double t0 = now_s();
-    ramd_csr_s* S = gpu_classical_strength(L.A, o.strong_threshold);
+    ramd_csr_s* S = o.strength_type == RAMD_SYMMETRIC ? gpu_symmetric_strength(L.A, o.strong_threshold)
+                                                      : gpu_classical_strength(L.A, o.strong_threshold);
     int* st = nullptr;
     ramd_csr_s* P = nullptr;
     try {
         double t1 = now_s();
         st = dalloc<int>((size_t)std::max(n, 1));
-        gpu_split_pmis(S, weights, st);
+        if (o.coarsen_type == RAMD_HMIS) {
+            Csr Sh = download(S);
+            std::vector<int> prior;
+            host::hmis_first_pass(Sh, prior);
+            gpu_split_pmis(S, weights, st, prior.data());
+        } else {
+            gpu_split_pmis(S, weights, st);
+        }
         states.resize((size_t)n);
         if (n) RAMD_HIP(hipMemcpyAsync(states.data(), st, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
         RAMD_HIP(hipStreamSynchronize(ctx->stream));

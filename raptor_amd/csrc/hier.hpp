@@ -70,7 +70,10 @@ __device__ __forceinline__ void coarse_solve_wave(int n, const double* __restric
 
 // Device setup steps (setup_gpu.hip), bit-identical to the host ones.
 ramd_csr_s* gpu_classical_strength(ramd_csr_s* A, double theta);
-void gpu_split_pmis(const ramd_csr_s* S, const double* weights, int* states);
+ramd_csr_s* gpu_symmetric_strength(ramd_csr_s* A, double theta);
+// PMIS on the device; prior_states (host, n; nullptr for plain PMIS) are
+// states an earlier pass decided (HMIS), applied before the first round
+void gpu_split_pmis(const ramd_csr_s* S, const double* weights, int* states, const int* prior_states = nullptr);
 ramd_csr_s* gpu_extended_interpolation(const ramd_csr_s* A, const ramd_csr_s* S, const int* states_d,
                                        const std::vector<int>& states_h, double filter);
 

@@ -684,6 +684,14 @@ void split_hmis(Csr& S, const double* weights, std::vector<int>& states)
     pmis_main_loop(S, states, weights);
 }
 
+// the Ruge-Stuben part of split_hmis (states before its PMIS loop)
+void hmis_first_pass(Csr& S, std::vector<int>& states)
+{
+    move_diag(S);
+    set_initial_states(S, states);
+    split_rs_serial(S, states, true, false);
+}
+
 // split_cljp :32-48
 void split_cljp(Csr& S, const double* weights, std::vector<int>& states)
 {

@@ -122,6 +122,119 @@ ramd_csr_s* gpu_classical_strength(ramd_csr_s* A, double theta)
 }
 
 // ---------------------------------------------------------------------------
+// symmetric strength (par_strength.cpp:342-534): each row's sign flag and
+// threshold (theta times its extreme off-diagonal of the sign opposite to
+// the diagonal), then an entry is strong if it passes the test of its row
+// or of its column's row.  The row's first stored entry is kept first as
+// the diagonal slot, whatever its column (as there).
+// ---------------------------------------------------------------------------
+__global__ void sym_scale_kernel(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                 const double* __restrict__ val, double theta, int* __restrict__ neg,
+                                 double* __restrict__ scale)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = rp[i];
+    const int e = rp[i + 1];
+    if (e == s) {
+        neg[i] = 0;
+        scale[i] = 0.0;
+        return;
+    }
+    double diag = 0.0;
+    if (col[s] == i) {
+        diag = val[s];
+        s++;
+    }
+    double row_scale;
+    if (diag < 0.0) {
+        row_scale = -(double)INT_MAX;
+        for (int j = s; j < e; j++)
+            if (val[j] > row_scale) row_scale = val[j];
+    } else {
+        row_scale = (double)INT_MAX;
+        for (int j = s; j < e; j++)
+            if (val[j] < row_scale) row_scale = val[j];
+    }
+    neg[i] = diag < 0.0 ? 1 : 0;
+    scale[i] = row_scale * theta;
+}
+
+template <bool FILL>
+__global__ void sym_strength_kernel(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                    const double* __restrict__ val, const int* __restrict__ neg,
+                                    const double* __restrict__ scale, int* __restrict__ cnt,
+                                    const int* __restrict__ srp, int* __restrict__ scol, double* __restrict__ sval)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = rp[i];
+    const int e = rp[i + 1];
+    if (e == s) {
+        if (!FILL) cnt[i] = 0;
+        return;
+    }
+    const bool nd = neg[i] != 0;
+    const double th = scale[i];
+    int o = FILL ? srp[i] : 0;
+    if (FILL) {
+        scol[o] = i;
+        sval[o] = val[s];
+    }
+    o++;
+    s++;
+    for (int j = s; j < e; j++) {
+        const double v = val[j];
+        const int c = col[j];
+        if ((nd && v > th) || (!nd && v < th) || (neg[c] && v > scale[c]) || (!neg[c] && v < scale[c])) {
+            if (FILL) {
+                scol[o] = c;
+                sval[o] = v;
+            }
+            o++;
+        }
+    }
+    if (!FILL) cnt[i] = o;
+}
+
+ramd_csr_s* gpu_symmetric_strength(ramd_csr_s* A, double theta)
+{
+    ramd_ctx ctx = A->ctx;
+    hipStream_t s = ctx->stream;
+    const int n = A->n_rows;
+    const size_t m = (size_t)std::max(n, 1);
+    struct Bufs {
+        int *cnt = nullptr, *neg = nullptr;
+        double* scale = nullptr;
+        ~Bufs()
+        {
+            dfree(cnt);
+            dfree(neg);
+            dfree(scale);
+        }
+    } b;
+    b.cnt = dalloc<int>(m);
+    b.neg = dalloc<int>(m);
+    b.scale = dalloc<double>(m);
+    hipLaunchKernelGGL(sym_scale_kernel, rows_grid(n), dim3(256), 0, s, n, A->rowptr, A->col, A->val, theta, b.neg,
+                       b.scale);
+    hipLaunchKernelGGL((sym_strength_kernel<false>), rows_grid(n), dim3(256), 0, s, n, A->rowptr, A->col, A->val,
+                       b.neg, b.scale, b.cnt, nullptr, nullptr, nullptr);
+    RAMD_HIP(hipGetLastError());
+    std::vector<int> rp = host_rowptr(ctx, b.cnt, n);
+    const int64_t nnz = rp[n];
+    int* d_rp = dalloc<int>((size_t)n + 1);
+    int* d_col = dalloc<int>((size_t)std::max<int64_t>(nnz, 1));
+    double* d_val = dalloc<double>((size_t)std::max<int64_t>(nnz, 1));
+    RAMD_HIP(hipMemcpyAsync(d_rp, rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((sym_strength_kernel<true>), rows_grid(n), dim3(256), 0, s, n, A->rowptr, A->col, A->val,
+                       b.neg, b.scale, nullptr, d_rp, d_col, d_val);
+    RAMD_HIP(hipGetLastError());
+    RAMD_HIP(hipStreamSynchronize(s));  // the scratch buffers are freed on return
+    return csr_wrap(ctx, n, A->n_cols, nnz, d_rp, d_col, d_val, rp.data());
+}
+
+// ---------------------------------------------------------------------------
 // PMIS.  Every step of a round reads only what the previous step wrote:
 // selection reads weights, marking reads the new selections, the update
 // rewrites each vertex from its own state, so one thread per vertex gives
@@ -151,6 +264,20 @@ __global__ void pmis_transpose_kernel(int n, const int* __restrict__ srp, const 
     for (int j = s; j < e; j++) {
         const int c = scol[j];
         tidx[tptr[c] + atomicAdd(&tfill[c], 1)] = i;
+    }
+}
+
+// states already decided (HMIS: the Ruge-Stuben first pass) make the rows
+// that strongly depend on a C point F (the loop before the first round of
+// the reference's PMIS)
+__global__ void pmis_prior_kernel(int n, const int* __restrict__ tptr, const int* __restrict__ tidx,
+                                  int* __restrict__ states)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n || states[u] != kSelectedState) return;
+    for (int j = tptr[u]; j < tptr[u + 1]; j++) {
+        const int r = tidx[j];
+        if (states[r] == kUnassignedState) states[r] = kUnselectedState;
     }
 }
 
@@ -223,7 +350,7 @@ __global__ void pmis_update_kernel(int n, int* __restrict__ states, double* __re
 
 // states (device, n ints, caller-owned) of PMIS on S; weights = the random
 // values (host, n)
-void gpu_split_pmis(const ramd_csr_s* S, const double* weights, int* states)
+void gpu_split_pmis(const ramd_csr_s* S, const double* weights, int* states, const int* prior_states)
 {
     ramd_ctx ctx = S->ctx;
     hipStream_t s = ctx->stream;
@@ -257,6 +384,10 @@ void gpu_split_pmis(const ramd_csr_s* S, const double* weights, int* states)
     RAMD_HIP(hipMemsetAsync(b.tcnt, 0, sizeof(int) * m, s));  // reused as fill counters
     hipLaunchKernelGGL(pmis_transpose_kernel, rows_grid(n), dim3(256), 0, s, n, S->rowptr, S->col, b.tptr, b.tcnt,
                        b.tidx);
+    if (prior_states) {
+        RAMD_HIP(hipMemcpyAsync(states, prior_states, sizeof(int) * n, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(pmis_prior_kernel, rows_grid(n), dim3(256), 0, s, n, b.tptr, b.tidx, states);
+    }
     hipLaunchKernelGGL(pmis_weights_kernel, rows_grid(n), dim3(256), 0, s, n, b.rnd, b.tptr, states, b.w);
     RAMD_HIP(hipGetLastError());
     for (int round = 0;; round++) {
@@ -567,6 +698,16 @@ int ramd_strength_gpu(ramd_csr A, double theta, ramd_csr* S)
     });
 }
 
+int ramd_symmetric_strength_gpu(ramd_csr A, double theta, ramd_csr* S)
+{
+    return guarded([&] {
+        RAMD_CHECK(A && S, RAMD_E_ARG, "ramd_symmetric_strength_gpu: NULL argument");
+        RAMD_CHECK(A->n_rows == A->n_cols, RAMD_E_ARG, "ramd_symmetric_strength_gpu: A is not square");
+        A->ctx->set_current();
+        *S = gpu_symmetric_strength(A, theta);
+    });
+}
+
 int ramd_split_pmis_gpu(ramd_csr S, const double* weights, int32_t* states)
 {
     return guarded([&] {
@@ -581,6 +722,44 @@ int ramd_split_pmis_gpu(ramd_csr S, const double* weights, int32_t* states)
         int* d = dalloc<int>((size_t)std::max(n, 1));
         try {
             gpu_split_pmis(S, w.data(), d);
+            if (n) RAMD_HIP(hipMemcpyAsync(states, d, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+            RAMD_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            dfree(d);
+            throw;
+        }
+        dfree(d);
+    });
+}
+
+int ramd_split_hmis_gpu(ramd_csr S, const double* weights, int32_t* states)
+{
+    return guarded([&] {
+        RAMD_CHECK(S && states, RAMD_E_ARG, "ramd_split_hmis_gpu: NULL argument");
+        RAMD_CHECK(S->n_rows == S->n_cols, RAMD_E_ARG, "ramd_split_hmis_gpu: S is not square");
+        ramd_ctx ctx = S->ctx;
+        ctx->set_current();
+        const int n = S->n_rows;
+        std::vector<double> w;
+        if (weights) w.assign(weights, weights + n);
+        else w = host::reference_weights(n, 0);
+        // the Ruge-Stuben first pass on the host copy of S
+        host::Csr Sh;
+        Sh.n_rows = Sh.n_cols = n;
+        Sh.rowptr = S->h_rowptr;
+        Sh.col.resize((size_t)S->nnz);
+        Sh.val.resize((size_t)S->nnz);
+        if (S->nnz) {
+            RAMD_HIP(hipMemcpyAsync(Sh.col.data(), S->col, sizeof(int) * S->nnz, hipMemcpyDeviceToHost, ctx->stream));
+            RAMD_HIP(hipMemcpyAsync(Sh.val.data(), S->val, sizeof(double) * S->nnz, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        }
+        RAMD_HIP(hipStreamSynchronize(ctx->stream));
+        std::vector<int> prior;
+        host::hmis_first_pass(Sh, prior);
+        int* d = dalloc<int>((size_t)std::max(n, 1));
+        try {
+            gpu_split_pmis(S, w.data(), d, prior.data());
             if (n) RAMD_HIP(hipMemcpyAsync(states, d, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
             RAMD_HIP(hipStreamSynchronize(ctx->stream));
         } catch (...) {

@@ -63,6 +63,10 @@ def test_strength_pmis_extended_match_host(ctx, name, gen, theta):
     st = ra.split_pmis_gpu(S, w)
     hst = ra.split(hrp, hcol, ra.PMIS, w)
     assert np.array_equal(st, hst)
+    assert np.array_equal(ra.split_hmis_gpu(S, w), ra.split(hrp, hcol, ra.HMIS, w))
+    ssrp, sscol, _ = ra.strength_gpu(D, theta, ra.Symmetric).to_csr()
+    hsrp, hscol = ra.strength(A.rowptr, A.col, A.val, ra.Symmetric, theta)
+    assert np.array_equal(ssrp, hsrp) and np.array_equal(sscol, hscol)
     assert set(np.unique(st)) <= {-2, 0, 1}
     for filt in (0.3, 0.0):
         P = ra.interp_extended_gpu(D, S, st, filt)
@@ -82,15 +86,18 @@ def test_pmis_reference_weights(ctx):
     assert np.array_equal(ra.split_pmis_gpu(S), ra.split(hrp, hcol, ra.PMIS))
 
 
+@pytest.mark.parametrize("coarsen,stype", [(ra.PMIS, ra.Classical), (ra.HMIS, ra.Classical),
+                                           (ra.PMIS, ra.Symmetric)], ids=["pmis", "hmis", "pmis_sym"])
 @pytest.mark.parametrize("name,gen", [("lap27_16", lambda: ra.gallery_laplace27(16)),
                                       ("aniso80", lambda: ra.gallery_aniso2d(80, 0.001, math.pi / 8))])
-def test_hierarchy_device_setup_equals_host(ctx, name, gen):
-    """RS(PMIS, extended) hierarchies built with the device steps and with the
-    host steps: every level's A and P, and the C/F states, identical."""
+def test_hierarchy_device_setup_equals_host(ctx, name, gen, coarsen, stype):
+    """RS(PMIS or HMIS, extended) hierarchies with classical or symmetric
+    strength built with the device steps and with the host steps: every
+    level's A and P, and the C/F states, identical."""
     rp, col, val = gen()
     mls = []
     for dev in (True, False):
-        ml = ra.ParRugeStubenSolver(ctx, 0.25, ra.PMIS, ra.Extended, ra.Classical, ra.Jacobi)
+        ml = ra.ParRugeStubenSolver(ctx, 0.25, coarsen, ra.Extended, stype, ra.Jacobi)
         ml.device_setup = dev
         ml.setup(rp, col, val)
         mls.append(ml)
@@ -113,10 +120,11 @@ def test_device_steps_golden(ctx, golden, name):
     (theta 0.25) and extended interpolation (filter 0) on the device."""
     A = K.move_diag(K.fixture_csr(golden, name))
     D = ra.ParCSRMatrix.from_csr(ctx, A.rowptr, A.col, A.val)
-    srp, scol, _ = ra.strength_gpu(D, 0.25).to_csr()
-    E = K.move_diag(K.fixture_csr(golden, name + "_S"))
-    S = K.move_diag(K.sort_rows(K.Csr(srp, scol, np.ones(scol.size))))
-    assert np.array_equal(S.rowptr, E.rowptr) and np.array_equal(S.col, E.col)
+    for stype, suffix in ((ra.Symmetric, "_SS"), (ra.Classical, "_S")):
+        srp, scol, _ = ra.strength_gpu(D, 0.25, stype).to_csr()
+        E = K.move_diag(K.fixture_csr(golden, name + suffix))
+        S = K.move_diag(K.sort_rows(K.Csr(srp, scol, np.ones(scol.size))))
+        assert np.array_equal(S.rowptr, E.rowptr) and np.array_equal(S.col, E.col)
     Sf = ra.ParCSRMatrix.from_csr(ctx, E.rowptr, E.col, E.val)
     states = golden[name + "_split"].astype(np.int32)
     P = ra.interp_extended_gpu(D, Sf, states, 0.0)
