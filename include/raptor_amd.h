@@ -423,6 +423,13 @@ int ramd_split_pmis_gpu(ramd_csr S, const double* weights, int32_t* states);
  * states; bit-identical to ramd_host_split(..., RAMD_HMIS, ...). */
 int ramd_split_hmis_gpu(ramd_csr S, const double* weights, int32_t* states);
 int ramd_interp_extended_gpu(ramd_csr A, ramd_csr S, const int32_t* states, double filter, ramd_csr* P);
+/* CLJP (ruge_stuben/par_cf_splitting.cpp:32-48, 1383-1607) and modified
+ * classical interpolation (ruge_stuben/par_interpolation.cpp:991-1451) on
+ * the device, bit-identical to ramd_host_split(..., RAMD_CLJP, ...) and
+ * ramd_host_interp(..., RAMD_MOD_CLASSICAL, ...) (RAMD_E_ARG when a row has
+ * more than 256 strong C neighbours). */
+int ramd_split_cljp_gpu(ramd_csr S, const double* weights, int32_t* states);
+int ramd_interp_mod_classical_gpu(ramd_csr A, ramd_csr S, const int32_t* states, ramd_csr* P);
 
 /* Greedy colouring of the graph of A + A^T without the diagonal (rows in
  * ascending order take the smallest colour no coloured neighbour holds),

@@ -602,6 +602,22 @@ def interp_extended_gpu(A, S, states, filter_threshold=0.3):
     return ParCSRMatrix(A.ctx, h)
 
 
+def split_cljp_gpu(S, weights=None):
+    """CLJP C/F states of a device strength matrix (ramd_split_cljp_gpu)."""
+    states = np.empty(S.n_rows, np.int32)
+    w = None if weights is None else _f64(weights)
+    check(lib().ramd_split_cljp_gpu(S.h, None if w is None else _ptr(w, C.c_double), _ptr(states, C.c_int32)))
+    return states
+
+
+def interp_mod_classical_gpu(A, S, states):
+    """Modified classical interpolation on the device (ramd_interp_mod_classical_gpu)."""
+    states = _i32(states)
+    h = C.c_void_p()
+    check(lib().ramd_interp_mod_classical_gpu(A.h, S.h, _ptr(states, C.c_int32), C.byref(h)))
+    return ParCSRMatrix(A.ctx, h)
+
+
 def split_rs_serial(s_rowptr, s_col, second_pass=True):
     """split_rs(CSRMatrix*, states, false, second_pass) of the reference."""
     s_rowptr, s_col = _i32(s_rowptr), _i32(s_col)

@@ -453,17 +453,18 @@ static Csr rs_level(ramd_hier_s* h, Csr& A, int level, const double* weights, st
     return P;
 }
 
-// The same level built on the device (classical or symmetric strength,
-// PMIS or HMIS, extended interpolation; results identical to rs_level's), when the
-// options name that combination.  HMIS's Ruge-Stuben first pass is
-// sequential and runs on the host between the device steps.  A level whose
-// rows would need more interpolation columns than the device kernel holds
-// is interpolated on the host.
+// The same level built on the device (classical or symmetric strength;
+// PMIS, HMIS, CLJP, Falgout or RS splitting; extended or modified classical
+// interpolation; results identical to rs_level's), when the options name
+// such a combination.  The sequential Ruge-Stuben passes (RS, and the first
+// passes of HMIS and Falgout) run on the host between the device steps.  A
+// level whose rows would need more interpolation columns than the device
+// kernels hold is interpolated on the host.
 static bool device_rs(const ramd_amg_opts& o)
 {
     return o.device_setup && o.solver == RAMD_SOLVER_RS &&
            (o.strength_type == RAMD_CLASSICAL || o.strength_type == RAMD_SYMMETRIC) &&
-           (o.coarsen_type == RAMD_PMIS || o.coarsen_type == RAMD_HMIS) && o.interp_type == RAMD_EXTENDED;
+           (o.interp_type == RAMD_EXTENDED || o.interp_type == RAMD_MOD_CLASSICAL);
 }
 
 static ramd_csr_s* rs_level_device(ramd_hier_s* h, int level, const double* weights, std::vector<int>& states)
@@ -480,22 +481,44 @@ static ramd_csr_s* rs_level_device(ramd_hier_s* h, int level, const double* weig
     try {
         double t1 = now_s();
         st = dalloc<int>((size_t)std::max(n, 1));
-        if (o.coarsen_type == RAMD_HMIS) {
+        std::vector<int> prior;
+        // RS: Ruge-Stuben on the first three levels, Falgout below
+        // (par_ruge_stuben_solver.hpp:76-86)
+        const int ct = (o.coarsen_type == RAMD_RS && level >= 3) ? RAMD_FALGOUT : o.coarsen_type;
+        switch (ct) {
+        case RAMD_PMIS: gpu_split_pmis(S, weights, st); break;
+        case RAMD_CLJP: gpu_split_cljp(S, weights, st); break;
+        case RAMD_HMIS: {
             Csr Sh = download(S);
-            std::vector<int> prior;
             host::hmis_first_pass(Sh, prior);
             gpu_split_pmis(S, weights, st, prior.data());
-        } else {
-            gpu_split_pmis(S, weights, st);
+            break;
+        }
+        case RAMD_FALGOUT: {
+            Csr Sh = download(S);
+            host::falgout_first_pass(Sh, prior);
+            gpu_split_cljp(S, weights, st, prior.data());
+            break;
+        }
+        case RAMD_RS: {
+            Csr Sh = download(S);
+            host::split_rs(Sh, prior);
+            if (n) RAMD_HIP(hipMemcpyAsync(st, prior.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+            break;
+        }
+        default: fail(RAMD_E_ARG, "unknown coarsen_type %d", o.coarsen_type);
         }
         states.resize((size_t)n);
         if (n) RAMD_HIP(hipMemcpyAsync(states.data(), st, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
         RAMD_HIP(hipStreamSynchronize(ctx->stream));
         double t2 = now_s();
-        P = gpu_extended_interpolation(L.A, S, st, states, o.interp_filter);
+        const bool ext = o.interp_type == RAMD_EXTENDED;
+        P = ext ? gpu_extended_interpolation(L.A, S, st, states, o.interp_filter)
+                : gpu_mod_classical_interpolation(L.A, S, st, states);
         if (!P) {
             Csr Ah = download(L.A), Sh = download(S);
-            Csr Ph = host::extended_interpolation(Ah, Sh, states, o.interp_filter);
+            Csr Ph = ext ? host::extended_interpolation(Ah, Sh, states, o.interp_filter)
+                         : host::mod_classical_interpolation(Ah, Sh, states);
             P = upload(ctx, Ph, false);
         }
         double t3 = now_s();

@@ -74,6 +74,10 @@ ramd_csr_s* gpu_symmetric_strength(ramd_csr_s* A, double theta);
 // PMIS on the device; prior_states (host, n; nullptr for plain PMIS) are
 // states an earlier pass decided (HMIS), applied before the first round
 void gpu_split_pmis(const ramd_csr_s* S, const double* weights, int* states, const int* prior_states = nullptr);
+// CLJP on the device (prior_states: Falgout's Ruge-Stuben states, or nullptr)
+void gpu_split_cljp(const ramd_csr_s* S, const double* weights, int* states, const int* prior_states = nullptr);
+ramd_csr_s* gpu_mod_classical_interpolation(const ramd_csr_s* A, const ramd_csr_s* S, const int* states_d,
+                                            const std::vector<int>& states_h);
 ramd_csr_s* gpu_extended_interpolation(const ramd_csr_s* A, const ramd_csr_s* S, const int* states_d,
                                        const std::vector<int>& states_h, double filter);
 

@@ -692,6 +692,14 @@ void hmis_first_pass(Csr& S, std::vector<int>& states)
     split_rs_serial(S, states, true, false);
 }
 
+// the Ruge-Stuben part of split_falgout (states before its CLJP loop)
+void falgout_first_pass(Csr& S, std::vector<int>& states)
+{
+    move_diag(S);
+    set_initial_states(S, states);
+    split_rs_serial(S, states, true, true);
+}
+
 // split_cljp :32-48
 void split_cljp(Csr& S, const double* weights, std::vector<int>& states)
 {

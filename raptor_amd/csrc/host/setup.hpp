@@ -61,6 +61,7 @@ void split_rs(Csr& S, std::vector<int>& states);
 // split_rs(CSRMatrix*, states, prev_states, second_pass) of cf_splitting.cpp
 void split_rs_serial(Csr& S, std::vector<int>& states, bool prev_states, bool second_pass);
 void hmis_first_pass(Csr& S, std::vector<int>& states);
+void falgout_first_pass(Csr& S, std::vector<int>& states);
 
 // Interpolation (ruge_stuben/par_interpolation.cpp).  P has one column per
 // coarse point, numbered in increasing fine index.

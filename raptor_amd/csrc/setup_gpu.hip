@@ -281,17 +281,73 @@ __global__ void pmis_prior_kernel(int n, const int* __restrict__ tptr, const int
     }
 }
 
+// CLJP weight updates after a round's selections (update_row_weights and
+// update_local_dist2_weights, ruge_stuben/par_cf_splitting.cpp:497-640).
+// Every strong edge loses its mark and lowers its target's weight by 1.0 at
+// most once (an atomic exchange on the mark decides which thread does it),
+// and a weight that stays >= 1 moves by exact integer steps, so the order of
+// the decrements changes no result: a weight that falls below 1 makes its
+// vertex F at the end of the round whatever its exact value.
+__global__ void cljp_row_weights_kernel(int n, const int* __restrict__ srp, const int* __restrict__ scol,
+                                        const int* __restrict__ states, int* __restrict__ edgemark,
+                                        double* __restrict__ w, int new_state)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n || states[c] != new_state) return;
+    int s = srp[c];
+    const int e = srp[c + 1];
+    if (s < e && scol[s] == c) s++;
+    for (int j = s; j < e; j++) {
+        const int idx = scol[j];
+        if (states[idx] == kUnassignedState && atomicExch(&edgemark[j], 0) == 1) atomicAdd(&w[idx], -1.0);
+    }
+}
+
+// row `r` of S (diagonal slot skipped) holds column c
+__device__ __forceinline__ bool s_row_has(const int* __restrict__ srp, const int* __restrict__ scol, int r, int c)
+{
+    int s = srp[r];
+    const int e = srp[r + 1];
+    if (s < e && scol[s] == r) s++;
+    for (int j = s; j < e; j++)
+        if (scol[j] == c) return true;
+    return false;
+}
+
+// for a new C point c: every row idx depending on c (not an earlier C point)
+// loses the marks of its edges to unassigned rows that also depend on c
+__global__ void cljp_dist2_kernel(int n, const int* __restrict__ srp, const int* __restrict__ scol,
+                                  const int* __restrict__ tptr, const int* __restrict__ tidx,
+                                  const int* __restrict__ states, int* __restrict__ edgemark, double* __restrict__ w,
+                                  int new_state)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n || states[c] != new_state) return;
+    for (int t = tptr[c]; t < tptr[c + 1]; t++) {
+        const int idx = tidx[t];
+        if (states[idx] == kSelectedState) continue;
+        int is = srp[idx];
+        const int ie = srp[idx + 1];
+        if (is < ie && scol[is] == idx) is++;
+        for (int k = is; k < ie; k++) {
+            const int ik = scol[k];
+            if (states[ik] == kUnassignedState && s_row_has(srp, scol, ik, c) && atomicExch(&edgemark[k], 0) == 1)
+                atomicAdd(&w[ik], -1.0);
+        }
+    }
+}
+
 // initial_weights: the random value plus 1.0 once per row that strongly
 // depends on i (added one at a time, as there); then the first pass over
 // unassigned vertices with weight < 1
 __global__ void pmis_weights_kernel(int n, const double* __restrict__ rnd, const int* __restrict__ tptr,
-                                    int* __restrict__ states, double* __restrict__ w)
+                                    int* __restrict__ states, double* __restrict__ w, int first_pass)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double v = rnd[i];
     for (int k = tptr[i]; k < tptr[i + 1]; k++) v += 1;
-    if (states[i] == kUnassignedState && v < 1) {
+    if (first_pass && states[i] == kUnassignedState && v < 1) {
         states[i] = kUnselectedState;
         v = 0.0;
     } else if (states[i] != kUnassignedState) {
@@ -350,13 +406,34 @@ __global__ void pmis_update_kernel(int n, int* __restrict__ states, double* __re
 
 // states (device, n ints, caller-owned) of PMIS on S; weights = the random
 // values (host, n)
+__global__ void ones_kernel(int* __restrict__ p, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 1;
+}
+
+// PMIS (cljp = false) or CLJP (cljp = true, cljp_main_loop :1383-1607)
+static void gpu_split_mis(const ramd_csr_s* S, const double* weights, int* states, const int* prior_states,
+                          bool cljp);
+
 void gpu_split_pmis(const ramd_csr_s* S, const double* weights, int* states, const int* prior_states)
+{
+    gpu_split_mis(S, weights, states, prior_states, false);
+}
+
+void gpu_split_cljp(const ramd_csr_s* S, const double* weights, int* states, const int* prior_states)
+{
+    gpu_split_mis(S, weights, states, prior_states, true);
+}
+
+static void gpu_split_mis(const ramd_csr_s* S, const double* weights, int* states, const int* prior_states,
+                          bool cljp)
 {
     ramd_ctx ctx = S->ctx;
     hipStream_t s = ctx->stream;
     const int n = S->n_rows;
     struct Bufs {
-        int *tcnt = nullptr, *tptr = nullptr, *tidx = nullptr, *rem = nullptr;
+        int *tcnt = nullptr, *tptr = nullptr, *tidx = nullptr, *rem = nullptr, *mark = nullptr;
         double *rnd = nullptr, *w = nullptr;
         ~Bufs()
         {
@@ -364,6 +441,7 @@ void gpu_split_pmis(const ramd_csr_s* S, const double* weights, int* states, con
             dfree(tptr);
             dfree(tidx);
             dfree(rem);
+            dfree(mark);
             dfree(rnd);
             dfree(w);
         }
@@ -384,18 +462,41 @@ void gpu_split_pmis(const ramd_csr_s* S, const double* weights, int* states, con
     RAMD_HIP(hipMemsetAsync(b.tcnt, 0, sizeof(int) * m, s));  // reused as fill counters
     hipLaunchKernelGGL(pmis_transpose_kernel, rows_grid(n), dim3(256), 0, s, n, S->rowptr, S->col, b.tptr, b.tcnt,
                        b.tidx);
+    bool prior_c = false;
     if (prior_states) {
         RAMD_HIP(hipMemcpyAsync(states, prior_states, sizeof(int) * n, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(pmis_prior_kernel, rows_grid(n), dim3(256), 0, s, n, b.tptr, b.tidx, states);
+        if (!cljp) hipLaunchKernelGGL(pmis_prior_kernel, rows_grid(n), dim3(256), 0, s, n, b.tptr, b.tidx, states);
+        for (int i = 0; i < n && !prior_c; i++) prior_c = prior_states[i] == kSelectedState;
     }
-    hipLaunchKernelGGL(pmis_weights_kernel, rows_grid(n), dim3(256), 0, s, n, b.rnd, b.tptr, states, b.w);
+    hipLaunchKernelGGL(pmis_weights_kernel, rows_grid(n), dim3(256), 0, s, n, b.rnd, b.tptr, states, b.w,
+                       cljp ? 0 : 1);
     RAMD_HIP(hipGetLastError());
+    if (cljp) {
+        const int64_t ne = std::max<int64_t>(S->nnz, 1);
+        b.mark = dalloc<int>((size_t)ne);
+        hipLaunchKernelGGL(ones_kernel, dim3((unsigned)std::min<int64_t>((ne + 255) / 256, 8192)), dim3(256), 0, s,
+                           b.mark, ne);
+        if (prior_c) {  // the C points an earlier pass chose update the weights first
+            hipLaunchKernelGGL(cljp_row_weights_kernel, rows_grid(n), dim3(256), 0, s, n, S->rowptr, S->col, states,
+                               b.mark, b.w, kSelectedState);
+            hipLaunchKernelGGL(cljp_dist2_kernel, rows_grid(n), dim3(256), 0, s, n, S->rowptr, S->col, b.tptr,
+                               b.tidx, states, b.mark, b.w, kSelectedState);
+        }
+        RAMD_HIP(hipGetLastError());
+    }
     for (int round = 0;; round++) {
-        RAMD_CHECK(round <= n + 1, RAMD_E_NUMERIC, "PMIS did not finish");
+        RAMD_CHECK(round <= n + 1, RAMD_E_NUMERIC, "C/F splitting did not finish");
         RAMD_HIP(hipMemsetAsync(b.rem, 0, sizeof(int), s));
         hipLaunchKernelGGL(pmis_select_kernel, rows_grid(n), dim3(256), 0, s, n, S->rowptr, S->col, b.tptr, b.tidx,
                            b.w, states);
-        hipLaunchKernelGGL(pmis_mark_kernel, rows_grid(n), dim3(256), 0, s, n, b.tptr, b.tidx, states);
+        if (cljp) {
+            hipLaunchKernelGGL(cljp_row_weights_kernel, rows_grid(n), dim3(256), 0, s, n, S->rowptr, S->col, states,
+                               b.mark, b.w, kNewSelectionState);
+            hipLaunchKernelGGL(cljp_dist2_kernel, rows_grid(n), dim3(256), 0, s, n, S->rowptr, S->col, b.tptr,
+                               b.tidx, states, b.mark, b.w, kNewSelectionState);
+        } else {
+            hipLaunchKernelGGL(pmis_mark_kernel, rows_grid(n), dim3(256), 0, s, n, b.tptr, b.tidx, states);
+        }
         hipLaunchKernelGGL(pmis_update_kernel, rows_grid(n), dim3(256), 0, s, n, states, b.w, b.rem);
         RAMD_HIP(hipGetLastError());
         int rem = 0;
@@ -637,9 +738,121 @@ __global__ __launch_bounds__(64 * kExtWaves) void ext_interp_kernel(
     }
 }
 
-// P on the device, or nullptr when a row needs more than kExtCols columns
-ramd_csr_s* gpu_extended_interpolation(const ramd_csr_s* A, const ramd_csr_s* S, const int* states_d,
-                                       const std::vector<int>& states_h, double filter)
+// ---------------------------------------------------------------------------
+// Modified classical interpolation (ruge_stuben/par_interpolation.cpp:
+// 991-1451), one wavefront per row.  The row's columns are its strong C
+// neighbours in S order (initial values the strong entries of A); the loop
+// over A's row runs with the wavefront in step, a strong F neighbour's row is
+// spread over the lanes, its coarse sum is formed in entry order by a serial
+// pass over a ballot, and the additions go to distinct entries -- the host's
+// sequence for every entry.  A row with more than kExtCols strong C
+// neighbours sets *overflow.
+// ---------------------------------------------------------------------------
+template <bool FILL>
+__global__ __launch_bounds__(64 * kExtWaves) void modc_interp_kernel(
+    int n, const int* __restrict__ arp, const int* __restrict__ acol, const double* __restrict__ aval,
+    const int* __restrict__ srp, const int* __restrict__ scol, const double* __restrict__ sval,
+    const int* __restrict__ states, const int* __restrict__ cmap, int* __restrict__ cnt, const int* __restrict__ prp,
+    int* __restrict__ pcol, double* __restrict__ pval, int* __restrict__ overflow)
+{
+    __shared__ int lcol[kExtWaves][kExtCols];
+    __shared__ double lval[kExtWaves][kExtCols];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * kExtWaves + wv;
+    if (i >= n) return;  // the whole wavefront leaves together
+    int* lc = lcol[wv];
+    double* lv = lval[wv];
+    if (states[i] == kSelectedState) {
+        if (lane == 0) {
+            if (FILL) {
+                pcol[prp[i]] = cmap[i];
+                pval[prp[i]] = 1;
+            } else {
+                cnt[i] = 1;
+            }
+        }
+        return;
+    }
+    // strong C neighbours in S order
+    const int s_end = srp[i + 1];
+    int len = 0;
+    for (int b0 = srp[i] + 1; b0 < s_end; b0 += 64) {
+        const int j = b0 + lane;
+        const int c = j < s_end ? scol[j] : -1;
+        const bool isc = c >= 0 && states[c] == kSelectedState;
+        const unsigned long long m = __ballot(isc);
+        if (len + __popcll(m) > kExtCols) {
+            if (lane == 0) {
+                atomicOr(overflow, 1);
+                if (!FILL) cnt[i] = 0;
+            }
+            return;
+        }
+        if (isc) {
+            const int p = len + lanes_below(m);
+            lc[p] = c;
+            lv[p] = sval[j];
+        }
+        len += __popcll(m);
+    }
+    wave_sync();
+    // position of column c in the list, or -1 (a scan: the lists are short)
+    auto pos_of = [&](int c) -> int {
+        for (int t = 0; t < len; t++)
+            if (lc[t] == c) return t;
+        return -1;
+    };
+    int start = arp[i];
+    const int end = arp[i + 1];
+    int ctr = srp[i] + 1;
+    double weak_sum = aval[start++];
+    for (int j = start; j < end; j++) {
+        const int c = acol[j];
+        const double v = aval[j];
+        if (ctr < s_end && scol[ctr] == c) {
+            ctr++;
+            if (states[c] == kSelectedState) continue;
+            const int sk = arp[c], ek = arp[c + 1];
+            const int sign = aval[sk] > 0 ? 1 : -1;
+            double coarse_sum = 0;
+            for (int b0 = sk + 1; b0 < ek; b0 += 64) {
+                const int k = b0 + lane;
+                const int ck = k < ek ? acol[k] : -1;
+                const double vk = k < ek ? aval[k] : 0.0;
+                const bool use = ck >= 0 && states[ck] == kSelectedState && vk * sign < 0 && pos_of(ck) >= 0;
+                for (unsigned long long m = __ballot(use); m; m &= m - 1)
+                    coarse_sum += __shfl(vk, __ffsll((long long)m) - 1);
+            }
+            if (fabs(coarse_sum) < kSetupZeroTol) weak_sum += v;
+            else coarse_sum = v / coarse_sum;
+            if (coarse_sum != 0.0) {
+                for (int b0 = sk + 1; b0 < ek; b0 += 64) {
+                    const int k = b0 + lane;
+                    const int ck = k < ek ? acol[k] : -1;
+                    const double vk = k < ek ? aval[k] : 0.0;
+                    const int pk = (ck >= 0 && states[ck] == kSelectedState && vk * sign < 0) ? pos_of(ck) : -1;
+                    if (pk >= 0) lv[pk] += (coarse_sum * vk);  // distinct columns: distinct entries
+                }
+            }
+        } else if (states[c] != kNoNeighborsState) {
+            weak_sum += v;
+        }
+        wave_sync();
+    }
+    for (int k = lane; k < len; k += 64) {
+        const double pv = lv[k] / -weak_sum;
+        if (FILL) {
+            pcol[prp[i] + k] = cmap[lc[k]];
+            pval[prp[i] + k] = pv;
+        }
+    }
+    if (!FILL && lane == 0) cnt[i] = len;
+}
+
+// P on the device (extended with filter, or modified classical), or nullptr
+// when a row needs more than kExtCols columns
+static ramd_csr_s* gpu_interpolation(const ramd_csr_s* A, const ramd_csr_s* S, const int* states_d,
+                                     const std::vector<int>& states_h, double filter, bool modc)
 {
     ramd_ctx ctx = A->ctx;
     hipStream_t s = ctx->stream;
@@ -663,9 +876,26 @@ ramd_csr_s* gpu_extended_interpolation(const ramd_csr_s* A, const ramd_csr_s* S,
     RAMD_HIP(hipMemcpyAsync(b.cmap, cmap.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
     RAMD_HIP(hipMemsetAsync(b.over, 0, sizeof(int), s));
     const dim3 grid((unsigned)std::max(1, (n + kExtWaves - 1) / kExtWaves)), block(64 * kExtWaves);
-    hipLaunchKernelGGL((ext_interp_kernel<false>), grid, block, 0, s, n, A->rowptr, A->col, A->val, S->rowptr,
-                       S->col, states_d, b.cmap, filter, b.cnt, nullptr, nullptr, nullptr, b.over);
-    RAMD_HIP(hipGetLastError());
+    // count pass (cnt) or fill pass (prp, pcol, pval)
+    auto launch = [&](int* cnt, const int* prp, int* pcol, double* pval) {
+        if (modc) {
+            if (cnt)
+                hipLaunchKernelGGL((modc_interp_kernel<false>), grid, block, 0, s, n, A->rowptr, A->col, A->val,
+                                   S->rowptr, S->col, S->val, states_d, b.cmap, cnt, prp, pcol, pval, b.over);
+            else
+                hipLaunchKernelGGL((modc_interp_kernel<true>), grid, block, 0, s, n, A->rowptr, A->col, A->val,
+                                   S->rowptr, S->col, S->val, states_d, b.cmap, cnt, prp, pcol, pval, b.over);
+        } else {
+            if (cnt)
+                hipLaunchKernelGGL((ext_interp_kernel<false>), grid, block, 0, s, n, A->rowptr, A->col, A->val,
+                                   S->rowptr, S->col, states_d, b.cmap, filter, cnt, prp, pcol, pval, b.over);
+            else
+                hipLaunchKernelGGL((ext_interp_kernel<true>), grid, block, 0, s, n, A->rowptr, A->col, A->val,
+                                   S->rowptr, S->col, states_d, b.cmap, filter, cnt, prp, pcol, pval, b.over);
+        }
+        RAMD_HIP(hipGetLastError());
+    };
+    launch(b.cnt, nullptr, nullptr, nullptr);
     int over = 0;
     RAMD_HIP(hipMemcpyAsync(&over, b.over, sizeof(int), hipMemcpyDeviceToHost, s));
     std::vector<int> rp = host_rowptr(ctx, b.cnt, n);  // synchronizes
@@ -675,11 +905,21 @@ ramd_csr_s* gpu_extended_interpolation(const ramd_csr_s* A, const ramd_csr_s* S,
     int* d_col = dalloc<int>((size_t)std::max<int64_t>(nnz, 1));
     double* d_val = dalloc<double>((size_t)std::max<int64_t>(nnz, 1));
     RAMD_HIP(hipMemcpyAsync(d_rp, rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL((ext_interp_kernel<true>), grid, block, 0, s, n, A->rowptr, A->col, A->val, S->rowptr, S->col,
-                       states_d, b.cmap, filter, nullptr, d_rp, d_col, d_val, b.over);
-    RAMD_HIP(hipGetLastError());
+    launch(nullptr, d_rp, d_col, d_val);
     RAMD_HIP(hipStreamSynchronize(s));
     return csr_wrap(ctx, n, nc, nnz, d_rp, d_col, d_val, rp.data());
+}
+
+ramd_csr_s* gpu_extended_interpolation(const ramd_csr_s* A, const ramd_csr_s* S, const int* states_d,
+                                       const std::vector<int>& states_h, double filter)
+{
+    return gpu_interpolation(A, S, states_d, states_h, filter, false);
+}
+
+ramd_csr_s* gpu_mod_classical_interpolation(const ramd_csr_s* A, const ramd_csr_s* S, const int* states_d,
+                                            const std::vector<int>& states_h)
+{
+    return gpu_interpolation(A, S, states_d, states_h, 0.0, true);
 }
 
 }  // namespace ramd
@@ -791,6 +1031,56 @@ int ramd_interp_extended_gpu(ramd_csr A, ramd_csr S, const int32_t* states, doub
         }
         dfree(d);
         RAMD_CHECK(out, RAMD_E_ARG, "ramd_interp_extended_gpu: a row needs more than %d columns", kExtCols);
+        *P = out;
+    });
+}
+
+int ramd_split_cljp_gpu(ramd_csr S, const double* weights, int32_t* states)
+{
+    return guarded([&] {
+        RAMD_CHECK(S && states, RAMD_E_ARG, "ramd_split_cljp_gpu: NULL argument");
+        RAMD_CHECK(S->n_rows == S->n_cols, RAMD_E_ARG, "ramd_split_cljp_gpu: S is not square");
+        ramd_ctx ctx = S->ctx;
+        ctx->set_current();
+        const int n = S->n_rows;
+        std::vector<double> w;
+        if (weights) w.assign(weights, weights + n);
+        else w = host::reference_weights(n, 0);
+        int* d = dalloc<int>((size_t)std::max(n, 1));
+        try {
+            gpu_split_cljp(S, w.data(), d);
+            if (n) RAMD_HIP(hipMemcpyAsync(states, d, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+            RAMD_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            dfree(d);
+            throw;
+        }
+        dfree(d);
+    });
+}
+
+int ramd_interp_mod_classical_gpu(ramd_csr A, ramd_csr S, const int32_t* states, ramd_csr* P)
+{
+    return guarded([&] {
+        RAMD_CHECK(A && S && states && P, RAMD_E_ARG, "ramd_interp_mod_classical_gpu: NULL argument");
+        RAMD_CHECK(A->n_rows == S->n_rows && A->n_rows == A->n_cols, RAMD_E_ARG,
+                   "ramd_interp_mod_classical_gpu: size mismatch");
+        ramd_ctx ctx = A->ctx;
+        ctx->set_current();
+        const int n = A->n_rows;
+        std::vector<int> st(states, states + n);
+        int* d = dalloc<int>((size_t)std::max(n, 1));
+        ramd_csr_s* out = nullptr;
+        try {
+            RAMD_HIP(hipMemcpyAsync(d, st.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+            out = gpu_mod_classical_interpolation(A, S, d, st);
+        } catch (...) {
+            dfree(d);
+            throw;
+        }
+        dfree(d);
+        RAMD_CHECK(out, RAMD_E_ARG, "ramd_interp_mod_classical_gpu: a row has more than %d strong C neighbours",
+                   kExtCols);
         *P = out;
     });
 }

@@ -64,6 +64,12 @@ def test_strength_pmis_extended_match_host(ctx, name, gen, theta):
     hst = ra.split(hrp, hcol, ra.PMIS, w)
     assert np.array_equal(st, hst)
     assert np.array_equal(ra.split_hmis_gpu(S, w), ra.split(hrp, hcol, ra.HMIS, w))
+    assert np.array_equal(ra.split_cljp_gpu(S, w), ra.split(hrp, hcol, ra.CLJP, w))
+    Pm = ra.interp_mod_classical_gpu(D, S, hst)
+    mrp, mcol, mval = Pm.to_csr()
+    hmrp, hmcol, hmval, hmnc = ra.interpolation(A.rowptr, A.col, A.val, hrp, hcol, hst, ra.ModClassical, 0.0)
+    assert Pm.n_cols == hmnc
+    assert np.array_equal(mrp, hmrp) and np.array_equal(mcol, hmcol) and np.array_equal(mval, hmval)
     ssrp, sscol, _ = ra.strength_gpu(D, theta, ra.Symmetric).to_csr()
     hsrp, hscol = ra.strength(A.rowptr, A.col, A.val, ra.Symmetric, theta)
     assert np.array_equal(ssrp, hsrp) and np.array_equal(sscol, hscol)
@@ -86,18 +92,24 @@ def test_pmis_reference_weights(ctx):
     assert np.array_equal(ra.split_pmis_gpu(S), ra.split(hrp, hcol, ra.PMIS))
 
 
-@pytest.mark.parametrize("coarsen,stype", [(ra.PMIS, ra.Classical), (ra.HMIS, ra.Classical),
-                                           (ra.PMIS, ra.Symmetric)], ids=["pmis", "hmis", "pmis_sym"])
+@pytest.mark.parametrize("coarsen,stype,interp", [(ra.PMIS, ra.Classical, ra.Extended),
+                                                  (ra.HMIS, ra.Classical, ra.Extended),
+                                                  (ra.PMIS, ra.Symmetric, ra.Extended),
+                                                  (ra.CLJP, ra.Classical, ra.ModClassical),
+                                                  (ra.Falgout, ra.Classical, ra.ModClassical),
+                                                  (ra.RS, ra.Classical, ra.Extended)],
+                         ids=["pmis", "hmis", "pmis_sym", "cljp_modc", "falgout_modc", "rs_ext"])
 @pytest.mark.parametrize("name,gen", [("lap27_16", lambda: ra.gallery_laplace27(16)),
                                       ("aniso80", lambda: ra.gallery_aniso2d(80, 0.001, math.pi / 8))])
-def test_hierarchy_device_setup_equals_host(ctx, name, gen, coarsen, stype):
-    """RS(PMIS or HMIS, extended) hierarchies with classical or symmetric
-    strength built with the device steps and with the host steps: every
-    level's A and P, and the C/F states, identical."""
+def test_hierarchy_device_setup_equals_host(ctx, name, gen, coarsen, stype, interp):
+    """Ruge-Stuben hierarchies (every splitting, extended or modified
+    classical interpolation, classical or symmetric strength) built with the
+    device steps and with the host steps: every level's A and P, and the C/F
+    states, identical."""
     rp, col, val = gen()
     mls = []
     for dev in (True, False):
-        ml = ra.ParRugeStubenSolver(ctx, 0.25, coarsen, ra.Extended, stype, ra.Jacobi)
+        ml = ra.ParRugeStubenSolver(ctx, 0.25, coarsen, interp, stype, ra.Jacobi)
         ml.device_setup = dev
         ml.setup(rp, col, val)
         mls.append(ml)
@@ -127,20 +139,22 @@ def test_device_steps_golden(ctx, golden, name):
         assert np.array_equal(S.rowptr, E.rowptr) and np.array_equal(S.col, E.col)
     Sf = ra.ParCSRMatrix.from_csr(ctx, E.rowptr, E.col, E.val)
     states = golden[name + "_split"].astype(np.int32)
-    P = ra.interp_extended_gpu(D, Sf, states, 0.0)
-    prp, pcol, pval = P.to_csr()
-    F = K.fixture_csr(golden, f"{name}_P_extend")
-    assert P.n_cols == F.n_cols
-    M = K.sort_rows(K.Csr(prp, pcol, pval, F.n_cols))
-    assert np.array_equal(M.rowptr, F.rowptr) and np.array_equal(M.col, F.col)
-    assert np.allclose(M.val, F.val, rtol=0, atol=1e-6)  # the fixture's own bound (par_compare.hpp)
+    for P, suffix in ((ra.interp_extended_gpu(D, Sf, states, 0.0), "extend"),
+                      (ra.interp_mod_classical_gpu(D, Sf, states), "mod_class")):
+        prp, pcol, pval = P.to_csr()
+        F = K.fixture_csr(golden, f"{name}_P_{suffix}")
+        assert P.n_cols == F.n_cols
+        M = K.sort_rows(K.Csr(prp, pcol, pval, F.n_cols))
+        assert np.array_equal(M.rowptr, F.rowptr) and np.array_equal(M.col, F.col)
+        assert np.allclose(M.val, F.val, rtol=0, atol=1e-6)  # the fixture's own bound (par_compare.hpp)
 
 
 @pytest.mark.parametrize("level", [0, 1])
-def test_device_pmis_golden(ctx, golden, level):
-    """ruge_stuben/tests/test_par_splitting.cpp: PMIS on rss_S{0,1} with
-    weights.txt gives rss_cf{0,1}_pmis on the device."""
+def test_device_pmis_cljp_golden(ctx, golden, level):
+    """ruge_stuben/tests/test_par_splitting.cpp: PMIS and CLJP on rss_S{0,1}
+    with weights.txt give rss_cf{0,1}_pmis and rss_cf{0,1} on the device."""
     S = K.move_diag(K.fixture_csr(golden, f"rss_S{level}"))
     D = ra.ParCSRMatrix.from_csr(ctx, S.rowptr, S.col, S.val)
     w = golden["weights"][:S.n_rows]
     assert np.array_equal(ra.split_pmis_gpu(D, w), golden[f"rss_cf{level}_pmis"])
+    assert np.array_equal(ra.split_cljp_gpu(D, w), golden[f"rss_cf{level}"])
