@@ -430,6 +430,9 @@ int ramd_interp_extended_gpu(ramd_csr A, ramd_csr S, const int32_t* states, doub
  * more than 256 strong C neighbours). */
 int ramd_split_cljp_gpu(ramd_csr S, const double* weights, int32_t* states);
 int ramd_interp_mod_classical_gpu(ramd_csr A, ramd_csr S, const int32_t* states, ramd_csr* P);
+/* Direct interpolation (ruge_stuben/par_interpolation.cpp:1453-1754) on the
+ * device, bit-identical to ramd_host_interp(..., RAMD_DIRECT, ...). */
+int ramd_interp_direct_gpu(ramd_csr A, ramd_csr S, const int32_t* states, ramd_csr* P);
 
 /* Greedy colouring of the graph of A + A^T without the diagonal (rows in
  * ascending order take the smallest colour no coloured neighbour holds),

@@ -618,6 +618,14 @@ def interp_mod_classical_gpu(A, S, states):
     return ParCSRMatrix(A.ctx, h)
 
 
+def interp_direct_gpu(A, S, states):
+    """Direct interpolation on the device (ramd_interp_direct_gpu)."""
+    states = _i32(states)
+    h = C.c_void_p()
+    check(lib().ramd_interp_direct_gpu(A.h, S.h, _ptr(states, C.c_int32), C.byref(h)))
+    return ParCSRMatrix(A.ctx, h)
+
+
 def split_rs_serial(s_rowptr, s_col, second_pass=True):
     """split_rs(CSRMatrix*, states, false, second_pass) of the reference."""
     s_rowptr, s_col = _i32(s_rowptr), _i32(s_col)

@@ -128,6 +128,7 @@ _SIGS = {
     "ramd_split_hmis_gpu": (P, PD, PI32),
     "ramd_interp_extended_gpu": (P, P, PI32, D, C.POINTER(P)),
     "ramd_interp_mod_classical_gpu": (P, P, PI32, C.POINTER(P)),
+    "ramd_interp_direct_gpu": (P, P, PI32, C.POINTER(P)),
     "ramd_pre_bicgstab": (P, P, P, P, D, I32, PD, I32, PI32, PI32),
     "ramd_adjoint": (P, P, P, P, D, C.c_int, C.c_int, P, P, PI32, PD),
     "ramd_fpcg_block": (P, P, I32, P, P, D, C.c_int, C.c_int, C.c_int, PI32, PD, PI32),

@@ -454,8 +454,8 @@ static Csr rs_level(ramd_hier_s* h, Csr& A, int level, const double* weights, st
 }
 
 // The same level built on the device (classical or symmetric strength;
-// PMIS, HMIS, CLJP, Falgout or RS splitting; extended or modified classical
-// interpolation; results identical to rs_level's), when the options name
+// PMIS, HMIS, CLJP, Falgout or RS splitting; direct, modified classical or
+// extended interpolation; results identical to rs_level's), when the options name
 // such a combination.  The sequential Ruge-Stuben passes (RS, and the first
 // passes of HMIS and Falgout) run on the host between the device steps.  A
 // level whose rows would need more interpolation columns than the device
@@ -464,7 +464,7 @@ static bool device_rs(const ramd_amg_opts& o)
 {
     return o.device_setup && o.solver == RAMD_SOLVER_RS &&
            (o.strength_type == RAMD_CLASSICAL || o.strength_type == RAMD_SYMMETRIC) &&
-           (o.interp_type == RAMD_EXTENDED || o.interp_type == RAMD_MOD_CLASSICAL);
+           (o.interp_type == RAMD_EXTENDED || o.interp_type == RAMD_MOD_CLASSICAL || o.interp_type == RAMD_DIRECT);
 }
 
 static ramd_csr_s* rs_level_device(ramd_hier_s* h, int level, const double* weights, std::vector<int>& states)
@@ -513,8 +513,9 @@ static ramd_csr_s* rs_level_device(ramd_hier_s* h, int level, const double* weig
         RAMD_HIP(hipStreamSynchronize(ctx->stream));
         double t2 = now_s();
         const bool ext = o.interp_type == RAMD_EXTENDED;
-        P = ext ? gpu_extended_interpolation(L.A, S, st, states, o.interp_filter)
-                : gpu_mod_classical_interpolation(L.A, S, st, states);
+        if (o.interp_type == RAMD_DIRECT) P = gpu_direct_interpolation(L.A, S, st, states);
+        else if (ext) P = gpu_extended_interpolation(L.A, S, st, states, o.interp_filter);
+        else P = gpu_mod_classical_interpolation(L.A, S, st, states);
         if (!P) {
             Csr Ah = download(L.A), Sh = download(S);
             Csr Ph = ext ? host::extended_interpolation(Ah, Sh, states, o.interp_filter)

@@ -78,6 +78,8 @@ void gpu_split_pmis(const ramd_csr_s* S, const double* weights, int* states, con
 void gpu_split_cljp(const ramd_csr_s* S, const double* weights, int* states, const int* prior_states = nullptr);
 ramd_csr_s* gpu_mod_classical_interpolation(const ramd_csr_s* A, const ramd_csr_s* S, const int* states_d,
                                             const std::vector<int>& states_h);
+ramd_csr_s* gpu_direct_interpolation(const ramd_csr_s* A, const ramd_csr_s* S, const int* states_d,
+                                     const std::vector<int>& states_h);
 ramd_csr_s* gpu_extended_interpolation(const ramd_csr_s* A, const ramd_csr_s* S, const int* states_d,
                                        const std::vector<int>& states_h, double filter);
 

@@ -849,6 +849,122 @@ __global__ __launch_bounds__(64 * kExtWaves) void modc_interp_kernel(
     if (!FILL && lane == 0) cnt[i] = len;
 }
 
+// ---------------------------------------------------------------------------
+// Direct interpolation (ruge_stuben/par_interpolation.cpp:1453-1754), one
+// thread per row: the strong C neighbours' entries of A (found by walking A's
+// row alongside S's), scaled by the ratios of all to strong negative and
+// positive sums, in S order.  Every row with a non-zero state (C points and
+// rows without neighbours) owns a coarse column, as there.
+// ---------------------------------------------------------------------------
+template <bool FILL>
+__global__ void direct_interp_kernel(int n, const int* __restrict__ arp, const int* __restrict__ acol,
+                                     const double* __restrict__ aval, const int* __restrict__ srp,
+                                     const int* __restrict__ scol, const int* __restrict__ states,
+                                     const int* __restrict__ cmap, int* __restrict__ cnt, const int* __restrict__ prp,
+                                     int* __restrict__ pcol, double* __restrict__ pval)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (states[i] == kSelectedState) {
+        if (FILL) {
+            pcol[prp[i]] = cmap[i];
+            pval[prp[i]] = 1;
+        } else {
+            cnt[i] = 1;
+        }
+        return;
+    }
+    int s = srp[i];
+    const int e = srp[i + 1];
+    if (s < e && scol[s] == i) s++;
+    const int as0 = arp[i], ae = arp[i + 1];
+    // A's value at column c of row i (S's columns appear in A's row in order)
+    int actr = as0;
+    auto a_at = [&](int c) {
+        while (acol[actr] != c) actr++;
+        return aval[actr];
+    };
+    double sum_strong_pos = 0, sum_strong_neg = 0, sum_all_pos = 0, sum_all_neg = 0;
+    int count = 0;
+    for (int j = s; j < e; j++) {
+        const int c = scol[j];
+        const double v = a_at(c);
+        if (states[c] == kSelectedState) {
+            if (v < 0) sum_strong_neg += v;
+            else sum_strong_pos += v;
+            count++;
+        }
+    }
+    if (!FILL) {
+        cnt[i] = count;
+        return;
+    }
+    double diag = aval[as0];
+    for (int j = as0 + 1; j < ae; j++) {
+        const double v = aval[j];
+        if (v < 0) sum_all_neg += v;
+        else sum_all_pos += v;
+    }
+    const double alpha = sum_all_neg / sum_strong_neg;
+    double beta;
+    if (sum_strong_pos == 0) {
+        diag += sum_all_pos;
+        beta = 0;
+    } else {
+        beta = sum_all_pos / sum_strong_pos;
+    }
+    const double neg_coeff = -alpha / diag;
+    const double pos_coeff = -beta / diag;
+    int o = prp[i];
+    actr = as0;
+    for (int j = s; j < e; j++) {
+        const int c = scol[j];
+        const double v = a_at(c);
+        if (states[c] == kSelectedState) {
+            pcol[o] = cmap[c];
+            pval[o] = v < 0 ? neg_coeff * v : pos_coeff * v;
+            o++;
+        }
+    }
+}
+
+ramd_csr_s* gpu_direct_interpolation(const ramd_csr_s* A, const ramd_csr_s* S, const int* states_d,
+                                     const std::vector<int>& states_h)
+{
+    ramd_ctx ctx = A->ctx;
+    hipStream_t s = ctx->stream;
+    const int n = A->n_rows;
+    std::vector<int> cmap((size_t)n, -1);
+    int nc = 0;
+    for (int i = 0; i < n; i++)
+        if (states_h[i] != 0) cmap[i] = nc++;
+    struct Bufs {
+        int *cmap = nullptr, *cnt = nullptr;
+        ~Bufs()
+        {
+            dfree(cmap);
+            dfree(cnt);
+        }
+    } b;
+    b.cmap = dalloc<int>((size_t)std::max(n, 1));
+    b.cnt = dalloc<int>((size_t)std::max(n, 1));
+    RAMD_HIP(hipMemcpyAsync(b.cmap, cmap.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((direct_interp_kernel<false>), rows_grid(n), dim3(256), 0, s, n, A->rowptr, A->col, A->val,
+                       S->rowptr, S->col, states_d, b.cmap, b.cnt, nullptr, nullptr, nullptr);
+    RAMD_HIP(hipGetLastError());
+    std::vector<int> rp = host_rowptr(ctx, b.cnt, n);
+    const int64_t nnz = rp[n];
+    int* d_rp = dalloc<int>((size_t)n + 1);
+    int* d_col = dalloc<int>((size_t)std::max<int64_t>(nnz, 1));
+    double* d_val = dalloc<double>((size_t)std::max<int64_t>(nnz, 1));
+    RAMD_HIP(hipMemcpyAsync(d_rp, rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((direct_interp_kernel<true>), rows_grid(n), dim3(256), 0, s, n, A->rowptr, A->col, A->val,
+                       S->rowptr, S->col, states_d, b.cmap, nullptr, d_rp, d_col, d_val);
+    RAMD_HIP(hipGetLastError());
+    RAMD_HIP(hipStreamSynchronize(s));
+    return csr_wrap(ctx, n, nc, nnz, d_rp, d_col, d_val, rp.data());
+}
+
 // P on the device (extended with filter, or modified classical), or nullptr
 // when a row needs more than kExtCols columns
 static ramd_csr_s* gpu_interpolation(const ramd_csr_s* A, const ramd_csr_s* S, const int* states_d,
@@ -1032,6 +1148,28 @@ int ramd_interp_extended_gpu(ramd_csr A, ramd_csr S, const int32_t* states, doub
         dfree(d);
         RAMD_CHECK(out, RAMD_E_ARG, "ramd_interp_extended_gpu: a row needs more than %d columns", kExtCols);
         *P = out;
+    });
+}
+
+int ramd_interp_direct_gpu(ramd_csr A, ramd_csr S, const int32_t* states, ramd_csr* P)
+{
+    return guarded([&] {
+        RAMD_CHECK(A && S && states && P, RAMD_E_ARG, "ramd_interp_direct_gpu: NULL argument");
+        RAMD_CHECK(A->n_rows == S->n_rows && A->n_rows == A->n_cols, RAMD_E_ARG,
+                   "ramd_interp_direct_gpu: size mismatch");
+        ramd_ctx ctx = A->ctx;
+        ctx->set_current();
+        const int n = A->n_rows;
+        std::vector<int> st(states, states + n);
+        int* d = dalloc<int>((size_t)std::max(n, 1));
+        try {
+            RAMD_HIP(hipMemcpyAsync(d, st.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+            *P = gpu_direct_interpolation(A, S, d, st);
+        } catch (...) {
+            dfree(d);
+            throw;
+        }
+        dfree(d);
     });
 }
 
