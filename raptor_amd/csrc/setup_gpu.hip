@@ -646,55 +646,103 @@ __global__ __launch_bounds__(64 * kExtWaves) void ext_interp_kernel(
     }
     for (int k = lane; k < len; k += 64) lv[k] = 0.0;
     wave_sync();
-    // accumulation (host: the loop over A row i)
+    // accumulation (host: the loop over A row i).  Row i's entries are
+    // taken 64 at a time, one per lane, with everything they need that does
+    // not change in the loop (list position, strength -- the column among
+    // S row i's -- state, the neighbour's row bounds and diagonal) loaded at
+    // once; the entries are then visited in order.
     int start = arp[i];
     const int end = arp[i + 1];
     double weak_sum = aval[start++];
-    int ctr = srp[i] + 1;
-    for (int j = start; j < end; j++) {
-        const int c = acol[j];
-        const double v = aval[j];
-        const int idx = ext_lookup(hk, hv, c);
-        if (idx >= 0) {
-            if (lane == 0) lv[idx] += v;
-            if (ctr < s_end && scol[ctr] == c) ctr++;
-        } else if (ctr < s_end && scol[ctr] == c) {
-            ctr++;
-            if (states[c] != kUnselectedState) continue;
-            const int sk = arp[c], ek = arp[c + 1];
-            const int sign = aval[sk] > 0 ? 1 : -1;
-            // entries of row c that count: opposite sign to its diagonal,
-            // column i or a column of the list; their sum in entry order
-            double col_sum = 0;
-            for (int b0 = sk + 1; b0 < ek; b0 += 64) {
-                const int k = b0 + lane;
-                const bool in = k < ek;
-                const double vk = in ? aval[k] : 0.0;
-                const int ck = in ? acol[k] : -1;
-                const bool use = in && vk * sign < 0 && (ck == i || ext_lookup(hk, hv, ck) >= 0);
-                for (unsigned long long m = __ballot(use); m; m &= m - 1) col_sum += __shfl(vk, __ffsll((long long)m) - 1);
+    for (int c0 = start; c0 < end; c0 += 64) {
+        const int jl = c0 + lane;
+        const bool inj = jl < end;
+        const int cj = inj ? acol[jl] : -1;
+        const double vj = inj ? aval[jl] : 0.0;
+        const int idxj = inj ? ext_lookup(hk, hv, cj) : -1;
+        bool strong = false;
+        for (int t0 = srp[i] + 1; t0 < s_end; t0 += 64) {
+            const int sc = t0 + lane < s_end ? scol[t0 + lane] : -2;
+            const int tn = min(64, s_end - t0);
+            for (int t = 0; t < tn; t++) {
+                const int st_t = __shfl(sc, t);  // every lane takes part in the shuffle
+                strong = strong || st_t == cj;
             }
-            if (fabs(col_sum) < kSetupZeroTol) {
-                weak_sum += v;
-            } else {
-                col_sum = v / col_sum;
-                for (int b0 = sk + 1; b0 < ek; b0 += 64) {
-                    const int k = b0 + lane;
+        }
+        const int stj = inj ? states[cj] : kNoNeighborsState;
+        const bool fnb = inj && idxj < 0 && strong && stj == kUnselectedState;
+        int skj = 0, ekj = 0;
+        double dgj = 0.0;
+        if (fnb) {
+            skj = arp[cj];
+            ekj = arp[cj + 1];
+            dgj = aval[skj];
+        }
+        const int nj = min(64, end - c0);
+        for (int t = 0; t < nj; t++) {
+            const double v = __shfl(vj, t);
+            const int idx = __shfl(idxj, t);
+            if (idx >= 0) {
+                if (lane == 0) lv[idx] += v;
+            } else if (__shfl((int)strong, t)) {
+                if (!__shfl((int)fnb, t)) continue;  // a strong neighbour that is not F
+                const int sk = __shfl(skj, t), ek = __shfl(ekj, t);
+                const int sign = __shfl(dgj, t) > 0 ? 1 : -1;
+                // entries of the neighbour's row that count: opposite sign to
+                // its diagonal, column i or a column of the list; their sum in
+                // entry order
+                if (ek - sk - 1 <= 64) {
+                    const int k = sk + 1 + lane;
                     const bool in = k < ek;
                     const double vk = in ? aval[k] : 0.0;
                     const int ck = in ? acol[k] : -1;
                     const bool neg = in && vk * sign < 0;
                     const int pk = (neg && ck != i) ? ext_lookup(hk, hv, ck) : -1;
-                    // distinct columns: each entry of the list gets at most one addition here
-                    if (pk >= 0) lv[pk] += (col_sum * vk);
-                    const unsigned long long mi = __ballot(neg && ck == i);
-                    if (mi) weak_sum += (col_sum * __shfl(vk, __ffsll((long long)mi) - 1));
+                    double col_sum = 0;
+                    for (unsigned long long m = __ballot(neg && (ck == i || pk >= 0)); m; m &= m - 1)
+                        col_sum += __shfl(vk, __ffsll((long long)m) - 1);
+                    if (fabs(col_sum) < kSetupZeroTol) {
+                        weak_sum += v;
+                    } else {
+                        col_sum = v / col_sum;
+                        // distinct columns: each entry of the list gets at most one addition here
+                        if (pk >= 0) lv[pk] += (col_sum * vk);
+                        const unsigned long long mi = __ballot(neg && ck == i);
+                        if (mi) weak_sum += (col_sum * __shfl(vk, __ffsll((long long)mi) - 1));
+                    }
+                } else {
+                    double col_sum = 0;
+                    for (int b0 = sk + 1; b0 < ek; b0 += 64) {
+                        const int k = b0 + lane;
+                        const bool in = k < ek;
+                        const double vk = in ? aval[k] : 0.0;
+                        const int ck = in ? acol[k] : -1;
+                        const bool use = in && vk * sign < 0 && (ck == i || ext_lookup(hk, hv, ck) >= 0);
+                        for (unsigned long long m = __ballot(use); m; m &= m - 1)
+                            col_sum += __shfl(vk, __ffsll((long long)m) - 1);
+                    }
+                    if (fabs(col_sum) < kSetupZeroTol) {
+                        weak_sum += v;
+                    } else {
+                        col_sum = v / col_sum;
+                        for (int b0 = sk + 1; b0 < ek; b0 += 64) {
+                            const int k = b0 + lane;
+                            const bool in = k < ek;
+                            const double vk = in ? aval[k] : 0.0;
+                            const int ck = in ? acol[k] : -1;
+                            const bool neg = in && vk * sign < 0;
+                            const int pk = (neg && ck != i) ? ext_lookup(hk, hv, ck) : -1;
+                            if (pk >= 0) lv[pk] += (col_sum * vk);
+                            const unsigned long long mi = __ballot(neg && ck == i);
+                            if (mi) weak_sum += (col_sum * __shfl(vk, __ffsll((long long)mi) - 1));
+                        }
+                    }
                 }
+            } else if (__shfl(stj, t) != kNoNeighborsState) {
+                weak_sum += v;
             }
-        } else {
-            if (states[c] != kNoNeighborsState) weak_sum += v;
+            wave_sync();
         }
-        wave_sync();
     }
     if (fabs(weak_sum) > kSetupZeroTol)
         for (int k = lane; k < len; k += 64) lv[k] /= -weak_sum;
