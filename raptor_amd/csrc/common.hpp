@@ -95,9 +95,8 @@ struct ramd_ctx_s {
     ProfSlot prof[10];
     std::vector<ProfPending> prof_pending;
     std::vector<hipEvent_t> prof_pool;
-    // scratch for reductions: partial sums and a completion counter
+    // scratch for reductions: partial sums
     double* red_partials = nullptr;
-    unsigned* red_counter = nullptr;
     double* red_result = nullptr;  // device scalars (16 slots)
     double* pinned = nullptr;      // pinned host staging for scalars (16 slots)
     // per-workgroup partial sums of fused kernels (grown on demand)

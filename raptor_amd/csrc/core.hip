@@ -687,9 +687,7 @@ int ramd_init(int device, ramd_ctx* out)
         RAMD_HIP(hipSetDevice(device));
         RAMD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->red_partials = dalloc<double>(4096);
-        c->red_counter = dalloc<unsigned>(16);
         c->red_result = dalloc<double>(16);
-        RAMD_HIP(hipMemsetAsync(c->red_counter, 0, 16 * sizeof(unsigned), c->stream));
         RAMD_HIP(hipHostMalloc((void**)&c->pinned, 16 * sizeof(double), hipHostMallocDefault));
         RAMD_HIP(hipStreamSynchronize(c->stream));
         *out = c;
@@ -708,7 +706,6 @@ int ramd_finalize(ramd_ctx ctx)
         }
         for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
         dfree(ctx->red_partials);
-        dfree(ctx->red_counter);
         dfree(ctx->red_result);
         dfree(ctx->big_partials);
         if (ctx->pinned) (void)hipHostFree(ctx->pinned);
