@@ -18,6 +18,24 @@
 
 namespace ramd {
 
+// acc (+ or -) prod[k0] (+ or -) ... prod[k1 - 1], left to right; the LDS
+// reads are issued eight at a time so their latency is paid once per eight
+// entries, the additions keep their order
+template <bool SUB, typename X>
+__device__ __forceinline__ X lds_row_sum(const X* prod, int k0, int k1, X acc)
+{
+    int k = k0;
+    for (; k + 8 <= k1; k += 8) {
+        X t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t[u] = prod[k + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc = SUB ? acc - t[u] : acc + t[u];
+    }
+    for (; k < k1; k++) acc = SUB ? acc - prod[k] : acc + prod[k];
+    return acc;
+}
+
 template <int OP, typename V, typename X, int CAP>
 __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
     const int* __restrict__ rowptr, const int* __restrict__ col, const V* __restrict__ val,
@@ -101,7 +119,7 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
         X acc = 0;
         if (row < r1) {
             const int k0 = rowptr[row] - s, k1 = rowptr[row + 1] - s;
-            for (int k = k0; k < k1; k++) acc += prod[k];
+            acc = lds_row_sum<false>(prod, k0, k1, acc);
             y[row] = acc;
         }
         if (dotp) {
@@ -114,13 +132,10 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
     const int k0 = rowptr[row] - s, k1 = rowptr[row + 1] - s;
     if (OP == OP_APPEND || OP == OP_ADD_INTO) {
         // CSR_append: the row sum is formed first, then added (spmv.cpp:96-101)
-        X acc = 0;
-        for (int k = k0; k < k1; k++) acc += prod[k];
+        const X acc = lds_row_sum<false>(prod, k0, k1, X(0));
         y[row] = (OP == OP_APPEND ? y[row] : b[row]) + acc;
     } else if (OP == OP_RESIDUAL) {
-        X acc = b[row];
-        for (int k = k0; k < k1; k++) acc -= prod[k];
-        y[row] = acc;
+        y[row] = lds_row_sum<true>(prod, k0, k1, b[row]);
     } else {
         // relax.cpp:23-41: x holds the old iterate (tmp), y receives the new
         const X xo = x[row];
@@ -136,13 +151,13 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
             const X dg = dvec[row];
             if (fabs(dg) > X(1e-16)) {
                 diag = dg;
-                for (int k = k0 + 1; k < k1; k++) acc += prod[k];
+                acc = lds_row_sum<false>(prod, k0 + 1, k1, acc);
             }
         } else if (diag_first) {
             // diagonal stored first, or absent (then the row keeps its value)
             if (col[s + k0] == row) {
                 diag = X(val[s + k0]);
-                for (int k = k0 + 1; k < k1; k++) acc += prod[k];
+                acc = lds_row_sum<false>(prod, k0 + 1, k1, acc);
             }
         } else {
             for (int k = k0; k < k1; k++) {
