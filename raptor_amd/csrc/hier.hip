@@ -330,7 +330,9 @@ static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_ze
     if (cur != x) RAMD_HIP(hipMemcpyAsync(x, cur, sizeof(double) * L.n, hipMemcpyDeviceToDevice, s));
 }
 
-static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero)
+// x_zero: x is zero on entry; first_done: x already holds the first Jacobi
+// sweep from zero (formed by the caller with jacobi_from_zero's operations)
+static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero, bool first_done = false)
 {
     Level& L = h->levels[l];
     hipStream_t s = h->ctx->stream;
@@ -355,7 +357,9 @@ static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero
     float* other = L.t32;
     if (mc) mc_relax<float, float>(h, L, L.A->val32, cur, b, x_zero);
     for (int it = 0; it < sweeps && !mc; it++) {
-        if (it == 0 && x_zero) {
+        if (it == 0 && first_done) {
+            continue;
+        } else if (it == 0 && x_zero) {
             hipLaunchKernelGGL((jacobi_from_zero<float>), grid1(L.n), dim3(256), 0, s, L.n, L.diag32, b, cur, w);
             RAMD_HIP(hipGetLastError());
         } else {
@@ -766,13 +770,17 @@ void vcycle_from_zero(ramd_hier_s* h, double* x, const double* b) { cycle64(h, 0
 // r -= alpha Ap; r32 = (float) r when given; partial sums of r^2.  alpha =
 // rz / pAp is formed from the device scalars (every thread computes the
 // same value; workgroup 0 stores it for the direction update, which also
-// takes the step x += alpha p).
+// takes the step x += alpha p).  With z32 given, also the FP32 cycle's first
+// Jacobi sweep from zero on r32 (jacobi_from_zero's operations, diagonal
+// d32, weight w), so the cycle starts from it.
 __global__ __launch_bounds__(kBlockThreads) void fpcg_update_norm(int n, const double* __restrict__ sc_rz,
                                                                   const double* __restrict__ sc_pap,
                                                                   double* __restrict__ sc_alpha,
                                                                   const double* __restrict__ Ap,
                                                                   double* __restrict__ r, float* __restrict__ r32,
-                                                                  double* __restrict__ partials)
+                                                                  double* __restrict__ partials,
+                                                                  const float* __restrict__ d32,
+                                                                  float* __restrict__ z32, float w)
 {
     const double alpha = *sc_rz / *sc_pap;
     if (blockIdx.x == 0 && threadIdx.x == 0) *sc_alpha = alpha;
@@ -781,7 +789,17 @@ __global__ __launch_bounds__(kBlockThreads) void fpcg_update_norm(int n, const d
     for (int i = blockIdx.x * kBlockThreads + threadIdx.x; i < n; i += stride) {
         const double ri = r[i] - alpha * Ap[i];
         r[i] = ri;
-        if (r32) r32[i] = (float)ri;
+        if (r32) {
+            const float rf = (float)ri;
+            r32[i] = rf;
+            if (z32) {
+                const float di = d32[i];
+                const float zero = 0.0f;
+                float v = zero;
+                if (fabs(di) > float(1e-16)) v = ((1.0f - w) * zero) + (w * ((rf - zero) / di));
+                z32[i] = v;
+            }
+        }
         s += ri * ri;
     }
     s = block_sum(s);
@@ -887,9 +905,14 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
     const int g = fixed_blocks(n);
     const dim3 gv = grid1(n), bv(256);
     // z = M^-1 r from the current r (FP32 form: from r32, which must hold (float) r)
+    // FP32 form: the update pass also forms the cycle's first Jacobi sweep
+    // from zero on the fine level (Jacobi smoothing, more than one level)
+    const bool fuse_first = f32 && !multicolour(h) && h->opts.num_smooth_sweeps >= 1 && h->levels.size() > 1;
+    bool z_first = false;  // z32 holds that sweep of the current r32
     auto cycle = [&] {
-        if (f32) cycle32(h, 0, z32, r32, true);
+        if (f32) cycle32(h, 0, z32, r32, true, z_first);
         else cycle64(h, 0, z, r, true);
+        z_first = false;
     };
     // device scalars: [0] p.Ap, [1] r.r, [2]/[5] r.z (current / next), [3] z.Ap, [4] alpha
     double* sc = ctx->red_result;
@@ -929,8 +952,10 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
             // (the partials buffer may have grown inside launch_spmv_dot)
             double* part = ctx_partials(ctx, (size_t)g);
             hipLaunchKernelGGL(fpcg_update_norm, dim3(g), dim3(kBlockThreads), 0, s, n, sc + cur, sc, sc + 4, Ap, r,
-                               f32 ? r32 : nullptr, part);
+                               f32 ? r32 : nullptr, part, fuse_first ? h->levels[0].diag32 : nullptr,
+                               fuse_first ? z32 : nullptr, (float)h->opts.relax_weight);
             RAMD_HIP(hipGetLastError());
+            z_first = fuse_first;
             launch_reduce_final(ctx, part, g, sc + 1);
             double pr[2];
             fetch_scalars(ctx, sc, 2, pr);
@@ -957,6 +982,7 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
                     hipLaunchKernelGGL(to_f32, gv, bv, 0, s, (int64_t)n, r, r32);
                     RAMD_HIP(hipGetLastError());
                 }
+                z_first = false;  // r32 was replaced
             } else if (relres_hist) {
                 relres_hist[it] = relres;
             }
