@@ -98,9 +98,11 @@ int ramd_csr_diagonals(ramd_csr A, int* nd);
 int ramd_csr_set_dia_staging(ramd_csr A, int pp, int mode);
 /* CSR kernel of the operator for FP64 (prec 0) or FP32 (prec 1) vectors:
  * on = 1 builds and selects the sliced-row form (64 rows per slice, one
- * lane per row), 0 returns to the row-block kernel.  Both give identical
- * bits; the autotuner picks one per operator at hierarchy setup and at
- * FP32 enable.  No reference counterpart. */
+ * lane per row), 2 the same with 16-bit column offsets (col - row; square
+ * operators whose offsets fit, RAMD_E_ARG otherwise), 0 returns to the
+ * row-block kernel.  All give identical bits; the autotuner picks one per
+ * operator at hierarchy setup and at FP32 enable.  No reference
+ * counterpart. */
 int ramd_csr_set_sliced(ramd_csr A, int prec, int on);
 /* The same choice for the lane-per-row kernel over the CSR arrays (one lane
  * walks its own row; no extra storage): on = 1 selects it (and leaves the
@@ -245,7 +247,8 @@ int ramd_hier_states(ramd_hier h, int level, int32_t* states);
 /* Row-block caps (products per workgroup) the autotuner chose for the
  * CSR kernels: six entries per level, [A, P, R] x [FP64, FP32 vectors];
  * -1 where the sliced-row kernel (64 rows per slice, one lane per row) was
- * measured faster, 1 where the lane-per-row kernel over the CSR arrays was,
+ * measured faster, -2 the sliced-row kernel with 16-bit column offsets,
+ * 1 where the lane-per-row kernel over the CSR arrays was,
  * 0 for a stencil-diagonal operator or an absent one.  No
  * reference counterpart. */
 int ramd_hier_row_caps(ramd_hier h, int32_t* caps);

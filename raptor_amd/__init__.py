@@ -213,9 +213,10 @@ class ParCSRMatrix:
         check(lib().ramd_csr_set_dia_staging(self.h, int(pp), int(mode)))
 
     def set_sliced(self, prec, on):
-        """CSR kernel for FP64 (prec 0) or FP32 (prec 1) vectors: True selects
-        the sliced-row form, False the row-block kernel (identical bits)."""
-        check(lib().ramd_csr_set_sliced(self.h, int(prec), int(bool(on))))
+        """CSR kernel for FP64 (prec 0) or FP32 (prec 1) vectors: True (or 1)
+        selects the sliced-row form, 2 the sliced-row form with 16-bit column
+        offsets, False the row-block kernel (identical bits)."""
+        check(lib().ramd_csr_set_sliced(self.h, int(prec), int(on)))
 
     def set_lane_rows(self, prec, on):
         """CSR kernel for FP64 (prec 0) or FP32 (prec 1) vectors: True selects

@@ -177,11 +177,16 @@ namespace ramd {
 // sptr[q] + 64k + l) and padded to its longest row with column -1.  The
 // one-lane-per-row form of operators with long rows: every load of a
 // wavefront is one contiguous run.
+// A square operator whose columns all lie within 32767 of their row also
+// gets 16-bit column offsets (col - row; padding kSliced16Pad), two bytes per
+// entry less to stream.
+constexpr int16_t kSliced16Pad = -32768;
 struct Sliced {
     int n_slices = 0;
     int64_t entries = 0;
     int64_t* sptr = nullptr;  // device, n_slices + 1
     int* col = nullptr;       // device, entries
+    int16_t* col16 = nullptr; // device, entries (when the offsets fit)
     double* v64 = nullptr;    // device, entries (built on demand)
     float* v32 = nullptr;
 };
@@ -192,6 +197,7 @@ struct ramd_csr_s {
     Dia* dia = nullptr;         // stencil-diagonal form (fine-level operators)
     ramd::Sliced* sl = nullptr; // sliced-row form (long-row operators)
     bool sl_on[2] = {false, false};  // per vector precision: use the sliced form
+    bool sl16[2] = {false, false};   // ... with its 16-bit column offsets
     bool lr_on[2] = {false, false};  // per vector precision: one lane per CSR row
     int32_t n_rows = 0, n_cols = 0;
     int64_t nnz = 0;

@@ -362,16 +362,49 @@ __global__ void sliced_scatter_kernel(int n, const int* __restrict__ rp, const i
     for (int k = s; k < e; k++) out[base + (int64_t)64 * (k - s)] = in[k];
 }
 
+// 16-bit offsets col - r of row r's entries, in the sliced positions
+__global__ void sliced16_scatter_kernel(int n, const int* __restrict__ rp, const int64_t* __restrict__ sptr,
+                                        const int* __restrict__ col, int16_t* __restrict__ out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t base = sptr[r >> 6] + (r & 63);
+    const int s = rp[r], e = rp[r + 1];
+    for (int k = s; k < e; k++) out[base + (int64_t)64 * (k - s)] = (int16_t)(col[k] - r);
+}
+
+__global__ void fill_i16_kernel(int16_t* __restrict__ p, int64_t n, int16_t v)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// largest |col - row| over the matrix, into *out (atomic maximum)
+__global__ void col_offset_range_kernel(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                        int* __restrict__ out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int m = 0;
+    for (int k = rp[r]; k < rp[r + 1]; k++) {
+        const int d = col[k] - r;
+        m = max(m, d < 0 ? -d : d);
+    }
+    if (m) atomicMax(out, m);
+}
+
 void sliced_free(ramd_csr_s* A)
 {
     if (!A->sl) return;
     dfree(A->sl->sptr);
     dfree(A->sl->col);
+    dfree(A->sl->col16);
     dfree(A->sl->v64);
     dfree(A->sl->v32);
     delete A->sl;
     A->sl = nullptr;
     A->sl_on[0] = A->sl_on[1] = false;
+    A->sl16[0] = A->sl16[1] = false;
 }
 
 void sliced_build(ramd_csr_s* A)
@@ -400,6 +433,30 @@ void sliced_build(ramd_csr_s* A)
         hipLaunchKernelGGL((sliced_scatter_kernel<int>), dim3((n + 255) / 256), dim3(256), 0, s, n, A->rowptr,
                            S->sptr, A->col, S->col);
     RAMD_HIP(hipGetLastError());
+    // 16-bit column offsets when every |col - row| fits
+    if (n && A->n_rows == A->n_cols) {
+        int* range = dalloc<int>(1);
+        int m = 0;
+        try {
+            hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, range, (int64_t)1, 0);
+            hipLaunchKernelGGL(col_offset_range_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, A->rowptr, A->col,
+                               range);
+            RAMD_HIP(hipGetLastError());
+            RAMD_HIP(hipMemcpyAsync(&m, range, sizeof(int), hipMemcpyDeviceToHost, s));
+            RAMD_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            dfree(range);
+            throw;
+        }
+        dfree(range);
+        if (m <= 32767) {
+            S->col16 = dalloc<int16_t>((size_t)std::max<int64_t>(S->entries, 1));
+            hipLaunchKernelGGL(fill_i16_kernel, dim3(g), dim3(256), 0, s, S->col16, S->entries, kSliced16Pad);
+            hipLaunchKernelGGL(sliced16_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, A->rowptr, S->sptr,
+                               A->col, S->col16);
+            RAMD_HIP(hipGetLastError());
+        }
+    }
     RAMD_HIP(hipStreamSynchronize(s));  // sptr's host copy goes out of scope
 }
 
@@ -853,7 +910,7 @@ int ramd_csr_set_dia_staging(ramd_csr A, int pp, int mode)
 int ramd_csr_set_sliced(ramd_csr A, int prec, int on)
 {
     return guarded([&] {
-        RAMD_CHECK(A && (prec == 0 || prec == 1) && (on == 0 || on == 1), RAMD_E_ARG,
+        RAMD_CHECK(A && (prec == 0 || prec == 1) && on >= 0 && on <= 2, RAMD_E_ARG,
                    "ramd_csr_set_sliced: bad argument");
         RAMD_CHECK(!A->dia, RAMD_E_STATE, "ramd_csr_set_sliced: matrix uses its stencil-diagonal form");
         A->ctx->set_current();
@@ -864,7 +921,10 @@ int ramd_csr_set_sliced(ramd_csr A, int prec, int on)
             sliced_values(A, prec);
             RAMD_HIP(hipStreamSynchronize(A->ctx->stream));
         }
+        RAMD_CHECK(on != 2 || A->sl->col16, RAMD_E_ARG,
+                   "ramd_csr_set_sliced: the column offsets do not fit 16 bits (or the matrix is not square)");
         A->sl_on[prec] = on != 0;
+        A->sl16[prec] = on == 2;
         if (on) A->lr_on[prec] = false;
     });
 }
@@ -877,7 +937,7 @@ int ramd_csr_set_lane_rows(ramd_csr A, int prec, int on)
         RAMD_CHECK(!A->dia, RAMD_E_STATE, "ramd_csr_set_lane_rows: matrix uses its stencil-diagonal form");
         RAMD_CHECK(prec == 0 || A->val32 || !on, RAMD_E_STATE, "ramd_csr_set_lane_rows: FP32 values were not built");
         A->lr_on[prec] = on != 0;
-        if (on) A->sl_on[prec] = false;
+        if (on) A->sl_on[prec] = A->sl16[prec] = false;
     });
 }
 

@@ -1399,7 +1399,7 @@ int ramd_hier_row_caps(ramd_hier h, int32_t* caps)
             for (int m = 0; m < 3; m++)
                 for (int p = 0; p < 2; p++)
                     caps[6 * l + 2 * m + p] =
-                        (M[m] && !M[m]->dia) ? (M[m]->sl_on[p] ? -1 : M[m]->lr_on[p] ? 1 : M[m]->rb[p].cap) : 0;
+                        (M[m] && !M[m]->dia) ? (M[m]->sl_on[p] ? (M[m]->sl16[p] ? -2 : -1) : M[m]->lr_on[p] ? 1 : M[m]->rb[p].cap) : 0;
         }
     });
 }

@@ -169,22 +169,24 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
     }
 }
 
-// One-lane-per-row kernels, over two layouts.  LAYOUT 0, the sliced form:
+// One-lane-per-row kernels, over three layouts.  LAYOUT 0, the sliced form:
 // every wavefront reads its slice's columns and values as contiguous runs
-// and padding (column -1) is loaded and dropped by a select.  LAYOUT 1, the
-// CSR arrays themselves (short rows, no LDS staging): a lane walks its own
-// row, entries past the row's end are not loaded and count as padding.  A
+// and padding (column -1) is loaded and dropped by a select.  LAYOUT 2, the
+// sliced form with 16-bit column offsets (column = row + offset, padding
+// kSliced16Pad).  LAYOUT 1, the CSR arrays themselves (short rows, no LDS
+// staging): a lane walks its own row, entries past the row's end are not
+// loaded and count as padding.  A
 // lane adds its row's entries in stored order with the same operations as
 // phase 2 of stream_kernel, so the bits are the same; eight entries' loads
 // are in flight per lane.
 template <int OP, typename V, typename X, int LAYOUT>
 __global__ __launch_bounds__(kBlockThreads) void sliced_kernel(
     int n, const int64_t* __restrict__ sptr, const int* __restrict__ rowptr, const int* __restrict__ col,
-    const V* __restrict__ val, const X* __restrict__ x, const X* __restrict__ b, X* __restrict__ y, X omega,
-    int diag_first, double* __restrict__ dotp, const X* __restrict__ dvec)
+    const int16_t* __restrict__ col16, const V* __restrict__ val, const X* __restrict__ x, const X* __restrict__ b,
+    X* __restrict__ y, X omega, int diag_first, double* __restrict__ dotp, const X* __restrict__ dvec)
 {
     constexpr int B = 8;
-    constexpr int64_t STEP = LAYOUT == 0 ? 64 : 1;
+    constexpr int64_t STEP = LAYOUT == 1 ? 1 : 64;  // CSR rows: consecutive entries; sliced: 64 apart
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int row = blk * kBlockThreads + threadIdx.x;
     const bool valid = row < n;
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(kBlockThreads) void sliced_kernel(
     int64_t base = 0;
     int w = 0;
     if (valid) {
-        if (LAYOUT == 0) {
+        if (LAYOUT != 1) {
             base = sptr[row >> 6] + lane;
             w = (int)((sptr[(row >> 6) + 1] - sptr[row >> 6]) >> 6);
         } else {
@@ -215,7 +217,7 @@ __global__ __launch_bounds__(kBlockThreads) void sliced_kernel(
             } else {
                 k_first = w;
             }
-        } else if (w > 0 && col[base] == row) {
+        } else if (w > 0 && (LAYOUT == 2 ? col16[base] == 0 : col[base] == row)) {
             diag = X(val[base]);
             k_first = 1;
         } else {
@@ -229,7 +231,12 @@ __global__ __launch_bounds__(kBlockThreads) void sliced_kernel(
 #pragma unroll
         for (int t = 0; t < B; t++) {
             const bool in = k0 + t < w;
-            cc[t] = in ? col[base + STEP * (k0 + t)] : -1;
+            if (LAYOUT == 2) {
+                const int d = in ? col16[base + STEP * (k0 + t)] : kSliced16Pad;
+                cc[t] = d == kSliced16Pad ? -1 : row + d;
+            } else {
+                cc[t] = in ? col[base + STEP * (k0 + t)] : -1;
+            }
             vv[t] = in ? val[base + STEP * (k0 + t)] : V(0);
         }
 #pragma unroll
@@ -621,7 +628,8 @@ static void launch_stream_cap(const ramd_csr_s* A, const ramd_csr_s::RowBlocks& 
     }
 }
 
-// LAYOUT 0: the sliced form; 1: lanes over the CSR rows (val: CSR values)
+// LAYOUT 0: the sliced form; 2: the same with 16-bit column offsets; 1: lanes
+// over the CSR rows (val: CSR values)
 template <typename V, typename X, int LAYOUT>
 static void launch_sliced(const ramd_csr_s* A, const V* csr_val, int op, const X* x, const X* b, X* y, X omega,
                           hipStream_t s, double* dotp, const X* dvec)
@@ -630,19 +638,22 @@ static void launch_sliced(const ramd_csr_s* A, const V* csr_val, int op, const X
     const V* val = csr_val;
     const int64_t* sptr = nullptr;
     const int* col = A->col;
-    if (LAYOUT == 0) {
+    const int16_t* col16 = nullptr;
+    if (LAYOUT != 1) {
         if constexpr (sizeof(V) == 4) val = S->v32;
         else val = S->v64;
         sptr = S->sptr;
         col = S->col;
+        col16 = S->col16;
+        RAMD_CHECK(LAYOUT == 0 || col16, RAMD_E_STATE, "16-bit column offsets of the sliced form were not built");
     }
     RAMD_CHECK(val, RAMD_E_STATE, "values of this precision were not built for the lane-per-row kernel");
     const int n = A->n_rows;
     const dim3 grid((n + kBlockThreads - 1) / kBlockThreads), block(kBlockThreads);
     const int df = (int)A->diag_first;
 #define SLICED(OPV, DOT, DV)                                                                                    \
-    hipLaunchKernelGGL((sliced_kernel<OPV, V, X, LAYOUT>), grid, block, 0, s, n, sptr, A->rowptr, col, val, x, b, \
-                       y, omega, df, DOT, DV)
+    hipLaunchKernelGGL((sliced_kernel<OPV, V, X, LAYOUT>), grid, block, 0, s, n, sptr, A->rowptr, col, col16, val, \
+                       x, b, y, omega, df, DOT, DV)
     switch (op) {
     case OP_SPMV: SLICED(OP_SPMV, dotp, nullptr); break;
     case OP_APPEND: SLICED(OP_APPEND, nullptr, nullptr); break;
@@ -683,6 +694,7 @@ static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int 
     }
     if (A->lr_on[sizeof(X) == 4 ? 1 : 0] || (A->sl && A->sl_on[sizeof(X) == 4 ? 1 : 0])) {
         if (A->lr_on[sizeof(X) == 4 ? 1 : 0]) launch_sliced<V, X, 1>(A, val, op, x, b, y, omega, s, dotp, dvec);
+        else if (A->sl16[sizeof(X) == 4 ? 1 : 0]) launch_sliced<V, X, 2>(A, val, op, x, b, y, omega, s, dotp, dvec);
         else launch_sliced<V, X, 0>(A, val, op, x, b, y, omega, s, dotp, dvec);
         if (prof) {
             RAMD_HIP(hipEventRecord(pend.end, s));
@@ -757,6 +769,7 @@ void csr_autotune(ramd_csr_s* A, int prec)
         A->rb[prec] = {kStreamCap, A->n_blocks, A->blocks};
     }
     A->sl_on[prec] = false;
+    A->sl16[prec] = false;
     A->lr_on[prec] = false;
     const size_t w = prec ? sizeof(float) : sizeof(double);
     struct Scratch {
@@ -838,8 +851,15 @@ void csr_autotune(ramd_csr_s* A, int prec)
             const bool lr = A->lr_on[prec];
             A->lr_on[prec] = false;
             A->sl_on[prec] = true;
-            const float ms = time_ms();
+            float ms = time_ms();
+            if (A->sl->col16) {  // and with 16-bit column offsets
+                A->sl16[prec] = true;
+                const float ms16 = time_ms();
+                A->sl16[prec] = ms16 < ms;
+                ms = std::min(ms, ms16);
+            }
             A->sl_on[prec] = ms < best_ms;
+            if (!A->sl_on[prec]) A->sl16[prec] = false;
             A->lr_on[prec] = lr && !A->sl_on[prec];
         }
         if (!A->sl_on[prec]) {  // release what this precision built

@@ -89,7 +89,7 @@ def sliced_cases():
             ("empty_rows", K.Csr([0, 0, 2, 2, 3], [1, 3, 0], [1.5, -2.0, 4.0], 4))]
 
 
-@pytest.mark.parametrize("layout", ["sliced", "lane_rows"])
+@pytest.mark.parametrize("layout", ["sliced", "sliced16", "lane_rows"])
 @pytest.mark.parametrize("name,A", sliced_cases(), ids=[c[0] for c in sliced_cases()])
 def test_sliced_kernel_bit_exact(ctx, name, A, layout):
     """The one-lane-per-row kernels -- over the sliced form (64 rows per
@@ -100,6 +100,12 @@ def test_sliced_kernel_bit_exact(ctx, name, A, layout):
     D = to_dev(ctx, A)
     if layout == "sliced":
         D.set_sliced(0, True)
+    elif layout == "sliced16":
+        if A.n_rows != A.n_cols:  # offsets col - row need a square operator
+            with pytest.raises(ra.RamdError):
+                D.set_sliced(0, 2)
+            return
+        D.set_sliced(0, 2)
     else:
         D.set_lane_rows(0, True)
     y = ra.ParVector(ctx, A.n_rows)

@@ -171,10 +171,12 @@ def test_vcycle_large_coarse_system(ctx, max_coarse):
     assert np.array_equal(x32.numpy(), H.cycle_fp32(np.zeros(n, np.float32), b32))
 
 
-def test_vcycle_sliced_bit_exact(problem, ctx):
-    """Every CSR level operator switched to the sliced-row kernel (the
-    Jacobi sweeps then read the level's diagonal array): FP64 and FP32 cycles
-    still bit-identical to the oracle.  The tuned choices are restored."""
+@pytest.mark.parametrize("mode", [1, 2], ids=["cols32", "offsets16"])
+def test_vcycle_sliced_bit_exact(problem, ctx, mode):
+    """Every CSR level operator switched to the sliced-row kernel, with 32-bit
+    columns or 16-bit column offsets (the Jacobi sweeps then read the level's
+    diagonal array): FP64 and FP32 cycles still bit-identical to the oracle.
+    The tuned choices are restored."""
     ml, As, Ps = problem["ml"], problem["As"], problem["Ps"]
     ml.enable_fp32()
     before = ml.row_caps()
@@ -184,10 +186,10 @@ def test_vcycle_sliced_bit_exact(problem, ctx):
         if A.diagonals() == 0 and A.nnz > 0:
             mats.append((l, A))
             for p in (0, 1):
-                A.set_sliced(p, True)
+                A.set_sliced(p, mode)
     if not mats:
         pytest.skip("no CSR level operator")
-    assert np.all(ml.row_caps()[[l for l, _ in mats], 0, :] == -1)
+    assert np.all(ml.row_caps()[[l for l, _ in mats], 0, :] == -mode)
     try:
         H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=OMEGA)
         n = As[0].n_rows
@@ -203,7 +205,7 @@ def test_vcycle_sliced_bit_exact(problem, ctx):
     finally:
         for l, A in mats:
             for p in (0, 1):
-                A.set_sliced(p, bool(before[l, 0, p] == -1))
+                A.set_sliced(p, int(-before[l, 0, p]) if before[l, 0, p] < 0 else 0)
                 if before[l, 0, p] == 1:
                     A.set_lane_rows(p, True)
     assert np.array_equal(ml.row_caps(), before)
@@ -243,8 +245,8 @@ def test_vcycle_lane_rows_bit_exact(problem, ctx):
     finally:
         for l, m, M in mats:
             for p in (0, 1):
-                if before[l, m, p] == -1:
-                    M.set_sliced(p, True)
+                if before[l, m, p] < 0:
+                    M.set_sliced(p, int(-before[l, m, p]))
                 else:
                     M.set_lane_rows(p, bool(before[l, m, p] == 1))
     assert np.array_equal(ml.row_caps(), before)
@@ -412,7 +414,7 @@ def test_row_cap_autotuner_choices(problem):
     ml.enable_fp32()
     caps = ml.row_caps()
     assert caps.shape == (ml.num_levels, 3, 2)
-    assert set(np.unique(caps)) <= {-1, 0, 1, 2048, 4096, 8192}  # -1: sliced, 1: lane rows
+    assert set(np.unique(caps)) <= {-2, -1, 0, 1, 2048, 4096, 8192}  # -1/-2: sliced, 1: lane rows
     assert not np.any(caps[:, :, 0] == 8192)
     assert np.all(caps[-1, 1:] == 0)  # no P, R at the coarsest level
 
