@@ -47,6 +47,24 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
     const int r0 = blocks[blk], r1 = blocks[blk + 1];
     const int s = rowptr[r0], e = rowptr[r1];
     const int tid = threadIdx.x;
+    // the lane's row bounds and per-row operands, loaded together with the
+    // block's bounds instead of after phase 1
+    const int row = r0 + tid;
+    const bool has_row = row < r1;
+    int rs = 0, re = 0;
+    X pre = X(0), bj = X(0), dgv = X(0);  // b / y / x of the row, b and a_ii for Jacobi
+    if (has_row) {
+        rs = rowptr[row];
+        re = rowptr[row + 1];
+        if (OP == OP_RESIDUAL || OP == OP_ADD_INTO) pre = b[row];
+        else if (OP == OP_APPEND) pre = y[row];
+        else if (OP == OP_SPMV && dotp) pre = x[row];
+        else if (OP == OP_JACOBI) {
+            pre = x[row];
+            bj = b[row];
+            if (dvec) dgv = dvec[row];
+        }
+    }
 
     if (e - s > CAP) {
         // one long row, processed in LDS-sized chunks, summed by lane 0
@@ -114,31 +132,29 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
     __syncthreads();
 
     // phase 2: one lane per row, products added left to right
-    const int row = r0 + tid;
+    const int k0 = rs - s, k1 = re - s;
     if (OP == OP_SPMV) {
         X acc = 0;
-        if (row < r1) {
-            const int k0 = rowptr[row] - s, k1 = rowptr[row + 1] - s;
+        if (has_row) {
             acc = lds_row_sum<false>(prod, k0, k1, acc);
             y[row] = acc;
         }
         if (dotp) {
-            const double v = block_sum(row < r1 ? (double)x[row] * (double)acc : 0.0);
+            const double v = block_sum(has_row ? (double)pre * (double)acc : 0.0);
             if (tid == 0) dotp[blk] = v;
         }
         return;
     }
-    if (row >= r1) return;
-    const int k0 = rowptr[row] - s, k1 = rowptr[row + 1] - s;
+    if (!has_row) return;
     if (OP == OP_APPEND || OP == OP_ADD_INTO) {
         // CSR_append: the row sum is formed first, then added (spmv.cpp:96-101)
         const X acc = lds_row_sum<false>(prod, k0, k1, X(0));
-        y[row] = (OP == OP_APPEND ? y[row] : b[row]) + acc;
+        y[row] = pre + acc;
     } else if (OP == OP_RESIDUAL) {
-        y[row] = lds_row_sum<true>(prod, k0, k1, b[row]);
+        y[row] = lds_row_sum<true>(prod, k0, k1, pre);
     } else {
         // relax.cpp:23-41: x holds the old iterate (tmp), y receives the new
-        const X xo = x[row];
+        const X xo = pre;
         if (k0 == k1) {
             y[row] = xo;
             return;
@@ -148,7 +164,7 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
             // the level's diagonal array holds a_ii (0 where absent); a row
             // whose |a_ii| > 1e-16 stores it first, any other row keeps its
             // value -- the row's first entry is not read again
-            const X dg = dvec[row];
+            const X dg = dgv;
             if (fabs(dg) > X(1e-16)) {
                 diag = dg;
                 acc = lds_row_sum<false>(prod, k0 + 1, k1, acc);
@@ -165,7 +181,7 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
                 else acc += prod[k];
             }
         }
-        y[row] = (fabs(diag) > X(1e-16)) ? ((X(1.0) - omega) * xo) + (omega * ((b[row] - acc) / diag)) : xo;
+        y[row] = (fabs(diag) > X(1e-16)) ? ((X(1.0) - omega) * xo) + (omega * ((bj - acc) / diag)) : xo;
     }
 }
 
