@@ -170,3 +170,59 @@ def test_device_pmis_cljp_golden(ctx, golden, level):
     w = golden["weights"][:S.n_rows]
     assert np.array_equal(ra.split_pmis_gpu(D, w), golden[f"rss_cf{level}_pmis"])
     assert np.array_equal(ra.split_cljp_gpu(D, w), golden[f"rss_cf{level}"])
+
+
+def hub_mmatrix(n, hub_deg, seed):
+    """A sparse M-matrix (a chain) with one hub row that depends strongly on
+    hub_deg rows while they depend on it only weakly: the hub stays F with
+    more than 256 strong C neighbours, so the device interpolation hands that
+    level to the host."""
+    rng = np.random.default_rng(seed)
+    rows = [dict() for _ in range(n)]
+    for i in range(n - 1):
+        rows[i][i + 1] = -1.0
+        rows[i + 1][i] = -1.0
+    for j in rng.choice(np.arange(2, n), size=hub_deg, replace=False):
+        rows[0][int(j)] = -1.0
+        rows[int(j)][0] = -0.01
+    rp, col, val = [0], [], []
+    for i in range(n):
+        items = sorted(rows[i].items())
+        col += [i] + [j for j, _ in items]
+        val += [-sum(v for _, v in items) + 0.1] + [v for _, v in items]
+        rp.append(len(col))
+    return np.array(rp, np.int32), np.array(col, np.int32), np.array(val)
+
+
+@pytest.mark.parametrize("interp", [ra.Extended, ra.ModClassical], ids=["extended", "modclass"])
+def test_device_setup_row_overflow_falls_back(ctx, interp):
+    """A level with a row past the device kernels' 256 columns is built by
+    the host restatement; the hierarchy still equals the all-host one (and
+    the device entry point reports the overflow)."""
+    rp, col, val = hub_mmatrix(3000, 1200, 5)
+    A = K.move_diag(K.sort_rows(K.Csr(rp, col, val)))
+    D = ra.ParCSRMatrix.from_csr(ctx, A.rowptr, A.col, A.val)
+    S = ra.strength_gpu(D, 0.25)
+    st = ra.split_pmis_gpu(S, np.random.default_rng(1).random(A.n_rows))
+    fn = ra.interp_extended_gpu if interp == ra.Extended else ra.interp_mod_classical_gpu
+    args = (D, S, st, 0.3) if interp == ra.Extended else (D, S, st)
+    overflow = False
+    try:
+        fn(*args)
+    except ra.RamdError:
+        overflow = True
+    mls = []
+    for dev in (True, False):
+        ml = ra.ParRugeStubenSolver(ctx, 0.25, ra.PMIS, interp, ra.Classical, ra.Jacobi)
+        ml.device_setup = dev
+        ml.setup(rp, col, val)
+        mls.append(ml)
+    a, b = mls
+    assert a.num_levels == b.num_levels
+    for l in range(a.num_levels):
+        for M1, M2 in zip(a.level(l), b.level(l)):
+            if M1 is None:
+                assert M2 is None
+                continue
+            assert all(np.array_equal(u, v) for u, v in zip(M1.to_csr(), M2.to_csr())), l
+    assert overflow  # the hub row needs more than 256 columns
