@@ -211,6 +211,22 @@ def main():
         vc[prec_name] = dict(bytes=by, s=round(dt, 6), gbs=round(by / dt / 1e9, 1))
         log(f"V-cycle {prec_name}: {dt * 1e3:.3f} ms, {by / dt / 1e9:.0f} GB/s algorithmic")
 
+    # ---- the reference's PCG rules (krylov/par_cg.cpp:121-263, FP64 cycle) on
+    # the same system, once, outside the timed region: its iteration count
+    xr = ra.ParVector(ctx, n)
+    ctx.sync()
+    t0 = time.perf_counter()
+    _, it_ref = ra.PCG(A, ml, xr, b, tol=1e-8)
+    ctx.sync()
+    t_ref = time.perf_counter() - t0
+    rr = ra.ParVector(ctx, n)
+    A.residual(xr, b, rr)
+    rel_ref = float(np.linalg.norm(rr.numpy()) / np.linalg.norm(b_h))
+    pcg_ref = dict(iterations=it_ref, s=round(t_ref, 5), true_relres=rel_ref)
+    log(f"PCG with the reference's rules (FP64 cycle, tol 1e-8): {it_ref} iterations, {t_ref:.4f}s, "
+        f"true relres {rel_ref:.3e}")
+    del xr, rr
+
     # ---- roofline: dominant kernel (most total event time)
     dom_name, dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
     per_launch_bytes = dom["bytes"] / dom["launches"]
@@ -254,6 +270,7 @@ def main():
         "relres_fp32": rel32,
         "relres_fp64": rel64,
         "error_fp32_vs_fp64_reference": err32,
+        "pcg_reference_rule": pcg_ref,
         "setup_s": round(setup_s, 2),
         "vcycle": vc,
         "kernels": prof,
