@@ -472,13 +472,25 @@ def bench_adjoint(args, m, cfg):
     g = ra.ParVector(ctx, n)
     rtol = 1e-10
 
-    def step():
+    def step(prec=ra.PREC_FP64):
         x.set_const_value(0.0)
-        _, itf, _ = ra.FPCG(A, ml, x, b, rtol, 500, ra.PREC_FP64)
+        _, itf, _ = ra.FPCG(A, ml, x, b, rtol, 500, prec)
         g.copy(x)
         g.axpy(xs, -1.0)
-        dA, db, ita = ra.adjoint(A, ml, x, g, rtol=rtol)
+        dA, db, ita = ra.adjoint(A, ml, x, g, rtol=rtol, cycle_prec=prec)
         return itf, ita, dA, db
+
+    # the same step with FP32 cycles inside the FP64 flexible PCG (reported
+    # beside the FP64-cycle value)
+    step(ra.PREC_FP32)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        itf32, ita32, dA32, _ = step(ra.PREC_FP32)
+    ctx.sync()
+    dt32 = (time.perf_counter() - t0) / args.steps
+    gA32 = dA32.numpy()
+    log(f"forward + adjoint with FP32 cycles: {dt32 * 1e3:.2f} ms (forward {itf32} iterations, adjoint {ita32})")
 
     for _ in range(args.warmup):
         step()
@@ -490,6 +502,8 @@ def bench_adjoint(args, m, cfg):
     dt = (time.perf_counter() - t0) / args.steps
     log(f"forward + adjoint: {dt * 1e3:.2f} ms (forward {itf} iterations, adjoint {ita})")
     xh, gA, gb = x.numpy(), dA.numpy(), db.numpy()
+    grad32_diff = float(np.linalg.norm(gA32 - gA) / np.linalg.norm(gA))
+    log(f"dL/dA with FP32 cycles vs FP64 cycles: relative difference {grad32_diff:.2e}")
     loss = 0.5 * float(np.sum((xh - xs_h) ** 2))
 
     # central differences on 32 stored entries (dense-free: perturbation
@@ -576,6 +590,8 @@ def bench_adjoint(args, m, cfg):
         "loss": loss,
         "fd_max_rel_error": max(errs),
         "lambda_rel_diff_vs_oracle": lam_err,
+        "fp32_cycle": dict(s=round(dt32, 5), iterations_forward=itf32, iterations_adjoint=ita32,
+                           dA_rel_diff_vs_fp64_cycles=grad32_diff),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
