@@ -926,11 +926,12 @@ __global__ void direct_interp_kernel(int n, const int* __restrict__ arp, const i
     const int e = srp[i + 1];
     if (s < e && scol[s] == i) s++;
     const int as0 = arp[i], ae = arp[i + 1];
-    // A's value at column c of row i (S's columns appear in A's row in order)
+    // A's value at column c of row i (S's columns appear in A's row in order;
+    // a column missing from A's row reads as 0 instead of running past it)
     int actr = as0;
     auto a_at = [&](int c) {
-        while (acol[actr] != c) actr++;
-        return aval[actr];
+        while (actr < ae && acol[actr] != c) actr++;
+        return actr < ae ? aval[actr] : 0.0;
     };
     double sum_strong_pos = 0, sum_strong_neg = 0, sum_all_pos = 0, sum_all_neg = 0;
     int count = 0;
