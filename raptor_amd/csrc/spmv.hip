@@ -322,17 +322,17 @@ __device__ __forceinline__ void dia_load(const V* __restrict__ val, int c0, int 
     constexpr int G = 16 / (int)sizeof(V);
     constexpr int NSL = (ND + G - 1) / G * G;
     static_assert(CH % G == 0, "chunk must hold whole groups");
-    using VG = typename std::conditional<sizeof(V) == 4, float4, double2>::type;
+    typedef V VG __attribute__((ext_vector_type(G)));
     const VG* vg = reinterpret_cast<const VG*>(val);
 #pragma unroll
     for (int g = 0; g < CH / G; g++) {
         const int s0 = c0 + g * G;
         const bool ok = (ND == kDiaMax) ? s0 < nsl : s0 < NSL;
-        VG v{};
-        if (ok) v = vg[(size_t)(s0 / G) * n + row];
-        const V* pv = reinterpret_cast<const V*>(&v);
+        VG v = VG(V(0));
+        // values are read once per operation: streamed past the caches
+        if (ok) v = __builtin_nontemporal_load(&vg[(size_t)(s0 / G) * n + row]);
 #pragma unroll
-        for (int t = 0; t < G; t++) vv[g * G + t] = pv[t];
+        for (int t = 0; t < G; t++) vv[g * G + t] = v[t];
     }
 }
 
