@@ -114,9 +114,9 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const int j = tid + u * kBlockThreads;
-        if (j < len) {
-            cc[u] = col[s + j];
-            vv[u] = X(val[s + j]);
+        if (j < len) {  // streamed once: past the caches
+            cc[u] = __builtin_nontemporal_load(&col[s + j]);
+            vv[u] = X(__builtin_nontemporal_load(&val[s + j]));
         }
     }
 #pragma unroll
@@ -247,13 +247,14 @@ __global__ __launch_bounds__(kBlockThreads) void sliced_kernel(
 #pragma unroll
         for (int t = 0; t < B; t++) {
             const bool in = k0 + t < w;
+            // entries are streamed once: past the caches
             if (LAYOUT == 2) {
-                const int d = in ? col16[base + STEP * (k0 + t)] : kSliced16Pad;
+                const int d = in ? (int)__builtin_nontemporal_load(&col16[base + STEP * (k0 + t)]) : kSliced16Pad;
                 cc[t] = d == kSliced16Pad ? -1 : row + d;
             } else {
-                cc[t] = in ? col[base + STEP * (k0 + t)] : -1;
+                cc[t] = in ? __builtin_nontemporal_load(&col[base + STEP * (k0 + t)]) : -1;
             }
-            vv[t] = in ? val[base + STEP * (k0 + t)] : V(0);
+            vv[t] = in ? __builtin_nontemporal_load(&val[base + STEP * (k0 + t)]) : V(0);
         }
 #pragma unroll
         for (int t = 0; t < B; t++) xx[t] = x[cc[t] < 0 ? 0 : cc[t]];
