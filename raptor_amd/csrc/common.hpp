@@ -38,6 +38,9 @@ constexpr int kBlockThreads = 256;   // one workgroup = 4 wavefronts of 64
 // many entries per row, when padding adds at most half the entries again
 constexpr double kSlicedMinRowLen = 12.0;
 constexpr double kSlicedMaxPad = 1.5;
+// the lane-per-row CSR kernel is tried for operators averaging at most this
+// many entries per row
+constexpr double kLaneRowsMaxLen = 8.0;
 constexpr int kStreamCap = 4096;     // products staged in LDS per workgroup
 constexpr int kMaxRowsPerBlock = 256;
 

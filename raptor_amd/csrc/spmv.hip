@@ -811,7 +811,9 @@ void csr_autotune(ramd_csr_s* A, int prec)
     RAMD_HIP(hipEventCreate(&e1));
     ramd_csr_s::RowBlocks best = A->rb[prec];
     float best_ms = 1e30f;
-    // one warm-up launch, then the time of three
+    // one warm-up launch, then the time of three (of twenty for small
+    // operators, whose single launches are too short to rank reliably)
+    const int reps = A->nnz < (int64_t)4 << 20 ? 20 : 3;
     auto time_ms = [&] {
         auto run = [&] {
             if (prec) launch_stream<float, float>(A, A->val32, nullptr, op, (const float*)t.x, (const float*)t.b,
@@ -821,7 +823,7 @@ void csr_autotune(ramd_csr_s* A, int prec)
         };
         run();
         RAMD_HIP(hipEventRecord(e0, s));
-        for (int r = 0; r < 3; r++) run();
+        for (int r = 0; r < reps; r++) run();
         RAMD_HIP(hipEventRecord(e1, s));
         RAMD_HIP(hipEventSynchronize(e1));
         float ms = 0;
@@ -853,9 +855,10 @@ void csr_autotune(ramd_csr_s* A, int prec)
     for (int*& p : t.blocks)
         if (p == best.d) p = nullptr;
     A->rb[prec] = best;
-    // lanes straight over the CSR rows (no extra storage)
-    A->lr_on[prec] = true;
-    {
+    // lanes straight over the CSR rows (no extra storage), for short rows:
+    // a lane adds its whole row alone
+    if ((double)A->nnz <= kLaneRowsMaxLen * (double)A->n_rows) {
+        A->lr_on[prec] = true;
         const float ms = time_ms();
         A->lr_on[prec] = ms < best_ms;
         if (ms < best_ms) best_ms = ms;
