@@ -922,9 +922,12 @@ static int block_tune(ramd_hier_s* h, const ramd_csr_s* M, int k, const X* x, co
     if (aligned16(x) && aligned16(b) && aligned16(y))
         for (int lpr : {4, 8, 16})
             if (vec_ok<X>(lpr, k)) codes.push_back(-lpr);
+    // small operators: twenty timed launches (microsecond launches rank unreliably over three)
+    const int reps = M->nnz * (int64_t)k < ((int64_t)4 << 20) ? 20 : 3;
     for (int code : codes) {
         const double ms = time_ms(
-            s, [&] { blaunch<X>(M, code, M->n_rows == M->n_cols ? OP_RESIDUAL : OP_SPMV, k, x, b, y, X(0), s); }, 3);
+            s, [&] { blaunch<X>(M, code, M->n_rows == M->n_cols ? OP_RESIDUAL : OP_SPMV, k, x, b, y, X(0), s); },
+            reps);
         if (ms < best_ms) {
             best_ms = ms;
             best = code;
