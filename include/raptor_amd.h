@@ -254,6 +254,23 @@ int ramd_hier_states(ramd_hier h, int level, int32_t* states);
 int ramd_hier_row_caps(ramd_hier h, int32_t* caps);
 /* Host-seconds spent per setup phase: [strength, coarsen, interp, galerkin, other]. */
 int ramd_hier_setup_times(ramd_hier h, double* times5);
+/* Per-level timing buckets in the reference's layout: times[5 * l + 0] is
+ * level l's total (ParMultilevel::setup_times / solve_times with
+ * track_times = true, multilevel/par_multilevel.hpp:133-211,343-456; the
+ * collective / p2p / vector / matrix communication buckets 1-4 are 0 at one
+ * rank).  phase 0: host seconds of each level's setup step (level l's
+ * extend_hierarchy, the coarsest level's dense factorisation); phase 1:
+ * device seconds of the V-cycle work done on each level since tracking was
+ * switched on.  times holds 5 * num_levels entries. */
+int ramd_hier_level_times(ramd_hier h, int phase, double* times);
+/* Switches the per-level solve timing on (clearing the buckets) or off.
+ * Replaces setting ParMultilevel::track_times before a solve. */
+int ramd_hier_track_times(ramd_hier h, int on);
+/* Size of the coarsest (dense) system and dgetrf_'s info for it (> 0: the
+ * info-th pivot is exactly zero).  As in the reference
+ * (multilevel/par_multilevel.hpp:330), a zero pivot does not fail the setup;
+ * a cycle then divides by it and the solvers report RAMD_E_NUMERIC. */
+int ramd_hier_coarse_info(ramd_hier h, int32_t* n_coarse, int32_t* info);
 /* Writes the hierarchy (options, every level's A in its stored order, P,
  * the splitting or aggregates, the coarse LU factors and pivots) to a file,
  * and rebuilds a device hierarchy from one without redoing the setup; the
@@ -293,6 +310,13 @@ int ramd_amg_solve(ramd_hier h, double* x_d, const double* b_d, double tol,
  * Returns RAMD_E_INDEFINITE where the reference calls exit(-1). */
 int ramd_pcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double tol,
              int max_iter, double* res, int32_t max_res, int32_t* n_res, int32_t* iters);
+/* The same PCG with the reference's two timing outputs (PCG's precond_t and
+ * comm_t, krylov/par_cg.hpp:14-16, krylov/par_cg.cpp:150-228): when not
+ * NULL, *precond_t gets the device seconds of the V-cycles added and
+ * *comm_t those of the inner products (the reference's Allreduce time). */
+int ramd_pcg_timed(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double tol,
+                   int max_iter, double* res, int32_t max_res, int32_t* n_res, int32_t* iters,
+                   double* precond_t, double* comm_t);
 
 /* CG(A, x, b, res, tol, max_iter), krylov/par_cg.cpp:8-118: unpreconditioned
  * CG with the reference's rules (tol *= ||r0||, res[k] = ||r_k|| / ||b||,

@@ -359,6 +359,24 @@ class ParMultilevel:
     def enable_fp32(self):
         check(lib().ramd_hier_enable_fp32(self.h))
 
+    def set_track_times(self, on=True):
+        """ParMultilevel::track_times: per-level device time of the cycle
+        work from now on (clears the solve buckets)."""
+        check(lib().ramd_hier_track_times(self.h, 1 if on else 0))
+
+    def level_times(self, phase):
+        """setup_times (phase 0) / solve_times (phase 1) in the reference's
+        layout: array (levels, 5), column 0 the level's total seconds."""
+        t = np.zeros(5 * self.num_levels)
+        check(lib().ramd_hier_level_times(self.h, phase, _ptr(t, C.c_double)))
+        return t.reshape(-1, 5)
+
+    def coarse_info(self):
+        """(rows of the dense coarse system, dgetrf info: > 0 for a zero pivot)."""
+        n, info = C.c_int32(), C.c_int32()
+        check(lib().ramd_hier_coarse_info(self.h, C.byref(n), C.byref(info)))
+        return n.value, info.value
+
     def cycle(self, x, b, x_is_zero=False):
         check(lib().ramd_vcycle(self.h, x.ptr, b.ptr, 1 if x_is_zero else 0))
 
@@ -428,13 +446,23 @@ class ParSmoothedAggregationSolver(ParMultilevel):
         o.prolong_weight = self.prolong_weight
         return o
 
-def PCG(A, ml, x, b, tol=1e-5, max_iter=-1, max_res=10000):
-    """PCG(A, ml, x, b, res, tol, max_iter), krylov/par_cg.cpp:121-263.
-    Returns (residual history, iteration count)."""
+def PCG(A, ml, x, b, tol=1e-5, max_iter=-1, max_res=10000, times=None):
+    """PCG(A, ml, x, b, res, tol, max_iter, precond_t, comm_t),
+    krylov/par_cg.cpp:121-263.  Returns (residual history, iteration count).
+    times, when given, is a float64 array of two entries to which the
+    device seconds of the V-cycles (precond_t) and of the inner products
+    (comm_t) are added."""
     res = np.zeros(max_res)
     nres, it = C.c_int32(), C.c_int32()
-    check(lib().ramd_pcg(ml.h, A.h, x.ptr, b.ptr, float(tol), int(max_iter), _ptr(res, C.c_double), max_res,
-                         C.byref(nres), C.byref(it)))
+    if times is None:
+        check(lib().ramd_pcg(ml.h, A.h, x.ptr, b.ptr, float(tol), int(max_iter), _ptr(res, C.c_double), max_res,
+                             C.byref(nres), C.byref(it)))
+    else:
+        assert times.dtype == np.float64 and times.size >= 2 and times.flags["C_CONTIGUOUS"]
+        pt, ct = C.c_double(times[0]), C.c_double(times[1])
+        check(lib().ramd_pcg_timed(ml.h, A.h, x.ptr, b.ptr, float(tol), int(max_iter), _ptr(res, C.c_double),
+                                   max_res, C.byref(nres), C.byref(it), C.byref(pt), C.byref(ct)))
+        times[0], times[1] = pt.value, ct.value
     return res[:min(nres.value, max_res)], it.value
 
 

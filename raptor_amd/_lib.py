@@ -117,6 +117,10 @@ _SIGS = {
     "ramd_vcycle_fp32": (P, P, P),
     "ramd_amg_solve": (P, P, P, D, C.c_int, PD, PI32),
     "ramd_pcg": (P, P, P, P, D, C.c_int, PD, I32, PI32, PI32),
+    "ramd_pcg_timed": (P, P, P, P, D, C.c_int, PD, I32, PI32, PI32, PD, PD),
+    "ramd_hier_level_times": (P, C.c_int, PD),
+    "ramd_hier_track_times": (P, C.c_int),
+    "ramd_hier_coarse_info": (P, PI32, PI32),
     "ramd_fpcg": (P, P, P, P, D, C.c_int, C.c_int, PD, PI32, PD),
     "ramd_cg": (P, P, P, D, C.c_int, PD, I32, PI32, PI32),
     "ramd_bicgstab": (P, P, P, D, I32, PD, I32, PI32, PI32),

@@ -22,6 +22,64 @@ static double now_s()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+int EventSpans::open(hipStream_t s, int slot)
+{
+    hipEvent_t ev[2];
+    for (hipEvent_t& e : ev) {
+        if (!pool.empty()) {
+            e = pool.back();
+            pool.pop_back();
+        } else {
+            RAMD_HIP(hipEventCreate(&e));
+        }
+    }
+    RAMD_HIP(hipEventRecord(ev[0], s));
+    pending.push_back(Span{ev[0], ev[1], slot});
+    return (int)pending.size() - 1;
+}
+
+void EventSpans::close(hipStream_t s, int idx) { RAMD_HIP(hipEventRecord(pending[(size_t)idx].e, s)); }
+
+void EventSpans::resolve(std::vector<double>& acc)
+{
+    if (pending.empty()) return;
+    RAMD_HIP(hipEventSynchronize(pending.back().e));
+    for (const Span& sp : pending) {
+        float ms = 0;
+        RAMD_HIP(hipEventElapsedTime(&ms, sp.b, sp.e));
+        if (sp.slot >= 0 && (size_t)sp.slot < acc.size()) acc[(size_t)sp.slot] += 1e-3 * ms;
+        pool.push_back(sp.b);
+        pool.push_back(sp.e);
+    }
+    pending.clear();
+}
+
+void EventSpans::release()
+{
+    for (const Span& sp : pending) {
+        (void)hipEventDestroy(sp.b);
+        (void)hipEventDestroy(sp.e);
+    }
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    pending.clear();
+    pool.clear();
+}
+
+// Per-level solve timing: a span on the hierarchy's stream attributed to
+// level l while tracking is on (-1: not tracking).  Long runs are resolved
+// as they go so the pending list stays short.
+static int level_span_open(ramd_hier_s* h, int l)
+{
+    if (!h->track) return -1;
+    if (h->spans.pending.size() >= 4096) h->spans.resolve(h->solve_level_s);
+    return h->spans.open(h->ctx->stream, l);
+}
+
+static void level_span_close(ramd_hier_s* h, int idx)
+{
+    if (idx >= 0) h->spans.close(h->ctx->stream, idx);
+}
+
 // dia = false skips the stencil-diagonal form (matrices that are only
 // multiplied by other sparse matrices).
 static ramd_csr_s* upload(ramd_ctx ctx, const Csr& M, bool dia = true)
@@ -296,8 +354,10 @@ static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_ze
 {
     Level& L = h->levels[l];
     hipStream_t s = h->ctx->stream;
+    int tk = level_span_open(h, l);
     if (l == (int)h->levels.size() - 1) {
         coarse_solve64(h, b, x);
+        level_span_close(h, tk);
         return;
     }
     const double w = h->opts.relax_weight;
@@ -318,7 +378,9 @@ static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_ze
     launch_stream_op(L.A, OP_RESIDUAL, cur, b, L.r, 0.0, s);
     Level& C = h->levels[l + 1];
     launch_stream_op(L.R, OP_SPMV, L.r, nullptr, C.b, 0.0, s);
+    level_span_close(h, tk);
     cycle64(h, l + 1, C.x, C.b, true);
+    tk = level_span_open(h, l);
     // x <- x + P e (CSR_append: row sum first, then added), into the other buffer
     launch_stream_op(L.P, OP_ADD_INTO, C.x, cur, other, 0.0, s);
     std::swap(cur, other);
@@ -328,6 +390,7 @@ static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_ze
         std::swap(cur, other);
     }
     if (cur != x) RAMD_HIP(hipMemcpyAsync(x, cur, sizeof(double) * L.n, hipMemcpyDeviceToDevice, s));
+    level_span_close(h, tk);
 }
 
 // x_zero: x is zero on entry; first_done: x already holds the first Jacobi
@@ -336,6 +399,7 @@ static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero
 {
     Level& L = h->levels[l];
     hipStream_t s = h->ctx->stream;
+    int tk = level_span_open(h, l);
     if (l == (int)h->levels.size() - 1) {
         const int n = h->coarse_n;
         if (n <= kCoarseWave) {
@@ -348,6 +412,7 @@ static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero
             hipLaunchKernelGGL(to_f32, grid1(n), dim3(256), 0, s, (int64_t)n, h->cx, x);
         }
         RAMD_HIP(hipGetLastError());
+        level_span_close(h, tk);
         return;
     }
     const float w = (float)h->opts.relax_weight;
@@ -370,7 +435,9 @@ static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero
     launch_stream_op32(L.A, OP_RESIDUAL, cur, b, L.r32, 0.f, s);
     Level& C = h->levels[l + 1];
     launch_stream_op32(L.R, OP_SPMV, L.r32, nullptr, C.b32, 0.f, s);
+    level_span_close(h, tk);
     cycle32(h, l + 1, C.x32, C.b32, true);
+    tk = level_span_open(h, l);
     launch_stream_op32(L.P, OP_ADD_INTO, C.x32, cur, other, 0.f, s);
     std::swap(cur, other);
     if (mc) mc_relax<float, float>(h, L, L.A->val32, cur, b, false);
@@ -379,6 +446,7 @@ static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero
         std::swap(cur, other);
     }
     if (cur != x) RAMD_HIP(hipMemcpyAsync(x, cur, sizeof(float) * L.n, hipMemcpyDeviceToDevice, s));
+    level_span_close(h, tk);
 }
 
 // ---------------------------------------------------------------------------
@@ -414,6 +482,7 @@ static void hier_free(ramd_hier_s* h)
     if (!h) return;
     if (h->ctx) (void)hipStreamSynchronize(h->ctx->stream);
     block_free(h->blk);
+    h->spans.release();
     for (Level& L : h->levels) free_level(L);
     for (void* p : {(void*)h->lu, (void*)h->piv, (void*)h->cb, (void*)h->cx, (void*)h->pr, (void*)h->pz, (void*)h->pp,
                     (void*)h->pAp, (void*)h->pr32, (void*)h->pz32})
@@ -643,6 +712,7 @@ static void hier_setup(ramd_hier_s* h, Csr Af)
     const ramd_amg_opts& o = h->opts;
     ramd_ctx ctx = h->ctx;
     double tstart = now_s();
+    double tlevel = tstart;
     // levels[0]->A = Af->copy(); sort(); move_diag()  (par_multilevel.hpp:141-143)
     host::sort_rows(Af);
     host::move_diag(Af);
@@ -681,6 +751,10 @@ static void hier_setup(ramd_hier_s* h, Csr Af)
         h->levels.back().A = Ac;
         alloc_level_vectors(h, h->levels.back());
         host_A = false;
+        RAMD_HIP(hipStreamSynchronize(ctx->stream));
+        const double tl = now_s();
+        h->setup_level_s.push_back(tl - tlevel);
+        tlevel = tl;
     }
     if (!host_A) A = download(h->levels.back().A);
     // duplicate_coarse (par_multilevel.hpp:229-339): dense row-major A_c, dgetrf_
@@ -691,10 +765,15 @@ static void hier_setup(ramd_hier_s* h, Csr Af)
     for (int i = 0; i < n; i++)
         for (int j = A.rowptr[i]; j < A.rowptr[i + 1]; j++) dense[(size_t)i * n + A.col[j]] = A.val[j];
     std::vector<int> piv;
-    const int info = lu_factor_colmajor(n, dense, piv);
-    RAMD_CHECK(info == 0, RAMD_E_NUMERIC, "coarse matrix (%d rows) is singular (info=%d)", n, info);
+    // as the reference, a zero pivot does not stop the setup (dgetrf_'s info
+    // is not inspected, multilevel/par_multilevel.hpp:330): the factors are
+    // kept and a cycle then divides by the zero pivot.  The info is kept for
+    // ramd_hier_coarse_info.
+    h->coarse_info = lu_factor_colmajor(n, dense, piv);
     finish_hierarchy(h, n, dense, piv);
-    double total = now_s() - tstart;
+    const double tend = now_s();
+    h->setup_level_s.push_back(tend - tlevel);
+    double total = tend - tstart;
     h->times[4] = total - h->times[0] - h->times[1] - h->times[2] - h->times[3];
 }
 
@@ -886,19 +965,15 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
     const int n = A->n_rows;
     RAMD_CHECK(n == h->levels[0].n, RAMD_E_ARG, "ramd_fpcg: A has %d rows, hierarchy %d", n, h->levels[0].n);
     const bool f32 = cycle_prec == RAMD_PREC_FP32;
-    // FP32 form: the outer FP64 operator reads FP32-stored values where
-    // that is lossless (same FP64 results, half the value bytes)
-    auto outer = [f32](const ramd_csr_s* M, int op, const double* x, const double* b, double* y, double w,
-                       hipStream_t st) {
-        if (f32) launch_stream_op_c(M, op, x, b, y, w, st);
-        else launch_stream_op(M, op, x, b, y, w, st);
-    };
-    if (f32) {
-        enable_fp32(h);
+    // Both forms: the outer FP64 operator reads FP32-stored values where
+    // every value is exactly representable in FP32 (same FP64 arithmetic and
+    // results, half the value bytes); otherwise the FP64 values
+    auto outer = [](const ramd_csr_s* M, int op, const double* x, const double* b, double* y, double w,
+                    hipStream_t st) { launch_stream_op_c(M, op, x, b, y, w, st); };
+    if (f32) enable_fp32(h);
+    if (A->dia) {
         csr_make_fp32(A);
         dia_autotune(A, 2);  // the outer operator reads FP32 values into FP64 vectors
-    } else {
-        dia_autotune(A, 0);
     }
     double *r = h->pr, *z = h->pz, *p = h->pp, *Ap = h->pAp;
     float *r32 = h->pr32, *z32 = h->pz32;
@@ -948,7 +1023,7 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
         while (it < max_iter) {
             // Ap = A p and p.Ap in one pass over A; then r -= alpha Ap,
             // ||r||^2 (and r32) in one pass; one host read
-            launch_spmv_dot(A, f32, p, Ap, sc, s);
+            launch_spmv_dot(A, true, p, Ap, sc, s);
             // (the partials buffer may have grown inside launch_spmv_dot)
             double* part = ctx_partials(ctx, (size_t)g);
             hipLaunchKernelGGL(fpcg_update_norm, dim3(g), dim3(kBlockThreads), 0, s, n, sc + cur, sc, sc + 4, Ap, r,
@@ -1206,6 +1281,10 @@ static void hier_load(ramd_hier_s* h, const char* path)
     RAMD_CHECK((int64_t)lu.size() == (int64_t)n * n && (int)piv.size() == n, RAMD_E_ARG, "%s: corrupt coarse LU",
                path);
     for (int p : piv) RAMD_CHECK(p >= 0 && p < n, RAMD_E_ARG, "%s: corrupt pivots", path);
+    // dgetrf's info: the first zero pivot of U (1-based), 0 when none
+    h->coarse_info = 0;
+    for (int j = 0; j < n && !h->coarse_info; j++)
+        if (lu[(size_t)j + (size_t)j * n] == 0.0) h->coarse_info = j + 1;
     finish_hierarchy(h, n, lu, piv);
 }
 
@@ -1412,6 +1491,43 @@ int ramd_hier_setup_times(ramd_hier h, double* t)
     });
 }
 
+int ramd_hier_coarse_info(ramd_hier h, int32_t* n_coarse, int32_t* info)
+{
+    return guarded([&] {
+        RAMD_CHECK(h && info, RAMD_E_ARG, "ramd_hier_coarse_info: NULL argument");
+        if (n_coarse) *n_coarse = h->coarse_n;
+        *info = h->coarse_info;
+    });
+}
+
+int ramd_hier_track_times(ramd_hier h, int on)
+{
+    return guarded([&] {
+        RAMD_CHECK(h, RAMD_E_ARG, "ramd_hier_track_times: NULL handle");
+        h->ctx->set_current();
+        h->spans.resolve(h->solve_level_s);
+        h->track = on != 0;
+        h->solve_level_s.assign(h->levels.size(), 0.0);
+    });
+}
+
+int ramd_hier_level_times(ramd_hier h, int phase, double* times)
+{
+    return guarded([&] {
+        RAMD_CHECK(h && times && (phase == 0 || phase == 1), RAMD_E_ARG, "ramd_hier_level_times: bad argument");
+        h->ctx->set_current();
+        const size_t nl = h->levels.size();
+        for (size_t i = 0; i < 5 * nl; i++) times[i] = 0.0;
+        if (phase == 0) {
+            for (size_t l = 0; l < nl && l < h->setup_level_s.size(); l++) times[5 * l] = h->setup_level_s[l];
+        } else {
+            h->solve_level_s.resize(nl, 0.0);
+            h->spans.resolve(h->solve_level_s);
+            for (size_t l = 0; l < nl; l++) times[5 * l] = h->solve_level_s[l];
+        }
+    });
+}
+
 int ramd_hier_enable_fp32(ramd_hier h)
 {
     return guarded([&] {
@@ -1472,9 +1588,11 @@ int ramd_amg_solve(ramd_hier h, double* x_d, const double* b_d, double tol, int 
     });
 }
 
-// PCG krylov/par_cg.cpp:121-263
-int ramd_pcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double tol, int max_iter, double* res,
-             int32_t max_res, int32_t* n_res, int32_t* iters)
+// PCG krylov/par_cg.cpp:121-263.  precond_t / comm_t (may be NULL) get the
+// device seconds of the V-cycles and of the inner products added, the
+// buckets the reference times with MPI_Wtime (par_cg.cpp:150-228).
+static int pcg_run(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double tol, int max_iter, double* res,
+                   int32_t max_res, int32_t* n_res, int32_t* iters, double* precond_t, double* comm_t)
 {
     int rc = RAMD_OK;
     int st = guarded([&] {
@@ -1491,52 +1609,90 @@ int ramd_pcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double tol
             if (nres < max_res) res[nres] = v;
             nres++;
         };
-        if (max_iter <= 0) max_iter = ((int)(1.3 * n)) + 2;
-        vcycle_zero(h, z, b_d);
-        const double b_inner = dot64(ctx, n, b_d, z);
-        const double norm_b = sqrt(b_inner);
-        if (norm_b > 1e-16) tol = tol * norm_b;
-        launch_stream_op(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
-        vcycle_zero(h, z, r);
-        RAMD_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        double rz = dot64(ctx, n, r, z);
-        push(sqrt(rz));
-        int iter = 0;
-        const int recompute_r = 8;
-        while (iter < max_iter) {
-            iter++;
-            launch_stream_op(A, OP_SPMV, p, nullptr, Ap, 0.0, s);
-            const double App = dot64(ctx, n, Ap, p);
-            if (App < 0.0) {
-                rc = RAMD_E_INDEFINITE;
-                break;
-            }
-            const double alpha = rz / App;
-            const bool full_r = recompute_r && iter % recompute_r == 0;
-            hipLaunchKernelGGL(pcg_update_xr, grid1(n), dim3(256), 0, s, n, alpha, p, Ap, x_d, r, full_r ? 0 : 1);
-            RAMD_HIP(hipGetLastError());
-            if (full_r) launch_stream_op(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
-            vcycle_zero(h, z, r);
-            const double next = dot64(ctx, n, r, z);
-            const double beta = next / rz;
-            push(next / b_inner);
-            if (next < tol) break;
-            if (full_r) {
-                RAMD_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-            } else {
-                hipLaunchKernelGGL(pcg_update_p, grid1(n), dim3(256), 0, s, n, beta, z, p);
+        // buckets: [0] preconditioner, [1] inner products
+        const bool timed = precond_t || comm_t;
+        EventSpans spans;
+        std::vector<double> acc(2, 0.0);
+        auto precondition = [&](double* zz, const double* rr) {
+            const int t = timed ? spans.open(s, 0) : -1;
+            vcycle_zero(h, zz, rr);
+            if (t >= 0) spans.close(s, t);
+        };
+        auto inner = [&](const double* a, const double* b) {
+            const int t = timed ? spans.open(s, 1) : -1;
+            launch_dot(ctx, n, a, b, ctx->red_result);
+            if (t >= 0) spans.close(s, t);
+            return fetch_scalar(ctx, ctx->red_result);
+        };
+        try {
+            if (max_iter <= 0) max_iter = ((int)(1.3 * n)) + 2;
+            precondition(z, b_d);
+            const double b_inner = inner(b_d, z);
+            const double norm_b = sqrt(b_inner);
+            if (norm_b > 1e-16) tol = tol * norm_b;
+            launch_stream_op(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
+            precondition(z, r);
+            RAMD_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            double rz = inner(r, z);
+            push(sqrt(rz));
+            int iter = 0;
+            const int recompute_r = 8;
+            while (iter < max_iter) {
+                iter++;
+                launch_stream_op(A, OP_SPMV, p, nullptr, Ap, 0.0, s);
+                const double App = inner(Ap, p);
+                if (App < 0.0) {
+                    rc = RAMD_E_INDEFINITE;
+                    break;
+                }
+                const double alpha = rz / App;
+                const bool full_r = recompute_r && iter % recompute_r == 0;
+                hipLaunchKernelGGL(pcg_update_xr, grid1(n), dim3(256), 0, s, n, alpha, p, Ap, x_d, r,
+                                   full_r ? 0 : 1);
                 RAMD_HIP(hipGetLastError());
+                if (full_r) launch_stream_op(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
+                precondition(z, r);
+                const double next = inner(r, z);
+                const double beta = next / rz;
+                push(next / b_inner);
+                if (next < tol) break;
+                if (full_r) {
+                    RAMD_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+                } else {
+                    hipLaunchKernelGGL(pcg_update_p, grid1(n), dim3(256), 0, s, n, beta, z, p);
+                    RAMD_HIP(hipGetLastError());
+                }
+                rz = next;
             }
-            rz = next;
+            RAMD_HIP(hipStreamSynchronize(s));
+            *n_res = nres;
+            *iters = iter;
+            if (timed) {
+                spans.resolve(acc);
+                if (precond_t) *precond_t += acc[0];
+                if (comm_t) *comm_t += acc[1];
+            }
+        } catch (...) {
+            spans.release();
+            throw;
         }
-        RAMD_HIP(hipStreamSynchronize(s));
-        *n_res = nres;
-        *iters = iter;
-        if (rc == RAMD_OK && iter == max_iter) rc = RAMD_OK;  // reference prints "Max Iterations Reached"
+        spans.release();
     });
     if (st != RAMD_OK) return st;
     if (rc == RAMD_E_INDEFINITE) set_error("ramd_pcg: indefinite matrix detected (p.Ap < 0)");
     return rc;
+}
+
+int ramd_pcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double tol, int max_iter, double* res,
+             int32_t max_res, int32_t* n_res, int32_t* iters)
+{
+    return pcg_run(h, A, x_d, b_d, tol, max_iter, res, max_res, n_res, iters, nullptr, nullptr);
+}
+
+int ramd_pcg_timed(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double tol, int max_iter, double* res,
+                   int32_t max_res, int32_t* n_res, int32_t* iters, double* precond_t, double* comm_t)
+{
+    return pcg_run(h, A, x_d, b_d, tol, max_iter, res, max_res, n_res, iters, precond_t, comm_t);
 }
 
 int ramd_cg(ramd_csr A, double* x_d, const double* b_d, double tol, int max_iter, double* res, int32_t max_res,

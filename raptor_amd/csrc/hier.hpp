@@ -89,6 +89,24 @@ void vcycle_from_zero(ramd_hier_s* h, double* x, const double* b);
 struct BlockState;  // multi-vector work space (block.hip)
 void block_free(BlockState* b);
 
+// Device-time buckets: pairs of hipEvents recorded on a stream around a
+// piece of work, summed per slot when resolved.  Used for the per-level
+// solve times (ParMultilevel::track_times) and PCG's precond_t / comm_t.
+struct EventSpans {
+    struct Span {
+        hipEvent_t b, e;
+        int slot;
+    };
+    std::vector<Span> pending;
+    std::vector<hipEvent_t> pool;
+    // records the start event of a span for `slot`; returns its index
+    int open(hipStream_t s, int slot);
+    void close(hipStream_t s, int idx);
+    // waits for the last recorded event and adds every span's seconds to acc[slot]
+    void resolve(std::vector<double>& acc);
+    void release();
+};
+
 }  // namespace ramd
 
 struct ramd_hier_s {
@@ -96,11 +114,21 @@ struct ramd_hier_s {
     ramd_amg_opts opts{};
     std::vector<ramd::Level> levels;
     int coarse_n = 0;
+    int coarse_info = 0;   // dgetrf's info for the coarse matrix (> 0: zero pivot)
     double* lu = nullptr;  // column-major LU of the coarse matrix read column-major (A_c^T)
     int* piv = nullptr;
     double* cb = nullptr;  // FP64 staging for the FP32 cycle's coarse solve
     double* cx = nullptr;
     double times[5] = {0, 0, 0, 0, 0};
+    // host seconds of each level's setup step (the level's extend_hierarchy;
+    // the coarsest level gets the dense factorisation), as the reference's
+    // setup_times[5 * l] (multilevel/par_multilevel.hpp:171-208)
+    std::vector<double> setup_level_s;
+    // per-level device seconds of the cycle's work on that level while
+    // tracking is on (solve_times[5 * l], multilevel/par_multilevel.hpp:343-456)
+    bool track = false;
+    std::vector<double> solve_level_s;
+    ramd::EventSpans spans;
     bool fp32 = false;
     // PCG work vectors (fine level)
     double *pr = nullptr, *pz = nullptr, *pp = nullptr, *pAp = nullptr;
