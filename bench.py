@@ -10,9 +10,12 @@ V-cycle in FP32 (the headline `value`, seconds, lower is better); the same
 solve with the V-cycle in FP64 is timed beside it.  `--config c2` runs the
 128^3 grid with b = A x_true, x_true ~ U(-1,1) (seed 1).
 
-Also reported: live HIP-event timing of the dominant kernel (roofline) and
-the oracle's single-core CPU time for one V-cycle + one SpMV on the same
-hierarchy, scaled to the solve's iteration count (cpu_baseline).
+Also reported: live HIP-event timing of the dominant kernel (roofline); the
+per-level timing of the setup's two sparse products (galerkin); and, outside
+every timed GPU region, the oracle's whole solve of the same system on the
+same hierarchy on the host's cores (cpu_baseline, with the one-core figure
+beside it) and the error of the GPU solutions against the oracle's FP64 solve
+to 1e-12 (oracle_fp64_solve).
 """
 import argparse
 import json
@@ -76,6 +79,10 @@ def main():
     ap.add_argument("--m", type=int, default=0, help="override the grid edge")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-cycle comparison solves")
+    ap.add_argument("--no-galerkin", action="store_true", help="skip the per-level SpGEMM timing")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads of the CPU baseline (default: OMP_NUM_THREADS or the CPUs this "
+                         "process may use, at most 16)")
     ap.add_argument("--no-graph", action="store_true", help="c4: launch the block iteration eagerly (no HIP graphs)")
     ap.add_argument("--calibrate", action="store_true",
                     help="end with one axpy over 2^27 doubles (2 GiB read, 1 GiB written), the reference "
@@ -177,10 +184,11 @@ def main():
     err32 = float(np.linalg.norm(x32_h - xref) / np.linalg.norm(xref))
     log(f"FP32-cycle solution vs FP64 reference (1e-12): relative error {err32:.3e}")
 
-    # ---- FP64-cycle solves (same protocol)
+    # ---- FP64-cycle solves (same protocol, same outer operator)
     t64 = None
     its64 = []
     rel64 = None
+    x64_h = None
     if not args.no_fp64:
         ctx.sync()
         t0 = time.perf_counter()
@@ -189,6 +197,7 @@ def main():
             its64.append(it)
         ctx.sync()
         t64 = (time.perf_counter() - t0) / args.steps
+        x64_h = x.numpy()
         log(f"FP64-cycle solve: {t64:.4f}s, iterations {its64}, relres {rel64:.3e}")
 
     # ---- V-cycle throughput (FP32 and FP64), 10 cycles each
@@ -240,11 +249,18 @@ def main():
                     us_per_launch=round(per_launch_s * 1e6, 2))
     log(f"dominant kernel {dom_name}: {achieved:.0f} GB/s ({achieved / PEAK_HBM_GBS:.1%} of 8 TB/s)")
 
-    # ---- CPU baseline: the oracle (plain C, one core) on the same hierarchy
+    # ---- Galerkin product per level (the setup's two SpGEMMs, timed alone)
+    galerkin = None
+    if not args.no_galerkin:
+        galerkin = galerkin_report(ctx, ml)
+
+    # ---- CPU baseline and FP64-oracle error: the oracle (plain C, every
+    # host core) solves the same system on the same hierarchy
     cpu = None
+    oracle = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(ml, rp, col, val, b_h, max(its64 or its32))
+            cpu, oracle = cpu_baseline(ml, rp, col, val, b_h, x32_h, x64_h)
         except Exception as e:  # the checker is optional on machines without it
             log(f"cpu baseline skipped: {e}")
 
@@ -264,15 +280,20 @@ def main():
         "config": dict(workload=f"27pt-laplacian-{m}^3 RS(PMIS, extended, theta=0.25) Jacobi(2/3) FPCG to 1e-8",
                        n=n, nnz=int(col.size), rhs=cfg["rhs"], seed=cfg["seed"], levels=list(map(int, nl))),
         "fp64_cycle_s": round(t64, 5) if t64 else None,
+        # both forms run the same outer FP64 operator (FP32-stored values
+        # where they are exact) and the same vector updates and dots
         "fp32_speedup_vs_fp64": round(t64 / t32, 3) if t64 else None,
+        "outer_operator": outer_operator(A),
         "iterations_fp32": its32,
         "iterations_fp64": its64,
         "relres_fp32": rel32,
         "relres_fp64": rel64,
         "error_fp32_vs_fp64_reference": err32,
+        "oracle_fp64_solve": oracle,
         "pcg_reference_rule": pcg_ref,
         "setup_s": round(setup_s, 2),
         "vcycle": vc,
+        "galerkin": galerkin,
         "kernels": prof,
         "row_caps": caps.tolist(),
         "roofline": roofline,
@@ -598,37 +619,153 @@ def bench_adjoint(args, m, cfg):
     ctx.close()
 
 
-def cpu_baseline(ml, rp, col, val, b_h, iters):
-    """Oracle (oracle/oracle.c, one core) timing of one V-cycle plus one SpMV
-    on the product's own hierarchy, scaled to a full solve of `iters`
-    iterations (+2 for the initial cycles)."""
+def outer_operator(A):
+    """How the FPCG's outer FP64 operator reads A's values (both forms)."""
+    return "stencil-diagonal, FP32-stored exact values, FP64 arithmetic" if A.diagonals() else \
+        "CSR, FP64 values"
+
+
+def host_cpu():
+    """(model name, CPUs this process may use)"""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def cpu_threads(args_threads):
+    if args_threads > 0:
+        return args_threads
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return env if env > 0 else min(16, host_cpu()[1])
+
+
+def cpu_baseline(ml, rp, col, val, b_h, x32_h, x64_h, threads=0):
+    """The oracle on the GPU box's host cores, on the product's own
+    hierarchy (outside every timed GPU region):
+
+      - cpu_baseline: one whole solve of the same system to a true relative
+        residual of 1e-8 by oracle/oracle_mt.c (the oracle's V-cycle,
+        bit-identical to orc_cycle, row-parallel over `threads` OpenMP
+        threads, inside the FP64 flexible PCG), timed; beside it the one-core
+        figure: one V-cycle + one SpMV on one thread, scaled by the
+        iteration count;
+      - oracle_fp64_solve: the same oracle solve run to 1e-12, and the
+        relative 2-norm error of the GPU's FP32-cycle and FP64-cycle
+        solutions against it."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import checkers as K
-    As, Ps = [], []
-    for l in range(ml.num_levels):
-        Al, Pl = ml.level(l)
-        As.append(K.Csr(*Al.to_csr(), Al.n_cols))
-        if Pl is not None:
-            Ps.append(K.Csr(*Pl.to_csr(), Pl.n_cols))
+    threads = cpu_threads(threads)
+    model, ncpu = host_cpu()
+    As, Ps = _levels_of(ml)
     H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=2.0 / 3.0)
     A = K.Csr(rp, col, val)
-    reps, tc, ts = 0, 0.0, 0.0
-    t_start = time.perf_counter()
-    while reps < 3 and (reps == 0 or time.perf_counter() - t_start < 20.0):
-        t = time.perf_counter()
-        H.cycle(np.zeros(A.n_rows), b_h)
-        tc += time.perf_counter() - t
-        t = time.perf_counter()
-        K.spmv(A, b_h)
-        ts += time.perf_counter() - t
-        reps += 1
-    per_iter = (tc + ts) / reps
-    est = per_iter * (iters + 2)
-    log(f"cpu baseline: {per_iter:.2f}s per iteration (1 core) -> {est:.1f}s per solve")
-    return dict(value=round(est, 2), unit="s", cores=1, kind="port",
-                sample=f"{reps} x (one FP64 V-cycle + one SpMV) of the oracle on the full hierarchy, "
-                       f"{per_iter:.3f}s each, scaled by {iters}+2 iterations")
+    n = A.n_rows
+    M = K.MtHier(H, threads)
+    used = M.threads
+    t = time.perf_counter()
+    x8, it8, rel8 = M.fpcg(A, np.zeros(n), b_h, 1e-8, 500)
+    t_solve = time.perf_counter() - t
+    log(f"cpu baseline: oracle FPCG on {used} threads ({model}): {t_solve:.2f}s, {it8} iterations, "
+        f"true relres {rel8:.3e}")
+    t = time.perf_counter()
+    xo, ito, relo = M.fpcg(A, np.zeros(n), b_h, 1e-12, 500)
+    t_ref = time.perf_counter() - t
+    M.free()
+    oracle = dict(rtol=1e-12, iterations=ito, true_relres=relo, s=round(t_ref, 2), threads=used,
+                  error_fp32_cycle=float(np.linalg.norm(x32_h - xo) / np.linalg.norm(xo)),
+                  error_fp64_cycle=(float(np.linalg.norm(x64_h - xo) / np.linalg.norm(xo))
+                                    if x64_h is not None else None),
+                  iterations_to_1e8=it8)
+    log(f"oracle FP64 solve to 1e-12 ({ito} iterations, {t_ref:.1f}s): FP32-cycle solution error "
+        f"{oracle['error_fp32_cycle']:.3e}, FP64-cycle {oracle['error_fp64_cycle']}")
+    # one-core figure (one cycle + one SpMV, scaled)
+    M1 = K.MtHier(H, 1)
+    t = time.perf_counter()
+    M1.cycle(np.zeros(n), b_h)
+    K.spmv(A, b_h)
+    per_iter = time.perf_counter() - t
+    M1.free()
+    one_core = per_iter * (it8 + 1)
+    log(f"cpu baseline, one core: {per_iter:.2f}s per iteration -> {one_core:.1f}s per solve")
+    cpu = dict(value=round(t_solve, 3), unit="s", cores=used, kind="port", cpu_model=model, host_cpus=ncpu,
+               sample=f"one whole solve of the same 27-point system to a true relative residual of 1e-8 "
+                      f"({it8} iterations of the oracle's FP64 V-cycle in flexible PCG, oracle/oracle_mt.c, "
+                      f"{used} OpenMP threads) on the product's hierarchy",
+               single_core_s=round(one_core, 2),
+               single_core_sample="one FP64 V-cycle + one SpMV of the oracle on one thread, "
+                                  f"{per_iter:.3f}s, scaled by {it8}+1 iterations")
+    return cpu, oracle
+
+
+def galerkin_report(ctx, ml):
+    """Per level l: A_l P_l and P_l^T (A_l P_l) by the device SpGEMMs, each
+    timed alone (host clock around the call, device synchronised), with the
+    algorithmic bytes |A| + |B| + |C| (CSR: 4 (rows + 1) + 12 nnz) of the
+    numeric phase plus the integer bytes (4 (rows + 1) + 4 nnz) of the
+    symbolic phase, the flops 2 x (intermediate products) and the
+    intermediate-product count."""
+    import ctypes as C
+
+    import numpy as np
+
+    from raptor_amd import _lib
+    L = _lib.lib()
+
+    def rowptr(M):
+        rp = np.empty(M.n_rows + 1, np.int32)
+        _lib.check(L.ramd_csr_download(M.h, rp.ctypes.data_as(C.POINTER(C.c_int32)), None, None))
+        return rp
+
+    def cols(M):
+        rp = np.empty(M.n_rows + 1, np.int32)
+        c = np.empty(max(M.nnz, 1), np.int32)
+        _lib.check(L.ramd_csr_download(M.h, rp.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       c.ctypes.data_as(C.POINTER(C.c_int32)), None))
+        return c[:M.nnz]
+
+    def csr_bytes(rows, nnz):
+        return 4 * (rows + 1) + 12 * nnz, 4 * (rows + 1) + 4 * nnz
+
+    out = []
+    total_s = 0.0
+    for l in range(ml.num_levels - 1):
+        A, P = ml.level(l)
+        ctx.sync()
+        t0 = time.perf_counter()
+        AP = A.mult(P)
+        ctx.sync()
+        t1 = time.perf_counter()
+        Ac = AP.mult_T(P)
+        ctx.sync()
+        t2 = time.perf_counter()
+        lp = np.diff(rowptr(P)).astype(np.int64)
+        lap = np.diff(rowptr(AP)).astype(np.int64)
+        prod_ap = int(lp[cols(A)].sum())
+        prod_rap = int((lp * lap).sum())
+        phases = []
+        for name, (ra_, na_), (rb_, nb_), (rc_, nc_), prods, dt in (
+                ("A*P", (A.n_rows, A.nnz), (P.n_rows, P.nnz), (AP.n_rows, AP.nnz), prod_ap, t1 - t0),
+                ("P^T*(AP)", (P.n_rows, P.nnz), (AP.n_rows, AP.nnz), (Ac.n_rows, Ac.nnz), prod_rap, t2 - t1)):
+            num = sum(csr_bytes(r, z)[0] for r, z in ((ra_, na_), (rb_, nb_), (rc_, nc_)))
+            sym = sum(csr_bytes(r, z)[1] for r, z in ((ra_, na_), (rb_, nb_), (rc_, nc_)))
+            phases.append(dict(product=name, s=round(dt, 5), bytes=num + sym, flops=2 * prods,
+                               products=prods, nnz_out=int(nc_),
+                               gbs=round((num + sym) / dt / 1e9, 1), gflops=round(2 * prods / dt / 1e9, 2)))
+        total_s += t2 - t0
+        out.append(dict(level=l, phases=phases))
+        AP.free()
+        Ac.free()
+    log("galerkin per level: " + "; ".join(
+        f"{d['level']}: " + ", ".join(f"{p['product']} {p['s'] * 1e3:.1f} ms {p['gbs']} GB/s" for p in d["phases"])
+        for d in out))
+    return dict(levels=out, total_s=round(total_s, 4))
 
 
 if __name__ == "__main__":

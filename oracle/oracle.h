@@ -97,6 +97,17 @@ int  orc_pre_bicgstab(const orc_csr* A, const orc_hier* h, double* x, const doub
 int  orc_cg(const orc_csr* A, double* x, const double* b, double tol, int max_iter,
             double* res, int max_res, int* n_res);
 
+/* ---- oracle_mt.c: the cycle and a flexible PCG on every host core
+ *      (OpenMP, row-parallel; cycle results bit-identical to orc_cycle) ---- */
+typedef struct orc_mt_hier orc_mt_hier;
+void orc_mt_set_threads(int n);
+int  orc_mt_threads(void);
+orc_mt_hier* orc_mt_prepare(const orc_hier* h);
+void orc_mt_free(orc_mt_hier* m);
+int  orc_mt_cycle(orc_mt_hier* m, double* x, const double* b);
+int  orc_mt_fpcg(const orc_csr* A, orc_mt_hier* m, double* x, const double* b, double rtol, int max_iter,
+                 double* relres);
+
 #ifdef __cplusplus
 }
 #endif

@@ -859,20 +859,34 @@ struct BlockState {
     double* partials = nullptr;  // 2 * kColBlocks * kMaxRhs
     double* pinned = nullptr;    // host, 2 + 2 * kMaxRhs
     BScal S{};
-    // captured iteration (operator step, preconditioner step) for one (A, precision)
+    // captured iteration (operator step, preconditioner step) for one (A,
+    // precision, rtol); the graphs hold raw pointers to A's buffers and to
+    // every level operator's, so they are keyed on A's identity and on the
+    // buffer counts of all of them (a handle freed and another created at the
+    // same address has a new uid)
     hipGraphExec_t gA = nullptr, gB = nullptr;
-    const ramd_csr_s* g_A = nullptr;
-    const void* g_sig[4] = {nullptr, nullptr, nullptr, nullptr};  // buffers of the captured operator
+    uint64_t g_uid = 0;
+    uint64_t g_forms = 0;
     int g_prec = -1;
     double g_rtol = -1;
 };
+
+// buffer counts of every operator a captured iteration launches on
+static uint64_t forms_key(const ramd_hier_s* h, const ramd_csr_s* A)
+{
+    uint64_t k = A->forms;
+    for (const Level& L : h->levels)
+        for (const ramd_csr_s* M : {(const ramd_csr_s*)L.A, (const ramd_csr_s*)L.P, (const ramd_csr_s*)L.R})
+            if (M) k = k * 1000003ull + M->uid * 31ull + M->forms;
+    return k;
+}
 
 static void free_graphs(BlockState* b)
 {
     if (b->gA) (void)hipGraphExecDestroy(b->gA);
     if (b->gB) (void)hipGraphExecDestroy(b->gB);
     b->gA = b->gB = nullptr;
-    b->g_A = nullptr;
+    b->g_uid = 0;
 }
 
 void block_free(BlockState* b)
@@ -1218,14 +1232,14 @@ static int fpcg_block_run(ramd_hier_s* h, ramd_csr_s* A, int k, double* X_d, con
             else hipLaunchKernelGGL((bdir<double>), dim3(gc), dim3(256), 0, s, n, k, kp, S.beta, bs->Z, bs->P);
             RAMD_HIP(hipGetLastError());
         };
-        const void* sig[4] = {A->val, A->col, A->rowptr, A->dia ? (const void*)A->dia->v64 : nullptr};
-        if (use_graph && (bs->g_A != A || std::memcmp(sig, bs->g_sig, sizeof sig) != 0 || bs->g_prec != prec ||
-                          bs->g_rtol != rtol || !bs->gA)) {
+        const uint64_t fk = forms_key(h, A);
+        if (use_graph && (bs->g_uid != A->uid || bs->g_forms != fk || bs->g_prec != prec || bs->g_rtol != rtol ||
+                          !bs->gA)) {
             free_graphs(bs);
             bs->gA = capture(s, stepA);
             bs->gB = capture(s, stepB);
-            bs->g_A = A;
-            std::memcpy(bs->g_sig, sig, sizeof sig);
+            bs->g_uid = A->uid;
+            bs->g_forms = fk;
             bs->g_prec = prec;
             bs->g_rtol = rtol;
         }

@@ -228,6 +228,12 @@ struct ramd_csr_s {
     // every row stores its diagonal first or has none (move_diag layout,
     // core/matrix.cpp:729-787); lets Jacobi read one column per row
     bool diag_first = false;
+    // identity of the handle (never reused within a process) and a count of
+    // the device buffers built for it after creation (FP32 copies, the
+    // stencil-diagonal form): captured HIP graphs hold raw pointers to these
+    // buffers and are keyed on both
+    uint64_t uid = 0;
+    uint32_t forms = 0;
 };
 
 namespace ramd {

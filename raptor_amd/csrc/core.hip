@@ -1,5 +1,6 @@
 // Context, device memory, device CSR handles and the C entry points that
 // manage them.
+#include <atomic>
 #include <algorithm>
 #include <climits>
 #include <cstdarg>
@@ -75,7 +76,9 @@ std::vector<int> make_row_blocks(const int* rp, int n_rows, int cap)
 ramd_csr_s* csr_wrap(ramd_ctx ctx, int n_rows, int n_cols, int64_t nnz, int* rowptr_d, int* col_d,
                      double* val_d, const int* rowptr_host)
 {
+    static std::atomic<uint64_t> next_uid{1};
     ramd_csr_s* A = new ramd_csr_s();
+    A->uid = next_uid.fetch_add(1);
     A->ctx = ctx;
     A->n_rows = n_rows;
     A->n_cols = n_cols;
@@ -134,7 +137,10 @@ static void dia_make_fp32(ramd_csr_s* A)
     Dia* d = A->dia;
     const int n = A->n_rows;
     const int ns = dia_slots<float>(d->nd);
-    if (!d->v32) d->v32 = dalloc<float>((size_t)ns * n);
+    if (!d->v32) {
+        d->v32 = dalloc<float>((size_t)ns * n);
+        A->forms++;
+    }
     const int grid = (int)std::min<int64_t>(((int64_t)ns * n + 255) / 256, 8192);
     hipLaunchKernelGGL(dia_fp32_kernel, dim3(grid), dim3(256), 0, A->ctx->stream, n, d->nd, ns, d->v64, d->v32);
     RAMD_HIP(hipGetLastError());
@@ -335,6 +341,7 @@ bool dia_build(ramd_csr_s* A)
         throw;
     }
     A->dia = d;
+    A->forms++;
     return true;
 }
 
@@ -633,6 +640,7 @@ void csr_make_fp32(ramd_csr_s* A)
 {
     if (A->val32) return;
     A->val32 = dalloc<float>((size_t)A->nnz);
+    A->forms++;
     if (A->nnz) {
         int grid = (int)std::min<int64_t>((A->nnz + 255) / 256, 8192);
         hipLaunchKernelGGL(to_fp32_kernel, dim3(grid), dim3(256), 0, A->ctx->stream, A->val, A->val32, A->nnz);

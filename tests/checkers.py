@@ -60,6 +60,11 @@ def orc():
         L.orc_norm2.restype = C.c_double
         L.orc_spgemm.restype = C.c_int64
         L.orc_spgemm_tn.restype = C.c_int64
+        L.orc_mt_prepare.restype = C.c_void_p
+        L.orc_mt_prepare.argtypes = [C.c_void_p]
+        L.orc_mt_free.argtypes = [C.c_void_p]
+        L.orc_mt_cycle.argtypes = [C.c_void_p, PD, PD]
+        L.orc_mt_fpcg.argtypes = [C.c_void_p, C.c_void_p, PD, PD, C.c_double, C.c_int, PD]
         _orc = L
     return _orc
 
@@ -261,6 +266,40 @@ class Hier:
         it = orc().orc_pcg(C.byref(cA), C.byref(self.h), dp(x), dp(f64(b)), C.c_double(tol), max_iter, dp(res),
                            max_res, C.byref(nres))
         return x, res[:min(nres.value, max_res)], it
+
+
+class MtHier:
+    """oracle_mt.c over a Hier: the cycle and a flexible PCG on `threads`
+    host threads (OpenMP).  The cycle is bit-identical to Hier.cycle."""
+
+    def __init__(self, hier, threads):
+        L = orc()
+        L.orc_mt_set_threads(int(threads))
+        self.threads = L.orc_mt_threads()
+        self.hier = hier
+        self.p = L.orc_mt_prepare(C.byref(hier.h))
+
+    def cycle(self, x, b):
+        x = f64(x).copy()
+        assert orc().orc_mt_cycle(self.p, dp(x), dp(f64(b))) == 0
+        return x
+
+    def fpcg(self, A, x, b, rtol=1e-8, max_iter=500):
+        """(x, iterations, final true relative residual)"""
+        x = f64(x).copy()
+        rel = C.c_double()
+        cA = A.orc()
+        it = orc().orc_mt_fpcg(C.byref(cA), self.p, dp(x), dp(f64(b)), C.c_double(rtol), int(max_iter),
+                               C.byref(rel))
+        return x, it, rel.value
+
+    def free(self):
+        if self.p:
+            orc().orc_mt_free(self.p)
+            self.p = None
+
+    def __del__(self):
+        self.free()
 
 
 def cg(A, x, b, tol=1e-5, max_iter=-1, max_res=5000):

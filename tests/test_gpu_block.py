@@ -40,6 +40,37 @@ def problem(request, ctx):
     return dict(rp=rp, col=col, val=val, n=rp.size - 1, ml=ml, A=A)
 
 
+def oracle_hier(ml):
+    As, Ps = [], []
+    for l in range(ml.num_levels):
+        Al, Pl = ml.level(l)
+        As.append(K.Csr(*Al.to_csr(), Al.n_cols))
+        if Pl is not None:
+            Ps.append(K.Csr(*Pl.to_csr(), Pl.n_cols))
+    return K.Hier(As, Ps, relax_type=0, sweeps=1, omega=OMEGA)
+
+
+@pytest.mark.parametrize("k", [4, 32])
+def test_block_vcycle_columns_equal_oracle(problem, ctx, k):
+    """Every column of the block V-cycle (FP64 and FP32) equals the oracle's
+    cycle (multilevel/par_multilevel.hpp:341-457; its single-precision
+    sequence for FP32) applied to that column on the same hierarchy, bit for
+    bit."""
+    ml, n = problem["ml"], problem["n"]
+    H = oracle_hier(ml)
+    B = np.random.default_rng(100 + k).standard_normal((n, k))
+    X = ra.ParVector(ctx, n * k)
+    ra.vcycle_block(ml, X, ra.ParVector.from_numpy(ctx, B.ravel()), k, ra.PREC_FP64)
+    Xh = X.numpy().reshape(n, k)
+    B32 = B.astype(np.float32)
+    X32 = ra.ParVector(ctx, n * k, np.float32)
+    ra.vcycle_block(ml, X32, ra.ParVector.from_numpy(ctx, B32.ravel()), k, ra.PREC_FP32)
+    X32h = X32.numpy().reshape(n, k)
+    for c in range(k):
+        assert np.array_equal(Xh[:, c], H.cycle(np.zeros(n), B[:, c])), ("fp64", c)
+        assert np.array_equal(X32h[:, c], H.cycle_fp32(np.zeros(n, np.float32), B32[:, c])), ("fp32", c)
+
+
 @pytest.mark.parametrize("k", [1, 5, 32])
 def test_block_vcycle_columns_bit_exact(problem, ctx, k):
     ml, n = problem["ml"], problem["n"]

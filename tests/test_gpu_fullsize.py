@@ -1,9 +1,11 @@
 """Parity at the benchmark sizes.
 
-27-point Laplacian 128^3 (2.1M rows, RS(PMIS, extended), theta 0.25): the
-device-built hierarchy equals the host-built one level by level, and the
-device V-cycle (FP64 and FP32) is bit-identical to the oracle's cycle on
-that hierarchy.
+27-point Laplacian 128^3 (2.1M rows, RS(PMIS or HMIS, extended), theta
+0.25): the device-built hierarchy equals the host-built one level by level
+and matches the fingerprints of the reference's own setup of the same
+problem (tests/golden/ref_hier_27pt128.json: pattern, splitting and value
+hashes per level), and the device V-cycle (FP64 and FP32) is bit-identical
+to the oracle's cycle on that hierarchy.
 
 27-point Laplacian 256^3 (16.8M rows): properties that do not depend on an
 oracle run at this size -- the FP32-cycle and FP64-cycle flexible PCG both
@@ -11,6 +13,10 @@ reach a true FP64 relative residual of 1e-8 in the same number of
 iterations, their solutions agree to the solver tolerance, and the V-cycle
 is exactly linear under scaling by 2 (every operation of the cycle scales
 exactly by a power of two)."""
+import hashlib
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -23,8 +29,8 @@ pytestmark = pytest.mark.gpu
 OMEGA = 2.0 / 3.0
 
 
-def rs_pmis_ext(ctx, rp, col, val, device_setup=True):
-    ml = ra.ParRugeStubenSolver(ctx, 0.25, ra.PMIS, ra.Extended, ra.Classical, ra.Jacobi)
+def rs_pmis_ext(ctx, rp, col, val, device_setup=True, coarsen=ra.PMIS):
+    ml = ra.ParRugeStubenSolver(ctx, 0.25, coarsen, ra.Extended, ra.Classical, ra.Jacobi)
     ml.relax_weight = OMEGA
     ml.device_setup = device_setup
     ml.setup(rp, col, val)
@@ -38,10 +44,59 @@ def c2(ctx):
     return dict(rp=rp, col=col, val=val, ml=ml)
 
 
-def test_c2_device_hierarchy_equals_host(ctx, c2):
+@pytest.fixture(scope="module")
+def c2_hmis(ctx, c2):
+    return rs_pmis_ext(ctx, c2["rp"], c2["col"], c2["val"], coarsen=ra.HMIS)
+
+
+def fingerprint(rp, col, val):
+    """The fingerprint tests/golden/make_ref_hier_fingerprints.py records."""
+    h = hashlib.sha256()
+    h.update(np.asarray(rp, np.int32).astype("<i4").tobytes())
+    h.update(np.asarray(col, np.int32).astype("<i4").tobytes())
+    val = np.asarray(val, np.float64)
+    w = (np.arange(val.size) % 97 + 1).astype(np.float64)
+    return dict(n_rows=int(len(rp) - 1), nnz=int(len(col)), pattern_sha256=h.hexdigest(),
+                sum=float(np.sum(val)), abs_sum=float(np.sum(np.abs(val))), weighted_sum=float(np.dot(w, val)))
+
+
+def assert_matches_reference(ml, ref):
+    """Levels, patterns and splittings exact; value checksums to 1e-12."""
+    assert ml.num_levels == len(ref["levels"])
+    for l, lev in enumerate(ref["levels"]):
+        A, P = ml.level(l)
+        mats = [(A, lev["A"])] + ([(P, lev["P"])] if "P" in lev else [])
+        assert (P is None) == ("P" not in lev), l
+        for M, want in mats:
+            got = fingerprint(*M.to_csr())
+            for key in ("n_rows", "nnz", "pattern_sha256"):
+                assert got[key] == want[key], (l, key)
+            for key in ("sum", "abs_sum", "weighted_sum"):
+                assert abs(got[key] - want[key]) <= 1e-12 * max(abs(want[key]), want["abs_sum"]), (l, key)
+        if "P" in lev:
+            assert P.n_cols == lev["P"]["n_cols"]
+            st = ml.states(l)
+            assert hashlib.sha256(st.astype("<i4").tobytes()).hexdigest() == lev["states_sha256"], l
+
+
+REF_FP = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_hier_27pt128.json")
+
+
+@pytest.mark.parametrize("name", ["pmis", "hmis"])
+def test_c2_hierarchy_matches_reference_fingerprints(ctx, c2, c2_hmis, name):
+    """128^3 RS(PMIS / HMIS, extended) hierarchies against the reference's
+    own setup (ruge_stuben/par_ruge_stuben_solver.hpp:57-162)."""
+    with open(REF_FP) as f:
+        ref = json.load(f)
+    assert ref["n"] == c2["rp"].size - 1
+    assert_matches_reference(c2["ml"] if name == "pmis" else c2_hmis, ref["hierarchies"][name])
+
+
+@pytest.mark.parametrize("coarsen", [ra.PMIS, ra.HMIS], ids=["pmis", "hmis"])
+def test_c2_device_hierarchy_equals_host(ctx, c2, c2_hmis, coarsen):
     """Every level's A and P and every splitting at 128^3, device vs host setup."""
-    a = c2["ml"]
-    b = rs_pmis_ext(ctx, c2["rp"], c2["col"], c2["val"], device_setup=False)
+    a = c2["ml"] if coarsen == ra.PMIS else c2_hmis
+    b = rs_pmis_ext(ctx, c2["rp"], c2["col"], c2["val"], device_setup=False, coarsen=coarsen)
     assert a.num_levels == b.num_levels >= 4
     for l in range(a.num_levels):
         for M1, M2 in zip(a.level(l), b.level(l)):

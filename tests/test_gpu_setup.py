@@ -112,19 +112,16 @@ def test_hierarchy_device_setup_equals_host(ctx, name, gen, coarsen, stype, inte
     device steps and with the host steps: every level's A and P, and the C/F
     states, identical."""
     rp, col, val = gen()
-    mls, errs = [], []
+    mls = []
     for dev in (True, False):
         ml = ra.ParRugeStubenSolver(ctx, 0.25, coarsen, interp, stype, ra.Jacobi)
         ml.device_setup = dev
-        try:
-            ml.setup(rp, col, val)
-        except ra.RamdError as e:  # e.g. a singular coarsest matrix: both builds must fail alike
-            errs.append(str(e))
+        # a coarse matrix with a zero pivot does not stop the setup (the
+        # reference ignores dgetrf_'s info): every level is compared
+        ml.setup(rp, col, val)
         mls.append(ml)
-    if errs:
-        assert len(errs) == 2 and errs[0] == errs[1]
-        return
     a, b = mls
+    assert a.coarse_info() == b.coarse_info()
     assert a.num_levels == b.num_levels >= 3
     for l in range(a.num_levels):
         for M1, M2 in zip(a.level(l), b.level(l)):

@@ -458,3 +458,27 @@ def test_mc_sor_oracle_invariants(symmetric):
             K.orc().orc_mc_sor(n, K.ip(A.rowptr), K.ip(A.col), K.dp(A.val), K.dp(b), K.dp(x), 1, C.c_double(w),
                                symmetric)
         assert np.linalg.norm(b - K.spmv(A, x)) < 0.5 * r0
+
+
+@needs_ref
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_mt_cycle_and_fpcg(threads):
+    """oracle_mt.c (the CPU baseline): its row-parallel cycle equals
+    orc_cycle bit for bit at any thread count (the restriction as a gather
+    over P^T keeps each entry's term order); its flexible PCG reaches the
+    true relative residual it reports and gives the same iterates for every
+    thread count (fixed-chunk inner products)."""
+    A = stencil_grid_np(diffusion_2d(0.001, math.pi / 8), [60, 60])
+    As, Ps = _ref_hierarchy(A, 0.25, 1, 1, 0, 2.0 / 3.0)
+    H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=2.0 / 3.0)
+    M = K.MtHier(H, threads)
+    n = A.n_rows
+    b = np.random.default_rng(9).standard_normal(n)
+    assert np.array_equal(M.cycle(np.zeros(n), b), H.cycle(np.zeros(n), b))
+    x0 = np.random.default_rng(10).standard_normal(n)
+    assert np.array_equal(M.cycle(x0, b), H.cycle(x0, b))
+    x, it, rel = M.fpcg(A, np.zeros(n), b, 1e-8, 200)
+    assert 0 < it < 200 and rel <= 1e-8
+    assert np.linalg.norm(b - K.spmv(A, x)) / np.linalg.norm(b) == pytest.approx(rel, rel=1e-12)
+    x1, it1, rel1 = K.MtHier(H, 1).fpcg(A, np.zeros(n), b, 1e-8, 200)
+    assert it1 == it and rel1 == rel and np.array_equal(x1, x)
