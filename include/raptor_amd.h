@@ -109,6 +109,14 @@ int ramd_csr_set_sliced(ramd_csr A, int prec, int on);
  * sliced form), 0 returns to the row-block kernel.  Identical bits.  No
  * reference counterpart. */
 int ramd_csr_set_lane_rows(ramd_csr A, int prec, int on);
+/* The same choice for the wavefront-per-row kernel (one 64-lane wavefront
+ * streams a row's entries with coalesced loads and stages the products in
+ * LDS; one lane adds them in stored order): on = 1 selects it, 0 returns to
+ * the row-block kernel.  Identical bits; the autotuner tries it for
+ * operators averaging >= 24 entries per row.  It is CSR_spmv's row loop
+ * (util/linalg/spmv.cpp:52-67) with the loads of a row spread over a
+ * wavefront, for operators with few, long rows.  No reference counterpart. */
+int ramd_csr_set_wave_rows(ramd_csr A, int prec, int on);
 
 /* ------------------------------------------------------------------ */
 /* SpMV family (FP64; each row summed left to right in stored order, as  */
@@ -248,7 +256,8 @@ int ramd_hier_states(ramd_hier h, int level, int32_t* states);
  * CSR kernels: six entries per level, [A, P, R] x [FP64, FP32 vectors];
  * -1 where the sliced-row kernel (64 rows per slice, one lane per row) was
  * measured faster, -2 the sliced-row kernel with 16-bit column offsets,
- * 1 where the lane-per-row kernel over the CSR arrays was,
+ * 1 where the lane-per-row kernel over the CSR arrays was, 2 where the
+ * wavefront-per-row kernel was,
  * 0 for a stencil-diagonal operator or an absent one.  No
  * reference counterpart. */
 int ramd_hier_row_caps(ramd_hier h, int32_t* caps);
@@ -280,6 +289,14 @@ int ramd_hier_save(ramd_hier h, const char* path);
 int ramd_hier_load(ramd_ctx ctx, const char* path, ramd_hier* out);
 /* Builds the FP32 copies of every level used by RAMD_PREC_FP32 cycles. */
 int ramd_hier_enable_fp32(ramd_hier h);
+/* on != 0 (the default): ramd_fpcg and ramd_adjoint replay each iteration
+ * as two captured HIP graphs (operator step, preconditioner step; kept per
+ * hierarchy and rebuilt when the operator, a level's buffers, the precision
+ * or x change); 0: every kernel is launched eagerly.  Same results either
+ * way.  While kernel timing (ramd_prof_enable) or per-level tracking
+ * (ramd_hier_track_times) is on, iterations run eagerly.  No reference
+ * counterpart. */
+int ramd_set_graphs(ramd_ctx ctx, int on);
 
 /* ------------------------------------------------------------------ */
 /* solve path                                                          */

@@ -60,13 +60,23 @@ class Context:
     """One GPU and its stream (ramd_init).  Closing it first releases every
     live object created on it, so nothing is freed after its context."""
 
+    # whether new contexts replay solver iterations from captured HIP graphs
+    graphs_default = True
+
     def __init__(self, device=0):
         self.h = C.c_void_p()
         self._children = weakref.WeakSet()
         check(lib().ramd_init(device, C.byref(self.h)))
+        if not Context.graphs_default:
+            self.set_graphs(False)
 
     def _adopt(self, obj):
         self._children.add(obj)
+
+    def set_graphs(self, on=True):
+        """Replay the solvers' iterations as captured HIP graphs (default) or
+        launch every kernel eagerly (ramd_set_graphs); same results."""
+        check(lib().ramd_set_graphs(self.h, 1 if on else 0))
 
     def sync(self):
         check(lib().ramd_sync(self.h))
@@ -223,6 +233,11 @@ class ParCSRMatrix:
         one lane per row over the CSR arrays, False the row-block kernel
         (identical bits)."""
         check(lib().ramd_csr_set_lane_rows(self.h, int(prec), int(bool(on))))
+
+    def set_wave_rows(self, prec, on):
+        """CSR kernel for FP64 (prec 0) or FP32 (prec 1) vectors: True selects
+        one wavefront per row, False the row-block kernel (identical bits)."""
+        check(lib().ramd_csr_set_wave_rows(self.h, int(prec), int(bool(on))))
 
     def set_values(self, vals):
         """New values (a device ParVector of nnz entries, stored order) for the same pattern."""

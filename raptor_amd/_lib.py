@@ -87,6 +87,7 @@ _SIGS = {
     "ramd_csr_set_dia_staging": (P, C.c_int, C.c_int),
     "ramd_csr_set_sliced": (P, C.c_int, C.c_int),
     "ramd_csr_set_lane_rows": (P, C.c_int, C.c_int),
+    "ramd_csr_set_wave_rows": (P, C.c_int, C.c_int),
     "ramd_spmv": (P, P, P),
     "ramd_spmv_append": (P, P, P),
     "ramd_residual": (P, P, P, P),
@@ -113,6 +114,7 @@ _SIGS = {
     "ramd_host_read_petsc": (C.c_char_p, PI32, PI32, PPI32, PPI32, PPD, PI64),
     "ramd_host_write_petsc": (C.c_char_p, I32, I32, PI32, PI32, PD),
     "ramd_hier_enable_fp32": (P,),
+    "ramd_set_graphs": (P, C.c_int),
     "ramd_vcycle": (P, P, P, C.c_int),
     "ramd_vcycle_fp32": (P, P, P),
     "ramd_amg_solve": (P, P, P, D, C.c_int, PD, PI32),

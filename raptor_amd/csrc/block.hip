@@ -871,15 +871,6 @@ struct BlockState {
     double g_rtol = -1;
 };
 
-// buffer counts of every operator a captured iteration launches on
-static uint64_t forms_key(const ramd_hier_s* h, const ramd_csr_s* A)
-{
-    uint64_t k = A->forms;
-    for (const Level& L : h->levels)
-        for (const ramd_csr_s* M : {(const ramd_csr_s*)L.A, (const ramd_csr_s*)L.P, (const ramd_csr_s*)L.R})
-            if (M) k = k * 1000003ull + M->uid * 31ull + M->forms;
-    return k;
-}
 
 static void free_graphs(BlockState* b)
 {
@@ -1150,25 +1141,6 @@ static void bprecondition(ramd_hier_s* h, BlockState* bs, bool f32)
     else bcycle<double>(h, bs, 0, bs->Z, bs->R);
 }
 
-static hipGraphExec_t capture(hipStream_t s, const std::function<void()>& fn)
-{
-    hipGraph_t g = nullptr;
-    RAMD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    try {
-        fn();
-    } catch (...) {
-        (void)hipStreamEndCapture(s, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
-    }
-    RAMD_HIP(hipStreamEndCapture(s, &g));
-    hipGraphExec_t ex = nullptr;
-    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    RAMD_HIP(e);
-    return ex;
-}
-
 static int fpcg_block_run(ramd_hier_s* h, ramd_csr_s* A, int k, double* X_d, const double* B_d, double rtol,
                           int max_iter, int prec, int use_graph, int32_t* iters_col, double* relres_col,
                           int32_t* iters_total)
@@ -1236,8 +1208,8 @@ static int fpcg_block_run(ramd_hier_s* h, ramd_csr_s* A, int k, double* X_d, con
         if (use_graph && (bs->g_uid != A->uid || bs->g_forms != fk || bs->g_prec != prec || bs->g_rtol != rtol ||
                           !bs->gA)) {
             free_graphs(bs);
-            bs->gA = capture(s, stepA);
-            bs->gB = capture(s, stepB);
+            bs->gA = capture_graph(s, stepA);
+            bs->gB = capture_graph(s, stepB);
             bs->g_uid = A->uid;
             bs->g_forms = fk;
             bs->g_prec = prec;

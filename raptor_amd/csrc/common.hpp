@@ -41,6 +41,9 @@ constexpr double kSlicedMaxPad = 1.5;
 // the lane-per-row CSR kernel is tried for operators averaging at most this
 // many entries per row
 constexpr double kLaneRowsMaxLen = 8.0;
+// the wavefront-per-row kernel is tried for operators averaging at least
+// this many entries per row
+constexpr double kWaveRowsMinLen = 24.0;
 constexpr int kStreamCap = 4096;     // products staged in LDS per workgroup
 constexpr int kMaxRowsPerBlock = 256;
 
@@ -105,6 +108,8 @@ struct ramd_ctx_s {
     // per-workgroup partial sums of fused kernels (grown on demand)
     double* big_partials = nullptr;
     size_t big_cap = 0;
+    // solvers replay captured HIP graphs of their iterations (ramd_set_graphs)
+    bool graphs = true;
     void set_current() const;
 };
 
@@ -202,6 +207,7 @@ struct ramd_csr_s {
     bool sl_on[2] = {false, false};  // per vector precision: use the sliced form
     bool sl16[2] = {false, false};   // ... with its 16-bit column offsets
     bool lr_on[2] = {false, false};  // per vector precision: one lane per CSR row
+    bool wr_on[2] = {false, false};  // per vector precision: one wavefront per CSR row
     int32_t n_rows = 0, n_cols = 0;
     int64_t nnz = 0;
     int* rowptr = nullptr;   // device, n_rows+1
@@ -293,7 +299,9 @@ void launch_stream_op_c(const ramd_csr_s* A, int op, const double* x, const doub
 // reduced into *out_d (fixed order: deterministic).  values_fp32 selects the
 // lossless FP32-value form of launch_stream_op_c.
 void launch_spmv_dot(const ramd_csr_s* A, bool values_fp32, const double* x, double* y, double* out_d,
-                     hipStream_t s);
+                     hipStream_t s, double* partials = nullptr);
+// per-workgroup partials launch_spmv_dot writes for A
+int spmv_dot_partials(const ramd_csr_s* A);
 
 // Deterministic device reductions; result left in *out_d (device scalar).
 void launch_dot(ramd_ctx ctx, int64_t n, const double* x, const double* y, double* out_d);

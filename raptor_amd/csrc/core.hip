@@ -933,7 +933,7 @@ int ramd_csr_set_sliced(ramd_csr A, int prec, int on)
                    "ramd_csr_set_sliced: the column offsets do not fit 16 bits (or the matrix is not square)");
         A->sl_on[prec] = on != 0;
         A->sl16[prec] = on == 2;
-        if (on) A->lr_on[prec] = false;
+        if (on) A->lr_on[prec] = A->wr_on[prec] = false;
     });
 }
 
@@ -945,7 +945,19 @@ int ramd_csr_set_lane_rows(ramd_csr A, int prec, int on)
         RAMD_CHECK(!A->dia, RAMD_E_STATE, "ramd_csr_set_lane_rows: matrix uses its stencil-diagonal form");
         RAMD_CHECK(prec == 0 || A->val32 || !on, RAMD_E_STATE, "ramd_csr_set_lane_rows: FP32 values were not built");
         A->lr_on[prec] = on != 0;
-        if (on) A->sl_on[prec] = A->sl16[prec] = false;
+        if (on) A->sl_on[prec] = A->sl16[prec] = A->wr_on[prec] = false;
+    });
+}
+
+int ramd_csr_set_wave_rows(ramd_csr A, int prec, int on)
+{
+    return guarded([&] {
+        RAMD_CHECK(A && (prec == 0 || prec == 1) && (on == 0 || on == 1), RAMD_E_ARG,
+                   "ramd_csr_set_wave_rows: bad argument");
+        RAMD_CHECK(!A->dia, RAMD_E_STATE, "ramd_csr_set_wave_rows: matrix uses its stencil-diagonal form");
+        RAMD_CHECK(prec == 0 || A->val32 || !on, RAMD_E_STATE, "ramd_csr_set_wave_rows: FP32 values were not built");
+        A->wr_on[prec] = on != 0;
+        if (on) A->sl_on[prec] = A->sl16[prec] = A->lr_on[prec] = false;
     });
 }
 

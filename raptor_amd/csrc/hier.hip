@@ -482,6 +482,11 @@ static void hier_free(ramd_hier_s* h)
     if (!h) return;
     if (h->ctx) (void)hipStreamSynchronize(h->ctx->stream);
     block_free(h->blk);
+    if (h->fg) {
+        fpcg_graphs_free(h->fg);
+        delete h->fg;
+    }
+    dfree(h->fpart);
     h->spans.release();
     for (Level& L : h->levels) free_level(L);
     for (void* p : {(void*)h->lu, (void*)h->piv, (void*)h->cb, (void*)h->cx, (void*)h->pr, (void*)h->pz, (void*)h->pp,
@@ -936,12 +941,12 @@ __global__ void fpcg_step_x(int n, const double* __restrict__ sc_alpha, const do
 static int fixed_blocks(int n) { return std::max(1, std::min(kRedBlocks, (n + kBlockThreads - 1) / kBlockThreads)); }
 
 // r.z and z.Ap of the flexible-PCG step, in one pass, into device scalars
+// (part: 2 * fixed_blocks(n) partial sums)
 template <typename Z>
 static void fpcg_rz_zap(ramd_ctx ctx, int n, const double* r, const Z* z, const double* Ap, double* rz_d,
-                        double* zap_d)
+                        double* zap_d, double* part)
 {
     const int g = fixed_blocks(n);
-    double* part = ctx_partials(ctx, 2 * (size_t)g);
     hipLaunchKernelGGL((fpcg_dots<Z>), dim3(g), dim3(kBlockThreads), 0, ctx->stream, n, r, z, Ap, part);
     RAMD_HIP(hipGetLastError());
     launch_reduce_final(ctx, part, g, rz_d);
@@ -951,10 +956,63 @@ static void fpcg_rz_zap(ramd_ctx ctx, int n, const double* r, const Z* z, const 
 // plain 2-norm (no small-entry skipping, NaN propagates)
 static double norm2_64(ramd_ctx ctx, int n, const double* a) { return std::sqrt(dot64(ctx, n, a, a)); }
 
+hipGraphExec_t capture_graph(hipStream_t s, const std::function<void()>& fn)
+{
+    hipGraph_t g = nullptr;
+    RAMD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        fn();
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    RAMD_HIP(hipStreamEndCapture(s, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    RAMD_HIP(e);
+    return ex;
+}
+
+void fpcg_graphs_free(FpcgGraphs* g)
+{
+    if (!g) return;
+    for (int i = 0; i < 2; i++) {
+        if (g->gA[i]) (void)hipGraphExecDestroy(g->gA[i]);
+        if (g->gB[i]) (void)hipGraphExecDestroy(g->gB[i]);
+        g->gA[i] = g->gB[i] = nullptr;
+    }
+    g->a_uid = 0;
+}
+
+// The FPCG's own partial-sum buffer (>= count doubles).  Growing it
+// releases the captured iterations, which hold the old buffer.
+static double* fpcg_partials(ramd_hier_s* h, size_t count)
+{
+    if (h->fpart_cap < count) {
+        RAMD_HIP(hipStreamSynchronize(h->ctx->stream));
+        fpcg_graphs_free(h->fg);
+        dfree(h->fpart);
+        h->fpart = dalloc<double>(count);
+        h->fpart_cap = count;
+    }
+    return h->fpart;
+}
+
 // Flexible PCG in FP64 (true residual, inner products, coarse LU in FP64)
 // preconditioned by one V-cycle in `cycle_prec`; stops on the true relative
 // residual.  Returns RAMD_OK, RAMD_E_INDEFINITE, RAMD_E_NUMERIC or
 // RAMD_E_NOT_CONVERGED; throws on argument and device errors.
+//
+// An iteration is two steps: the operator step (A p with p.Ap, the
+// residual update with ||r||^2) and, after the host has read those 16
+// bytes, the preconditioner step (V-cycle, r.z and z.Ap, new direction).
+// With the context's graphs on (and no kernel timing or per-level tracking
+// active) each step is replayed as a captured HIP graph, one pair per
+// parity of the alternating r.z slots; the iterations that differ (the
+// last one, a true-residual restart) run eagerly.  Replays and eager steps
+// launch the same kernels with the same arguments: same bits.
 static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_d, double rtol, int max_iter,
                     int cycle_prec, double* relres_hist, int32_t* iters, double* final_relres)
 {
@@ -979,23 +1037,51 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
     float *r32 = h->pr32, *z32 = h->pz32;
     const int g = fixed_blocks(n);
     const dim3 gv = grid1(n), bv(256);
-    // z = M^-1 r from the current r (FP32 form: from r32, which must hold (float) r)
+    double* part = fpcg_partials(h, (size_t)std::max(spmv_dot_partials(A), 2 * g));
     // FP32 form: the update pass also forms the cycle's first Jacobi sweep
     // from zero on the fine level (Jacobi smoothing, more than one level)
     const bool fuse_first = f32 && !multicolour(h) && h->opts.num_smooth_sweeps >= 1 && h->levels.size() > 1;
-    bool z_first = false;  // z32 holds that sweep of the current r32
-    auto cycle = [&] {
-        if (f32) cycle32(h, 0, z32, r32, true, z_first);
-        else cycle64(h, 0, z, r, true);
-        z_first = false;
-    };
     // device scalars: [0] p.Ap, [1] r.r, [2]/[5] r.z (current / next), [3] z.Ap, [4] alpha
     double* sc = ctx->red_result;
     int cur = 2, nxt = 5;
-    auto dots = [&](const double* ap, double* rz_d) {
-        if (f32) fpcg_rz_zap(ctx, n, r, (const float*)z32, ap, rz_d, sc + 3);
-        else fpcg_rz_zap(ctx, n, r, (const double*)z, ap, rz_d, sc + 3);
+    // operator step: Ap = A p and p.Ap in one pass over A; then r -= alpha Ap,
+    // ||r||^2 (and r32, and in the FP32 form the first sweep z32) in one pass
+    auto stepA = [&](int c) {
+        launch_spmv_dot(A, true, p, Ap, sc, s, part);
+        hipLaunchKernelGGL(fpcg_update_norm, dim3(g), dim3(kBlockThreads), 0, s, n, sc + c, sc, sc + 4, Ap, r,
+                           f32 ? r32 : nullptr, part, fuse_first ? h->levels[0].diag32 : nullptr,
+                           fuse_first ? z32 : nullptr, (float)h->opts.relax_weight);
+        RAMD_HIP(hipGetLastError());
+        launch_reduce_final(ctx, part, g, sc + 1);
     };
+    // preconditioner step: z = M^-1 r (first_done: z32 holds the fused first
+    // sweep), r.z into slot nx and z.Ap, then the step x += alpha p (unless
+    // taken already: xs null) and p = z + beta p with the flexible beta
+    // (r_new - r_old = -alpha Ap)
+    auto stepB = [&](int c, int nx, double* xs, bool first_done) {
+        if (f32) cycle32(h, 0, z32, r32, true, first_done);
+        else cycle64(h, 0, z, r, true);
+        if (f32) fpcg_rz_zap(ctx, n, r, (const float*)z32, Ap, sc + nx, sc + 3, part);
+        else fpcg_rz_zap(ctx, n, r, (const double*)z, Ap, sc + nx, sc + 3, part);
+        if (f32) hipLaunchKernelGGL((fpcg_dir<float>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + c, z32, p, xs);
+        else hipLaunchKernelGGL((fpcg_dir<double>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + c, z, p, xs);
+        RAMD_HIP(hipGetLastError());
+    };
+    const bool use_graphs = ctx->graphs && ctx->prof_min_nnz < 0 && !h->track;
+    if (use_graphs) {
+        if (!h->fg) h->fg = new FpcgGraphs();
+        FpcgGraphs* fg = h->fg;
+        const uint64_t key = forms_key(h, A);
+        if (fg->a_uid != A->uid || fg->key != key || fg->prec != cycle_prec || fg->x != x_d || fg->part != part) {
+            RAMD_HIP(hipStreamSynchronize(s));
+            fpcg_graphs_free(fg);
+            fg->a_uid = A->uid;
+            fg->key = key;
+            fg->prec = cycle_prec;
+            fg->x = x_d;
+            fg->part = part;
+        }
+    }
     const double bnorm = norm2_64(ctx, n, b_d);
     const double scale = bnorm > 0 ? bnorm : 1.0;
     outer(A, OP_RESIDUAL, x_d, b_d, r, 0.0, s);
@@ -1008,30 +1094,31 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
         if (f32) {
             hipLaunchKernelGGL(to_f32, gv, bv, 0, s, (int64_t)n, r, r32);
             RAMD_HIP(hipGetLastError());
+            cycle32(h, 0, z32, r32, true, false);
+            hipLaunchKernelGGL(to_f64, gv, bv, 0, s, (int64_t)n, z32, p);
+        } else {
+            cycle64(h, 0, z, r, true);
+            RAMD_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         }
-        cycle();
-        if (f32) hipLaunchKernelGGL(to_f64, gv, bv, 0, s, (int64_t)n, z32, p);
-        else RAMD_HIP(hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         RAMD_HIP(hipGetLastError());
-        dots(r, sc + cur);
+        if (f32) fpcg_rz_zap(ctx, n, r, (const float*)z32, r, sc + cur, sc + 3, part);
+        else fpcg_rz_zap(ctx, n, r, (const double*)z, r, sc + cur, sc + 3, part);
         // the step x += alpha p of an iteration is taken by its direction
         // update (one read of p), or here before x is read or returned
         auto step_x = [&] {
             hipLaunchKernelGGL(fpcg_step_x, gv, bv, 0, s, n, sc + 4, p, x_d);
             RAMD_HIP(hipGetLastError());
         };
+        bool z_first = false;  // z32 holds the fused first sweep of the current r32
         while (it < max_iter) {
-            // Ap = A p and p.Ap in one pass over A; then r -= alpha Ap,
-            // ||r||^2 (and r32) in one pass; one host read
-            launch_spmv_dot(A, true, p, Ap, sc, s);
-            // (the partials buffer may have grown inside launch_spmv_dot)
-            double* part = ctx_partials(ctx, (size_t)g);
-            hipLaunchKernelGGL(fpcg_update_norm, dim3(g), dim3(kBlockThreads), 0, s, n, sc + cur, sc, sc + 4, Ap, r,
-                               f32 ? r32 : nullptr, part, fuse_first ? h->levels[0].diag32 : nullptr,
-                               fuse_first ? z32 : nullptr, (float)h->opts.relax_weight);
-            RAMD_HIP(hipGetLastError());
+            const int par = cur == 2 ? 0 : 1;
+            if (use_graphs) {
+                if (!h->fg->gA[par]) h->fg->gA[par] = capture_graph(s, [&] { stepA(cur); });
+                RAMD_HIP(hipGraphLaunch(h->fg->gA[par], s));
+            } else {
+                stepA(cur);
+            }
             z_first = fuse_first;
-            launch_reduce_final(ctx, part, g, sc + 1);
             double pr[2];
             fetch_scalars(ctx, sc, 2, pr);
             if (!(pr[0] > 0.0)) {
@@ -1065,13 +1152,16 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
                 rc = RAMD_E_NUMERIC;
                 break;
             }
-            cycle();
-            // flexible beta with r_new - r_old = -alpha Ap
-            dots(Ap, sc + nxt);
-            double* xs = x_done ? nullptr : x_d;
-            if (f32) hipLaunchKernelGGL((fpcg_dir<float>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + cur, z32, p, xs);
-            else hipLaunchKernelGGL((fpcg_dir<double>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + cur, z, p, xs);
-            RAMD_HIP(hipGetLastError());
+            // the common iteration (x not stepped yet, the fused sweep in
+            // place) replays the captured step; the others run eagerly
+            const bool plain = !x_done && z_first == fuse_first;
+            if (use_graphs && plain) {
+                if (!h->fg->gB[par]) h->fg->gB[par] = capture_graph(s, [&] { stepB(cur, nxt, x_d, fuse_first); });
+                RAMD_HIP(hipGraphLaunch(h->fg->gB[par], s));
+            } else {
+                stepB(cur, nxt, x_done ? nullptr : x_d, z_first);
+            }
+            z_first = false;
             std::swap(cur, nxt);
         }
     }
@@ -1478,7 +1568,12 @@ int ramd_hier_row_caps(ramd_hier h, int32_t* caps)
             for (int m = 0; m < 3; m++)
                 for (int p = 0; p < 2; p++)
                     caps[6 * l + 2 * m + p] =
-                        (M[m] && !M[m]->dia) ? (M[m]->sl_on[p] ? (M[m]->sl16[p] ? -2 : -1) : M[m]->lr_on[p] ? 1 : M[m]->rb[p].cap) : 0;
+                        (M[m] && !M[m]->dia)
+                            ? (M[m]->wr_on[p]   ? 2
+                               : M[m]->sl_on[p] ? (M[m]->sl16[p] ? -2 : -1)
+                               : M[m]->lr_on[p] ? 1
+                                                : M[m]->rb[p].cap)
+                            : 0;
         }
     });
 }
@@ -1525,6 +1620,14 @@ int ramd_hier_level_times(ramd_hier h, int phase, double* times)
             h->spans.resolve(h->solve_level_s);
             for (size_t l = 0; l < nl; l++) times[5 * l] = h->solve_level_s[l];
         }
+    });
+}
+
+int ramd_set_graphs(ramd_ctx ctx, int on)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx, RAMD_E_ARG, "ramd_set_graphs: NULL context");
+        ctx->graphs = on != 0;
     });
 }
 

@@ -2,6 +2,7 @@
 // (hier.hip) and the multi-vector path (block.hip).
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "common.hpp"
@@ -107,6 +108,23 @@ struct EventSpans {
     void release();
 };
 
+// Stream capture of fn's launches into an instantiated graph (thread-local
+// capture mode; no memset node is ever recorded: buffers are cleared by
+// kernels).
+hipGraphExec_t capture_graph(hipStream_t s, const std::function<void()>& fn);
+
+// Captured flexible-PCG iteration of the single-vector solve: the operator
+// step and the preconditioner step, one pair per parity of the alternating
+// r.z slots, valid for one (A, hierarchy forms, precision, x, partials).
+struct FpcgGraphs {
+    hipGraphExec_t gA[2] = {nullptr, nullptr}, gB[2] = {nullptr, nullptr};
+    uint64_t a_uid = 0, key = 0;
+    int prec = -1;
+    const double* x = nullptr;
+    const double* part = nullptr;
+};
+void fpcg_graphs_free(FpcgGraphs* g);
+
 }  // namespace ramd
 
 struct ramd_hier_s {
@@ -134,7 +152,25 @@ struct ramd_hier_s {
     double *pr = nullptr, *pz = nullptr, *pp = nullptr, *pAp = nullptr;
     float *pr32 = nullptr, *pz32 = nullptr;
     ramd::BlockState* blk = nullptr;
+    // captured FPCG iteration and its own partial-sum buffer (never
+    // reallocated while graphs hold it)
+    ramd::FpcgGraphs* fg = nullptr;
+    double* fpart = nullptr;
+    size_t fpart_cap = 0;
 };
+
+namespace ramd {
+// Identity and buffer counts of A and of every level operator: captured
+// graphs hold raw pointers to all of their buffers and are keyed on this.
+inline uint64_t forms_key(const ramd_hier_s* h, const ramd_csr_s* A)
+{
+    uint64_t k = A->uid * 1000003ull + A->forms;
+    for (const Level& L : h->levels)
+        for (const ramd_csr_s* M : {(const ramd_csr_s*)L.A, (const ramd_csr_s*)L.P, (const ramd_csr_s*)L.R})
+            if (M) k = k * 1000003ull + M->uid * 31ull + M->forms;
+    return k;
+}
+}  // namespace ramd
 
 namespace ramd {
 // Builds the FP32 copies of every level (values, diagonal, level vectors).

@@ -289,6 +289,135 @@ __global__ __launch_bounds__(kBlockThreads) void sliced_kernel(
     }
 }
 
+// Wavefront-level barrier: LDS written by lanes of one wavefront is visible
+// to its other lanes after it (no workgroup barrier: the wavefronts of a
+// workgroup run different trip counts).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wavefront per row: the form for operators with few, long rows (the
+// coarse Galerkin operators of aggregation hierarchies), where one lane per
+// row leaves most of the GPU idle and walks a long dependent chain of
+// loads.  The wavefront streams the row's columns and values with coalesced
+// loads (64 consecutive entries per step), gathers x and stages the products
+// in LDS, kWaveChunk at a time; lane 0 then adds them left to right with the
+// operations of stream_kernel's phase 2 (the first-entry diagonal rule of a
+// diagonal-first row, or the entry whose column is the row itself), so the
+// bits are the same.
+constexpr int kWaveRows = kBlockThreads / 64;  // rows per workgroup
+constexpr int kWaveChunk = 512;                // products staged per wavefront and pass
+
+template <int OP, typename V, typename X>
+__global__ __launch_bounds__(kBlockThreads) void wave_kernel(int n, const int* __restrict__ rowptr,
+                                                            const int* __restrict__ col, const V* __restrict__ val,
+                                                            const X* __restrict__ x, const X* __restrict__ b,
+                                                            X* __restrict__ y, X omega, int diag_first,
+                                                            double* __restrict__ dotp, const X* __restrict__ dvec)
+{
+    __shared__ X prod[kWaveRows * kWaveChunk];
+    __shared__ int dpos[kWaveRows];
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = blk * kWaveRows + wv;
+    const bool valid = row < n;
+    X* pw = prod + wv * kWaveChunk;
+    int s = 0, e = 0;
+    if (valid) {
+        s = rowptr[row];
+        e = rowptr[row + 1];
+    }
+    const int len = e - s;
+    // entries [first, len) are added, except entry `skip` (Jacobi's diagonal
+    // when it is not stored first); a Jacobi row without a usable diagonal
+    // keeps its value
+    X acc = X(0), diag = X(0);
+    int first = 0;
+    if (lane == 0) dpos[wv] = -1;
+    if (OP == OP_RESIDUAL && valid) acc = b[row];
+    if (OP == OP_JACOBI && valid && diag_first) {
+        if (dvec) {
+            const X dg = dvec[row];
+            if (fabs(dg) > X(1e-16)) {
+                diag = dg;
+                first = 1;
+            } else {
+                first = len;
+            }
+        } else if (len > 0 && col[s] == row) {
+            diag = X(val[s]);
+            first = 1;
+        } else {
+            first = len;
+        }
+    }
+    wave_sync();
+    for (int c0 = 0; c0 < len; c0 += kWaveChunk) {
+        const int cn = min(kWaveChunk, len - c0);
+        constexpr int U = kWaveChunk / 64;
+        int cc[U];
+        V vv[U];
+        X xx[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {  // streamed once: past the caches
+            const int j = lane + 64 * u;
+            if (j < cn) {
+                cc[u] = __builtin_nontemporal_load(&col[s + c0 + j]);
+                vv[u] = __builtin_nontemporal_load(&val[s + c0 + j]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int j = lane + 64 * u;
+            if (j < cn) xx[u] = x[cc[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int j = lane + 64 * u;
+            if (j < cn) {
+                pw[j] = X(vv[u]) * xx[u];
+                if (OP == OP_JACOBI && !diag_first && cc[u] == row) dpos[wv] = c0 + j;
+            }
+        }
+        wave_sync();
+        if (lane == 0) {
+            const int k0 = max(first - c0, 0);
+            if (OP == OP_JACOBI && !diag_first) {
+                const int dp = dpos[wv];
+                if (dp >= c0 && dp < c0 + cn) {
+                    diag = X(val[s + dp]);
+                    acc = lds_row_sum<false>(pw, k0, dp - c0, acc);
+                    acc = lds_row_sum<false>(pw, dp - c0 + 1, cn, acc);
+                } else {
+                    acc = lds_row_sum<false>(pw, k0, cn, acc);
+                }
+            } else {
+                acc = lds_row_sum<OP == OP_RESIDUAL>(pw, min(k0, cn), cn, acc);
+            }
+        }
+        wave_sync();
+    }
+    if (OP == OP_SPMV && dotp) {
+        const double v = block_sum((valid && lane == 0) ? (double)x[row] * (double)acc : 0.0);
+        if (threadIdx.x == 0) dotp[blk] = v;
+    }
+    if (!valid || lane != 0) return;
+    if (OP == OP_SPMV || OP == OP_RESIDUAL) {
+        y[row] = acc;
+    } else if (OP == OP_APPEND) {
+        y[row] += acc;
+    } else if (OP == OP_ADD_INTO) {
+        y[row] = b[row] + acc;
+    } else {
+        const X xo = x[row];
+        y[row] = (len > 0 && fabs(diag) > X(1e-16)) ? ((X(1.0) - omega) * xo) + (omega * ((b[row] - acc) / diag))
+                                                    : xo;
+    }
+}
+
 // Stencil-diagonal kernels: one lane per row, diagonals visited in the
 // row's stored order (slot t holds diagonal a.ord[t]).  A lane reads its
 // row's values a 16-byte group of slots at a time (dia_index layout).  Every
@@ -683,6 +812,28 @@ static void launch_sliced(const ramd_csr_s* A, const V* csr_val, int op, const X
     RAMD_HIP(hipGetLastError());
 }
 
+template <typename V, typename X>
+static void launch_wave(const ramd_csr_s* A, const V* val, int op, const X* x, const X* b, X* y, X omega,
+                        hipStream_t s, double* dotp, const X* dvec)
+{
+    const int n = A->n_rows;
+    const dim3 grid((n + kWaveRows - 1) / kWaveRows), block(kBlockThreads);
+    const int df = (int)A->diag_first;
+#define WAVE(OPV, DOT, DV)                                                                                     \
+    hipLaunchKernelGGL((wave_kernel<OPV, V, X>), grid, block, 0, s, n, A->rowptr, A->col, val, x, b, y, omega, df, \
+                       DOT, DV)
+    switch (op) {
+    case OP_SPMV: WAVE(OP_SPMV, dotp, nullptr); break;
+    case OP_APPEND: WAVE(OP_APPEND, nullptr, nullptr); break;
+    case OP_RESIDUAL: WAVE(OP_RESIDUAL, nullptr, nullptr); break;
+    case OP_JACOBI: WAVE(OP_JACOBI, nullptr, dvec); break;
+    case OP_ADD_INTO: WAVE(OP_ADD_INTO, nullptr, nullptr); break;
+    default: fail(RAMD_E_ARG, "unknown stream op %d", op);
+    }
+#undef WAVE
+    RAMD_HIP(hipGetLastError());
+}
+
 // val: CSR values; dval: stencil-diagonal values (nullptr -> CSR kernel)
 template <typename V, typename X>
 static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int op, const X* x, const X* b, X* y,
@@ -703,6 +854,14 @@ static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int 
     }
     if (dval) {
         launch_dia(A, dval, op, x, b, y, omega, s, dotp);
+        if (prof) {
+            RAMD_HIP(hipEventRecord(pend.end, s));
+            ctx->prof_pending.push_back(pend);
+        }
+        return;
+    }
+    if (A->wr_on[sizeof(X) == 4 ? 1 : 0]) {
+        launch_wave<V, X>(A, val, op, x, b, y, omega, s, dotp, dvec);
         if (prof) {
             RAMD_HIP(hipEventRecord(pend.end, s));
             ctx->prof_pending.push_back(pend);
@@ -788,6 +947,7 @@ void csr_autotune(ramd_csr_s* A, int prec)
     A->sl_on[prec] = false;
     A->sl16[prec] = false;
     A->lr_on[prec] = false;
+    A->wr_on[prec] = false;
     const size_t w = prec ? sizeof(float) : sizeof(double);
     struct Scratch {
         void *x = nullptr, *b = nullptr, *y = nullptr;
@@ -894,6 +1054,32 @@ void csr_autotune(ramd_csr_s* A, int prec)
             if (!A->sl_on[0] && !A->sl_on[1]) sliced_free(A);
         }
     }
+    // few long rows: one wavefront per row against the best so far
+    if ((double)A->nnz >= kWaveRowsMinLen * (double)A->n_rows) {
+        const bool lr = A->lr_on[prec], sl = A->sl_on[prec];
+        A->wr_on[prec] = true;
+        const float ms = time_ms();
+        if (ms < best_ms) {
+            best_ms = ms;
+            A->lr_on[prec] = false;
+            if (sl) {
+                A->sl_on[prec] = false;
+                A->sl16[prec] = false;
+                RAMD_HIP(hipStreamSynchronize(s));
+                if (prec) {
+                    dfree(A->sl->v32);
+                    A->sl->v32 = nullptr;
+                } else {
+                    dfree(A->sl->v64);
+                    A->sl->v64 = nullptr;
+                }
+                if (!A->sl_on[0] && !A->sl_on[1]) sliced_free(A);
+            }
+        } else {
+            A->wr_on[prec] = false;
+            A->lr_on[prec] = lr;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
 }
@@ -963,13 +1149,21 @@ double* ctx_partials(ramd_ctx ctx, size_t count)
     return ctx->big_partials;
 }
 
+int spmv_dot_partials(const ramd_csr_s* A)
+{
+    const int np = A->wr_on[0] ? (A->n_rows + kWaveRows - 1) / kWaveRows
+                   : (A->dia || A->lr_on[0] || (A->sl && A->sl_on[0])) ? (A->n_rows + kBlockThreads - 1) / kBlockThreads
+                                                                        : A->rb[0].n;
+    return std::max(np, 1);
+}
+
 void launch_spmv_dot(const ramd_csr_s* A, bool values_fp32, const double* x, double* y, double* out_d,
-                     hipStream_t s)
+                     hipStream_t s, double* partials)
 {
     ramd_ctx ctx = A->ctx;
     const bool dia32 = values_fp32 && A->dia && A->dia->v32 && A->dia->v32_exact;
-    const int np = (A->dia || A->lr_on[0] || (A->sl && A->sl_on[0])) ? (A->n_rows + kBlockThreads - 1) / kBlockThreads : A->rb[0].n;
-    double* part = ctx_partials(ctx, (size_t)std::max(np, 1));
+    const int np = spmv_dot_partials(A);
+    double* part = partials ? partials : ctx_partials(ctx, (size_t)np);
     if (dia32)
         launch_stream<float, double>(A, nullptr, A->dia->v32, OP_SPMV, x, nullptr, y, 0.0, s, part);
     else

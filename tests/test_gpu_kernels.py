@@ -83,18 +83,35 @@ def sliced_cases():
         val += [v[j] for j in o]
         rp.append(len(col))
     S = K.Csr(rp, col, val, 900)
+    # rows longer than the wavefront kernel's 512-entry LDS chunk, each with
+    # its diagonal somewhere past the first chunk
+    L = random_csr(rng, 1500, 1500, 40, long_rows=(3, 700, 1499))
+    rp, col, val = [0], [], []
+    for i in range(L.n_rows):
+        c = list(L.col[L.rowptr[i]:L.rowptr[i + 1]])
+        v = list(L.val[L.rowptr[i]:L.rowptr[i + 1]])
+        if i not in c:
+            c.append(i)
+            v.append(50.0)
+        o = np.argsort(c)
+        col += [c[j] for j in o]
+        val += [v[j] for j in o]
+        rp.append(len(col))
+    SL = K.Csr(rp, col, val, 1500)
     return [("square_sorted", S), ("square_diag_first", K.move_diag(S)),
+            ("square_long_rows", SL), ("square_long_rows_diag_first", K.move_diag(SL)),
             ("rect", random_csr(rng, 700, 333, 9)),
             ("long_rows", random_csr(rng, 300, 9000, 4, long_rows=(5, 77, 299))),
             ("empty_rows", K.Csr([0, 0, 2, 2, 3], [1, 3, 0], [1.5, -2.0, 4.0], 4))]
 
 
-@pytest.mark.parametrize("layout", ["sliced", "sliced16", "lane_rows"])
+@pytest.mark.parametrize("layout", ["sliced", "sliced16", "lane_rows", "wave_rows"])
 @pytest.mark.parametrize("name,A", sliced_cases(), ids=[c[0] for c in sliced_cases()])
 def test_sliced_kernel_bit_exact(ctx, name, A, layout):
     """The one-lane-per-row kernels -- over the sliced form (64 rows per
     slice, padding dropped by a select) and straight over the CSR rows --
-    give the oracle's bits for every operation."""
+    and the wavefront-per-row kernel (rows longer than its 512-entry LDS
+    chunk included) give the oracle's bits for every operation."""
     rng = np.random.default_rng(32)
     x, b = rng.standard_normal(A.n_cols), rng.standard_normal(A.n_rows)
     D = to_dev(ctx, A)
@@ -106,8 +123,10 @@ def test_sliced_kernel_bit_exact(ctx, name, A, layout):
                 D.set_sliced(0, 2)
             return
         D.set_sliced(0, 2)
-    else:
+    elif layout == "lane_rows":
         D.set_lane_rows(0, True)
+    else:
+        D.set_wave_rows(0, True)
     y = ra.ParVector(ctx, A.n_rows)
     D.mult(dev(ctx, x), y)
     assert np.array_equal(y.numpy(), K.spmv(A, x))
@@ -123,6 +142,7 @@ def test_sliced_kernel_bit_exact(ctx, name, A, layout):
         assert np.array_equal(xj.numpy(), K.jacobi(A, x, b, 2, 2.0 / 3.0))
     D.set_sliced(0, False)
     D.set_lane_rows(0, False)
+    D.set_wave_rows(0, False)
     y3 = ra.ParVector(ctx, A.n_rows)
     D.mult(dev(ctx, x), y3)
     assert np.array_equal(y3.numpy(), y.numpy())

@@ -120,6 +120,20 @@ def test_hierarchy_matches_reference(problem):
             assert np.array_equal(ml.states(l), st), l
 
 
+def set_kernel(M, p, code):
+    """Selects the CSR kernel of row_caps' encoding (-1/-2 sliced, 1 lane
+    rows, 2 wavefront rows, a cap: the tuned row blocks) for precision p."""
+    M.set_wave_rows(p, False)
+    M.set_lane_rows(p, False)
+    M.set_sliced(p, False)
+    if code in (-1, -2):
+        M.set_sliced(p, int(-code))
+    elif code == 1:
+        M.set_lane_rows(p, True)
+    elif code == 2:
+        M.set_wave_rows(p, True)
+
+
 def test_vcycle_bit_exact(problem, ctx):
     ml, As, Ps = problem["ml"], problem["As"], problem["Ps"]
     H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=OMEGA)
@@ -205,9 +219,7 @@ def test_vcycle_sliced_bit_exact(problem, ctx, mode):
     finally:
         for l, A in mats:
             for p in (0, 1):
-                A.set_sliced(p, int(-before[l, 0, p]) if before[l, 0, p] < 0 else 0)
-                if before[l, 0, p] == 1:
-                    A.set_lane_rows(p, True)
+                set_kernel(A, p, before[l, 0, p])
     assert np.array_equal(ml.row_caps(), before)
 
 
@@ -245,10 +257,45 @@ def test_vcycle_lane_rows_bit_exact(problem, ctx):
     finally:
         for l, m, M in mats:
             for p in (0, 1):
-                if before[l, m, p] < 0:
-                    M.set_sliced(p, int(-before[l, m, p]))
-                else:
-                    M.set_lane_rows(p, bool(before[l, m, p] == 1))
+                set_kernel(M, p, before[l, m, p])
+    assert np.array_equal(ml.row_caps(), before)
+
+
+def test_vcycle_wave_rows_bit_exact(problem, ctx):
+    """Every CSR level operator and interpolation switched to the
+    wavefront-per-row kernel: FP64 and FP32 cycles still bit-identical to
+    the oracle.  The tuned choices are restored."""
+    ml, As, Ps = problem["ml"], problem["As"], problem["Ps"]
+    ml.enable_fp32()
+    before = ml.row_caps()
+    mats = []
+    for l in range(ml.num_levels):
+        A, P = ml.level(l)
+        if A.diagonals() == 0 and A.nnz > 0:
+            mats.append((l, 0, A))
+        if P is not None and P.nnz > 0:
+            mats.append((l, 1, P))
+    if not mats:
+        pytest.skip("no CSR operator")
+    for _, _, M in mats:
+        for p in (0, 1):
+            M.set_wave_rows(p, True)
+    assert all(np.all(ml.row_caps()[l, m, :] == 2) for l, m, _ in mats)
+    try:
+        H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=OMEGA)
+        n = As[0].n_rows
+        b = np.random.default_rng(26).standard_normal(n)
+        x = ra.ParVector.from_numpy(ctx, np.zeros(n))
+        ml.cycle(x, ra.ParVector.from_numpy(ctx, b), x_is_zero=True)
+        assert np.array_equal(x.numpy(), H.cycle(np.zeros(n), b))
+        b32 = b.astype(np.float32)
+        x32 = ra.ParVector(ctx, n, np.float32)
+        ml.cycle_fp32(x32, ra.ParVector.from_numpy(ctx, b32))
+        assert np.array_equal(x32.numpy(), H.cycle_fp32(np.zeros(n, np.float32), b32))
+    finally:
+        for l, m, M in mats:
+            for p in (0, 1):
+                set_kernel(M, p, before[l, m, p])
     assert np.array_equal(ml.row_caps(), before)
 
 
@@ -321,16 +368,46 @@ def test_fpcg_reaches_true_residual(problem, ctx, prec):
     xs = x.numpy()
     true_rel = np.linalg.norm(b - K.spmv(A, xs)) / np.linalg.norm(b)
     assert rel <= 1e-8 and true_rel <= 1.0000001e-8
-    # error against the FP64 oracle solution of the same system (the
-    # reference's PCG rules, run to 1e-14 in its units): the oracle solve
-    # itself is that accurate, and the device solution is within the
-    # solver tolerance's reach of it
+    # error against an FP64 oracle solve of the same system to 1e-12 (the
+    # oracle's V-cycle, bit-identical to the reference's, in the oracle's
+    # flexible PCG): the oracle solution is that accurate, and the device
+    # solution is within the solver tolerance's reach of it
     H = K.Hier(problem["As"], problem["Ps"], relax_type=0, sweeps=1, omega=OMEGA)
-    xo, _, _ = H.pcg(A, np.zeros(n), b, tol=1e-14, max_iter=500)
-    assert np.linalg.norm(b - K.spmv(A, xo)) <= 1e-10 * np.linalg.norm(b)
+    xo, ito, relo = K.MtHier(H, 4).fpcg(A, np.zeros(n), b, 1e-12, 500)
+    assert relo <= 1e-12
+    assert np.linalg.norm(b - K.spmv(A, xo)) <= 1.0000001e-12 * np.linalg.norm(b)
     assert np.linalg.norm(xo - xt) <= 1e-7 * np.linalg.norm(xt)
     assert np.linalg.norm(xs - xo) <= 1e-5 * np.linalg.norm(xo)
     assert np.linalg.norm(xs - xt) / np.linalg.norm(xt) < 1e-5
+    if prec == ra.PREC_FP64:
+        # the same algorithm in FP64: the oracle's iteration count to 1e-8
+        _, it8, _ = K.MtHier(H, 4).fpcg(A, np.zeros(n), b, 1e-8, 500)
+        assert abs(it8 - it) <= 1, (it8, it)
+
+
+@pytest.mark.parametrize("prec", [ra.PREC_FP64, ra.PREC_FP32])
+def test_fpcg_graph_replay_equals_eager(problem, ctx, prec):
+    """The FPCG iteration replayed from captured HIP graphs (first call
+    captures, second replays the cached graphs) gives exactly the eager
+    iterates: solution, iteration count and residual history."""
+    A = K.Csr(problem["rp"], problem["col"], problem["val"])
+    n = A.n_rows
+    b = np.random.default_rng(27).standard_normal(n)
+    Ad = ra.ParCSRMatrix.from_csr(ctx, A.rowptr, A.col, A.val)
+    out = []
+    x = ra.ParVector(ctx, n)
+    bd = ra.ParVector.from_numpy(ctx, b)
+    try:
+        for graphs in (False, True, True):
+            ctx.set_graphs(graphs)
+            x.set_const_value(0.0)
+            hist, it, rel = ra.FPCG(Ad, problem["ml"], x, bd, 1e-10, 300, prec)
+            out.append((x.numpy(), it, rel, np.array(hist)))
+    finally:
+        ctx.set_graphs(True)
+    for x, it, rel, hist in out[1:]:
+        assert it == out[0][1] and rel == out[0][2]
+        assert np.array_equal(hist, out[0][3]) and np.array_equal(x, out[0][0])
 
 
 def test_pcg_indefinite_reports_error(ctx):
@@ -420,7 +497,7 @@ def test_row_cap_autotuner_choices(problem):
     ml.enable_fp32()
     caps = ml.row_caps()
     assert caps.shape == (ml.num_levels, 3, 2)
-    assert set(np.unique(caps)) <= {-2, -1, 0, 1, 2048, 4096, 8192}  # -1/-2: sliced, 1: lane rows
+    assert set(np.unique(caps)) <= {-2, -1, 0, 1, 2, 2048, 4096, 8192}  # -1/-2 sliced, 1 lane, 2 wave rows
     assert not np.any(caps[:, :, 0] == 8192)
     assert np.all(caps[-1, 1:] == 0)  # no P, R at the coarsest level
 
