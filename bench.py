@@ -249,11 +249,16 @@ def main():
     per_launch_bytes = dom["bytes"] / dom["launches"]
     per_launch_s = dom["ms"] / 1e3 / dom["launches"]
     achieved = per_launch_bytes / per_launch_s / 1e9
-    kname = kernel_template(dom_name, dom["kernel"], ml.level(0)[0].diagonals())
+    # the outer product runs on the caller's matrix A (FP32 values, FP64
+    # vectors: pair 2); the cycle's operations on level 0 (pair 1)
+    outer_slot = dom_name.endswith("fp64")
+    dom_op = A if outer_slot else ml.level(0)[0]
+    staging = dom_op.dia_staging(2 if outer_slot else 1) if dom_op.diagonals() else None
+    kname = kernel_template(dom_name, dom["kernel"], ml.level(0)[0].diagonals(), staging)
     traffic = measured_traffic(m, kname)
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
                     frac=round(achieved / PEAK_HBM_GBS, 4), traffic=traffic, kernel=f"{dom['kernel']}:{dom_name}",
-                    kernel_symbol=kname, bytes_per_launch=per_launch_bytes,
+                    kernel_symbol=kname, staging=staging, bytes_per_launch=per_launch_bytes,
                     us_per_launch=round(per_launch_s * 1e6, 2))
     log(f"dominant kernel {dom_name}: {achieved:.0f} GB/s ({achieved / PEAK_HBM_GBS:.1%} of 8 TB/s)")
 
@@ -313,13 +318,21 @@ def main():
     ctx.close()
 
 
-def kernel_template(slot, kind, nd):
+def kernel_template(slot, kind, nd, staging=None):
     """Symbol of the kernel behind a timing slot of the FP32-form solve: the
-    FP64-vector slots there read the lossless FP32 values."""
+    FP64-vector slots there read the lossless FP32 values.  staging: the
+    stencil-diagonal variant (mode, planes, L, P) of the operator the slot
+    runs on (gather, LDS x window, or plane-march over the box grid)."""
     op = {"spmv": 0, "append": 1, "residual": 2, "jacobi": 3, "add_into": 4}[slot.rsplit("_", 1)[0]]
     vec = "float" if slot.endswith("fp32") else "double"
     if kind == "dia_kernel":
-        return f"ramd::dia_kernel<{op}, float, {vec}, {nd}>"
+        mode = staging[0] if staging else 0
+        if mode == 2:
+            L = staging[2]
+            tx = 64 if L % 64 == 0 else 32 if L % 32 == 0 else 16 if L % 16 == 0 else 64
+            return f"ramd::dia_box_kernel<{op}, float, {vec}, {nd}, {tx}>"
+        name = "dia_lds_kernel" if mode == 1 else "dia_kernel"
+        return f"ramd::{name}<{op}, float, {vec}, {nd}>"
     return f"ramd::stream_kernel<{op}, {vec}, {vec}>"
 
 

@@ -93,9 +93,17 @@ int ramd_csr_diagonals(ramd_csr A, int* nd);
 /* Kernel variant of the stencil-diagonal form for the precision pair pp
  * (0 FP64 values and vectors, 1 FP32/FP32, 2 FP32 values/FP64 vectors):
  * mode 0 gathers x from memory, 1 stages each workgroup's x window in LDS,
- * -1 lets the autotuner time both at the next hierarchy setup or solve.
- * Both variants give identical bits.  No reference counterpart. */
-int ramd_csr_set_dia_staging(ramd_csr A, int pp, int mode);
+ * 2 marches a workgroup's tile of a box-structured grid through `planes`
+ * planes (4..256; 0 keeps the current count) with the x planes in LDS
+ * (only when every offset is dz*P + dy*L + dx, |d| <= 1, of an L x (P/L) x
+ * (n/P) grid), -1 lets the autotuner time them at the next hierarchy setup
+ * or solve.  Every variant gives identical bits.  No reference counterpart. */
+int ramd_csr_set_dia_staging(ramd_csr A, int pp, int mode, int planes);
+/* The variant in use for the precision pair pp (mode as above, -1 not timed
+ * yet) and, for the plane-march kernel, its planes per workgroup and the
+ * grid's line and plane lengths L and P (0 when the offsets have no box
+ * geometry).  No reference counterpart. */
+int ramd_csr_dia_staging(ramd_csr A, int pp, int* mode, int* planes, int* L, int* P);
 /* CSR kernel of the operator for FP64 (prec 0) or FP32 (prec 1) vectors:
  * on = 1 builds and selects the sliced-row form (64 rows per slice, one
  * lane per row), 2 the same with 16-bit column offsets (col - row; square

@@ -217,10 +217,19 @@ class ParCSRMatrix:
         check(lib().ramd_galerkin(self.h, P.h, C.byref(h)))
         return ParCSRMatrix(self.ctx, h)
 
-    def set_dia_staging(self, pp, mode):
+    def set_dia_staging(self, pp, mode, planes=0):
         """Stencil-diagonal kernel variant for precision pair pp (0 FP64, 1 FP32,
-        2 FP32 values with FP64 vectors): 0 gather, 1 LDS-staged x, -1 autotune."""
-        check(lib().ramd_csr_set_dia_staging(self.h, int(pp), int(mode)))
+        2 FP32 values with FP64 vectors): 0 gather, 1 LDS-staged x window,
+        2 plane-march over a box grid (`planes` per workgroup, 0 keeps the
+        current count), -1 autotune.  Every variant gives the same bits."""
+        check(lib().ramd_csr_set_dia_staging(self.h, int(pp), int(mode), int(planes)))
+
+    def dia_staging(self, pp):
+        """(mode, planes per workgroup, L, P) of the stencil-diagonal kernel
+        for precision pair pp; L = P = 0 when the offsets have no box geometry."""
+        v = [C.c_int() for _ in range(4)]
+        check(lib().ramd_csr_dia_staging(self.h, int(pp), *[C.byref(t) for t in v]))
+        return tuple(t.value for t in v)
 
     def set_sliced(self, prec, on):
         """CSR kernel for FP64 (prec 0) or FP32 (prec 1) vectors: True (or 1)

@@ -84,7 +84,9 @@ _SIGS = {
     "ramd_csr_download": (P, PI32, PI32, PD),
     "ramd_csr_destroy": (P,),
     "ramd_csr_diagonals": (P, C.POINTER(C.c_int)),
-    "ramd_csr_set_dia_staging": (P, C.c_int, C.c_int),
+    "ramd_csr_set_dia_staging": (P, C.c_int, C.c_int, C.c_int),
+    "ramd_csr_dia_staging": (P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                             C.POINTER(C.c_int)),
     "ramd_csr_set_sliced": (P, C.c_int, C.c_int),
     "ramd_csr_set_lane_rows": (P, C.c_int, C.c_int),
     "ramd_csr_set_wave_rows": (P, C.c_int, C.c_int),

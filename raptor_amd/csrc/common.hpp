@@ -164,14 +164,31 @@ struct DiaWindow {
     int base[kDiaMax];
     int lofs[kDiaMax];
 };
+// Box geometry of a stencil-diagonal form: every offset is
+// dz * P + dy * L + dx with |dx|, |dy|, |dz| <= 1, P a multiple of L and n a
+// multiple of P (rows of an L x (P / L) x (n / P) grid in x-fastest order).
+// The plane-march kernel tiles rows TX x TY per workgroup and walks zc
+// planes; diagonal k reads x from the LDS plane dz[k] at lane offset rel[k].
+struct BoxArgs {
+    int dz[kDiaMax];   // plane step of diagonal k: -1, 0 or 1
+    int rel[kDiaMax];  // dy * (TX + 2) + dx of diagonal k
+};
+struct DiaBox {
+    int L = 0, P = 0;  // 0: the offsets have no box decomposition
+    int tx = 64;       // tile width (TY = kBlockThreads / tx)
+    BoxArgs args{};
+    int zc[3] = {16, 16, 16};  // planes per workgroup, per precision pair
+};
 struct Dia {
     int nd = 0;
     int center = -1;            // diagonal index of offset 0 (-1 if none)
     DiaArgs args{};
     DiaWindow window{};
+    DiaBox box{};
     // per (values, vectors) precision ([0] FP64/FP64, [1] FP32/FP32,
-    // [2] FP32/FP64): 1 = stage x in LDS, 0 = gather, -1 = not timed yet
-    // (gather).  Chosen by timing both; the result bits are the same.
+    // [2] FP32/FP64): 2 = plane-march with the x planes in LDS (box
+    // geometry only), 1 = stage the x window in LDS, 0 = gather, -1 = not
+    // timed yet (gather).  Chosen by timing; the result bits are the same.
     int lds_x[3] = {-1, -1, -1};
     unsigned* mask = nullptr;   // device, n
     double* v64 = nullptr;      // device, dia_slots<double>(nd) * n, dia_index layout
@@ -235,9 +252,11 @@ struct ramd_csr_s {
     // core/matrix.cpp:729-787); lets Jacobi read one column per row
     bool diag_first = false;
     // identity of the handle (never reused within a process) and a count of
-    // the device buffers built for it after creation (FP32 copies, the
-    // stencil-diagonal form): captured HIP graphs hold raw pointers to these
-    // buffers and are keyed on both
+    // the changes to its device forms and kernel choices after creation (FP32
+    // copies, the stencil-diagonal and sliced forms, row blocks, the kernel
+    // picked per precision, the FP32-value operator's exactness): captured
+    // HIP graphs hold raw pointers to these buffers and the kernels chosen
+    // at capture, and are keyed on both
     uint64_t uid = 0;
     uint32_t forms = 0;
 };
@@ -259,6 +278,17 @@ bool host_diag_first(int n_rows, const int* rowptr, const int* col);
 // one common diagonal order).  Returns whether it was built.
 constexpr int kDiaMinRows = 4096;
 bool dia_build(ramd_csr_s* A);
+// Box geometry of the offsets (sorted ascending) for n rows: fills b.L, b.P,
+// b.tx and b.args, or leaves b.L = 0 when there is none.
+void dia_box_detect(const long long* offs, int nd, int n, DiaBox& b);
+// Workgroups of the stencil-diagonal kernels of A for precision pair pp
+// (the number of per-workgroup partial sums of a fused product).
+int dia_grid(const ramd_csr_s* A, int pp);
+// Planes per workgroup of the plane-march kernel: at least this many.
+constexpr int kBoxMinPlanes = 4;
+constexpr int kBoxMaxPlanes = 256;
+// An upper bound of dia_grid over every kernel choice and precision pair.
+int dia_grid_bound(const ramd_csr_s* A);
 void dia_free(Dia* d);
 // Builds (once) A->trans, the CSR of A^T with rows in ascending source-row order.
 ramd_csr_s* csr_transpose(ramd_csr_s* A);
@@ -270,10 +300,11 @@ constexpr int kStreamCaps[3] = {2048, 4096, 8192};
 // blocks.  The cap changes only which workgroup owns a row, never the order
 // of a row's sum, so results are identical for every choice.
 void csr_autotune(ramd_csr_s* A, int prec);
-// For a matrix with the stencil-diagonal form whose x window fits kDiaWin:
-// times the gather and the LDS-staged kernels for the precision pair `pp`
-// (0 FP64 values/vectors, 1 FP32/FP32, 2 FP32 values/FP64 vectors) and keeps
-// the faster one (same result bits either way).  Timed once per pair.
+// For a matrix with the stencil-diagonal form: times the gather kernel, the
+// LDS x-window kernel (when the window fits kDiaWin) and the plane-march
+// kernel (box geometry; several planes per workgroup) for the precision pair
+// `pp` (0 FP64 values/vectors, 1 FP32/FP32, 2 FP32 values/FP64 vectors) and
+// keeps the fastest (same result bits for every choice).  Timed once per pair.
 void dia_autotune(ramd_csr_s* A, int pp);
 // New values for the same pattern (device array); refreshes every derived
 // copy (FP32 values, stencil-diagonal planes, the cached transpose).

@@ -237,6 +237,52 @@ __global__ void dia_fill_kernel(int n, int nd, int center, DiaArgs offs, const i
     if (f) atomicOr(flags, f);
 }
 
+void dia_box_detect(const long long* offs, int nd, int n, DiaBox& b)
+{
+    b = DiaBox();
+    // a decomposition of every offset as dz * P + dy * L + dx, |d.| <= 1
+    auto split = [&](long long o, long long L, long long P, int& dz, int& dy, int& dx) {
+        for (dz = -1; dz <= 1; dz++)
+            for (dy = -1; dy <= 1; dy++) {
+                const long long r = o - dz * P - dy * L;
+                if (r >= -1 && r <= 1) {
+                    dx = (int)r;
+                    return true;
+                }
+            }
+        return false;
+    };
+    std::vector<long long> cand;
+    for (int k = 0; k < nd; k++)
+        for (int e = -1; e <= 1; e++)
+            if (offs[k] + e >= 3) cand.push_back(offs[k] + e);
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    // the true grid: L the smallest line length, P the smallest plane size
+    // that is a multiple of L with at least two lines and divides n
+    for (long long L : cand)
+        for (long long P : cand) {
+            if (P < 2 * L || P % L || n % P || b.L) continue;
+            bool ok = true;
+            for (int k = 0; k < nd && ok; k++) {
+                int dz, dy, dx;
+                ok = split(offs[k], L, P, dz, dy, dx);
+            }
+            if (!ok) continue;
+            b.L = (int)L;
+            b.P = (int)P;
+        }
+    if (!b.L) return;
+    // tile width: the widest of 64, 32, 16 that divides the line, else 64
+    b.tx = b.L % 64 == 0 ? 64 : b.L % 32 == 0 ? 32 : b.L % 16 == 0 ? 16 : 64;
+    for (int k = 0; k < kDiaMax; k++) {
+        int dz = 0, dy = 0, dx = 0;
+        if (k < nd) split(offs[k], b.L, b.P, dz, dy, dx);
+        b.args.dz[k] = dz;
+        b.args.rel[k] = dy * (b.tx + 2) + dx;
+    }
+}
+
 // The stencil-diagonal form, built on the device from the matrix's device
 // CSR arrays (the host reads back only the distinct offsets and four flags).
 bool dia_build(ramd_csr_s* A)
@@ -327,6 +373,7 @@ bool dia_build(ramd_csr_s* A)
         w.win = tot;
     }
     d->v32_exact = exact32;
+    dia_box_detect(offs.data(), nd, n, d->box);
     try {
         const int ns = dia_slots<double>(nd);
         d->v64 = dalloc<double>((size_t)ns * n);
@@ -412,6 +459,7 @@ void sliced_free(ramd_csr_s* A)
     A->sl = nullptr;
     A->sl_on[0] = A->sl_on[1] = false;
     A->sl16[0] = A->sl16[1] = false;
+    A->forms++;
 }
 
 void sliced_build(ramd_csr_s* A)
@@ -428,6 +476,7 @@ void sliced_build(ramd_csr_s* A)
     }
     hipStream_t s = A->ctx->stream;
     Sliced* S = new Sliced;
+    A->forms++;
     S->n_slices = nq;
     S->entries = sptr[nq];
     A->sl = S;
@@ -485,6 +534,7 @@ void sliced_values(ramd_csr_s* A, int prec)
     RAMD_CHECK(A->sl, RAMD_E_STATE, "sliced form not built");
     Sliced* S = A->sl;
     const size_t m = (size_t)std::max<int64_t>(S->entries, 1);
+    A->forms++;
     if (prec == 0) {
         if (!S->v64) S->v64 = dalloc<double>(m);
         sliced_copy<double>(A, A->val, S->v64);
@@ -708,6 +758,8 @@ void csr_set_values(ramd_csr_s* A, const double* val_d)
         launch_reduce_final(ctx, part, g, ctx->red_result + 8);
         double inexact = 0;
         fetch_scalars(ctx, ctx->red_result + 8, 1, &inexact);
+        // the FP32-value operator is chosen only while it is exact
+        if (d->v32_exact != (inexact == 0.0)) A->forms++;
         d->v32_exact = inexact == 0.0;
         if (d->v32) dia_make_fp32(A);
     }
@@ -903,15 +955,35 @@ int ramd_csr_set_values(ramd_csr A, const double* val_d)
     });
 }
 
-int ramd_csr_set_dia_staging(ramd_csr A, int pp, int mode)
+int ramd_csr_set_dia_staging(ramd_csr A, int pp, int mode, int planes)
 {
     return guarded([&] {
-        RAMD_CHECK(A && pp >= 0 && pp < 3 && mode >= -1 && mode <= 1, RAMD_E_ARG,
+        RAMD_CHECK(A && pp >= 0 && pp < 3 && mode >= -1 && mode <= 2, RAMD_E_ARG,
                    "ramd_csr_set_dia_staging: bad argument");
         RAMD_CHECK(A->dia, RAMD_E_STATE, "ramd_csr_set_dia_staging: matrix has no stencil-diagonal form");
         RAMD_CHECK(mode != 1 || A->dia->window.win <= kDiaWin, RAMD_E_ARG,
                    "ramd_csr_set_dia_staging: x window of %d values exceeds %d", A->dia->window.win, kDiaWin);
+        RAMD_CHECK(mode != 2 || A->dia->box.L > 0, RAMD_E_ARG,
+                   "ramd_csr_set_dia_staging: the offsets have no box geometry");
+        RAMD_CHECK(planes == 0 || (mode == 2 && planes >= kBoxMinPlanes && planes <= kBoxMaxPlanes), RAMD_E_ARG,
+                   "ramd_csr_set_dia_staging: %d planes per workgroup (%d..%d, plane-march only)", planes,
+                   kBoxMinPlanes, kBoxMaxPlanes);
         A->dia->lds_x[pp] = mode;
+        if (planes) A->dia->box.zc[pp] = planes;
+        A->forms++;
+    });
+}
+
+int ramd_csr_dia_staging(ramd_csr A, int pp, int* mode, int* planes, int* L, int* P)
+{
+    return guarded([&] {
+        RAMD_CHECK(A && pp >= 0 && pp < 3 && mode && planes && L && P, RAMD_E_ARG,
+                   "ramd_csr_dia_staging: bad argument");
+        RAMD_CHECK(A->dia, RAMD_E_STATE, "ramd_csr_dia_staging: matrix has no stencil-diagonal form");
+        *mode = A->dia->lds_x[pp];
+        *planes = A->dia->box.zc[pp];
+        *L = A->dia->box.L;
+        *P = A->dia->box.P;
     });
 }
 
@@ -934,6 +1006,7 @@ int ramd_csr_set_sliced(ramd_csr A, int prec, int on)
         A->sl_on[prec] = on != 0;
         A->sl16[prec] = on == 2;
         if (on) A->lr_on[prec] = A->wr_on[prec] = false;
+        A->forms++;
     });
 }
 
@@ -946,6 +1019,7 @@ int ramd_csr_set_lane_rows(ramd_csr A, int prec, int on)
         RAMD_CHECK(prec == 0 || A->val32 || !on, RAMD_E_STATE, "ramd_csr_set_lane_rows: FP32 values were not built");
         A->lr_on[prec] = on != 0;
         if (on) A->sl_on[prec] = A->sl16[prec] = A->wr_on[prec] = false;
+        A->forms++;
     });
 }
 
@@ -958,6 +1032,7 @@ int ramd_csr_set_wave_rows(ramd_csr A, int prec, int on)
         RAMD_CHECK(prec == 0 || A->val32 || !on, RAMD_E_STATE, "ramd_csr_set_wave_rows: FP32 values were not built");
         A->wr_on[prec] = on != 0;
         if (on) A->sl_on[prec] = A->sl16[prec] = A->lr_on[prec] = false;
+        A->forms++;
     });
 }
 

@@ -670,6 +670,165 @@ __global__ __launch_bounds__(kBlockThreads) void dia_lds_kernel(int n, DiaArgs a
     }
 }
 
+// Plane-march form for a box geometry (DiaBox): a workgroup owns a TX x TY
+// tile of (x, y) positions and walks zc consecutive planes.  For every plane
+// it stages the tile's x values with a one-value halo, (TX + 2) x (TY + 2),
+// in a ring of four LDS planes, so the 27 reads of a row come from LDS and
+// each x value is loaded from memory about (1 + 2 / TX + 2 / TY) times per
+// operation instead of once per diagonal.  The staged value of tile position
+// (i, j) on plane z' is x at clamp(z' P + (y0 + j - 1) L + x0 + i - 1) into
+// [0, n), which is x at dia_col(row, off) for every row of the tile and every
+// box offset, so each row reads exactly what dia_kernel gathers and runs its
+// operation sequence: the same bits.  One barrier per plane: plane z + 1 is
+// written into slot (z + 1) & 3 while slots z - 1 .. z + 1 are read; the
+// next plane's loads are in flight during the current plane's rows.
+template <int OP, typename V, typename X, int ND, int TX>
+__global__ __launch_bounds__(kBlockThreads) void dia_box_kernel(int n, int L, int P, int nxt, int nyt, int zc,
+                                                               DiaArgs a, BoxArgs bx, int nd, int center,
+                                                               const unsigned* __restrict__ mask,
+                                                               const V* __restrict__ val, const X* __restrict__ x,
+                                                               const X* __restrict__ b, X* __restrict__ y, X omega,
+                                                               double* __restrict__ dotp)
+{
+    constexpr int TY = kBlockThreads / TX;
+    constexpr int RW = TX + 2;
+    constexpr int TILE = RW * (TY + 2);
+    constexpr int PL = (TILE + 7) / 8 * 8;
+    constexpr int NLD = (TILE + kBlockThreads - 1) / kBlockThreads;
+    __shared__ X xs[4 * PL];
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int xt = blk % nxt, yt = (blk / nxt) % nyt, zt = blk / (nxt * nyt);
+    const int x0 = xt * TX, y0 = yt * TY, z0 = zt * zc;
+    const int nz = n / P, ny = P / L;
+    const int z1 = min(z0 + zc, nz);
+    const int tid = threadIdx.x;
+    const int tx = tid % TX, ty = tid / TX;
+    const bool col_ok = x0 + tx < L && y0 + ty < ny;
+    const int lb = (ty + 1) * RW + tx + 1;  // the lane's own position in a plane
+    // tile positions this thread stages, as offsets from the plane's start
+    int goff[NLD];
+#pragma unroll
+    for (int i = 0; i < NLD; i++) {
+        const int e = tid + i * kBlockThreads;
+        goff[i] = (y0 + e / RW - 1) * L + x0 + e % RW - 1;
+    }
+    auto load_plane = [&](int zp, X (&r)[NLD]) {
+#pragma unroll
+        for (int i = 0; i < NLD; i++) {
+            if (tid + i * kBlockThreads < TILE) {
+                int g = zp * P + goff[i];
+                g = g < 0 ? 0 : (g >= n ? n - 1 : g);
+                r[i] = dia_gather(x, g);
+            }
+        }
+    };
+    auto store_plane = [&](int zp, const X (&r)[NLD]) {
+        X* dst = xs + (zp & 3) * PL;
+#pragma unroll
+        for (int i = 0; i < NLD; i++)
+            if (tid + i * kBlockThreads < TILE) dst[tid + i * kBlockThreads] = r[i];
+    };
+    X nx[NLD];
+    load_plane(z0 - 1, nx);
+    store_plane(z0 - 1, nx);
+    load_plane(z0, nx);
+    store_plane(z0, nx);
+    load_plane(z0 + 1, nx);
+    double dsum = 0.0;
+    constexpr int CH = dia_chunk<V>();
+    const int nsl = dia_slots<V>(nd);
+    for (int z = z0; z < z1; z++) {
+        store_plane(z + 1, nx);
+        if (z + 2 <= z1) load_plane(z + 2, nx);
+        const int row = z * P + (y0 + ty) * L + x0 + tx;
+        const int vrow = col_ok ? row : 0;
+        // the row's mask and first value chunk are in flight across the barrier
+        const unsigned m = col_ok ? mask[vrow] : 0u;
+        V vv0[CH];
+        dia_load<V, CH, ND>(val, 0, vrow, n, nsl, vv0);
+        __syncthreads();
+        X acc = (OP == OP_RESIDUAL && col_ok) ? b[vrow] : X(0);
+        X diag = 0;
+#pragma unroll
+        for (int c0 = 0; c0 < ND; c0 += CH) {
+            V vv[CH];
+            if (c0 == 0) {
+#pragma unroll
+                for (int t = 0; t < CH; t++) vv[t] = vv0[t];
+            } else {
+                dia_load<V, CH, ND>(val, c0, vrow, n, nsl, vv);
+            }
+#pragma unroll
+            for (int t = 0; t < CH; t++) {
+                const int kk = c0 + t;
+                if (kk < ND) {
+                    const bool live = (ND == kDiaMax) ? (kk < nd) : true;
+                    const int k = live ? a.ord[kk] : 0;
+                    const bool on = live && ((m >> k) & 1u);
+                    const X v = X(vv[t]);
+                    // uniform part of the LDS address: the plane's slot and the offset
+                    const int so = ((z + bx.dz[k] + 4) & 3) * PL + bx.rel[k];
+                    const X p = v * xs[lb + so];
+                    if (OP == OP_JACOBI) {
+                        if (k == center) diag = on ? v : diag;
+                        else acc = on ? acc + p : acc;
+                    } else if (OP == OP_RESIDUAL) {
+                        acc = on ? acc - p : acc;
+                    } else {
+                        acc = on ? acc + p : acc;
+                    }
+                }
+            }
+        }
+        if (col_ok) {
+            if (OP == OP_SPMV || OP == OP_RESIDUAL) {
+                y[row] = acc;
+            } else if (OP == OP_APPEND) {
+                y[row] += acc;
+            } else if (OP == OP_ADD_INTO) {
+                y[row] = b[row] + acc;
+            } else {
+                const X xo = xs[(z & 3) * PL + lb];
+                y[row] = (m != 0u && fabs(diag) > X(1e-16)) ? ((X(1.0) - omega) * xo) + (omega * ((b[row] - acc) / diag))
+                                                           : xo;
+            }
+            if (OP == OP_SPMV && dotp) dsum += (double)xs[(z & 3) * PL + lb] * (double)acc;
+        }
+    }
+    if (OP == OP_SPMV && dotp) {
+        const double v = block_sum(dsum);
+        if (tid == 0) dotp[blk] = v;
+    }
+}
+
+int dia_grid(const ramd_csr_s* A, int pp)
+{
+    const Dia* d = A->dia;
+    const int n = A->n_rows;
+    if (d->lds_x[pp] == 2 && d->box.L > 0) {
+        const DiaBox& bx = d->box;
+        const int ty = kBlockThreads / bx.tx;
+        const int nxt = (bx.L + bx.tx - 1) / bx.tx, nyt = (bx.P / bx.L + ty - 1) / ty;
+        const int nzt = (n / bx.P + bx.zc[pp] - 1) / bx.zc[pp];
+        return std::max(1, nxt * nyt * nzt);
+    }
+    return std::max(1, (n + kBlockThreads - 1) / kBlockThreads);
+}
+
+int dia_grid_bound(const ramd_csr_s* A)
+{
+    const Dia* d = A->dia;
+    const int n = A->n_rows;
+    int np = (n + kBlockThreads - 1) / kBlockThreads;
+    if (d->box.L > 0) {
+        const DiaBox& bx = d->box;
+        const int ty = kBlockThreads / bx.tx;
+        const int nxt = (bx.L + bx.tx - 1) / bx.tx, nyt = (bx.P / bx.L + ty - 1) / ty;
+        np = std::max(np, nxt * nyt * ((n / bx.P + kBoxMinPlanes - 1) / kBoxMinPlanes));
+    }
+    return std::max(np, 1);
+}
+
 template <int OP, typename V, typename X>
 static void launch_dia_op(const ramd_csr_s* A, const V* val, const X* x, const X* b, X* y, X omega, hipStream_t s,
                           double* dotp)
@@ -678,7 +837,29 @@ static void launch_dia_op(const ramd_csr_s* A, const V* val, const X* x, const X
     const int n = A->n_rows;
     dim3 grid((n + kBlockThreads - 1) / kBlockThreads), block(kBlockThreads);
     const int pp = sizeof(X) == 4 ? 1 : (sizeof(V) == 4 ? 2 : 0);
-    if (d->lds_x[pp] == 1 && d->window.win <= kDiaWin) {
+    if (d->lds_x[pp] == 2 && d->box.L > 0) {
+        const DiaBox& bx = d->box;
+        const int ty = kBlockThreads / bx.tx;
+        const int nxt = (bx.L + bx.tx - 1) / bx.tx, nyt = (bx.P / bx.L + ty - 1) / ty;
+        const int zc = bx.zc[pp];
+        const dim3 bgrid((unsigned)dia_grid(A, pp));
+#define DIA_BOX(NDV, TXV)                                                                                            \
+    hipLaunchKernelGGL((dia_box_kernel<OP, V, X, NDV, TXV>), bgrid, block, 0, s, n, bx.L, bx.P, nxt, nyt, zc, d->args, \
+                       bx.args, d->nd, d->center, d->mask, val, x, b, y, omega, dotp)
+#define DIA_BOX_TX(NDV)              \
+    if (bx.tx == 64) DIA_BOX(NDV, 64); \
+    else if (bx.tx == 32) DIA_BOX(NDV, 32); \
+    else DIA_BOX(NDV, 16)
+        if (d->nd == 27) {
+            DIA_BOX_TX(27);
+        } else if (d->nd == 7) {
+            DIA_BOX_TX(7);
+        } else {
+            DIA_BOX_TX(kDiaMax);
+        }
+#undef DIA_BOX_TX
+#undef DIA_BOX
+    } else if (d->lds_x[pp] == 1 && d->window.win <= kDiaWin) {
 #define DIA_LDS(NDV)                                                                                            \
     hipLaunchKernelGGL((dia_lds_kernel<OP, V, X, NDV>), grid, block, 0, s, n, d->args, d->window, d->nd, d->center, \
                        d->mask, val, x, b, y, omega, dotp)
@@ -940,6 +1121,7 @@ void csr_autotune(ramd_csr_s* A, int prec)
     ramd_ctx ctx = A->ctx;
     hipStream_t s = ctx->stream;
     RAMD_HIP(hipStreamSynchronize(s));
+    A->forms++;  // kernel choice and row blocks change
     if (A->rb[prec].d != A->blocks) {  // tuned before: start from the default blocks
         dfree(A->rb[prec].d);
         A->rb[prec] = {kStreamCap, A->n_blocks, A->blocks};
@@ -1087,7 +1269,7 @@ void csr_autotune(ramd_csr_s* A, int prec)
 void dia_autotune(ramd_csr_s* A, int pp)
 {
     Dia* d = A->dia;
-    if (!d || d->window.win > kDiaWin || A->n_rows == 0 || d->lds_x[pp] >= 0) return;
+    if (!d || A->n_rows == 0 || d->lds_x[pp] >= 0) return;
     RAMD_CHECK(pp == 0 || d->v32, RAMD_E_STATE, "dia_autotune: FP32 values were not built");
     ramd_ctx ctx = A->ctx;
     hipStream_t s = ctx->stream;
@@ -1095,31 +1277,45 @@ void dia_autotune(ramd_csr_s* A, int pp)
     const size_t bytes = w * (size_t)A->n_rows;
     struct Scratch {
         void *x = nullptr, *b = nullptr, *y = nullptr;
+        double* part = nullptr;
         ~Scratch()
         {
             dfree(x);
             dfree(b);
             dfree(y);
+            dfree(part);
         }
     } t;
     t.x = dmalloc(bytes);
     t.b = dmalloc(bytes);
     t.y = dmalloc(bytes);
+    t.part = dalloc<double>((size_t)dia_grid_bound(A));
     RAMD_HIP(hipMemsetAsync(t.x, 0, bytes, s));
     RAMD_HIP(hipMemsetAsync(t.b, 0, bytes, s));
     hipEvent_t e0, e1;
     RAMD_HIP(hipEventCreate(&e0));
     RAMD_HIP(hipEventCreate(&e1));
-    float ms[2] = {0, 0};
-    for (int mode = 0; mode < 2; mode++) {
-        d->lds_x[pp] = mode;
+    // candidates: (mode, planes per workgroup)
+    std::vector<std::pair<int, int>> cands = {{0, 0}};
+    if (d->window.win <= kDiaWin) cands.push_back({1, 0});
+    if (d->box.L > 0)
+        for (int zc : {4, 8, 16, 32})
+            if (zc <= std::max(kBoxMinPlanes, A->n_rows / d->box.P)) cands.push_back({2, zc});
+    float best = 1e30f;
+    std::pair<int, int> pick = cands[0];
+    const int zc0 = d->box.zc[pp];
+    for (const auto& c : cands) {
+        d->lds_x[pp] = c.first;
+        if (c.first == 2) d->box.zc[pp] = c.second;
+        // the operation each pair runs most: the cycle's residual, the outer
+        // product with its fused dot
         auto run = [&] {
             if (pp == 1)
                 launch_dia<float, float>(A, d->v32, OP_RESIDUAL, (const float*)t.x, (const float*)t.b, (float*)t.y,
                                          0.f, s, nullptr);
             else if (pp == 2)
                 launch_dia<float, double>(A, d->v32, OP_SPMV, (const double*)t.x, (const double*)t.b, (double*)t.y,
-                                          0.0, s, nullptr);
+                                          0.0, s, t.part);
             else
                 launch_dia<double, double>(A, d->v64, OP_RESIDUAL, (const double*)t.x, (const double*)t.b,
                                            (double*)t.y, 0.0, s, nullptr);
@@ -1129,11 +1325,18 @@ void dia_autotune(ramd_csr_s* A, int pp)
         for (int r = 0; r < 3; r++) run();
         RAMD_HIP(hipEventRecord(e1, s));
         RAMD_HIP(hipEventSynchronize(e1));
-        RAMD_HIP(hipEventElapsedTime(&ms[mode], e0, e1));
+        float ms = 0;
+        RAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            pick = c;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    d->lds_x[pp] = ms[1] < ms[0] ? 1 : 0;
+    d->lds_x[pp] = pick.first;
+    d->box.zc[pp] = pick.first == 2 ? pick.second : zc0;
+    A->forms++;
 }
 
 double* ctx_partials(ramd_ctx ctx, size_t count)
@@ -1151,9 +1354,10 @@ double* ctx_partials(ramd_ctx ctx, size_t count)
 
 int spmv_dot_partials(const ramd_csr_s* A)
 {
+    if (A->dia) return dia_grid_bound(A);  // every kernel choice of either value precision
     const int np = A->wr_on[0] ? (A->n_rows + kWaveRows - 1) / kWaveRows
-                   : (A->dia || A->lr_on[0] || (A->sl && A->sl_on[0])) ? (A->n_rows + kBlockThreads - 1) / kBlockThreads
-                                                                        : A->rb[0].n;
+                   : (A->lr_on[0] || (A->sl && A->sl_on[0])) ? (A->n_rows + kBlockThreads - 1) / kBlockThreads
+                                                             : A->rb[0].n;
     return std::max(np, 1);
 }
 
@@ -1162,7 +1366,7 @@ void launch_spmv_dot(const ramd_csr_s* A, bool values_fp32, const double* x, dou
 {
     ramd_ctx ctx = A->ctx;
     const bool dia32 = values_fp32 && A->dia && A->dia->v32 && A->dia->v32_exact;
-    const int np = spmv_dot_partials(A);
+    const int np = A->dia ? dia_grid(A, dia32 ? 2 : 0) : spmv_dot_partials(A);
     double* part = partials ? partials : ctx_partials(ctx, (size_t)np);
     if (dia32)
         launch_stream<float, double>(A, nullptr, A->dia->v32, OP_SPMV, x, nullptr, y, 0.0, s, part);

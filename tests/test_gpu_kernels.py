@@ -148,19 +148,33 @@ def test_sliced_kernel_bit_exact(ctx, name, A, layout):
     assert np.array_equal(y3.numpy(), y.numpy())
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode,planes", [(0, 0), (1, 0), (2, 4), (2, 7)])
 @pytest.mark.parametrize("name,gen", [("aniso64", lambda: ra.gallery_aniso2d(64, 0.001, math.pi / 8)),
                                       ("lap27_20", lambda: ra.gallery_laplace27(20)),
-                                      ("fv7_18", lambda: ra.gallery_fv7(18, seed=2))])
-def test_dia_staging_variants_bit_exact(ctx, name, gen, mode):
-    """Both kernels of the stencil-diagonal form (x gathered, x window staged
-    in LDS), with FP64 values and with the exact FP32 copy, give the CSR
-    oracle's bits for every operation."""
+                                      ("lap27_16", lambda: ra.gallery_laplace27(16)),
+                                      ("fv7_18", lambda: ra.gallery_fv7(18, seed=2)),
+                                      ("fv7_32", lambda: ra.gallery_fv7(32, seed=2))])
+def test_dia_staging_variants_bit_exact(ctx, name, gen, mode, planes):
+    """Every kernel of the stencil-diagonal form (x gathered, x window staged
+    in LDS, plane-march over the box grid with 4 or 7 planes per workgroup),
+    with FP64 values and with the exact FP32 copy, gives the CSR oracle's
+    bits for every operation.  The 2-D operator has no box geometry, so the
+    plane-march kernel is refused for it."""
     A = K.move_diag(K.Csr(*gen()))
     D = to_dev(ctx, A)
     assert D.diagonals() > 0
+    if mode == 2:
+        _, _, L, P = D.dia_staging(0)
+        if name.startswith("aniso"):
+            assert L == 0
+            with pytest.raises(ra.RamdError):
+                D.set_dia_staging(0, 2, planes)
+            return
+        m = round(A.n_rows ** (1 / 3))
+        assert (L, P) == (m, m * m)
     for pp in (0, 2):
-        D.set_dia_staging(pp, mode)
+        D.set_dia_staging(pp, mode, planes)
+        assert D.dia_staging(pp)[0] == mode
     rng = np.random.default_rng(14)
     x, b = rng.standard_normal(A.n_rows), rng.standard_normal(A.n_rows)
     y = ra.ParVector(ctx, A.n_rows)
