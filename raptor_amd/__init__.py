@@ -97,13 +97,16 @@ class Context:
 
 
 class ParVector:
-    """Device FP64 vector of length n (core/par_vector.hpp:44-176)."""
+    """Device FP64 vector of length n (core/par_vector.hpp:44-176), zero on
+    construction like the reference's (core/vector.hpp:54-77: the values are
+    a resized std::vector)."""
 
     def __init__(self, ctx, n, dtype=np.float64):
         self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
         self.ptr = C.c_void_p()
         check(lib().ramd_malloc(ctx.h, max(1, self.n) * self.dtype.itemsize, C.byref(self.ptr)))
         ctx._adopt(self)
+        check(lib().ramd_memset_zero(ctx.h, self.ptr, max(1, self.n) * self.dtype.itemsize))
 
     @classmethod
     def from_numpy(cls, ctx, a):

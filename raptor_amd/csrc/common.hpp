@@ -177,7 +177,7 @@ struct DiaBox {
     int L = 0, P = 0;  // 0: the offsets have no box decomposition
     int tx = 64;       // tile width (TY = kBlockThreads / tx)
     BoxArgs args{};
-    int zc[3] = {16, 16, 16};  // planes per workgroup, per precision pair
+    int zc[3] = {8, 8, 8};  // planes per workgroup, per precision pair
 };
 struct Dia {
     int nd = 0;
@@ -286,10 +286,15 @@ void dia_box_detect(const long long* offs, int nd, int n, DiaBox& b);
 int dia_grid(const ramd_csr_s* A, int pp);
 // Planes per workgroup of the plane-march kernel: at least this many.
 constexpr int kBoxMinPlanes = 4;
+constexpr int kBoxPlanes = 8;  // the autotuner's choice
 constexpr int kBoxMaxPlanes = 256;
 // An upper bound of dia_grid over every kernel choice and precision pair.
 int dia_grid_bound(const ramd_csr_s* A);
 void dia_free(Dia* d);
+// Row pointers rp[0..n] from row counts cnt[0..n) on the device (exclusive
+// prefix sum in int64; rp[n] is the total), one 8-byte read of the total;
+// throws RAMD_E_ARG naming `what` when it exceeds 2^31 - 1.
+int64_t device_rowptr(ramd_ctx ctx, const int* cnt, int n, int* rp, const char* what);
 // Builds (once) A->trans, the CSR of A^T with rows in ascending source-row order.
 ramd_csr_s* csr_transpose(ramd_csr_s* A);
 void csr_make_fp32(ramd_csr_s* A);

@@ -607,6 +607,116 @@ __global__ void transpose_sort_kernel(int m, const int* __restrict__ tp, int* __
     }
 }
 
+// ---------------------------------------------------------------------------
+// Row pointers from row counts on the device: an exclusive prefix sum in
+// int64, 4096 counts per workgroup (per-workgroup sums, one workgroup scans
+// the sums, every workgroup then scans its counts from its offset).
+// ---------------------------------------------------------------------------
+constexpr int kScanPer = 16;
+constexpr int kScanItems = kBlockThreads * kScanPer;
+
+__device__ __forceinline__ int64_t block_exclusive_scan64(int64_t v, int64_t* lds, int64_t* total)
+{
+    const int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kBlockThreads; off <<= 1) {
+        const int64_t a = t >= off ? lds[t - off] : 0;
+        __syncthreads();
+        lds[t] += a;
+        __syncthreads();
+    }
+    const int64_t incl = lds[t];
+    if (total) *total = lds[kBlockThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(kBlockThreads) void scan_sums_kernel(const int* __restrict__ cnt, int n,
+                                                                  int64_t* __restrict__ bsum)
+{
+    __shared__ int64_t lds[kBlockThreads];
+    const int64_t base = (int64_t)blockIdx.x * kScanItems;
+    int64_t s = 0;
+    for (int j = 0; j < kScanPer; j++) {
+        const int64_t i = base + j * kBlockThreads + threadIdx.x;
+        if (i < n) s += cnt[i];
+    }
+    int64_t tot;
+    block_exclusive_scan64(s, lds, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// in place: bsum[b] <- sum of bsum[0..b); total <- sum of all
+__global__ __launch_bounds__(kBlockThreads) void scan_offsets_kernel(int64_t* __restrict__ bsum, int nb,
+                                                                     int64_t* __restrict__ total)
+{
+    __shared__ int64_t lds[kBlockThreads];
+    int64_t carry = 0;
+    for (int c0 = 0; c0 < nb; c0 += kBlockThreads) {
+        const int b = c0 + threadIdx.x;
+        const int64_t v = b < nb ? bsum[b] : 0;
+        int64_t tot;
+        const int64_t ex = block_exclusive_scan64(v, lds, &tot);
+        if (b < nb) bsum[b] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kBlockThreads) void scan_write_kernel(const int* __restrict__ cnt, int n,
+                                                                   const int64_t* __restrict__ boff,
+                                                                   const int64_t* __restrict__ total,
+                                                                   int* __restrict__ rp)
+{
+    __shared__ int vals[kScanItems];
+    __shared__ int64_t lds[kBlockThreads];
+    const int64_t base = (int64_t)blockIdx.x * kScanItems;
+    for (int j = 0; j < kScanPer; j++) {
+        const int k = j * kBlockThreads + threadIdx.x;
+        vals[k] = base + k < n ? cnt[base + k] : 0;
+    }
+    __syncthreads();
+    int64_t s = 0;
+    for (int j = 0; j < kScanPer; j++) s += vals[threadIdx.x * kScanPer + j];
+    int64_t run = boff[blockIdx.x] + block_exclusive_scan64(s, lds, nullptr);
+    // this thread's counts -> their row pointers, in LDS, then written out
+    int* own = vals + threadIdx.x * kScanPer;
+    for (int j = 0; j < kScanPer; j++) {
+        const int c = own[j];
+        own[j] = (int)run;
+        run += c;
+    }
+    __syncthreads();
+    for (int j = 0; j < kScanPer; j++) {
+        const int k = j * kBlockThreads + threadIdx.x;
+        if (base + k < n) rp[base + k] = vals[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) rp[n] = (int)*total;
+}
+
+int64_t device_rowptr(ramd_ctx ctx, const int* cnt, int n, int* rp, const char* what)
+{
+    hipStream_t s = ctx->stream;
+    const int nb = std::max(1, (int)(((int64_t)n + kScanItems - 1) / kScanItems));
+    int64_t* buf = dalloc<int64_t>((size_t)nb + 1);
+    int64_t total = 0;
+    try {
+        hipLaunchKernelGGL(scan_sums_kernel, dim3(nb), dim3(kBlockThreads), 0, s, cnt, n, buf);
+        hipLaunchKernelGGL(scan_offsets_kernel, dim3(1), dim3(kBlockThreads), 0, s, buf, nb, buf + nb);
+        hipLaunchKernelGGL(scan_write_kernel, dim3(nb), dim3(kBlockThreads), 0, s, cnt, n, buf, buf + nb, rp);
+        RAMD_HIP(hipGetLastError());
+        RAMD_HIP(hipMemcpyAsync(&total, buf + nb, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        RAMD_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        dfree(buf);
+        throw;
+    }
+    dfree(buf);
+    RAMD_CHECK(total <= INT32_MAX, RAMD_E_ARG, "%s exceeds 2^31 stored entries", what);
+    return total;
+}
+
 // longest column the per-thread insertion sort takes (longer: host sort)
 constexpr int kTransposeMaxCol = 2048;
 

@@ -483,8 +483,8 @@ static void hier_free(ramd_hier_s* h)
     if (h->ctx) (void)hipStreamSynchronize(h->ctx->stream);
     block_free(h->blk);
     if (h->fg) {
-        fpcg_graphs_free(h->fg);
-        delete h->fg;
+        for (int i = 0; i < kFpcgGraphSets; i++) fpcg_graphs_free(&h->fg[i]);
+        delete[] h->fg;
     }
     dfree(h->fpart);
     h->spans.release();
@@ -992,7 +992,8 @@ static double* fpcg_partials(ramd_hier_s* h, size_t count)
 {
     if (h->fpart_cap < count) {
         RAMD_HIP(hipStreamSynchronize(h->ctx->stream));
-        fpcg_graphs_free(h->fg);
+        if (h->fg)
+            for (int i = 0; i < kFpcgGraphSets; i++) fpcg_graphs_free(&h->fg[i]);
         dfree(h->fpart);
         h->fpart = dalloc<double>(count);
         h->fpart_cap = count;
@@ -1068,11 +1069,18 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
         RAMD_HIP(hipGetLastError());
     };
     const bool use_graphs = ctx->graphs && ctx->prof_min_nnz < 0 && !h->track;
+    FpcgGraphs* fg = nullptr;
     if (use_graphs) {
-        if (!h->fg) h->fg = new FpcgGraphs();
-        FpcgGraphs* fg = h->fg;
+        if (!h->fg) h->fg = new FpcgGraphs[kFpcgGraphSets];
         const uint64_t key = forms_key(h, A);
-        if (fg->a_uid != A->uid || fg->key != key || fg->prec != cycle_prec || fg->x != x_d || fg->part != part) {
+        for (int i = 0; i < kFpcgGraphSets && !fg; i++) {
+            FpcgGraphs& e = h->fg[i];
+            if (e.a_uid == A->uid && e.key == key && e.prec == cycle_prec && e.x == x_d && e.part == part) fg = &e;
+        }
+        if (!fg) {  // the least recently used entry is captured anew
+            fg = &h->fg[0];
+            for (int i = 1; i < kFpcgGraphSets; i++)
+                if (h->fg[i].used < fg->used) fg = &h->fg[i];
             RAMD_HIP(hipStreamSynchronize(s));
             fpcg_graphs_free(fg);
             fg->a_uid = A->uid;
@@ -1081,6 +1089,7 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
             fg->x = x_d;
             fg->part = part;
         }
+        fg->used = ++h->fg_tick;
     }
     const double bnorm = norm2_64(ctx, n, b_d);
     const double scale = bnorm > 0 ? bnorm : 1.0;
@@ -1113,8 +1122,8 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
         while (it < max_iter) {
             const int par = cur == 2 ? 0 : 1;
             if (use_graphs) {
-                if (!h->fg->gA[par]) h->fg->gA[par] = capture_graph(s, [&] { stepA(cur); });
-                RAMD_HIP(hipGraphLaunch(h->fg->gA[par], s));
+                if (!fg->gA[par]) fg->gA[par] = capture_graph(s, [&] { stepA(cur); });
+                RAMD_HIP(hipGraphLaunch(fg->gA[par], s));
             } else {
                 stepA(cur);
             }
@@ -1156,8 +1165,8 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
             // place) replays the captured step; the others run eagerly
             const bool plain = !x_done && z_first == fuse_first;
             if (use_graphs && plain) {
-                if (!h->fg->gB[par]) h->fg->gB[par] = capture_graph(s, [&] { stepB(cur, nxt, x_d, fuse_first); });
-                RAMD_HIP(hipGraphLaunch(h->fg->gB[par], s));
+                if (!fg->gB[par]) fg->gB[par] = capture_graph(s, [&] { stepB(cur, nxt, x_d, fuse_first); });
+                RAMD_HIP(hipGraphLaunch(fg->gB[par], s));
             } else {
                 stepB(cur, nxt, x_done ? nullptr : x_d, z_first);
             }

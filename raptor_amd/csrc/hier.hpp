@@ -122,7 +122,11 @@ struct FpcgGraphs {
     int prec = -1;
     const double* x = nullptr;
     const double* part = nullptr;
+    uint64_t used = 0;  // last use (least recently used entry is replaced)
 };
+// captured iterations kept per hierarchy: solves that alternate between
+// solution vectors (a forward and an adjoint solve) each keep theirs
+constexpr int kFpcgGraphSets = 4;
 void fpcg_graphs_free(FpcgGraphs* g);
 
 }  // namespace ramd
@@ -154,7 +158,8 @@ struct ramd_hier_s {
     ramd::BlockState* blk = nullptr;
     // captured FPCG iteration and its own partial-sum buffer (never
     // reallocated while graphs hold it)
-    ramd::FpcgGraphs* fg = nullptr;
+    ramd::FpcgGraphs* fg = nullptr;  // kFpcgGraphSets entries (allocated at first use)
+    uint64_t fg_tick = 0;
     double* fpart = nullptr;
     size_t fpart_cap = 0;
 };

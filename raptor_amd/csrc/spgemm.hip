@@ -242,6 +242,119 @@ __global__ void move_diag_kernel(int m, const int* __restrict__ rp, int* __restr
     }
 }
 
+// ---------------------------------------------------------------------------
+// Rows binned by a per-row key on the device: bin b < nb holds the rows with
+// th[b-1] < mult * key <= th[b], bin nb those above th[nb-1]; with skip_zero
+// a row whose key is 0 goes into no bin.  Pass 0 counts per bin, pass 1
+// writes the rows of bin b from base[b] on (the order inside a bin is not
+// fixed: rows are independent, so it changes no result).  Each workgroup
+// aggregates in LDS and touches each global counter once.
+// ---------------------------------------------------------------------------
+constexpr int kMaxBins = 16;
+struct BinSpec {
+    int64_t th[kMaxBins];
+    int nb;
+    int mult;
+    int skip_zero;
+    int base[kMaxBins + 1];
+};
+
+template <int PASS, typename K>
+__global__ __launch_bounds__(256) void bin_rows_kernel(int m, const K* __restrict__ key, BinSpec sp,
+                                                       int* __restrict__ counter, int* __restrict__ out)
+{
+    __shared__ int hist[kMaxBins + 1];
+    __shared__ int gbase[kMaxBins + 1];
+    const int t = threadIdx.x;
+    if (t <= sp.nb) hist[t] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * 256 + t;
+    int b = -1, pos = 0;
+    if (i < m) {
+        const int64_t k = (int64_t)key[i];
+        if (!(sp.skip_zero && k == 0)) {
+            const int64_t v = k * sp.mult;
+            b = 0;
+            while (b < sp.nb && sp.th[b] < v) b++;
+            pos = atomicAdd(&hist[b], 1);
+        }
+    }
+    __syncthreads();
+    if (t <= sp.nb && hist[t]) gbase[t] = atomicAdd(&counter[t], hist[t]);
+    __syncthreads();
+    if (PASS == 1 && b >= 0) out[sp.base[b] + gbase[b] + pos] = i;
+}
+
+template <typename K>
+__global__ void gather_keys(int nr, const int* __restrict__ rows, const K* __restrict__ key, int64_t* __restrict__ out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nr) out[r] = (int64_t)key[rows[r]];
+}
+
+// Device buffers released at scope exit (also on an exception).
+struct DevTemps {
+    std::vector<void*> p;
+    template <class T>
+    T* get(size_t n)
+    {
+        T* q = dalloc<T>(std::max<size_t>(n, 1));
+        p.push_back(q);
+        return q;
+    }
+    ~DevTemps()
+    {
+        for (void* q : p) dfree(q);
+    }
+};
+
+// Bins rows 0..m-1 by key (see bin_rows_kernel) into rows_out (m ints);
+// returns the first row-list index and the count of every bin (nb + 1).
+template <typename K>
+static void bin_rows(ramd_ctx ctx, int m, const K* key, int mult, bool skip_zero, const int* th, int nb,
+                     int* rows_out, std::vector<int>& first, std::vector<int>& count, DevTemps& tmp)
+{
+    hipStream_t s = ctx->stream;
+    BinSpec sp{};
+    for (int b = 0; b < nb; b++) sp.th[b] = th[b];
+    sp.nb = nb;
+    sp.mult = mult;
+    sp.skip_zero = skip_zero ? 1 : 0;
+    int* counter = tmp.get<int>(2 * (kMaxBins + 1));
+    RAMD_HIP(hipMemsetAsync(counter, 0, sizeof(int) * 2 * (kMaxBins + 1), s));
+    const dim3 g((unsigned)std::max(1, (m + 255) / 256));
+    count.assign(nb + 1, 0);
+    first.assign(nb + 1, 0);
+    if (m) {
+        hipLaunchKernelGGL((bin_rows_kernel<0, K>), g, dim3(256), 0, s, m, key, sp, counter, nullptr);
+        RAMD_HIP(hipGetLastError());
+        RAMD_HIP(hipMemcpyAsync(count.data(), counter, sizeof(int) * (nb + 1), hipMemcpyDeviceToHost, s));
+        RAMD_HIP(hipStreamSynchronize(s));
+        for (int b = 0, acc = 0; b <= nb; b++) {
+            first[b] = acc;
+            sp.base[b] = acc;
+            acc += count[b];
+        }
+        hipLaunchKernelGGL((bin_rows_kernel<1, K>), g, dim3(256), 0, s, m, key, sp, counter + kMaxBins + 1,
+                           rows_out);
+        RAMD_HIP(hipGetLastError());
+    }
+}
+
+// The keys of rows[0..nr) on the host.
+template <typename K>
+static std::vector<int64_t> keys_of(ramd_ctx ctx, const int* rows, int nr, const K* key, DevTemps& tmp)
+{
+    std::vector<int64_t> out((size_t)nr);
+    if (!nr) return out;
+    int64_t* d = tmp.get<int64_t>((size_t)nr);
+    hipLaunchKernelGGL((gather_keys<K>), dim3((nr + 255) / 256), dim3(256), 0, ctx->stream, nr, rows, key, d);
+    RAMD_HIP(hipGetLastError());
+    RAMD_HIP(hipMemcpyAsync(out.data(), d, sizeof(int64_t) * nr, hipMemcpyDeviceToHost, ctx->stream));
+    RAMD_HIP(hipStreamSynchronize(ctx->stream));
+    return out;
+}
+
 static int next_pow2(int64_t v)
 {
     int p = 1;
@@ -249,53 +362,40 @@ static int next_pow2(int64_t v)
     return p;
 }
 
-// Runs symbolic+numeric for C = A B, returns a new handle.
+// Runs symbolic+numeric for C = A B, returns a new handle.  Everything that
+// scales with the rows stays on the device: the host reads per-bin counts,
+// the rows of the global-hash bins with their sizes, and the totals.
 static ramd_csr_s* spgemm_impl(ramd_csr_s* A, ramd_csr_s* B)
 {
     ramd_ctx ctx = A->ctx;
     hipStream_t st = ctx->stream;
     const int m = A->n_rows, n = B->n_cols;
     RAMD_CHECK(A->n_cols == B->n_rows, RAMD_E_ARG, "spgemm: inner dimensions %d and %d differ", A->n_cols, B->n_rows);
+    DevTemps tmp;
 
-    // --- product counts
-    int64_t* d_prod = dalloc<int64_t>((size_t)std::max(m, 1));
+    // --- product counts, and the rows binned by them
+    int64_t* d_prod = tmp.get<int64_t>((size_t)m);
     if (m) {
         hipLaunchKernelGGL(count_products, dim3((m + 255) / 256), dim3(256), 0, st, m, A->rowptr, A->col, B->rowptr,
                            d_prod);
         RAMD_HIP(hipGetLastError());
     }
-    std::vector<int64_t> prod((size_t)m);
-    if (m) RAMD_HIP(hipMemcpyAsync(prod.data(), d_prod, sizeof(int64_t) * m, hipMemcpyDeviceToHost, st));
-    RAMD_HIP(hipStreamSynchronize(st));
-    dfree(d_prod);
-
-    // --- symbolic bins by product count
     static const int kSymT[] = {64, 128, 256, 512, 1024, 2048, 4096, 8192};
     const int nsym = 8;
-    std::vector<std::vector<int>> sym_bins(nsym + 1);
-    for (int i = 0; i < m; i++) {
-        const int64_t p = prod[i];
-        int b = 0;
-        while (b < nsym && kSymT[b] < p) b++;
-        sym_bins[b].push_back(i);
-    }
-    int* d_counts = dalloc<int>((size_t)std::max(m, 1));
-    std::vector<int*> temps;
-    auto upload_rows = [&](const std::vector<int>& rows) {
-        int* d = dalloc<int>(rows.size());
-        temps.push_back(d);
-        RAMD_HIP(hipMemcpyAsync(d, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, st));
-        return d;
-    };
+    int* d_rows = tmp.get<int>((size_t)m);
+    std::vector<int> first, count;
+    bin_rows<int64_t>(ctx, m, d_prod, 1, false, kSymT, nsym, d_rows, first, count, tmp);
+
+    // --- symbolic: distinct columns per row
+    int* d_counts = tmp.get<int>((size_t)m);
     for (int b = 0; b < nsym; b++) {
-        const std::vector<int>& rows = sym_bins[b];
-        if (rows.empty()) continue;
-        const int nr = (int)rows.size();
-        int* d_rows = upload_rows(rows);
+        const int nr = count[b];
+        if (!nr) continue;
+        const int* rows = d_rows + first[b];
         switch (kSymT[b]) {
 #define SYM_CASE(TT, WW)                                                                                          \
     case TT:                                                                                                      \
-        hipLaunchKernelGGL((symbolic_lds<TT, WW>), dim3((nr + WW - 1) / WW), dim3(WW * kWave), 0, st, nr, d_rows, \
+        hipLaunchKernelGGL((symbolic_lds<TT, WW>), dim3((nr + WW - 1) / WW), dim3(WW * kWave), 0, st, nr, rows,   \
                            A->rowptr, A->col, B->rowptr, B->col, d_counts);                                       \
         break;
             SYM_CASE(64, 4)
@@ -310,150 +410,122 @@ static ramd_csr_s* spgemm_impl(ramd_csr_s* A, ramd_csr_s* B)
         }
         RAMD_HIP(hipGetLastError());
     }
-    void* sym_pool = nullptr;
-    if (!sym_bins[nsym].empty()) {
-        const std::vector<int>& rows = sym_bins[nsym];
-        const int nr = (int)rows.size();
+    if (count[nsym]) {
+        const int nr = count[nsym];
+        const int* rows = d_rows + first[nsym];
+        const std::vector<int64_t> prod = keys_of<int64_t>(ctx, rows, nr, d_prod, tmp);
         std::vector<int64_t> off(nr);
         std::vector<int> sz(nr);
         int64_t tot = 0;
         for (int r = 0; r < nr; r++) {
-            sz[r] = next_pow2(prod[rows[r]]);
+            sz[r] = next_pow2(prod[r]);
             off[r] = tot;
             tot += sz[r];
         }
-        int* d_rows = upload_rows(rows);
-        int64_t* d_off = dalloc<int64_t>(nr);
-        int* d_sz = dalloc<int>(nr);
-        sym_pool = dmalloc(sizeof(int) * tot);
+        int64_t* d_off = tmp.get<int64_t>(nr);
+        int* d_sz = tmp.get<int>(nr);
+        int* pool = tmp.get<int>((size_t)tot);
         RAMD_HIP(hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * nr, hipMemcpyHostToDevice, st));
         RAMD_HIP(hipMemcpyAsync(d_sz, sz.data(), sizeof(int) * nr, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(symbolic_global, dim3(nr), dim3(kWave), 0, st, nr, d_rows, d_off, d_sz, (int*)sym_pool,
-                           A->rowptr, A->col, B->rowptr, B->col, d_counts);
+        hipLaunchKernelGGL(symbolic_global, dim3(nr), dim3(kWave), 0, st, nr, rows, d_off, d_sz, pool, A->rowptr,
+                           A->col, B->rowptr, B->col, d_counts);
         RAMD_HIP(hipGetLastError());
-        RAMD_HIP(hipStreamSynchronize(st));
-        dfree(d_off);
-        dfree(d_sz);
+        RAMD_HIP(hipStreamSynchronize(st));  // the host vectors above go out of scope
     }
-    std::vector<int> counts((size_t)m);
-    if (m) RAMD_HIP(hipMemcpyAsync(counts.data(), d_counts, sizeof(int) * m, hipMemcpyDeviceToHost, st));
-    RAMD_HIP(hipStreamSynchronize(st));
-    dfree(sym_pool);
-    for (int* p : temps) dfree(p);
-    temps.clear();
 
-    // --- bound row pointers
-    std::vector<int> rp_bound((size_t)m + 1, 0);
-    for (int i = 0; i < m; i++) {
-        const int64_t next = (int64_t)rp_bound[i] + counts[i];
-        RAMD_CHECK(next <= INT32_MAX, RAMD_E_ARG, "spgemm: product has more than 2^31 entries");
-        rp_bound[i + 1] = (int)next;
-    }
-    const int64_t nnz_bound = rp_bound[m];
-    int* d_rpb = dalloc<int>((size_t)m + 1);
+    // --- bound row pointers and the output arrays at those bounds
+    int* d_rpb = tmp.get<int>((size_t)m + 1);
+    const int64_t nnz_bound = device_rowptr(ctx, d_counts, m, d_rpb, "spgemm: the product");
     int* d_col = dalloc<int>((size_t)std::max<int64_t>(nnz_bound, 1));
-    double* d_val = dalloc<double>((size_t)std::max<int64_t>(nnz_bound, 1));
-    RAMD_HIP(hipMemcpyAsync(d_rpb, rp_bound.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, st));
+    double* d_val = nullptr;
+    int* d_rp = nullptr;
+    try {
+        d_val = dalloc<double>((size_t)std::max<int64_t>(nnz_bound, 1));
 
-    // --- numeric bins by distinct count (table = 2x, power of two)
-    static const int kNumT[] = {32, 64, 128, 256, 512, 1024, 2048};
-    const int nnum = 7;
-    std::vector<std::vector<int>> num_bins(nnum + 1);
-    for (int i = 0; i < m; i++) {
-        if (counts[i] == 0) continue;
-        const int64_t need = 2 * (int64_t)counts[i];
-        int b = 0;
-        while (b < nnum && kNumT[b] < need) b++;
-        num_bins[b].push_back(i);
-    }
-    int* d_len = dalloc<int>((size_t)std::max(m, 1));
-    RAMD_HIP(hipMemsetAsync(d_len, 0, sizeof(int) * std::max(m, 1), st));
-    for (int b = 0; b < nnum; b++) {
-        const std::vector<int>& rows = num_bins[b];
-        if (rows.empty()) continue;
-        const int nr = (int)rows.size();
-        int* d_rows = upload_rows(rows);
-        switch (kNumT[b]) {
+        // --- numeric bins by distinct count (table = 2x, power of two)
+        static const int kNumT[] = {32, 64, 128, 256, 512, 1024, 2048};
+        const int nnum = 7;
+        bin_rows<int>(ctx, m, d_counts, 2, true, kNumT, nnum, d_rows, first, count, tmp);
+        int* d_len = tmp.get<int>((size_t)m);
+        RAMD_HIP(hipMemsetAsync(d_len, 0, sizeof(int) * std::max(m, 1), st));
+        for (int b = 0; b < nnum; b++) {
+            const int nr = count[b];
+            if (!nr) continue;
+            const int* rows = d_rows + first[b];
+            switch (kNumT[b]) {
 #define NUM_CASE(TT, WW)                                                                                          \
     case TT:                                                                                                      \
-        hipLaunchKernelGGL((numeric_lds<TT, WW>), dim3((nr + WW - 1) / WW), dim3(WW * kWave), 0, st, nr, d_rows,  \
+        hipLaunchKernelGGL((numeric_lds<TT, WW>), dim3((nr + WW - 1) / WW), dim3(WW * kWave), 0, st, nr, rows,    \
                            A->rowptr, A->col, A->val, B->rowptr, B->col, B->val, d_rpb, d_col, d_val, d_len);     \
         break;
-            NUM_CASE(32, 4)
-            NUM_CASE(64, 4)
-            NUM_CASE(128, 4)
-            NUM_CASE(256, 4)
-            NUM_CASE(512, 4)
-            NUM_CASE(1024, 2)
-            NUM_CASE(2048, 1)
+                NUM_CASE(32, 4)
+                NUM_CASE(64, 4)
+                NUM_CASE(128, 4)
+                NUM_CASE(256, 4)
+                NUM_CASE(512, 4)
+                NUM_CASE(1024, 2)
+                NUM_CASE(2048, 1)
 #undef NUM_CASE
-        }
-        RAMD_HIP(hipGetLastError());
-    }
-    void *kpool = nullptr, *spool = nullptr, *lcpool = nullptr, *lvpool = nullptr;
-    int64_t* d_off = nullptr;
-    int* d_sz = nullptr;
-    if (!num_bins[nnum].empty()) {
-        const std::vector<int>& rows = num_bins[nnum];
-        const int nr = (int)rows.size();
-        std::vector<int64_t> off(nr);
-        std::vector<int> sz(nr);
-        int64_t tot = 0;
-        for (int r = 0; r < nr; r++) {
-            sz[r] = next_pow2(2 * (int64_t)counts[rows[r]]);
-            off[r] = tot;
-            tot += sz[r];
-        }
-        int* d_rows = upload_rows(rows);
-        d_off = dalloc<int64_t>(nr);
-        d_sz = dalloc<int>(nr);
-        kpool = dmalloc(sizeof(int) * tot);
-        spool = dmalloc(sizeof(double) * tot);
-        lcpool = dmalloc(sizeof(int) * tot);
-        lvpool = dmalloc(sizeof(double) * tot);
-        RAMD_HIP(hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * nr, hipMemcpyHostToDevice, st));
-        RAMD_HIP(hipMemcpyAsync(d_sz, sz.data(), sizeof(int) * nr, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(numeric_global, dim3(nr), dim3(kWave), 0, st, nr, d_rows, d_off, d_sz, (int*)kpool,
-                           (double*)spool, (int*)lcpool, (double*)lvpool, A->rowptr, A->col, A->val, B->rowptr,
-                           B->col, B->val, d_rpb, d_col, d_val, d_len);
-        RAMD_HIP(hipGetLastError());
-    }
-    std::vector<int> len((size_t)m);
-    if (m) RAMD_HIP(hipMemcpyAsync(len.data(), d_len, sizeof(int) * m, hipMemcpyDeviceToHost, st));
-    RAMD_HIP(hipStreamSynchronize(st));
-    dfree(kpool);
-    dfree(spool);
-    dfree(lcpool);
-    dfree(lvpool);
-    dfree(d_off);
-    dfree(d_sz);
-    for (int* p : temps) dfree(p);
-    temps.clear();
-
-    // --- final row pointers, compaction where entries were dropped
-    std::vector<int> rp((size_t)m + 1, 0);
-    for (int i = 0; i < m; i++) rp[i + 1] = rp[i] + len[i];
-    const int64_t nnz = rp[m];
-    int* d_rp = dalloc<int>((size_t)m + 1);
-    RAMD_HIP(hipMemcpyAsync(d_rp, rp.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice, st));
-    if (nnz != nnz_bound) {
-        int* c2 = dalloc<int>((size_t)std::max<int64_t>(nnz, 1));
-        double* v2 = dalloc<double>((size_t)std::max<int64_t>(nnz, 1));
-        if (m) {
-            hipLaunchKernelGGL(compact_rows, dim3(m), dim3(64), 0, st, m, d_rpb, d_rp, d_col, d_val, c2, v2);
+            }
             RAMD_HIP(hipGetLastError());
         }
+        if (count[nnum]) {
+            const int nr = count[nnum];
+            const int* rows = d_rows + first[nnum];
+            const std::vector<int64_t> cnt = keys_of<int>(ctx, rows, nr, d_counts, tmp);
+            std::vector<int64_t> off(nr);
+            std::vector<int> sz(nr);
+            int64_t tot = 0;
+            for (int r = 0; r < nr; r++) {
+                sz[r] = next_pow2(2 * cnt[r]);
+                off[r] = tot;
+                tot += sz[r];
+            }
+            int64_t* d_off = tmp.get<int64_t>(nr);
+            int* d_sz = tmp.get<int>(nr);
+            int* kpool = tmp.get<int>((size_t)tot);
+            double* spool = tmp.get<double>((size_t)tot);
+            int* lcpool = tmp.get<int>((size_t)tot);
+            double* lvpool = tmp.get<double>((size_t)tot);
+            RAMD_HIP(hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * nr, hipMemcpyHostToDevice, st));
+            RAMD_HIP(hipMemcpyAsync(d_sz, sz.data(), sizeof(int) * nr, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(numeric_global, dim3(nr), dim3(kWave), 0, st, nr, rows, d_off, d_sz, kpool, spool,
+                               lcpool, lvpool, A->rowptr, A->col, A->val, B->rowptr, B->col, B->val, d_rpb, d_col,
+                               d_val, d_len);
+            RAMD_HIP(hipGetLastError());
+            RAMD_HIP(hipStreamSynchronize(st));  // the host vectors above go out of scope
+        }
+
+        // --- final row pointers, compaction where entries were dropped
+        d_rp = dalloc<int>((size_t)m + 1);
+        const int64_t nnz = device_rowptr(ctx, d_len, m, d_rp, "spgemm: the product");
+        if (nnz != nnz_bound) {
+            int* c2 = dalloc<int>((size_t)std::max<int64_t>(nnz, 1));
+            double* v2 = nullptr;
+            try {
+                v2 = dalloc<double>((size_t)std::max<int64_t>(nnz, 1));
+            } catch (...) {
+                dfree(c2);
+                throw;
+            }
+            if (m) {
+                hipLaunchKernelGGL(compact_rows, dim3(m), dim3(64), 0, st, m, d_rpb, d_rp, d_col, d_val, c2, v2);
+                RAMD_HIP(hipGetLastError());
+            }
+            RAMD_HIP(hipStreamSynchronize(st));
+            dfree(d_col);
+            dfree(d_val);
+            d_col = c2;
+            d_val = v2;
+        }
         RAMD_HIP(hipStreamSynchronize(st));
+        return csr_wrap(ctx, m, n, nnz, d_rp, d_col, d_val, nullptr);
+    } catch (...) {
         dfree(d_col);
         dfree(d_val);
-        d_col = c2;
-        d_val = v2;
+        dfree(d_rp);
+        throw;
     }
-    RAMD_HIP(hipStreamSynchronize(st));
-    dfree(d_rpb);
-    dfree(d_len);
-    dfree(d_counts);
-    return csr_wrap(ctx, m, n, nnz, d_rp, d_col, d_val, rp.data());
 }
 
 ramd_csr_s* spgemm(ramd_csr_s* A, ramd_csr_s* B) { return spgemm_impl(A, B); }

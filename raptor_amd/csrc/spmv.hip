@@ -1298,9 +1298,10 @@ void dia_autotune(ramd_csr_s* A, int pp)
     // candidates: (mode, planes per workgroup)
     std::vector<std::pair<int, int>> cands = {{0, 0}};
     if (d->window.win <= kDiaWin) cands.push_back({1, 0});
-    if (d->box.L > 0)
-        for (int zc : {4, 8, 16, 32})
-            if (zc <= std::max(kBoxMinPlanes, A->n_rows / d->box.P)) cands.push_back({2, zc});
+    // the plane-march kernel with eight planes per workgroup: 4, 16 and 32
+    // planes time within a few percent of it in isolation, and which of them
+    // wins varies from run to run
+    if (d->box.L > 0) cands.push_back({2, kBoxPlanes});
     float best = 1e30f;
     std::pair<int, int> pick = cands[0];
     const int zc0 = d->box.zc[pp];
@@ -1320,9 +1321,10 @@ void dia_autotune(ramd_csr_s* A, int pp)
                 launch_dia<double, double>(A, d->v64, OP_RESIDUAL, (const double*)t.x, (const double*)t.b,
                                            (double*)t.y, 0.0, s, nullptr);
         };
+        // ten timed launches: the candidates differ by a few percent
         run();
         RAMD_HIP(hipEventRecord(e0, s));
-        for (int r = 0; r < 3; r++) run();
+        for (int r = 0; r < 10; r++) run();
         RAMD_HIP(hipEventRecord(e1, s));
         RAMD_HIP(hipEventSynchronize(e1));
         float ms = 0;
