@@ -513,13 +513,15 @@ def bench_adjoint(args, m, cfg):
     x = ra.ParVector(ctx, n)
     g = ra.ParVector(ctx, n)
     rtol = 1e-10
+    # gradient outputs allocated once (a training loop reuses them)
+    dA_buf, db_buf = ra.ParVector(ctx, col.size), ra.ParVector(ctx, n)
 
     def step(prec=ra.PREC_FP64):
         x.set_const_value(0.0)
         _, itf, _ = ra.FPCG(A, ml, x, b, rtol, 500, prec)
         g.copy(x)
         g.axpy(xs, -1.0)
-        dA, db, ita = ra.adjoint(A, ml, x, g, rtol=rtol, cycle_prec=prec)
+        dA, db, ita = ra.adjoint(A, ml, x, g, rtol=rtol, cycle_prec=prec, dA=dA_buf, db=db_buf)
         return itf, ita, dA, db
 
     # the same step with FP32 cycles inside the FP64 flexible PCG (reported

@@ -579,13 +579,18 @@ def block_set_tuning(ml, k, codes, prec=PREC_FP64):
     check(lib().ramd_block_set_tuning(ml.h, int(k), int(prec), _ptr(codes, C.c_int32)))
 
 
-def adjoint(A, ml, x, g, rtol=1e-10, max_iter=500, cycle_prec=PREC_FP64, with_dA=True):
+def adjoint(A, ml, x, g, rtol=1e-10, max_iter=500, cycle_prec=PREC_FP64, with_dA=True, dA=None, db=None):
     """Gradients of L(x) for x = A^-1 b from g = dL/dx (ramd_adjoint): solves
     A^T lambda = g through ml's hierarchy; returns (dL/dA on A's pattern as a
-    device ParVector of nnz entries or None, dL/db = lambda, iterations)."""
+    device ParVector of nnz entries or None, dL/db = lambda, iterations).
+    dA / db: device vectors to write into (nnz and n entries) instead of new
+    ones, for loops that solve repeatedly."""
     nnz = A.nnz
-    db = ParVector(A.ctx, x.n)
-    dA = ParVector(A.ctx, nnz) if with_dA else None
+    if db is None:
+        db = ParVector(A.ctx, x.n)
+    if dA is None and with_dA:
+        dA = ParVector(A.ctx, nnz)
+    assert db.n == x.n and (dA is None or dA.n == nnz)
     it, rel = C.c_int32(), C.c_double()
     check(lib().ramd_adjoint(ml.h, A.h, x.ptr, g.ptr, float(rtol), int(max_iter), int(cycle_prec),
                              dA.ptr if dA is not None else None, db.ptr, C.byref(it), C.byref(rel)))

@@ -487,6 +487,7 @@ static void hier_free(ramd_hier_s* h)
         delete[] h->fg;
     }
     dfree(h->fpart);
+    dfree((void*)h->xslot);
     h->spans.release();
     for (Level& L : h->levels) free_level(L);
     for (void* p : {(void*)h->lu, (void*)h->piv, (void*)h->cb, (void*)h->cx, (void*)h->pr, (void*)h->pz, (void*)h->pp,
@@ -914,18 +915,27 @@ __global__ __launch_bounds__(kBlockThreads) void fpcg_dots(int n, const double* 
 // x += alpha p (when x is given: the step of this iteration, taken here so
 // that p is read once), then p = z + beta p with the flexible
 // (Polak-Ribiere) beta = -alpha (z.Ap) / rz_old from the device scalars
+// (x, or with x null the vector whose address xslot holds: the captured
+// iterations read x's address from the device, so one capture serves every
+// solution vector)
 template <typename Z>
 __global__ void fpcg_dir(int n, const double* __restrict__ sc_alpha, const double* __restrict__ sc_zap,
                          const double* __restrict__ sc_rz_old, const Z* __restrict__ z, double* __restrict__ p,
-                         double* __restrict__ x)
+                         double* x, double* const* __restrict__ xslot)
 {
     const double alpha = *sc_alpha;
     const double beta = -alpha * (*sc_zap) / (*sc_rz_old);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (xslot) x = *xslot;
     const double pi = p[i];
     if (x) x[i] += alpha * pi;
     p[i] = (double)z[i] + beta * pi;
+}
+
+__global__ void set_slot_kernel(double** slot, double* v)
+{
+    if (threadIdx.x == 0) *slot = v;
 }
 
 // x += alpha p with alpha from the device scalar (the step not yet taken
@@ -1059,13 +1069,14 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
     // sweep), r.z into slot nx and z.Ap, then the step x += alpha p (unless
     // taken already: xs null) and p = z + beta p with the flexible beta
     // (r_new - r_old = -alpha Ap)
-    auto stepB = [&](int c, int nx, double* xs, bool first_done) {
+    auto stepB = [&](int c, int nx, double* xs, bool first_done, double* const* xslot) {
         if (f32) cycle32(h, 0, z32, r32, true, first_done);
         else cycle64(h, 0, z, r, true);
         if (f32) fpcg_rz_zap(ctx, n, r, (const float*)z32, Ap, sc + nx, sc + 3, part);
         else fpcg_rz_zap(ctx, n, r, (const double*)z, Ap, sc + nx, sc + 3, part);
-        if (f32) hipLaunchKernelGGL((fpcg_dir<float>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + c, z32, p, xs);
-        else hipLaunchKernelGGL((fpcg_dir<double>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + c, z, p, xs);
+        if (f32)
+            hipLaunchKernelGGL((fpcg_dir<float>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + c, z32, p, xs, xslot);
+        else hipLaunchKernelGGL((fpcg_dir<double>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + c, z, p, xs, xslot);
         RAMD_HIP(hipGetLastError());
     };
     const bool use_graphs = ctx->graphs && ctx->prof_min_nnz < 0 && !h->track;
@@ -1073,9 +1084,13 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
     if (use_graphs) {
         if (!h->fg) h->fg = new FpcgGraphs[kFpcgGraphSets];
         const uint64_t key = forms_key(h, A);
+        if (!h->xslot) h->xslot = dalloc<double*>(1);
+        // the captured direction update reads x's address from here
+        hipLaunchKernelGGL(set_slot_kernel, dim3(1), dim3(64), 0, s, h->xslot, x_d);
+        RAMD_HIP(hipGetLastError());
         for (int i = 0; i < kFpcgGraphSets && !fg; i++) {
             FpcgGraphs& e = h->fg[i];
-            if (e.a_uid == A->uid && e.key == key && e.prec == cycle_prec && e.x == x_d && e.part == part) fg = &e;
+            if (e.a_uid == A->uid && e.key == key && e.prec == cycle_prec && e.part == part) fg = &e;
         }
         if (!fg) {  // the least recently used entry is captured anew
             fg = &h->fg[0];
@@ -1086,7 +1101,6 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
             fg->a_uid = A->uid;
             fg->key = key;
             fg->prec = cycle_prec;
-            fg->x = x_d;
             fg->part = part;
         }
         fg->used = ++h->fg_tick;
@@ -1165,10 +1179,11 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
             // place) replays the captured step; the others run eagerly
             const bool plain = !x_done && z_first == fuse_first;
             if (use_graphs && plain) {
-                if (!fg->gB[par]) fg->gB[par] = capture_graph(s, [&] { stepB(cur, nxt, x_d, fuse_first); });
+                if (!fg->gB[par])
+                    fg->gB[par] = capture_graph(s, [&] { stepB(cur, nxt, nullptr, fuse_first, h->xslot); });
                 RAMD_HIP(hipGraphLaunch(fg->gB[par], s));
             } else {
-                stepB(cur, nxt, x_done ? nullptr : x_d, z_first);
+                stepB(cur, nxt, x_done ? nullptr : x_d, z_first, nullptr);
             }
             z_first = false;
             std::swap(cur, nxt);

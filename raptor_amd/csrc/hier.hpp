@@ -115,17 +115,17 @@ hipGraphExec_t capture_graph(hipStream_t s, const std::function<void()>& fn);
 
 // Captured flexible-PCG iteration of the single-vector solve: the operator
 // step and the preconditioner step, one pair per parity of the alternating
-// r.z slots, valid for one (A, hierarchy forms, precision, x, partials).
+// r.z slots, valid for one (A, hierarchy forms, precision, partials).
 struct FpcgGraphs {
     hipGraphExec_t gA[2] = {nullptr, nullptr}, gB[2] = {nullptr, nullptr};
     uint64_t a_uid = 0, key = 0;
     int prec = -1;
-    const double* x = nullptr;
     const double* part = nullptr;
     uint64_t used = 0;  // last use (least recently used entry is replaced)
 };
-// captured iterations kept per hierarchy: solves that alternate between
-// solution vectors (a forward and an adjoint solve) each keep theirs
+// captured iterations kept per hierarchy (per outer operator, hierarchy
+// forms and precision; the solution vector's address is read from the
+// device at replay, so one capture serves every solution vector)
 constexpr int kFpcgGraphSets = 4;
 void fpcg_graphs_free(FpcgGraphs* g);
 
@@ -160,6 +160,7 @@ struct ramd_hier_s {
     // reallocated while graphs hold it)
     ramd::FpcgGraphs* fg = nullptr;  // kFpcgGraphSets entries (allocated at first use)
     uint64_t fg_tick = 0;
+    double** xslot = nullptr;  // device: the solution vector the captured iterations update
     double* fpart = nullptr;
     size_t fpart_cap = 0;
 };
