@@ -385,6 +385,22 @@ int ramd_fpcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double rt
  * relres_col (k entries, may be NULL) receive each column's iteration count
  * and final relative residual; *iters_total the number of lockstep
  * iterations. */
+/* ramd_fpcg that allocates no device memory: the same solve with the same
+ * kernels and arguments (eager launches, so bit-identical results to
+ * ramd_fpcg on the same hierarchy and operator), with its partial sums in
+ * a fixed per-GPU scratch array of 65536 doubles (512 KiB, held until the
+ * process ends) instead of memory it would allocate.  Needs a hierarchy
+ * whose FP32 copies exist for an FP32 cycle (ramd_hier_enable_fp32) and,
+ * for an operator with the stencil-diagonal form, one earlier ramd_fpcg
+ * call with that operator (its FP32 values and kernel choice); refuses
+ * (RAMD_E_ARG) an operator needing more than 65536 partial sums (about
+ * 16.7M rows) and (RAMD_E_STATE) per-level timing or kernel profiling.
+ * Calls that overlap on one GPU share the scratch and race.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid hipMalloc). */
+int ramd_fpcg_noalloc(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double rtol,
+                      int max_iter, int cycle_prec, double* relres_hist, int32_t* iters,
+                      double* final_relres);
 int ramd_fpcg_block(ramd_hier h, ramd_csr A, int32_t k, double* X_d, const double* B_d, double rtol,
                     int max_iter, int cycle_prec, int use_graph, int32_t* iters_col, double* relres_col,
                     int32_t* iters_total);

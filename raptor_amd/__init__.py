@@ -544,6 +544,17 @@ def FPCG(A, ml, x, b, rtol=1e-8, max_iter=200, cycle_prec=PREC_FP64):
     return hist[:it.value + 1], it.value, fin.value
 
 
+def FPCG_noalloc(A, ml, x, b, rtol=1e-8, max_iter=200, cycle_prec=PREC_FP64):
+    """FPCG without any device allocation (ramd_fpcg_noalloc): partial sums
+    in the library's fixed per-GPU scratch, eager launches; the same results
+    as FPCG once FPCG has run with the same hierarchy and operator."""
+    hist = np.zeros(max_iter + 1)
+    it, fin = C.c_int32(), C.c_double()
+    check(lib().ramd_fpcg_noalloc(ml.h, A.h, x.ptr, b.ptr, float(rtol), int(max_iter), int(cycle_prec),
+                                  _ptr(hist, C.c_double), C.byref(it), C.byref(fin)))
+    return hist[:it.value + 1], it.value, fin.value
+
+
 def FPCG_block(A, ml, X, B, k, rtol=1e-8, max_iter=200, cycle_prec=PREC_FP64, use_graph=True):
     """k right-hand sides (row-interleaved n x k device blocks) solved in
     lockstep (ramd_fpcg_block).  Returns (per-column iterations, per-column

@@ -139,6 +139,7 @@ _SIGS = {
     "ramd_interp_direct_gpu": (P, P, PI32, C.POINTER(P)),
     "ramd_pre_bicgstab": (P, P, P, P, D, I32, PD, I32, PI32, PI32),
     "ramd_adjoint": (P, P, P, P, D, C.c_int, C.c_int, P, P, PI32, PD),
+    "ramd_fpcg_noalloc": (P, P, P, P, D, C.c_int, C.c_int, PD, PI32, PD),
     "ramd_fpcg_block": (P, P, I32, P, P, D, C.c_int, C.c_int, C.c_int, PI32, PD, PI32),
     "ramd_vcycle_block": (P, I32, C.c_int, P, P),
     "ramd_block_tuning": (P, I32, I32, PI32),

@@ -291,6 +291,11 @@ constexpr int kBoxMaxPlanes = 256;
 // An upper bound of dia_grid over every kernel choice and precision pair.
 int dia_grid_bound(const ramd_csr_s* A);
 void dia_free(Dia* d);
+// Fixed scratch of ramd_fpcg_noalloc (partial sums), per GPU, and the set-up
+// of its per-device address (called by ramd_init with the device current).
+constexpr int kNoallocPartials = 65536;
+constexpr int kMaxDevices = 64;
+void noalloc_scratch_setup(int device);
 // Row pointers rp[0..n] from row counts cnt[0..n) on the device (exclusive
 // prefix sum in int64; rp[n] is the total), one 8-byte read of the total;
 // throws RAMD_E_ARG naming `what` when it exceeds 2^31 - 1.

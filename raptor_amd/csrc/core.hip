@@ -913,6 +913,7 @@ int ramd_init(int device, ramd_ctx* out)
         c->device = device;
         RAMD_HIP(hipSetDevice(device));
         RAMD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        noalloc_scratch_setup(device);
         c->red_partials = dalloc<double>(4096);
         c->red_result = dalloc<double>(16);
         RAMD_HIP(hipHostMalloc((void**)&c->pinned, 16 * sizeof(double), hipHostMallocDefault));

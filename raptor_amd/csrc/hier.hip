@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 #include "common.hpp"
 #include "host/setup.hpp"
@@ -1024,8 +1025,41 @@ static double* fpcg_partials(ramd_hier_s* h, size_t count)
 // parity of the alternating r.z slots; the iterations that differ (the
 // last one, a true-residual restart) run eagerly.  Replays and eager steps
 // launch the same kernels with the same arguments: same bits.
+// Partial sums of ramd_fpcg_noalloc: a fixed array in every GPU's memory
+// (kNoallocPartials doubles, 512 KiB), so that entry point allocates
+// nothing.  Host code uses the address hipGetSymbolAddress gives for the
+// device that is current at set-up (noalloc_scratch_setup, from ramd_init);
+// the array's own name is not a device address in host code.  Every call
+// on a GPU shares that GPU's array: calls that overlap on one GPU race.
+static __device__ __align__(256) double scratch[kNoallocPartials];
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+void noalloc_scratch_setup(int device)
+{
+    RAMD_CHECK(device >= 0 && device < kMaxDevices, RAMD_E_ARG, "device %d past the scratch table (%d)", device,
+               kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* ptr = nullptr;
+    RAMD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+}
+
+static double* noalloc_scratch(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    RAMD_CHECK(device >= 0 && device < kMaxDevices && g_scratch_ptr[device], RAMD_E_STATE,
+               "ramd_fpcg_noalloc: no scratch address for device %d (the context was not set up by ramd_init)",
+               device);
+    return g_scratch_ptr[device];
+}
+
+// noalloc: ramd_fpcg_noalloc's form -- no device allocation and no graph:
+// the FP32 copies, the outer operator's kernel choice and the per-hierarchy
+// buffers must exist already; partial sums go to the fixed scratch array.
 static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_d, double rtol, int max_iter,
-                    int cycle_prec, double* relres_hist, int32_t* iters, double* final_relres)
+                    int cycle_prec, double* relres_hist, int32_t* iters, double* final_relres, bool noalloc = false)
 {
     int rc = RAMD_OK;
     ramd_ctx ctx = h->ctx;
@@ -1039,16 +1073,31 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
     // results, half the value bytes); otherwise the FP64 values
     auto outer = [](const ramd_csr_s* M, int op, const double* x, const double* b, double* y, double w,
                     hipStream_t st) { launch_stream_op_c(M, op, x, b, y, w, st); };
-    if (f32) enable_fp32(h);
-    if (A->dia) {
-        csr_make_fp32(A);
-        dia_autotune(A, 2);  // the outer operator reads FP32 values into FP64 vectors
+    if (noalloc) {
+        RAMD_CHECK(!f32 || h->fp32, RAMD_E_STATE,
+                   "ramd_fpcg_noalloc: the hierarchy's FP32 copies are not built (ramd_hier_enable_fp32)");
+        RAMD_CHECK(!A->dia || (A->val32 && A->dia->v32 && A->dia->lds_x[2] >= 0), RAMD_E_STATE,
+                   "ramd_fpcg_noalloc: the operator's FP32 values and kernel choice are made by a first "
+                   "ramd_fpcg call with it");
+        RAMD_CHECK(!h->track && ctx->prof_min_nnz < 0, RAMD_E_STATE,
+                   "ramd_fpcg_noalloc: per-level timing or kernel profiling is on (they create events)");
+    } else {
+        if (f32) enable_fp32(h);
+        if (A->dia) {
+            csr_make_fp32(A);
+            dia_autotune(A, 2);  // the outer operator reads FP32 values into FP64 vectors
+        }
     }
     double *r = h->pr, *z = h->pz, *p = h->pp, *Ap = h->pAp;
     float *r32 = h->pr32, *z32 = h->pz32;
     const int g = fixed_blocks(n);
     const dim3 gv = grid1(n), bv(256);
-    double* part = fpcg_partials(h, (size_t)std::max(spmv_dot_partials(A), 2 * g));
+    const size_t npart = (size_t)std::max(spmv_dot_partials(A), 2 * g);
+    if (noalloc)
+        RAMD_CHECK(npart <= (size_t)kNoallocPartials, RAMD_E_ARG,
+                   "ramd_fpcg_noalloc: the operator needs %zu partial sums, the scratch holds %d (about %d rows)",
+                   npart, kNoallocPartials, kNoallocPartials * kBlockThreads);
+    double* part = noalloc ? noalloc_scratch(ctx->device) : fpcg_partials(h, npart);
     // FP32 form: the update pass also forms the cycle's first Jacobi sweep
     // from zero on the fine level (Jacobi smoothing, more than one level)
     const bool fuse_first = f32 && !multicolour(h) && h->opts.num_smooth_sweeps >= 1 && h->levels.size() > 1;
@@ -1079,7 +1128,7 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
         else hipLaunchKernelGGL((fpcg_dir<double>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + c, z, p, xs, xslot);
         RAMD_HIP(hipGetLastError());
     };
-    const bool use_graphs = ctx->graphs && ctx->prof_min_nnz < 0 && !h->track;
+    const bool use_graphs = !noalloc && ctx->graphs && ctx->prof_min_nnz < 0 && !h->track;
     FpcgGraphs* fg = nullptr;
     if (use_graphs) {
         if (!h->fg) h->fg = new FpcgGraphs[kFpcgGraphSets];
@@ -1910,6 +1959,21 @@ int ramd_fpcg(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double rt
     });
     if (st != RAMD_OK) return st;
     fpcg_error("ramd_fpcg", rc, max_iter);
+    return rc;
+}
+
+int ramd_fpcg_noalloc(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, double rtol, int max_iter,
+                      int cycle_prec, double* relres_hist, int32_t* iters, double* final_relres)
+{
+    int rc = RAMD_OK;
+    int st = guarded([&] {
+        RAMD_CHECK(h && A && x_d && b_d && iters && final_relres, RAMD_E_ARG, "ramd_fpcg_noalloc: NULL argument");
+        RAMD_CHECK(cycle_prec == RAMD_PREC_FP64 || cycle_prec == RAMD_PREC_FP32, RAMD_E_ARG,
+                   "ramd_fpcg_noalloc: unknown precision %d", cycle_prec);
+        rc = fpcg_run(h, A, x_d, b_d, rtol, max_iter, cycle_prec, relres_hist, iters, final_relres, true);
+    });
+    if (st != RAMD_OK) return st;
+    fpcg_error("ramd_fpcg_noalloc", rc, max_iter);
     return rc;
 }
 
