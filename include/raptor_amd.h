@@ -297,11 +297,15 @@ int ramd_hier_save(ramd_hier h, const char* path);
 int ramd_hier_load(ramd_ctx ctx, const char* path, ramd_hier* out);
 /* Builds the FP32 copies of every level used by RAMD_PREC_FP32 cycles. */
 int ramd_hier_enable_fp32(ramd_hier h);
-/* on != 0 (the default): ramd_fpcg and ramd_adjoint replay each iteration
- * as two captured HIP graphs (operator step, preconditioner step; kept per
- * hierarchy and rebuilt when the operator, a level's buffers, the precision
- * or x change); 0: every kernel is launched eagerly.  Same results either
- * way.  While kernel timing (ramd_prof_enable) or per-level tracking
+/* on != 0: ramd_fpcg and ramd_adjoint replay each iteration as two
+ * captured HIP graphs (operator step, preconditioner step; kept per
+ * hierarchy for up to four operator / precision combinations and rebuilt
+ * when a level's buffers or kernel choices change; the solution vector's
+ * address is read from the device, so any x replays the same capture);
+ * 0 (the default): every kernel is launched eagerly, which measured faster
+ * on MI355X (the host sets up the next launches while the GPU runs, where
+ * a graph launch is prepared before its first kernel starts).  Same results
+ * either way.  While kernel timing (ramd_prof_enable) or per-level tracking
  * (ramd_hier_track_times) is on, iterations run eagerly.  No reference
  * counterpart. */
 int ramd_set_graphs(ramd_ctx ctx, int on);

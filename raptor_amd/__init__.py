@@ -61,21 +61,22 @@ class Context:
     live object created on it, so nothing is freed after its context."""
 
     # whether new contexts replay solver iterations from captured HIP graphs
-    graphs_default = True
+    # (the library's default is eager launches)
+    graphs_default = False
 
     def __init__(self, device=0):
         self.h = C.c_void_p()
         self._children = weakref.WeakSet()
         check(lib().ramd_init(device, C.byref(self.h)))
-        if not Context.graphs_default:
-            self.set_graphs(False)
+        if Context.graphs_default:
+            self.set_graphs(True)
 
     def _adopt(self, obj):
         self._children.add(obj)
 
     def set_graphs(self, on=True):
-        """Replay the solvers' iterations as captured HIP graphs (default) or
-        launch every kernel eagerly (ramd_set_graphs); same results."""
+        """Replay the solvers' iterations as captured HIP graphs or launch
+        every kernel eagerly (the default; ramd_set_graphs); same results."""
         check(lib().ramd_set_graphs(self.h, 1 if on else 0))
 
     def sync(self):

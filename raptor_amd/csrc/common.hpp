@@ -108,8 +108,9 @@ struct ramd_ctx_s {
     // per-workgroup partial sums of fused kernels (grown on demand)
     double* big_partials = nullptr;
     size_t big_cap = 0;
-    // solvers replay captured HIP graphs of their iterations (ramd_set_graphs)
-    bool graphs = true;
+    // solvers replay captured HIP graphs of their iterations (ramd_set_graphs;
+    // off by default: eager launches measured faster)
+    bool graphs = false;
     void set_current() const;
 };
 
@@ -323,11 +324,31 @@ void csr_set_values(ramd_csr_s* A, const double* val_d);
 // Kernel launchers (stream-ordered), FP64 and FP32.
 // OP_ADD_INTO: y = b + A x with b a separate input (x <- x + P e written to a
 // second buffer); same arithmetic as OP_APPEND.
-enum StreamOp { OP_SPMV = 0, OP_APPEND = 1, OP_RESIDUAL = 2, OP_JACOBI = 3, OP_ADD_INTO = 4 };
+// OP_SPMV_JFZ: y = A x, and y2 = the first Jacobi sweep from zero of the
+// system with right-hand side y (jacobi_zero_row with the diagonal dvec):
+// the restriction writes the coarse level's first smoothing sweep with it
+enum StreamOp { OP_SPMV = 0, OP_APPEND = 1, OP_RESIDUAL = 2, OP_JACOBI = 3, OP_ADD_INTO = 4, OP_SPMV_JFZ = 5 };
+
+// One row of a Jacobi sweep from x = 0 (util/linalg/relax.cpp:23-41 with
+// x = 0, the same operations): 0 for a row whose |a_ii| <= 1e-16.
+template <typename X>
+__device__ __forceinline__ X jacobi_zero_row(X b, X d, X omega)
+{
+    const X zero = X(0.0);
+    X v = zero;
+    if (fabs(d) > X(1e-16)) v = ((X(1.0) - omega) * zero) + (omega * ((b - zero) / d));
+    return v;
+}
 // dvec (Jacobi only, may be NULL): the matrix's diagonal (0 where absent);
 // on a diagonal-first matrix it replaces re-reading each row's first entry.
 void launch_stream_op(const ramd_csr_s* A, int op, const double* x, const double* b, double* y,
                       double omega, hipStream_t s, const double* dvec = nullptr);
+// Restriction y = R x fused with the coarse level's first Jacobi sweep from
+// zero: x_c = jacobi_zero_row(y, d_c) (OP_SPMV_JFZ), FP64 and FP32.
+void launch_restrict_jfz(const ramd_csr_s* R, const double* x, double* y, double* xc, const double* dc,
+                         double omega, hipStream_t s);
+void launch_restrict_jfz32(const ramd_csr_s* R, const float* x, float* y, float* xc, const float* dc, float omega,
+                           hipStream_t s);
 void launch_stream_op32(const ramd_csr_s* A, int op, const float* x, const float* b, float* y,
                         float omega, hipStream_t s, const float* dvec = nullptr);
 // FP64 arithmetic reading the FP32 copy of the values where every value is

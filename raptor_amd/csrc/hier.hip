@@ -207,11 +207,8 @@ __global__ void jacobi_from_zero(int n, const X* __restrict__ d, const X* __rest
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const X di = d[i];  // 0 for a row without a stored diagonal (empty rows included)
-    const X zero = X(0.0);
-    X v = zero;
-    if (fabs(di) > X(1e-16)) v = ((X(1.0) - omega) * zero) + (omega * ((b[i] - zero) / di));
-    x[i] = v;
+    // d[i] is 0 for a row without a stored diagonal (empty rows included)
+    x[i] = jacobi_zero_row(b[i], d[i], omega);
 }
 
 template <typename X>
@@ -350,8 +347,17 @@ static void mc_relax(ramd_hier_s* h, const Level& L, const V* val, X* x, const X
     RAMD_HIP(hipGetLastError());
 }
 
+// Whether the restriction into level l also forms level l's first Jacobi
+// sweep from zero (a smoothed level, not the coarsest, Jacobi smoothing).
+static bool jfz_fused(const ramd_hier_s* h, int l)
+{
+    return l < (int)h->levels.size() - 1 && !multicolour(h) && h->opts.num_smooth_sweeps >= 1;
+}
+
 // The off-diagonal Jacobi kernel reads `cur` and writes the other buffer.
-static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_zero)
+// first_done: x already holds the first Jacobi sweep from zero (written by
+// the restriction that formed b, with jacobi_from_zero's operations).
+static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_zero, bool first_done = false)
 {
     Level& L = h->levels[l];
     hipStream_t s = h->ctx->stream;
@@ -368,7 +374,9 @@ static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_ze
     double* other = L.t;
     if (mc) mc_relax<double, double>(h, L, L.A->val, cur, b, x_zero);
     for (int it = 0; it < sweeps && !mc; it++) {
-        if (it == 0 && x_zero) {
+        if (it == 0 && first_done) {
+            continue;
+        } else if (it == 0 && x_zero) {
             hipLaunchKernelGGL((jacobi_from_zero<double>), grid1(L.n), dim3(256), 0, s, L.n, L.diag, b, cur, w);
             RAMD_HIP(hipGetLastError());
         } else {
@@ -378,9 +386,12 @@ static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_ze
     }
     launch_stream_op(L.A, OP_RESIDUAL, cur, b, L.r, 0.0, s);
     Level& C = h->levels[l + 1];
-    launch_stream_op(L.R, OP_SPMV, L.r, nullptr, C.b, 0.0, s);
+    // the coarse level's first Jacobi sweep from zero comes with its right-hand side
+    const bool cj = jfz_fused(h, l + 1);
+    if (cj) launch_restrict_jfz(L.R, L.r, C.b, C.x, C.diag, w, s);
+    else launch_stream_op(L.R, OP_SPMV, L.r, nullptr, C.b, 0.0, s);
     level_span_close(h, tk);
-    cycle64(h, l + 1, C.x, C.b, true);
+    cycle64(h, l + 1, C.x, C.b, true, cj);
     tk = level_span_open(h, l);
     // x <- x + P e (CSR_append: row sum first, then added), into the other buffer
     launch_stream_op(L.P, OP_ADD_INTO, C.x, cur, other, 0.0, s);
@@ -435,9 +446,11 @@ static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero
     }
     launch_stream_op32(L.A, OP_RESIDUAL, cur, b, L.r32, 0.f, s);
     Level& C = h->levels[l + 1];
-    launch_stream_op32(L.R, OP_SPMV, L.r32, nullptr, C.b32, 0.f, s);
+    const bool cj = jfz_fused(h, l + 1);
+    if (cj) launch_restrict_jfz32(L.R, L.r32, C.b32, C.x32, C.diag32, w, s);
+    else launch_stream_op32(L.R, OP_SPMV, L.r32, nullptr, C.b32, 0.f, s);
     level_span_close(h, tk);
-    cycle32(h, l + 1, C.x32, C.b32, true);
+    cycle32(h, l + 1, C.x32, C.b32, true, cj);
     tk = level_span_open(h, l);
     launch_stream_op32(L.P, OP_ADD_INTO, C.x32, cur, other, 0.f, s);
     std::swap(cur, other);

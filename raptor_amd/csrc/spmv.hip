@@ -40,7 +40,8 @@ template <int OP, typename V, typename X, int CAP>
 __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
     const int* __restrict__ rowptr, const int* __restrict__ col, const V* __restrict__ val,
     const int* __restrict__ blocks, int nb, const X* __restrict__ x, const X* __restrict__ b,
-    X* __restrict__ y, X omega, int diag_first, double* __restrict__ dotp, const X* __restrict__ dvec)
+    X* __restrict__ y, X omega, int diag_first, double* __restrict__ dotp, const X* __restrict__ dvec,
+    X* __restrict__ y2)
 {
     __shared__ X prod[CAP];
     const int blk = xcd_remap(blockIdx.x, nb);
@@ -59,6 +60,7 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
         if (OP == OP_RESIDUAL || OP == OP_ADD_INTO) pre = b[row];
         else if (OP == OP_APPEND) pre = y[row];
         else if (OP == OP_SPMV && dotp) pre = x[row];
+        else if (OP == OP_SPMV_JFZ) dgv = dvec[row];
         else if (OP == OP_JACOBI) {
             pre = x[row];
             bj = b[row];
@@ -94,7 +96,10 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
         }
         if (tid == 0) {
             if (OP == OP_SPMV || OP == OP_RESIDUAL) y[r0] = acc;
-            else if (OP == OP_APPEND) y[r0] += acc;
+            else if (OP == OP_SPMV_JFZ) {
+                y[r0] = acc;
+                y2[r0] = jacobi_zero_row(acc, dvec[r0], omega);
+            } else if (OP == OP_APPEND) y[r0] += acc;
             else if (OP == OP_ADD_INTO) y[r0] = b[r0] + acc;
             else {
                 const X xo = x[r0];
@@ -133,13 +138,14 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
 
     // phase 2: one lane per row, products added left to right
     const int k0 = rs - s, k1 = re - s;
-    if (OP == OP_SPMV) {
+    if (OP == OP_SPMV || OP == OP_SPMV_JFZ) {
         X acc = 0;
         if (has_row) {
             acc = lds_row_sum<false>(prod, k0, k1, acc);
             y[row] = acc;
+            if (OP == OP_SPMV_JFZ) y2[row] = jacobi_zero_row(acc, dgv, omega);
         }
-        if (dotp) {
+        if (OP == OP_SPMV && dotp) {
             const double v = block_sum(has_row ? (double)pre * (double)acc : 0.0);
             if (tid == 0) dotp[blk] = v;
         }
@@ -199,7 +205,8 @@ template <int OP, typename V, typename X, int LAYOUT>
 __global__ __launch_bounds__(kBlockThreads) void sliced_kernel(
     int n, const int64_t* __restrict__ sptr, const int* __restrict__ rowptr, const int* __restrict__ col,
     const int16_t* __restrict__ col16, const V* __restrict__ val, const X* __restrict__ x, const X* __restrict__ b,
-    X* __restrict__ y, X omega, int diag_first, double* __restrict__ dotp, const X* __restrict__ dvec)
+    X* __restrict__ y, X omega, int diag_first, double* __restrict__ dotp, const X* __restrict__ dvec,
+    X* __restrict__ y2)
 {
     constexpr int B = 8;
     constexpr int64_t STEP = LAYOUT == 1 ? 1 : 64;  // CSR rows: consecutive entries; sliced: 64 apart
@@ -279,6 +286,9 @@ __global__ __launch_bounds__(kBlockThreads) void sliced_kernel(
     if (!valid) return;
     if (OP == OP_SPMV || OP == OP_RESIDUAL) {
         y[row] = acc;
+    } else if (OP == OP_SPMV_JFZ) {
+        y[row] = acc;
+        y2[row] = jacobi_zero_row(acc, dvec[row], omega);
     } else if (OP == OP_APPEND) {
         y[row] += acc;
     } else if (OP == OP_ADD_INTO) {
@@ -316,7 +326,8 @@ __global__ __launch_bounds__(kBlockThreads) void wave_kernel(int n, const int* _
                                                             const int* __restrict__ col, const V* __restrict__ val,
                                                             const X* __restrict__ x, const X* __restrict__ b,
                                                             X* __restrict__ y, X omega, int diag_first,
-                                                            double* __restrict__ dotp, const X* __restrict__ dvec)
+                                                            double* __restrict__ dotp, const X* __restrict__ dvec,
+                                                            X* __restrict__ y2)
 {
     __shared__ X prod[kWaveRows * kWaveChunk];
     __shared__ int dpos[kWaveRows];
@@ -407,6 +418,9 @@ __global__ __launch_bounds__(kBlockThreads) void wave_kernel(int n, const int* _
     if (!valid || lane != 0) return;
     if (OP == OP_SPMV || OP == OP_RESIDUAL) {
         y[row] = acc;
+    } else if (OP == OP_SPMV_JFZ) {
+        y[row] = acc;
+        y2[row] = jacobi_zero_row(acc, dvec[row], omega);
     } else if (OP == OP_APPEND) {
         y[row] += acc;
     } else if (OP == OP_ADD_INTO) {
@@ -918,13 +932,15 @@ static double op_bytes(const ramd_csr_s* A, int op, int sv, int sx, bool dia)
     if (dia) b = (double)A->dia->nd * A->n_rows * sv + 4.0 * A->n_rows;
     else b = (double)A->nnz * (sv + 4.0) + 4.0 * (A->n_rows + 1);
     b += (double)sx * A->n_cols + (double)sx * A->n_rows;
-    if (op != OP_SPMV) b += (double)sx * A->n_rows;
+    if (op == OP_SPMV_JFZ) b += 2.0 * sx * A->n_rows;  // the diagonal read, the sweep written
+    else if (op != OP_SPMV) b += (double)sx * A->n_rows;
     return b;
 }
 
 template <typename V, typename X, int CAP>
 static void launch_stream_cap(const ramd_csr_s* A, const ramd_csr_s::RowBlocks& rb, const V* val, int op,
-                              const X* x, const X* b, X* y, X omega, hipStream_t s, double* dotp, const X* dvec)
+                              const X* x, const X* b, X* y, X omega, hipStream_t s, double* dotp, const X* dvec,
+                              X* y2)
 {
     const int nb = rb.n;
     const dim3 grid(nb), block(kBlockThreads);
@@ -932,23 +948,27 @@ static void launch_stream_cap(const ramd_csr_s* A, const ramd_csr_s::RowBlocks& 
     switch (op) {
     case OP_SPMV:
         hipLaunchKernelGGL((stream_kernel<OP_SPMV, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d, nb,
-                           x, b, y, omega, df, dotp, nullptr);
+                           x, b, y, omega, df, dotp, nullptr, nullptr);
+        break;
+    case OP_SPMV_JFZ:
+        hipLaunchKernelGGL((stream_kernel<OP_SPMV_JFZ, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d,
+                           nb, x, b, y, omega, df, nullptr, dvec, y2);
         break;
     case OP_APPEND:
         hipLaunchKernelGGL((stream_kernel<OP_APPEND, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d,
-                           nb, x, b, y, omega, df, nullptr, nullptr);
+                           nb, x, b, y, omega, df, nullptr, nullptr, nullptr);
         break;
     case OP_RESIDUAL:
         hipLaunchKernelGGL((stream_kernel<OP_RESIDUAL, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d,
-                           nb, x, b, y, omega, df, nullptr, nullptr);
+                           nb, x, b, y, omega, df, nullptr, nullptr, nullptr);
         break;
     case OP_JACOBI:
         hipLaunchKernelGGL((stream_kernel<OP_JACOBI, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d,
-                           nb, x, b, y, omega, df, nullptr, dvec);
+                           nb, x, b, y, omega, df, nullptr, dvec, nullptr);
         break;
     case OP_ADD_INTO:
         hipLaunchKernelGGL((stream_kernel<OP_ADD_INTO, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d,
-                           nb, x, b, y, omega, df, nullptr, nullptr);
+                           nb, x, b, y, omega, df, nullptr, nullptr, nullptr);
         break;
     default:
         fail(RAMD_E_ARG, "unknown stream op %d", op);
@@ -959,7 +979,7 @@ static void launch_stream_cap(const ramd_csr_s* A, const ramd_csr_s::RowBlocks& 
 // over the CSR rows (val: CSR values)
 template <typename V, typename X, int LAYOUT>
 static void launch_sliced(const ramd_csr_s* A, const V* csr_val, int op, const X* x, const X* b, X* y, X omega,
-                          hipStream_t s, double* dotp, const X* dvec)
+                          hipStream_t s, double* dotp, const X* dvec, X* y2)
 {
     const Sliced* S = A->sl;
     const V* val = csr_val;
@@ -978,15 +998,16 @@ static void launch_sliced(const ramd_csr_s* A, const V* csr_val, int op, const X
     const int n = A->n_rows;
     const dim3 grid((n + kBlockThreads - 1) / kBlockThreads), block(kBlockThreads);
     const int df = (int)A->diag_first;
-#define SLICED(OPV, DOT, DV)                                                                                    \
+#define SLICED(OPV, DOT, DV, Y2)                                                                                \
     hipLaunchKernelGGL((sliced_kernel<OPV, V, X, LAYOUT>), grid, block, 0, s, n, sptr, A->rowptr, col, col16, val, \
-                       x, b, y, omega, df, DOT, DV)
+                       x, b, y, omega, df, DOT, DV, Y2)
     switch (op) {
-    case OP_SPMV: SLICED(OP_SPMV, dotp, nullptr); break;
-    case OP_APPEND: SLICED(OP_APPEND, nullptr, nullptr); break;
-    case OP_RESIDUAL: SLICED(OP_RESIDUAL, nullptr, nullptr); break;
-    case OP_JACOBI: SLICED(OP_JACOBI, nullptr, dvec); break;
-    case OP_ADD_INTO: SLICED(OP_ADD_INTO, nullptr, nullptr); break;
+    case OP_SPMV: SLICED(OP_SPMV, dotp, nullptr, nullptr); break;
+    case OP_SPMV_JFZ: SLICED(OP_SPMV_JFZ, nullptr, dvec, y2); break;
+    case OP_APPEND: SLICED(OP_APPEND, nullptr, nullptr, nullptr); break;
+    case OP_RESIDUAL: SLICED(OP_RESIDUAL, nullptr, nullptr, nullptr); break;
+    case OP_JACOBI: SLICED(OP_JACOBI, nullptr, dvec, nullptr); break;
+    case OP_ADD_INTO: SLICED(OP_ADD_INTO, nullptr, nullptr, nullptr); break;
     default: fail(RAMD_E_ARG, "unknown stream op %d", op);
     }
 #undef SLICED
@@ -995,20 +1016,21 @@ static void launch_sliced(const ramd_csr_s* A, const V* csr_val, int op, const X
 
 template <typename V, typename X>
 static void launch_wave(const ramd_csr_s* A, const V* val, int op, const X* x, const X* b, X* y, X omega,
-                        hipStream_t s, double* dotp, const X* dvec)
+                        hipStream_t s, double* dotp, const X* dvec, X* y2)
 {
     const int n = A->n_rows;
     const dim3 grid((n + kWaveRows - 1) / kWaveRows), block(kBlockThreads);
     const int df = (int)A->diag_first;
-#define WAVE(OPV, DOT, DV)                                                                                     \
+#define WAVE(OPV, DOT, DV, Y2)                                                                                     \
     hipLaunchKernelGGL((wave_kernel<OPV, V, X>), grid, block, 0, s, n, A->rowptr, A->col, val, x, b, y, omega, df, \
-                       DOT, DV)
+                       DOT, DV, Y2)
     switch (op) {
-    case OP_SPMV: WAVE(OP_SPMV, dotp, nullptr); break;
-    case OP_APPEND: WAVE(OP_APPEND, nullptr, nullptr); break;
-    case OP_RESIDUAL: WAVE(OP_RESIDUAL, nullptr, nullptr); break;
-    case OP_JACOBI: WAVE(OP_JACOBI, nullptr, dvec); break;
-    case OP_ADD_INTO: WAVE(OP_ADD_INTO, nullptr, nullptr); break;
+    case OP_SPMV: WAVE(OP_SPMV, dotp, nullptr, nullptr); break;
+    case OP_SPMV_JFZ: WAVE(OP_SPMV_JFZ, nullptr, dvec, y2); break;
+    case OP_APPEND: WAVE(OP_APPEND, nullptr, nullptr, nullptr); break;
+    case OP_RESIDUAL: WAVE(OP_RESIDUAL, nullptr, nullptr, nullptr); break;
+    case OP_JACOBI: WAVE(OP_JACOBI, nullptr, dvec, nullptr); break;
+    case OP_ADD_INTO: WAVE(OP_ADD_INTO, nullptr, nullptr, nullptr); break;
     default: fail(RAMD_E_ARG, "unknown stream op %d", op);
     }
 #undef WAVE
@@ -1018,16 +1040,19 @@ static void launch_wave(const ramd_csr_s* A, const V* val, int op, const X* x, c
 // val: CSR values; dval: stencil-diagonal values (nullptr -> CSR kernel)
 template <typename V, typename X>
 static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int op, const X* x, const X* b, X* y,
-                          X omega, hipStream_t s, double* dotp = nullptr, const X* dvec = nullptr)
+                          X omega, hipStream_t s, double* dotp = nullptr, const X* dvec = nullptr,
+                          X* y2 = nullptr)
 {
     if (A->n_rows == 0) return;
+    RAMD_CHECK(op != OP_SPMV_JFZ || (!dval && dvec && y2), RAMD_E_STATE,
+               "restriction with the first sweep needs a CSR operator, a diagonal and a second output");
     ramd_ctx ctx = A->ctx;
     const bool prof = ctx->prof_min_nnz >= 0 && A->nnz >= ctx->prof_min_nnz;
     ProfPending pend{};
     if (prof) {
         pend.beg = prof_event(ctx);
         pend.end = prof_event(ctx);
-        pend.slot = op * 2 + (sizeof(X) == 4 ? 1 : 0);
+        pend.slot = (op == OP_SPMV_JFZ ? OP_SPMV : op) * 2 + (sizeof(X) == 4 ? 1 : 0);
         RAMD_HIP(hipEventRecord(pend.beg, s));
         ctx->prof[pend.slot].bytes += op_bytes(A, op, (int)sizeof(V), (int)sizeof(X), dval != nullptr);
         ctx->prof[pend.slot].launches++;
@@ -1042,7 +1067,7 @@ static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int 
         return;
     }
     if (A->wr_on[sizeof(X) == 4 ? 1 : 0]) {
-        launch_wave<V, X>(A, val, op, x, b, y, omega, s, dotp, dvec);
+        launch_wave<V, X>(A, val, op, x, b, y, omega, s, dotp, dvec, y2);
         if (prof) {
             RAMD_HIP(hipEventRecord(pend.end, s));
             ctx->prof_pending.push_back(pend);
@@ -1050,9 +1075,10 @@ static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int 
         return;
     }
     if (A->lr_on[sizeof(X) == 4 ? 1 : 0] || (A->sl && A->sl_on[sizeof(X) == 4 ? 1 : 0])) {
-        if (A->lr_on[sizeof(X) == 4 ? 1 : 0]) launch_sliced<V, X, 1>(A, val, op, x, b, y, omega, s, dotp, dvec);
-        else if (A->sl16[sizeof(X) == 4 ? 1 : 0]) launch_sliced<V, X, 2>(A, val, op, x, b, y, omega, s, dotp, dvec);
-        else launch_sliced<V, X, 0>(A, val, op, x, b, y, omega, s, dotp, dvec);
+        if (A->lr_on[sizeof(X) == 4 ? 1 : 0]) launch_sliced<V, X, 1>(A, val, op, x, b, y, omega, s, dotp, dvec, y2);
+        else if (A->sl16[sizeof(X) == 4 ? 1 : 0])
+            launch_sliced<V, X, 2>(A, val, op, x, b, y, omega, s, dotp, dvec, y2);
+        else launch_sliced<V, X, 0>(A, val, op, x, b, y, omega, s, dotp, dvec, y2);
         if (prof) {
             RAMD_HIP(hipEventRecord(pend.end, s));
             ctx->prof_pending.push_back(pend);
@@ -1061,13 +1087,14 @@ static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int 
     }
     const ramd_csr_s::RowBlocks& rb = A->rb[sizeof(X) == 4 ? 1 : 0];
     switch (rb.cap) {
-    case 2048: launch_stream_cap<V, X, 2048>(A, rb, val, op, x, b, y, omega, s, dotp, dvec); break;
+    case 2048: launch_stream_cap<V, X, 2048>(A, rb, val, op, x, b, y, omega, s, dotp, dvec, y2); break;
     case 8192:
         // 8192 FP64 products would fill the whole 64 KiB of LDS a workgroup may use
-        if constexpr (sizeof(X) == 4) launch_stream_cap<V, X, 8192>(A, rb, val, op, x, b, y, omega, s, dotp, dvec);
+        if constexpr (sizeof(X) == 4)
+            launch_stream_cap<V, X, 8192>(A, rb, val, op, x, b, y, omega, s, dotp, dvec, y2);
         else fail(RAMD_E_STATE, "row-block cap 8192 is for FP32 vectors only");
         break;
-    default: launch_stream_cap<V, X, 4096>(A, rb, val, op, x, b, y, omega, s, dotp, dvec); break;
+    default: launch_stream_cap<V, X, 4096>(A, rb, val, op, x, b, y, omega, s, dotp, dvec, y2); break;
     }
     RAMD_HIP(hipGetLastError());
     if (prof) {
@@ -1104,6 +1131,19 @@ void launch_stream_op_c(const ramd_csr_s* A, int op, const double* x, const doub
         launch_stream<float, double>(A, nullptr, A->dia->v32, op, x, b, y, omega, s);
     else
         launch_stream_op(A, op, x, b, y, omega, s);
+}
+
+void launch_restrict_jfz(const ramd_csr_s* R, const double* x, double* y, double* xc, const double* dc,
+                         double omega, hipStream_t s)
+{
+    launch_stream<double, double>(R, R->val, nullptr, OP_SPMV_JFZ, x, nullptr, y, omega, s, nullptr, dc, xc);
+}
+
+void launch_restrict_jfz32(const ramd_csr_s* R, const float* x, float* y, float* xc, const float* dc, float omega,
+                           hipStream_t s)
+{
+    RAMD_CHECK(R->val32, RAMD_E_STATE, "FP32 values of the matrix were not built");
+    launch_stream<float, float>(R, R->val32, nullptr, OP_SPMV_JFZ, x, nullptr, y, omega, s, nullptr, dc, xc);
 }
 
 void launch_stream_op32(const ramd_csr_s* A, int op, const float* x, const float* b, float* y, float omega,

@@ -404,7 +404,7 @@ def test_fpcg_graph_replay_equals_eager(problem, ctx, prec):
             hist, it, rel = ra.FPCG(Ad, problem["ml"], x, bd, 1e-10, 300, prec)
             out.append((x.numpy(), it, rel, np.array(hist)))
     finally:
-        ctx.set_graphs(True)
+        ctx.set_graphs(False)
     for x, it, rel, hist in out[1:]:
         assert it == out[0][1] and rel == out[0][2]
         assert np.array_equal(hist, out[0][3]) and np.array_equal(x, out[0][0])
