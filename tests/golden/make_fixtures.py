@@ -9,10 +9,15 @@ line) and stores them as plain arrays:
     <name>/rowptr, <name>/col, <name>/val, <name>/shape   for matrices
     <name>                                                 for vectors
 
+It also copies three of the .pm files unchanged into pm/ (PM_FILES), so the
+library's PETSc reader and writer are checked against the reference's own
+files byte for byte.
+
 Run in the container that holds the reference:
     python tests/golden/make_fixtures.py [/root/reference/test_data]
 """
 import os
+import shutil
 import sys
 
 import numpy as np
@@ -34,6 +39,8 @@ MATRICES = [
     # smoothed aggregation (aggregation/tests/test_par_smoothed_aggregation.cpp)
     "sas_A0", "sas_S0", "sas_T0", "sas_P0", "sas_A1", "sas_S1", "sas_T1", "sas_P1",
 ]
+# kept as files: the byte-for-byte reader / writer check
+PM_FILES = ["aniso", "random", "laplacian_P_extend"]
 INT_VECTORS = [
     "rss_cf0", "rss_cf1", "rss_cf0_pmis", "rss_cf1_pmis", "rss_cf0_rs", "rss_cf1_rs",
     "aniso_split", "laplacian_split", "sas_mis0", "sas_mis1", "sas_agg0", "sas_agg1",
@@ -80,6 +87,10 @@ def main(src):
         arrays[name] = np.loadtxt(os.path.join(src, name + ".txt"), dtype=np.float64)
     np.savez_compressed(OUT, **arrays)
     print("wrote", OUT, os.path.getsize(OUT), "bytes,", len(arrays), "arrays")
+    os.makedirs(os.path.join(HERE, "pm"), exist_ok=True)
+    for name in PM_FILES:
+        shutil.copyfile(os.path.join(src, name + ".pm"), os.path.join(HERE, "pm", name + ".pm"))
+    print("copied", len(PM_FILES), "PETSc binary files to", os.path.join(HERE, "pm"))
 
 
 if __name__ == "__main__":
