@@ -109,30 +109,94 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
         return;
     }
 
-    // phase 1: every lane issues all of its column/value loads, then all of
-    // its x gathers, before the first multiply (16 independent loads in
-    // flight per lane instead of one dependent chain)
-    constexpr int U = CAP / kBlockThreads;
-    const int len = e - s;
-    int cc[U];
-    X vv[U], xx[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-        const int j = tid + u * kBlockThreads;
-        if (j < len) {  // streamed once: past the caches
-            cc[u] = __builtin_nontemporal_load(&col[s + j]);
-            vv[u] = X(__builtin_nontemporal_load(&val[s + j]));
+    if constexpr (sizeof(X) == 4) {
+        // phase 1 (FP32 vectors): the block's entries [s, e) are read in aligned
+        // groups of four -- one 16-byte column load and one 16-byte value load per
+        // group and lane instead of four of each -- then the gathers, then the
+        // products; all of a step's loads are in flight before the first multiply.
+        // Group entries outside [s, e) are neither gathered nor stored.  The last
+        // block's final partial group, which may end past the arrays, is read
+        // entry by entry.  (FP64 vectors keep one entry per load: their wide
+        // loads measured slower.)
+        typedef int I4 __attribute__((ext_vector_type(4)));
+        typedef V V4 __attribute__((ext_vector_type(4)));
+        constexpr int QPL = (CAP / 4 + 1 + kBlockThreads - 1) / kBlockThreads;  // groups per lane, at most
+        constexpr int QC = 4;                                                      // groups per step
+        const int q0 = s >> 2, nq = ((e + 3) >> 2) - q0;
+        const bool tail_scalar = blk == nb - 1 && (e & 3) != 0;
+    #pragma unroll
+        for (int u0 = 0; u0 < QPL; u0 += QC) {
+            int cc[QC][4];
+            V vv[QC][4];
+            X xx[QC][4];
+    #pragma unroll
+            for (int t = 0; t < QC; t++) {
+                const int q = tid + (u0 + t) * kBlockThreads;
+                if (u0 + t < QPL && q < nq) {
+                    const int g = q0 + q;
+                    if (tail_scalar && q == nq - 1) {
+    #pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            const int idx = 4 * g + k;
+                            cc[t][k] = idx < e ? col[idx] : 0;
+                            vv[t][k] = idx < e ? val[idx] : V(0);
+                        }
+                    } else {  // streamed once: past the caches
+                        const I4 c4 = __builtin_nontemporal_load(reinterpret_cast<const I4*>(col) + g);
+                        const V4 v4 = __builtin_nontemporal_load(reinterpret_cast<const V4*>(val) + g);
+    #pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            cc[t][k] = c4[k];
+                            vv[t][k] = v4[k];
+                        }
+                    }
+                }
+            }
+    #pragma unroll
+            for (int t = 0; t < QC; t++) {
+                const int q = tid + (u0 + t) * kBlockThreads;
+    #pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int idx = 4 * (q0 + q) + k;
+                    if (u0 + t < QPL && q < nq && idx >= s && idx < e) xx[t][k] = x[cc[t][k]];
+                }
+            }
+    #pragma unroll
+            for (int t = 0; t < QC; t++) {
+                const int q = tid + (u0 + t) * kBlockThreads;
+    #pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int idx = 4 * (q0 + q) + k;
+                    if (u0 + t < QPL && q < nq && idx >= s && idx < e) prod[idx - s] = X(vv[t][k]) * xx[t][k];
+                }
+            }
         }
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-        const int j = tid + u * kBlockThreads;
-        if (j < len) xx[u] = x[cc[u]];
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-        const int j = tid + u * kBlockThreads;
-        if (j < len) prod[j] = vv[u] * xx[u];
+    } else {
+        // phase 1 (FP64 vectors): every lane issues all of its column/value
+        // loads, then all of its x gathers, before the first multiply (16
+        // independent loads in flight per lane instead of one dependent chain)
+        constexpr int U = CAP / kBlockThreads;
+        const int len = e - s;
+        int cc[U];
+        X vv[U], xx[U];
+    #pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int j = tid + u * kBlockThreads;
+            if (j < len) {  // streamed once: past the caches
+                cc[u] = __builtin_nontemporal_load(&col[s + j]);
+                vv[u] = X(__builtin_nontemporal_load(&val[s + j]));
+            }
+        }
+    #pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int j = tid + u * kBlockThreads;
+            if (j < len) xx[u] = x[cc[u]];
+        }
+    #pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int j = tid + u * kBlockThreads;
+            if (j < len) prod[j] = vv[u] * xx[u];
+        }
     }
     __syncthreads();
 
