@@ -312,6 +312,10 @@ def main():
         "galerkin": galerkin,
         "kernels": prof,
         "row_caps": caps.tolist(),
+        # stencil-diagonal kernel per precision pair: level 0 FP64 and FP32
+        # cycles, the outer operator (mode, planes, L, P)
+        "dia_staging": dict(level0_fp64=ml.level(0)[0].dia_staging(0), level0_fp32=ml.level(0)[0].dia_staging(1),
+                            outer=A.dia_staging(2)) if A.diagonals() else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

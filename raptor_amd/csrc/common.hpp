@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -108,6 +109,9 @@ struct ramd_ctx_s {
     // per-workgroup partial sums of fused kernels (grown on demand)
     double* big_partials = nullptr;
     size_t big_cap = 0;
+    // a buffer larger than the L2 and infinity caches together, written
+    // before each autotuner launch (allocated at the first autotune)
+    void* flush_buf = nullptr;
     // solvers replay captured HIP graphs of their iterations (ramd_set_graphs;
     // off by default: eager launches measured faster)
     bool graphs = false;
@@ -311,6 +315,13 @@ constexpr int kStreamCaps[3] = {2048, 4096, 8192};
 // blocks.  The cap changes only which workgroup owns a row, never the order
 // of a row's sum, so results are identical for every choice.
 void csr_autotune(ramd_csr_s* A, int prec);
+// Median time (ms) of `reps` launches of run(), each after a pass that
+// writes kFlushBytes (more than the L2 and infinity caches hold): a launch
+// then finds its operands in HBM, as inside a cycle after the other
+// levels' kernels, rather than left in the caches by its own previous
+// launch.  The autotuners rank kernel choices by it.
+constexpr size_t kFlushBytes = (size_t)320 << 20;
+float cold_median_ms(ramd_ctx ctx, const std::function<void()>& run, int reps);
 // For a matrix with the stencil-diagonal form: times the gather kernel, the
 // LDS x-window kernel (when the window fits kDiaWin) and the plane-march
 // kernel (box geometry; several planes per workgroup) for the precision pair

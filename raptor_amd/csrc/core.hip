@@ -936,6 +936,7 @@ int ramd_finalize(ramd_ctx ctx)
         dfree(ctx->red_partials);
         dfree(ctx->red_result);
         dfree(ctx->big_partials);
+        dfree(ctx->flush_buf);
         if (ctx->pinned) (void)hipHostFree(ctx->pinned);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;

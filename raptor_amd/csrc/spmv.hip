@@ -1218,6 +1218,41 @@ void launch_stream_op32(const ramd_csr_s* A, int op, const float* x, const float
     launch_stream<float, float>(A, A->val32, A->dia ? A->dia->v32 : nullptr, op, x, b, y, omega, s, nullptr, dvec);
 }
 
+__global__ void flush_kernel(uint4* __restrict__ p, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = make_uint4(0, 0, 0, 0);
+}
+
+float cold_median_ms(ramd_ctx ctx, const std::function<void()>& run, int reps)
+{
+    hipStream_t s = ctx->stream;
+    if (!ctx->flush_buf) ctx->flush_buf = dmalloc(kFlushBytes);
+    std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+    std::vector<float> ms((size_t)reps, 0.f);
+    try {
+        for (hipEvent_t& e : ev) RAMD_HIP(hipEventCreate(&e));
+        run();  // first launch: code loading
+        for (int r = 0; r < reps; r++) {
+            hipLaunchKernelGGL(flush_kernel, dim3(4096), dim3(256), 0, s, (uint4*)ctx->flush_buf,
+                               (int64_t)(kFlushBytes / sizeof(uint4)));
+            RAMD_HIP(hipGetLastError());
+            RAMD_HIP(hipEventRecord(ev[2 * r], s));
+            run();
+            RAMD_HIP(hipEventRecord(ev[2 * r + 1], s));
+        }
+        RAMD_HIP(hipEventSynchronize(ev.back()));
+        for (int r = 0; r < reps; r++) RAMD_HIP(hipEventElapsedTime(&ms[r], ev[2 * r], ev[2 * r + 1]));
+    } catch (...) {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        throw;
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    std::nth_element(ms.begin(), ms.begin() + reps / 2, ms.end());
+    return ms[reps / 2];
+}
+
 void csr_autotune(ramd_csr_s* A, int prec)
 {
     if (A->n_rows == 0 || A->nnz == 0 || A->dia) return;
@@ -1252,29 +1287,18 @@ void csr_autotune(ramd_csr_s* A, int prec)
     RAMD_HIP(hipMemsetAsync(t.x, 0, w * std::max(A->n_cols, 1), s));
     RAMD_HIP(hipMemsetAsync(t.b, 0, w * std::max(A->n_rows, 1), s));
     const int op = A->n_rows == A->n_cols ? OP_RESIDUAL : OP_SPMV;
-    hipEvent_t e0, e1;
-    RAMD_HIP(hipEventCreate(&e0));
-    RAMD_HIP(hipEventCreate(&e1));
     ramd_csr_s::RowBlocks best = A->rb[prec];
     float best_ms = 1e30f;
-    // one warm-up launch, then the time of three (of twenty for small
-    // operators, whose single launches are too short to rank reliably)
-    const int reps = A->nnz < (int64_t)4 << 20 ? 20 : 3;
+    // the median of cold launches (seven for small operators, whose single
+    // launches are too short to rank reliably, three above)
+    const int reps = A->nnz < (int64_t)4 << 20 ? 7 : 3;
     auto time_ms = [&] {
-        auto run = [&] {
+        return cold_median_ms(ctx, [&] {
             if (prec) launch_stream<float, float>(A, A->val32, nullptr, op, (const float*)t.x, (const float*)t.b,
                                                   (float*)t.y, 0.f, s);
             else launch_stream<double, double>(A, A->val, nullptr, op, (const double*)t.x, (const double*)t.b,
                                                (double*)t.y, 0.0, s);
-        };
-        run();
-        RAMD_HIP(hipEventRecord(e0, s));
-        for (int r = 0; r < reps; r++) run();
-        RAMD_HIP(hipEventRecord(e1, s));
-        RAMD_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        RAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
-        return ms;
+        }, reps);
     };
     for (int cap : kStreamCaps) {
         if (prec == 0 && cap > 4096) continue;
@@ -1366,8 +1390,6 @@ void csr_autotune(ramd_csr_s* A, int prec)
             A->lr_on[prec] = lr;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
 }
 
 void dia_autotune(ramd_csr_s* A, int pp)
@@ -1396,9 +1418,6 @@ void dia_autotune(ramd_csr_s* A, int pp)
     t.part = dalloc<double>((size_t)dia_grid_bound(A));
     RAMD_HIP(hipMemsetAsync(t.x, 0, bytes, s));
     RAMD_HIP(hipMemsetAsync(t.b, 0, bytes, s));
-    hipEvent_t e0, e1;
-    RAMD_HIP(hipEventCreate(&e0));
-    RAMD_HIP(hipEventCreate(&e1));
     // candidates: (mode, planes per workgroup)
     std::vector<std::pair<int, int>> cands = {{0, 0}};
     if (d->window.win <= kDiaWin) cands.push_back({1, 0});
@@ -1425,21 +1444,15 @@ void dia_autotune(ramd_csr_s* A, int pp)
                 launch_dia<double, double>(A, d->v64, OP_RESIDUAL, (const double*)t.x, (const double*)t.b,
                                            (double*)t.y, 0.0, s, nullptr);
         };
-        // ten timed launches: the candidates differ by a few percent
-        run();
-        RAMD_HIP(hipEventRecord(e0, s));
-        for (int r = 0; r < 10; r++) run();
-        RAMD_HIP(hipEventRecord(e1, s));
-        RAMD_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        RAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        // the median of seven cold launches: the candidates differ by a few
+        // percent, and back-to-back launches of a small operator find x in
+        // the caches, which a cycle's launch does not
+        const float ms = cold_median_ms(ctx, run, 7);
         if (ms < best) {
             best = ms;
             pick = c;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     d->lds_x[pp] = pick.first;
     d->box.zc[pp] = pick.first == 2 ? pick.second : zc0;
     A->forms++;
