@@ -106,6 +106,11 @@ struct ramd_ctx_s {
     double* red_partials = nullptr;
     double* red_result = nullptr;  // device scalars (16 slots)
     double* pinned = nullptr;      // pinned host staging for scalars (16 slots)
+    // coherent pinned host memory that kernels write (8 slots; the solvers'
+    // per-iteration scalars) and the event recorded after such a write
+    double* step_pin = nullptr;
+    double* step_pin_d = nullptr;  // its device address
+    hipEvent_t step_ev = nullptr;
     // per-workgroup partial sums of fused kernels (grown on demand)
     double* big_partials = nullptr;
     size_t big_cap = 0;
@@ -381,6 +386,11 @@ void launch_dot(ramd_ctx ctx, int64_t n, const double* x, const double* y, doubl
 void launch_norm2sq(ramd_ctx ctx, int64_t n, const double* x, double* out_d);
 // Sums partials[0..np) into *out_d in a fixed order (one workgroup).
 void launch_reduce_final(ramd_ctx ctx, const double* partials, int np, double* out_d);
+// launch_reduce_final for np <= 4096 that also writes host[0] = *also_d and
+// host[1] = the sum (host: the device address of device-writable host
+// memory such as ctx->step_pin_d)
+void launch_reduce_final_pub(ramd_ctx ctx, const double* partials, int np, double* out_d, const double* also_d,
+                             double* host);
 // Device scratch of at least `count` doubles for per-workgroup partials.
 double* ctx_partials(ramd_ctx ctx, size_t count);
 double fetch_scalar(ramd_ctx ctx, const double* d);

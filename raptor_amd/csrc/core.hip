@@ -917,6 +917,9 @@ int ramd_init(int device, ramd_ctx* out)
         c->red_partials = dalloc<double>(4096);
         c->red_result = dalloc<double>(16);
         RAMD_HIP(hipHostMalloc((void**)&c->pinned, 16 * sizeof(double), hipHostMallocDefault));
+        RAMD_HIP(hipHostMalloc((void**)&c->step_pin, 8 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+        RAMD_HIP(hipHostGetDevicePointer((void**)&c->step_pin_d, c->step_pin, 0));
+        RAMD_HIP(hipEventCreateWithFlags(&c->step_ev, hipEventDisableTiming));
         RAMD_HIP(hipStreamSynchronize(c->stream));
         *out = c;
     });
@@ -938,6 +941,8 @@ int ramd_finalize(ramd_ctx ctx)
         dfree(ctx->big_partials);
         dfree(ctx->flush_buf);
         if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+        if (ctx->step_pin) (void)hipHostFree(ctx->step_pin);
+        if (ctx->step_ev) (void)hipEventDestroy(ctx->step_ev);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
     });

@@ -501,7 +501,6 @@ static void hier_free(ramd_hier_s* h)
         delete[] h->fg;
     }
     dfree(h->fpart);
-    dfree((void*)h->xslot);
     h->spans.release();
     for (Level& L : h->levels) free_level(L);
     for (void* p : {(void*)h->lu, (void*)h->piv, (void*)h->cb, (void*)h->cx, (void*)h->pr, (void*)h->pz, (void*)h->pp,
@@ -929,27 +928,18 @@ __global__ __launch_bounds__(kBlockThreads) void fpcg_dots(int n, const double* 
 // x += alpha p (when x is given: the step of this iteration, taken here so
 // that p is read once), then p = z + beta p with the flexible
 // (Polak-Ribiere) beta = -alpha (z.Ap) / rz_old from the device scalars
-// (x, or with x null the vector whose address xslot holds: the captured
-// iterations read x's address from the device, so one capture serves every
-// solution vector)
 template <typename Z>
 __global__ void fpcg_dir(int n, const double* __restrict__ sc_alpha, const double* __restrict__ sc_zap,
                          const double* __restrict__ sc_rz_old, const Z* __restrict__ z, double* __restrict__ p,
-                         double* x, double* const* __restrict__ xslot)
+                         double* __restrict__ x)
 {
     const double alpha = *sc_alpha;
     const double beta = -alpha * (*sc_zap) / (*sc_rz_old);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (xslot) x = *xslot;
     const double pi = p[i];
     if (x) x[i] += alpha * pi;
     p[i] = (double)z[i] + beta * pi;
-}
-
-__global__ void set_slot_kernel(double** slot, double* v)
-{
-    if (threadIdx.x == 0) *slot = v;
 }
 
 // x += alpha p with alpha from the device scalar (the step not yet taken
@@ -1030,14 +1020,17 @@ static double* fpcg_partials(ramd_hier_s* h, size_t count)
 // residual.  Returns RAMD_OK, RAMD_E_INDEFINITE, RAMD_E_NUMERIC or
 // RAMD_E_NOT_CONVERGED; throws on argument and device errors.
 //
-// An iteration is two steps: the operator step (A p with p.Ap, the
-// residual update with ||r||^2) and, after the host has read those 16
-// bytes, the preconditioner step (V-cycle, r.z and z.Ap, new direction).
-// With the context's graphs on (and no kernel timing or per-level tracking
-// active) each step is replayed as a captured HIP graph, one pair per
-// parity of the alternating r.z slots; the iterations that differ (the
-// last one, a true-residual restart) run eagerly.  Replays and eager steps
-// launch the same kernels with the same arguments: same bits.
+// An iteration is the operator step (A p with p.Ap, the residual update
+// with ||r||^2; the last kernel also writes those 16 bytes to host memory),
+// the preconditioner application (V-cycle, r.z and z.Ap) and, after the host
+// has read the 16 bytes, the direction update.  While the residual is not
+// close to rtol the application is queued before the host waits for the
+// operator step, so the device does not idle while the host reads and
+// decides.  With the context's graphs on (and no kernel timing or per-level
+// tracking active) the operator step and the application are replayed as
+// captured HIP graphs, one pair per parity of the alternating r.z slots;
+// the application after a true-residual restart runs eagerly.  Replays and
+// eager steps launch the same kernels with the same arguments: same bits.
 // Partial sums of ramd_fpcg_noalloc: a fixed array in every GPU's memory
 // (kNoallocPartials doubles, 512 KiB), so that entry point allocates
 // nothing.  Host code uses the address hipGetSymbolAddress gives for the
@@ -1118,27 +1111,31 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
     double* sc = ctx->red_result;
     int cur = 2, nxt = 5;
     // operator step: Ap = A p and p.Ap in one pass over A; then r -= alpha Ap,
-    // ||r||^2 (and r32, and in the FP32 form the first sweep z32) in one pass
+    // ||r||^2 (and r32, and in the FP32 form the first sweep z32) in one pass;
+    // p.Ap and r.r are also written to host memory, read after step_ev
     auto stepA = [&](int c) {
         launch_spmv_dot(A, true, p, Ap, sc, s, part);
         hipLaunchKernelGGL(fpcg_update_norm, dim3(g), dim3(kBlockThreads), 0, s, n, sc + c, sc, sc + 4, Ap, r,
                            f32 ? r32 : nullptr, part, fuse_first ? h->levels[0].diag32 : nullptr,
                            fuse_first ? z32 : nullptr, (float)h->opts.relax_weight);
         RAMD_HIP(hipGetLastError());
-        launch_reduce_final(ctx, part, g, sc + 1);
+        launch_reduce_final_pub(ctx, part, g, sc + 1, sc, ctx->step_pin_d);
     };
-    // preconditioner step: z = M^-1 r (first_done: z32 holds the fused first
-    // sweep), r.z into slot nx and z.Ap, then the step x += alpha p (unless
-    // taken already: xs null) and p = z + beta p with the flexible beta
-    // (r_new - r_old = -alpha Ap)
-    auto stepB = [&](int c, int nx, double* xs, bool first_done, double* const* xslot) {
+    // preconditioner application: z = M^-1 r (first_done: z32 holds the
+    // fused first sweep), r.z into slot nx and z.Ap.  It writes only z, the
+    // cycle's level vectors, the partial sums and scalar slots nx and 3.
+    auto stepB = [&](int nx, bool first_done) {
         if (f32) cycle32(h, 0, z32, r32, true, first_done);
         else cycle64(h, 0, z, r, true);
         if (f32) fpcg_rz_zap(ctx, n, r, (const float*)z32, Ap, sc + nx, sc + 3, part);
         else fpcg_rz_zap(ctx, n, r, (const double*)z, Ap, sc + nx, sc + 3, part);
-        if (f32)
-            hipLaunchKernelGGL((fpcg_dir<float>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + c, z32, p, xs, xslot);
-        else hipLaunchKernelGGL((fpcg_dir<double>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + c, z, p, xs, xslot);
+    };
+    // direction update: the step x += alpha p (unless taken already: xs
+    // null), then p = z + beta p with the flexible beta (r_new - r_old =
+    // -alpha Ap) and r.z of slot c as the old one
+    auto stepD = [&](int c, double* xs) {
+        if (f32) hipLaunchKernelGGL((fpcg_dir<float>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + c, z32, p, xs);
+        else hipLaunchKernelGGL((fpcg_dir<double>), gv, bv, 0, s, n, sc + 4, sc + 3, sc + c, z, p, xs);
         RAMD_HIP(hipGetLastError());
     };
     const bool use_graphs = !noalloc && ctx->graphs && ctx->prof_min_nnz < 0 && !h->track;
@@ -1146,10 +1143,6 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
     if (use_graphs) {
         if (!h->fg) h->fg = new FpcgGraphs[kFpcgGraphSets];
         const uint64_t key = forms_key(h, A);
-        if (!h->xslot) h->xslot = dalloc<double*>(1);
-        // the captured direction update reads x's address from here
-        hipLaunchKernelGGL(set_slot_kernel, dim3(1), dim3(64), 0, s, h->xslot, x_d);
-        RAMD_HIP(hipGetLastError());
         for (int i = 0; i < kFpcgGraphSets && !fg; i++) {
             FpcgGraphs& e = h->fg[i];
             if (e.a_uid == A->uid && e.key == key && e.prec == cycle_prec && e.part == part) fg = &e;
@@ -1194,7 +1187,7 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
             hipLaunchKernelGGL(fpcg_step_x, gv, bv, 0, s, n, sc + 4, p, x_d);
             RAMD_HIP(hipGetLastError());
         };
-        bool z_first = false;  // z32 holds the fused first sweep of the current r32
+        double relres_prev = 0.0;  // relative residual one iteration back (0: none yet)
         while (it < max_iter) {
             const int par = cur == 2 ? 0 : 1;
             if (use_graphs) {
@@ -1203,9 +1196,30 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
             } else {
                 stepA(cur);
             }
-            z_first = fuse_first;
-            double pr[2];
-            fetch_scalars(ctx, sc, 2, pr);
+            RAMD_HIP(hipEventRecord(ctx->step_ev, s));
+            // Queue the preconditioner application before the host reads
+            // this step's scalars, so the device stays busy meanwhile, when
+            // the residual is not about to reach rtol (at the last ratio of
+            // successive residuals, not within two more iterations).  It
+            // changes nothing the iteration keeps when the iteration then
+            // stops or restarts from the true residual (it is run again on
+            // that residual).
+            const double rho = relres_prev > 0.0 ? relres / relres_prev : 0.1;
+            const bool ahead = relres * rho * rho > rtol;
+            const bool plain_b = fuse_first;  // first_done of a plain iteration's application
+            auto run_b = [&](bool first_done) {
+                if (use_graphs && first_done == plain_b) {
+                    if (!fg->gB[par]) fg->gB[par] = capture_graph(s, [&] { stepB(nxt, plain_b); });
+                    RAMD_HIP(hipGraphLaunch(fg->gB[par], s));
+                } else {
+                    stepB(nxt, first_done);
+                }
+            };
+            if (ahead) run_b(plain_b);
+            bool z_first = fuse_first;  // z32 holds the fused first sweep of the current r32
+            RAMD_HIP(hipEventSynchronize(ctx->step_ev));
+            const volatile double* pin = ctx->step_pin;
+            const double pr[2] = {pin[0], pin[1]};
             if (!(pr[0] > 0.0)) {
                 // x and r take the step; the error is reported instead of a result
                 step_x();
@@ -1213,8 +1227,9 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
                 break;
             }
             it++;
+            relres_prev = relres;
             relres = std::sqrt(pr[1]) / scale;
-            bool x_done = false;
+            bool x_done = false, r_new = false;
             if (relres <= rtol || !std::isfinite(relres) || it == max_iter) {
                 step_x();
                 x_done = true;
@@ -1230,6 +1245,7 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
                     RAMD_HIP(hipGetLastError());
                 }
                 z_first = false;  // r32 was replaced
+                r_new = true;
             } else if (relres_hist) {
                 relres_hist[it] = relres;
             }
@@ -1237,17 +1253,8 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
                 rc = RAMD_E_NUMERIC;
                 break;
             }
-            // the common iteration (x not stepped yet, the fused sweep in
-            // place) replays the captured step; the others run eagerly
-            const bool plain = !x_done && z_first == fuse_first;
-            if (use_graphs && plain) {
-                if (!fg->gB[par])
-                    fg->gB[par] = capture_graph(s, [&] { stepB(cur, nxt, nullptr, fuse_first, h->xslot); });
-                RAMD_HIP(hipGraphLaunch(fg->gB[par], s));
-            } else {
-                stepB(cur, nxt, x_done ? nullptr : x_d, z_first, nullptr);
-            }
-            z_first = false;
+            if (!ahead || r_new) run_b(z_first);
+            stepD(cur, x_done ? nullptr : x_d);
             std::swap(cur, nxt);
         }
     }

@@ -114,8 +114,10 @@ struct EventSpans {
 hipGraphExec_t capture_graph(hipStream_t s, const std::function<void()>& fn);
 
 // Captured flexible-PCG iteration of the single-vector solve: the operator
-// step and the preconditioner step, one pair per parity of the alternating
-// r.z slots, valid for one (A, hierarchy forms, precision, partials).
+// step and the preconditioner application (V-cycle and inner products; the
+// direction update that writes x runs eagerly), one pair per parity of the
+// alternating r.z slots, valid for one (A, hierarchy forms, precision,
+// partials).
 struct FpcgGraphs {
     hipGraphExec_t gA[2] = {nullptr, nullptr}, gB[2] = {nullptr, nullptr};
     uint64_t a_uid = 0, key = 0;
@@ -124,8 +126,8 @@ struct FpcgGraphs {
     uint64_t used = 0;  // last use (least recently used entry is replaced)
 };
 // captured iterations kept per hierarchy (per outer operator, hierarchy
-// forms and precision; the solution vector's address is read from the
-// device at replay, so one capture serves every solution vector)
+// forms and precision; no captured kernel touches the solution vector, so
+// one capture serves every solution vector)
 constexpr int kFpcgGraphSets = 4;
 void fpcg_graphs_free(FpcgGraphs* g);
 
@@ -160,7 +162,6 @@ struct ramd_hier_s {
     // reallocated while graphs hold it)
     ramd::FpcgGraphs* fg = nullptr;  // kFpcgGraphSets entries (allocated at first use)
     uint64_t fg_tick = 0;
-    double** xslot = nullptr;  // device: the solution vector the captured iterations update
     double* fpart = nullptr;
     size_t fpart_cap = 0;
 };

@@ -1528,6 +1528,33 @@ __global__ __launch_bounds__(kBlockThreads) void reduce_final(const double* __re
     if (threadIdx.x == 0) *out = s;
 }
 
+// reduce_final (same operations, same bits) that also publishes two scalars
+// to device-writable host memory: host[0] = *also (a scalar an earlier
+// kernel of the stream wrote), host[1] = the sum.  The host then waits on an
+// event recorded after it instead of copying the scalars back.
+__global__ __launch_bounds__(kBlockThreads) void reduce_final_pub(const double* __restrict__ partials, int np,
+                                                                  double* __restrict__ out,
+                                                                  const double* __restrict__ also, double* host)
+{
+    double s = 0;
+    for (int i = threadIdx.x; i < np; i += kBlockThreads) s += partials[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) {
+        *out = s;
+        host[0] = *also;
+        host[1] = s;
+    }
+}
+
+void launch_reduce_final_pub(ramd_ctx ctx, const double* partials, int np, double* out_d, const double* also_d,
+                             double* host)
+{
+    RAMD_CHECK(np >= 1 && np <= 4096, RAMD_E_ARG, "reduce_final_pub: %d partial sums", np);
+    hipLaunchKernelGGL(reduce_final_pub, dim3(1), dim3(kBlockThreads), 0, ctx->stream, partials, np, out_d, also_d,
+                       host);
+    RAMD_HIP(hipGetLastError());
+}
+
 static int red_blocks(int64_t n)
 {
     int64_t b = (n + kBlockThreads - 1) / kBlockThreads;
