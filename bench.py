@@ -316,6 +316,7 @@ def main():
         # cycles, the outer operator (mode, planes, L, P)
         "dia_staging": dict(level0_fp64=ml.level(0)[0].dia_staging(0), level0_fp32=ml.level(0)[0].dia_staging(1),
                             outer=A.dia_staging(2)) if A.diagonals() else None,
+        "coarse_tail": coarse_tail(ml),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -643,10 +644,20 @@ def bench_adjoint(args, m, cfg):
         "lambda_rel_diff_vs_oracle": lam_err,
         "fp32_cycle": dict(s=round(dt32, 5), iterations_forward=itf32, iterations_adjoint=ita32,
                            dA_rel_diff_vs_fp64_cycles=grad32_diff),
+        "coarse_tail": coarse_tail(ml),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def coarse_tail(ml):
+    """First level of the coarse-tail kernel each cycle precision uses (-1:
+    none) and the smallest level that could start it."""
+    import raptor_amd as ra
+    l64, first = ml.tail_level(ra.PREC_FP64)
+    l32, _ = ml.tail_level(ra.PREC_FP32)
+    return dict(fp64=l64, fp32=l32, first_possible=first)
 
 
 def outer_operator(A):

@@ -297,6 +297,21 @@ int ramd_hier_save(ramd_hier h, const char* path);
 int ramd_hier_load(ramd_ctx ctx, const char* path, ramd_hier* out);
 /* Builds the FP32 copies of every level used by RAMD_PREC_FP32 cycles. */
 int ramd_hier_enable_fp32(ramd_hier h);
+/* Coarse tail: the cycle of the small coarsest levels (up to 2048 rows and
+ * 16384 entries of A and P each, coarse system of at most 64 rows), with
+ * the restriction into them and the interpolation out, as one workgroup in
+ * one launch instead of one launch per operation.  Same bits either way.
+ * level -1: off; 0 (default): automatic -- the first cycle in each
+ * precision times a V-cycle without a tail and with every level that can
+ * start one, and keeps the fastest; >= 1: the tail starts at that level
+ * (RAMD_E_ARG if it cannot).  No reference counterpart
+ * (ParMultilevel::cycle, multilevel/par_multilevel.hpp:341-457, is one
+ * routine per level); a tuning switch. */
+int ramd_hier_set_tail(ramd_hier h, int32_t level);
+/* The tail's first level for the next cycle in precision prec (-1 none, -2
+ * automatic and not timed yet), and the smallest level that can start one
+ * (-1 none). */
+int ramd_hier_tail_level(ramd_hier h, int32_t prec, int32_t* level, int32_t* first);
 /* on != 0: ramd_fpcg and ramd_adjoint replay each iteration as two
  * captured HIP graphs (operator step, preconditioner step; kept per
  * hierarchy for up to four operator / precision combinations and rebuilt

@@ -387,6 +387,19 @@ class ParMultilevel:
     def enable_fp32(self):
         check(lib().ramd_hier_enable_fp32(self.h))
 
+    def set_tail(self, level=0):
+        """Coarse tail (ramd_hier_set_tail; same bits either way): -1 off, 0
+        automatic (timed at the first cycle of each precision), >= 1 forced
+        to start at that level."""
+        check(lib().ramd_hier_set_tail(self.h, int(level)))
+
+    def tail_level(self, prec=PREC_FP64):
+        """(level the next cycle in prec starts the tail at: -1 none, -2 not
+        timed yet; smallest level that can start one, -1 none)."""
+        v, f = C.c_int32(), C.c_int32()
+        check(lib().ramd_hier_tail_level(self.h, int(prec), C.byref(v), C.byref(f)))
+        return v.value, f.value
+
     def set_track_times(self, on=True):
         """ParMultilevel::track_times: per-level device time of the cycle
         work from now on (clears the solve buckets)."""

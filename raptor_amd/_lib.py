@@ -116,6 +116,8 @@ _SIGS = {
     "ramd_host_read_petsc": (C.c_char_p, PI32, PI32, PPI32, PPI32, PPD, PI64),
     "ramd_host_write_petsc": (C.c_char_p, I32, I32, PI32, PI32, PD),
     "ramd_hier_enable_fp32": (P,),
+    "ramd_hier_set_tail": (P, C.c_int32),
+    "ramd_hier_tail_level": (P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)),
     "ramd_set_graphs": (P, C.c_int),
     "ramd_vcycle": (P, P, P, C.c_int),
     "ramd_vcycle_fp32": (P, P, P),

@@ -1073,6 +1073,12 @@ static void bcycle(ramd_hier_s* h, BlockState* bs, int l, X* x, const X* b)
     const X* diag;
     if constexpr (sizeof(X) == 8) diag = L.diag;
     else diag = L.diag32;
+    if (sweeps <= 0) {  // no sweep to start from zero: X = 0 as the reference sets it
+        const int64_t m = (int64_t)L.n * k;
+        if constexpr (sizeof(X) == 8) hipLaunchKernelGGL(bfill, dim3(256), dim3(256), 0, s, m, 0.0, x);
+        else hipLaunchKernelGGL(bfill32, dim3(256), dim3(256), 0, s, m, 0.0f, x);
+        RAMD_HIP(hipGetLastError());
+    }
     for (int it = 0; it < sweeps; it++) {
         if (it == 0) {
             const int kp = pow2_at_least(k);

@@ -2,6 +2,7 @@
 // interpolation, device Galerkin products), the V-cycle in FP64 and FP32,
 // the coarse dense LU solve, stationary AMG, PCG with the reference's rules
 // and the mixed-precision flexible PCG.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cmath>
@@ -242,6 +243,448 @@ __global__ __launch_bounds__(64) void coarse_wave_kernel(int n, const double* __
     coarse_solve_wave<TB, TX>(n, lu, piv, b, x, 1, ls);
 }
 
+// ---------------------------------------------------------------------------
+// Coarse tail: the restriction into a run of small levels, their whole
+// V-cycle (smoothing, residuals, restrictions, the coarse LU solve,
+// interpolations) and the interpolation back out, by one workgroup in one
+// launch instead of about five launches per level.  Each operation runs
+// over its rows with a workgroup barrier after it; a row is computed by one
+// thread with the operations of stream_kernel's phase 2 on the CSR arrays
+// (products rounded, then added left to right in stored order; Jacobi's
+// diagonal rules), so every value is the one the per-level kernels produce.
+// ---------------------------------------------------------------------------
+constexpr int kTailMax = 8;          // levels in the tail, the coarsest included
+constexpr int kTailThreads = 1024;
+constexpr int kTailMaxRows = 2048;   // a tail level's rows
+constexpr int64_t kTailMaxNnz = 16384;   // a tail level's A (and P) entries
+constexpr int kTailMaxOuterRows = 8192;  // rows of the level above the tail
+constexpr int64_t kTailMaxOuterNnz = 65536;  // entries of its P
+
+template <typename X>
+struct TailLevel {
+    int n = 0;
+    int jfz = 0;  // the restriction into this level forms its first Jacobi sweep from zero
+    int adf = 0;  // A stores every row's diagonal first
+    const int *arp = nullptr, *acol = nullptr;
+    const X* aval = nullptr;
+    const X* diag = nullptr;
+    const int *prp = nullptr, *pcol = nullptr;  // P: this level to the next
+    const X* pval = nullptr;
+    const int *rrp = nullptr, *rcol = nullptr;  // R = P^T
+    const X* rval = nullptr;
+    X *x = nullptr, *b = nullptr, *t = nullptr, *r = nullptr;
+};
+template <typename X>
+struct TailArgs {
+    int nl = 0;  // levels lv[0 .. nl), lv[nl - 1] the coarsest
+    int nb = 0;  // rows of the level above the tail
+    const int *bprp = nullptr, *bpcol = nullptr;  // its P and R
+    const X* bpval = nullptr;
+    const int *brrp = nullptr, *brcol = nullptr;
+    const X* brval = nullptr;
+    const X* rb = nullptr;    // its residual
+    const X* curb = nullptr;  // its iterate
+    X* outb = nullptr;        // receives curb + P x_tail
+    int coarse_n = 0;
+    const double* lu = nullptr;
+    const int* piv = nullptr;
+    X omega = 0;
+    int sweeps = 0;
+    TailLevel<X> lv[kTailMax];
+};
+
+// One operation over rows [0, n) of a CSR matrix (n <= kTailThreads *
+// kTailRowsPerThread): OP_SPMV, OP_SPMV_JFZ (y2 = the first Jacobi sweep
+// from zero on y, diagonal dvec), OP_RESIDUAL, OP_ADD_INTO (y = b + A x) and
+// OP_JACOBI (x old iterate, y new; df: diagonal stored first, dvec its
+// values).  The workgroup forms the products of a window of the matrix's
+// entries together (independent loads) into prod (LDS, kTailLds entries),
+// then each row's thread adds its entries of the window in stored order,
+// carrying its sum to the next window: stream_kernel's operations.
+constexpr int kTailRowsPerThread = 8;
+constexpr int kTailLdsBytes = 65536;
+static_assert(kCoarseWave * kCoarseWave * (int)sizeof(double) <= kTailLdsBytes, "coarse LU must fit the tail's LDS");
+static_assert(kTailMaxOuterRows <= kTailThreads * kTailRowsPerThread && kTailMaxRows <= kTailMaxOuterRows,
+              "tail rows per thread");
+
+template <int OP, typename X>
+__device__ void tail_op(X* prod, int n, const int* __restrict__ rp, const int* __restrict__ col,
+                        const X* __restrict__ val, const X* x, const X* b, X* y, X omega, const X* dvec, X* y2,
+                        int df)
+{
+    constexpr int CAP = kTailLdsBytes / (int)sizeof(X);
+    constexpr int RT = kTailRowsPerThread;
+    const int tid = threadIdx.x;
+    const int e0 = rp[0], E = rp[n] - e0;
+    int lo[RT], hi[RT];
+    X acc[RT], diag[RT];
+#pragma unroll
+    for (int u = 0; u < RT; u++) {
+        const int row = tid + u * kTailThreads;
+        lo[u] = hi[u] = 0;
+        acc[u] = diag[u] = X(0);
+        if (row < n) {
+            const int rs = rp[row] - e0, re = rp[row + 1] - e0;
+            lo[u] = rs;
+            hi[u] = re;
+            if (OP == OP_RESIDUAL) acc[u] = b[row];
+            if (OP == OP_JACOBI && df && rs < re) {
+                // the diagonal (first) entry is not added; a row without a
+                // usable one keeps its value
+                bool use;
+                if (dvec) {
+                    const X dg = dvec[row];
+                    use = fabs(dg) > X(1e-16);
+                    if (use) diag[u] = dg;
+                } else {
+                    use = col[e0 + rs] == row;
+                    if (use) diag[u] = val[e0 + rs];
+                }
+                lo[u] = use ? rs + 1 : re;
+            }
+        }
+    }
+    for (int c0 = 0; c0 < E; c0 += CAP) {
+        const int cn = min(CAP, E - c0);
+        __syncthreads();  // the previous window's products are read
+        // eight entries per thread and step, every load issued before the products
+        constexpr int U = 8;
+        for (int j0 = tid; j0 < cn; j0 += U * kTailThreads) {
+            int cc[U];
+            X vv[U], xx[U];
+#pragma unroll
+            for (int v = 0; v < U; v++) {
+                const int j = j0 + v * kTailThreads;
+                if (j < cn) {
+                    cc[v] = col[e0 + c0 + j];
+                    vv[v] = val[e0 + c0 + j];
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < U; v++)
+                if (j0 + v * kTailThreads < cn) xx[v] = x[cc[v]];
+#pragma unroll
+            for (int v = 0; v < U; v++)
+                if (j0 + v * kTailThreads < cn) prod[j0 + v * kTailThreads] = vv[v] * xx[v];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < RT; u++) {
+            const int k0 = max(lo[u], c0), k1 = min(hi[u], c0 + cn);
+            const int row = tid + u * kTailThreads;
+            if (OP == OP_JACOBI && !df) {
+                for (int k = k0; k < k1; k++) {
+                    if (col[e0 + k] == row) diag[u] = val[e0 + k];
+                    else acc[u] = acc[u] + prod[k - c0];
+                }
+            } else {
+                // eight LDS reads in flight, the additions in order
+                int k = k0;
+                for (; k + 8 <= k1; k += 8) {
+                    X t[8];
+#pragma unroll
+                    for (int v = 0; v < 8; v++) t[v] = prod[k - c0 + v];
+#pragma unroll
+                    for (int v = 0; v < 8; v++) acc[u] = (OP == OP_RESIDUAL) ? acc[u] - t[v] : acc[u] + t[v];
+                }
+                for (; k < k1; k++) acc[u] = (OP == OP_RESIDUAL) ? acc[u] - prod[k - c0] : acc[u] + prod[k - c0];
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < RT; u++) {
+        const int row = tid + u * kTailThreads;
+        if (row >= n) continue;
+        if (OP == OP_JACOBI) {
+            const X xo = x[row];
+            y[row] = (fabs(diag[u]) > X(1e-16)) ? ((X(1.0) - omega) * xo) + (omega * ((b[row] - acc[u]) / diag[u]))
+                                                 : xo;
+        } else if (OP == OP_ADD_INTO) {
+            y[row] = b[row] + acc[u];
+        } else {
+            y[row] = acc[u];
+            if (OP == OP_SPMV_JFZ) y2[row] = jacobi_zero_row(acc[u], dvec[row], omega);
+        }
+    }
+}
+
+// The cycle of cycle64 / cycle32 from the restriction into lv[0] to the
+// interpolation out of it, entered with every tail level's x = 0.
+template <typename X>
+__global__ __launch_bounds__(kTailThreads) void tail_cycle_kernel(TailArgs<X> a)
+{
+    // LDS: each operation's products, and the coarse LU for the solve
+    __shared__ double lds[kTailLdsBytes / sizeof(double)];
+    X* prod = reinterpret_cast<X*>(lds);
+    double* ls = lds;
+    {
+        const TailLevel<X>& C = a.lv[0];
+        if (C.jfz)
+            tail_op<OP_SPMV_JFZ, X>(prod, C.n, a.brrp, a.brcol, a.brval, a.rb, nullptr, C.b, a.omega, C.diag, C.x, 0);
+        else tail_op<OP_SPMV, X>(prod, C.n, a.brrp, a.brcol, a.brval, a.rb, nullptr, C.b, X(0), nullptr, nullptr, 0);
+        __syncthreads();
+    }
+    unsigned in_t = 0;  // bit j: level j's iterate is in t after its pre-smoothing
+    for (int j = 0; j + 1 < a.nl; j++) {
+        const TailLevel<X>& L = a.lv[j];
+        X* cur = L.x;
+        X* other = L.t;
+        if (a.sweeps <= 0) {  // no sweep to start from zero: x = 0
+            for (int i = threadIdx.x; i < L.n; i += kTailThreads) cur[i] = X(0);
+            __syncthreads();
+        }
+        for (int it = 0; it < a.sweeps; it++) {
+            if (it == 0 && L.jfz) continue;
+            if (it == 0) {
+                for (int i = threadIdx.x; i < L.n; i += kTailThreads)
+                    cur[i] = jacobi_zero_row(L.b[i], L.diag[i], a.omega);
+            } else {
+                tail_op<OP_JACOBI, X>(prod, L.n, L.arp, L.acol, L.aval, cur, L.b, other, a.omega, L.diag, nullptr,
+                                      L.adf);
+                X* tmp = cur;
+                cur = other;
+                other = tmp;
+            }
+            __syncthreads();
+        }
+        if (cur != L.x) in_t |= 1u << j;
+        tail_op<OP_RESIDUAL, X>(prod, L.n, L.arp, L.acol, L.aval, cur, L.b, L.r, X(0), nullptr, nullptr, 0);
+        __syncthreads();
+        const TailLevel<X>& C = a.lv[j + 1];
+        if (C.jfz)
+            tail_op<OP_SPMV_JFZ, X>(prod, C.n, L.rrp, L.rcol, L.rval, L.r, nullptr, C.b, a.omega, C.diag, C.x, 0);
+        else tail_op<OP_SPMV, X>(prod, C.n, L.rrp, L.rcol, L.rval, L.r, nullptr, C.b, X(0), nullptr, nullptr, 0);
+        __syncthreads();
+    }
+    {  // the coarsest level: the wavefront LU solve (FP64, rounded once on output)
+        const TailLevel<X>& C = a.lv[a.nl - 1];
+        const int n = a.coarse_n;
+        for (int t = threadIdx.x; t < n * n; t += kTailThreads) ls[t] = a.lu[t];
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            const int pv = lane < n ? a.piv[lane] : 0;
+            double v = lane < n ? (double)C.b[lane] : 0.0;
+            v = coarse_lu_lane(n, ls, pv, v, lane);
+            if (lane < n) C.x[lane] = (X)v;
+        }
+        __syncthreads();
+    }
+    for (int j = a.nl - 2; j >= 0; j--) {
+        const TailLevel<X>& L = a.lv[j];
+        const TailLevel<X>& C = a.lv[j + 1];
+        X* cur = ((in_t >> j) & 1u) ? L.t : L.x;
+        X* other = ((in_t >> j) & 1u) ? L.x : L.t;
+        tail_op<OP_ADD_INTO, X>(prod, L.n, L.prp, L.pcol, L.pval, C.x, cur, other, X(0), nullptr, nullptr, 0);
+        X* tmp = cur;
+        cur = other;
+        other = tmp;
+        __syncthreads();
+        for (int it = 0; it < a.sweeps; it++) {
+            tail_op<OP_JACOBI, X>(prod, L.n, L.arp, L.acol, L.aval, cur, L.b, other, a.omega, L.diag, nullptr,
+                                  L.adf);
+            tmp = cur;
+            cur = other;
+            other = tmp;
+            __syncthreads();
+        }
+        if (cur != L.x) {
+            for (int i = threadIdx.x; i < L.n; i += kTailThreads) L.x[i] = cur[i];
+            __syncthreads();
+        }
+    }
+    tail_op<OP_ADD_INTO, X>(prod, a.nb, a.bprp, a.bpcol, a.bpval, a.lv[0].x, a.curb, a.outb, X(0), nullptr, nullptr, 0);
+}
+
+static bool multicolour(const ramd_hier_s* h);
+static bool jfz_fused(const ramd_hier_s* h, int l);
+
+// Levels that can start the coarse tail, the largest tail first (none with
+// per-level timing on, multicolour smoothing or a coarse system past one
+// wavefront).  Tail levels hold at most kTailMaxRows rows and kTailMaxNnz
+// entries of A and of P; the level above them at most kTailMaxOuterRows
+// rows and kTailMaxOuterNnz entries of P.
+static std::vector<int> tail_candidates(const ramd_hier_s* h)
+{
+    std::vector<int> c;
+    const int nlev = (int)h->levels.size();
+    if (h->track || multicolour(h) || nlev < 2 || h->coarse_n > kCoarseWave) return c;
+    auto small = [&](int l) {
+        const Level& L = h->levels[l];
+        return L.n <= kTailMaxRows && (!L.A || L.A->nnz <= kTailMaxNnz) && (!L.P || L.P->nnz <= kTailMaxNnz);
+    };
+    if (!small(nlev - 1)) return c;
+    int l0 = nlev - 1;
+    while (l0 - 1 >= 1 && nlev - (l0 - 1) <= kTailMax && small(l0 - 1)) l0--;
+    for (int l = l0; l < nlev; l++) {
+        const Level& B = h->levels[l - 1];
+        if (B.n <= kTailMaxOuterRows && B.P->nnz <= kTailMaxOuterNnz) c.push_back(l);
+    }
+    return c;
+}
+
+// First level of the coarse tail for a cycle in precision prec (0 FP64,
+// 1 FP32), or -1: the forced level, or in automatic mode the timed choice
+// (none before it is made).
+static int tail_level(const ramd_hier_s* h, int prec)
+{
+    if (h->tail_force != -2) return h->tail_force;
+    if (h->track || multicolour(h) || h->coarse_n > kCoarseWave || h->tail_mode < 0) return -1;
+    if (h->tail_mode > 0) return h->tail_mode;
+    return h->tail_pick[prec] >= 0 ? h->tail_pick[prec] : -1;
+}
+
+template <typename X>
+__global__ void fill_value(int n, X v, X* __restrict__ x)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_zero, bool first_done);
+static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero, bool first_done);
+
+// Automatic mode: the first cycle in each precision times whole V-cycles
+// without a tail and with every candidate start, back to back and in turn
+// (seven rounds: the coarse levels stay cached between cycles as they do in
+// a solve), and keeps the candidate with the smallest median (same bits for
+// all).  Uses the fine level's own x and b vectors.
+void cycle_prepare(ramd_hier_s* h, int prec)
+{
+    if (prec == 1) enable_fp32(h);
+    if (h->tail_mode != 0 || h->tail_pick[prec] != -2 || h->track) return;
+    std::vector<int> cand = tail_candidates(h);
+    if (cand.empty()) {
+        h->tail_pick[prec] = -1;
+        return;
+    }
+    cand.insert(cand.begin(), -1);
+    Level& L = h->levels[0];
+    hipStream_t s = h->ctx->stream;
+    std::function<void()> run;
+    if (prec == 1) {
+        hipLaunchKernelGGL((fill_value<float>), grid1(L.n), dim3(256), 0, s, L.n, 1.0f, L.b32);
+        run = [&] { cycle32(h, 0, L.x32, L.b32, true, false); };
+    } else {
+        hipLaunchKernelGGL((fill_value<double>), grid1(L.n), dim3(256), 0, s, L.n, 1.0, L.b);
+        run = [&] { cycle64(h, 0, L.x, L.b, true, false); };
+    }
+    RAMD_HIP(hipGetLastError());
+    constexpr int kRounds = 7;
+    const int nc = (int)cand.size();
+    std::vector<hipEvent_t> ev(2 * (size_t)nc * kRounds, nullptr);
+    std::vector<float> ms(ev.size() / 2, 0.f);
+    try {
+        for (hipEvent_t& e : ev) RAMD_HIP(hipEventCreate(&e));
+        for (int i = 0; i < nc; i++) {  // first launches: code loading
+            h->tail_force = cand[i];
+            run();
+        }
+        for (int r = 0; r < kRounds; r++)
+            for (int i = 0; i < nc; i++) {
+                const size_t k = (size_t)r * nc + i;
+                h->tail_force = cand[i];
+                RAMD_HIP(hipEventRecord(ev[2 * k], s));
+                run();
+                RAMD_HIP(hipEventRecord(ev[2 * k + 1], s));
+            }
+        h->tail_force = -2;
+        RAMD_HIP(hipEventSynchronize(ev.back()));
+        for (size_t k = 0; k < ms.size(); k++) RAMD_HIP(hipEventElapsedTime(&ms[k], ev[2 * k], ev[2 * k + 1]));
+    } catch (...) {
+        h->tail_force = -2;
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        throw;
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    int best = -1;
+    float best_ms = 0;
+    for (int i = 0; i < nc; i++) {
+        std::vector<float> t(kRounds);
+        for (int r = 0; r < kRounds; r++) t[r] = ms[(size_t)r * nc + i];
+        std::nth_element(t.begin(), t.begin() + kRounds / 2, t.end());
+        if (i == 0 || t[kRounds / 2] < best_ms) {
+            best_ms = t[kRounds / 2];
+            best = cand[i];
+        }
+    }
+    h->tail_pick[prec] = best;
+}
+
+template <typename X>
+static const X* vals_of(const ramd_csr_s* M)
+{
+    if constexpr (sizeof(X) == 8) {
+        return M->val;
+    } else {
+        RAMD_CHECK(M->val32, RAMD_E_STATE, "coarse tail: FP32 values missing");
+        return M->val32;
+    }
+}
+
+// Launches the tail below level lb (restriction of r, the tail levels'
+// cycle, out = cur + P_lb x).
+template <typename X>
+static void launch_tail(ramd_hier_s* h, int lb, const X* r, const X* cur, X* out)
+{
+    const int nlev = (int)h->levels.size();
+    TailArgs<X> a;
+    a.nl = nlev - (lb + 1);
+    RAMD_CHECK(a.nl >= 1 && a.nl <= kTailMax, RAMD_E_STATE, "coarse tail: %d levels", a.nl);
+    const Level& B = h->levels[lb];
+    a.nb = B.n;
+    a.bprp = B.P->rowptr;
+    a.bpcol = B.P->col;
+    a.bpval = vals_of<X>(B.P);
+    a.brrp = B.R->rowptr;
+    a.brcol = B.R->col;
+    a.brval = vals_of<X>(B.R);
+    a.rb = r;
+    a.curb = cur;
+    a.outb = out;
+    a.coarse_n = h->coarse_n;
+    a.lu = h->lu;
+    a.piv = h->piv;
+    a.omega = (X)h->opts.relax_weight;
+    a.sweeps = h->opts.num_smooth_sweeps;
+    for (int j = 0; j < a.nl; j++) {
+        const int l = lb + 1 + j;
+        Level& L = h->levels[l];
+        TailLevel<X>& t = a.lv[j];
+        t.n = L.n;
+        t.jfz = (int)jfz_fused(h, l);
+        if constexpr (sizeof(X) == 8) {
+            t.x = L.x;
+            t.b = L.b;
+            t.t = L.t;
+            t.r = L.r;
+            t.diag = L.diag;
+        } else {
+            t.x = L.x32;
+            t.b = L.b32;
+            t.t = L.t32;
+            t.r = L.r32;
+            t.diag = L.diag32;
+        }
+        if (l == nlev - 1) continue;
+        t.adf = (int)L.A->diag_first;
+        t.arp = L.A->rowptr;
+        t.acol = L.A->col;
+        t.aval = vals_of<X>(L.A);
+        t.prp = L.P->rowptr;
+        t.pcol = L.P->col;
+        t.pval = vals_of<X>(L.P);
+        t.rrp = L.R->rowptr;
+        t.rcol = L.R->col;
+        t.rval = vals_of<X>(L.R);
+    }
+    RAMD_CHECK(a.lv[a.nl - 1].n == a.coarse_n, RAMD_E_STATE, "coarse tail: coarsest level has %d rows, LU %d",
+               a.lv[a.nl - 1].n, a.coarse_n);
+    hipLaunchKernelGGL((tail_cycle_kernel<X>), dim3(1), dim3(kTailThreads), 0, h->ctx->stream, a);
+    RAMD_HIP(hipGetLastError());
+}
+
 static void coarse_solve64(ramd_hier_s* h, const double* b, double* x)
 {
     const int n = h->coarse_n;
@@ -372,6 +815,9 @@ static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_ze
     const bool mc = multicolour(h);
     double* cur = x;
     double* other = L.t;
+    // no sweep to start from zero: x = 0 as the reference sets it
+    // (multilevel/par_multilevel.hpp:389)
+    if (x_zero && !mc && sweeps <= 0) hipLaunchKernelGGL((fill_zero<double>), grid1(L.n), dim3(256), 0, s, L.n, x);
     if (mc) mc_relax<double, double>(h, L, L.A->val, cur, b, x_zero);
     for (int it = 0; it < sweeps && !mc; it++) {
         if (it == 0 && first_done) {
@@ -385,16 +831,21 @@ static void cycle64(ramd_hier_s* h, int l, double* x, const double* b, bool x_ze
         }
     }
     launch_stream_op(L.A, OP_RESIDUAL, cur, b, L.r, 0.0, s);
-    Level& C = h->levels[l + 1];
-    // the coarse level's first Jacobi sweep from zero comes with its right-hand side
-    const bool cj = jfz_fused(h, l + 1);
-    if (cj) launch_restrict_jfz(L.R, L.r, C.b, C.x, C.diag, w, s);
-    else launch_stream_op(L.R, OP_SPMV, L.r, nullptr, C.b, 0.0, s);
-    level_span_close(h, tk);
-    cycle64(h, l + 1, C.x, C.b, true, cj);
-    tk = level_span_open(h, l);
-    // x <- x + P e (CSR_append: row sum first, then added), into the other buffer
-    launch_stream_op(L.P, OP_ADD_INTO, C.x, cur, other, 0.0, s);
+    if (l + 1 == tail_level(h, 0)) {
+        // the restriction, the coarser levels' cycle and x + P e in one launch
+        launch_tail<double>(h, l, L.r, cur, other);
+    } else {
+        Level& C = h->levels[l + 1];
+        // the coarse level's first Jacobi sweep from zero comes with its right-hand side
+        const bool cj = jfz_fused(h, l + 1);
+        if (cj) launch_restrict_jfz(L.R, L.r, C.b, C.x, C.diag, w, s);
+        else launch_stream_op(L.R, OP_SPMV, L.r, nullptr, C.b, 0.0, s);
+        level_span_close(h, tk);
+        cycle64(h, l + 1, C.x, C.b, true, cj);
+        tk = level_span_open(h, l);
+        // x <- x + P e (CSR_append: row sum first, then added), into the other buffer
+        launch_stream_op(L.P, OP_ADD_INTO, C.x, cur, other, 0.0, s);
+    }
     std::swap(cur, other);
     if (mc) mc_relax<double, double>(h, L, L.A->val, cur, b, false);
     for (int it = 0; it < sweeps && !mc; it++) {
@@ -432,6 +883,7 @@ static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero
     const bool mc = multicolour(h);
     float* cur = x;
     float* other = L.t32;
+    if (x_zero && !mc && sweeps <= 0) hipLaunchKernelGGL((fill_zero<float>), grid1(L.n), dim3(256), 0, s, L.n, x);
     if (mc) mc_relax<float, float>(h, L, L.A->val32, cur, b, x_zero);
     for (int it = 0; it < sweeps && !mc; it++) {
         if (it == 0 && first_done) {
@@ -445,14 +897,18 @@ static void cycle32(ramd_hier_s* h, int l, float* x, const float* b, bool x_zero
         }
     }
     launch_stream_op32(L.A, OP_RESIDUAL, cur, b, L.r32, 0.f, s);
-    Level& C = h->levels[l + 1];
-    const bool cj = jfz_fused(h, l + 1);
-    if (cj) launch_restrict_jfz32(L.R, L.r32, C.b32, C.x32, C.diag32, w, s);
-    else launch_stream_op32(L.R, OP_SPMV, L.r32, nullptr, C.b32, 0.f, s);
-    level_span_close(h, tk);
-    cycle32(h, l + 1, C.x32, C.b32, true, cj);
-    tk = level_span_open(h, l);
-    launch_stream_op32(L.P, OP_ADD_INTO, C.x32, cur, other, 0.f, s);
+    if (l + 1 == tail_level(h, 1)) {
+        launch_tail<float>(h, l, L.r32, cur, other);
+    } else {
+        Level& C = h->levels[l + 1];
+        const bool cj = jfz_fused(h, l + 1);
+        if (cj) launch_restrict_jfz32(L.R, L.r32, C.b32, C.x32, C.diag32, w, s);
+        else launch_stream_op32(L.R, OP_SPMV, L.r32, nullptr, C.b32, 0.f, s);
+        level_span_close(h, tk);
+        cycle32(h, l + 1, C.x32, C.b32, true, cj);
+        tk = level_span_open(h, l);
+        launch_stream_op32(L.P, OP_ADD_INTO, C.x32, cur, other, 0.f, s);
+    }
     std::swap(cur, other);
     if (mc) mc_relax<float, float>(h, L, L.A->val32, cur, b, false);
     for (int it = 0; it < sweeps && !mc; it++) {
@@ -1088,7 +1544,7 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
         RAMD_CHECK(!h->track && ctx->prof_min_nnz < 0, RAMD_E_STATE,
                    "ramd_fpcg_noalloc: per-level timing or kernel profiling is on (they create events)");
     } else {
-        if (f32) enable_fp32(h);
+        cycle_prepare(h, f32 ? 1 : 0);
         if (A->dia) {
             csr_make_fp32(A);
             dia_autotune(A, 2);  // the outer operator reads FP32 values into FP64 vectors
@@ -1724,6 +2180,35 @@ int ramd_set_graphs(ramd_ctx ctx, int on)
     });
 }
 
+int ramd_hier_set_tail(ramd_hier h, int32_t level)
+{
+    return guarded([&] {
+        RAMD_CHECK(h, RAMD_E_ARG, "ramd_hier_set_tail: NULL handle");
+        if (level >= 1) {
+            const std::vector<int> c = tail_candidates(h);
+            RAMD_CHECK(std::find(c.begin(), c.end(), (int)level) != c.end(), RAMD_E_ARG,
+                       "ramd_hier_set_tail: level %d cannot start the coarse tail", (int)level);
+        }
+        RAMD_CHECK(level >= -1, RAMD_E_ARG, "ramd_hier_set_tail: level %d", (int)level);
+        h->ctx->set_current();
+        RAMD_HIP(hipStreamSynchronize(h->ctx->stream));
+        h->tail_mode = level;
+    });
+}
+
+int ramd_hier_tail_level(ramd_hier h, int32_t prec, int32_t* level, int32_t* first)
+{
+    return guarded([&] {
+        RAMD_CHECK(h && level && first, RAMD_E_ARG, "ramd_hier_tail_level: NULL argument");
+        RAMD_CHECK(prec == RAMD_PREC_FP64 || prec == RAMD_PREC_FP32, RAMD_E_ARG, "ramd_hier_tail_level: precision %d",
+                   (int)prec);
+        const int p = prec == RAMD_PREC_FP32 ? 1 : 0;
+        *level = (h->tail_mode == 0 && h->tail_pick[p] == -2) ? -2 : tail_level(h, p);
+        const std::vector<int> c = tail_candidates(h);
+        *first = c.empty() ? -1 : c[0];
+    });
+}
+
 int ramd_hier_enable_fp32(ramd_hier h)
 {
     return guarded([&] {
@@ -1738,6 +2223,7 @@ int ramd_vcycle(ramd_hier h, double* x_d, const double* b_d, int x_is_zero)
     return guarded([&] {
         RAMD_CHECK(h && x_d && b_d, RAMD_E_ARG, "ramd_vcycle: NULL argument");
         h->ctx->set_current();
+        cycle_prepare(h, 0);
         cycle64(h, 0, x_d, b_d, x_is_zero != 0);
     });
 }
@@ -1747,7 +2233,7 @@ int ramd_vcycle_fp32(ramd_hier h, float* x_d, const float* b_d)
     return guarded([&] {
         RAMD_CHECK(h && x_d && b_d, RAMD_E_ARG, "ramd_vcycle_fp32: NULL argument");
         h->ctx->set_current();
-        enable_fp32(h);
+        cycle_prepare(h, 1);
         cycle32(h, 0, x_d, b_d, true);
     });
 }
@@ -1760,6 +2246,7 @@ int ramd_amg_solve(ramd_hier h, double* x_d, const double* b_d, double tol, int 
         RAMD_CHECK(h && x_d && b_d && iters, RAMD_E_ARG, "ramd_amg_solve: NULL argument");
         ramd_ctx ctx = h->ctx;
         ctx->set_current();
+        cycle_prepare(h, 0);
         Level& L0 = h->levels[0];
         auto norm = [&](const double* v) {
             launch_norm2sq(ctx, L0.n, v, ctx->red_result);
@@ -1796,6 +2283,7 @@ static int pcg_run(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, doub
                    "ramd_pcg: NULL argument");
         ramd_ctx ctx = h->ctx;
         ctx->set_current();
+        cycle_prepare(h, 0);
         hipStream_t s = ctx->stream;
         const int n = A->n_rows;
         RAMD_CHECK(n == h->levels[0].n, RAMD_E_ARG, "ramd_pcg: A has %d rows, hierarchy %d", n, h->levels[0].n);

@@ -33,22 +33,11 @@ struct Level {
 // Largest coarse system solved by one wavefront (LU staged in LDS).
 constexpr int kCoarseWave = 64;
 
-// dgetrs_('N') on one right-hand side by one wavefront, n <= kCoarseWave:
-// lane i holds b_i; the row swaps, the unit-lower and the upper substitution
-// run in the reference BLAS column order with the same operations per entry
-// as the workgroup kernel (b_i -= b_j * l_ij; b_j /= u_jj), so the result is
-// bit-identical.  b and x are read and written with element stride `stride`;
-// `ls` is n*n doubles of LDS.  All 64 lanes must call it.
-template <typename TB, typename TX>
-__device__ __forceinline__ void coarse_solve_wave(int n, const double* __restrict__ lu, const int* __restrict__ piv,
-                                                  const TB* __restrict__ b, TX* __restrict__ x, size_t stride,
-                                                  double* ls)
+// The substitutions of coarse_solve_wave: lane `lane` of the wavefront
+// holds b_i (i = lane) and its pivot pv, the LU factors are in ls; returns
+// x_i.  Every lane of the wavefront calls it.
+__device__ __forceinline__ double coarse_lu_lane(int n, const double* ls, int pv, double v, int lane)
 {
-    const int lane = threadIdx.x;
-    for (int t = lane; t < n * n; t += 64) ls[t] = lu[t];
-    const int pv = lane < n ? piv[lane] : 0;
-    double v = lane < n ? (double)b[(size_t)lane * stride] : 0.0;
-    __syncthreads();
     for (int j = 0; j < n; j++) {
         const int p = __shfl(pv, j);
         if (p != j) {
@@ -66,6 +55,26 @@ __device__ __forceinline__ void coarse_solve_wave(int n, const double* __restric
         const double bj = __shfl(v, j);
         if (lane < j) v -= bj * ls[lane + j * n];
     }
+    return v;
+}
+
+// dgetrs_('N') on one right-hand side by one wavefront, n <= kCoarseWave:
+// lane i holds b_i; the row swaps, the unit-lower and the upper substitution
+// run in the reference BLAS column order with the same operations per entry
+// as the workgroup kernel (b_i -= b_j * l_ij; b_j /= u_jj), so the result is
+// bit-identical.  b and x are read and written with element stride `stride`;
+// `ls` is n*n doubles of LDS.  All 64 lanes must call it.
+template <typename TB, typename TX>
+__device__ __forceinline__ void coarse_solve_wave(int n, const double* __restrict__ lu, const int* __restrict__ piv,
+                                                  const TB* __restrict__ b, TX* __restrict__ x, size_t stride,
+                                                  double* ls)
+{
+    const int lane = threadIdx.x;
+    for (int t = lane; t < n * n; t += 64) ls[t] = lu[t];
+    const int pv = lane < n ? piv[lane] : 0;
+    double v = lane < n ? (double)b[(size_t)lane * stride] : 0.0;
+    __syncthreads();
+    v = coarse_lu_lane(n, ls, pv, v, lane);
     if (lane < n) x[(size_t)lane * stride] = (TX)v;
 }
 
@@ -154,6 +163,13 @@ struct ramd_hier_s {
     std::vector<double> solve_level_s;
     ramd::EventSpans spans;
     bool fp32 = false;
+    // coarse tail (the small coarsest levels' cycle as one workgroup
+    // kernel; same bits either way): mode -1 off, 0 automatic (timed per
+    // precision at the first cycle: tail_pick, -2 not yet), >= 1 forced to
+    // start at that level; tail_force overrides both while timing
+    int tail_mode = 0;
+    int tail_pick[2] = {-2, -2};
+    int tail_force = -2;
     // PCG work vectors (fine level)
     double *pr = nullptr, *pz = nullptr, *pp = nullptr, *pAp = nullptr;
     float *pr32 = nullptr, *pz32 = nullptr;
@@ -167,11 +183,13 @@ struct ramd_hier_s {
 };
 
 namespace ramd {
-// Identity and buffer counts of A and of every level operator: captured
-// graphs hold raw pointers to all of their buffers and are keyed on this.
+// Identity and buffer counts of A and of every level operator, and the
+// coarse-tail switch: captured graphs hold raw pointers to all of their
+// buffers and the launches these choose, and are keyed on this.
 inline uint64_t forms_key(const ramd_hier_s* h, const ramd_csr_s* A)
 {
     uint64_t k = A->uid * 1000003ull + A->forms;
+    for (int t : {h->tail_mode, h->tail_pick[0], h->tail_pick[1]}) k = k * 131ull + (uint64_t)(t + 3);
     for (const Level& L : h->levels)
         for (const ramd_csr_s* M : {(const ramd_csr_s*)L.A, (const ramd_csr_s*)L.P, (const ramd_csr_s*)L.R})
             if (M) k = k * 1000003ull + M->uid * 31ull + M->forms;
@@ -182,6 +200,9 @@ inline uint64_t forms_key(const ramd_hier_s* h, const ramd_csr_s* A)
 namespace ramd {
 // Builds the FP32 copies of every level (values, diagonal, level vectors).
 void enable_fp32(ramd_hier_s* h);
+// Before cycling in precision prec (0 FP64, 1 FP32): the FP32 copies (prec
+// 1) and, in the coarse tail's automatic mode, its timed choice.
+void cycle_prepare(ramd_hier_s* h, int prec);
 inline dim3 grid1(int64_t n, int bs = 256)
 {
     int64_t g = (n + bs - 1) / bs;

@@ -171,6 +171,7 @@ int ramd_pre_bicgstab(ramd_hier h, ramd_csr A, double* x_d, const double* b_d, d
                    "ramd_pre_bicgstab: A has %d rows, the hierarchy %d", A->n_rows, h->levels[0].n);
         RAMD_CHECK(A->ctx == h->ctx, RAMD_E_ARG, "ramd_pre_bicgstab: A and the hierarchy use different contexts");
         h->ctx->set_current();
+        cycle_prepare(h, 0);
         rc = bicgstab_run(h, A, x_d, b_d, tol, max_iter, res, max_res, n_res, iters);
     });
     if (st != RAMD_OK) return st;

@@ -71,6 +71,37 @@ def test_block_vcycle_columns_equal_oracle(problem, ctx, k):
         assert np.array_equal(X32h[:, c], H.cycle_fp32(np.zeros(n, np.float32), B32[:, c])), ("fp32", c)
 
 
+@pytest.mark.parametrize("sweeps", [0, 2])
+def test_block_vcycle_sweep_counts(ctx, sweeps):
+    """Block V-cycles with no smoothing (each level's iterate starts from the
+    zero the reference sets) and with two sweeps: every column equals the
+    oracle's cycle, FP64 and FP32, bit for bit."""
+    rp, col, val = ra.gallery_laplace27(12)
+    ml = ra.ParRugeStubenSolver(ctx, 0.25, ra.PMIS, ra.Extended, ra.Classical, ra.Jacobi)
+    ml.relax_weight = OMEGA
+    ml.num_smooth_sweeps = sweeps
+    ml.setup(rp, col, val)
+    ml.enable_fp32()
+    n, k = rp.size - 1, 3
+    As, Ps = [], []
+    for l in range(ml.num_levels):
+        Al, Pl = ml.level(l)
+        As.append(K.Csr(*Al.to_csr(), Al.n_cols))
+        if Pl is not None:
+            Ps.append(K.Csr(*Pl.to_csr(), Pl.n_cols))
+    H = K.Hier(As, Ps, relax_type=0, sweeps=sweeps, omega=OMEGA)
+    B = np.random.default_rng(40 + sweeps).standard_normal((n, k))
+    X = ra.ParVector.from_numpy(ctx, np.full(n * k, 5.0))
+    ra.vcycle_block(ml, X, ra.ParVector.from_numpy(ctx, B.ravel()), k, ra.PREC_FP64)
+    B32 = B.astype(np.float32)
+    X32 = ra.ParVector.from_numpy(ctx, np.full(n * k, 5.0, np.float32))
+    ra.vcycle_block(ml, X32, ra.ParVector.from_numpy(ctx, B32.ravel()), k, ra.PREC_FP32)
+    Xh, X32h = X.numpy().reshape(n, k), X32.numpy().reshape(n, k)
+    for c in range(k):
+        assert np.array_equal(Xh[:, c], H.cycle(np.zeros(n), B[:, c])), ("fp64", c)
+        assert np.array_equal(X32h[:, c], H.cycle_fp32(np.zeros(n, np.float32), B32[:, c])), ("fp32", c)
+
+
 @pytest.mark.parametrize("k", [1, 5, 32])
 def test_block_vcycle_columns_bit_exact(problem, ctx, k):
     ml, n = problem["ml"], problem["n"]

@@ -164,6 +164,77 @@ def test_vcycle_fp32_bit_exact(problem, ctx):
     assert np.array_equal(x.numpy(), H.cycle_fp32(np.zeros(n, np.float32), b))
 
 
+def test_vcycle_coarse_tail(problem, ctx):
+    """The small coarsest levels can run as one workgroup kernel (the coarse
+    tail).  With the tail forced to start at every level that can start it,
+    FP64 and FP32 cycles equal the one-launch-per-operation cycles (which
+    equal the oracle's) bit for bit; the automatic mode keeps one of these
+    choices."""
+    ml, As = problem["ml"], problem["As"]
+    ml.enable_fp32()
+    _, first = ml.tail_level()
+    assert 1 <= first <= ml.num_levels - 1
+    n = As[0].n_rows
+    b = np.random.default_rng(26).standard_normal(n)
+    out = {}
+    try:
+        for mode in [-1] + list(range(first, ml.num_levels)):
+            ml.set_tail(mode)
+            assert ml.tail_level(ra.PREC_FP64)[0] == mode and ml.tail_level(ra.PREC_FP32)[0] == mode
+            x = ra.ParVector.from_numpy(ctx, np.zeros(n))
+            ml.cycle(x, ra.ParVector.from_numpy(ctx, b), x_is_zero=True)
+            x32 = ra.ParVector(ctx, n, np.float32)
+            ml.cycle_fp32(x32, ra.ParVector.from_numpy(ctx, b.astype(np.float32)))
+            out[mode] = (x.numpy(), x32.numpy())
+        ml.set_tail(0)
+        x = ra.ParVector.from_numpy(ctx, np.zeros(n))
+        ml.cycle(x, ra.ParVector.from_numpy(ctx, b), x_is_zero=True)
+        assert ml.tail_level(ra.PREC_FP64)[0] in out
+    finally:
+        ml.set_tail(0)
+    for mode in out:
+        assert np.array_equal(out[mode][0], out[-1][0]), mode
+        assert np.array_equal(out[mode][1], out[-1][1]), mode
+    assert np.array_equal(x.numpy(), out[-1][0])
+
+
+@pytest.mark.parametrize("sweeps", [0, 2, 3])
+@pytest.mark.parametrize("cfg", ["lap27_16_pmis_ext", "c5_fv7_20_sa"])
+def test_vcycle_sweep_counts(ctx, cfg, sweeps):
+    """Other sweep counts (0: no smoothing, the coarse iterates start from
+    the zero the reference sets; 2, 3: sweeps after the first from zero),
+    with the largest coarse tail and without: FP64 and FP32 cycles
+    bit-identical to the oracle's."""
+    name, gen, c = [t for t in CONFIGS if t[0] == cfg][0]
+    rp, col, val = gen()
+    if c["solver"] == "sa":
+        ml = ra.ParSmoothedAggregationSolver(ctx, c["theta"], ra.Symmetric, ra.Jacobi)
+    else:
+        ml = ra.ParRugeStubenSolver(ctx, 0.25, c["coarsen"], c["interp"], ra.Classical, ra.Jacobi)
+    ml.relax_weight = OMEGA
+    ml.max_coarse = 50
+    ml.num_smooth_sweeps = sweeps
+    ml.setup(rp, col, val)
+    ml.enable_fp32()
+    As, Ps = levels(ml)
+    H = K.Hier(As, Ps, relax_type=0, sweeps=sweeps, omega=OMEGA)
+    n = As[0].n_rows
+    b = np.random.default_rng(27).standard_normal(n)
+    b32 = b.astype(np.float32)
+    xo, xo32 = H.cycle(np.zeros(n), b), H.cycle_fp32(np.zeros(n, np.float32), b32)
+    _, first = ml.tail_level()
+    assert first >= 1
+    for mode in (first, -1):
+        ml.set_tail(mode)
+        # a nonzero buffer: the cycle from zero must not read it
+        x = ra.ParVector.from_numpy(ctx, np.full(n, 7.0))
+        ml.cycle(x, ra.ParVector.from_numpy(ctx, b), x_is_zero=True)
+        assert np.array_equal(x.numpy(), xo), mode
+        x32 = ra.ParVector.from_numpy(ctx, np.full(n, 7.0, np.float32))
+        ml.cycle_fp32(x32, ra.ParVector.from_numpy(ctx, b32))
+        assert np.array_equal(x32.numpy(), xo32), mode
+
+
 @pytest.mark.parametrize("max_coarse", [400])
 def test_vcycle_large_coarse_system(ctx, max_coarse):
     """A coarsest system larger than one wavefront takes the workgroup
@@ -204,6 +275,7 @@ def test_vcycle_sliced_bit_exact(problem, ctx, mode):
     if not mats:
         pytest.skip("no CSR level operator")
     assert np.all(ml.row_caps()[[l for l, _ in mats], 0, :] == -mode)
+    ml.set_tail(-1)  # every level through the switched kernels
     try:
         H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=OMEGA)
         n = As[0].n_rows
@@ -220,6 +292,7 @@ def test_vcycle_sliced_bit_exact(problem, ctx, mode):
         for l, A in mats:
             for p in (0, 1):
                 set_kernel(A, p, before[l, 0, p])
+        ml.set_tail(0)
     assert np.array_equal(ml.row_caps(), before)
 
 
@@ -242,6 +315,7 @@ def test_vcycle_lane_rows_bit_exact(problem, ctx):
         pytest.skip("no CSR operator")
     caps = ml.row_caps()
     assert all(np.all(caps[l, m, :] == 1) for l, m, _ in mats)
+    ml.set_tail(-1)  # every level through the switched kernels
     try:
         H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=OMEGA)
         n = As[0].n_rows
@@ -258,6 +332,7 @@ def test_vcycle_lane_rows_bit_exact(problem, ctx):
         for l, m, M in mats:
             for p in (0, 1):
                 set_kernel(M, p, before[l, m, p])
+        ml.set_tail(0)
     assert np.array_equal(ml.row_caps(), before)
 
 
@@ -281,6 +356,7 @@ def test_vcycle_wave_rows_bit_exact(problem, ctx):
         for p in (0, 1):
             M.set_wave_rows(p, True)
     assert all(np.all(ml.row_caps()[l, m, :] == 2) for l, m, _ in mats)
+    ml.set_tail(-1)  # every level through the switched kernels
     try:
         H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=OMEGA)
         n = As[0].n_rows
@@ -296,6 +372,7 @@ def test_vcycle_wave_rows_bit_exact(problem, ctx):
         for l, m, M in mats:
             for p in (0, 1):
                 set_kernel(M, p, before[l, m, p])
+        ml.set_tail(0)
     assert np.array_equal(ml.row_caps(), before)
 
 
