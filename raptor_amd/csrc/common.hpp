@@ -320,13 +320,15 @@ constexpr int kStreamCaps[3] = {2048, 4096, 8192};
 // blocks.  The cap changes only which workgroup owns a row, never the order
 // of a row's sum, so results are identical for every choice.
 void csr_autotune(ramd_csr_s* A, int prec);
-// Median time (ms) of `reps` launches of run(), each after a pass that
-// writes kFlushBytes (more than the L2 and infinity caches hold): a launch
-// then finds its operands in HBM, as inside a cycle after the other
-// levels' kernels, rather than left in the caches by its own previous
-// launch.  The autotuners rank kernel choices by it.
+// Median time (ms) of `reps` launches of run().  For an operation that
+// moves fewer than kFlushBytes (more than the L2 and infinity caches hold)
+// each launch follows a pass that writes kFlushBytes, so it finds its
+// operands in HBM, as inside a cycle after the other levels' kernels,
+// rather than left in the caches by its own previous launch; a larger one
+// cannot keep its operands cached and is timed back to back, without the
+// flush's write-backs in its time.  The autotuners rank kernel choices by it.
 constexpr size_t kFlushBytes = (size_t)320 << 20;
-float cold_median_ms(ramd_ctx ctx, const std::function<void()>& run, int reps);
+float cold_median_ms(ramd_ctx ctx, const std::function<void()>& run, int reps, size_t op_bytes);
 // For a matrix with the stencil-diagonal form: times the gather kernel, the
 // LDS x-window kernel (when the window fits kDiaWin) and the plane-march
 // kernel (box geometry; several planes per workgroup) for the precision pair
@@ -386,6 +388,9 @@ void launch_dot(ramd_ctx ctx, int64_t n, const double* x, const double* y, doubl
 void launch_norm2sq(ramd_ctx ctx, int64_t n, const double* x, double* out_d);
 // Sums partials[0..np) into *out_d in a fixed order (one workgroup).
 void launch_reduce_final(ramd_ctx ctx, const double* partials, int np, double* out_d);
+// two launch_reduce_final sums (n1, n2 <= 4096) in one launch
+void launch_reduce_final_two(ramd_ctx ctx, const double* p1, int n1, double* o1, const double* p2, int n2,
+                             double* o2);
 // launch_reduce_final for np <= 4096 that also writes host[0] = *also_d and
 // host[1] = the sum (host: the device address of device-writable host
 // memory such as ctx->step_pin_d)

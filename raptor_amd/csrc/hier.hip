@@ -1419,8 +1419,7 @@ static void fpcg_rz_zap(ramd_ctx ctx, int n, const double* r, const Z* z, const 
     const int g = fixed_blocks(n);
     hipLaunchKernelGGL((fpcg_dots<Z>), dim3(g), dim3(kBlockThreads), 0, ctx->stream, n, r, z, Ap, part);
     RAMD_HIP(hipGetLastError());
-    launch_reduce_final(ctx, part, g, rz_d);
-    launch_reduce_final(ctx, part + g, g, zap_d);
+    launch_reduce_final_two(ctx, part, g, rz_d, part + g, g, zap_d);
 }
 
 // plain 2-norm (no small-entry skipping, NaN propagates)

@@ -1224,19 +1224,22 @@ __global__ void flush_kernel(uint4* __restrict__ p, int64_t n)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = make_uint4(0, 0, 0, 0);
 }
 
-float cold_median_ms(ramd_ctx ctx, const std::function<void()>& run, int reps)
+float cold_median_ms(ramd_ctx ctx, const std::function<void()>& run, int reps, size_t op_bytes)
 {
     hipStream_t s = ctx->stream;
-    if (!ctx->flush_buf) ctx->flush_buf = dmalloc(kFlushBytes);
+    const bool flush = op_bytes < kFlushBytes;
+    if (flush && !ctx->flush_buf) ctx->flush_buf = dmalloc(kFlushBytes);
     std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
     std::vector<float> ms((size_t)reps, 0.f);
     try {
         for (hipEvent_t& e : ev) RAMD_HIP(hipEventCreate(&e));
         run();  // first launch: code loading
         for (int r = 0; r < reps; r++) {
-            hipLaunchKernelGGL(flush_kernel, dim3(4096), dim3(256), 0, s, (uint4*)ctx->flush_buf,
-                               (int64_t)(kFlushBytes / sizeof(uint4)));
-            RAMD_HIP(hipGetLastError());
+            if (flush) {
+                hipLaunchKernelGGL(flush_kernel, dim3(4096), dim3(256), 0, s, (uint4*)ctx->flush_buf,
+                                   (int64_t)(kFlushBytes / sizeof(uint4)));
+                RAMD_HIP(hipGetLastError());
+            }
             RAMD_HIP(hipEventRecord(ev[2 * r], s));
             run();
             RAMD_HIP(hipEventRecord(ev[2 * r + 1], s));
@@ -1289,16 +1292,17 @@ void csr_autotune(ramd_csr_s* A, int prec)
     const int op = A->n_rows == A->n_cols ? OP_RESIDUAL : OP_SPMV;
     ramd_csr_s::RowBlocks best = A->rb[prec];
     float best_ms = 1e30f;
-    // the median of cold launches (seven for small operators, whose single
-    // launches are too short to rank reliably, three above)
+    // the median of seven launches for small operators, whose single
+    // launches are too short to rank reliably, three above
     const int reps = A->nnz < (int64_t)4 << 20 ? 7 : 3;
+    const size_t op_b = (size_t)A->nnz * (4 + w) + ((size_t)A->n_rows * 2 + (size_t)A->n_cols) * w;
     auto time_ms = [&] {
         return cold_median_ms(ctx, [&] {
             if (prec) launch_stream<float, float>(A, A->val32, nullptr, op, (const float*)t.x, (const float*)t.b,
                                                   (float*)t.y, 0.f, s);
             else launch_stream<double, double>(A, A->val, nullptr, op, (const double*)t.x, (const double*)t.b,
                                                (double*)t.y, 0.0, s);
-        }, reps);
+        }, reps, op_b);
     };
     for (int cap : kStreamCaps) {
         if (prec == 0 && cap > 4096) continue;
@@ -1418,6 +1422,8 @@ void dia_autotune(ramd_csr_s* A, int pp)
     t.part = dalloc<double>((size_t)dia_grid_bound(A));
     RAMD_HIP(hipMemsetAsync(t.x, 0, bytes, s));
     RAMD_HIP(hipMemsetAsync(t.b, 0, bytes, s));
+    // bytes an operation moves: values, mask and three vectors per row
+    const size_t op_b = (size_t)A->n_rows * ((size_t)d->nd * (pp == 0 ? 8 : 4) + 4 + 3 * w);
     // candidates: (mode, planes per workgroup)
     std::vector<std::pair<int, int>> cands = {{0, 0}};
     if (d->window.win <= kDiaWin) cands.push_back({1, 0});
@@ -1431,23 +1437,22 @@ void dia_autotune(ramd_csr_s* A, int pp)
     for (const auto& c : cands) {
         d->lds_x[pp] = c.first;
         if (c.first == 2) d->box.zc[pp] = c.second;
-        // the operation each pair runs most: the cycle's residual, the outer
-        // product with its fused dot
-        auto run = [&] {
+        // the operations each pair runs: the cycle's residual and Jacobi
+        // sweep, the outer product with its fused dot
+        auto run = [&](int op) {
             if (pp == 1)
-                launch_dia<float, float>(A, d->v32, OP_RESIDUAL, (const float*)t.x, (const float*)t.b, (float*)t.y,
-                                         0.f, s, nullptr);
+                launch_dia<float, float>(A, d->v32, op, (const float*)t.x, (const float*)t.b, (float*)t.y, 0.5f, s,
+                                         nullptr);
             else if (pp == 2)
                 launch_dia<float, double>(A, d->v32, OP_SPMV, (const double*)t.x, (const double*)t.b, (double*)t.y,
                                           0.0, s, t.part);
             else
-                launch_dia<double, double>(A, d->v64, OP_RESIDUAL, (const double*)t.x, (const double*)t.b,
-                                           (double*)t.y, 0.0, s, nullptr);
+                launch_dia<double, double>(A, d->v64, op, (const double*)t.x, (const double*)t.b, (double*)t.y, 0.5,
+                                           s, nullptr);
         };
-        // the median of seven cold launches: the candidates differ by a few
-        // percent, and back-to-back launches of a small operator find x in
-        // the caches, which a cycle's launch does not
-        const float ms = cold_median_ms(ctx, run, 7);
+        // medians of seven launches: the candidates differ by a few percent
+        float ms = cold_median_ms(ctx, [&] { run(pp == 2 ? OP_SPMV : OP_RESIDUAL); }, 7, op_b);
+        if (pp != 2) ms += cold_median_ms(ctx, [&] { run(OP_JACOBI); }, 7, op_b);
         if (ms < best) {
             best = ms;
             pick = c;
@@ -1526,6 +1531,33 @@ __global__ __launch_bounds__(kBlockThreads) void reduce_final(const double* __re
     for (int i = threadIdx.x; i < np; i += kBlockThreads) s += partials[i];
     s = block_sum(s);
     if (threadIdx.x == 0) *out = s;
+}
+
+// Two reduce_final sums in one launch (same operations, same bits):
+// *o1 = the sum of p1[0, n1), *o2 = the sum of p2[0, n2).
+__global__ __launch_bounds__(kBlockThreads) void reduce_final_two(const double* __restrict__ p1, int n1,
+                                                                  double* __restrict__ o1,
+                                                                  const double* __restrict__ p2, int n2,
+                                                                  double* __restrict__ o2)
+{
+    double s1 = 0, s2 = 0;
+    for (int i = threadIdx.x; i < n1; i += kBlockThreads) s1 += p1[i];
+    s1 = block_sum(s1);
+    for (int i = threadIdx.x; i < n2; i += kBlockThreads) s2 += p2[i];
+    s2 = block_sum(s2);
+    if (threadIdx.x == 0) {
+        *o1 = s1;
+        *o2 = s2;
+    }
+}
+
+void launch_reduce_final_two(ramd_ctx ctx, const double* p1, int n1, double* o1, const double* p2, int n2,
+                             double* o2)
+{
+    RAMD_CHECK(n1 >= 1 && n1 <= 4096 && n2 >= 1 && n2 <= 4096, RAMD_E_ARG, "reduce_final_two: %d and %d partial sums",
+               n1, n2);
+    hipLaunchKernelGGL(reduce_final_two, dim3(1), dim3(kBlockThreads), 0, ctx->stream, p1, n1, o1, p2, n2, o2);
+    RAMD_HIP(hipGetLastError());
 }
 
 // reduce_final (same operations, same bits) that also publishes two scalars
