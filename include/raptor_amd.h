@@ -125,6 +125,16 @@ int ramd_csr_set_lane_rows(ramd_csr A, int prec, int on);
  * (util/linalg/spmv.cpp:52-67) with the loads of a row spread over a
  * wavefront, for operators with few, long rows.  No reference counterpart. */
 int ramd_csr_set_wave_rows(ramd_csr A, int prec, int on);
+/* Sorted-gather order for the row-block kernel's SpMV at vector precision
+ * prec (0 FP64, 1 FP32): within each row block the entries are read in
+ * column order, so a wavefront's gathers of x touch few cache lines, and
+ * each product is placed at its stored position; rows still add their
+ * products in stored order.  on = 1 builds it for the current row blocks
+ * (extra column, position and value arrays), 0 releases it.  Identical
+ * bits; the autotuner tries it for operators with more columns than rows
+ * (restrictions R = P^T, CSR_spmv over P^T: util/linalg/spmv.cpp:52-67).
+ * No reference counterpart. */
+int ramd_csr_set_sorted_gather(ramd_csr A, int prec, int on);
 
 /* ------------------------------------------------------------------ */
 /* SpMV family (FP64; each row summed left to right in stored order, as  */
@@ -257,6 +267,10 @@ int ramd_hier_info(ramd_hier h, int* num_levels, int32_t* n_rows, int64_t* nnz_A
 /* Borrowed handles of level l's A (diagonal first) and P (NULL at the
  * coarsest level); valid until ramd_hier_destroy. */
 int ramd_hier_level(ramd_hier h, int level, ramd_csr* A, ramd_csr* P);
+/* Level l's restriction R = P_l^T as the cycle stores it (the gather form
+ * of P->mult_T, util/linalg/par_spmv.cpp:157-187), NULL on the coarsest
+ * level; a borrowed handle, for reading and kernel overrides. */
+int ramd_hier_restriction(ramd_hier h, int level, ramd_csr* R);
 /* C/F states (1 = C, 0 = F, -2 = no neighbours) of level l, or for SA the
  * aggregate of each row (its root row, -1 for none). */
 int ramd_hier_states(ramd_hier h, int level, int32_t* states);
@@ -265,7 +279,8 @@ int ramd_hier_states(ramd_hier h, int level, int32_t* states);
  * -1 where the sliced-row kernel (64 rows per slice, one lane per row) was
  * measured faster, -2 the sliced-row kernel with 16-bit column offsets,
  * 1 where the lane-per-row kernel over the CSR arrays was, 2 where the
- * wavefront-per-row kernel was,
+ * wavefront-per-row kernel was, minus the cap (-2048, -4096, -8192) where
+ * the row-block kernel in sorted-gather order was,
  * 0 for a stencil-diagonal operator or an absent one.  No
  * reference counterpart. */
 int ramd_hier_row_caps(ramd_hier h, int32_t* caps);

@@ -252,6 +252,12 @@ class ParCSRMatrix:
         one wavefront per row, False the row-block kernel (identical bits)."""
         check(lib().ramd_csr_set_wave_rows(self.h, int(prec), int(bool(on))))
 
+    def set_sorted_gather(self, prec, on):
+        """Row-block kernel's SpMV for FP64 (prec 0) or FP32 (prec 1) vectors
+        with the entries read in column order within each block (True) or in
+        stored order (False); identical bits."""
+        check(lib().ramd_csr_set_sorted_gather(self.h, int(prec), int(bool(on))))
+
     def set_values(self, vals):
         """New values (a device ParVector of nnz entries, stored order) for the same pattern."""
         check(lib().ramd_csr_set_values(self.h, vals.ptr))
@@ -346,6 +352,12 @@ class ParMultilevel:
         A = ParCSRMatrix(self.ctx, a, owned=False)
         P = ParCSRMatrix(self.ctx, p, owned=False) if p.value else None
         return A, P
+
+    def restriction(self, l):
+        """Level l's restriction operator R = P_l^T (None on the coarsest level)."""
+        r = C.c_void_p()
+        check(lib().ramd_hier_restriction(self.h, l, C.byref(r)))
+        return ParCSRMatrix(self.ctx, r, owned=False) if r.value else None
 
     def states(self, l):
         n, _, _ = self.level_sizes()

@@ -90,6 +90,7 @@ _SIGS = {
     "ramd_csr_set_sliced": (P, C.c_int, C.c_int),
     "ramd_csr_set_lane_rows": (P, C.c_int, C.c_int),
     "ramd_csr_set_wave_rows": (P, C.c_int, C.c_int),
+    "ramd_csr_set_sorted_gather": (P, C.c_int, C.c_int),
     "ramd_spmv": (P, P, P),
     "ramd_spmv_append": (P, P, P),
     "ramd_residual": (P, P, P, P),
@@ -108,6 +109,7 @@ _SIGS = {
     "ramd_hier_destroy": (P,),
     "ramd_hier_info": (P, C.POINTER(C.c_int), PI32, PI64, PI64),
     "ramd_hier_level": (P, C.c_int, C.POINTER(P), C.POINTER(P)),
+    "ramd_hier_restriction": (P, C.c_int, C.POINTER(C.c_void_p)),
     "ramd_hier_states": (P, C.c_int, PI32),
     "ramd_hier_setup_times": (P, PD),
     "ramd_hier_row_caps": (P, PI32),

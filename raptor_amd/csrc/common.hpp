@@ -225,6 +225,22 @@ struct Sliced {
     double* v64 = nullptr;    // device, entries (built on demand)
     float* v32 = nullptr;
 };
+// Sorted-gather order for the row-block kernels' products (restrictions,
+// whose rows gather the finer vector at scattered columns): within each row
+// block of one partition, the entries in ascending column order (ties in
+// stored order) with each entry's position in the block's stored order.  A
+// wavefront's gathers then touch a few cache lines instead of dozens; the
+// products land in LDS at their stored positions and each row adds them in
+// stored order as before, so the results are the same bits.  The arrays run
+// parallel to the CSR arrays (the same block bounds); a block holding one
+// row longer than the cap keeps its stored order.
+struct SortedGather {
+    int cap = 0;
+    const int* blocks = nullptr;  // the partition it was built for
+    int* col = nullptr;           // device, nnz
+    uint16_t* pos = nullptr;      // device, nnz: position within the block
+    void* val = nullptr;          // device, nnz: FP64 (vectors FP64) or FP32 values
+};
 }  // namespace ramd
 
 struct ramd_csr_s {
@@ -235,6 +251,9 @@ struct ramd_csr_s {
     bool sl16[2] = {false, false};   // ... with its 16-bit column offsets
     bool lr_on[2] = {false, false};  // per vector precision: one lane per CSR row
     bool wr_on[2] = {false, false};  // per vector precision: one wavefront per CSR row
+    // per vector precision: the row-block kernels' SpMV in sorted-gather order
+    // (used while built for that precision's row blocks)
+    ramd::SortedGather* sg[2] = {nullptr, nullptr};
     int32_t n_rows = 0, n_cols = 0;
     int64_t nnz = 0;
     int* rowptr = nullptr;   // device, n_rows+1
@@ -281,6 +300,10 @@ void csr_free(ramd_csr_s* A);
 void sliced_build(ramd_csr_s* A);
 void sliced_values(ramd_csr_s* A, int prec);
 void sliced_free(ramd_csr_s* A);
+// Sorted-gather order for vector precision prec over A->rb[prec] (values
+// from A->val or A->val32), and its release.
+void sg_build(ramd_csr_s* A, int prec);
+void sg_free(ramd_csr_s* A, int prec);
 // True when every row has its diagonal first or no diagonal at all.
 bool host_diag_first(int n_rows, const int* rowptr, const int* col);
 // Builds A->dia from host arrays when the matrix qualifies (square, at least

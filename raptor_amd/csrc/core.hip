@@ -545,9 +545,110 @@ void sliced_values(ramd_csr_s* A, int prec)
     }
 }
 
+// One workgroup per row block: the block's (column, position) keys sorted
+// in LDS (bitonic, up to kSortedGatherMax entries), then the columns,
+// positions and values written in that order.  A block of one row longer
+// than the cap is copied in stored order (its kernel path reads it so).
+constexpr int kSortedGatherMax = 8192;
+template <typename V>
+__global__ __launch_bounds__(1024) void sg_build_kernel(const int* __restrict__ rowptr, const int* __restrict__ blocks,
+                                                        int cap, const int* __restrict__ col,
+                                                        const V* __restrict__ val, int* __restrict__ scol,
+                                                        uint16_t* __restrict__ spos, V* __restrict__ sval)
+{
+    __shared__ unsigned long long key[kSortedGatherMax];
+    const int s = rowptr[blocks[blockIdx.x]], e = rowptr[blocks[blockIdx.x + 1]];
+    const int len = e - s;
+    const int tid = threadIdx.x;
+    if (len > cap) {
+        for (int j = tid; j < len; j += 1024) {
+            scol[s + j] = col[s + j];
+            spos[s + j] = 0;
+            sval[s + j] = val[s + j];
+        }
+        return;
+    }
+    int P = 1;
+    while (P < len) P <<= 1;
+    for (int j = tid; j < P; j += 1024)
+        key[j] = j < len ? ((unsigned long long)(unsigned)col[s + j] << 16) | (unsigned)j : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1)
+        for (int h = k >> 1; h > 0; h >>= 1) {
+            for (int i = tid; i < P; i += 1024) {
+                const int l = i ^ h;
+                if (l > i) {
+                    const unsigned long long a = key[i], c = key[l];
+                    if ((a > c) == ((i & k) == 0)) {
+                        key[i] = c;
+                        key[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int j = tid; j < len; j += 1024) {
+        const unsigned long long kv = key[j];
+        const int p = (int)(kv & 0xFFFFull);
+        scol[s + j] = (int)(kv >> 16);
+        spos[s + j] = (uint16_t)p;
+        sval[s + j] = val[s + p];
+    }
+}
+
+void sg_free(ramd_csr_s* A, int prec)
+{
+    SortedGather*& g = A->sg[prec];
+    if (!g) return;
+    RAMD_HIP(hipStreamSynchronize(A->ctx->stream));
+    dfree(g->col);
+    dfree(g->pos);
+    dfree(g->val);
+    delete g;
+    g = nullptr;
+    A->forms++;
+}
+
+void sg_build(ramd_csr_s* A, int prec)
+{
+    const ramd_csr_s::RowBlocks& rb = A->rb[prec];
+    RAMD_CHECK(rb.cap >= 1 && rb.cap <= kSortedGatherMax, RAMD_E_STATE, "sorted gather: row-block cap %d", rb.cap);
+    RAMD_CHECK(prec == 0 || A->val32, RAMD_E_STATE, "sorted gather: FP32 values were not built");
+    sg_free(A, prec);
+    const size_t nz = (size_t)std::max<int64_t>(A->nnz, 1);
+    SortedGather* g = new SortedGather();
+    try {
+        g->cap = rb.cap;
+        g->blocks = rb.d;
+        g->col = dalloc<int>(nz);
+        g->pos = dalloc<uint16_t>(nz);
+        g->val = dmalloc(nz * (prec ? sizeof(float) : sizeof(double)));
+        if (rb.n > 0) {
+            hipStream_t s = A->ctx->stream;
+            if (prec)
+                hipLaunchKernelGGL(sg_build_kernel<float>, dim3(rb.n), dim3(1024), 0, s, A->rowptr, rb.d, rb.cap,
+                                   A->col, A->val32, g->col, g->pos, (float*)g->val);
+            else
+                hipLaunchKernelGGL(sg_build_kernel<double>, dim3(rb.n), dim3(1024), 0, s, A->rowptr, rb.d, rb.cap,
+                                   A->col, A->val, g->col, g->pos, (double*)g->val);
+            RAMD_HIP(hipGetLastError());
+        }
+    } catch (...) {
+        dfree(g->col);
+        dfree(g->pos);
+        dfree(g->val);
+        delete g;
+        throw;
+    }
+    A->sg[prec] = g;
+    A->forms++;
+}
+
 void csr_free(ramd_csr_s* A)
 {
     if (!A) return;
+    sg_free(A, 0);
+    sg_free(A, 1);
     sliced_free(A);
     if (A->trans) csr_free(A->trans);
     dfree(A->rowptr);
@@ -855,6 +956,8 @@ void csr_set_values(ramd_csr_s* A, const double* val_d)
         if (A->sl->v64) sliced_values(A, 0);
         if (A->sl->v32) sliced_values(A, 1);
     }
+    for (int p = 0; p < 2; p++)  // the sorted values follow (same order)
+        if (A->sg[p]) sg_build(A, p);
     if (Dia* d = A->dia) {
         const int n = A->n_rows;
         hipLaunchKernelGGL(dia_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, d->args, d->nd, A->rowptr,
@@ -1136,6 +1239,23 @@ int ramd_csr_set_lane_rows(ramd_csr A, int prec, int on)
         RAMD_CHECK(prec == 0 || A->val32 || !on, RAMD_E_STATE, "ramd_csr_set_lane_rows: FP32 values were not built");
         A->lr_on[prec] = on != 0;
         if (on) A->sl_on[prec] = A->sl16[prec] = A->wr_on[prec] = false;
+        A->forms++;
+    });
+}
+
+int ramd_csr_set_sorted_gather(ramd_csr A, int prec, int on)
+{
+    return guarded([&] {
+        RAMD_CHECK(A && (prec == 0 || prec == 1) && (on == 0 || on == 1), RAMD_E_ARG,
+                   "ramd_csr_set_sorted_gather: bad argument");
+        RAMD_CHECK(!A->dia, RAMD_E_STATE, "ramd_csr_set_sorted_gather: matrix uses its stencil-diagonal form");
+        A->ctx->set_current();
+        if (on) {
+            A->sl_on[prec] = A->sl16[prec] = A->lr_on[prec] = A->wr_on[prec] = false;
+            sg_build(A, prec);
+        } else {
+            sg_free(A, prec);
+        }
         A->forms++;
     });
 }

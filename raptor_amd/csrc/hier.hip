@@ -2095,6 +2095,15 @@ int ramd_hier_level(ramd_hier h, int level, ramd_csr* A, ramd_csr* P)
     });
 }
 
+int ramd_hier_restriction(ramd_hier h, int level, ramd_csr* R)
+{
+    return guarded([&] {
+        RAMD_CHECK(h && R && level >= 0 && level < (int)h->levels.size(), RAMD_E_ARG,
+                   "ramd_hier_restriction: bad argument");
+        *R = h->levels[level].R;
+    });
+}
+
 int ramd_hier_states(ramd_hier h, int level, int32_t* states)
 {
     return guarded([&] {
@@ -2120,7 +2129,8 @@ int ramd_hier_row_caps(ramd_hier h, int32_t* caps)
                             ? (M[m]->wr_on[p]   ? 2
                                : M[m]->sl_on[p] ? (M[m]->sl16[p] ? -2 : -1)
                                : M[m]->lr_on[p] ? 1
-                                                : M[m]->rb[p].cap)
+                               : (M[m]->sg[p] && M[m]->sg[p]->blocks == M[m]->rb[p].d) ? -M[m]->rb[p].cap
+                                                                                         : M[m]->rb[p].cap)
                             : 0;
         }
     });

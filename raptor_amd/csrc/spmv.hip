@@ -36,13 +36,16 @@ __device__ __forceinline__ X lds_row_sum(const X* prod, int k0, int k1, X acc)
     return acc;
 }
 
-template <int OP, typename V, typename X, int CAP>
+// SGA: col and val are a SortedGather's arrays (SpMV and its first-sweep
+// form only), spos the entries' positions in their block.
+template <int OP, typename V, typename X, int CAP, bool SGA = false>
 __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
     const int* __restrict__ rowptr, const int* __restrict__ col, const V* __restrict__ val,
     const int* __restrict__ blocks, int nb, const X* __restrict__ x, const X* __restrict__ b,
     X* __restrict__ y, X omega, int diag_first, double* __restrict__ dotp, const X* __restrict__ dvec,
-    X* __restrict__ y2)
+    X* __restrict__ y2, const uint16_t* __restrict__ spos)
 {
+    static_assert(!SGA || OP == OP_SPMV || OP == OP_SPMV_JFZ, "sorted gathers serve the SpMV forms only");
     __shared__ X prod[CAP];
     const int blk = xcd_remap(blockIdx.x, nb);
     const int r0 = blocks[blk], r1 = blocks[blk + 1];
@@ -109,7 +112,35 @@ __global__ __launch_bounds__(kBlockThreads) void stream_kernel(
         return;
     }
 
-    if constexpr (sizeof(X) == 4) {
+    if constexpr (SGA) {
+        // phase 1 (sorted gathers): lane j of the block takes the block's j-th
+        // entry in column order, so a wavefront's 64 gathers fall on a few
+        // adjacent cache lines; the product goes to its stored position.
+        // Eight entries per lane and step, their loads issued first.
+        constexpr int U = CAP / kBlockThreads, UB = 8;
+        const int len = e - s;
+    #pragma unroll
+        for (int u0 = 0; u0 < U; u0 += UB) {
+            if (u0 * kBlockThreads >= len) break;
+            int cc[UB], pp[UB];
+            X vv[UB], xx[UB];
+    #pragma unroll
+            for (int t = 0; t < UB; t++) {
+                const int j = tid + (u0 + t) * kBlockThreads;
+                if (j < len) {  // streamed once: past the caches
+                    cc[t] = __builtin_nontemporal_load(&col[s + j]);
+                    vv[t] = X(__builtin_nontemporal_load(&val[s + j]));
+                    pp[t] = __builtin_nontemporal_load(&spos[s + j]);
+                }
+            }
+    #pragma unroll
+            for (int t = 0; t < UB; t++)
+                if (tid + (u0 + t) * kBlockThreads < len) xx[t] = x[cc[t]];
+    #pragma unroll
+            for (int t = 0; t < UB; t++)
+                if (tid + (u0 + t) * kBlockThreads < len) prod[pp[t]] = vv[t] * xx[t];
+        }
+    } else if constexpr (sizeof(X) == 4) {
         // phase 1 (FP32 vectors): the block's entries [s, e) are read in aligned
         // groups of four -- one 16-byte column load and one 16-byte value load per
         // group and lane instead of four of each -- then the gathers, then the
@@ -1009,30 +1040,44 @@ static void launch_stream_cap(const ramd_csr_s* A, const ramd_csr_s::RowBlocks& 
     const int nb = rb.n;
     const dim3 grid(nb), block(kBlockThreads);
     const int df = (int)A->diag_first;
+    // the sorted-gather order, when built for these row blocks and values
+    // (it holds the CSR values of the vector precision: FP64 or FP32)
+    const SortedGather* g = A->sg[sizeof(X) == 4 ? 1 : 0];
+    if (g && g->blocks == rb.d && (op == OP_SPMV || op == OP_SPMV_JFZ) && sizeof(V) == sizeof(X) &&
+        (sizeof(V) == 4 ? (const void*)val == (const void*)A->val32 : (const void*)val == (const void*)A->val)) {
+        const V* gv = static_cast<const V*>(g->val);
+        if (op == OP_SPMV)
+            hipLaunchKernelGGL((stream_kernel<OP_SPMV, V, X, CAP, true>), grid, block, 0, s, A->rowptr, g->col, gv,
+                               rb.d, nb, x, b, y, omega, df, dotp, nullptr, nullptr, g->pos);
+        else
+            hipLaunchKernelGGL((stream_kernel<OP_SPMV_JFZ, V, X, CAP, true>), grid, block, 0, s, A->rowptr, g->col,
+                               gv, rb.d, nb, x, b, y, omega, df, nullptr, dvec, y2, g->pos);
+        return;
+    }
     switch (op) {
     case OP_SPMV:
         hipLaunchKernelGGL((stream_kernel<OP_SPMV, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d, nb,
-                           x, b, y, omega, df, dotp, nullptr, nullptr);
+                           x, b, y, omega, df, dotp, nullptr, nullptr, nullptr);
         break;
     case OP_SPMV_JFZ:
         hipLaunchKernelGGL((stream_kernel<OP_SPMV_JFZ, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d,
-                           nb, x, b, y, omega, df, nullptr, dvec, y2);
+                           nb, x, b, y, omega, df, nullptr, dvec, y2, nullptr);
         break;
     case OP_APPEND:
         hipLaunchKernelGGL((stream_kernel<OP_APPEND, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d,
-                           nb, x, b, y, omega, df, nullptr, nullptr, nullptr);
+                           nb, x, b, y, omega, df, nullptr, nullptr, nullptr, nullptr);
         break;
     case OP_RESIDUAL:
         hipLaunchKernelGGL((stream_kernel<OP_RESIDUAL, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d,
-                           nb, x, b, y, omega, df, nullptr, nullptr, nullptr);
+                           nb, x, b, y, omega, df, nullptr, nullptr, nullptr, nullptr);
         break;
     case OP_JACOBI:
         hipLaunchKernelGGL((stream_kernel<OP_JACOBI, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d,
-                           nb, x, b, y, omega, df, nullptr, dvec, nullptr);
+                           nb, x, b, y, omega, df, nullptr, dvec, nullptr, nullptr);
         break;
     case OP_ADD_INTO:
         hipLaunchKernelGGL((stream_kernel<OP_ADD_INTO, V, X, CAP>), grid, block, 0, s, A->rowptr, A->col, val, rb.d,
-                           nb, x, b, y, omega, df, nullptr, nullptr, nullptr);
+                           nb, x, b, y, omega, df, nullptr, nullptr, nullptr, nullptr);
         break;
     default:
         fail(RAMD_E_ARG, "unknown stream op %d", op);
@@ -1272,6 +1317,7 @@ void csr_autotune(ramd_csr_s* A, int prec)
     A->sl16[prec] = false;
     A->lr_on[prec] = false;
     A->wr_on[prec] = false;
+    sg_free(A, prec);
     const size_t w = prec ? sizeof(float) : sizeof(double);
     struct Scratch {
         void *x = nullptr, *b = nullptr, *y = nullptr;
@@ -1329,6 +1375,15 @@ void csr_autotune(ramd_csr_s* A, int prec)
     for (int*& p : t.blocks)
         if (p == best.d) p = nullptr;
     A->rb[prec] = best;
+    // restrictions (more columns than rows: rows gather the finer vector at
+    // scattered columns): the same blocks with the entries in sorted-gather
+    // order
+    if (A->n_cols > A->n_rows) {
+        sg_build(A, prec);
+        const float ms = time_ms();
+        if (ms < best_ms) best_ms = ms;
+        else sg_free(A, prec);
+    }
     // lanes straight over the CSR rows (no extra storage), for short rows:
     // a lane adds its whole row alone
     if ((double)A->nnz <= kLaneRowsMaxLen * (double)A->n_rows) {
@@ -1394,6 +1449,7 @@ void csr_autotune(ramd_csr_s* A, int prec)
             A->lr_on[prec] = lr;
         }
     }
+    if (A->lr_on[prec] || A->sl_on[prec] || A->wr_on[prec]) sg_free(A, prec);
 }
 
 void dia_autotune(ramd_csr_s* A, int pp)

@@ -122,16 +122,20 @@ def test_hierarchy_matches_reference(problem):
 
 def set_kernel(M, p, code):
     """Selects the CSR kernel of row_caps' encoding (-1/-2 sliced, 1 lane
-    rows, 2 wavefront rows, a cap: the tuned row blocks) for precision p."""
+    rows, 2 wavefront rows, a cap: the tuned row blocks, minus a cap: those
+    blocks in sorted-gather order) for precision p."""
     M.set_wave_rows(p, False)
     M.set_lane_rows(p, False)
     M.set_sliced(p, False)
+    M.set_sorted_gather(p, False)
     if code in (-1, -2):
         M.set_sliced(p, int(-code))
     elif code == 1:
         M.set_lane_rows(p, True)
     elif code == 2:
         M.set_wave_rows(p, True)
+    elif code < -2:
+        M.set_sorted_gather(p, True)
 
 
 def test_vcycle_bit_exact(problem, ctx):
@@ -327,6 +331,46 @@ def test_vcycle_lane_rows_bit_exact(problem, ctx):
         x32 = ra.ParVector(ctx, n, np.float32)
         b32d = ra.ParVector.from_numpy(ctx, b32)
         ml.cycle_fp32(x32, b32d)
+        assert np.array_equal(x32.numpy(), H.cycle_fp32(np.zeros(n, np.float32), b32))
+    finally:
+        for l, m, M in mats:
+            for p in (0, 1):
+                set_kernel(M, p, before[l, m, p])
+        ml.set_tail(0)
+    assert np.array_equal(ml.row_caps(), before)
+
+
+def test_vcycle_sorted_gather_bit_exact(problem, ctx):
+    """Every restriction R = P^T switched to the row-block kernel with its
+    entries read in sorted-gather order: FP64 and FP32 cycles still
+    bit-identical to the oracle.  The tuned choices are restored."""
+    ml, As, Ps = problem["ml"], problem["As"], problem["Ps"]
+    ml.enable_fp32()
+    before = ml.row_caps()
+    mats = []
+    for l in range(ml.num_levels - 1):
+        R = ml.restriction(l)
+        if R is not None and R.nnz > 0 and R.diagonals() == 0:
+            mats.append((l, 2, R))
+    if not mats:
+        pytest.skip("no CSR restriction")
+    for _, _, M in mats:
+        for p in (0, 1):
+            set_kernel(M, p, 0)
+            M.set_sorted_gather(p, True)
+    caps = ml.row_caps()
+    assert all(np.all(caps[l, m, :] < -2) for l, m, _ in mats)
+    ml.set_tail(-1)  # every level through the switched kernels
+    try:
+        H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=OMEGA)
+        n = As[0].n_rows
+        b = np.random.default_rng(28).standard_normal(n)
+        x = ra.ParVector.from_numpy(ctx, np.zeros(n))
+        ml.cycle(x, ra.ParVector.from_numpy(ctx, b), x_is_zero=True)
+        assert np.array_equal(x.numpy(), H.cycle(np.zeros(n), b))
+        b32 = b.astype(np.float32)
+        x32 = ra.ParVector(ctx, n, np.float32)
+        ml.cycle_fp32(x32, ra.ParVector.from_numpy(ctx, b32))
         assert np.array_equal(x32.numpy(), H.cycle_fp32(np.zeros(n, np.float32), b32))
     finally:
         for l, m, M in mats:
@@ -574,8 +618,10 @@ def test_row_cap_autotuner_choices(problem):
     ml.enable_fp32()
     caps = ml.row_caps()
     assert caps.shape == (ml.num_levels, 3, 2)
-    assert set(np.unique(caps)) <= {-2, -1, 0, 1, 2, 2048, 4096, 8192}  # -1/-2 sliced, 1 lane, 2 wave rows
-    assert not np.any(caps[:, :, 0] == 8192)
+    # -1/-2 sliced, 1 lane, 2 wave rows, minus a cap: sorted-gather order
+    assert set(np.unique(caps)) <= {-8192, -4096, -2048, -2, -1, 0, 1, 2, 2048, 4096, 8192}
+    assert not np.any(np.abs(caps[:, :, 0]) == 8192)
+    assert np.all(caps[:, :2, :] >= -2)  # sorted gathers only for restrictions
     assert np.all(caps[-1, 1:] == 0)  # no P, R at the coarsest level
 
 
