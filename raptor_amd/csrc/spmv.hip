@@ -1483,10 +1483,15 @@ void dia_autotune(ramd_csr_s* A, int pp)
     // candidates: (mode, planes per workgroup)
     std::vector<std::pair<int, int>> cands = {{0, 0}};
     if (d->window.win <= kDiaWin) cands.push_back({1, 0});
-    // the plane-march kernel with four planes per workgroup: 8, 16 and 32
-    // planes time within 0.3-1.2% of it (the whole 256^3 FP32-cycle solve:
-    // 24.55, 24.62 and 24.77 ms), too close to rank by timing here
-    if (d->box.L > 0) cands.push_back({2, kBoxPlanes});
+    // the plane-march kernel with four and with eight planes per workgroup
+    // (fewer planes: more workgroups in flight; more: fewer halo planes of x
+    // loaded, which matters most for FP64 x); 16 and 32 planes were slower
+    // on the whole 256^3 FP32-cycle solve (24.55, 24.62, 24.77 ms for 4, 8,
+    // 16 planes on every operator)
+    if (d->box.L > 0) {
+        cands.push_back({2, kBoxPlanes});
+        cands.push_back({2, 2 * kBoxPlanes});
+    }
     float best = 1e30f;
     std::pair<int, int> pick = cands[0];
     const int zc0 = d->box.zc[pp];
