@@ -148,6 +148,36 @@ def test_sliced_kernel_bit_exact(ctx, name, A, layout):
     assert np.array_equal(y3.numpy(), y.numpy())
 
 
+@pytest.mark.parametrize("name,A", cases(), ids=[c[0] for c in cases()])
+def test_sorted_gather_bit_exact(ctx, name, A):
+    """The row-block SpMV with each block's entries read in column order
+    (sorted-gather order; rows longer than the block cap -- the long_rows
+    case -- keep their stored order) gives the oracle's bits, before and
+    after the values change, and the stored-order kernel again once it is
+    released."""
+    rng = np.random.default_rng(33)
+    x = rng.standard_normal(A.n_cols)
+    D = to_dev(ctx, A)
+    if D.diagonals():  # the stencil-diagonal form has no row blocks to reorder
+        with pytest.raises(ra.RamdError):
+            D.set_sorted_gather(0, True)
+        return
+    D.set_sorted_gather(0, True)
+    y = ra.ParVector(ctx, A.n_rows)
+    D.mult(dev(ctx, x), y)
+    assert np.array_equal(y.numpy(), K.spmv(A, x))
+    # new values: the sorted copy follows
+    v2 = rng.standard_normal(A.col.size)
+    D.set_values(dev(ctx, v2))
+    A2 = K.Csr(A.rowptr, A.col, v2, A.n_cols)
+    D.mult(dev(ctx, x), y)
+    assert np.array_equal(y.numpy(), K.spmv(A2, x))
+    D.set_sorted_gather(0, False)
+    y2 = ra.ParVector(ctx, A.n_rows)
+    D.mult(dev(ctx, x), y2)
+    assert np.array_equal(y2.numpy(), y.numpy())
+
+
 @pytest.mark.parametrize("mode,planes", [(0, 0), (1, 0), (2, 4), (2, 7)])
 @pytest.mark.parametrize("name,gen", [("aniso64", lambda: ra.gallery_aniso2d(64, 0.001, math.pi / 8)),
                                       ("lap27_20", lambda: ra.gallery_laplace27(20)),
