@@ -846,15 +846,38 @@ __global__ __launch_bounds__(kBlockThreads) void dia_box_kernel(int n, int L, in
     double dsum = 0.0;
     constexpr int CH = dia_chunk<V>();
     const int nsl = dia_slots<V>(nd);
+    // FP32 values (all of a row's in one chunk): the next plane's row mask
+    // and values are loaded one plane ahead, so they arrive while this
+    // plane's rows run
+    constexpr bool PF = sizeof(V) == 4;
+    V vpf[PF ? CH : 1];
+    unsigned mpf = 0u;
+    if constexpr (PF) {
+        const int r0 = col_ok ? z0 * P + (y0 + ty) * L + x0 + tx : 0;
+        mpf = col_ok ? mask[r0] : 0u;
+        dia_load<V, CH, ND>(val, 0, r0, n, nsl, vpf);
+    }
     for (int z = z0; z < z1; z++) {
         store_plane(z + 1, nx);
         if (z + 2 <= z1) load_plane(z + 2, nx);
         const int row = z * P + (y0 + ty) * L + x0 + tx;
         const int vrow = col_ok ? row : 0;
         // the row's mask and first value chunk are in flight across the barrier
-        const unsigned m = col_ok ? mask[vrow] : 0u;
+        unsigned m;
         V vv0[CH];
-        dia_load<V, CH, ND>(val, 0, vrow, n, nsl, vv0);
+        if constexpr (PF) {
+            m = mpf;
+#pragma unroll
+            for (int t = 0; t < CH; t++) vv0[t] = vpf[t];
+            if (z + 1 < z1) {
+                const int rn = col_ok ? row + P : 0;
+                mpf = col_ok ? mask[rn] : 0u;
+                dia_load<V, CH, ND>(val, 0, rn, n, nsl, vpf);
+            }
+        } else {
+            m = col_ok ? mask[vrow] : 0u;
+            dia_load<V, CH, ND>(val, 0, vrow, n, nsl, vv0);
+        }
         __syncthreads();
         X acc = (OP == OP_RESIDUAL && col_ok) ? b[vrow] : X(0);
         X diag = 0;

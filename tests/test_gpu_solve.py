@@ -239,6 +239,40 @@ def test_vcycle_sweep_counts(ctx, cfg, sweeps):
         assert np.array_equal(x32.numpy(), xo32), mode
 
 
+@pytest.mark.parametrize("planes", [4, 7])
+@pytest.mark.parametrize("cfg", ["lap27_16_pmis_ext", "c5_fv7_20_sa"])
+def test_vcycle_plane_march_bit_exact(ctx, cfg, planes):
+    """The fine level's residual and Jacobi sweeps on the plane-march kernel
+    (FP64 values with FP64 vectors, FP32 values with FP32 vectors, the
+    latter loading the next plane's values one plane ahead), whatever the
+    tuner picked: FP64 and FP32 cycles bit-identical to the oracle's."""
+    name, gen, c = [t for t in CONFIGS if t[0] == cfg][0]
+    rp, col, val = gen()
+    ml = build(ctx, rp, col, val, c)
+    ml.enable_fp32()
+    A0 = ml.level(0)[0]
+    assert A0.diagonals() > 0 and A0.dia_staging(0)[2] > 0  # a box geometry
+    before = [A0.dia_staging(pp)[:2] for pp in (0, 1)]
+    try:
+        for pp in (0, 1):
+            A0.set_dia_staging(pp, 2, planes)
+        As, Ps = levels(ml)
+        H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=OMEGA)
+        n = As[0].n_rows
+        b = np.random.default_rng(29).standard_normal(n)
+        x = ra.ParVector.from_numpy(ctx, np.zeros(n))
+        ml.cycle(x, ra.ParVector.from_numpy(ctx, b), x_is_zero=True)
+        assert np.array_equal(x.numpy(), H.cycle(np.zeros(n), b))
+        b32 = b.astype(np.float32)
+        x32 = ra.ParVector(ctx, n, np.float32)
+        ml.cycle_fp32(x32, ra.ParVector.from_numpy(ctx, b32))
+        assert np.array_equal(x32.numpy(), H.cycle_fp32(np.zeros(n, np.float32), b32))
+    finally:
+        for pp in (0, 1):
+            mode, pl = before[pp]
+            A0.set_dia_staging(pp, mode, pl if mode == 2 else 0)
+
+
 @pytest.mark.parametrize("max_coarse", [400])
 def test_vcycle_large_coarse_system(ctx, max_coarse):
     """A coarsest system larger than one wavefront takes the workgroup
