@@ -846,10 +846,10 @@ __global__ __launch_bounds__(kBlockThreads) void dia_box_kernel(int n, int L, in
     double dsum = 0.0;
     constexpr int CH = dia_chunk<V>();
     const int nsl = dia_slots<V>(nd);
-    // FP32 values (all of a row's in one chunk): the next plane's row mask
-    // and values are loaded one plane ahead, so they arrive while this
-    // plane's rows run
-    constexpr bool PF = sizeof(V) == 4;
+    // the next plane's row mask and first value chunk (all of a row's FP32
+    // values, ten FP64 ones) are loaded one plane ahead, so they arrive
+    // while this plane's rows run
+    constexpr bool PF = true;
     V vpf[PF ? CH : 1];
     unsigned mpf = 0u;
     if constexpr (PF) {
