@@ -337,7 +337,7 @@ int64_t device_rowptr(ramd_ctx ctx, const int* cnt, int n, int* rp, const char* 
 ramd_csr_s* csr_transpose(ramd_csr_s* A);
 void csr_make_fp32(ramd_csr_s* A);
 // Row-block caps the LDS-staged kernels can use (products per workgroup).
-constexpr int kStreamCaps[3] = {2048, 4096, 8192};
+constexpr int kStreamCaps[4] = {1024, 2048, 4096, 8192};
 // Times the CSR kernels of A with every cap for vectors of precision prec
 // (0 FP64, 1 FP32; the FP32 values must exist) and keeps the fastest row
 // blocks.  The cap changes only which workgroup owns a row, never the order

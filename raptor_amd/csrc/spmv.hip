@@ -1219,6 +1219,7 @@ static void launch_stream(const ramd_csr_s* A, const V* val, const V* dval, int 
     }
     const ramd_csr_s::RowBlocks& rb = A->rb[sizeof(X) == 4 ? 1 : 0];
     switch (rb.cap) {
+    case 1024: launch_stream_cap<V, X, 1024>(A, rb, val, op, x, b, y, omega, s, dotp, dvec, y2); break;
     case 2048: launch_stream_cap<V, X, 2048>(A, rb, val, op, x, b, y, omega, s, dotp, dvec, y2); break;
     case 8192:
         // 8192 FP64 products would fill the whole 64 KiB of LDS a workgroup may use

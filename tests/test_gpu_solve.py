@@ -653,7 +653,7 @@ def test_row_cap_autotuner_choices(problem):
     caps = ml.row_caps()
     assert caps.shape == (ml.num_levels, 3, 2)
     # -1/-2 sliced, 1 lane, 2 wave rows, minus a cap: sorted-gather order
-    assert set(np.unique(caps)) <= {-8192, -4096, -2048, -2, -1, 0, 1, 2, 2048, 4096, 8192}
+    assert set(np.unique(caps)) <= {-8192, -4096, -2048, -1024, -2, -1, 0, 1, 2, 1024, 2048, 4096, 8192}
     assert not np.any(np.abs(caps[:, :, 0]) == 8192)
     assert np.all(caps[:, :2, :] >= -2)  # sorted gathers only for restrictions
     assert np.all(caps[-1, 1:] == 0)  # no P, R at the coarsest level
