@@ -745,8 +745,7 @@ __global__ __launch_bounds__(256) void bcol_final(int np, int k, const double* _
 {
     __shared__ double sh[256];
     const int c = blockIdx.x;
-    double s = 0;
-    for (int g = threadIdx.x; g < np; g += 256) s += partials[(size_t)g * k + c];
+    const double s = strided_sum(partials + c, 0, np, (size_t)k);
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {

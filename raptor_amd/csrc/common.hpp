@@ -63,6 +63,27 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb)
 }
 
 
+// Thread t's share of a one-workgroup reduction: p[i0 + t], p[i0 + t + T],
+// ... below i1 (T = threads per workgroup, `stride` apart in memory), added
+// in that order; eight loads are issued before their additions, so a
+// thread waits for one memory round trip per eight entries instead of per
+// entry (the additions and their order are the plain loop's).
+__device__ __forceinline__ double strided_sum(const double* __restrict__ p, int i0, int i1, size_t stride = 1)
+{
+    constexpr int T = kBlockThreads;
+    double s = 0;
+    int i = i0 + (int)threadIdx.x;
+    for (; i + 7 * T < i1; i += 8 * T) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = p[(size_t)(i + u * T) * stride];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += v[u];
+    }
+    for (; i < i1; i += T) s += p[(size_t)i * stride];
+    return s;
+}
+
 // Workgroup sum in a fixed order (wavefront shuffle tree, then the four
 // wavefront sums in index order); the result is valid in thread 0.  Every
 // thread of the workgroup must call it.

@@ -1340,21 +1340,40 @@ __global__ __launch_bounds__(kBlockThreads) void fpcg_update_norm(int n, const d
     if (blockIdx.x == 0 && threadIdx.x == 0) *sc_alpha = alpha;
     double s = 0;
     const int stride = gridDim.x * kBlockThreads;
-    for (int i = blockIdx.x * kBlockThreads + threadIdx.x; i < n; i += stride) {
-        const double ri = r[i] - alpha * Ap[i];
-        r[i] = ri;
-        if (r32) {
-            const float rf = (float)ri;
-            r32[i] = rf;
-            if (z32) {
-                const float di = d32[i];
-                const float zero = 0.0f;
-                float v = zero;
-                if (fabs(di) > float(1e-16)) v = ((1.0f - w) * zero) + (w * ((rf - zero) / di));
-                z32[i] = v;
+    // four of the thread's entries per step, their loads issued first (the
+    // operations and the order of the sum are the plain loop's)
+    constexpr int U = 4;
+    for (int i0 = blockIdx.x * kBlockThreads + threadIdx.x; i0 < n; i0 += U * stride) {
+        double rv[U], av[U];
+        float dv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = i0 + u * stride;
+            if (i < n) {
+                rv[u] = r[i];
+                av[u] = Ap[i];
+                if (z32) dv[u] = d32[i];
             }
         }
-        s += ri * ri;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = i0 + u * stride;
+            if (i >= n) break;
+            const double ri = rv[u] - alpha * av[u];
+            r[i] = ri;
+            if (r32) {
+                const float rf = (float)ri;
+                r32[i] = rf;
+                if (z32) {
+                    const float di = dv[u];
+                    const float zero = 0.0f;
+                    float v = zero;
+                    if (fabs(di) > float(1e-16)) v = ((1.0f - w) * zero) + (w * ((rf - zero) / di));
+                    z32[i] = v;
+                }
+            }
+            s += ri * ri;
+        }
     }
     s = block_sum(s);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
@@ -1368,10 +1387,26 @@ __global__ __launch_bounds__(kBlockThreads) void fpcg_dots(int n, const double* 
 {
     double s1 = 0, s2 = 0;
     const int stride = gridDim.x * kBlockThreads;
-    for (int i = blockIdx.x * kBlockThreads + threadIdx.x; i < n; i += stride) {
-        const double zi = (double)z[i];
-        s1 += r[i] * zi;
-        s2 += zi * Ap[i];
+    // four of the thread's entries per step, their loads issued first (the
+    // products and the order of the sums are the plain loop's)
+    constexpr int U = 4;
+    for (int i0 = blockIdx.x * kBlockThreads + threadIdx.x; i0 < n; i0 += U * stride) {
+        double rv[U], zv[U], av[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = i0 + u * stride;
+            if (i < n) {
+                zv[u] = (double)z[i];
+                rv[u] = r[i];
+                av[u] = Ap[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (i0 + u * stride >= n) break;
+            s1 += rv[u] * zv[u];
+            s2 += zv[u] * av[u];
+        }
     }
     s1 = block_sum(s1);
     s2 = block_sum(s2);

@@ -1612,8 +1612,7 @@ __global__ __launch_bounds__(kBlockThreads) void reduce_partial(const double* __
 __global__ __launch_bounds__(kBlockThreads) void reduce_final(const double* __restrict__ partials, int np,
                                                               double* __restrict__ out)
 {
-    double s = 0;
-    for (int i = threadIdx.x; i < np; i += kBlockThreads) s += partials[i];
+    double s = strided_sum(partials, 0, np);
     s = block_sum(s);
     if (threadIdx.x == 0) *out = s;
 }
@@ -1625,10 +1624,9 @@ __global__ __launch_bounds__(kBlockThreads) void reduce_final_two(const double* 
                                                                   const double* __restrict__ p2, int n2,
                                                                   double* __restrict__ o2)
 {
-    double s1 = 0, s2 = 0;
-    for (int i = threadIdx.x; i < n1; i += kBlockThreads) s1 += p1[i];
+    double s1 = strided_sum(p1, 0, n1);
     s1 = block_sum(s1);
-    for (int i = threadIdx.x; i < n2; i += kBlockThreads) s2 += p2[i];
+    double s2 = strided_sum(p2, 0, n2);
     s2 = block_sum(s2);
     if (threadIdx.x == 0) {
         *o1 = s1;
@@ -1653,8 +1651,7 @@ __global__ __launch_bounds__(kBlockThreads) void reduce_final_pub(const double* 
                                                                   double* __restrict__ out,
                                                                   const double* __restrict__ also, double* host)
 {
-    double s = 0;
-    for (int i = threadIdx.x; i < np; i += kBlockThreads) s += partials[i];
+    double s = strided_sum(partials, 0, np);
     s = block_sum(s);
     if (threadIdx.x == 0) {
         *out = s;
@@ -1702,8 +1699,7 @@ __global__ __launch_bounds__(kBlockThreads) void reduce_chunks(const double* __r
                                                                double* __restrict__ out)
 {
     const int b0 = blockIdx.x * chunk, b1 = min(np, b0 + chunk);
-    double s = 0;
-    for (int i = b0 + threadIdx.x; i < b1; i += kBlockThreads) s += partials[i];
+    double s = strided_sum(partials, b0, b1);
     s = block_sum(s);
     if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
