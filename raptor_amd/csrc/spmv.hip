@@ -1597,12 +1597,28 @@ __global__ __launch_bounds__(kBlockThreads) void reduce_partial(const double* __
 {
     double s = 0;
     const int64_t stride = (int64_t)gridDim.x * kBlockThreads;
-    for (int64_t i = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x; i < n; i += stride) {
-        if (MODE == 0) {
-            s += x[i] * y[i];
-        } else {
-            const double v = x[i];
-            if (fabs(v) > 1e-16) s += v * v;
+    // four of the thread's entries per step, their loads issued first (the
+    // order of the sum is the plain loop's)
+    constexpr int U = 4;
+    for (int64_t i0 = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x; i0 < n; i0 += U * stride) {
+        double xv[U], yv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t i = i0 + u * stride;
+            if (i < n) {
+                xv[u] = x[i];
+                if (MODE == 0) yv[u] = y[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (i0 + u * stride >= n) break;
+            if (MODE == 0) {
+                s += xv[u] * yv[u];
+            } else {
+                const double v = xv[u];
+                if (fabs(v) > 1e-16) s += v * v;
+            }
         }
     }
     s = block_sum(s);
