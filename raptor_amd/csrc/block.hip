@@ -671,10 +671,11 @@ __global__ __launch_bounds__(256) void bcol_dots(int n, int k, int kp, const dou
     if (MODE == 1) col_partial(s2, k, kp, partials + (size_t)gridDim.x * k);
 }
 
-// x += alpha p; r -= alpha Ap; r32 = (float) r; per-column sums of r^2
+// r -= alpha Ap; r32 = (float) r; per-column sums of r^2.  The step
+// x += alpha p of the iteration is taken by the direction update (bdir),
+// which reads p anyway, or by bstep_x when the iteration ends there.
 __global__ __launch_bounds__(256) void bupdate(int n, int k, int kp, const double* __restrict__ alpha,
-                                               const double* __restrict__ p, const double* __restrict__ Ap,
-                                               double* __restrict__ x, double* __restrict__ r,
+                                               const double* __restrict__ Ap, double* __restrict__ r,
                                                float* __restrict__ r32, double* __restrict__ partials)
 {
     const int c = threadIdx.x & (kp - 1);
@@ -684,13 +685,11 @@ __global__ __launch_bounds__(256) void bupdate(int n, int k, int kp, const doubl
     if (c < k) {
         const double al = alpha[c];
         for (int64_t i0 = (int64_t)blockIdx.x * rpb + threadIdx.x / kp; i0 < n; i0 += kColUnroll * stride) {
-            double xv[kColUnroll], pv[kColUnroll], rv[kColUnroll], av[kColUnroll];
+            double rv[kColUnroll], av[kColUnroll];
 #pragma unroll
             for (int u = 0; u < kColUnroll; u++) {
                 const int64_t i = i0 + u * stride;
                 const size_t e = (size_t)(i < n ? i : 0) * k + c;
-                xv[u] = i < n ? x[e] : 0.0;
-                pv[u] = i < n ? p[e] : 0.0;
                 rv[u] = i < n ? r[e] : 0.0;
                 av[u] = i < n ? Ap[e] : 0.0;
             }
@@ -699,7 +698,6 @@ __global__ __launch_bounds__(256) void bupdate(int n, int k, int kp, const doubl
                 const int64_t i = i0 + u * stride;
                 if (i >= n) break;
                 const size_t e = (size_t)i * k + c;
-                x[e] = xv[u] + al * pv[u];
                 const double ri = rv[u] - al * av[u];
                 r[e] = ri;
                 if (r32) r32[e] = (float)ri;
@@ -713,28 +711,48 @@ __global__ __launch_bounds__(256) void bupdate(int n, int k, int kp, const doubl
 // p = z + beta p
 template <typename Z>
 __global__ __launch_bounds__(256) void bdir(int n, int k, int kp, const double* __restrict__ beta,
-                                            const Z* __restrict__ z, double* __restrict__ p)
+                                            const Z* __restrict__ z, double* __restrict__ p,
+                                            const double* __restrict__ alpha, double* __restrict__ x)
 {
     const int c = threadIdx.x & (kp - 1);
     const int rpb = 256 / kp;
     if (c >= k) return;
     const double be = beta[c];
+    const double al = x ? alpha[c] : 0.0;
     const int64_t stride = (int64_t)gridDim.x * rpb;
     for (int64_t i0 = (int64_t)blockIdx.x * rpb + threadIdx.x / kp; i0 < n; i0 += kColUnroll * stride) {
-        double zv[kColUnroll], pv[kColUnroll];
+        double zv[kColUnroll], pv[kColUnroll], xv[kColUnroll];
 #pragma unroll
         for (int u = 0; u < kColUnroll; u++) {
             const int64_t i = i0 + u * stride;
             const size_t e = (size_t)(i < n ? i : 0) * k + c;
             zv[u] = i < n ? (double)z[e] : 0.0;
             pv[u] = i < n ? p[e] : 0.0;
+            if (x) xv[u] = i < n ? x[e] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < kColUnroll; u++) {
             const int64_t i = i0 + u * stride;
             if (i >= n) break;
-            p[(size_t)i * k + c] = zv[u] + be * pv[u];
+            const size_t e = (size_t)i * k + c;
+            if (x) x[e] = xv[u] + al * pv[u];
+            p[e] = zv[u] + be * pv[u];
         }
+    }
+}
+
+// x += alpha p (the iteration's step, when no direction update follows)
+__global__ __launch_bounds__(256) void bstep_x(int n, int k, int kp, const double* __restrict__ alpha,
+                                               const double* __restrict__ p, double* __restrict__ x)
+{
+    const int c = threadIdx.x & (kp - 1);
+    const int rpb = 256 / kp;
+    if (c >= k) return;
+    const double al = alpha[c];
+    const int64_t stride = (int64_t)gridDim.x * rpb;
+    for (int64_t i = (int64_t)blockIdx.x * rpb + threadIdx.x / kp; i < n; i += stride) {
+        const size_t e = (size_t)i * k + c;
+        x[e] = x[e] + al * p[e];
     }
 }
 
@@ -1194,19 +1212,30 @@ static int fpcg_block_run(ramd_hier_s* h, ramd_csr_s* A, int k, double* X_d, con
             bouter(h, bs, A, OP_SPMV, bs->P, nullptr, bs->AP);
             col_dots(ctx, bs, n, 0, bs->P, bs->AP, false, nullptr, S.pap, nullptr);
             hipLaunchKernelGGL(bscal_alpha, dim3(1), dim3(64), 0, s, k, S);
-            hipLaunchKernelGGL(bupdate, dim3(gc), dim3(256), 0, s, n, k, kp, S.alpha, bs->P, bs->AP, bs->X, bs->R,
+            hipLaunchKernelGGL(bupdate, dim3(gc), dim3(256), 0, s, n, k, kp, S.alpha, bs->AP, bs->R,
                                f32 ? bs->R32 : nullptr, bs->partials);
             hipLaunchKernelGGL(bcol_final, dim3(k), dim3(256), 0, s, gc, k, bs->partials, S.rr);
             hipLaunchKernelGGL(bscal_relres, dim3(1), dim3(64), 0, s, k, rtol, S);
             RAMD_HIP(hipGetLastError());
         };
-        auto stepB = [&] {
+        // preconditioner step; with step_x the direction update also takes
+        // the iteration's step x += alpha p (one read of p)
+        auto stepB = [&](bool step_x) {
             bprecondition(h, bs, f32);
             col_dots(ctx, bs, n, 1, bs->R, f32 ? (const void*)bs->Z32 : (const void*)bs->Z, f32, bs->AP, S.rzn,
                      S.zap);
             hipLaunchKernelGGL(bscal_beta, dim3(1), dim3(64), 0, s, k, S);
-            if (f32) hipLaunchKernelGGL((bdir<float>), dim3(gc), dim3(256), 0, s, n, k, kp, S.beta, bs->Z32, bs->P);
-            else hipLaunchKernelGGL((bdir<double>), dim3(gc), dim3(256), 0, s, n, k, kp, S.beta, bs->Z, bs->P);
+            double* xs = step_x ? bs->X : nullptr;
+            if (f32)
+                hipLaunchKernelGGL((bdir<float>), dim3(gc), dim3(256), 0, s, n, k, kp, S.beta, bs->Z32, bs->P, S.alpha,
+                                   xs);
+            else
+                hipLaunchKernelGGL((bdir<double>), dim3(gc), dim3(256), 0, s, n, k, kp, S.beta, bs->Z, bs->P, S.alpha,
+                                   xs);
+            RAMD_HIP(hipGetLastError());
+        };
+        auto step_x = [&] {
+            hipLaunchKernelGGL(bstep_x, dim3(gc), dim3(256), 0, s, n, k, kp, S.alpha, bs->P, bs->X);
             RAMD_HIP(hipGetLastError());
         };
         const uint64_t fk = forms_key(h, A);
@@ -1214,7 +1243,7 @@ static int fpcg_block_run(ramd_hier_s* h, ramd_csr_s* A, int k, double* X_d, con
                           !bs->gA)) {
             free_graphs(bs);
             bs->gA = capture_graph(s, stepA);
-            bs->gB = capture_graph(s, stepB);
+            bs->gB = capture_graph(s, [&] { stepB(true); });
             bs->g_uid = A->uid;
             bs->g_forms = fk;
             bs->g_prec = prec;
@@ -1226,11 +1255,15 @@ static int fpcg_block_run(ramd_hier_s* h, ramd_csr_s* A, int k, double* X_d, con
             it++;
             read_flags(fl);
             if (fl[1] != 0) {
+                step_x();
                 rc = fl[1] == 1 ? RAMD_E_INDEFINITE : RAMD_E_NUMERIC;
                 break;
             }
+            bool x_done = false;
             if (fl[0] == 0) {
                 // confirm every column with its true residual
+                step_x();
+                x_done = true;
                 bouter(h, bs, A, OP_RESIDUAL, bs->X, bs->B, bs->R);
                 col_dots(ctx, bs, n, 0, bs->R, bs->R, false, nullptr, S.rr, nullptr);
                 hipLaunchKernelGGL(bscal_recheck, dim3(1), dim3(64), 0, s, k, rtol, S);
@@ -1239,8 +1272,8 @@ static int fpcg_block_run(ramd_hier_s* h, ramd_csr_s* A, int k, double* X_d, con
                 if (fl[0] == 0) break;
                 if (f32) hipLaunchKernelGGL((bconvert<double, float>), ge, dim3(256), 0, s, m, bs->R, bs->R32);
             }
-            if (use_graph) RAMD_HIP(hipGraphLaunch(bs->gB, s));
-            else stepB();
+            if (use_graph && !x_done) RAMD_HIP(hipGraphLaunch(bs->gB, s));
+            else stepB(!x_done);
         }
     }
     RAMD_HIP(hipMemcpyAsync(X_d, bs->X, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
