@@ -135,6 +135,13 @@ int ramd_csr_set_wave_rows(ramd_csr A, int prec, int on);
  * (restrictions R = P^T, CSR_spmv over P^T: util/linalg/spmv.cpp:52-67).
  * No reference counterpart. */
 int ramd_csr_set_sorted_gather(ramd_csr A, int prec, int on);
+/* Whether the flexible PCG's FP64 products with A (its outer operator and
+ * true residuals, ramd_fpcg and ramd_adjoint) may read A's FP32 copy of the
+ * values when every value is exactly representable in FP32: the same FP64
+ * products from half the value bytes.  on = 1 (default) allows it, 0 reads the FP64 values as the
+ * reference stores them (ParCSRMatrix::mult, util/linalg/par_spmv.cpp:25-59).
+ * Identical results either way.  No reference counterpart. */
+int ramd_csr_set_fp32_outer(ramd_csr A, int on);
 
 /* ------------------------------------------------------------------ */
 /* SpMV family (FP64; each row summed left to right in stored order, as  */

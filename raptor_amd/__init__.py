@@ -252,6 +252,11 @@ class ParCSRMatrix:
         one wavefront per row, False the row-block kernel (identical bits)."""
         check(lib().ramd_csr_set_wave_rows(self.h, int(prec), int(bool(on))))
 
+    def set_fp32_outer(self, on):
+        """FP64 products with this matrix read its exact FP32 value copy (True,
+        default) or the FP64 values (False); identical results."""
+        check(lib().ramd_csr_set_fp32_outer(self.h, int(bool(on))))
+
     def set_sorted_gather(self, prec, on):
         """Row-block kernel's SpMV for FP64 (prec 0) or FP32 (prec 1) vectors
         with the entries read in column order within each block (True) or in

@@ -91,6 +91,7 @@ _SIGS = {
     "ramd_csr_set_lane_rows": (P, C.c_int, C.c_int),
     "ramd_csr_set_wave_rows": (P, C.c_int, C.c_int),
     "ramd_csr_set_sorted_gather": (P, C.c_int, C.c_int),
+    "ramd_csr_set_fp32_outer": (P, C.c_int),
     "ramd_spmv": (P, P, P),
     "ramd_spmv_append": (P, P, P),
     "ramd_residual": (P, P, P, P),

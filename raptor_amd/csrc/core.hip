@@ -1243,6 +1243,17 @@ int ramd_csr_set_lane_rows(ramd_csr A, int prec, int on)
     });
 }
 
+int ramd_csr_set_fp32_outer(ramd_csr A, int on)
+{
+    return guarded([&] {
+        RAMD_CHECK(A && (on == 0 || on == 1), RAMD_E_ARG, "ramd_csr_set_fp32_outer: bad argument");
+        A->ctx->set_current();
+        RAMD_HIP(hipStreamSynchronize(A->ctx->stream));
+        A->v32_outer = on != 0;
+        A->forms++;
+    });
+}
+
 int ramd_csr_set_sorted_gather(ramd_csr A, int prec, int on)
 {
     return guarded([&] {

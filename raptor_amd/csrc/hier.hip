@@ -1572,7 +1572,7 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
     if (noalloc) {
         RAMD_CHECK(!f32 || h->fp32, RAMD_E_STATE,
                    "ramd_fpcg_noalloc: the hierarchy's FP32 copies are not built (ramd_hier_enable_fp32)");
-        RAMD_CHECK(!A->dia || (A->val32 && A->dia->v32 && A->dia->lds_x[2] >= 0), RAMD_E_STATE,
+        RAMD_CHECK(!A->dia || (A->val32 && A->dia->v32 && A->dia->lds_x[A->v32_outer ? 2 : 0] >= 0), RAMD_E_STATE,
                    "ramd_fpcg_noalloc: the operator's FP32 values and kernel choice are made by a first "
                    "ramd_fpcg call with it");
         RAMD_CHECK(!h->track && ctx->prof_min_nnz < 0, RAMD_E_STATE,
@@ -1581,7 +1581,9 @@ static int fpcg_run(ramd_hier_s* h, ramd_csr_s* A, double* x_d, const double* b_
         cycle_prepare(h, f32 ? 1 : 0);
         if (A->dia) {
             csr_make_fp32(A);
-            dia_autotune(A, 2);  // the outer operator reads FP32 values into FP64 vectors
+            // the outer operator reads FP32 values into FP64 vectors (or,
+            // with that turned off, FP64 values)
+            dia_autotune(A, A->v32_outer ? 2 : 0);
         }
     }
     double *r = h->pr, *z = h->pz, *p = h->pp, *Ap = h->pAp;

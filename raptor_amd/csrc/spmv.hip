@@ -1260,7 +1260,7 @@ void launch_stream_op(const ramd_csr_s* A, int op, const double* x, const double
 void launch_stream_op_c(const ramd_csr_s* A, int op, const double* x, const double* b, double* y, double omega,
                         hipStream_t s)
 {
-    if (A->dia && A->dia->v32 && A->dia->v32_exact)
+    if (A->dia && A->dia->v32 && A->dia->v32_exact && A->v32_outer)
         launch_stream<float, double>(A, nullptr, A->dia->v32, op, x, b, y, omega, s);
     else
         launch_stream_op(A, op, x, b, y, omega, s);
@@ -1574,7 +1574,7 @@ void launch_spmv_dot(const ramd_csr_s* A, bool values_fp32, const double* x, dou
                      hipStream_t s, double* partials)
 {
     ramd_ctx ctx = A->ctx;
-    const bool dia32 = values_fp32 && A->dia && A->dia->v32 && A->dia->v32_exact;
+    const bool dia32 = values_fp32 && A->dia && A->dia->v32 && A->dia->v32_exact && A->v32_outer;
     const int np = A->dia ? dia_grid(A, dia32 ? 2 : 0) : spmv_dot_partials(A);
     double* part = partials ? partials : ctx_partials(ctx, (size_t)np);
     if (dia32)

@@ -477,6 +477,32 @@ def test_fpcg_sliced_outer_operator(ctx):
     assert np.array_equal(out[0][0], out[1][0])
 
 
+@pytest.mark.parametrize("prec", [ra.PREC_FP64, ra.PREC_FP32])
+def test_fpcg_fp32_outer_switch(ctx, prec):
+    """FPCG on a stencil operator whose values are exact in FP32 gives the
+    same iterates with its FP64 products reading the FP32 value copy (the
+    default) as reading the FP64 values."""
+    rp, col, val = ra.gallery_laplace27(16)
+    ml = build(ctx, rp, col, val, rs(ra.PMIS, ra.Extended))
+    A = ra.ParCSRMatrix.from_csr(ctx, rp, col, val)
+    assert A.diagonals() > 0
+    n = rp.size - 1
+    b = np.random.default_rng(27).standard_normal(n)
+    out = []
+    try:
+        for on in (True, False, True):
+            A.set_fp32_outer(on)
+            x = ra.ParVector.from_numpy(ctx, np.zeros(n))
+            hist, it, rel = ra.FPCG(A, ml, x, ra.ParVector.from_numpy(ctx, b), 1e-8, 200, prec)
+            out.append((x.numpy(), it, rel))
+    finally:
+        A.set_fp32_outer(True)
+    for o in out[1:]:
+        assert o[1] == out[0][1] and o[2] == out[0][2]
+        assert np.array_equal(o[0], out[0][0])
+    assert ra._lib.lib().ramd_csr_set_fp32_outer(A.h, 2) != 0  # only 0 or 1
+
+
 def test_amg_solve_matches_oracle(problem, ctx):
     ml, As, Ps = problem["ml"], problem["As"], problem["Ps"]
     H = K.Hier(As, Ps, relax_type=0, sweeps=1, omega=OMEGA)
