@@ -251,9 +251,11 @@ typedef struct {
                                (multilevel/par_multilevel.hpp:215-225)          */
     int prolong_smooth_steps;/* SA Jacobi prolongation steps (default 1)       */
     int relax_type;         /* RAMD_JACOBI (default), RAMD_MC_SOR, RAMD_MC_SSOR */
-    int device_setup;       /* 1 (default): classical strength, PMIS and
-                               extended interpolation run on the device
-                               (same results); 0: on the host             */
+    int device_setup;       /* 1 (default): the setup steps run on the
+                               device (strength, splitting, interpolation;
+                               SA: strength, MIS-2, aggregation and the
+                               prolongation smoothing; same results);
+                               0: on the host                             */
 } ramd_amg_opts;
 
 /* Fills opts with the reference's defaults for `solver`. */
@@ -510,9 +512,6 @@ int ramd_host_read_petsc(const char* path, int32_t* n_rows, int32_t* n_cols, int
 /* The matching writer (the reference has none): big-endian PETSc binary. */
 int ramd_host_write_petsc(const char* path, int32_t n_rows, int32_t n_cols, const int32_t* rowptr,
                           const int32_t* col, const double* val);
-/* mis2(S, states, off_proc_states, false, weights), aggregation/par_mis.cpp:
- * 204-645.  weights NULL = the srand(2448422) stream.  states: 1 selected,
- * 0 not selected. */
 /* Device forms of the setup steps of the 27-point benchmarks' RS path,
  * bit-identical to the host ones (ramd_host_strength / ramd_host_split /
  * ramd_host_interp) on a matrix stored diagonal first:
@@ -542,12 +541,25 @@ int ramd_interp_mod_classical_gpu(ramd_csr A, ramd_csr S, const int32_t* states,
 /* Direct interpolation (ruge_stuben/par_interpolation.cpp:1453-1754) on the
  * device, bit-identical to ramd_host_interp(..., RAMD_DIRECT, ...). */
 int ramd_interp_direct_gpu(ramd_csr A, ramd_csr S, const int32_t* states, ramd_csr* P);
+/* MIS-2 (aggregation/par_mis.cpp:204-645) and aggregation
+ * (aggregation/par_aggregate.cpp:5-185) on the device, bit-identical to
+ * ramd_host_mis2 and ramd_host_aggregate; A and S stored sorted with the
+ * diagonal first (the hierarchy's levels and the device strength give them
+ * so).  weights NULL = the srand(2448422) stream; rand_vals NULL = zeros,
+ * as the solver calls it.  states, rand_vals and aggregates are host
+ * arrays of n. */
+int ramd_mis2_gpu(ramd_csr S, const double* weights, int32_t* states);
+int ramd_aggregate_gpu(ramd_csr A, ramd_csr S, const int32_t* states, const double* rand_vals, int32_t* aggregates,
+                       int32_t* n_aggs);
 
 /* Greedy colouring of the graph of A + A^T without the diagonal (rows in
  * ascending order take the smallest colour no coloured neighbour holds),
  * the colouring the multicolour smoothers use.  No reference counterpart. */
 int ramd_host_greedy_colors(int32_t n, const int32_t* rowptr, const int32_t* col, int32_t* colors,
                             int32_t* n_colors);
+/* mis2(S, states, off_proc_states, false, weights), aggregation/par_mis.cpp:
+ * 204-645.  weights NULL = the srand(2448422) stream.  states: 1 selected,
+ * 0 not selected. */
 int ramd_host_mis2(int32_t n, const int32_t* s_rowptr, const int32_t* s_col, const double* weights,
                    int32_t* states);
 /* aggregate(A, S, states, off, aggregates, false, rand_vals),

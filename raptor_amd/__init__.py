@@ -38,7 +38,7 @@ from ._lib import (CLJP, HMIS, MC_SOR, MC_SSOR, PMIS, RS, Classical, Direct, Ext
 __all__ = [
     "Context", "ParVector", "ParCSRMatrix", "ParMultilevel", "ParRugeStubenSolver",
     "ParSmoothedAggregationSolver", "PCG", "CG", "BiCGStab", "Pre_BiCGStab", "FPCG", "FPCG_block", "vcycle_block", "block_tuning", "block_set_tuning", "adjoint", "strength", "split", "interpolation", "mis2",
-    "aggregate", "fit_candidates", "strength_gpu", "split_pmis_gpu", "interp_extended_gpu", "jacobi_prolongation", "read_petsc", "write_petsc", "gallery_laplace27", "gallery_aniso2d", "gallery_fv7", "RamdError",
+    "aggregate", "fit_candidates", "strength_gpu", "split_pmis_gpu", "interp_extended_gpu", "mis2_gpu", "aggregate_gpu", "jacobi_prolongation", "read_petsc", "write_petsc", "gallery_laplace27", "gallery_aniso2d", "gallery_fv7", "RamdError",
     "RS", "CLJP", "Falgout", "PMIS", "HMIS", "Direct", "ModClassical", "Extended", "Classical",
     "Symmetric", "Jacobi", "MC_SOR", "MC_SSOR", "greedy_colors", "PREC_FP64", "PREC_FP32",
 ]
@@ -727,6 +727,27 @@ def interp_direct_gpu(A, S, states):
     h = C.c_void_p()
     check(lib().ramd_interp_direct_gpu(A.h, S.h, _ptr(states, C.c_int32), C.byref(h)))
     return ParCSRMatrix(A.ctx, h)
+
+
+def mis2_gpu(S, weights=None):
+    """MIS-2 states (1 selected, 0 not) of a device strength matrix on the
+    device (ramd_mis2_gpu)."""
+    states = np.empty(S.n_rows, np.int32)
+    w = None if weights is None else _f64(weights)
+    check(lib().ramd_mis2_gpu(S.h, None if w is None else _ptr(w, C.c_double), _ptr(states, C.c_int32)))
+    return states
+
+
+def aggregate_gpu(A, S, states, rand_vals=None):
+    """Aggregation of device A and S on the device (ramd_aggregate_gpu).
+    Returns (aggregates: root row per row or -1, n_aggs)."""
+    states = _i32(states)
+    agg = np.empty(A.n_rows, np.int32)
+    na = C.c_int32()
+    r = None if rand_vals is None else _f64(rand_vals)
+    check(lib().ramd_aggregate_gpu(A.h, S.h, _ptr(states, C.c_int32), None if r is None else _ptr(r, C.c_double),
+                                   _ptr(agg, C.c_int32), C.byref(na)))
+    return agg, na.value
 
 
 def split_rs_serial(s_rowptr, s_col, second_pass=True):

@@ -1143,6 +1143,70 @@ static Csr sa_level(ramd_hier_s* h, Csr& A, const double* weights, std::vector<i
     return P;
 }
 
+// sa_level with every step but fit_candidates on the device: strength,
+// MIS-2, aggregation, the scaled operator, its products with the
+// prolongator and the subtractions, with sa_level's results.  fit_candidates
+// (one pass over the aggregates, with the candidate vector B kept on the
+// host between levels) runs on the host.
+static bool device_sa(const ramd_amg_opts& o) { return o.device_setup && o.solver == RAMD_SOLVER_SA; }
+
+static ramd_csr_s* sa_level_device(ramd_hier_s* h, int level, const double* weights, std::vector<int>& aggregates,
+                                   std::vector<double>& B)
+{
+    const ramd_amg_opts& o = h->opts;
+    ramd_ctx ctx = h->ctx;
+    Level& L = h->levels[level];
+    const int n = L.n;
+    double t0 = now_s();
+    ramd_csr_s* S = o.strength_type == RAMD_SYMMETRIC ? gpu_symmetric_strength(L.A, o.strong_threshold)
+                                                      : gpu_classical_strength(L.A, o.strong_threshold);
+    int *st = nullptr, *agg = nullptr;
+    ramd_csr_s *P = nullptr, *M = nullptr, *Q = nullptr;
+    try {
+        double t1 = now_s();
+        st = dalloc<int>((size_t)std::max(n, 1));
+        agg = dalloc<int>((size_t)std::max(n, 1));
+        gpu_mis2(S, weights, st);
+        const int n_aggs = gpu_aggregate(L.A, S, st, agg);
+        aggregates.resize((size_t)n);
+        if (n) RAMD_HIP(hipMemcpyAsync(aggregates.data(), agg, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+        RAMD_HIP(hipStreamSynchronize(ctx->stream));
+        double t2 = now_s();
+        std::vector<double> R;
+        P = upload(ctx, host::fit_candidates(n_aggs, aggregates, B, R), false);
+        if (o.prolong_smooth_steps > 0) {
+            M = gpu_sa_scaled_operator(L.A, o.prolong_weight);
+            for (int step = 0; step < o.prolong_smooth_steps; step++) {
+                Q = spgemm(M, P);
+                ramd_csr_s* next = gpu_sa_subtract(P, Q);
+                csr_free(Q);
+                Q = nullptr;
+                csr_free(P);
+                P = next;
+            }
+            csr_free(M);
+            M = nullptr;
+        }
+        B = std::move(R);
+        double t3 = now_s();
+        h->times[0] += t1 - t0;
+        h->times[1] += t2 - t1;
+        h->times[2] += t3 - t2;
+    } catch (...) {
+        csr_free(S);
+        dfree(st);
+        dfree(agg);
+        csr_free(P);
+        csr_free(M);
+        csr_free(Q);
+        throw;
+    }
+    csr_free(S);
+    dfree(st);
+    dfree(agg);
+    return P;
+}
+
 // Device state every hierarchy needs once its levels exist: the coarse LU
 // (column-major factors of A_c^T and pivots), the PCG work vectors, the
 // kernel tuning of every cycle operator and the ownership marks.
@@ -1210,6 +1274,8 @@ static void hier_setup(ramd_hier_s* h, Csr Af)
         ramd_csr_s* Pd = nullptr;
         if (device_rs(o)) {
             Pd = rs_level_device(h, l, weights.data(), L.states);
+        } else if (device_sa(o)) {
+            Pd = sa_level_device(h, l, weights.data(), L.states, B);
         } else {
             if (!host_A) A = download(L.A);
             Csr P = o.solver == RAMD_SOLVER_SA ? sa_level(h, A, weights.data(), L.states, B)

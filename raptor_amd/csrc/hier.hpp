@@ -92,6 +92,17 @@ ramd_csr_s* gpu_direct_interpolation(const ramd_csr_s* A, const ramd_csr_s* S, c
                                      const std::vector<int>& states_h);
 ramd_csr_s* gpu_extended_interpolation(const ramd_csr_s* A, const ramd_csr_s* S, const int* states_d,
                                        const std::vector<int>& states_h, double filter);
+// Smoothed aggregation's setup steps on the device, with the host
+// restatement's results: MIS-2 of S (weights: host, n) into states (device);
+// aggregation of A and S (both stored sorted, diagonal first) into agg
+// (device; the root row or -1; rand_vals: host, n, or nullptr for zeros),
+// returning the number of aggregates; the Jacobi prolongation's scaled
+// operator and its subtraction P - Q.
+void gpu_mis2(const ramd_csr_s* S, const double* weights, int* states);
+int gpu_aggregate(const ramd_csr_s* A, const ramd_csr_s* S, const int* states, int* agg,
+                  const double* rand_vals = nullptr);
+ramd_csr_s* gpu_sa_scaled_operator(const ramd_csr_s* A, double omega);
+ramd_csr_s* gpu_sa_subtract(const ramd_csr_s* P, const ramd_csr_s* Q);
 
 // One FP64 V-cycle from x = 0 (ParMultilevel::cycle on the fine level).
 void vcycle_from_zero(ramd_hier_s* h, double* x, const double* b);

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <climits>
 #include <cstring>
+#include <stdexcept>
 #include <vector>
 
 #include "hier.hpp"
@@ -1087,6 +1088,478 @@ ramd_csr_s* gpu_mod_classical_interpolation(const ramd_csr_s* A, const ramd_csr_
     return gpu_interpolation(A, S, states_d, states_h, 0.0, true);
 }
 
+
+// ---------------------------------------------------------------------------
+// Smoothed aggregation on the device: MIS-2 (aggregation/par_mis.cpp:
+// 204-645), aggregation (aggregation/par_aggregate.cpp:5-185) and the two
+// row passes of the Jacobi prolongation (the scaled operator and the
+// subtraction P - Q, aggregation/par_prolongation.cpp:7-103 with
+// util/linalg/par_add.cpp:184-310), one thread per row, with the host
+// restatement's results.
+//
+// MIS-2 states: -1 unassigned and 4 tentatively selected (both still in the
+// loop), 3 / 2 this iteration's selection / unselection, 1 / 0 settled.  The
+// reference's loops over the remaining vertices only ask whether a vertex is
+// still in the loop, whether it was just selected, and for maxima over sets,
+// so the order in which they visit vertices does not matter.  Its marking of
+// the neighbours of new selections keeps two quirks that make the marks
+// persistent (:505-593): a vertex marked in an iteration is unmarked at the
+// end of it the first time only (the list reset writes next[w] = 0, which
+// is not "unassigned"), so from its second marking on it stays marked.
+// mark[] holds those bits: 1 marked (C[w] == Selected), 2 marked before,
+// 4 marked in this iteration.
+// ---------------------------------------------------------------------------
+constexpr int kTmpSelectionState = 4;
+
+__global__ void set_int_kernel(int* __restrict__ p, int n, int v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ bool mis2_in_loop(int s) { return s == kUnassignedState || s == kTmpSelectionState; }
+
+// tentative selection (:378-415): a vertex in the loop whose neighbours in S
+// with smaller random values have all left the loop.  A vertex in the loop
+// reads as in the loop whether or not this launch has marked it yet (-1 and
+// 4 both count), so the concurrent marks change no decision.
+__global__ void mis2_tentative_kernel(int n, const int* __restrict__ srp, const int* __restrict__ scol,
+                                      const double* __restrict__ r, int* states, int* remaining)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *remaining = 0;
+    if (i >= n || !mis2_in_loop(states[i])) return;
+    const double ri = r[i];
+    for (int j = srp[i]; j < srp[i + 1]; j++) {
+        const int c = scol[j];
+        if (ri > r[c]) {
+            const int t = states[c];
+            if (t == kUnassignedState || t > kSelectedState) return;
+        }
+    }
+    states[i] = kTmpSelectionState;
+}
+
+// largest random value of a tentative vertex in each row of S (:420-446)
+__global__ void mis2_row_max_kernel(int n, const int* __restrict__ srp, const int* __restrict__ scol,
+                                    const double* __restrict__ r, const int* __restrict__ states,
+                                    double* __restrict__ row_max)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double m = -1.0;
+    for (int j = srp[i]; j < srp[i + 1]; j++) {
+        const int u = scol[j];
+        if (states[u] > kSelectedState && r[u] > m) m = r[u];
+    }
+    row_max[i] = m;
+}
+
+// a tentative vertex with no larger tentative value within distance two is
+// selected (:449-473)
+__global__ void mis2_select_kernel(int n, const int* __restrict__ srp, const int* __restrict__ scol,
+                                   const double* __restrict__ r, const double* __restrict__ row_max,
+                                   int* __restrict__ states)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || states[i] != kTmpSelectionState) return;
+    const double ri = r[i];
+    for (int j = srp[i]; j < srp[i + 1]; j++)
+        if (row_max[scol[j]] > ri) return;
+    states[i] = kNewSelectionState;
+}
+
+// rows of S holding a new selection are marked (:505-526; the reference
+// walks the columns of the new selections)
+__global__ void mis2_mark_kernel(int n, const int* __restrict__ srp, const int* __restrict__ scol,
+                                 const int* __restrict__ states, int* __restrict__ mark)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int j = srp[i]; j < srp[i + 1]; j++)
+        if (states[scol[j]] == kNewSelectionState) {
+            mark[i] |= 5;
+            return;
+        }
+}
+
+// a vertex in the loop next to a new selection or to a marked vertex is
+// unselected (:555-586).  No vertex becomes a new selection here, so the
+// concurrent writes change no decision.
+__global__ void mis2_unselect_kernel(int n, const int* __restrict__ srp, const int* __restrict__ scol,
+                                     const int* __restrict__ mark, int* states)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !mis2_in_loop(states[i])) return;
+    for (int j = srp[i]; j < srp[i + 1]; j++) {
+        const int c = scol[j];
+        if (states[c] == kNewSelectionState || (mark[c] & 1)) {
+            states[i] = kNewUnselectionState;
+            return;
+        }
+    }
+}
+
+// the list reset (:587-593) and the settling of this iteration's decisions
+// (:595-612); counts the vertices left in the loop
+__global__ void mis2_settle_kernel(int n, int* __restrict__ states, int* __restrict__ mark,
+                                   int* __restrict__ remaining)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool left = false;
+    if (i < n) {
+        int m = mark[i];
+        if (m & 4) {
+            m &= ~4;
+            if (!(m & 2)) m = 2;  // first marking: unmarked, remembered
+            mark[i] = m;
+        }
+        const int s = states[i];
+        if (s == kNewSelectionState) states[i] = kSelectedState;
+        else if (s == kNewUnselectionState) states[i] = kUnselectedState;
+        else left = mis2_in_loop(s);
+    }
+    const unsigned long long b = __ballot(left);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(remaining, (int)__popcll(b));
+}
+
+void gpu_mis2(const ramd_csr_s* S, const double* weights, int* states)
+{
+    ramd_ctx ctx = S->ctx;
+    hipStream_t s = ctx->stream;
+    const int n = S->n_rows;
+    if (!n) return;
+    struct Bufs {
+        double *r = nullptr, *row_max = nullptr;
+        int *mark = nullptr, *remaining = nullptr;
+        ~Bufs()
+        {
+            dfree(r);
+            dfree(row_max);
+            dfree(mark);
+            dfree(remaining);
+        }
+    } b;
+    b.r = dalloc<double>((size_t)n);
+    b.row_max = dalloc<double>((size_t)n);
+    b.mark = dalloc<int>((size_t)n);
+    b.remaining = dalloc<int>(1);
+    RAMD_HIP(hipMemcpyAsync(b.r, weights, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    const dim3 g = rows_grid(n), blk(256);
+    hipLaunchKernelGGL(set_int_kernel, g, blk, 0, s, states, n, kUnassignedState);
+    hipLaunchKernelGGL(set_int_kernel, g, blk, 0, s, b.mark, n, 0);
+    int remaining = 0;
+    int64_t iter = 0;
+    do {
+        if (++iter > (int64_t)n + 2) throw std::runtime_error("mis2: independent-set loop does not terminate");
+        hipLaunchKernelGGL(mis2_tentative_kernel, g, blk, 0, s, n, S->rowptr, S->col, b.r, states, b.remaining);
+        hipLaunchKernelGGL(mis2_row_max_kernel, g, blk, 0, s, n, S->rowptr, S->col, b.r, states, b.row_max);
+        hipLaunchKernelGGL(mis2_select_kernel, g, blk, 0, s, n, S->rowptr, S->col, b.r, b.row_max, states);
+        hipLaunchKernelGGL(mis2_mark_kernel, g, blk, 0, s, n, S->rowptr, S->col, states, b.mark);
+        hipLaunchKernelGGL(mis2_unselect_kernel, g, blk, 0, s, n, S->rowptr, S->col, b.mark, states);
+        hipLaunchKernelGGL(mis2_settle_kernel, g, blk, 0, s, n, states, b.mark, b.remaining);
+        RAMD_HIP(hipGetLastError());
+        RAMD_HIP(hipMemcpyAsync(&remaining, b.remaining, sizeof(int), hipMemcpyDeviceToHost, s));
+        RAMD_HIP(hipStreamSynchronize(s));
+    } while (remaining > 0);
+}
+
+// aggregate: every row with more than its diagonal in S and selected roots
+// its own aggregate; a row with at most its diagonal is -n (:45-60)
+__global__ void agg_init_kernel(int n, const int* __restrict__ srp, const int* __restrict__ states,
+                                int* __restrict__ agg, int* __restrict__ n_aggs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool root = false;
+    if (i < n) {
+        if (srp[i + 1] - srp[i] <= 1) {
+            agg[i] = -n;
+        } else if (states[i] == kSelectedState) {
+            agg[i] = i;
+            root = true;
+        } else {
+            agg[i] = -1;
+        }
+    }
+    const unsigned long long b = __ballot(root);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(n_aggs, (int)__popcll(b));
+}
+
+// pass 1 (:96-126): an unselected row joins the aggregate of its first
+// selected neighbour in S that roots one.  Only selected rows are read and
+// only unselected rows written.
+__global__ void agg_pass1_kernel(int n, const int* __restrict__ srp, const int* __restrict__ scol,
+                                 const int* __restrict__ states, int* agg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || states[i] == kSelectedState) return;
+    for (int j = srp[i]; j < srp[i + 1]; j++) {
+        const int c = scol[j];
+        if (states[c] == kSelectedState && agg[c] >= 0) {
+            agg[i] = agg[c];
+            return;
+        }
+    }
+}
+
+// pass 2 for row i (:134-172): the aggregate of its strongest aggregated
+// neighbour, stored as -(aggregate + 1); with none, -(-n + 1) = n - 1.  The
+// neighbour's weight |a_ij| + r_j (r: the random values, zero when absent,
+// as the solver calls it) is found by walking A's row alongside S's (both
+// sorted with the diagonal first).  Returns false when an S entry is
+// missing from A.
+__device__ __forceinline__ bool agg_pass2_row(int i, int n, const int* __restrict__ srp,
+                                              const int* __restrict__ scol, const int* __restrict__ arp,
+                                              const int* __restrict__ acol, const double* __restrict__ aval,
+                                              const double* __restrict__ r, const int* agg, int& out, int& best)
+{
+    int ctr = arp[i];
+    const int a_end = arp[i + 1];
+    double max_val = 0.0;
+    best = -n;
+    for (int j = srp[i]; j < srp[i + 1]; j++) {
+        const int c = scol[j];
+        while (ctr < a_end && acol[ctr] != c) ctr++;
+        if (ctr == a_end) return false;
+        const double val = fabs(aval[ctr]) + (r ? r[c] : 0.0);
+        if (val > max_val && agg[c] >= 0) {
+            max_val = val;
+            best = agg[c];
+        }
+    }
+    out = -(best + 1);
+    return true;
+}
+
+// Pass 2 over every row at once, reading pass 1's values.  The reference
+// runs it in row order, so a row also sees the pass-2 values of the rows
+// before it; those are negative (never "aggregated") unless a row found no
+// aggregated neighbour (n - 1, flag 2), in which case the caller reruns the
+// pass in row order.  Flag 1: an S entry missing from A.
+__global__ void agg_pass2_kernel(int n, const int* __restrict__ srp, const int* __restrict__ scol,
+                                 const int* __restrict__ arp, const int* __restrict__ acol,
+                                 const double* __restrict__ aval, const double* __restrict__ r,
+                                 const int* __restrict__ agg, int* __restrict__ out, int* __restrict__ flags)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (agg[i] != -1) {
+        out[i] = agg[i];
+        return;
+    }
+    int v, best;
+    if (!agg_pass2_row(i, n, srp, scol, arp, acol, aval, r, agg, v, best)) {
+        atomicOr(flags, 1);
+        out[i] = -1;
+        return;
+    }
+    out[i] = v;
+    if (best == -n) atomicOr(flags, 2);
+}
+
+// pass 2 in the reference's row order by one thread (rare: some row found no
+// aggregated neighbour)
+__global__ void agg_pass2_serial_kernel(int n, const int* __restrict__ srp, const int* __restrict__ scol,
+                                        const int* __restrict__ arp, const int* __restrict__ acol,
+                                        const double* __restrict__ aval, const double* __restrict__ r, int* agg,
+                                        int* __restrict__ flags)
+{
+    if (blockIdx.x || threadIdx.x) return;
+    for (int i = 0; i < n; i++) {
+        if (agg[i] != -1) continue;
+        int v, best;
+        if (!agg_pass2_row(i, n, srp, scol, arp, acol, aval, r, agg, v, best)) {
+            atomicOr(flags, 1);
+            return;
+        }
+        agg[i] = v;
+    }
+}
+
+// decoding (:174-183): <= -n none, other negatives -(v + 1)
+__global__ void agg_decode_kernel(int n, const int* src, int* agg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = src[i];
+    agg[i] = v <= -n ? -1 : (v < 0 ? -(v + 1) : v);
+}
+
+int gpu_aggregate(const ramd_csr_s* A, const ramd_csr_s* S, const int* states, int* agg, const double* rand_vals)
+{
+    ramd_ctx ctx = A->ctx;
+    hipStream_t s = ctx->stream;
+    const int n = A->n_rows;
+    if (!n) return 0;
+    struct Bufs {
+        int *out = nullptr, *cnt = nullptr;
+        double* r = nullptr;
+        ~Bufs()
+        {
+            dfree(out);
+            dfree(cnt);
+            dfree(r);
+        }
+    } b;
+    b.out = dalloc<int>((size_t)n);
+    b.cnt = dalloc<int>(2);  // aggregates, flags
+    if (rand_vals) {
+        b.r = dalloc<double>((size_t)n);
+        RAMD_HIP(hipMemcpyAsync(b.r, rand_vals, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    }
+    const dim3 g = rows_grid(n), blk(256);
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(64), 0, s, b.cnt, 2, 0);
+    hipLaunchKernelGGL(agg_init_kernel, g, blk, 0, s, n, S->rowptr, states, agg, b.cnt);
+    hipLaunchKernelGGL(agg_pass1_kernel, g, blk, 0, s, n, S->rowptr, S->col, states, agg);
+    hipLaunchKernelGGL(agg_pass2_kernel, g, blk, 0, s, n, S->rowptr, S->col, A->rowptr, A->col, A->val, b.r, agg,
+                       b.out, b.cnt + 1);
+    RAMD_HIP(hipGetLastError());
+    int h[2] = {0, 0};
+    RAMD_HIP(hipMemcpyAsync(h, b.cnt, sizeof(h), hipMemcpyDeviceToHost, s));
+    RAMD_HIP(hipStreamSynchronize(s));
+    if (h[1] & 1) throw std::runtime_error("aggregate: strength entry missing from A");
+    if (h[1] & 2) {
+        hipLaunchKernelGGL(agg_pass2_serial_kernel, dim3(1), dim3(64), 0, s, n, S->rowptr, S->col, A->rowptr, A->col,
+                           A->val, b.r, agg, b.cnt + 1);
+        hipLaunchKernelGGL(agg_decode_kernel, g, blk, 0, s, n, agg, agg);
+    } else {
+        hipLaunchKernelGGL(agg_decode_kernel, g, blk, 0, s, n, b.out, agg);
+    }
+    RAMD_HIP(hipGetLastError());
+    RAMD_HIP(hipStreamSynchronize(s));
+    return h[0];
+}
+
+// the scaled operator of the Jacobi prolongation: row i of A times
+// omega / sum_j |a_ij| (absolute row sum in stored order; zero rows stay zero)
+__global__ void sa_scale_kernel(int n, const int* __restrict__ rp, const double* __restrict__ val, double omega,
+                                double* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = rp[i], e = rp[i + 1];
+    double row_sum = 0.0;
+    for (int j = s; j < e; j++) row_sum += fabs(val[j]);
+    double inv = 0.0;
+    if (row_sum != 0.0) inv = (1.0 / fabs(row_sum)) * omega;
+    for (int j = s; j < e; j++) out[j] = val[j] * inv;
+}
+
+ramd_csr_s* gpu_sa_scaled_operator(const ramd_csr_s* A, double omega)
+{
+    ramd_ctx ctx = A->ctx;
+    hipStream_t s = ctx->stream;
+    const int n = A->n_rows;
+    const int64_t nnz = A->nnz;
+    int* d_rp = dalloc<int>((size_t)n + 1);
+    int* d_col = dalloc<int>((size_t)std::max<int64_t>(nnz, 1));
+    double* d_val = dalloc<double>((size_t)std::max<int64_t>(nnz, 1));
+    RAMD_HIP(hipMemcpyAsync(d_rp, A->rowptr, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToDevice, s));
+    if (nnz) RAMD_HIP(hipMemcpyAsync(d_col, A->col, sizeof(int) * nnz, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(sa_scale_kernel, rows_grid(n), dim3(256), 0, s, n, A->rowptr, A->val, omega, d_val);
+    RAMD_HIP(hipGetLastError());
+    return csr_wrap(ctx, n, A->n_cols, nnz, d_rp, d_col, d_val, A->h_rowptr.data());
+}
+
+// Row i of P - Q (the reference's subtract: P's row then Q's row negated,
+// stably sorted by column, equal columns summed in that order, |v| < 1e-16
+// dropped, the diagonal (column i) moved first).  P's rows hold distinct
+// columns, the diagonal first when present and the rest ascending (T, or an
+// earlier difference); Q's rows hold distinct ascending columns (the sparse
+// product), so one merge gives the sorted order.
+template <bool FILL>
+__global__ void sa_subtract_kernel(int n, const int* __restrict__ prp, const int* __restrict__ pcol,
+                                   const double* __restrict__ pval, const int* __restrict__ qrp,
+                                   const int* __restrict__ qcol, const double* __restrict__ qval,
+                                   int* __restrict__ cnt, const int* __restrict__ crp, int* __restrict__ ccol,
+                                   double* __restrict__ cval)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int ps = prp[i];
+    const int pe = prp[i + 1];
+    int pd = -1;  // P's diagonal entry, merged at its column
+    if (ps < pe && pcol[ps] == i) pd = ps++;
+    int q = qrp[i];
+    const int qe = qrp[i + 1];
+    const int o0 = FILL ? crp[i] : 0;
+    int o = o0, diag_at = -1;
+    while (true) {
+        // next P entry in column order
+        int pj = -1;
+        if (pd >= 0 && (ps >= pe || i < pcol[ps])) pj = pd;
+        else if (ps < pe) pj = ps;
+        const int pc = pj >= 0 ? pcol[pj] : INT_MAX;
+        const int qc = q < qe ? qcol[q] : INT_MAX;
+        if (pc == INT_MAX && qc == INT_MAX) break;
+        int c;
+        double v;
+        if (pc == qc) {
+            c = pc;
+            v = pval[pj] + (-qval[q]);
+        } else if (pc < qc) {
+            c = pc;
+            v = pval[pj];
+        } else {
+            c = qc;
+            v = -qval[q];
+        }
+        if (pc <= qc) {
+            if (pj == pd) pd = -1;
+            else ps++;
+        }
+        if (qc <= pc) q++;
+        if (!(fabs(v) < kSetupZeroTol)) {
+            if (FILL) {
+                if (c == i) diag_at = o;
+                ccol[o] = c;
+                cval[o] = v;
+            }
+            o++;
+        }
+    }
+    if (!FILL) {
+        cnt[i] = o;
+    } else if (diag_at > o0) {
+        const double dv = cval[diag_at];
+        for (int m = diag_at; m > o0; m--) {
+            ccol[m] = ccol[m - 1];
+            cval[m] = cval[m - 1];
+        }
+        ccol[o0] = i;
+        cval[o0] = dv;
+    }
+}
+
+ramd_csr_s* gpu_sa_subtract(const ramd_csr_s* P, const ramd_csr_s* Q)
+{
+    ramd_ctx ctx = P->ctx;
+    hipStream_t s = ctx->stream;
+    const int n = P->n_rows;
+    int* cnt = dalloc<int>((size_t)std::max(n, 1));
+    std::vector<int> rp;
+    try {
+        hipLaunchKernelGGL((sa_subtract_kernel<false>), rows_grid(n), dim3(256), 0, s, n, P->rowptr, P->col, P->val,
+                           Q->rowptr, Q->col, Q->val, cnt, nullptr, nullptr, nullptr);
+        RAMD_HIP(hipGetLastError());
+        rp = host_rowptr(ctx, cnt, n);
+    } catch (...) {
+        dfree(cnt);
+        throw;
+    }
+    dfree(cnt);
+    const int64_t nnz = rp[n];
+    int* d_rp = dalloc<int>((size_t)n + 1);
+    int* d_col = dalloc<int>((size_t)std::max<int64_t>(nnz, 1));
+    double* d_val = dalloc<double>((size_t)std::max<int64_t>(nnz, 1));
+    RAMD_HIP(hipMemcpyAsync(d_rp, rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((sa_subtract_kernel<true>), rows_grid(n), dim3(256), 0, s, n, P->rowptr, P->col, P->val,
+                       Q->rowptr, Q->col, Q->val, nullptr, d_rp, d_col, d_val);
+    RAMD_HIP(hipGetLastError());
+    RAMD_HIP(hipStreamSynchronize(s));
+    return csr_wrap(ctx, n, P->n_cols, nnz, d_rp, d_col, d_val, rp.data());
+}
+
 }  // namespace ramd
 
 using namespace ramd;
@@ -1269,6 +1742,56 @@ int ramd_interp_mod_classical_gpu(ramd_csr A, ramd_csr S, const int32_t* states,
         RAMD_CHECK(out, RAMD_E_ARG, "ramd_interp_mod_classical_gpu: a row has more than %d strong C neighbours",
                    kExtCols);
         *P = out;
+    });
+}
+
+int ramd_mis2_gpu(ramd_csr S, const double* weights, int32_t* states)
+{
+    return guarded([&] {
+        RAMD_CHECK(S && states, RAMD_E_ARG, "ramd_mis2_gpu: NULL argument");
+        RAMD_CHECK(S->n_rows == S->n_cols, RAMD_E_ARG, "ramd_mis2_gpu: S is not square");
+        ramd_ctx ctx = S->ctx;
+        ctx->set_current();
+        const int n = S->n_rows;
+        std::vector<double> w;
+        if (weights) w.assign(weights, weights + n);
+        else w = host::reference_weights(n, 0);
+        int* d = dalloc<int>((size_t)std::max(n, 1));
+        try {
+            gpu_mis2(S, w.data(), d);
+            if (n) RAMD_HIP(hipMemcpyAsync(states, d, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+            RAMD_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            dfree(d);
+            throw;
+        }
+        dfree(d);
+    });
+}
+
+int ramd_aggregate_gpu(ramd_csr A, ramd_csr S, const int32_t* states, const double* rand_vals, int32_t* aggregates,
+                       int32_t* n_aggs)
+{
+    return guarded([&] {
+        RAMD_CHECK(A && S && states && aggregates && n_aggs, RAMD_E_ARG, "ramd_aggregate_gpu: NULL argument");
+        RAMD_CHECK(A->n_rows == A->n_cols && S->n_rows == A->n_rows && S->n_cols == A->n_cols, RAMD_E_ARG,
+                   "ramd_aggregate_gpu: size mismatch");
+        ramd_ctx ctx = A->ctx;
+        ctx->set_current();
+        const int n = A->n_rows;
+        int* d = dalloc<int>((size_t)std::max(n, 1) * 2);
+        try {
+            if (n) RAMD_HIP(hipMemcpyAsync(d, states, sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+            *n_aggs = gpu_aggregate(A, S, d, d + std::max(n, 1), rand_vals);
+            if (n)
+                RAMD_HIP(hipMemcpyAsync(aggregates, d + std::max(n, 1), sizeof(int) * n, hipMemcpyDeviceToHost,
+                                        ctx->stream));
+            RAMD_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            dfree(d);
+            throw;
+        }
+        dfree(d);
     });
 }
 

@@ -223,3 +223,134 @@ def test_device_setup_row_overflow_falls_back(ctx, interp):
                 continue
             assert all(np.array_equal(u, v) for u, v in zip(M1.to_csr(), M2.to_csr())), l
     assert overflow  # the hub row needs more than 256 columns
+
+
+# ---------------------------------------------------------------------------
+# smoothed aggregation (aggregation/tests/test_par_smoothed_aggregation.cpp)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("level", [0, 1])
+def test_device_sa_steps_golden(ctx, golden, level):
+    """Symmetric strength (0.25) -> MIS-2 with weights.txt -> aggregation on
+    the device against sas_S/mis/agg of the reference's test (aggregate
+    numbers there index the selected rows in increasing order)."""
+    A = K.move_diag(K.sort_rows(K.fixture_csr(golden, f"sas_A{level}", sort=False)))
+    n = A.n_rows
+    w = golden["weights"][:n]
+    D = ra.ParCSRMatrix.from_csr(ctx, A.rowptr, A.col, A.val)
+    S = ra.strength_gpu(D, 0.25, ra.Symmetric)
+    srp, scol, _ = S.to_csr()
+    E = K.move_diag(K.fixture_csr(golden, f"sas_S{level}"))
+    assert np.array_equal(srp, E.rowptr) and np.array_equal(scol, E.col)
+    st = ra.mis2_gpu(S, w)
+    assert np.array_equal(st, golden[f"sas_mis{level}"])
+    agg, na = ra.aggregate_gpu(D, S, st, w)
+    roots = np.flatnonzero(st > 0)
+    assert na == roots.size
+    assert np.array_equal(roots[golden[f"sas_agg{level}"]], agg)
+
+
+@pytest.mark.parametrize("stype", [ra.Symmetric, ra.Classical], ids=["symmetric", "classical"])
+@pytest.mark.parametrize("theta", [0.0, 0.08, 0.25])
+@pytest.mark.parametrize("name,gen", CASES, ids=[c[0] for c in CASES])
+def test_device_mis2_aggregate_match_host(ctx, name, gen, theta, stype):
+    """MIS-2 and aggregation (with and without random values) on the device
+    equal the host restatement's on the same S, which the compiled reference
+    pins (test_host_setup.py)."""
+    A = diag_first(*gen())
+    n = A.n_rows
+    D = ra.ParCSRMatrix.from_csr(ctx, A.rowptr, A.col, A.val)
+    S = ra.strength_gpu(D, theta, stype)
+    srp, scol, _ = S.to_csr()
+    w = np.random.default_rng(7).random(n)
+    st = ra.mis2_gpu(S, w)
+    assert np.array_equal(st, ra.mis2(srp, scol, w))
+    assert np.array_equal(ra.mis2_gpu(S), ra.mis2(srp, scol))  # the srand(2448422) stream
+    for rv in (None, w):
+        agg, na = ra.aggregate_gpu(D, S, st, rv)
+        agg_h, na_h = ra.aggregate(A.rowptr, A.col, A.val, srp, scol, st, rv)
+        assert na == na_h and np.array_equal(agg, agg_h)
+
+
+@pytest.mark.parametrize("frac", [0.0, 0.002, 0.05])
+def test_device_aggregate_row_order_pass(ctx, frac):
+    """States that leave rows with no aggregated neighbour (none or very few
+    selected rows): the second pass then depends on the row order (such a
+    row's code n - 1 reads as an aggregate to the rows after it), and the
+    device reruns it in row order; the result equals the host's."""
+    A = diag_first(*random_mmatrix(900, 11))
+    n = A.n_rows
+    D = ra.ParCSRMatrix.from_csr(ctx, A.rowptr, A.col, A.val)
+    S = ra.strength_gpu(D, 0.0, ra.Symmetric)
+    srp, scol, _ = S.to_csr()
+    st = (np.random.default_rng(12).random(n) < frac).astype(np.int32)
+    agg, na = ra.aggregate_gpu(D, S, st)
+    agg_h, na_h = ra.aggregate(A.rowptr, A.col, A.val, srp, scol, st)
+    assert na == na_h and np.array_equal(agg, agg_h)
+    if frac == 0.0:
+        assert na == 0 and (agg == n - 1).any()  # the code n - 1 survives decoding
+
+
+def test_device_aggregate_missing_entry_raises(ctx):
+    """An S entry that A does not store stops the aggregation (host and
+    device alike)."""
+    A = diag_first(*ra.gallery_aniso2d(12, 0.001, math.pi / 8))
+    n = A.n_rows
+    # S: each row's diagonal and a column that A's row does not hold
+    srp = np.arange(0, 2 * n + 1, 2, dtype=np.int32)
+    far = (np.arange(n) + n // 2) % n
+    scol = np.empty(2 * n, np.int32)
+    scol[0::2] = np.arange(n)
+    scol[1::2] = far
+    assert not any(far[i] in A.col[A.rowptr[i]:A.rowptr[i + 1]] for i in range(n))
+    D = ra.ParCSRMatrix.from_csr(ctx, A.rowptr, A.col, A.val)
+    S = ra.ParCSRMatrix.from_csr(ctx, srp, scol, np.ones(2 * n))
+    st = np.zeros(n, np.int32)
+    with pytest.raises(ra.RamdError):
+        ra.aggregate(A.rowptr, A.col, A.val, srp, scol, st)
+    with pytest.raises(ra.RamdError):
+        ra.aggregate_gpu(D, S, st)
+
+
+SA_CASES = [
+    ("fv7_14_s3", lambda: ra.gallery_fv7(14, seed=3)),
+    ("fv7_20_s9", lambda: ra.gallery_fv7(20, seed=9)),
+    ("aniso60", lambda: ra.gallery_aniso2d(60, 0.001, math.pi / 8)),
+    ("lap27_12", lambda: ra.gallery_laplace27(12)),
+    ("random_mm_700", lambda: random_mmatrix(700, 5)),
+]
+
+
+@pytest.mark.parametrize("steps", [0, 1, 2])
+@pytest.mark.parametrize("theta,stype", [(0.0, ra.Symmetric), (0.08, ra.Symmetric), (0.25, ra.Classical)],
+                         ids=["sym0", "sym008", "cls025"])
+@pytest.mark.parametrize("name,gen", SA_CASES, ids=[c[0] for c in SA_CASES])
+def test_sa_hierarchy_device_setup_equals_host(ctx, name, gen, theta, stype, steps):
+    """Smoothed-aggregation hierarchies built with the device steps
+    (strength, MIS-2, aggregation, the prolongation smoothing) and with the
+    host steps: every level's A and P and the aggregates identical."""
+    rp, col, val = gen()
+    mls, errs = [], []
+    for dev in (True, False):
+        ml = ra.ParSmoothedAggregationSolver(ctx, theta, stype, ra.Jacobi, prolong_smooth_steps=steps)
+        ml.device_setup = dev
+        ml.max_coarse = 20
+        try:
+            ml.setup(rp, col, val)
+            errs.append(None)
+        except ra.RamdError as e:  # e.g. an aggregate without a root: both builds stop alike
+            errs.append(str(e))
+        mls.append(ml)
+    assert errs[0] == errs[1]
+    if errs[0] is not None:
+        pytest.skip(f"both builds stop alike: {errs[0]}")
+    a, b = mls
+    assert a.num_levels == b.num_levels >= 2
+    for l in range(a.num_levels):
+        for M1, M2 in zip(a.level(l), b.level(l)):
+            if M1 is None:
+                assert M2 is None
+                continue
+            assert M1.n_cols == M2.n_cols
+            assert all(np.array_equal(u, v) for u, v in zip(M1.to_csr(), M2.to_csr())), l
+        if l < a.num_levels - 1:
+            assert np.array_equal(a.states(l), b.states(l)), l
