@@ -551,6 +551,12 @@ int ramd_interp_direct_gpu(ramd_csr A, ramd_csr S, const int32_t* states, ramd_c
 int ramd_mis2_gpu(ramd_csr S, const double* weights, int32_t* states);
 int ramd_aggregate_gpu(ramd_csr A, ramd_csr S, const int32_t* states, const double* rand_vals, int32_t* aggregates,
                        int32_t* n_aggs);
+/* fit_candidates (aggregation/par_candidates.cpp:6-216, one candidate) on
+ * the device of ctx, bit-identical to ramd_host_fit_candidates: T (n x
+ * n_aggs, a device matrix) and the aggregate norms R (host, n_aggs) from
+ * the aggregates and B (host, n). */
+int ramd_fit_candidates_gpu(ramd_ctx ctx, int32_t n, const int32_t* aggregates, int32_t n_aggs, const double* B,
+                            double* R, ramd_csr* T);
 
 /* Greedy colouring of the graph of A + A^T without the diagonal (rows in
  * ascending order take the smallest colour no coloured neighbour holds),

@@ -38,7 +38,7 @@ from ._lib import (CLJP, HMIS, MC_SOR, MC_SSOR, PMIS, RS, Classical, Direct, Ext
 __all__ = [
     "Context", "ParVector", "ParCSRMatrix", "ParMultilevel", "ParRugeStubenSolver",
     "ParSmoothedAggregationSolver", "PCG", "CG", "BiCGStab", "Pre_BiCGStab", "FPCG", "FPCG_block", "vcycle_block", "block_tuning", "block_set_tuning", "adjoint", "strength", "split", "interpolation", "mis2",
-    "aggregate", "fit_candidates", "strength_gpu", "split_pmis_gpu", "interp_extended_gpu", "mis2_gpu", "aggregate_gpu", "jacobi_prolongation", "read_petsc", "write_petsc", "gallery_laplace27", "gallery_aniso2d", "gallery_fv7", "RamdError",
+    "aggregate", "fit_candidates", "strength_gpu", "split_pmis_gpu", "interp_extended_gpu", "mis2_gpu", "aggregate_gpu", "fit_candidates_gpu", "jacobi_prolongation", "read_petsc", "write_petsc", "gallery_laplace27", "gallery_aniso2d", "gallery_fv7", "RamdError",
     "RS", "CLJP", "Falgout", "PMIS", "HMIS", "Direct", "ModClassical", "Extended", "Classical",
     "Symmetric", "Jacobi", "MC_SOR", "MC_SSOR", "greedy_colors", "PREC_FP64", "PREC_FP32",
 ]
@@ -748,6 +748,17 @@ def aggregate_gpu(A, S, states, rand_vals=None):
     check(lib().ramd_aggregate_gpu(A.h, S.h, _ptr(states, C.c_int32), None if r is None else _ptr(r, C.c_double),
                                    _ptr(agg, C.c_int32), C.byref(na)))
     return agg, na.value
+
+
+def fit_candidates_gpu(ctx, aggregates, n_aggs, B):
+    """fit_candidates on the device (ramd_fit_candidates_gpu): returns (T, a
+    device matrix, and the aggregate norms R)."""
+    agg, B = _i32(aggregates), _f64(B)
+    R = np.empty(n_aggs)
+    h = C.c_void_p()
+    check(lib().ramd_fit_candidates_gpu(ctx.h, agg.size, _ptr(agg, C.c_int32), int(n_aggs), _ptr(B, C.c_double),
+                                        _ptr(R, C.c_double), C.byref(h)))
+    return ParCSRMatrix(ctx, h), R
 
 
 def split_rs_serial(s_rowptr, s_col, second_pass=True):

@@ -144,6 +144,7 @@ _SIGS = {
     "ramd_interp_direct_gpu": (P, P, PI32, C.POINTER(P)),
     "ramd_mis2_gpu": (P, PD, PI32),
     "ramd_aggregate_gpu": (P, P, PI32, PD, PI32, PI32),
+    "ramd_fit_candidates_gpu": (P, I32, PI32, I32, PD, PD, C.POINTER(P)),
     "ramd_pre_bicgstab": (P, P, P, P, D, I32, PD, I32, PI32, PI32),
     "ramd_adjoint": (P, P, P, P, D, C.c_int, C.c_int, P, P, PI32, PD),
     "ramd_fpcg_noalloc": (P, P, P, P, D, C.c_int, C.c_int, PD, PI32, PD),

@@ -1143,15 +1143,15 @@ static Csr sa_level(ramd_hier_s* h, Csr& A, const double* weights, std::vector<i
     return P;
 }
 
-// sa_level with every step but fit_candidates on the device: strength,
-// MIS-2, aggregation, the scaled operator, its products with the
-// prolongator and the subtractions, with sa_level's results.  fit_candidates
-// (one pass over the aggregates, with the candidate vector B kept on the
-// host between levels) runs on the host.
+// sa_level on the device: strength, MIS-2, aggregation, fit_candidates,
+// the scaled operator, its products with the prolongator and the
+// subtractions, with sa_level's results.  The candidate vector B (device,
+// n) is replaced by the aggregate norms (n_aggs), the next level's B.  The
+// host forms only the aggregates' column numbers and T's row pointers.
 static bool device_sa(const ramd_amg_opts& o) { return o.device_setup && o.solver == RAMD_SOLVER_SA; }
 
 static ramd_csr_s* sa_level_device(ramd_hier_s* h, int level, const double* weights, std::vector<int>& aggregates,
-                                   std::vector<double>& B)
+                                   double*& B)
 {
     const ramd_amg_opts& o = h->opts;
     ramd_ctx ctx = h->ctx;
@@ -1161,6 +1161,7 @@ static ramd_csr_s* sa_level_device(ramd_hier_s* h, int level, const double* weig
     ramd_csr_s* S = o.strength_type == RAMD_SYMMETRIC ? gpu_symmetric_strength(L.A, o.strong_threshold)
                                                       : gpu_classical_strength(L.A, o.strong_threshold);
     int *st = nullptr, *agg = nullptr;
+    double* R = nullptr;
     ramd_csr_s *P = nullptr, *M = nullptr, *Q = nullptr;
     try {
         double t1 = now_s();
@@ -1172,8 +1173,8 @@ static ramd_csr_s* sa_level_device(ramd_hier_s* h, int level, const double* weig
         if (n) RAMD_HIP(hipMemcpyAsync(aggregates.data(), agg, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
         RAMD_HIP(hipStreamSynchronize(ctx->stream));
         double t2 = now_s();
-        std::vector<double> R;
-        P = upload(ctx, host::fit_candidates(n_aggs, aggregates, B, R), false);
+        R = dalloc<double>((size_t)std::max(n_aggs, 1));
+        P = gpu_fit_candidates(ctx, n, n_aggs, agg, aggregates, B, R);
         if (o.prolong_smooth_steps > 0) {
             M = gpu_sa_scaled_operator(L.A, o.prolong_weight);
             for (int step = 0; step < o.prolong_smooth_steps; step++) {
@@ -1187,7 +1188,9 @@ static ramd_csr_s* sa_level_device(ramd_hier_s* h, int level, const double* weig
             csr_free(M);
             M = nullptr;
         }
-        B = std::move(R);
+        dfree(B);
+        B = R;
+        R = nullptr;
         double t3 = now_s();
         h->times[0] += t1 - t0;
         h->times[1] += t2 - t1;
@@ -1196,6 +1199,7 @@ static ramd_csr_s* sa_level_device(ramd_hier_s* h, int level, const double* weig
         csr_free(S);
         dfree(st);
         dfree(agg);
+        dfree(R);
         csr_free(P);
         csr_free(M);
         csr_free(Q);
@@ -1266,7 +1270,19 @@ static void hier_setup(ramd_hier_s* h, Csr Af)
     bool host_A = true;  // A holds the current level (the device path skips the copy)
     // SA near-nullspace candidate (aggregation/par_smoothed_aggregation_solver.hpp:38-49)
     std::vector<double> B;
-    if (o.solver == RAMD_SOLVER_SA) B.assign((size_t)A.n_rows, 1.0);
+    struct DeviceB {
+        double* p = nullptr;
+        ~DeviceB() { dfree(p); }
+    } Bd;
+    if (o.solver == RAMD_SOLVER_SA) {
+        B.assign((size_t)A.n_rows, 1.0);
+        if (device_sa(o)) {
+            Bd.p = dalloc<double>((size_t)std::max(A.n_rows, 1));
+            if (A.n_rows)
+                RAMD_HIP(hipMemcpyAsync(Bd.p, B.data(), sizeof(double) * A.n_rows, hipMemcpyHostToDevice,
+                                        ctx->stream));
+        }
+    }
     while (h->levels.back().n > o.max_coarse &&
            (o.max_levels == -1 || (int)h->levels.size() < o.max_levels)) {
         const int l = (int)h->levels.size() - 1;
@@ -1275,7 +1291,7 @@ static void hier_setup(ramd_hier_s* h, Csr Af)
         if (device_rs(o)) {
             Pd = rs_level_device(h, l, weights.data(), L.states);
         } else if (device_sa(o)) {
-            Pd = sa_level_device(h, l, weights.data(), L.states, B);
+            Pd = sa_level_device(h, l, weights.data(), L.states, Bd.p);
         } else {
             if (!host_A) A = download(L.A);
             Csr P = o.solver == RAMD_SOLVER_SA ? sa_level(h, A, weights.data(), L.states, B)

@@ -101,6 +101,11 @@ ramd_csr_s* gpu_extended_interpolation(const ramd_csr_s* A, const ramd_csr_s* S,
 void gpu_mis2(const ramd_csr_s* S, const double* weights, int* states);
 int gpu_aggregate(const ramd_csr_s* A, const ramd_csr_s* S, const int* states, int* agg,
                   const double* rand_vals = nullptr);
+// fit_candidates with one candidate: T (n x n_aggs) from the aggregates
+// (device agg_d and its host copy) and the candidate B_d (device, n); the
+// aggregate norms go to R_d (device, n_aggs)
+ramd_csr_s* gpu_fit_candidates(ramd_ctx ctx, int n, int n_aggs, const int* agg_d, const std::vector<int>& agg_h,
+                               const double* B_d, double* R_d);
 ramd_csr_s* gpu_sa_scaled_operator(const ramd_csr_s* A, double omega);
 ramd_csr_s* gpu_sa_subtract(const ramd_csr_s* P, const ramd_csr_s* Q);
 

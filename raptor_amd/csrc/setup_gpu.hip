@@ -1560,6 +1560,102 @@ ramd_csr_s* gpu_sa_subtract(const ramd_csr_s* P, const ramd_csr_s* Q)
     return csr_wrap(ctx, n, P->n_cols, nnz, d_rp, d_col, d_val, rp.data());
 }
 
+// fit_candidates (aggregation/par_candidates.cpp:6-216, one candidate):
+// aggregated row i holds (colmap[agg[i]], B[i]) before scaling
+__global__ void fit_fill_kernel(int n, const int* __restrict__ agg, const int* __restrict__ colmap,
+                                const int* __restrict__ trp, const double* __restrict__ B, int* __restrict__ tcol,
+                                double* __restrict__ tval)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int a = agg[i];
+    if (a < 0) return;
+    const int j = trp[i];
+    tcol[j] = colmap[a];
+    tval[j] = B[i];
+}
+
+// aggregate k's norm: the sum of B_i^2 over its rows in ascending order (the
+// transpose's row k), its square root (R, the next level's candidate) and
+// the reciprocal that scales the aggregate's entries
+__global__ void fit_norm_kernel(int m, const int* __restrict__ ttrp, const double* __restrict__ ttval,
+                                double* __restrict__ R, double* __restrict__ scale)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    double sum = 0.0;
+    for (int j = ttrp[k]; j < ttrp[k + 1]; j++) sum += ttval[j] * ttval[j];
+    const double r = sqrt(sum);
+    R[k] = r;
+    scale[k] = 1.0 / r;
+}
+
+__global__ void fit_scale_kernel(int n, const int* __restrict__ trp, const int* __restrict__ tcol,
+                                 const double* __restrict__ B, const double* __restrict__ scale,
+                                 double* __restrict__ tval)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int j = trp[i];
+    if (j < trp[i + 1]) tval[j] = B[i] * scale[tcol[j]];
+}
+
+ramd_csr_s* gpu_fit_candidates(ramd_ctx ctx, int n, int n_aggs, const int* agg_d, const std::vector<int>& agg_h,
+                               const double* B_d, double* R_d)
+{
+    hipStream_t s = ctx->stream;
+    // the column of each aggregate (roots in increasing order) and T's row
+    // pointers, from the host copy of the aggregates
+    std::vector<int> colmap((size_t)std::max(n, 1), 0), rp((size_t)n + 1, 0);
+    for (int a : agg_h)
+        if (a >= 0) colmap[a] = 1;
+    int nc = 0;
+    for (int i = 0; i < n; i++)
+        if (colmap[i]) colmap[i] = nc++;
+    if (nc != n_aggs)
+        throw std::runtime_error("fit_candidates: an aggregate has no selected root (the reference's aggregation "
+                                 "leaves such a point undefined)");
+    for (int i = 0; i < n; i++) rp[i + 1] = rp[i] + (agg_h[i] >= 0 ? 1 : 0);
+    const int64_t nnz = rp[n];
+    struct Bufs {
+        int* colmap = nullptr;
+        double* scale = nullptr;
+        ramd_csr_s* Tb = nullptr;
+        ~Bufs()
+        {
+            dfree(colmap);
+            dfree(scale);
+            csr_free(Tb);
+        }
+    } b;
+    b.colmap = dalloc<int>((size_t)std::max(n, 1));
+    b.scale = dalloc<double>((size_t)std::max(n_aggs, 1));
+    RAMD_HIP(hipMemcpyAsync(b.colmap, colmap.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    auto make = [&]() {
+        int* d_rp = dalloc<int>((size_t)n + 1);
+        int* d_col = dalloc<int>((size_t)std::max<int64_t>(nnz, 1));
+        double* d_val = dalloc<double>((size_t)std::max<int64_t>(nnz, 1));
+        RAMD_HIP(hipMemcpyAsync(d_rp, rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+        ramd_csr_s* M = csr_wrap(ctx, n, n_aggs, nnz, d_rp, d_col, d_val, rp.data());
+        hipLaunchKernelGGL(fit_fill_kernel, rows_grid(n), dim3(256), 0, s, n, agg_d, b.colmap, M->rowptr, B_d, M->col,
+                           M->val);
+        RAMD_HIP(hipGetLastError());
+        return M;
+    };
+    // unscaled entries, transposed (members of each aggregate in ascending
+    // row order) for the norms; then the scaled T
+    b.Tb = make();
+    ramd_csr_s* Tt = csr_transpose(b.Tb);
+    hipLaunchKernelGGL(fit_norm_kernel, rows_grid(n_aggs), dim3(256), 0, s, n_aggs, Tt->rowptr, Tt->val, R_d,
+                       b.scale);
+    RAMD_HIP(hipGetLastError());
+    ramd_csr_s* T = make();
+    hipLaunchKernelGGL(fit_scale_kernel, rows_grid(n), dim3(256), 0, s, n, T->rowptr, T->col, B_d, b.scale, T->val);
+    RAMD_HIP(hipGetLastError());
+    RAMD_HIP(hipStreamSynchronize(s));
+    return T;
+}
+
 }  // namespace ramd
 
 using namespace ramd;
@@ -1766,6 +1862,45 @@ int ramd_mis2_gpu(ramd_csr S, const double* weights, int32_t* states)
             throw;
         }
         dfree(d);
+    });
+}
+
+int ramd_fit_candidates_gpu(ramd_ctx ctx, int32_t n, const int32_t* aggregates, int32_t n_aggs, const double* B,
+                            double* R, ramd_csr* T)
+{
+    return guarded([&] {
+        RAMD_CHECK(ctx && aggregates && B && R && T && n >= 0 && n_aggs >= 0 && n_aggs <= n, RAMD_E_ARG,
+                   "ramd_fit_candidates_gpu: bad argument");
+        for (int i = 0; i < n; i++)
+            RAMD_CHECK(aggregates[i] >= -1 && aggregates[i] < n, RAMD_E_ARG,
+                       "ramd_fit_candidates_gpu: aggregate %d of row %d out of range", aggregates[i], i);
+        ctx->set_current();
+        std::vector<int> agg(aggregates, aggregates + n);
+        const size_t m = (size_t)std::max(n, 1);
+        int* agg_d = dalloc<int>(m);
+        double* B_d = nullptr;
+        double* R_d = nullptr;
+        try {
+            B_d = dalloc<double>(m);
+            R_d = dalloc<double>((size_t)std::max(n_aggs, 1));
+            if (n) {
+                RAMD_HIP(hipMemcpyAsync(agg_d, agg.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+                RAMD_HIP(hipMemcpyAsync(B_d, B, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+            }
+            ramd_csr_s* out = gpu_fit_candidates(ctx, n, n_aggs, agg_d, agg, B_d, R_d);
+            if (n_aggs)
+                RAMD_HIP(hipMemcpyAsync(R, R_d, sizeof(double) * n_aggs, hipMemcpyDeviceToHost, ctx->stream));
+            RAMD_HIP(hipStreamSynchronize(ctx->stream));
+            *T = out;
+        } catch (...) {
+            dfree(agg_d);
+            dfree(B_d);
+            dfree(R_d);
+            throw;
+        }
+        dfree(agg_d);
+        dfree(B_d);
+        dfree(R_d);
     });
 }
 

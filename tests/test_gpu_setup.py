@@ -354,3 +354,51 @@ def test_sa_hierarchy_device_setup_equals_host(ctx, name, gen, theta, stype, ste
             assert all(np.array_equal(u, v) for u, v in zip(M1.to_csr(), M2.to_csr())), l
         if l < a.num_levels - 1:
             assert np.array_equal(a.states(l), b.states(l)), l
+
+
+def candidate(kind, n, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "ones":
+        return np.ones(n)
+    if kind == "uniform":
+        return rng.random(n) + 0.5
+    return np.exp(rng.normal(0.0, 20.0, n)) * rng.choice([-1.0, 1.0], n)  # magnitudes over ~1e-26..1e26
+
+
+@pytest.mark.parametrize("bkind", ["ones", "uniform", "wide"])
+@pytest.mark.parametrize("name,gen", CASES, ids=[c[0] for c in CASES])
+def test_device_fit_candidates_match_host(ctx, name, gen, bkind):
+    """fit_candidates on the device (aggregate norms in ascending row order,
+    their square roots and reciprocals) equals the host restatement bit for
+    bit, on aggregates from the device MIS-2 and aggregation."""
+    A = diag_first(*gen())
+    n = A.n_rows
+    D = ra.ParCSRMatrix.from_csr(ctx, A.rowptr, A.col, A.val)
+    S = ra.strength_gpu(D, 0.08, ra.Symmetric)
+    agg, na = ra.aggregate_gpu(D, S, ra.mis2_gpu(S, np.random.default_rng(8).random(n)))
+    B = candidate(bkind, n, 9)
+    T, R = ra.fit_candidates_gpu(ctx, agg, na, B)
+    trp, tcol, tval, Rh = ra.fit_candidates(agg, na, B)
+    drp, dcol, dval = T.to_csr()
+    assert T.n_cols == na
+    assert np.array_equal(drp, trp) and np.array_equal(dcol, tcol) and np.array_equal(dval, tval)
+    assert np.array_equal(R, Rh)
+
+
+def test_device_fit_candidates_many_aggregates(ctx):
+    """200000 rows in 50000 aggregates of random membership with candidate
+    values over ~52 decades: every norm (a device square root) and every
+    scaled entry equals the host's."""
+    n, m = 200000, 50000
+    rng = np.random.default_rng(10)
+    roots = np.sort(rng.choice(n, m, replace=False))
+    agg = roots[rng.integers(0, m, n)].astype(np.int32)
+    agg[roots] = roots
+    agg[rng.choice(n, 1000, replace=False)] = -1  # unaggregated rows
+    agg[roots] = roots
+    B = candidate("wide", n, 11)
+    T, R = ra.fit_candidates_gpu(ctx, agg, m, B)
+    trp, tcol, tval, Rh = ra.fit_candidates(agg, m, B)
+    drp, dcol, dval = T.to_csr()
+    assert np.array_equal(drp, trp) and np.array_equal(dcol, tcol) and np.array_equal(dval, tval)
+    assert np.array_equal(R, Rh)
