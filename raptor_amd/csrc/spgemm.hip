@@ -2,11 +2,13 @@
 // reference's accumulation order and drop rule (util/linalg/matmult.cpp:
 // 71-204, util/linalg/add.cpp:39-78, core/matrix.cpp:922-986).
 //
-// Phase 1 (symbolic): one wavefront per row inserts every product's column
-// into a hash set (LDS when it fits, global memory otherwise) and counts the
-// distinct columns -> an upper bound of the row length.
-// Phase 2 (numeric): one wavefront per row walks the row of A in stored
-// order; for each a_ij the lanes spread over row j of B.  Columns inside one
+// Phase 1 (symbolic): one wavefront per row (a group of 8 lanes for rows of
+// at most 128 products) inserts every product's column into a hash set (LDS
+// when it fits, global memory otherwise) and counts the distinct columns ->
+// an upper bound of the row length.
+// Phase 2 (numeric): one wavefront per row (8 lanes for rows of at most 32
+// distinct columns, 16 for at most 64) walks the row of A in stored order;
+// for each a_ij the row's lanes spread over row j of B.  Columns inside one
 // row of B are distinct, so no two lanes ever update the same accumulator in
 // the same step, and every c_ik receives its products in exactly the
 // reference's order (a_ij in A's order, then b_jk in B's order), starting
@@ -22,23 +24,45 @@
 namespace ramd {
 
 constexpr int kWave = 64;
+// lanes per row for the bins of short rows (tables of at most 128 slots)
+constexpr int kSmallGroup = 8;
+// rows per workgroup of WAVES wavefronts with G lanes per row
+template <int WAVES, int G>
+constexpr int kGroups = WAVES * kWave / G;
 
 
 
 __device__ __forceinline__ unsigned hash_col(int c) { return (unsigned)c * 2654435761u; }
 
-__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
-
-__device__ __forceinline__ int wave_count(bool pred)
+// Lane groups of G lanes (G divides 64) inside a wavefront: one row per
+// group.  The group's bits of a ballot, its lane index, and the count and
+// prefix of a predicate over the group.
+template <int G>
+__device__ __forceinline__ int grp_lane()
 {
-    return __popcll(__ballot(pred));
+    return threadIdx.x & (G - 1);
 }
 
-__device__ __forceinline__ int wave_prefix(bool pred)
+template <int G>
+__device__ __forceinline__ unsigned long long grp_bits(bool pred)
 {
     const unsigned long long m = __ballot(pred);
-    const int lane = lane_id();
-    return __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+    if (G == kWave) return m;
+    return (m >> ((threadIdx.x & (kWave - 1)) & ~(G - 1))) & ((1ull << (G == kWave ? 0 : G)) - 1);
+}
+
+template <int G>
+__device__ __forceinline__ int grp_count(bool pred)
+{
+    return __popcll(grp_bits<G>(pred));
+}
+
+template <int G>
+__device__ __forceinline__ int grp_prefix(bool pred)
+{
+    const unsigned long long m = grp_bits<G>(pred);
+    const int lane = grp_lane<G>();
+    return __popcll(m & ((1ull << lane) - 1));
 }
 
 // products per row: sum over j in A_i of nnz(B_j)
@@ -71,39 +95,42 @@ __device__ __forceinline__ int hash_insert(int* keys, int T, int key)
 }
 
 // ---------------------------------------------------------------------------
-// symbolic: rows[] lists the rows of this bin; table size T per wave
+// symbolic: rows[] lists the rows of this bin; table size T per row, one
+// row per group of G lanes (a whole wavefront for the larger tables)
 // ---------------------------------------------------------------------------
+template <int G>
 __device__ void symbolic_row(int row, int* keys, int T, const int* __restrict__ arp, const int* __restrict__ acol,
                              const int* __restrict__ brp, const int* __restrict__ bcol, int* __restrict__ counts)
 {
-    const int lane = lane_id();
-    for (int t = lane; t < T; t += kWave) keys[t] = -1;
+    const int lane = grp_lane<G>();
+    for (int t = lane; t < T; t += G) keys[t] = -1;
     __builtin_amdgcn_wave_barrier();
     for (int j = arp[row]; j < arp[row + 1]; j++) {
         const int c = acol[j];
-        for (int t = brp[c] + lane; t < brp[c + 1]; t += kWave) hash_insert(keys, T, bcol[t]);
+        for (int t = brp[c] + lane; t < brp[c + 1]; t += G) hash_insert(keys, T, bcol[t]);
     }
     __builtin_amdgcn_wave_barrier();
     __threadfence_block();
     int cnt = 0;
-    for (int t0 = 0; t0 < T; t0 += kWave) {
+    for (int t0 = 0; t0 < T; t0 += G) {
         const int t = t0 + lane;
-        cnt += wave_count(t < T && keys[t] != -1);
+        cnt += grp_count<G>(t < T && keys[t] != -1);
     }
     if (lane == 0) counts[row] = cnt;
 }
 
-template <int T, int WAVES>
+template <int T, int WAVES, int G = kWave>
 __global__ __launch_bounds__(WAVES * kWave) void symbolic_lds(int nrows, const int* __restrict__ rows,
                                                             const int* __restrict__ arp, const int* __restrict__ acol,
                                                             const int* __restrict__ brp, const int* __restrict__ bcol,
                                                             int* __restrict__ counts)
 {
-    __shared__ int keys[WAVES][T];
-    const int w = threadIdx.x / kWave;
-    const int r = blockIdx.x * WAVES + w;
+    constexpr int GROUPS = kGroups<WAVES, G>;
+    __shared__ int keys[GROUPS][T];
+    const int w = threadIdx.x / G;
+    const int r = blockIdx.x * GROUPS + w;
     if (r >= nrows) return;
-    symbolic_row(rows[r], keys[w], T, arp, acol, brp, bcol, counts);
+    symbolic_row<G>(rows[r], keys[w], T, arp, acol, brp, bcol, counts);
 }
 
 __global__ __launch_bounds__(kWave) void symbolic_global(int nrows, const int* __restrict__ rows,
@@ -115,20 +142,21 @@ __global__ __launch_bounds__(kWave) void symbolic_global(int nrows, const int* _
 {
     const int r = blockIdx.x;
     if (r >= nrows) return;
-    symbolic_row(rows[r], pool + tab_off[r], tab_size[r], arp, acol, brp, bcol, counts);
+    symbolic_row<kWave>(rows[r], pool + tab_off[r], tab_size[r], arp, acol, brp, bcol, counts);
 }
 
 // ---------------------------------------------------------------------------
 // numeric
 // ---------------------------------------------------------------------------
+template <int G>
 __device__ void numeric_row(int row, int* keys, double* sums, int* lcol, double* lval, int T,
                             const int* __restrict__ arp, const int* __restrict__ acol, const double* __restrict__ aval,
                             const int* __restrict__ brp, const int* __restrict__ bcol, const double* __restrict__ bval,
                             const int* __restrict__ crp, int* __restrict__ ccol, double* __restrict__ cval,
                             int* __restrict__ clen)
 {
-    const int lane = lane_id();
-    for (int t = lane; t < T; t += kWave) {
+    const int lane = grp_lane<G>();
+    for (int t = lane; t < T; t += G) {
         keys[t] = -1;
         sums[t] = 0.0;
     }
@@ -136,7 +164,7 @@ __device__ void numeric_row(int row, int* keys, double* sums, int* lcol, double*
     for (int j = arp[row]; j < arp[row + 1]; j++) {
         const int c = acol[j];
         const double a = aval[j];
-        for (int t = brp[c] + lane; t < brp[c + 1]; t += kWave) {
+        for (int t = brp[c] + lane; t < brp[c + 1]; t += G) {
             const int slot = hash_insert(keys, T, bcol[t]);
             sums[slot] += a * bval[t];
         }
@@ -146,7 +174,7 @@ __device__ void numeric_row(int row, int* keys, double* sums, int* lcol, double*
     }
     // gather kept entries (|sum| > 1e-16) into a compact list
     int kept = 0;
-    for (int t0 = 0; t0 < T; t0 += kWave) {
+    for (int t0 = 0; t0 < T; t0 += G) {
         const int t = t0 + lane;
         bool keep = false;
         int k = -1;
@@ -156,18 +184,18 @@ __device__ void numeric_row(int row, int* keys, double* sums, int* lcol, double*
             v = sums[t];
             keep = (k != -1) && (fabs(v) > 1e-16);
         }
-        const int p = wave_prefix(keep);
+        const int p = grp_prefix<G>(keep);
         if (keep) {
             lcol[kept + p] = k;
             lval[kept + p] = v;
         }
-        kept += wave_count(keep);
+        kept += grp_count<G>(keep);
     }
     __builtin_amdgcn_wave_barrier();
     __threadfence_block();
     // rank sort by column, write out in ascending order
     const int base = crp[row];
-    for (int e = lane; e < kept; e += kWave) {
+    for (int e = lane; e < kept; e += G) {
         const int ce = lcol[e];
         int rank = 0;
         for (int f = 0; f < kept; f++) rank += (lcol[f] < ce);
@@ -177,21 +205,22 @@ __device__ void numeric_row(int row, int* keys, double* sums, int* lcol, double*
     if (lane == 0) clen[row] = kept;
 }
 
-template <int T, int WAVES>
+template <int T, int WAVES, int G = kWave>
 __global__ __launch_bounds__(WAVES * kWave) void numeric_lds(
     int nrows, const int* __restrict__ rows, const int* __restrict__ arp, const int* __restrict__ acol,
     const double* __restrict__ aval, const int* __restrict__ brp, const int* __restrict__ bcol,
     const double* __restrict__ bval, const int* __restrict__ crp, int* __restrict__ ccol,
     double* __restrict__ cval, int* __restrict__ clen)
 {
-    __shared__ int keys[WAVES][T];
-    __shared__ double sums[WAVES][T];
-    __shared__ int lcol[WAVES][T / 2];
-    __shared__ double lval[WAVES][T / 2];
-    const int w = threadIdx.x / kWave;
-    const int r = blockIdx.x * WAVES + w;
+    constexpr int GROUPS = kGroups<WAVES, G>;
+    __shared__ int keys[GROUPS][T];
+    __shared__ double sums[GROUPS][T];
+    __shared__ int lcol[GROUPS][T / 2];
+    __shared__ double lval[GROUPS][T / 2];
+    const int w = threadIdx.x / G;
+    const int r = blockIdx.x * GROUPS + w;
     if (r >= nrows) return;
-    numeric_row(rows[r], keys[w], sums[w], lcol[w], lval[w], T, arp, acol, aval, brp, bcol, bval, crp, ccol, cval,
+    numeric_row<G>(rows[r], keys[w], sums[w], lcol[w], lval[w], T, arp, acol, aval, brp, bcol, bval, crp, ccol, cval,
                 clen);
 }
 
@@ -205,7 +234,7 @@ __global__ __launch_bounds__(kWave) void numeric_global(
     const int r = blockIdx.x;
     if (r >= nrows) return;
     const int64_t o = tab_off[r];
-    numeric_row(rows[r], kpool + o, spool + o, lcpool + o, lvpool + o, tab_size[r], arp, acol, aval, brp, bcol, bval,
+    numeric_row<kWave>(rows[r], kpool + o, spool + o, lcpool + o, lvpool + o, tab_size[r], arp, acol, aval, brp, bcol, bval,
                 crp, ccol, cval, clen);
 }
 
@@ -398,8 +427,15 @@ static ramd_csr_s* spgemm_impl(ramd_csr_s* A, ramd_csr_s* B)
         hipLaunchKernelGGL((symbolic_lds<TT, WW>), dim3((nr + WW - 1) / WW), dim3(WW * kWave), 0, st, nr, rows,   \
                            A->rowptr, A->col, B->rowptr, B->col, d_counts);                                       \
         break;
-            SYM_CASE(64, 4)
-            SYM_CASE(128, 4)
+#define SYM_CASE_G(TT, WW, GG)                                                                                    \
+    case TT: {                                                                                                    \
+        constexpr int groups = kGroups<WW, GG>;                                                                   \
+        hipLaunchKernelGGL((symbolic_lds<TT, WW, GG>), dim3((nr + groups - 1) / groups), dim3(WW * kWave), 0, st,  \
+                           nr, rows, A->rowptr, A->col, B->rowptr, B->col, d_counts);                             \
+        break;                                                                                                    \
+    }
+            SYM_CASE_G(64, 4, kSmallGroup)
+            SYM_CASE_G(128, 4, kSmallGroup)
             SYM_CASE(256, 4)
             SYM_CASE(512, 4)
             SYM_CASE(1024, 4)
@@ -407,6 +443,7 @@ static ramd_csr_s* spgemm_impl(ramd_csr_s* A, ramd_csr_s* B)
             SYM_CASE(4096, 2)
             SYM_CASE(8192, 1)
 #undef SYM_CASE
+#undef SYM_CASE_G
         }
         RAMD_HIP(hipGetLastError());
     }
@@ -458,14 +495,23 @@ static ramd_csr_s* spgemm_impl(ramd_csr_s* A, ramd_csr_s* B)
         hipLaunchKernelGGL((numeric_lds<TT, WW>), dim3((nr + WW - 1) / WW), dim3(WW * kWave), 0, st, nr, rows,    \
                            A->rowptr, A->col, A->val, B->rowptr, B->col, B->val, d_rpb, d_col, d_val, d_len);     \
         break;
-                NUM_CASE(32, 4)
-                NUM_CASE(64, 4)
-                NUM_CASE(128, 4)
+#define NUM_CASE_G(TT, WW, GG)                                                                                    \
+    case TT: {                                                                                                    \
+        constexpr int groups = kGroups<WW, GG>;                                                                   \
+        hipLaunchKernelGGL((numeric_lds<TT, WW, GG>), dim3((nr + groups - 1) / groups), dim3(WW * kWave), 0, st,   \
+                           nr, rows, A->rowptr, A->col, A->val, B->rowptr, B->col, B->val, d_rpb, d_col, d_val,   \
+                           d_len);                                                                                \
+        break;                                                                                                    \
+    }
+                NUM_CASE_G(32, 4, kSmallGroup)
+                NUM_CASE_G(64, 4, kSmallGroup)
+                NUM_CASE_G(128, 4, 2 * kSmallGroup)
                 NUM_CASE(256, 4)
                 NUM_CASE(512, 4)
                 NUM_CASE(1024, 2)
                 NUM_CASE(2048, 1)
 #undef NUM_CASE
+#undef NUM_CASE_G
             }
             RAMD_HIP(hipGetLastError());
         }
