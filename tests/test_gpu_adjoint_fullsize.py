@@ -113,3 +113,25 @@ def test_c5_fp32_cycles_same_gradient(c5):
     assert b["rel"] <= 1e-10
     assert np.linalg.norm(b["dA"] - a["dA"]) <= 1e-8 * np.linalg.norm(a["dA"])
     assert np.linalg.norm(b["db"] - a["db"]) <= 1e-8 * np.linalg.norm(a["db"])
+
+
+def test_c5_sa_setup_device_equals_host(ctx, c5):
+    """The smoothed-aggregation hierarchy of the full-size workload built with
+    the device setup steps (strength, MIS-2, aggregation, fit_candidates,
+    Jacobi prolongation) equals the one built with the host steps: every
+    level's A and P and the aggregates, bit for bit."""
+    ml = ra.ParSmoothedAggregationSolver(ctx, 0.08)
+    ml.relax_weight = 2.0 / 3.0
+    ml.device_setup = False
+    ml.setup(c5["rp"], c5["col"], c5["val"])
+    a = c5["ml"]
+    assert a.num_levels == ml.num_levels >= 4
+    for l in range(a.num_levels):
+        for M1, M2 in zip(a.level(l), ml.level(l)):
+            if M1 is None:
+                assert M2 is None
+                continue
+            assert M1.n_cols == M2.n_cols
+            assert all(np.array_equal(u, v) for u, v in zip(M1.to_csr(), M2.to_csr())), l
+        if l < a.num_levels - 1:
+            assert np.array_equal(a.states(l), ml.states(l)), l
