@@ -215,6 +215,7 @@ def main():
     # matched comparison above gives both forms the exact FP32 value copy)
     t64p = None
     plain_same = None
+    plain_its, plain_rel = [], None
     if not args.no_fp64 and A.diagonals():
         A.set_fp32_outer(False)
         solve(ra.PREC_FP64)  # the outer operator's kernel choice for FP64 values
@@ -223,12 +224,15 @@ def main():
         plain_same = True
         for _ in range(args.steps):
             it, relp = solve(ra.PREC_FP64)
+            plain_its.append(it)
+            plain_rel = relp
             plain_same = plain_same and it == its64[-1] and relp == rel64
         ctx.sync()
         t64p = (time.perf_counter() - t0) / args.steps
         plain_same = bool(plain_same and np.array_equal(x.numpy(), x64_h))
         A.set_fp32_outer(True)
-        log(f"plain FP64 form (FP64 outer values): {t64p:.4f}s, same iterates: {plain_same}")
+        log(f"plain FP64 form (FP64 outer values): {t64p:.4f}s, iterations {plain_its}, relres {plain_rel:.3e}, "
+            f"bit-identical to the matched FP64 solve: {plain_same}")
 
     # ---- V-cycle throughput (FP32 and FP64), 10 cycles each
     vc = {}
@@ -323,6 +327,8 @@ def main():
         "fp64_plain_s": round(t64p, 5) if t64p else None,
         "fp32_speedup_vs_plain_fp64": round(t64p / t32, 3) if t64p else None,
         "fp64_plain_same_iterates": plain_same,
+        "iterations_fp64_plain": plain_its or None,
+        "relres_fp64_plain": plain_rel,
         "outer_operator": outer_operator(A),
         "launch": "graphs" if args.graphs else "eager",
         "iterations_fp32": its32,

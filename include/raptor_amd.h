@@ -97,7 +97,9 @@ int ramd_csr_diagonals(ramd_csr A, int* nd);
  * planes (4..256; 0 keeps the current count) with the x planes in LDS
  * (only when every offset is dz*P + dy*L + dx, |d| <= 1, of an L x (P/L) x
  * (n/P) grid), -1 lets the autotuner time them at the next hierarchy setup
- * or solve.  Every variant gives identical bits.  No reference counterpart. */
+ * or solve.  Every variant gives identical bits for the operator's output;
+ * the flexible PCG's p.Ap, summed per workgroup in the same launch, follows
+ * the workgroup shape in its last bits.  No reference counterpart. */
 int ramd_csr_set_dia_staging(ramd_csr A, int pp, int mode, int planes);
 /* The variant in use for the precision pair pp (mode as above, -1 not timed
  * yet) and, for the plane-march kernel, its planes per workgroup and the
@@ -138,9 +140,13 @@ int ramd_csr_set_sorted_gather(ramd_csr A, int prec, int on);
 /* Whether the flexible PCG's FP64 products with A (its outer operator and
  * true residuals, ramd_fpcg and ramd_adjoint) may read A's FP32 copy of the
  * values when every value is exactly representable in FP32: the same FP64
- * products from half the value bytes.  on = 1 (default) allows it, 0 reads the FP64 values as the
- * reference stores them (ParCSRMatrix::mult, util/linalg/par_spmv.cpp:25-59).
- * Identical results either way.  No reference counterpart. */
+ * products from half the value bytes.  on = 1 (default) allows it, 0 reads
+ * the FP64 values as the reference stores them (ParCSRMatrix::mult,
+ * util/linalg/par_spmv.cpp:25-59).  y = A p is bit-identical either way;
+ * p.Ap, summed per workgroup in the same launch, follows the workgroup shape
+ * the autotuner chose for each value precision, so the iterates agree to
+ * rounding, and bit for bit when both launches have the same shape
+ * (ramd_csr_set_dia_staging).  No reference counterpart. */
 int ramd_csr_set_fp32_outer(ramd_csr A, int on);
 
 /* ------------------------------------------------------------------ */

@@ -254,7 +254,8 @@ class ParCSRMatrix:
 
     def set_fp32_outer(self, on):
         """FP64 products with this matrix read its exact FP32 value copy (True,
-        default) or the FP64 values (False); identical results."""
+        default) or the FP64 values (False): the same products; FPCG's p.Ap
+        follows each launch's workgroup shape in its last bits."""
         check(lib().ramd_csr_set_fp32_outer(self.h, int(bool(on))))
 
     def set_sorted_gather(self, prec, on):

@@ -306,8 +306,8 @@ struct ramd_csr_s {
     // core/matrix.cpp:729-787); lets Jacobi read one column per row
     bool diag_first = false;
     // the flexible PCG's FP64 products with this matrix may read its FP32
-    // copy of the values when every value is exact in FP32 (same products;
-    // on by default; ramd_csr_set_fp32_outer)
+    // copy of the values when every value is exact in FP32 (the same
+    // products; on by default; ramd_csr_set_fp32_outer)
     bool v32_outer = true;
     // identity of the handle (never reused within a process) and a count of
     // the changes to its device forms and kernel choices after creation (FP32

@@ -479,27 +479,41 @@ def test_fpcg_sliced_outer_operator(ctx):
 
 @pytest.mark.parametrize("prec", [ra.PREC_FP64, ra.PREC_FP32])
 def test_fpcg_fp32_outer_switch(ctx, prec):
-    """FPCG on a stencil operator whose values are exact in FP32 gives the
-    same iterates with its FP64 products reading the FP32 value copy (the
-    default) as reading the FP64 values."""
-    rp, col, val = ra.gallery_laplace27(16)
+    """FPCG on a stencil operator whose values are exact in FP32, with its FP64
+    products reading the FP32 value copy (the default) or the FP64 values:
+    with both launches on the same workgroup shape the iterates are
+    bit-identical (the products are the same and p.Ap is summed over the
+    same workgroups); with different shapes (4 and 8 planes per workgroup)
+    only p.Ap's rounding differs: same iteration count, results equal to
+    rounding."""
+    rp, col, val = ra.gallery_laplace27(24)
     ml = build(ctx, rp, col, val, rs(ra.PMIS, ra.Extended))
     A = ra.ParCSRMatrix.from_csr(ctx, rp, col, val)
     assert A.diagonals() > 0
     n = rp.size - 1
     b = np.random.default_rng(27).standard_normal(n)
-    out = []
+
+    def run(on, pl64, pl32):
+        A.set_fp32_outer(on)
+        A.set_dia_staging(0, 2, pl64)
+        A.set_dia_staging(2, 2, pl32)
+        x = ra.ParVector.from_numpy(ctx, np.zeros(n))
+        hist, it, rel = ra.FPCG(A, ml, x, ra.ParVector.from_numpy(ctx, b), 1e-8, 200, prec)
+        assert A.dia_staging(0)[:2] == (2, pl64) and A.dia_staging(2)[:2] == (2, pl32)
+        return x.numpy(), it, rel
+
     try:
-        for on in (True, False, True):
-            A.set_fp32_outer(on)
-            x = ra.ParVector.from_numpy(ctx, np.zeros(n))
-            hist, it, rel = ra.FPCG(A, ml, x, ra.ParVector.from_numpy(ctx, b), 1e-8, 200, prec)
-            out.append((x.numpy(), it, rel))
+        ref = run(True, 4, 4)
+        for o in (run(False, 4, 4), run(True, 4, 4)):
+            assert o[1] == ref[1] and o[2] == ref[2]
+            assert np.array_equal(o[0], ref[0])
+        o = run(False, 8, 4)
+        assert o[1] == ref[1] and abs(o[2] - ref[2]) <= 1e-6 * ref[2]
+        assert np.allclose(o[0], ref[0], rtol=0, atol=1e-10 * np.abs(ref[0]).max())
     finally:
         A.set_fp32_outer(True)
-    for o in out[1:]:
-        assert o[1] == out[0][1] and o[2] == out[0][2]
-        assert np.array_equal(o[0], out[0][0])
+        A.set_dia_staging(0, -1)
+        A.set_dia_staging(2, -1)
     assert ra._lib.lib().ramd_csr_set_fp32_outer(A.h, 2) != 0  # only 0 or 1
 
 
