@@ -551,7 +551,7 @@ int ramd_interp_direct_gpu(ramd_csr A, ramd_csr S, const int32_t* states, ramd_c
  * (aggregation/par_aggregate.cpp:5-185) on the device, bit-identical to
  * ramd_host_mis2 and ramd_host_aggregate; A and S stored sorted with the
  * diagonal first (the hierarchy's levels and the device strength give them
- * so).  weights NULL = the srand(2448422) stream; rand_vals NULL = zeros,
+ * so; ramd_aggregate_gpu returns RAMD_E_ARG for other orders).  weights NULL = the srand(2448422) stream; rand_vals NULL = zeros,
  * as the solver calls it.  states, rand_vals and aggregates are host
  * arrays of n. */
 int ramd_mis2_gpu(ramd_csr S, const double* weights, int32_t* states);

@@ -1656,6 +1656,44 @@ ramd_csr_s* gpu_fit_candidates(ramd_ctx ctx, int n, int n_aggs, const int* agg_d
     return T;
 }
 
+// flags a row that is not stored sorted with its diagonal first (the
+// diagonal, when present, then ascending distinct columns)
+__global__ void diag_first_sorted_kernel(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                         int* __restrict__ bad)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = rp[i];
+    const int e = rp[i + 1];
+    if (s < e && col[s] == i) s++;
+    for (int j = s; j < e; j++)
+        if (col[j] == i || (j > s && col[j] <= col[j - 1])) {
+            atomicOr(bad, 1);
+            return;
+        }
+}
+
+static bool diag_first_sorted(const ramd_csr_s* M)
+{
+    hipStream_t s = M->ctx->stream;
+    int* bad = dalloc<int>(1);
+    int h = 0;
+    try {
+        hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(64), 0, s, bad, 1, 0);
+        if (M->n_rows)
+            hipLaunchKernelGGL(diag_first_sorted_kernel, rows_grid(M->n_rows), dim3(256), 0, s, M->n_rows,
+                               M->rowptr, M->col, bad);
+        RAMD_HIP(hipGetLastError());
+        RAMD_HIP(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+        RAMD_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        dfree(bad);
+        throw;
+    }
+    dfree(bad);
+    return h == 0;
+}
+
 }  // namespace ramd
 
 using namespace ramd;
@@ -1913,6 +1951,8 @@ int ramd_aggregate_gpu(ramd_csr A, ramd_csr S, const int32_t* states, const doub
                    "ramd_aggregate_gpu: size mismatch");
         ramd_ctx ctx = A->ctx;
         ctx->set_current();
+        RAMD_CHECK(diag_first_sorted(A) && diag_first_sorted(S), RAMD_E_ARG,
+                   "ramd_aggregate_gpu: A and S must store each row sorted with its diagonal first");
         const int n = A->n_rows;
         int* d = dalloc<int>((size_t)std::max(n, 1) * 2);
         try {

@@ -402,3 +402,30 @@ def test_device_fit_candidates_many_aggregates(ctx):
     drp, dcol, dval = T.to_csr()
     assert np.array_equal(drp, trp) and np.array_equal(dcol, tcol) and np.array_equal(dval, tval)
     assert np.array_equal(R, Rh)
+
+
+def test_device_aggregate_rejects_unsorted_rows(ctx):
+    """The device aggregation walks A's row alongside S's, so it takes both
+    stored sorted with the diagonal first; other orders are refused (the
+    host restatement sorts its copies instead)."""
+    A = diag_first(*ra.gallery_aniso2d(12, 0.001, math.pi / 8))
+    D = ra.ParCSRMatrix.from_csr(ctx, A.rowptr, A.col, A.val)
+    S = ra.strength_gpu(D, 0.25, ra.Symmetric)
+    st = ra.mis2_gpu(S, np.random.default_rng(3).random(A.n_rows))
+    ra.aggregate_gpu(D, S, st)  # sorted: accepted
+    col = A.col.copy()
+    s, e = A.rowptr[40] + 1, A.rowptr[41]
+    col[s:e] = col[s:e][::-1]  # row 40's off-diagonal entries descending
+    val = A.val.copy()
+    val[s:e] = val[s:e][::-1]
+    U = ra.ParCSRMatrix.from_csr(ctx, A.rowptr, col, val)
+    with pytest.raises(ra.RamdError):
+        ra.aggregate_gpu(U, S, st)
+    rp = A.rowptr
+    col = A.col.copy()
+    col[rp[7]], col[rp[7] + 1] = col[rp[7] + 1], col[rp[7]]  # row 7's diagonal second
+    val = A.val.copy()
+    val[rp[7]], val[rp[7] + 1] = val[rp[7] + 1], val[rp[7]]
+    U = ra.ParCSRMatrix.from_csr(ctx, rp, col, val)
+    with pytest.raises(ra.RamdError):
+        ra.aggregate_gpu(U, S, st)
