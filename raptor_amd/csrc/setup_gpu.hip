@@ -1360,8 +1360,9 @@ __global__ void agg_pass2_kernel(int n, const int* __restrict__ srp, const int* 
 // pass 2 in the reference's row order by one thread.  Slow for large n, and
 // only reached when some row finds no aggregated neighbour: after MIS-2 on a
 // symmetric S (the solver's default strength) every row left to pass 2 has
-// a neighbour that pass 1 joined to a root, so only a nonsymmetric S or
-// states from elsewhere get here.
+// a neighbour that pass 1 joined to a root, so only a nonsymmetric S, an
+// explicit zero of A kept in S (weight 0 is never chosen), or states from
+// elsewhere get here.
 __global__ void agg_pass2_serial_kernel(int n, const int* __restrict__ srp, const int* __restrict__ scol,
                                         const int* __restrict__ arp, const int* __restrict__ acol,
                                         const double* __restrict__ aval, const double* __restrict__ r, int* agg,
